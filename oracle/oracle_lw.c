@@ -1,0 +1,1177 @@
+/*
+ * Oracle: RRTMG_LW clear-sky column model in FP64 (TEST INFRASTRUCTURE ONLY;
+ * see oracle.h).  Restates, in order of evaluation:
+ *   rrtmg_lw_init.f90:28-175 (tables, rwgt), :178-363, :366-2015 (cmbgb1..16)
+ *   rrtmg_lw_rad.nomcica.f90:419-567 (column loop), :572-901 (inatm)
+ *   rrtmg_lw_setcoef.f90:31-415 (setcoef)
+ *   rrtmg_lw_taumol.f90:31-3149 (taugb1..16)
+ *   rrtmg_lw_rtrnmr.f90:32-779 (rtrnmr, clear-sky branch: icldlyr = 0)
+ * LW k-tables are seeded synthetic values (AER's rrtmg_lw_k_g.f90 is absent);
+ * the arithmetic is the reference's regardless of the table values.
+ */
+#include "oracle_internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NBLW 16
+#define NGLW 140
+#define MAXLAY 256
+
+typedef struct {
+    double *p;
+    int ext[4];
+    int nd;
+} rarr_t; /* reduced array, Fortran order, g-point dimension last (fracref: first) */
+
+typedef struct {
+    int ng, nspa, nspb;
+    rarr_t absa, absb, selfref, forref, fracrefa, fracrefb;
+    rarr_t ka_mn2, kb_mn2, ka_mn2o, kb_mn2o, ka_mo3, kb_mo3, ka_mco2, kb_mco2, ka_mco, ka_mo2, kb_mo2;
+    rarr_t ccl4, cfc11adj, cfc12, cfc22adj;
+} lwband_t;
+
+static struct {
+    int ready;
+    double heatfac, grav, avogad;
+    double tau_tbl[ORACLE_NTBL + 1], exp_tbl[ORACLE_NTBL + 1], tfn_tbl[ORACLE_NTBL + 1];
+    double bpade;
+    int ngc[NBLW], ngs[NBLW];
+    double delwave[NBLW];
+    double rwgt[NBLW * 16];
+    double pref[59], preflog[59], tref[59];
+    double chi_mls[7 * 59];
+    double totplnk[181 * 16];
+    double a0[16], a1[16], a2[16];
+    lwband_t b[NBLW];
+} L;
+
+#define CHI(i, j) L.chi_mls[((j)-1) * 7 + ((i)-1)]
+#define TOTPLNK(i, ib) L.totplnk[((ib)-1) * 181 + ((i)-1)]
+
+/* 2-D and 3-D accessors with the g-point index last */
+#define A2(r, i, ig) ((r).p[(long)((ig)-1) * (r).ext[0] + ((i)-1)])
+#define A3(r, i, j, ig) ((r).p[((long)((ig)-1) * (r).ext[1] + ((j)-1)) * (r).ext[0] + ((i)-1)])
+#define G1(r, ig) ((r).p[(ig)-1])
+/* fracref(ng, neta): g-point first */
+#define FR2(r, ig, j) ((r).p[((j)-1) * (r).ext[0] + ((ig)-1)])
+
+static const double oneminus_lw = 1.0 - 1.e-6; /* rrtmg_lw_rad.nomcica.f90:419 */
+
+static int reduce_lw(const otbl_t *t, int band, const char *oname, const char *rname, int gfirst,
+                     int weighted, const int *ngn, rarr_t *out)
+{
+    char key[64];
+    snprintf(key, sizeof key, "lw/b%02d/%s", band, oname);
+    int nd, ext[4], lb[4], isint;
+    const double *x = (const double *)otbl_get(t, key, &nd, ext, lb, &isint);
+    memset(out, 0, sizeof *out);
+    if (!x) return 0;
+    const int *first_ngn = ngn + (band == 1 ? 0 : L.ngs[band - 2]);
+    int ngc = L.ngc[band - 1];
+    out->p = oracle_reduce(x, nd, ext, gfirst ? 0 : 1, ngc, first_ngn, L.rwgt + (band - 1) * 16, weighted);
+    out->nd = nd;
+    for (int d = 0; d < 4; ++d) out->ext[d] = ext[d];
+    out->ext[gfirst ? 0 : nd - 1] = ngc;
+    long n = 1;
+    for (int d = 0; d < nd; ++d) n *= out->ext[d];
+    snprintf(key, sizeof key, "lw/b%02d/%s", band, rname);
+    oracle_register(key, out->p, n);
+    return 1;
+}
+
+int oracle_lw_init(const char *path, double cpdair)
+{
+    otbl_t t;
+    if (otbl_load(&t, path) != 0) return -1;
+    /* lwdatinit (rrtmg_lw_init.f90:178-281) */
+    L.grav = 9.8066;
+    L.avogad = 6.02214199e+23;
+    L.heatfac = L.grav * 8.6400e4 / (cpdair * 1.e2);
+    const int *ngc = otbl_i(&t, "lw/ngc", NULL), *ngs = otbl_i(&t, "lw/ngs", NULL);
+    const int *ngn = otbl_i(&t, "lw/ngn", NULL), *ngm = otbl_i(&t, "lw/ngm", NULL);
+    const int *nspa = otbl_i(&t, "lw/nspa", NULL), *nspb = otbl_i(&t, "lw/nspb", NULL);
+    const double *wt = otbl_d(&t, "lw/wt", NULL);
+    if (!ngc || !ngs || !ngn || !ngm || !nspa || !nspb || !wt) return -2;
+    memcpy(L.ngc, ngc, sizeof L.ngc);
+    memcpy(L.ngs, ngs, sizeof L.ngs);
+    memcpy(L.delwave, otbl_d(&t, "lw/delwave", NULL), sizeof L.delwave);
+    memcpy(L.pref, otbl_d(&t, "lw/pref", NULL), sizeof L.pref);
+    memcpy(L.preflog, otbl_d(&t, "lw/preflog", NULL), sizeof L.preflog);
+    memcpy(L.tref, otbl_d(&t, "lw/tref", NULL), sizeof L.tref);
+    memcpy(L.chi_mls, otbl_d(&t, "lw/chi_mls", NULL), sizeof L.chi_mls);
+    memcpy(L.totplnk, otbl_d(&t, "lw/totplnk", NULL), sizeof L.totplnk);
+    memcpy(L.a0, otbl_d(&t, "lw/a0", NULL), sizeof L.a0);
+    memcpy(L.a1, otbl_d(&t, "lw/a1", NULL), sizeof L.a1);
+    memcpy(L.a2, otbl_d(&t, "lw/a2", NULL), sizeof L.a2);
+
+    /* lookup tables (rrtmg_lw_init.f90:106-123) */
+    const double expeps = 1.e-20;
+    L.tau_tbl[0] = 0.0;
+    L.tau_tbl[ORACLE_NTBL] = 1.e10;
+    L.exp_tbl[0] = 1.0;
+    L.exp_tbl[ORACLE_NTBL] = expeps;
+    L.tfn_tbl[0] = 0.0;
+    L.tfn_tbl[ORACLE_NTBL] = 1.0;
+    L.bpade = 1.0 / 0.278;
+    for (int itr = 1; itr <= ORACLE_NTBL - 1; ++itr) {
+        double tfn = (double)itr / (double)ORACLE_NTBL;
+        L.tau_tbl[itr] = L.bpade * tfn / (1.0 - tfn);
+        L.exp_tbl[itr] = exp(-L.tau_tbl[itr]);
+        if (L.exp_tbl[itr] <= expeps) L.exp_tbl[itr] = expeps;
+        if (L.tau_tbl[itr] < 0.06)
+            L.tfn_tbl[itr] = L.tau_tbl[itr] / 6.0;
+        else
+            L.tfn_tbl[itr] = 1.0 - 2.0 * ((1.0 / L.tau_tbl[itr]) - (L.exp_tbl[itr] / (1. - L.exp_tbl[itr])));
+    }
+    oracle_make_rwgt(NBLW, ngc, ngn, ngm, wt, L.rwgt);
+    oracle_register("lw/rwgt", L.rwgt, NBLW * 16);
+
+    static const char *kn[][2] = {{"kao_mn2", "ka_mn2"},   {"kbo_mn2", "kb_mn2"},   {"kao_mn2o", "ka_mn2o"},
+                                  {"kbo_mn2o", "kb_mn2o"}, {"kao_mo3", "ka_mo3"},   {"kbo_mo3", "kb_mo3"},
+                                  {"kao_mco2", "ka_mco2"}, {"kbo_mco2", "kb_mco2"}, {"kao_mco", "ka_mco"},
+                                  {"kao_mo2", "ka_mo2"},   {"kbo_mo2", "kb_mo2"},   {"ccl4o", "ccl4"},
+                                  {"cfc11adjo", "cfc11adj"}, {"cfc12o", "cfc12"},   {"cfc22adjo", "cfc22adj"}};
+    for (int band = 1; band <= NBLW; ++band) {
+        lwband_t *bb = &L.b[band - 1];
+        memset(bb, 0, sizeof *bb);
+        bb->ng = ngc[band - 1];
+        bb->nspa = nspa[band - 1];
+        bb->nspb = nspb[band - 1];
+        reduce_lw(&t, band, "kao", "ka", 0, 1, ngn, &bb->absa);
+        /* absa(65*nspa, ng): flatten the leading dims */
+        if (bb->absa.p) {
+            int rows = 1;
+            for (int d = 0; d < bb->absa.nd - 1; ++d) rows *= bb->absa.ext[d];
+            bb->absa.ext[0] = rows;
+        }
+        reduce_lw(&t, band, "kbo", "kb", 0, 1, ngn, &bb->absb);
+        if (bb->absb.p) {
+            int rows = 1;
+            for (int d = 0; d < bb->absb.nd - 1; ++d) rows *= bb->absb.ext[d];
+            bb->absb.ext[0] = rows;
+        }
+        reduce_lw(&t, band, "selfrefo", "selfref", 0, 1, ngn, &bb->selfref);
+        reduce_lw(&t, band, "forrefo", "forref", 0, 1, ngn, &bb->forref);
+        reduce_lw(&t, band, "fracrefao", "fracrefa", 1, 0, ngn, &bb->fracrefa);
+        reduce_lw(&t, band, "fracrefbo", "fracrefb", 1, 0, ngn, &bb->fracrefb);
+        rarr_t *slots[] = {&bb->ka_mn2,  &bb->kb_mn2,  &bb->ka_mn2o, &bb->kb_mn2o,   &bb->ka_mo3,
+                           &bb->kb_mo3,  &bb->ka_mco2, &bb->kb_mco2, &bb->ka_mco,    &bb->ka_mo2,
+                           &bb->kb_mo2,  &bb->ccl4,    &bb->cfc11adj, &bb->cfc12,    &bb->cfc22adj};
+        for (int s = 0; s < 15; ++s) reduce_lw(&t, band, kn[s][0], kn[s][1], 0, 1, ngn, slots[s]);
+    }
+    otbl_free(&t);
+    L.ready = 1;
+    return 0;
+}
+
+double oracle_lw_tbl(int which, int i)
+{
+    return which == 0 ? L.tau_tbl[i] : which == 1 ? L.exp_tbl[i] : L.tfn_tbl[i];
+}
+
+double oracle_heatfac(int lw);
+double oracle_heatfac_sw(void);
+double oracle_heatfac(int lw) { return lw ? L.heatfac : oracle_heatfac_sw(); }
+
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int nlayers, laytrop;
+    double pavel[MAXLAY + 1], tavel[MAXLAY + 1], pz[MAXLAY + 1], tz[MAXLAY + 1], tbound;
+    double semiss[17], coldry[MAXLAY + 1], wkl[8][MAXLAY + 1], wbrodl[MAXLAY + 1], wx[5][MAXLAY + 1];
+    double pwvcm;
+    int jp[MAXLAY + 1], jt[MAXLAY + 1], jt1[MAXLAY + 1];
+    double planklay[MAXLAY + 1][17], planklev[MAXLAY + 1][17], plankbnd[17];
+    double colh2o[MAXLAY + 1], colco2[MAXLAY + 1], colo3[MAXLAY + 1], coln2o[MAXLAY + 1];
+    double colco[MAXLAY + 1], colch4[MAXLAY + 1], colo2[MAXLAY + 1], colbrd[MAXLAY + 1];
+    double fac00[MAXLAY + 1], fac01[MAXLAY + 1], fac10[MAXLAY + 1], fac11[MAXLAY + 1];
+    double rat_h2oco2[MAXLAY + 1], rat_h2oco2_1[MAXLAY + 1], rat_h2oo3[MAXLAY + 1], rat_h2oo3_1[MAXLAY + 1];
+    double rat_h2on2o[MAXLAY + 1], rat_h2on2o_1[MAXLAY + 1], rat_h2och4[MAXLAY + 1], rat_h2och4_1[MAXLAY + 1];
+    double rat_n2oco2[MAXLAY + 1], rat_n2oco2_1[MAXLAY + 1], rat_o3co2[MAXLAY + 1], rat_o3co2_1[MAXLAY + 1];
+    double selffac[MAXLAY + 1], selffrac[MAXLAY + 1], forfac[MAXLAY + 1], forfrac[MAXLAY + 1];
+    int indself[MAXLAY + 1], indfor[MAXLAY + 1], indminor[MAXLAY + 1];
+    double minorfrac[MAXLAY + 1], scaleminor[MAXLAY + 1], scaleminorn2[MAXLAY + 1];
+    double taug[MAXLAY + 1][NGLW + 1], fracs[MAXLAY + 1][NGLW + 1];
+} lwcol_t;
+
+/* inatm (rrtmg_lw_rad.nomcica.f90:729-852), clear sky, iaer handled by caller */
+static void inatm(lwcol_t *c, int iplon, int ncol, int nlay, const double *play, const double *plev,
+                  const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                  const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                  const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                  const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                  const double *emis)
+{
+    const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
+#define IN(a, i, k) a[(i) + (long)ncol * ((k)-1)]
+    c->nlayers = nlay;
+    for (int m = 0; m < 8; ++m)
+        for (int l = 0; l <= nlay; ++l) c->wkl[m][l] = 0.0;
+    for (int m = 0; m < 5; ++m)
+        for (int l = 0; l <= nlay; ++l) c->wx[m][l] = 0.0;
+    double amttl = 0.0, wvttl = 0.0;
+    c->tbound = tsfc[iplon];
+    c->pz[0] = IN(plev, iplon, 1);
+    c->tz[0] = IN(tlev, iplon, 1);
+    for (int l = 1; l <= nlay; ++l) {
+        c->pavel[l] = IN(play, iplon, l);
+        c->tavel[l] = IN(tlay, iplon, l);
+        c->pz[l] = IN(plev, iplon, l + 1);
+        c->tz[l] = IN(tlev, iplon, l + 1);
+        c->wkl[1][l] = (IN(h2ovmr, iplon, l) / (1.0 - IN(h2ovmr, iplon, l))) * amdw;
+        c->wkl[2][l] = IN(co2vmr, iplon, l);
+        c->wkl[3][l] = IN(o3vmr, iplon, l) * amdo;
+        c->wkl[4][l] = IN(n2ovmr, iplon, l);
+        c->wkl[6][l] = IN(ch4vmr, iplon, l);
+        c->wkl[7][l] = IN(o2vmr, iplon, l);
+        double amm = (1.0 - c->wkl[1][l]) * amd + c->wkl[1][l] * amw;
+        c->coldry[l] = (c->pz[l - 1] - c->pz[l]) * 1.e3 * L.avogad / (1.e2 * L.grav * amm * (1.0 + c->wkl[1][l]));
+    }
+    for (int l = 1; l <= nlay; ++l) {
+        c->wx[1][l] = IN(ccl4vmr, iplon, l);
+        c->wx[2][l] = IN(cfc11vmr, iplon, l);
+        c->wx[3][l] = IN(cfc12vmr, iplon, l);
+        c->wx[4][l] = IN(cfc22vmr, iplon, l);
+    }
+    for (int l = 1; l <= nlay; ++l) {
+        double summol = 0.0;
+        for (int imol = 2; imol <= 7; ++imol) summol = summol + c->wkl[imol][l];
+        c->wbrodl[l] = c->coldry[l] * (1.0 - summol);
+        for (int imol = 1; imol <= 7; ++imol) c->wkl[imol][l] = c->coldry[l] * c->wkl[imol][l];
+        amttl = amttl + c->coldry[l] + c->wkl[1][l];
+        wvttl = wvttl + c->wkl[1][l];
+        for (int ix = 1; ix <= 4; ++ix) c->wx[ix][l] = c->coldry[l] * c->wx[ix][l] * 1.e-20;
+    }
+    double wvsh = (amw * wvttl) / (amd * amttl);
+    c->pwvcm = wvsh * (1.e3 * c->pz[0]) / (1.e2 * L.grav);
+    for (int n = 1; n <= 16; ++n) c->semiss[n] = emis[iplon + (long)ncol * (n - 1)];
+#undef IN
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* setcoef (rrtmg_lw_setcoef.f90:150-413), istart = 1, idrv = 0 */
+static void setcoef(lwcol_t *c)
+{
+    const double stpfac = 296.0 / 1013.0;
+    int indbound = clampi((int)(c->tbound - 159.0), 1, 180);
+    double tbndfrac = c->tbound - 159.0 - (double)indbound;
+    int indlev0 = clampi((int)(c->tz[0] - 159.0), 1, 180);
+    double t0frac = c->tz[0] - 159.0 - (double)indlev0;
+    c->laytrop = 0;
+    for (int lay = 1; lay <= c->nlayers; ++lay) {
+        int indlay = clampi((int)(c->tavel[lay] - 159.0), 1, 180);
+        double tlayfrac = c->tavel[lay] - 159.0 - (double)indlay;
+        int indlev = clampi((int)(c->tz[lay] - 159.0), 1, 180);
+        double tlevfrac = c->tz[lay] - 159.0 - (double)indlev;
+        /* bands 1-15, then band 16 through the same totplnk path (istart = 1) */
+        for (int iband = 1; iband <= 16; ++iband) {
+            double dbdtlev, dbdtlay;
+            if (lay == 1) {
+                dbdtlev = TOTPLNK(indbound + 1, iband) - TOTPLNK(indbound, iband);
+                c->plankbnd[iband] = c->semiss[iband] * (TOTPLNK(indbound, iband) + tbndfrac * dbdtlev);
+                dbdtlev = TOTPLNK(indlev0 + 1, iband) - TOTPLNK(indlev0, iband);
+                c->planklev[0][iband] = TOTPLNK(indlev0, iband) + t0frac * dbdtlev;
+            }
+            dbdtlev = TOTPLNK(indlev + 1, iband) - TOTPLNK(indlev, iband);
+            dbdtlay = TOTPLNK(indlay + 1, iband) - TOTPLNK(indlay, iband);
+            c->planklay[lay][iband] = TOTPLNK(indlay, iband) + tlayfrac * dbdtlay;
+            c->planklev[lay][iband] = TOTPLNK(indlev, iband) + tlevfrac * dbdtlev;
+        }
+        double plog = log(c->pavel[lay]);
+        c->jp[lay] = clampi((int)(36.0 - 5 * (plog + 0.04)), 1, 58);
+        int jp1 = c->jp[lay] + 1;
+        double fp = 5.0 * (L.preflog[c->jp[lay] - 1] - plog);
+        c->jt[lay] = clampi((int)(3.0 + (c->tavel[lay] - L.tref[c->jp[lay] - 1]) / 15.0), 1, 4);
+        double ft = ((c->tavel[lay] - L.tref[c->jp[lay] - 1]) / 15.0) - (double)(c->jt[lay] - 3);
+        c->jt1[lay] = clampi((int)(3.0 + (c->tavel[lay] - L.tref[jp1 - 1]) / 15.0), 1, 4);
+        double ft1 = ((c->tavel[lay] - L.tref[jp1 - 1]) / 15.0) - (double)(c->jt1[lay] - 3);
+        double water = c->wkl[1][lay] / c->coldry[lay];
+        double scalefac = c->pavel[lay] * stpfac / c->tavel[lay];
+        const int jp_ = c->jp[lay];
+        if (plog > 4.56) {
+            c->laytrop = c->laytrop + 1;
+            c->forfac[lay] = scalefac / (1. + water);
+            double factor = (332.0 - c->tavel[lay]) / 36.0;
+            c->indfor[lay] = clampi((int)factor, 1, 2);
+            c->forfrac[lay] = factor - (double)c->indfor[lay];
+            c->selffac[lay] = water * c->forfac[lay];
+            factor = (c->tavel[lay] - 188.0) / 7.2;
+            c->indself[lay] = clampi((int)factor - 7, 1, 9);
+            c->selffrac[lay] = factor - (double)(c->indself[lay] + 7);
+            c->scaleminor[lay] = c->pavel[lay] / c->tavel[lay];
+            c->scaleminorn2[lay] = (c->pavel[lay] / c->tavel[lay]) * (c->wbrodl[lay] / (c->coldry[lay] + c->wkl[1][lay]));
+            factor = (c->tavel[lay] - 180.8) / 7.2;
+            c->indminor[lay] = clampi((int)factor, 1, 18);
+            c->minorfrac[lay] = factor - (double)c->indminor[lay];
+            c->rat_h2oco2[lay] = CHI(1, jp_) / CHI(2, jp_);
+            c->rat_h2oco2_1[lay] = CHI(1, jp_ + 1) / CHI(2, jp_ + 1);
+            c->rat_h2oo3[lay] = CHI(1, jp_) / CHI(3, jp_);
+            c->rat_h2oo3_1[lay] = CHI(1, jp_ + 1) / CHI(3, jp_ + 1);
+            c->rat_h2on2o[lay] = CHI(1, jp_) / CHI(4, jp_);
+            c->rat_h2on2o_1[lay] = CHI(1, jp_ + 1) / CHI(4, jp_ + 1);
+            c->rat_h2och4[lay] = CHI(1, jp_) / CHI(6, jp_);
+            c->rat_h2och4_1[lay] = CHI(1, jp_ + 1) / CHI(6, jp_ + 1);
+            c->rat_n2oco2[lay] = CHI(4, jp_) / CHI(2, jp_);
+            c->rat_n2oco2_1[lay] = CHI(4, jp_ + 1) / CHI(2, jp_ + 1);
+        } else {
+            c->forfac[lay] = scalefac / (1. + water);
+            double factor = (c->tavel[lay] - 188.0) / 36.0;
+            c->indfor[lay] = 3;
+            c->forfrac[lay] = factor - 1.0;
+            c->selffac[lay] = water * c->forfac[lay];
+            c->scaleminor[lay] = c->pavel[lay] / c->tavel[lay];
+            c->scaleminorn2[lay] = (c->pavel[lay] / c->tavel[lay]) * (c->wbrodl[lay] / (c->coldry[lay] + c->wkl[1][lay]));
+            factor = (c->tavel[lay] - 180.8) / 7.2;
+            c->indminor[lay] = clampi((int)factor, 1, 18);
+            c->minorfrac[lay] = factor - (double)c->indminor[lay];
+            c->rat_h2oco2[lay] = CHI(1, jp_) / CHI(2, jp_);
+            c->rat_h2oco2_1[lay] = CHI(1, jp_ + 1) / CHI(2, jp_ + 1);
+            c->rat_o3co2[lay] = CHI(3, jp_) / CHI(2, jp_);
+            c->rat_o3co2_1[lay] = CHI(3, jp_ + 1) / CHI(2, jp_ + 1);
+            c->indself[lay] = 0; /* not set by the reference above laytrop; unused there */
+            c->selffrac[lay] = 0.0;
+        }
+        c->colh2o[lay] = 1.e-20 * c->wkl[1][lay];
+        c->colco2[lay] = 1.e-20 * c->wkl[2][lay];
+        c->colo3[lay] = 1.e-20 * c->wkl[3][lay];
+        c->coln2o[lay] = 1.e-20 * c->wkl[4][lay];
+        c->colco[lay] = 1.e-20 * c->wkl[5][lay];
+        c->colch4[lay] = 1.e-20 * c->wkl[6][lay];
+        c->colo2[lay] = 1.e-20 * c->wkl[7][lay];
+        if (c->colco2[lay] == 0.0) c->colco2[lay] = 1.e-32 * c->coldry[lay];
+        if (c->colo3[lay] == 0.0) c->colo3[lay] = 1.e-32 * c->coldry[lay];
+        if (c->coln2o[lay] == 0.0) c->coln2o[lay] = 1.e-32 * c->coldry[lay];
+        if (c->colco[lay] == 0.0) c->colco[lay] = 1.e-32 * c->coldry[lay];
+        if (c->colch4[lay] == 0.0) c->colch4[lay] = 1.e-32 * c->coldry[lay];
+        c->colbrd[lay] = 1.e-20 * c->wbrodl[lay];
+        double compfp = 1. - fp;
+        c->fac10[lay] = compfp * ft;
+        c->fac00[lay] = compfp * (1.0 - ft);
+        c->fac11[lay] = fp * ft1;
+        c->fac01[lay] = fp * (1.0 - ft1);
+        c->selffac[lay] = c->colh2o[lay] * c->selffac[lay];
+        c->forfac[lay] = c->colh2o[lay] * c->forfac[lay];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* taumol helpers                                                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    double speccomb, specparm;
+    int js;
+    double fs;
+} eta_t;
+
+/* the speccomb/specparm/js/fs block that opens every binary-species band */
+static void eta(double cola, double colb, double rat, double mult, eta_t *e)
+{
+    e->speccomb = cola + rat * colb;
+    e->specparm = cola / e->speccomb;
+    if (e->specparm >= oneminus_lw) e->specparm = oneminus_lw;
+    double specmult = mult * (e->specparm);
+    e->js = 1 + (int)specmult;
+    e->fs = fmod(specmult, 1.0);
+}
+
+/* lower-atmosphere major absorber with the 3-point eta scheme near the edges
+ * (e.g. rrtmg_lw_taumol.f90:516-668): one of the two pressure neighbours.
+ * facA = fac00 or fac01, facB = fac10 or fac11 of the layer. */
+static double major3(const rarr_t *absa, const eta_t *e, int ind, double facA, double facB, int ig)
+{
+    double f0, f1, f2, g0, g1, g2;
+#define AB(i) A2(*absa, (i), ig)
+    if (e->specparm < 0.125) {
+        double p = e->fs - 1;
+        double p2 = p * p, p4 = p2 * p2;
+        double fk0 = p4, fk1 = 1 - p - 2.0 * p4, fk2 = p + p4;
+        f0 = fk0 * facA; f1 = fk1 * facA; f2 = fk2 * facA;
+        g0 = fk0 * facB; g1 = fk1 * facB; g2 = fk2 * facB;
+        return e->speccomb * (f0 * AB(ind) + f1 * AB(ind + 1) + f2 * AB(ind + 2) + g0 * AB(ind + 9) +
+                              g1 * AB(ind + 10) + g2 * AB(ind + 11));
+    } else if (e->specparm > 0.875) {
+        double p = -e->fs;
+        double p2 = p * p, p4 = p2 * p2;
+        double fk0 = p4, fk1 = 1 - p - 2.0 * p4, fk2 = p + p4;
+        f0 = fk0 * facA; f1 = fk1 * facA; f2 = fk2 * facA;
+        g0 = fk0 * facB; g1 = fk1 * facB; g2 = fk2 * facB;
+        return e->speccomb * (f2 * AB(ind - 1) + f1 * AB(ind) + f0 * AB(ind + 1) + g2 * AB(ind + 8) +
+                              g1 * AB(ind + 9) + g0 * AB(ind + 10));
+    } else {
+        f0 = (1.0 - e->fs) * facA;
+        g0 = (1.0 - e->fs) * facB;
+        f1 = e->fs * facA;
+        g1 = e->fs * facB;
+        return e->speccomb * (f0 * AB(ind) + f1 * AB(ind + 1) + g0 * AB(ind + 9) + g1 * AB(ind + 10));
+    }
+#undef AB
+}
+
+/* upper-atmosphere binary-species major term (e.g. rrtmg_lw_taumol.f90:740-749) */
+static double major_b(const rarr_t *absb, const eta_t *e, int ind, double facA, double facB, int ig)
+{
+    double f0 = (1.0 - e->fs) * facA, g0 = (1.0 - e->fs) * facB;
+    double f1 = e->fs * facA, g1 = e->fs * facB;
+#define AB(i) A2(*absb, (i), ig)
+    return e->speccomb * (f0 * AB(ind) + f1 * AB(ind + 1) + g0 * AB(ind + 5) + g1 * AB(ind + 6));
+#undef AB
+}
+
+static double four(const rarr_t *ab, const lwcol_t *c, int lay, int ind0, int ind1, int ig)
+{
+    return c->fac00[lay] * A2(*ab, ind0, ig) + c->fac10[lay] * A2(*ab, ind0 + 1, ig) +
+           c->fac01[lay] * A2(*ab, ind1, ig) + c->fac11[lay] * A2(*ab, ind1 + 1, ig);
+}
+
+static double tself(const lwband_t *bb, const lwcol_t *c, int lay, int ig)
+{
+    int inds = c->indself[lay];
+    return c->selffac[lay] *
+           (A2(bb->selfref, inds, ig) + c->selffrac[lay] * (A2(bb->selfref, inds + 1, ig) - A2(bb->selfref, inds, ig)));
+}
+
+static double tfor(const lwband_t *bb, const lwcol_t *c, int lay, int ig)
+{
+    int indf = c->indfor[lay];
+    return c->forfac[lay] *
+           (A2(bb->forref, indf, ig) + c->forfrac[lay] * (A2(bb->forref, indf + 1, ig) - A2(bb->forref, indf, ig)));
+}
+
+/* minor absorber interpolated in temperature only, k(19, ng) */
+static double minor1(const rarr_t *k, const lwcol_t *c, int lay, int ig)
+{
+    int indm = c->indminor[lay];
+    return A2(*k, indm, ig) + c->minorfrac[lay] * (A2(*k, indm + 1, ig) - A2(*k, indm, ig));
+}
+
+/* minor absorber interpolated in eta and temperature, k(neta, 19, ng) */
+static double minor2(const rarr_t *k, const lwcol_t *c, int lay, int jm, double fm, int ig)
+{
+    int indm = c->indminor[lay];
+    double m1 = A3(*k, jm, indm, ig) + fm * (A3(*k, jm + 1, indm, ig) - A3(*k, jm, indm, ig));
+    double m2 = A3(*k, jm, indm + 1, ig) + fm * (A3(*k, jm + 1, indm + 1, ig) - A3(*k, jm, indm + 1, ig));
+    return m1 + c->minorfrac[lay] * (m2 - m1);
+}
+
+static double planck_frac(const rarr_t *fr, const eta_t *e, int ig)
+{
+    return FR2(*fr, ig, e->js) + e->fs * (FR2(*fr, ig, e->js + 1) - FR2(*fr, ig, e->js));
+}
+
+/* column-amount adjustments of the minor CO2 / N2O (e.g. rrtmg_lw_taumol.f90:1314-1320) */
+static double adjcol(double col, double coldry, double chi_ref, double thresh, double base, double ex)
+{
+    double chi = col / coldry;
+    double rat = 1.e20 * chi / chi_ref;
+    if (rat > thresh) {
+        double adjfac = base + pow(rat - base, ex);
+        return adjfac * chi_ref * coldry * 1.e-20;
+    }
+    return col;
+}
+
+static void taumol(lwcol_t *c)
+{
+    const int nl = c->nlayers, lt = c->laytrop;
+    const int *jp = c->jp, *jt = c->jt, *jt1 = c->jt1;
+    int lay, ig, ind0, ind1;
+    const lwband_t *bb;
+    eta_t e, e1, em, ep;
+
+#define IND0A(nsp, js_) (((jp[lay] - 1) * 5 + (jt[lay] - 1)) * (nsp) + (js_))
+#define IND1A(nsp, js_) ((jp[lay] * 5 + (jt1[lay] - 1)) * (nsp) + (js_))
+#define IND0B(nsp, js_) (((jp[lay] - 13) * 5 + (jt[lay] - 1)) * (nsp) + (js_))
+#define IND1B(nsp, js_) (((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * (nsp) + (js_))
+
+    /* band 1: rrtmg_lw_taumol.f90:280-373 */
+    bb = &L.b[0];
+    for (lay = 1; lay <= lt; ++lay) {
+        ind0 = IND0A(bb->nspa, 1);
+        ind1 = IND1A(bb->nspa, 1);
+        double pp = c->pavel[lay];
+        double corradj = 1.;
+        if (pp < 250.0) corradj = 1.0 - 0.15 * (250.0 - pp) / 154.4;
+        double scalen2 = c->colbrd[lay] * c->scaleminorn2[lay];
+        for (ig = 1; ig <= bb->ng; ++ig) {
+            double taun2 = scalen2 * minor1(&bb->ka_mn2, c, lay, ig);
+            c->taug[lay][ig] = corradj * (c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                          tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) + taun2);
+            c->fracs[lay][ig] = G1(bb->fracrefa, ig);
+        }
+    }
+    for (lay = lt + 1; lay <= nl; ++lay) {
+        ind0 = IND0B(bb->nspb, 1);
+        ind1 = IND1B(bb->nspb, 1);
+        double pp = c->pavel[lay];
+        double corradj = 1.0 - 0.15 * (pp / 95.6);
+        double scalen2 = c->colbrd[lay] * c->scaleminorn2[lay];
+        for (ig = 1; ig <= bb->ng; ++ig) {
+            double taun2 = scalen2 * minor1(&bb->kb_mn2, c, lay, ig);
+            c->taug[lay][ig] = corradj * (c->colh2o[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) +
+                                          tfor(bb, c, lay, ig) + taun2);
+            c->fracs[lay][ig] = G1(bb->fracrefb, ig);
+        }
+    }
+
+    /* band 2: rrtmg_lw_taumol.f90:376-445 */
+    bb = &L.b[1];
+    {
+        const int ngs = 10;
+        for (lay = 1; lay <= lt; ++lay) {
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            double pp = c->pavel[lay];
+            double corradj = 1.0 - .05 * (pp - 100.0) / 900.0;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = corradj * (c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                                    tself(bb, c, lay, ig) + tfor(bb, c, lay, ig));
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 3: rrtmg_lw_taumol.f90:448-760 */
+    bb = &L.b[2];
+    {
+        const int ngs = 22;
+        const double refrat_planck_a = CHI(1, 9) / CHI(2, 9), refrat_planck_b = CHI(1, 13) / CHI(2, 13);
+        const double refrat_m_a = CHI(1, 3) / CHI(2, 3), refrat_m_b = CHI(1, 13) / CHI(2, 13);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_m_a, 8.0, &em);
+            double adjcoln2o = adjcol(c->coln2o[lay], c->coldry[lay], CHI(4, jp[lay] + 1), 1.5, 0.5, 0.65);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absn2o = minor2(&bb->ka_mn2o, c, lay, em.js, em.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) +
+                                         adjcoln2o * absn2o;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2[lay], 4.0, &e);
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2_1[lay], 4.0, &e1);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_m_b, 4.0, &em);
+            double adjcoln2o = adjcol(c->coln2o[lay], c->coldry[lay], CHI(4, jp[lay] + 1), 1.5, 0.5, 0.65);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_planck_b, 4.0, &ep);
+            ind0 = IND0B(bb->nspb, e.js);
+            ind1 = IND1B(bb->nspb, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absn2o = minor2(&bb->kb_mn2o, c, lay, em.js, em.fs, ig);
+                c->taug[lay][ngs + ig] = major_b(&bb->absb, &e, ind0, c->fac00[lay], c->fac10[lay], ig) +
+                                         major_b(&bb->absb, &e1, ind1, c->fac01[lay], c->fac11[lay], ig) +
+                                         tfor(bb, c, lay, ig) + adjcoln2o * absn2o;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefb, &ep, ig);
+            }
+        }
+    }
+
+    /* band 4: rrtmg_lw_taumol.f90:763-1019 */
+    bb = &L.b[3];
+    {
+        const int ngs = 38;
+        const double refrat_planck_a = CHI(1, 11) / CHI(2, 11), refrat_planck_b = CHI(3, 13) / CHI(2, 13);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            eta(c->colo3[lay], c->colco2[lay], c->rat_o3co2[lay], 4.0, &e);
+            eta(c->colo3[lay], c->colco2[lay], c->rat_o3co2_1[lay], 4.0, &e1);
+            eta(c->colo3[lay], c->colco2[lay], refrat_planck_b, 4.0, &ep);
+            ind0 = IND0B(bb->nspb, e.js);
+            ind1 = IND1B(bb->nspb, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = major_b(&bb->absb, &e, ind0, c->fac00[lay], c->fac10[lay], ig) +
+                                         major_b(&bb->absb, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefb, &ep, ig);
+            }
+            c->taug[lay][ngs + 8] = c->taug[lay][ngs + 8] * 0.92;
+            c->taug[lay][ngs + 9] = c->taug[lay][ngs + 9] * 0.88;
+            c->taug[lay][ngs + 10] = c->taug[lay][ngs + 10] * 1.07;
+            c->taug[lay][ngs + 11] = c->taug[lay][ngs + 11] * 1.1;
+            c->taug[lay][ngs + 12] = c->taug[lay][ngs + 12] * 0.99;
+            c->taug[lay][ngs + 13] = c->taug[lay][ngs + 13] * 0.88;
+            c->taug[lay][ngs + 14] = c->taug[lay][ngs + 14] * 0.943;
+        }
+    }
+
+    /* band 5: rrtmg_lw_taumol.f90:1022-1294 */
+    bb = &L.b[4];
+    {
+        const int ngs = 52;
+        const double refrat_planck_a = CHI(1, 5) / CHI(2, 5), refrat_planck_b = CHI(3, 43) / CHI(2, 43);
+        const double refrat_m_a = CHI(1, 7) / CHI(2, 7);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_m_a, 8.0, &em);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double abso3 = minor2(&bb->ka_mo3, c, lay, em.js, em.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) +
+                                         abso3 * c->colo3[lay] + c->wx[1][lay] * G1(bb->ccl4, ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            eta(c->colo3[lay], c->colco2[lay], c->rat_o3co2[lay], 4.0, &e);
+            eta(c->colo3[lay], c->colco2[lay], c->rat_o3co2_1[lay], 4.0, &e1);
+            eta(c->colo3[lay], c->colco2[lay], refrat_planck_b, 4.0, &ep);
+            ind0 = IND0B(bb->nspb, e.js);
+            ind1 = IND1B(bb->nspb, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = major_b(&bb->absb, &e, ind0, c->fac00[lay], c->fac10[lay], ig) +
+                                         major_b(&bb->absb, &e1, ind1, c->fac01[lay], c->fac11[lay], ig) +
+                                         c->wx[1][lay] * G1(bb->ccl4, ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefb, &ep, ig);
+            }
+        }
+    }
+
+    /* band 6: rrtmg_lw_taumol.f90:1297-1380 */
+    bb = &L.b[5];
+    {
+        const int ngs = 68;
+        for (lay = 1; lay <= lt; ++lay) {
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], CHI(2, jp[lay] + 1), 3.0, 2.0, 0.77);
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor1(&bb->ka_mco2, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                         tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) + adjcolco2 * absco2 +
+                                         c->wx[2][lay] * G1(bb->cfc11adj, ig) + c->wx[3][lay] * G1(bb->cfc12, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = 0.0 + c->wx[2][lay] * G1(bb->cfc11adj, ig) + c->wx[3][lay] * G1(bb->cfc12, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+    }
+
+    /* band 7: rrtmg_lw_taumol.f90:1383-1654 */
+    bb = &L.b[6];
+    {
+        const int ngs = 76;
+        const double refrat_planck_a = CHI(1, 3) / CHI(3, 3), refrat_m_a = CHI(1, 3) / CHI(3, 3);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colo3[lay], c->rat_h2oo3[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colo3[lay], c->rat_h2oo3_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colo3[lay], refrat_m_a, 8.0, &em);
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], CHI(2, jp[lay] + 1), 3.0, 3.0, 0.79);
+            eta(c->colh2o[lay], c->colo3[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor2(&bb->ka_mco2, c, lay, em.js, em.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) +
+                                         adjcolco2 * absco2;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], CHI(2, jp[lay] + 1), 3.0, 2.0, 0.79);
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor1(&bb->kb_mco2, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colo3[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) + adjcolco2 * absco2;
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+            c->taug[lay][ngs + 6] = c->taug[lay][ngs + 6] * 0.92;
+            c->taug[lay][ngs + 7] = c->taug[lay][ngs + 7] * 0.88;
+            c->taug[lay][ngs + 8] = c->taug[lay][ngs + 8] * 1.07;
+            c->taug[lay][ngs + 9] = c->taug[lay][ngs + 9] * 1.1;
+            c->taug[lay][ngs + 10] = c->taug[lay][ngs + 10] * 0.99;
+            c->taug[lay][ngs + 11] = c->taug[lay][ngs + 11] * 0.855;
+        }
+    }
+
+    /* band 8: rrtmg_lw_taumol.f90:1657-1777 */
+    bb = &L.b[7];
+    {
+        const int ngs = 88;
+        for (lay = 1; lay <= lt; ++lay) {
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], CHI(2, jp[lay] + 1), 3.0, 2.0, 0.65);
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor1(&bb->ka_mco2, c, lay, ig);
+                double abso3 = minor1(&bb->ka_mo3, c, lay, ig);
+                double absn2o = minor1(&bb->ka_mn2o, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                         tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) + adjcolco2 * absco2 +
+                                         c->colo3[lay] * abso3 + c->coln2o[lay] * absn2o +
+                                         c->wx[3][lay] * G1(bb->cfc12, ig) + c->wx[4][lay] * G1(bb->cfc22adj, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], CHI(2, jp[lay] + 1), 3.0, 2.0, 0.65);
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor1(&bb->kb_mco2, c, lay, ig);
+                double absn2o = minor1(&bb->kb_mn2o, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colo3[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) +
+                                         adjcolco2 * absco2 + c->coln2o[lay] * absn2o +
+                                         c->wx[3][lay] * G1(bb->cfc12, ig) + c->wx[4][lay] * G1(bb->cfc22adj, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 9: rrtmg_lw_taumol.f90:1780-2040 */
+    bb = &L.b[8];
+    {
+        const int ngs = 96;
+        const double refrat_planck_a = CHI(1, 9) / CHI(6, 9), refrat_m_a = CHI(1, 3) / CHI(6, 3);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colch4[lay], c->rat_h2och4[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colch4[lay], c->rat_h2och4_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colch4[lay], refrat_m_a, 8.0, &em);
+            double adjcoln2o = adjcol(c->coln2o[lay], c->coldry[lay], CHI(4, jp[lay] + 1), 1.5, 0.5, 0.65);
+            eta(c->colh2o[lay], c->colch4[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absn2o = minor2(&bb->ka_mn2o, c, lay, em.js, em.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) +
+                                         adjcoln2o * absn2o;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            double adjcoln2o = adjcol(c->coln2o[lay], c->coldry[lay], CHI(4, jp[lay] + 1), 1.5, 0.5, 0.65);
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absn2o = minor1(&bb->kb_mn2o, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colch4[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) + adjcoln2o * absn2o;
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 10: rrtmg_lw_taumol.f90:2043-2107 */
+    bb = &L.b[9];
+    {
+        const int ngs = 108;
+        for (lay = 1; lay <= lt; ++lay) {
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                         tself(bb, c, lay, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 11: rrtmg_lw_taumol.f90:2110-2187 */
+    bb = &L.b[10];
+    {
+        const int ngs = 114;
+        for (lay = 1; lay <= lt; ++lay) {
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            double scaleo2 = c->colo2[lay] * c->scaleminor[lay];
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double tauo2 = scaleo2 * minor1(&bb->ka_mo2, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                         tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) + tauo2;
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            double scaleo2 = c->colo2[lay] * c->scaleminor[lay];
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double tauo2 = scaleo2 * minor1(&bb->kb_mo2, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colh2o[lay] * four(&bb->absb, c, lay, ind0, ind1, ig) +
+                                         tfor(bb, c, lay, ig) + tauo2;
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 12: rrtmg_lw_taumol.f90:2190-2392 */
+    bb = &L.b[11];
+    {
+        const int ngs = 122;
+        const double refrat_planck_a = CHI(1, 10) / CHI(2, 10);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colco2[lay], c->rat_h2oco2_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colco2[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = 0.0;
+                c->fracs[lay][ngs + ig] = 0.0;
+            }
+    }
+
+    /* band 13: rrtmg_lw_taumol.f90:2395-2652 */
+    bb = &L.b[12];
+    {
+        const int ngs = 130;
+        const double refrat_planck_a = CHI(1, 5) / CHI(4, 5), refrat_m_a = CHI(1, 1) / CHI(4, 1);
+        const double refrat_m_a3 = CHI(1, 3) / CHI(4, 3);
+        eta_t emco;
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->coln2o[lay], c->rat_h2on2o[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->coln2o[lay], c->rat_h2on2o_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->coln2o[lay], refrat_m_a, 8.0, &em);
+            double adjcolco2 = adjcol(c->colco2[lay], c->coldry[lay], 3.55e-4, 3.0, 2.0, 0.68);
+            eta(c->colh2o[lay], c->coln2o[lay], refrat_m_a3, 8.0, &emco);
+            eta(c->colh2o[lay], c->coln2o[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double absco2 = minor2(&bb->ka_mco2, c, lay, em.js, em.fs, ig);
+                double absco = minor2(&bb->ka_mco, c, lay, emco.js, emco.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) +
+                                         adjcolco2 * absco2 + c->colco[lay] * absco;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double abso3 = minor1(&bb->kb_mo3, c, lay, ig);
+                c->taug[lay][ngs + ig] = c->colo3[lay] * abso3;
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+    }
+
+    /* band 14: rrtmg_lw_taumol.f90:2655-2713 */
+    bb = &L.b[13];
+    {
+        const int ngs = 134;
+        for (lay = 1; lay <= lt; ++lay) {
+            ind0 = IND0A(bb->nspa, 1);
+            ind1 = IND1A(bb->nspa, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colco2[lay] * four(&bb->absa, c, lay, ind0, ind1, ig) +
+                                         tself(bb, c, lay, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefa, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colco2[lay] * four(&bb->absb, c, lay, ind0, ind1, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+
+    /* band 15: rrtmg_lw_taumol.f90:2716-2938 */
+    bb = &L.b[14];
+    {
+        const int ngs = 136;
+        const double refrat_planck_a = CHI(4, 1) / CHI(2, 1), refrat_m_a = CHI(4, 1) / CHI(2, 1);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->coln2o[lay], c->colco2[lay], c->rat_n2oco2[lay], 8.0, &e);
+            eta(c->coln2o[lay], c->colco2[lay], c->rat_n2oco2_1[lay], 8.0, &e1);
+            eta(c->coln2o[lay], c->colco2[lay], refrat_m_a, 8.0, &em);
+            eta(c->coln2o[lay], c->colco2[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            double scalen2 = c->colbrd[lay] * c->scaleminor[lay];
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double taun2 = scalen2 * minor2(&bb->ka_mn2, c, lay, em.js, em.fs, ig);
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig) + taun2;
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = 0.0;
+                c->fracs[lay][ngs + ig] = 0.0;
+            }
+    }
+
+    /* band 16: rrtmg_lw_taumol.f90:2941-3147 (nspb(16) = 0: ind0 = ind1 = 1 aloft) */
+    bb = &L.b[15];
+    {
+        const int ngs = 138;
+        const double refrat_planck_a = CHI(1, 6) / CHI(6, 6);
+        for (lay = 1; lay <= lt; ++lay) {
+            eta(c->colh2o[lay], c->colch4[lay], c->rat_h2och4[lay], 8.0, &e);
+            eta(c->colh2o[lay], c->colch4[lay], c->rat_h2och4_1[lay], 8.0, &e1);
+            eta(c->colh2o[lay], c->colch4[lay], refrat_planck_a, 8.0, &ep);
+            ind0 = IND0A(bb->nspa, e.js);
+            ind1 = IND1A(bb->nspa, e1.js);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                double tau_major = major3(&bb->absa, &e, ind0, c->fac00[lay], c->fac10[lay], ig);
+                double tau_major1 = major3(&bb->absa, &e1, ind1, c->fac01[lay], c->fac11[lay], ig);
+                c->taug[lay][ngs + ig] = tau_major + tau_major1 + tself(bb, c, lay, ig) + tfor(bb, c, lay, ig);
+                c->fracs[lay][ngs + ig] = planck_frac(&bb->fracrefa, &ep, ig);
+            }
+        }
+        for (lay = lt + 1; lay <= nl; ++lay) {
+            ind0 = IND0B(bb->nspb, 1);
+            ind1 = IND1B(bb->nspb, 1);
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colch4[lay] * four(&bb->absb, c, lay, ind0, ind1, ig);
+                c->fracs[lay][ngs + ig] = G1(bb->fracrefb, ig);
+            }
+        }
+    }
+#undef IND0A
+#undef IND1A
+#undef IND0B
+#undef IND1B
+}
+
+/* rtrnmr clear-sky branch (rrtmg_lw_rtrnmr.f90:246-777 with icldlyr = 0,
+ * iout = 0, idrv = 0); taut = taug + taua with taua = 0 (iaer = 10). */
+static void rtrnmr(const lwcol_t *c, double *totuflux, double *totdflux, double *htr)
+{
+    const int nl = c->nlayers;
+    const double wtdiff = 0.5, rec_6 = 0.166667;
+    const double pi = 2.0 * asin(1.0);
+    const double fluxfac = pi * 2.e4;
+    double secdiff[17];
+    double urad[MAXLAY + 1], drad[MAXLAY + 1], atrans[MAXLAY + 1], bbugas[MAXLAY + 1];
+    double fnet[MAXLAY + 1];
+    for (int ibnd = 1; ibnd <= 16; ++ibnd) {
+        if (ibnd == 1 || ibnd == 4 || ibnd >= 10) {
+            secdiff[ibnd] = 1.66;
+        } else {
+            secdiff[ibnd] = L.a0[ibnd - 1] + L.a1[ibnd - 1] * exp(L.a2[ibnd - 1] * c->pwvcm);
+            if (secdiff[ibnd] > 1.80) secdiff[ibnd] = 1.80;
+            if (secdiff[ibnd] < 1.50) secdiff[ibnd] = 1.50;
+        }
+    }
+    for (int lev = 0; lev <= nl; ++lev) {
+        urad[lev] = 0.0;
+        drad[lev] = 0.0;
+        totuflux[lev] = 0.0;
+        totdflux[lev] = 0.0;
+    }
+    int igc = 1;
+    for (int iband = 1; iband <= 16; ++iband) {
+        for (; igc <= L.ngs[iband - 1]; ++igc) {
+            double radld = 0.0;
+            for (int lev = nl; lev >= 1; --lev) {
+                double plfrac = c->fracs[lev][igc];
+                double blay = c->planklay[lev][iband];
+                double dplankup = c->planklev[lev][iband] - blay;
+                double dplankdn = c->planklev[lev - 1][iband] - blay;
+                double odepth = secdiff[iband] * c->taug[lev][igc];
+                if (odepth < 0.0) odepth = 0.0;
+                double bbd;
+                if (odepth <= 0.06) {
+                    atrans[lev] = odepth - 0.5 * odepth * odepth;
+                    odepth = rec_6 * odepth;
+                    bbd = plfrac * (blay + dplankdn * odepth);
+                    bbugas[lev] = plfrac * (blay + dplankup * odepth);
+                } else {
+                    double tblind = odepth / (L.bpade + odepth);
+                    int itr = (int)(1.e4 * tblind + 0.5);
+                    double transc = L.exp_tbl[itr];
+                    atrans[lev] = 1.0 - transc;
+                    double tausfac = L.tfn_tbl[itr];
+                    bbd = plfrac * (blay + tausfac * dplankdn);
+                    bbugas[lev] = plfrac * (blay + tausfac * dplankup);
+                }
+                radld = radld + (bbd - radld) * atrans[lev];
+                drad[lev - 1] = drad[lev - 1] + radld;
+            }
+            double rad0 = c->fracs[1][igc] * c->plankbnd[iband];
+            double reflect = 1.0 - c->semiss[iband];
+            double radlu = rad0 + reflect * radld;
+            urad[0] = urad[0] + radlu;
+            for (int lev = 1; lev <= nl; ++lev) {
+                radlu = radlu + (bbugas[lev] - radlu) * atrans[lev];
+                urad[lev] = urad[lev] + radlu;
+            }
+        }
+        for (int lev = nl; lev >= 0; --lev) {
+            double uflux = urad[lev] * wtdiff;
+            double dflux = drad[lev] * wtdiff;
+            urad[lev] = 0.0;
+            drad[lev] = 0.0;
+            totuflux[lev] = totuflux[lev] + uflux * L.delwave[iband - 1];
+            totdflux[lev] = totdflux[lev] + dflux * L.delwave[iband - 1];
+        }
+    }
+    totuflux[0] = totuflux[0] * fluxfac;
+    totdflux[0] = totdflux[0] * fluxfac;
+    fnet[0] = totuflux[0] - totdflux[0];
+    for (int lev = 1; lev <= nl; ++lev) {
+        totuflux[lev] = totuflux[lev] * fluxfac;
+        totdflux[lev] = totdflux[lev] * fluxfac;
+        fnet[lev] = totuflux[lev] - totdflux[lev];
+        int l = lev - 1;
+        htr[l] = L.heatfac * (fnet[l] - fnet[lev]) / (c->pz[l] - c->pz[lev]);
+    }
+    htr[nl] = 0.0;
+}
+
+int oracle_rrtmg_lw(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                    const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                    const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                    const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                    const double *cfc22vmr, const double *ccl4vmr, const double *emis, double *uflx,
+                    double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc)
+{
+    if (!L.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    int err = 0;
+#pragma omp parallel
+    {
+        lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+        double tu[MAXLAY + 1], td[MAXLAY + 1], htr[MAXLAY + 1];
+        if (!c) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (!c) continue;
+            inatm(c, iplon, ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                  o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis);
+            setcoef(c);
+            taumol(c);
+            rtrnmr(c, tu, td, htr);
+            for (int k = 0; k <= nlay; ++k) {
+                uflx[iplon + (long)ncol * k] = tu[k];
+                dflx[iplon + (long)ncol * k] = td[k];
+                uflxc[iplon + (long)ncol * k] = tu[k];
+                dflxc[iplon + (long)ncol * k] = td[k];
+            }
+            for (int k = 0; k < nlay; ++k) {
+                hr[iplon + (long)ncol * k] = htr[k];
+                hrc[iplon + (long)ncol * k] = htr[k];
+            }
+        }
+        free(c);
+    }
+    return err;
+}
+
+static void one_lw(lwcol_t *c, int nlay, const double *play, const double *plev, const double *tlay,
+                   const double *tlev, double tsfc, const double *h2ovmr, const double *o3vmr,
+                   const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                   const double *cfc11, const double *cfc12, const double *cfc22, const double *ccl4)
+{
+    double emis[16];
+    for (int i = 0; i < 16; ++i) emis[i] = 1.0;
+    double zeros[MAXLAY];
+    memset(zeros, 0, sizeof zeros);
+    inatm(c, 0, 1, nlay, play, plev, tlay, tlev, &tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+          cfc11 ? cfc11 : zeros, cfc12 ? cfc12 : zeros, cfc22 ? cfc22 : zeros, ccl4 ? ccl4 : zeros, emis);
+    setcoef(c);
+}
+
+int oracle_lw_setcoef(int nlay, const double *play, const double *plev, const double *tlay,
+                      const double *tlev, double tsfc, const double *h2ovmr, const double *o3vmr,
+                      const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                      const double *o2vmr, int *jp, int *jt, int *jt1, double *fac00, double *fac01,
+                      double *fac10, double *fac11, double *planklay, double *planklev, double *plankbnd)
+{
+    if (!L.ready || nlay > MAXLAY) return -1;
+    lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+    one_lw(c, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, NULL, NULL,
+           NULL, NULL);
+    for (int k = 1; k <= nlay; ++k) {
+        jp[k - 1] = c->jp[k];
+        jt[k - 1] = c->jt[k];
+        jt1[k - 1] = c->jt1[k];
+        fac00[k - 1] = c->fac00[k];
+        fac01[k - 1] = c->fac01[k];
+        fac10[k - 1] = c->fac10[k];
+        fac11[k - 1] = c->fac11[k];
+        for (int b = 1; b <= 16; ++b) planklay[(b - 1) * nlay + (k - 1)] = c->planklay[k][b];
+    }
+    for (int k = 0; k <= nlay; ++k)
+        for (int b = 1; b <= 16; ++b) planklev[(b - 1) * (nlay + 1) + k] = c->planklev[k][b];
+    for (int b = 1; b <= 16; ++b) plankbnd[b - 1] = c->plankbnd[b];
+    int lt = c->laytrop;
+    free(c);
+    return lt;
+}
+
+int oracle_lw_taumol(int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *tlev, double tsfc, const double *h2ovmr, const double *o3vmr,
+                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                     const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                     const double *cfc22vmr, const double *ccl4vmr, double *taug, double *fracs)
+{
+    if (!L.ready || nlay > MAXLAY) return -1;
+    lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+    one_lw(c, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr,
+           cfc12vmr, cfc22vmr, ccl4vmr);
+    taumol(c);
+    for (int k = 1; k <= nlay; ++k)
+        for (int ig = 1; ig <= NGLW; ++ig) {
+            taug[(ig - 1) * nlay + (k - 1)] = c->taug[k][ig];
+            fracs[(ig - 1) * nlay + (k - 1)] = c->fracs[k][ig];
+        }
+    int lt = c->laytrop;
+    free(c);
+    return lt;
+}

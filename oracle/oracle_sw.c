@@ -1,0 +1,1204 @@
+/*
+ * Oracle: RRTMG_SW clear-sky column model in FP64 (TEST INFRASTRUCTURE ONLY;
+ * see oracle.h).  Restates, in order of evaluation:
+ *   rrtmg_sw_init.f90:28-154 (init, exp_tbl, rwgt, cmbgb16s..29)
+ *   rrtmg_sw_rad.nomcica.f90:500-729 (column loop), :761-1101 (inatm_sw)
+ *   rrtmg_sw_setcoef.f90:30-286 (setcoef_sw)
+ *   rrtmg_sw_taumol.f90:31-1538 (taumol16..29)
+ *   rrtmg_sw_spcvrt.f90:34-626 (spcvrt_sw), rrtmg_sw_reftra.f90:29-305,
+ *   rrtmg_sw_vrtqdr.f90:28-152.
+ * Arrays local to a column use 1-based indices as in the Fortran.
+ */
+#include "oracle_internal.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NBSW 14
+#define NGSW 112
+#define MAXLAY 256
+
+typedef struct {
+    int ng, nspa, nspb;
+    double *absa, *absb; /* (nrow, ng) */
+    int nrow_a, nrow_b;
+    double *selfref, *forref; /* (10, ng), (nfor, ng) */
+    int nfor;
+    double *sfluxref; /* (ng, nsf) */
+    int nsf;
+    double rayl;           /* scalar Rayleigh */
+    double *raylg;         /* rayl(ng) of bands 23, 25, 26, 27 */
+    double *rayla, *raylb; /* band 24: rayla(ng, 9), raylb(ng) */
+    double *abso3a, *abso3b, *absch4, *absco2, *absh2o;
+} swband_t;
+
+static struct {
+    int ready;
+    double heatfac, grav, avogad;
+    double exp_tbl[ORACLE_NTBL + 1];
+    double bpade;
+    int ngc[NBSW], ngs[NBSW];
+    double rwgt[NBSW * 16];
+    double pref[59], preflog[59], tref[59];
+    swband_t b[NBSW]; /* index band-16 */
+} S;
+
+#define B(jb) (&S.b[(jb)-16])
+#define ABSA(bb, ind, ig) ((bb)->absa[(long)((ig)-1) * (bb)->nrow_a + ((ind)-1)])
+#define ABSB(bb, ind, ig) ((bb)->absb[(long)((ig)-1) * (bb)->nrow_b + ((ind)-1)])
+#define SELFREF(bb, i, ig) ((bb)->selfref[((ig)-1) * 10 + ((i)-1)])
+#define FORREF(bb, i, ig) ((bb)->forref[((ig)-1) * (bb)->nfor + ((i)-1)])
+#define SFLUXREF(bb, ig, j) ((bb)->sfluxref[((j)-1) * (bb)->ng + ((ig)-1)])
+#define RAYLA(bb, ig, j) ((bb)->rayla[((j)-1) * (bb)->ng + ((ig)-1)])
+
+static const double oneminus_sw = 1.0 - 1.e-6; /* rrtmg_sw_rad.nomcica.f90:445-447 */
+
+/* ------------------------------------------------------------------------ */
+/* init                                                                     */
+/* ------------------------------------------------------------------------ */
+
+static double *reduce_entry(const otbl_t *t, int jb, const char *nm, int gdim, int weighted,
+                            const int *ngn_all, int *other_out)
+{
+    char key[64];
+    snprintf(key, sizeof key, "sw/b%d/%s", jb, nm);
+    int nd, ext[4], lb[4], isint;
+    const double *x = (const double *)otbl_get(t, key, &nd, ext, lb, &isint);
+    if (!x) return NULL;
+    int ibnd = jb - 15;
+    const int *first_ngn = ngn_all + (ibnd == 1 ? 0 : S.ngs[ibnd - 2]);
+    long total = 1;
+    for (int d = 0; d < nd; ++d) total *= ext[d];
+    if (other_out) *other_out = (int)(total / 16);
+    double *r = oracle_reduce(x, nd, ext, gdim, S.ngc[ibnd - 1], first_ngn,
+                              S.rwgt + (ibnd - 1) * 16, weighted);
+    char rkey[64];
+    /* register under the reduced Fortran name */
+    const char *rn = nm;
+    char buf[32];
+    size_t L = strlen(nm);
+    if (L > 1 && nm[L - 1] == 'o') {
+        memcpy(buf, nm, L - 1);
+        buf[L - 1] = 0;
+        if (!strcmp(buf, "ka")) rn = "ka";
+        else if (!strcmp(buf, "kb")) rn = "kb";
+        else rn = buf;
+    }
+    snprintf(rkey, sizeof rkey, "sw/b%d/%s", jb, rn);
+    oracle_register(rkey, r, (total / 16) * S.ngc[ibnd - 1]);
+    return r;
+}
+
+int oracle_sw_init(const char *path, double cpdair)
+{
+    otbl_t t;
+    if (otbl_load(&t, path) != 0) return -1;
+    /* swdatinit (rrtmg_sw_init.f90:157-241) */
+    S.grav = 9.8066;
+    S.avogad = 6.02214199e+23;
+    S.heatfac = S.grav * 8.6400e4 / (cpdair * 1.e2);
+    const int *ngc = otbl_i(&t, "sw/ngc", NULL), *ngs = otbl_i(&t, "sw/ngs", NULL);
+    const int *ngn = otbl_i(&t, "sw/ngn", NULL), *ngm = otbl_i(&t, "sw/ngm", NULL);
+    const int *nspa = otbl_i(&t, "sw/nspa", NULL), *nspb = otbl_i(&t, "sw/nspb", NULL);
+    const double *wt = otbl_d(&t, "sw/wt", NULL);
+    if (!ngc || !ngs || !ngn || !ngm || !wt || !nspa || !nspb) return -2;
+    memcpy(S.ngc, ngc, sizeof S.ngc);
+    memcpy(S.ngs, ngs, sizeof S.ngs);
+    memcpy(S.pref, otbl_d(&t, "sw/pref", NULL), sizeof S.pref);
+    memcpy(S.preflog, otbl_d(&t, "sw/preflog", NULL), sizeof S.preflog);
+    memcpy(S.tref, otbl_d(&t, "sw/tref", NULL), sizeof S.tref);
+
+    /* exp_tbl (rrtmg_sw_init.f90:94-104) */
+    const double expeps = 1.e-20;
+    S.exp_tbl[0] = 1.0;
+    S.exp_tbl[ORACLE_NTBL] = expeps;
+    S.bpade = 1.0 / 0.278;
+    for (int itr = 1; itr <= ORACLE_NTBL - 1; ++itr) {
+        double tfn = (double)itr / (double)ORACLE_NTBL;
+        double tau = S.bpade * tfn / (1.0 - tfn);
+        S.exp_tbl[itr] = exp(-tau);
+        if (S.exp_tbl[itr] <= expeps) S.exp_tbl[itr] = expeps;
+    }
+    oracle_make_rwgt(NBSW, ngc, ngn, ngm, wt, S.rwgt);
+    oracle_register("sw/rwgt", S.rwgt, NBSW * 16);
+
+    for (int jb = 16; jb <= 29; ++jb) {
+        swband_t *bb = B(jb);
+        memset(bb, 0, sizeof *bb);
+        bb->ng = ngc[jb - 16];
+        bb->nspa = nspa[jb - 16];
+        bb->nspb = nspb[jb - 16];
+        int other;
+        bb->absa = reduce_entry(&t, jb, "kao", 1, 1, ngn, &other);
+        bb->nrow_a = bb->absa ? other : 0;
+        bb->absb = reduce_entry(&t, jb, "kbo", 1, 1, ngn, &other);
+        bb->nrow_b = bb->absb ? other : 0;
+        bb->selfref = reduce_entry(&t, jb, "selfrefo", 1, 1, ngn, NULL);
+        bb->forref = reduce_entry(&t, jb, "forrefo", 1, 1, ngn, &other);
+        bb->nfor = bb->forref ? other : 0;
+        bb->sfluxref = reduce_entry(&t, jb, "sfluxrefo", 0, 0, ngn, &other);
+        bb->nsf = other;
+        char key[64];
+        snprintf(key, sizeof key, "sw/b%d/rayl", jb);
+        const double *rs = otbl_d(&t, key, NULL);
+        bb->rayl = rs ? rs[0] : 0.0;
+        bb->raylg = reduce_entry(&t, jb, "raylo", 0, 1, ngn, NULL);
+        bb->rayla = reduce_entry(&t, jb, "raylao", 0, 1, ngn, NULL);
+        bb->raylb = reduce_entry(&t, jb, "raylbo", 0, 1, ngn, NULL);
+        bb->abso3a = reduce_entry(&t, jb, "abso3ao", 0, 1, ngn, NULL);
+        bb->abso3b = reduce_entry(&t, jb, "abso3bo", 0, 1, ngn, NULL);
+        bb->absch4 = reduce_entry(&t, jb, "absch4o", 0, 1, ngn, NULL);
+        bb->absco2 = reduce_entry(&t, jb, "absco2o", 0, 1, ngn, NULL);
+        bb->absh2o = reduce_entry(&t, jb, "absh2oo", 0, 1, ngn, NULL);
+    }
+    otbl_free(&t);
+    S.ready = 1;
+    return 0;
+}
+
+double oracle_sw_exp_tbl(int i) { return S.exp_tbl[i]; }
+
+/* ------------------------------------------------------------------------ */
+/* per-column state                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    int nlayers, laytrop, layswtch, laylow;
+    double pavel[MAXLAY + 1], tavel[MAXLAY + 1], pz[MAXLAY + 1], tz[MAXLAY + 1], pdp[MAXLAY + 1];
+    double coldry[MAXLAY + 1], wkl[8][MAXLAY + 1];
+    int jp[MAXLAY + 2], jt[MAXLAY + 1], jt1[MAXLAY + 1];
+    double co2mult[MAXLAY + 1], colch4[MAXLAY + 1], colco2[MAXLAY + 1], colh2o[MAXLAY + 1];
+    double colmol[MAXLAY + 1], coln2o[MAXLAY + 1], colo2[MAXLAY + 1], colo3[MAXLAY + 1];
+    double fac00[MAXLAY + 1], fac01[MAXLAY + 1], fac10[MAXLAY + 1], fac11[MAXLAY + 1];
+    double selffac[MAXLAY + 1], selffrac[MAXLAY + 1], forfac[MAXLAY + 1], forfrac[MAXLAY + 1];
+    int indself[MAXLAY + 1], indfor[MAXLAY + 1];
+    double sfluxzen[NGSW + 1];
+    double taug[MAXLAY + 1][NGSW + 1], taur[MAXLAY + 1][NGSW + 1];
+} swcol_t;
+
+/* inatm_sw (rrtmg_sw_rad.nomcica.f90:761-1101), the clear-sky / iaer=0 part */
+static void inatm_sw(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
+                     const double *plev, const double *tlay, const double *tlev,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr)
+{
+    const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
+#define A2(a, i, k) a[(i) + (long)ncol * ((k)-1)]
+    c->nlayers = nlay;
+    for (int m = 0; m < 8; ++m)
+        for (int l = 0; l <= nlay; ++l) c->wkl[m][l] = 0.0;
+    c->pz[0] = A2(plev, iplon, 1);
+    c->tz[0] = A2(tlev, iplon, 1);
+    for (int l = 1; l <= nlay; ++l) {
+        c->pavel[l] = A2(play, iplon, l);
+        c->tavel[l] = A2(tlay, iplon, l);
+        c->pz[l] = A2(plev, iplon, l + 1);
+        c->tz[l] = A2(tlev, iplon, l + 1);
+        c->pdp[l] = c->pz[l - 1] - c->pz[l];
+        c->wkl[1][l] = (A2(h2ovmr, iplon, l) / (1.0 - A2(h2ovmr, iplon, l))) * amdw;
+        c->wkl[2][l] = A2(co2vmr, iplon, l);
+        c->wkl[3][l] = A2(o3vmr, iplon, l) * amdo;
+        c->wkl[4][l] = A2(n2ovmr, iplon, l);
+        c->wkl[6][l] = A2(ch4vmr, iplon, l);
+        c->wkl[7][l] = A2(o2vmr, iplon, l);
+        double amm = (1.0 - c->wkl[1][l]) * amd + c->wkl[1][l] * amw;
+        c->coldry[l] = (c->pz[l - 1] - c->pz[l]) * 1.e3 * S.avogad /
+                       (1.e2 * S.grav * amm * (1.0 + c->wkl[1][l]));
+    }
+    for (int l = 1; l <= nlay; ++l)
+        for (int imol = 1; imol <= 7; ++imol) c->wkl[imol][l] = c->coldry[l] * c->wkl[imol][l];
+#undef A2
+}
+
+/* setcoef_sw (rrtmg_sw_setcoef.f90:129-284) */
+static void setcoef_sw(swcol_t *c)
+{
+    const double stpfac = 296.0 / 1013.0;
+    c->laytrop = 0;
+    c->layswtch = 0;
+    c->laylow = 0;
+    for (int lay = 1; lay <= c->nlayers; ++lay) {
+        double plog = log(c->pavel[lay]);
+        c->jp[lay] = (int)(36.0 - 5 * (plog + 0.04));
+        if (c->jp[lay] < 1) c->jp[lay] = 1;
+        else if (c->jp[lay] > 58) c->jp[lay] = 58;
+        int jp1 = c->jp[lay] + 1;
+        double fp = 5.0 * (S.preflog[c->jp[lay] - 1] - plog);
+        c->jt[lay] = (int)(3.0 + (c->tavel[lay] - S.tref[c->jp[lay] - 1]) / 15.0);
+        if (c->jt[lay] < 1) c->jt[lay] = 1;
+        else if (c->jt[lay] > 4) c->jt[lay] = 4;
+        double ft = ((c->tavel[lay] - S.tref[c->jp[lay] - 1]) / 15.0) - (double)(c->jt[lay] - 3);
+        c->jt1[lay] = (int)(3.0 + (c->tavel[lay] - S.tref[jp1 - 1]) / 15.0);
+        if (c->jt1[lay] < 1) c->jt1[lay] = 1;
+        else if (c->jt1[lay] > 4) c->jt1[lay] = 4;
+        double ft1 = ((c->tavel[lay] - S.tref[jp1 - 1]) / 15.0) - (double)(c->jt1[lay] - 3);
+        double water = c->wkl[1][lay] / c->coldry[lay];
+        double scalefac = c->pavel[lay] * stpfac / c->tavel[lay];
+        if (plog > 4.56) {
+            c->laytrop += 1;
+            if (plog >= 6.62) c->laylow += 1;
+            c->forfac[lay] = scalefac / (1. + water);
+            double factor = (332.0 - c->tavel[lay]) / 36.0;
+            int ifac = (int)factor;
+            c->indfor[lay] = ifac < 1 ? 1 : (ifac > 2 ? 2 : ifac);
+            c->forfrac[lay] = factor - (double)c->indfor[lay];
+            c->selffac[lay] = water * c->forfac[lay];
+            factor = (c->tavel[lay] - 188.0) / 7.2;
+            int is = (int)factor - 7;
+            c->indself[lay] = is < 1 ? 1 : (is > 9 ? 9 : is);
+            c->selffrac[lay] = factor - (double)(c->indself[lay] + 7);
+        } else {
+            c->forfac[lay] = scalefac / (1. + water);
+            double factor = (c->tavel[lay] - 188.0) / 36.0;
+            c->indfor[lay] = 3;
+            c->forfrac[lay] = factor - 1.0;
+            c->selffac[lay] = 0.0;
+            c->selffrac[lay] = 0.0;
+            c->indself[lay] = 0;
+        }
+        c->colh2o[lay] = 1.e-20 * c->wkl[1][lay];
+        c->colco2[lay] = 1.e-20 * c->wkl[2][lay];
+        c->colo3[lay] = 1.e-20 * c->wkl[3][lay];
+        c->coln2o[lay] = 1.e-20 * c->wkl[4][lay];
+        c->colch4[lay] = 1.e-20 * c->wkl[6][lay];
+        c->colo2[lay] = 1.e-20 * c->wkl[7][lay];
+        c->colmol[lay] = 1.e-20 * c->coldry[lay] + c->colh2o[lay];
+        if (c->colco2[lay] == 0.0) c->colco2[lay] = 1.e-32 * c->coldry[lay];
+        if (c->coln2o[lay] == 0.0) c->coln2o[lay] = 1.e-32 * c->coldry[lay];
+        if (c->colch4[lay] == 0.0) c->colch4[lay] = 1.e-32 * c->coldry[lay];
+        if (c->colo2[lay] == 0.0) c->colo2[lay] = 1.e-32 * c->coldry[lay];
+        double co2reg = 3.55e-24 * c->coldry[lay];
+        c->co2mult[lay] = (c->colco2[lay] - co2reg) * 272.63 * exp(-1919.4 / c->tavel[lay]) /
+                          (8.7604e-4 * c->tavel[lay]);
+        double compfp = 1.0 - fp;
+        c->fac10[lay] = compfp * ft;
+        c->fac00[lay] = compfp * (1.0 - ft);
+        c->fac11[lay] = fp * ft1;
+        c->fac01[lay] = fp * (1.0 - ft1);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* taumol_sw (rrtmg_sw_taumol.f90)                                          */
+/* ------------------------------------------------------------------------ */
+
+/* helpers for the recurring 8-point (eta, T, p) interpolation pieces */
+typedef struct {
+    double fac000, fac001, fac010, fac011, fac100, fac101, fac110, fac111;
+} fac8_t;
+
+static void fac8(const swcol_t *c, int lay, double fs, fac8_t *f)
+{
+    f->fac000 = (1.0 - fs) * c->fac00[lay];
+    f->fac010 = (1.0 - fs) * c->fac10[lay];
+    f->fac100 = fs * c->fac00[lay];
+    f->fac110 = fs * c->fac10[lay];
+    f->fac001 = (1.0 - fs) * c->fac01[lay];
+    f->fac011 = (1.0 - fs) * c->fac11[lay];
+    f->fac101 = fs * c->fac01[lay];
+    f->fac111 = fs * c->fac11[lay];
+}
+
+#define MAJ8(ARR, f, ind0, ind1, ig, S1, S2)                                                      \
+    ((f).fac000 * ARR(bb, ind0, ig) + (f).fac100 * ARR(bb, (ind0) + 1, ig) +                      \
+     (f).fac010 * ARR(bb, (ind0) + (S1), ig) + (f).fac110 * ARR(bb, (ind0) + (S2), ig) +          \
+     (f).fac001 * ARR(bb, ind1, ig) + (f).fac101 * ARR(bb, (ind1) + 1, ig) +                      \
+     (f).fac011 * ARR(bb, (ind1) + (S1), ig) + (f).fac111 * ARR(bb, (ind1) + (S2), ig))
+#define MAJ4(ARR, c, lay, ind0, ind1, ig)                                                          \
+    ((c)->fac00[lay] * ARR(bb, ind0, ig) + (c)->fac10[lay] * ARR(bb, (ind0) + 1, ig) +            \
+     (c)->fac01[lay] * ARR(bb, ind1, ig) + (c)->fac11[lay] * ARR(bb, (ind1) + 1, ig))
+#define SELF(c, lay, inds, ig)                                                                     \
+    ((c)->selffac[lay] * (SELFREF(bb, inds, ig) +                                                  \
+                          (c)->selffrac[lay] * (SELFREF(bb, (inds) + 1, ig) - SELFREF(bb, inds, ig))))
+#define FOR(c, lay, indf, ig)                                                                      \
+    ((c)->forfac[lay] *                                                                            \
+     (FORREF(bb, indf, ig) + (c)->forfrac[lay] * (FORREF(bb, (indf) + 1, ig) - FORREF(bb, indf, ig))))
+
+static void eta(double cola, double colb, double strrat, double mult, double *speccomb, int *js,
+                double *fs)
+{
+    *speccomb = cola + strrat * colb;
+    double specparm = cola / *speccomb;
+    if (specparm >= oneminus_sw) specparm = oneminus_sw;
+    double specmult = mult * (specparm);
+    *js = 1 + (int)specmult;
+    *fs = fmod(specmult, 1.0);
+}
+
+static void taumol_sw(swcol_t *c)
+{
+    const int nlayers = c->nlayers, laytrop = c->laytrop;
+    const int *jp = c->jp, *jt = c->jt, *jt1 = c->jt1;
+    int lay, ig, ind0, ind1, inds, indf, js, laysolfr, layreffr;
+    double speccomb, fs, tauray;
+    fac8_t f;
+    const swband_t *bb;
+    /* jp(nlayers+1) is read by the lower-atmosphere reference-layer search of
+     * bands 18-25 only when laytrop == nlayers; keep it defined. */
+    c->jp[nlayers + 1] = c->jp[nlayers];
+
+    /* band 16: rrtmg_sw_taumol.f90:243-339 */
+    bb = B(16);
+    {
+        const double strrat1 = 252.131;
+        layreffr = 18;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            eta(c->colh2o[lay], c->colch4[lay], strrat1, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                   c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                c->taur[lay][ig] = tauray;
+            }
+        }
+        laysolfr = nlayers;
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            if (jp[lay - 1] < layreffr && jp[lay] >= layreffr) laysolfr = lay;
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ig] = c->colch4[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig);
+                if (lay == laysolfr) c->sfluxzen[ig] = bb->sfluxref[ig - 1];
+                c->taur[lay][ig] = tauray;
+            }
+        }
+    }
+
+    /* band 17: rrtmg_sw_taumol.f90:342-462 */
+    bb = B(17);
+    {
+        const int ngs = 6;
+        const double strrat = 0.364641;
+        layreffr = 30;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        laysolfr = nlayers;
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            if (jp[lay - 1] < layreffr && jp[lay] >= layreffr) laysolfr = lay;
+            eta(c->colh2o[lay], c->colco2[lay], strrat, 4.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + js;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + js;
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSB, f, ind0, ind1, ig, 5, 6) +
+                                         c->colh2o[lay] * FOR(c, lay, indf, ig);
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* bands 18, 19, 21, 22 (lower) share the h2o/x eta structure; each is
+     * written out because the upper-atmosphere treatment differs.
+     * band 18: rrtmg_sw_taumol.f90:465-561 */
+    bb = B(18);
+    {
+        const int ngs = 18;
+        const double strrat = 38.9589;
+        layreffr = 6;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            eta(c->colh2o[lay], c->colch4[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colch4[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig);
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 19: rrtmg_sw_taumol.f90:564-660 */
+    bb = B(19);
+    {
+        const int ngs = 26;
+        const double strrat = 5.49281;
+        layreffr = 3;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            eta(c->colh2o[lay], c->colco2[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colco2[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig);
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 20: rrtmg_sw_taumol.f90:663-746 */
+    bb = B(20);
+    {
+        const int ngs = 34;
+        layreffr = 3;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + 1;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + 1;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] =
+                    c->colh2o[lay] * ((MAJ4(ABSA, c, lay, ind0, ind1, ig)) + SELF(c, lay, inds, ig) +
+                                      FOR(c, lay, indf, ig)) +
+                    c->colch4[lay] * bb->absch4[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = bb->sfluxref[ig - 1];
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] =
+                    c->colh2o[lay] * (MAJ4(ABSB, c, lay, ind0, ind1, ig) + FOR(c, lay, indf, ig)) +
+                    c->colch4[lay] * bb->absch4[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 21: rrtmg_sw_taumol.f90:749-868 */
+    bb = B(21);
+    {
+        const int ngs = 44;
+        const double strrat = 0.0045321;
+        layreffr = 8;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            eta(c->colh2o[lay], c->colco2[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            eta(c->colh2o[lay], c->colco2[lay], strrat, 4.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + js;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + js;
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSB, f, ind0, ind1, ig, 5, 6) +
+                                         c->colh2o[lay] * FOR(c, lay, indf, ig);
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 22: rrtmg_sw_taumol.f90:871-977 */
+    bb = B(22);
+    {
+        const int ngs = 54;
+        const double o2adj = 1.6, strrat = 0.022708;
+        layreffr = 2;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            double o2cont = 4.35e-4 * c->colo2[lay] / (350.0 * 2.0);
+            eta(c->colh2o[lay], c->colo2[lay], o2adj * strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig)) +
+                                         o2cont;
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            double o2cont = 4.35e-4 * c->colo2[lay] / (350.0 * 2.0);
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = c->colo2[lay] * o2adj * MAJ4(ABSB, c, lay, ind0, ind1, ig) + o2cont;
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 23: rrtmg_sw_taumol.f90:980-1051 */
+    bb = B(23);
+    {
+        const int ngs = 56;
+        const double givfac = 1.029;
+        layreffr = 6;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + 1;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + 1;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylg[ig - 1];
+                c->taug[lay][ngs + ig] =
+                    c->colh2o[lay] * (givfac * MAJ4(ABSA, c, lay, ind0, ind1, ig) + SELF(c, lay, inds, ig) +
+                                      FOR(c, lay, indf, ig));
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = bb->sfluxref[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = 0.0;
+                c->taur[lay][ngs + ig] = c->colmol[lay] * bb->raylg[ig - 1];
+            }
+    }
+
+    /* band 24: rrtmg_sw_taumol.f90:1054-1153 */
+    bb = B(24);
+    {
+        const int ngs = 66;
+        const double strrat = 0.124692;
+        layreffr = 1;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            eta(c->colh2o[lay], c->colo2[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * (RAYLA(bb, ig, js) + fs * (RAYLA(bb, ig, js + 1) - RAYLA(bb, ig, js)));
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10) +
+                                         c->colo3[lay] * bb->abso3a[ig - 1] +
+                                         c->colh2o[lay] * (SELF(c, lay, inds, ig) + FOR(c, lay, indf, ig));
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylb[ig - 1];
+                c->taug[lay][ngs + ig] =
+                    c->colo2[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig) + c->colo3[lay] * bb->abso3b[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 25: rrtmg_sw_taumol.f90:1156-1217 */
+    bb = B(25);
+    {
+        const int ngs = 74;
+        layreffr = 2;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            if (jp[lay] < layreffr && jp[lay + 1] >= layreffr) laysolfr = (lay + 1 < laytrop ? lay + 1 : laytrop);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + 1;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + 1;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylg[ig - 1];
+                c->taug[lay][ngs + ig] =
+                    c->colh2o[lay] * MAJ4(ABSA, c, lay, ind0, ind1, ig) + c->colo3[lay] * bb->abso3a[ig - 1];
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = bb->sfluxref[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylg[ig - 1];
+                c->taug[lay][ngs + ig] = c->colo3[lay] * bb->abso3b[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+    }
+
+    /* band 26: rrtmg_sw_taumol.f90:1220-1268 */
+    bb = B(26);
+    {
+        const int ngs = 80;
+        laysolfr = laytrop;
+        for (lay = 1; lay <= laytrop; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = bb->sfluxref[ig - 1];
+                c->taug[lay][ngs + ig] = 0.0;
+                c->taur[lay][ngs + ig] = c->colmol[lay] * bb->raylg[ig - 1];
+            }
+        for (lay = laytrop + 1; lay <= nlayers; ++lay)
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = 0.0;
+                c->taur[lay][ngs + ig] = c->colmol[lay] * bb->raylg[ig - 1];
+            }
+    }
+
+    /* band 27: rrtmg_sw_taumol.f90:1271-1347 */
+    bb = B(27);
+    {
+        const int ngs = 86;
+        const double scalekur = 50.15 / 48.37;
+        layreffr = 32;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + 1;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + 1;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylg[ig - 1];
+                c->taug[lay][ngs + ig] = c->colo3[lay] * MAJ4(ABSA, c, lay, ind0, ind1, ig);
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        laysolfr = nlayers;
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            if (jp[lay - 1] < layreffr && jp[lay] >= layreffr) laysolfr = lay;
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                tauray = c->colmol[lay] * bb->raylg[ig - 1];
+                c->taug[lay][ngs + ig] = c->colo3[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig);
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = scalekur * bb->sfluxref[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 28: rrtmg_sw_taumol.f90:1350-1455 */
+    bb = B(28);
+    {
+        const int ngs = 94;
+        const double strrat = 6.67029e-07;
+        layreffr = 58;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            eta(c->colo3[lay], c->colo2[lay], strrat, 8.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + js;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + js;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSA, f, ind0, ind1, ig, 9, 10);
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        laysolfr = nlayers;
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            if (jp[lay - 1] < layreffr && jp[lay] >= layreffr) laysolfr = lay;
+            eta(c->colo3[lay], c->colo2[lay], strrat, 4.0, &speccomb, &js, &fs);
+            fac8(c, lay, fs, &f);
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + js;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + js;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] = speccomb * MAJ8(ABSB, f, ind0, ind1, ig, 5, 6);
+                if (lay == laysolfr)
+                    c->sfluxzen[ngs + ig] =
+                        SFLUXREF(bb, ig, js) + fs * (SFLUXREF(bb, ig, js + 1) - SFLUXREF(bb, ig, js));
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+
+    /* band 29: rrtmg_sw_taumol.f90:1458-1536 */
+    bb = B(29);
+    {
+        const int ngs = 100;
+        layreffr = 49;
+        for (lay = 1; lay <= laytrop; ++lay) {
+            ind0 = ((jp[lay] - 1) * 5 + (jt[lay] - 1)) * bb->nspa + 1;
+            ind1 = (jp[lay] * 5 + (jt1[lay] - 1)) * bb->nspa + 1;
+            inds = c->indself[lay];
+            indf = c->indfor[lay];
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] =
+                    c->colh2o[lay] * ((MAJ4(ABSA, c, lay, ind0, ind1, ig)) + SELF(c, lay, inds, ig) +
+                                      FOR(c, lay, indf, ig)) +
+                    c->colco2[lay] * bb->absco2[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+        laysolfr = nlayers;
+        for (lay = laytrop + 1; lay <= nlayers; ++lay) {
+            if (jp[lay - 1] < layreffr && jp[lay] >= layreffr) laysolfr = lay;
+            ind0 = ((jp[lay] - 13) * 5 + (jt[lay] - 1)) * bb->nspb + 1;
+            ind1 = ((jp[lay] - 12) * 5 + (jt1[lay] - 1)) * bb->nspb + 1;
+            tauray = c->colmol[lay] * bb->rayl;
+            for (ig = 1; ig <= bb->ng; ++ig) {
+                c->taug[lay][ngs + ig] =
+                    c->colco2[lay] * MAJ4(ABSB, c, lay, ind0, ind1, ig) + c->colh2o[lay] * bb->absh2o[ig - 1];
+                if (lay == laysolfr) c->sfluxzen[ngs + ig] = bb->sfluxref[ig - 1];
+                c->taur[lay][ngs + ig] = tauray;
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* reftra_sw (rrtmg_sw_reftra.f90:127-303), kmodts = 2                      */
+/* ------------------------------------------------------------------------ */
+
+static double tbl_exp(double x)
+{
+    /* table lookup used throughout the SW code (e.g. rrtmg_sw_reftra.f90:181-184) */
+    double tblind = x / (S.bpade + x);
+    int itind = (int)(1.e4 * tblind + 0.5);
+    return S.exp_tbl[itind];
+}
+
+static void reftra_sw(int nlayers, const int *lrtchk, const double *pgg, double prmuz,
+                      const double *ptau, const double *pw, double *pref, double *prefd,
+                      double *ptra, double *ptrad)
+{
+    const double eps = 1.e-08, od_lo = 0.06, zwcrit = 0.9999995;
+    for (int jk = 1; jk <= nlayers; ++jk) {
+        if (!lrtchk[jk]) {
+            pref[jk] = 0.0;
+            ptra[jk] = 1.0;
+            prefd[jk] = 0.0;
+            ptrad[jk] = 1.0;
+            continue;
+        }
+        double zto1 = ptau[jk], zw = pw[jk], zg = pgg[jk];
+        double zg3 = 3.0 * zg;
+        double zgamma1 = (8.0 - zw * (5.0 + zg3)) * 0.25;
+        double zgamma2 = 3.0 * (zw * (1.0 - zg)) * 0.25;
+        double zgamma3 = (2.0 - zg3 * prmuz) * 0.25;
+        double zgamma4 = 1.0 - zgamma3;
+        double q = zg / (1.0 - zg);
+        double zwo = zw / (1.0 - (1.0 - zw) * (q * q));
+        if (zwo >= zwcrit) {
+            double za = zgamma1 * prmuz;
+            double za1 = za - zgamma3;
+            double zgt = zgamma1 * zto1;
+            double ze1 = fmin(zto1 / prmuz, 500.0);
+            double ze2;
+            if (ze1 <= od_lo)
+                ze2 = 1.0 - ze1 + 0.5 * ze1 * ze1;
+            else
+                ze2 = tbl_exp(ze1);
+            pref[jk] = (zgt - za1 * (1.0 - ze2)) / (1.0 + zgt);
+            ptra[jk] = 1.0 - pref[jk];
+            prefd[jk] = zgt / (1.0 + zgt);
+            ptrad[jk] = 1.0 - prefd[jk];
+            if (ze2 == 1.0) {
+                pref[jk] = 0.0;
+                ptra[jk] = 1.0;
+                prefd[jk] = 0.0;
+                ptrad[jk] = 1.0;
+            }
+        } else {
+            double za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+            double za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+            double zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+            double zrp = zrk * prmuz;
+            double zrp1 = 1.0 + zrp;
+            double zrm1 = 1.0 - zrp;
+            double zrk2 = 2.0 * zrk;
+            double zrpp = 1.0 - zrp * zrp;
+            double zrkg = zrk + zgamma1;
+            double zr1 = zrm1 * (za2 + zrk * zgamma3);
+            double zr2 = zrp1 * (za2 - zrk * zgamma3);
+            double zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+            double zr4 = zrpp * zrkg;
+            double zr5 = zrpp * (zrk - zgamma1);
+            double zt1 = zrp1 * (za1 + zrk * zgamma4);
+            double zt2 = zrm1 * (za1 - zrk * zgamma4);
+            double zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+            double zt4 = zr4;
+            double zt5 = zr5;
+            double zbeta = (zgamma1 - zrk) / zrkg;
+            double ze1 = fmin(zrk * zto1, 500.0);
+            double ze2 = fmin(zto1 / prmuz, 500.0);
+            double zem1, zep1, zem2, zep2;
+            if (ze1 <= od_lo) {
+                zem1 = 1.0 - ze1 + 0.5 * ze1 * ze1;
+                zep1 = 1.0 / zem1;
+            } else {
+                zem1 = tbl_exp(ze1);
+                zep1 = 1.0 / zem1;
+            }
+            if (ze2 <= od_lo) {
+                zem2 = 1.0 - ze2 + 0.5 * ze2 * ze2;
+                zep2 = 1.0 / zem2;
+            } else {
+                zem2 = tbl_exp(ze2);
+                zep2 = 1.0 / zem2;
+            }
+            double zdenr = zr4 * zep1 + zr5 * zem1;
+            double zdent = zt4 * zep1 + zt5 * zem1;
+            if (zdenr >= -eps && zdenr <= eps) {
+                pref[jk] = eps;
+                ptra[jk] = zem2;
+            } else {
+                pref[jk] = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) / zdenr;
+                ptra[jk] = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) / zdent;
+            }
+            double zemm = zem1 * zem1;
+            double zdend = 1.0 / ((1.0 - zbeta * zemm) * zrkg);
+            prefd[jk] = zgamma2 * (1.0 - zemm) * zdend;
+            ptrad[jk] = zrk2 * zem1 * zdend;
+        }
+    }
+}
+
+void oracle_sw_reftra(int nlay, const double *gg, double prmuz, const double *tau, const double *w,
+                      double *pref, double *prefd, double *ptra, double *ptrad)
+{
+    int chk[MAXLAY + 2];
+    double g1[MAXLAY + 2], t1[MAXLAY + 2], w1[MAXLAY + 2];
+    double r[MAXLAY + 2], rd[MAXLAY + 2], tr[MAXLAY + 2], trd[MAXLAY + 2];
+    for (int k = 1; k <= nlay; ++k) {
+        chk[k] = 1;
+        g1[k] = gg[k - 1];
+        t1[k] = tau[k - 1];
+        w1[k] = w[k - 1];
+    }
+    reftra_sw(nlay, chk, g1, prmuz, t1, w1, r, rd, tr, trd);
+    for (int k = 1; k <= nlay; ++k) {
+        pref[k - 1] = r[k];
+        prefd[k - 1] = rd[k];
+        ptra[k - 1] = tr[k];
+        ptrad[k - 1] = trd[k];
+    }
+}
+
+/* vrtqdr_sw (rrtmg_sw_vrtqdr.f90:103-150); pfd/pfu for one g-point */
+static void vrtqdr_sw(int klev, const double *pref, const double *prefd, const double *ptra,
+                      const double *ptrad, const double *pdbt, double *prdnd, double *prup,
+                      double *prupd, const double *ptdbt, double *pfd, double *pfu)
+{
+    double ztdn[MAXLAY + 3];
+    double zreflect = 1.0 / (1.0 - prefd[klev + 1] * prefd[klev]);
+    prup[klev] = pref[klev] + (ptrad[klev] * ((ptra[klev] - pdbt[klev]) * prefd[klev + 1] +
+                                              pdbt[klev] * pref[klev + 1])) * zreflect;
+    prupd[klev] = prefd[klev] + ptrad[klev] * ptrad[klev] * prefd[klev + 1] * zreflect;
+    for (int jk = 1; jk <= klev - 1; ++jk) {
+        int ikp = klev + 1 - jk;
+        int ikx = ikp - 1;
+        zreflect = 1.0 / (1.0 - prupd[ikp] * prefd[ikx]);
+        prup[ikx] = pref[ikx] + (ptrad[ikx] * ((ptra[ikx] - pdbt[ikx]) * prupd[ikp] +
+                                               pdbt[ikx] * prup[ikp])) * zreflect;
+        prupd[ikx] = prefd[ikx] + ptrad[ikx] * ptrad[ikx] * prupd[ikp] * zreflect;
+    }
+    ztdn[1] = 1.0;
+    prdnd[1] = 0.0;
+    ztdn[2] = ptra[1];
+    prdnd[2] = prefd[1];
+    for (int jk = 2; jk <= klev; ++jk) {
+        int ikp = jk + 1;
+        zreflect = 1.0 / (1.0 - prefd[jk] * prdnd[jk]);
+        ztdn[ikp] = ptdbt[jk] * ptra[jk] +
+                    (ptrad[jk] * ((ztdn[jk] - ptdbt[jk]) + ptdbt[jk] * pref[jk] * prdnd[jk])) * zreflect;
+        prdnd[ikp] = prefd[jk] + ptrad[jk] * ptrad[jk] * prdnd[jk] * zreflect;
+    }
+    for (int jk = 1; jk <= klev + 1; ++jk) {
+        zreflect = 1.0 / (1.0 - prdnd[jk] * prupd[jk]);
+        pfu[jk] = (ptdbt[jk] * prup[jk] + (ztdn[jk] - ptdbt[jk]) * prupd[jk]) * zreflect;
+        pfd[jk] = ptdbt[jk] + (ztdn[jk] - ptdbt[jk] + ptdbt[jk] * prup[jk] * prdnd[jk]) * zreflect;
+    }
+}
+
+/* spcvrt_sw clear-sky path (rrtmg_sw_spcvrt.f90:259-621).  With no clouds and
+ * no aerosol the reference's total-sky pass sees identical reftra/vrtqdr
+ * inputs (lrtchkcld is false for every layer, zclear = 1, zcloud = 0), so the
+ * total-sky fluxes equal the clear-sky ones bit for bit; they are computed
+ * once here. */
+static void spcvrt_sw(swcol_t *c, double prmu0, const double *palbd, const double *palbp,
+                      const double *adjflux, double *pbbcd, double *pbbcu)
+{
+    const int klev = c->nlayers;
+    const double od_lo = 0.06;
+    double zrefc[MAXLAY + 3], zrefdc[MAXLAY + 3], ztrac[MAXLAY + 3], ztradc[MAXLAY + 3];
+    double zdbtc[MAXLAY + 3], ztdbtc[MAXLAY + 3], zrdndc[MAXLAY + 3], zrupc[MAXLAY + 3], zrupdc[MAXLAY + 3];
+    double ztauc[MAXLAY + 3], zomcc[MAXLAY + 3], zgcc[MAXLAY + 3];
+    double zcd[MAXLAY + 3], zcu[MAXLAY + 3];
+    int lrtchkclr[MAXLAY + 3];
+    for (int jk = 1; jk <= klev + 1; ++jk) {
+        pbbcd[jk] = 0.0;
+        pbbcu[jk] = 0.0;
+    }
+    taumol_sw(c);
+    int iw = 0;
+    for (int jb = 16; jb <= 29; ++jb) {
+        int ibm = jb - 15;
+        int igt = S.ngc[ibm - 1];
+        for (int jg = 1; jg <= igt; ++jg) {
+            iw = iw + 1;
+            double zincflx = adjflux[jb] * c->sfluxzen[iw] * prmu0;
+            ztdbtc[1] = 1.0;
+            zdbtc[klev + 1] = 0.0;
+            ztrac[klev + 1] = 0.0;
+            ztradc[klev + 1] = 0.0;
+            zrefc[klev + 1] = palbp[ibm];
+            zrefdc[klev + 1] = palbd[ibm];
+            zrupc[klev + 1] = palbp[ibm];
+            zrupdc[klev + 1] = palbd[ibm];
+            for (int jk = 1; jk <= klev; ++jk) {
+                int ikl = klev + 1 - jk;
+                lrtchkclr[jk] = 1;
+                /* ptaua = 0, pomga = 1, pasya = 0 (iaer = 0) */
+                ztauc[jk] = c->taur[ikl][iw] + c->taug[ikl][iw] + 0.0;
+                zomcc[jk] = c->taur[ikl][iw] * 1.0 + 0.0 * 1.0;
+                zgcc[jk] = 0.0 * 1.0 * 0.0 / zomcc[jk];
+                zomcc[jk] = zomcc[jk] / ztauc[jk];
+                double zf = zgcc[jk] * zgcc[jk];
+                double zwf = zomcc[jk] * zf;
+                ztauc[jk] = (1.0 - zwf) * ztauc[jk];
+                zomcc[jk] = (zomcc[jk] - zwf) / (1.0 - zwf);
+                zgcc[jk] = (zgcc[jk] - zf) / (1.0 - zf);
+            }
+            reftra_sw(klev, lrtchkclr, zgcc, prmu0, ztauc, zomcc, zrefc, zrefdc, ztrac, ztradc);
+            for (int jk = 1; jk <= klev; ++jk) {
+                double ze1 = ztauc[jk] / prmu0;
+                double zdbtmc;
+                if (ze1 <= od_lo)
+                    zdbtmc = 1.0 - ze1 + 0.5 * ze1 * ze1;
+                else
+                    zdbtmc = tbl_exp(ze1);
+                zdbtc[jk] = zdbtmc;
+                ztdbtc[jk + 1] = zdbtc[jk] * ztdbtc[jk];
+            }
+            vrtqdr_sw(klev, zrefc, zrefdc, ztrac, ztradc, zdbtc, zrdndc, zrupc, zrupdc, ztdbtc, zcd, zcu);
+            for (int jk = 1; jk <= klev + 1; ++jk) {
+                int ikl = klev + 2 - jk;
+                pbbcu[ikl] = pbbcu[ikl] + zincflx * zcu[jk];
+                pbbcd[ikl] = pbbcd[ikl] + zincflx * zcd[jk];
+            }
+        }
+    }
+}
+
+static void sw_column(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
+                      const double *plev, const double *tlay, const double *tlev,
+                      const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                      const double *ch4vmr, const double *n2ovmr, const double *o2vmr)
+{
+    inatm_sw(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
+    setcoef_sw(c);
+}
+
+/* earth_sun (rrtmg_sw_rad.nomcica.f90:734-758) */
+static double earth_sun(int idn)
+{
+    const double pi = 2.0 * asin(1.0);
+    double gamma = 2.0 * pi * (idn - 1) / 365.0;
+    return 1.000110 + .034221 * cos(gamma) + .001289 * sin(gamma) + .000719 * cos(2.0 * gamma) +
+           .000077 * sin(2.0 * gamma);
+}
+
+int oracle_rrtmg_sw(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                    const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                    const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                    const double *o2vmr, const double *asdir, const double *asdif,
+                    const double *aldir, const double *aldif, const double *coszen, double adjes,
+                    int dyofyr, double scon, double *swuflx, double *swdflx, double *swhr,
+                    double *swuflxc, double *swdflxc, double *swhrc)
+{
+    (void)tsfc;
+    if (!S.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    const double zepzen = 1.e-10;
+    double adjflx = adjes;
+    if (dyofyr > 0) adjflx = earth_sun(dyofyr);
+    double adjflux[30];
+    for (int ib = 16; ib <= 29; ++ib) adjflux[ib] = adjflx * (scon / 1.36822e+03);
+    int err = 0;
+#pragma omp parallel
+    {
+        swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
+        double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], swnflxc[MAXLAY + 3];
+        if (!c) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (!c) continue;
+#define O2(a, i, k) a[(i) + (long)ncol * ((k)-1)]
+            if (coszen[iplon] < zepzen) {
+                for (int k = 1; k <= nlay + 1; ++k) {
+                    O2(swuflx, iplon, k) = 0.0;
+                    O2(swdflx, iplon, k) = 0.0;
+                    O2(swuflxc, iplon, k) = 0.0;
+                    O2(swdflxc, iplon, k) = 0.0;
+                }
+                for (int k = 1; k <= nlay; ++k) {
+                    O2(swhr, iplon, k) = 0.0;
+                    O2(swhrc, iplon, k) = 0.0;
+                }
+                continue;
+            }
+            sw_column(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                      n2ovmr, o2vmr);
+            double cossza = coszen[iplon];
+            if (cossza < zepzen) cossza = zepzen;
+            double albdir[15], albdif[15];
+            for (int ib = 1; ib <= 9; ++ib) {
+                albdir[ib] = aldir[iplon];
+                albdif[ib] = aldif[iplon];
+            }
+            albdir[14] = aldir[iplon];
+            albdif[14] = aldif[iplon];
+            for (int ib = 10; ib <= 13; ++ib) {
+                albdir[ib] = asdir[iplon];
+                albdif[ib] = asdif[iplon];
+            }
+            spcvrt_sw(c, cossza, albdif, albdir, adjflux, zbbcd, zbbcu);
+            for (int i = 1; i <= nlay + 1; ++i) {
+                O2(swuflxc, iplon, i) = zbbcu[i];
+                O2(swdflxc, iplon, i) = zbbcd[i];
+                O2(swuflx, iplon, i) = zbbcu[i];
+                O2(swdflx, iplon, i) = zbbcd[i];
+                swnflxc[i] = zbbcd[i] - zbbcu[i];
+            }
+            for (int i = 1; i <= nlay; ++i) {
+                double zdpgcp = S.heatfac / c->pdp[i];
+                O2(swhrc, iplon, i) = (swnflxc[i + 1] - swnflxc[i]) * zdpgcp;
+                O2(swhr, iplon, i) = O2(swhrc, iplon, i);
+            }
+            O2(swhrc, iplon, nlay) = 0.0;
+            O2(swhr, iplon, nlay) = 0.0;
+#undef O2
+        }
+        free(c);
+    }
+    return err;
+}
+
+/* ---- single-column intermediates ---- */
+
+static void one_column(swcol_t *c, int nlay, const double *play, const double *plev,
+                       const double *tlay, const double *h2ovmr, const double *o3vmr,
+                       const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr)
+{
+    sw_column(c, 0, 1, nlay, play, plev, tlay, plev /* tlev unused by SW */, h2ovmr, o3vmr, co2vmr,
+              ch4vmr, n2ovmr, o2vmr);
+}
+
+int oracle_sw_setcoef(int nlay, const double *play, const double *plev, const double *tlay,
+                      const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                      const double *ch4vmr, const double *n2ovmr, const double *o2vmr, int *jp,
+                      int *jt, int *jt1, double *fac00, double *fac01, double *fac10, double *fac11,
+                      double *colmol)
+{
+    if (!S.ready || nlay > MAXLAY) return -1;
+    swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
+    one_column(c, nlay, play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
+    for (int k = 1; k <= nlay; ++k) {
+        jp[k - 1] = c->jp[k];
+        jt[k - 1] = c->jt[k];
+        jt1[k - 1] = c->jt1[k];
+        fac00[k - 1] = c->fac00[k];
+        fac01[k - 1] = c->fac01[k];
+        fac10[k - 1] = c->fac10[k];
+        fac11[k - 1] = c->fac11[k];
+        colmol[k - 1] = c->colmol[k];
+    }
+    int lt = c->laytrop;
+    free(c);
+    return lt;
+}
+
+int oracle_sw_taumol(int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr, double *taug,
+                     double *taur, double *sfluxzen)
+{
+    if (!S.ready || nlay > MAXLAY) return -1;
+    swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
+    one_column(c, nlay, play, plev, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
+    taumol_sw(c);
+    for (int k = 1; k <= nlay; ++k)
+        for (int ig = 1; ig <= NGSW; ++ig) {
+            taug[(ig - 1) * nlay + (k - 1)] = c->taug[k][ig];
+            taur[(ig - 1) * nlay + (k - 1)] = c->taur[k][ig];
+        }
+    for (int ig = 1; ig <= NGSW; ++ig) sfluxzen[ig - 1] = c->sfluxzen[ig];
+    int lt = c->laytrop;
+    free(c);
+    return lt;
+}
+
+double oracle_heatfac_sw(void) { return S.heatfac; }
