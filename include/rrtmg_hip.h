@@ -1,0 +1,113 @@
+/*
+ * rrtmg_hip.h -- C ABI of the MI355X (gfx950) clear-sky RRTMG column radiation.
+ *
+ * Drop-in boundary for MiMA's radiation driver.  Each entry point replaces one
+ * reference Fortran entry point with the same argument meaning, order and
+ * units; a thin iso_c_binding shim (mima_amd/fortran/) re-provides the
+ * reference module procedures on top of these symbols.  Paths below are
+ * relative to src/atmos_param/rrtm_radiation/ of mjucker/MiMA.
+ *
+ * Conventions (as in the reference drivers):
+ *   - (ncol, nlay) arrays are column-major: element (i, k) at [i + ncol*k];
+ *     k = 0 is the surface layer; level arrays have nlay+1 entries.
+ *   - pressures in hPa, temperatures in K, fluxes in W/m2, heating in K/day;
+ *     h2ovmr is specific humidity and o3vmr an O3 mass mixing ratio (MiMA's
+ *     usage, converted by the library exactly as inatm does).
+ *   - ptr_kind = RRTMG_HIP_HOST: all array arguments are host pointers; the
+ *     call is synchronous.  ptr_kind = RRTMG_HIP_DEVICE: all array arguments
+ *     are device pointers (FP64) on the current HIP device; work is enqueued
+ *     on `stream` (a hipStream_t, NULL = default stream) and the call returns
+ *     without synchronising.
+ *   - Return 0 on success, a negative RRTMG_HIP_E* code on failure; outputs
+ *     are then left untouched and rrtmg_hip_last_error() describes the cause.
+ *     Clear-sky contract: icld != 0, SW iaer != 0 or LW idrv != 0 return
+ *     RRTMG_HIP_ENOTSUP (MiMA passes icld = idrv = iaer = 0,
+ *     RR/rrtm_radiation.f90:106-108).
+ */
+#ifndef RRTMG_HIP_H
+#define RRTMG_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RRTMG_HIP_HOST 0
+#define RRTMG_HIP_DEVICE 1
+
+#define RRTMG_HIP_PREC_FP64 0  /* every kernel in FP64 (the GPU oracle path) */
+#define RRTMG_HIP_PREC_MIXED 1 /* FP32 gas optics/transfer with FP64 islands  */
+
+#define RRTMG_HIP_OK 0
+#define RRTMG_HIP_EINVAL (-22)   /* bad size, NaN/Inf input, non-monotone plev, coszen < 0 */
+#define RRTMG_HIP_ENOTSUP (-95)  /* outside the clear-sky contract */
+#define RRTMG_HIP_ENOINIT (-2)   /* called before the matching *_init */
+#define RRTMG_HIP_ENOMEM (-12)   /* device allocation failed */
+#define RRTMG_HIP_EDEVICE (-5)   /* a HIP runtime call failed */
+#define RRTMG_HIP_ENOENT (-3)    /* table file missing or malformed */
+
+#define RRTMG_HIP_MAX_NLAY 200   /* mxlay = 203 in parrrtm.f90 / parrrsw.f90 */
+
+/* Replaces rrtmg_sw_ini(cpdair) (rrtmg_sw/gcm_model/src/rrtmg_sw_init.f90:28):
+ * loads the SW tables from table_dir (NULL: the package's tables directory),
+ * reduces 224 -> 112 g-points, builds the exp table and uploads everything to
+ * the current device.  Idempotent per device. */
+int rrtmg_hip_sw_init(double cpdair, const char *table_dir);
+
+/* Replaces rrtmg_lw_ini(cpdair) (rrtmg_lw/gcm_model/src/rrtmg_lw_init.f90:28):
+ * 256 -> 140 g-points, tau/exp/tfn tables, Planck table.  Idempotent. */
+int rrtmg_hip_lw_init(double cpdair, const char *table_dir);
+
+/* Selects the arithmetic of subsequent calls (RRTMG_HIP_PREC_*); returns the
+ * previous mode.  No reference counterpart (the reference is FP64 only). */
+int rrtmg_hip_set_precision(int mode);
+
+/* Replaces rrtmg_sw (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.nomcica.f90:78-88).
+ * Same leading arguments; the cloud and aerosol arrays of the reference are
+ * not read because icld = iaer = 0 (the reference ignores them then too). */
+int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer,
+                 const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc,
+                 const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                 const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *asdir, const double *asdif, const double *aldir,
+                 const double *aldif, const double *coszen,
+                 double adjes, int dyofyr, double scon,
+                 double *swuflx, double *swdflx, double *swhr,
+                 double *swuflxc, double *swdflxc, double *swhrc,
+                 int ptr_kind, void *stream);
+
+/* Replaces rrtmg_lw (rrtmg_lw/gcm_model/src/rrtmg_lw_rad.nomcica.f90:80-89).
+ * emis is (ncol, 16).  tauaer is the reference's (ncol, nlay, 16) aerosol
+ * optical depth, added to the gas optical depth as iaer = 10 does; NULL means
+ * zero (what MiMA passes).  The cloud arrays are not read (icld = 0). */
+int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
+                 const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc,
+                 const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                 const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *cfc11vmr, const double *cfc12vmr,
+                 const double *cfc22vmr, const double *ccl4vmr,
+                 const double *emis, const double *tauaer,
+                 double *uflx, double *dflx, double *hr,
+                 double *uflxc, double *dflxc, double *hrc,
+                 int ptr_kind, void *stream);
+
+/* Message for the last failing call on this host thread ("" if none). */
+const char *rrtmg_hip_last_error(void);
+
+/* Debug/parity accessors (no reference counterpart).
+ * Per-layer table indices of the last call of the given side on this device,
+ * copied to host int arrays of (ncol, nlay): jp, jt, jt1 (1-based, as the
+ * reference's setcoef), and laytrop per column.  0 on success. */
+int rrtmg_hip_debug_indices(int lw, int *jp, int *jt, int *jt1, int *laytrop);
+/* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...), Fortran
+ * order, FP64, as built at init; returns the element count or -1. */
+long rrtmg_hip_debug_table(const char *name, double *out, long n);
+/* Builds (host memory only, no device needed) the reduced tables that
+ * rrtmg_hip_debug_table then returns; used to check the reduction on CPU. */
+int rrtmg_hip_debug_host_tables(int lw, const char *table_dir, double cpdair);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

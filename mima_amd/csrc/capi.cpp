@@ -1,0 +1,560 @@
+// C ABI of the library (declared in include/rrtmg_hip.h): per-device table
+// state, workspaces, argument validation, host/device pointer modes and
+// error reporting.  Reference entry points replaced: rrtmg_sw_ini, rrtmg_lw_ini
+// (rrtmg_*_init.f90:28), rrtmg_sw (rrtmg_sw_rad.nomcica.f90:78), rrtmg_lw
+// (rrtmg_lw_rad.nomcica.f90:80).
+#include <dlfcn.h>
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rrtmg_hip.h"
+#include "kernels.hpp"
+#include "tables.hpp"
+
+using namespace rrtmg;
+
+namespace {
+
+thread_local std::string t_err;
+
+int fail(int code, const std::string &msg)
+{
+    t_err = msg;
+    return code;
+}
+
+struct Buffer {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes)
+    {
+        if (bytes <= cap) return 0;
+        if (p) {
+            hipDeviceSynchronize();
+            hipFree(p);
+            p = nullptr;
+            cap = 0;
+        }
+        if (std::getenv("RRTMG_HIP_FAIL_ALLOC")) return RRTMG_HIP_ENOMEM;  // injected failure for tests
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            p = nullptr;
+            return RRTMG_HIP_ENOMEM;
+        }
+        cap = bytes;
+        return 0;
+    }
+};
+
+struct SideTables {
+    bool ready = false;
+    void *tab64 = nullptr, *tab32 = nullptr;
+    void *bands = nullptr;
+    WaveUnit *units = nullptr;
+    double *preflog = nullptr, *tref = nullptr;  // SW
+    LwConst *cst = nullptr;                      // LW
+    double cpdair = 0.0;
+};
+
+struct DeviceState {
+    SwHost sw;
+    LwHost lw;
+    SideTables swd, lwd;
+    Buffer sw_work, lw_work, sw_stage, lw_stage, zeros;
+    int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
+    SwWork sw_w;
+    LwWork lw_w;
+    std::mutex mu;
+};
+
+std::mutex g_mu;
+std::map<int, DeviceState *> g_dev;
+int g_prec = RRTMG_HIP_PREC_FP64;
+
+DeviceState *state()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_dev.find(dev);
+    if (it != g_dev.end()) return it->second;
+    DeviceState *s = new DeviceState();
+    g_dev[dev] = s;
+    return s;
+}
+
+std::string default_table_dir()
+{
+    const char *env = std::getenv("RRTMG_HIP_TABLES");
+    if (env && *env) return env;
+    Dl_info info;
+    if (dladdr((void *)&default_table_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        size_t k = p.find_last_of('/');
+        return (k == std::string::npos ? std::string(".") : p.substr(0, k)) + "/tables";
+    }
+    return "tables";
+}
+
+template <typename T>
+int upload(T **dst, const void *src, size_t bytes)
+{
+    if (hipMalloc((void **)dst, bytes) != hipSuccess) return RRTMG_HIP_ENOMEM;
+    if (hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) != hipSuccess) return RRTMG_HIP_EDEVICE;
+    return 0;
+}
+
+int upload_packed(const std::vector<double> &buf, void **d64, void **d32)
+{
+    std::vector<float> f(buf.size());
+    for (size_t i = 0; i < buf.size(); ++i) f[i] = (float)buf[i];
+    int rc = upload((double **)d64, buf.data(), buf.size() * sizeof(double));
+    if (rc) return rc;
+    return upload((float **)d32, f.data(), f.size() * sizeof(float));
+}
+
+bool bad(double x) { return !std::isfinite(x); }
+
+int check_host_array(const double *a, size_t n, const char *name)
+{
+    if (!a) return fail(RRTMG_HIP_EINVAL, std::string("null array argument ") + name);
+    for (size_t i = 0; i < n; ++i)
+        if (bad(a[i])) return fail(RRTMG_HIP_EINVAL, std::string("non-finite value in ") + name);
+    return 0;
+}
+
+int check_common(int ncol, int nlay, const double *plev, int ptr_kind)
+{
+    if (ncol <= 0) return fail(RRTMG_HIP_EINVAL, "ncol must be positive");
+    if (nlay < 2 || nlay > RRTMG_HIP_MAX_NLAY) return fail(RRTMG_HIP_EINVAL, "nlay out of range [2, 200]");
+    if (ptr_kind != RRTMG_HIP_HOST && ptr_kind != RRTMG_HIP_DEVICE) return fail(RRTMG_HIP_EINVAL, "bad ptr_kind");
+    if (ptr_kind == RRTMG_HIP_HOST && plev) {
+        for (int i = 0; i < ncol; ++i)
+            for (int k = 0; k < nlay; ++k)
+                if (!(plev[(size_t)k * ncol + i] > plev[(size_t)(k + 1) * ncol + i]))
+                    return fail(RRTMG_HIP_EINVAL, "plev must decrease strictly upward");
+        for (int i = 0; i < ncol; ++i)
+            if (!(plev[(size_t)nlay * ncol + i] > 0.0)) return fail(RRTMG_HIP_EINVAL, "top plev must be positive");
+    }
+    return 0;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The band kernels keep two per-level arrays per lane in LDS; refuse a layer
+// count whose footprint exceeds the device's per-workgroup limit.
+int check_lds(size_t bytes)
+{
+    static int limit = -1;
+    if (limit < 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        limit = v;
+    }
+    if (bytes > (size_t)limit)
+        return fail(RRTMG_HIP_ENOTSUP, "nlay too large for the LDS of one workgroup in this precision (" +
+                                           std::to_string(bytes) + " > " + std::to_string(limit) + " bytes)");
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rrtmg_hip_last_error(void) { return t_err.c_str(); }
+
+int rrtmg_hip_set_precision(int mode)
+{
+    if (mode != RRTMG_HIP_PREC_FP64 && mode != RRTMG_HIP_PREC_MIXED) return fail(RRTMG_HIP_EINVAL, "bad precision");
+    int prev = g_prec;
+    g_prec = mode;
+    return prev;
+}
+
+int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!(cpdair > 0.0) || bad(cpdair)) return fail(RRTMG_HIP_EINVAL, "cpdair must be positive");
+    if (s->swd.ready) {
+        s->sw.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);
+        s->swd.cpdair = cpdair;
+        return 0;
+    }
+    std::string dir = table_dir ? table_dir : default_table_dir();
+    std::string err;
+    if (!build_sw(dir + "/rrtmg_sw.tbl", cpdair, s->sw, err)) return fail(RRTMG_HIP_ENOENT, err);
+    int rc = upload_packed(s->sw.buf, &s->swd.tab64, &s->swd.tab32);
+    if (!rc) rc = upload((SwBand **)&s->swd.bands, s->sw.tab.band, sizeof s->sw.tab.band);
+    if (!rc) rc = upload(&s->swd.units, s->sw.units.data(), s->sw.units.size() * sizeof(WaveUnit));
+    if (!rc) rc = upload(&s->swd.preflog, s->sw.preflog.data(), 59 * sizeof(double));
+    if (!rc) rc = upload(&s->swd.tref, s->sw.tref.data(), 59 * sizeof(double));
+    if (rc) return fail(rc, "uploading SW tables failed");
+    s->swd.cpdair = cpdair;
+    s->swd.ready = true;
+    return 0;
+}
+
+int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!(cpdair > 0.0) || bad(cpdair)) return fail(RRTMG_HIP_EINVAL, "cpdair must be positive");
+    if (s->lwd.ready) {
+        s->lw.cst.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);
+        hipMemcpy(s->lwd.cst, &s->lw.cst, sizeof(LwConst), hipMemcpyHostToDevice);
+        s->lwd.cpdair = cpdair;
+        return 0;
+    }
+    std::string dir = table_dir ? table_dir : default_table_dir();
+    std::string err;
+    if (!build_lw(dir + "/rrtmg_lw.tbl", cpdair, s->lw, err)) return fail(RRTMG_HIP_ENOENT, err);
+    int rc = upload_packed(s->lw.buf, &s->lwd.tab64, &s->lwd.tab32);
+    if (!rc) rc = upload((LwBand **)&s->lwd.bands, s->lw.tab.band, sizeof s->lw.tab.band);
+    if (!rc) rc = upload(&s->lwd.units, s->lw.units.data(), s->lw.units.size() * sizeof(WaveUnit));
+    if (!rc) rc = upload(&s->lwd.cst, &s->lw.cst, sizeof(LwConst));
+    if (rc) return fail(rc, "uploading LW tables failed");
+    s->lwd.cpdair = cpdair;
+    s->lwd.ready = true;
+    return 0;
+}
+
+int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                 const double *coszen, double adjes, int dyofyr, double scon, double *swuflx, double *swdflx,
+                 double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind, void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->swd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_sw called before rrtmg_hip_sw_init");
+    if (icld != 0) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: only the clear-sky path is implemented");
+    if (iaer != 0) return fail(RRTMG_HIP_ENOTSUP, "iaer != 0: SW aerosol optics are not implemented");
+    int rc = check_common(ncol, nlay, plev, ptr_kind);
+    if (rc) return rc;
+    const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
+    const double *ins2[] = {play, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr};
+    const char *nm2[] = {"play", "tlay", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr"};
+    const double *ins3[] = {plev, tlev};
+    const double *ins1[] = {tsfc, asdir, asdif, aldir, aldif, coszen};
+    double *outs[] = {swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc};
+    for (double *o : outs)
+        if (!o) return fail(RRTMG_HIP_EINVAL, "null output array");
+    if (bad(adjes) || bad(scon)) return fail(RRTMG_HIP_EINVAL, "non-finite adjes/scon");
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        for (int i = 0; i < 8; ++i)
+            if ((rc = check_host_array(ins2[i], n2, nm2[i]))) return rc;
+        if ((rc = check_host_array(plev, n3, "plev")) || (rc = check_host_array(tlev, n3, "tlev"))) return rc;
+        for (int i = 0; i < 6; ++i)
+            if ((rc = check_host_array(ins1[i], ncol, "surface/zenith input"))) return rc;
+    } else {
+        for (auto p : ins2)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        for (auto p : ins3)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        for (auto p : ins1)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    if ((rc = check_lds(rs * ((size_t)2 * (nlay + 1) * 64 + 64)))) return rc;
+    int ck = 0;
+
+    // workspace
+    size_t off = 0;
+    const size_t o_coef = off; off = align_up(off + rs * SWF_COUNT * n2);
+    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
+    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
+    const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
+    const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
+    if ((rc = s->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)s->sw_work.p;
+    SwWork w;
+    w.coef = wb + o_coef;
+    w.ipk = (int *)(wb + o_ipk);
+    w.laytrop = (int *)(wb + o_lt);
+    w.laysolfr = (int *)(wb + o_lsf);
+    w.bflux = wb + o_bf;
+
+    ColumnInputs in{};
+    in.ncol = ncol;
+    in.nlay = nlay;
+    double adjflx = adjes;
+    if (dyofyr > 0) {  // earth_sun (rrtmg_sw_rad.nomcica.f90:734-758)
+        const double pi = 2.0 * std::asin(1.0);
+        double gamma = 2.0 * pi * (dyofyr - 1) / 365.0;
+        adjflx = 1.000110 + .034221 * std::cos(gamma) + .001289 * std::sin(gamma) + .000719 * std::cos(2.0 * gamma) +
+                 .000077 * std::sin(2.0 * gamma);
+    }
+    in.adjflux = adjflx * (scon / 1.36822e+03);
+    SwOutputs out{};
+    double *stage = nullptr;
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + 4 * n3 + 2 * n2);
+        if ((rc = s->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        stage = (double *)s->sw_stage.p;
+        double *p = stage;
+        const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4, &in.n2o, &in.o2};
+        for (int i = 0; i < 8; ++i) {
+            ck |= hipMemcpyAsync(p, ins2[i], n2 * sizeof(double), hipMemcpyHostToDevice, st);
+            *dst2[i] = p;
+            p += n2;
+        }
+        ck |= hipMemcpyAsync(p, plev, n3 * sizeof(double), hipMemcpyHostToDevice, st);
+        in.plev = p;
+        p += n3;
+        ck |= hipMemcpyAsync(p, tlev, n3 * sizeof(double), hipMemcpyHostToDevice, st);
+        in.tlev = p;
+        p += n3;
+        const double **dst1[] = {&in.tsfc, &in.asdir, &in.asdif, &in.aldir, &in.aldif, &in.coszen};
+        for (int i = 0; i < 6; ++i) {
+            ck |= hipMemcpyAsync(p, ins1[i], ncol * sizeof(double), hipMemcpyHostToDevice, st);
+            *dst1[i] = p;
+            p += ncol;
+        }
+        out.uflx = p; p += n3;
+        out.dflx = p; p += n3;
+        out.uflxc = p; p += n3;
+        out.dflxc = p; p += n3;
+        out.hr = p; p += n2;
+        out.hrc = p; p += n2;
+    } else {
+        in.play = play; in.plev = plev; in.tlay = tlay; in.tlev = tlev; in.tsfc = tsfc;
+        in.h2o = h2ovmr; in.o3 = o3vmr; in.co2 = co2vmr; in.ch4 = ch4vmr; in.n2o = n2ovmr; in.o2 = o2vmr;
+        in.asdir = asdir; in.asdif = asdif; in.aldir = aldir; in.aldif = aldif; in.coszen = coszen;
+        out.uflx = swuflx; out.dflx = swdflx; out.hr = swhr; out.uflxc = swuflxc; out.dflxc = swdflxc; out.hrc = swhrc;
+    }
+    SwDevTables t;
+    t.desc = s->sw.tab;
+    t.bands = (const SwBand *)s->swd.bands;
+    t.tab = mixed ? s->swd.tab32 : s->swd.tab64;
+    t.preflog = s->swd.preflog;
+    t.tref = s->swd.tref;
+    t.units = s->swd.units;
+    t.heatfac = s->sw.heatfac;
+    if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
+    hipError_t e = launch_sw(in, out, w, t, mixed, st);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW kernel launch: ") + hipGetErrorString(e));
+    s->sw_w = w;
+    s->sw_last_ncol = ncol;
+    s->sw_last_nlay = nlay;
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        ck |= hipMemcpyAsync(swuflx, out.uflx, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(swdflx, out.dflx, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(swuflxc, out.uflxc, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(swdflxc, out.dflxc, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(swhr, out.hr, n2 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(swhrc, out.hrc, n2 * sizeof(double), hipMemcpyDeviceToHost, st);
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("SW: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                 const double *emis, const double *tauaer, double *uflx, double *dflx, double *hr, double *uflxc,
+                 double *dflxc, double *hrc, int ptr_kind, void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->lwd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_lw called before rrtmg_hip_lw_init");
+    if (icld != 0) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: only the clear-sky path is implemented");
+    if (idrv != 0) return fail(RRTMG_HIP_ENOTSUP, "idrv != 0: dF/dTs output is not implemented");
+    int rc = check_common(ncol, nlay, plev, ptr_kind);
+    if (rc) return rc;
+    const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
+    const double *ins2[] = {play, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr};
+    const char *nm2[] = {"play", "tlay", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+                         "cfc11vmr", "cfc12vmr", "cfc22vmr", "ccl4vmr"};
+    double *outs[] = {uflx, dflx, hr, uflxc, dflxc, hrc};
+    for (double *o : outs)
+        if (!o) return fail(RRTMG_HIP_EINVAL, "null output array");
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        for (int i = 0; i < 12; ++i)
+            if ((rc = check_host_array(ins2[i], n2, nm2[i]))) return rc;
+        if ((rc = check_host_array(plev, n3, "plev")) || (rc = check_host_array(tlev, n3, "tlev")) ||
+            (rc = check_host_array(tsfc, ncol, "tsfc")) || (rc = check_host_array(emis, (size_t)ncol * 16, "emis")))
+            return rc;
+        if (tauaer && (rc = check_host_array(tauaer, n2 * 16, "tauaer"))) return rc;
+    } else {
+        for (auto p : ins2)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        if (!plev || !tlev || !tsfc || !emis) return fail(RRTMG_HIP_EINVAL, "null input array");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    if ((rc = check_lds(rs * ((size_t)2 * nlay * 64 + 64)))) return rc;
+    int ck = 0;
+    size_t off = 0;
+    const size_t o_coef = off; off = align_up(off + rs * LWF_COUNT * n2);
+    const size_t o_tlev = off; off = align_up(off + rs * n3);
+    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
+    const size_t o_ilev = off; off = align_up(off + sizeof(int) * n3);
+    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
+    const size_t o_cs = off; off = align_up(off + sizeof(double) * 3 * ncol);
+    const size_t o_bf = off; off = align_up(off + rs * 16 * 2 * n3);
+    if ((rc = s->lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)s->lw_work.p;
+    LwWork w;
+    w.coef = wb + o_coef;
+    w.tlev = wb + o_tlev;
+    w.ipk = (int *)(wb + o_ipk);
+    w.ilev = (int *)(wb + o_ilev);
+    w.laytrop = (int *)(wb + o_lt);
+    w.colscal = (double *)(wb + o_cs);
+    w.bflux = wb + o_bf;
+
+    ColumnInputs in{};
+    in.ncol = ncol;
+    in.nlay = nlay;
+    LwOutputs out{};
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        const size_t need = sizeof(double) * (12 * n2 + 2 * n3 + (size_t)ncol * 17 + (tauaer ? 16 * n2 : 0) +
+                                              4 * n3 + 2 * n2);
+        if ((rc = s->lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        double *p = (double *)s->lw_stage.p;
+        const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4,
+                                 &in.n2o, &in.o2, &in.cfc11, &in.cfc12, &in.cfc22, &in.ccl4};
+        for (int i = 0; i < 12; ++i) {
+            ck |= hipMemcpyAsync(p, ins2[i], n2 * sizeof(double), hipMemcpyHostToDevice, st);
+            *dst2[i] = p;
+            p += n2;
+        }
+        ck |= hipMemcpyAsync(p, plev, n3 * sizeof(double), hipMemcpyHostToDevice, st);
+        in.plev = p;
+        p += n3;
+        ck |= hipMemcpyAsync(p, tlev, n3 * sizeof(double), hipMemcpyHostToDevice, st);
+        in.tlev = p;
+        p += n3;
+        ck |= hipMemcpyAsync(p, tsfc, ncol * sizeof(double), hipMemcpyHostToDevice, st);
+        in.tsfc = p;
+        p += ncol;
+        ck |= hipMemcpyAsync(p, emis, (size_t)ncol * 16 * sizeof(double), hipMemcpyHostToDevice, st);
+        in.emis = p;
+        p += (size_t)ncol * 16;
+        if (tauaer) {
+            ck |= hipMemcpyAsync(p, tauaer, n2 * 16 * sizeof(double), hipMemcpyHostToDevice, st);
+            in.tauaer = p;
+            p += n2 * 16;
+        }
+        out.uflx = p; p += n3;
+        out.dflx = p; p += n3;
+        out.uflxc = p; p += n3;
+        out.dflxc = p; p += n3;
+        out.hr = p; p += n2;
+        out.hrc = p; p += n2;
+    } else {
+        in.play = play; in.plev = plev; in.tlay = tlay; in.tlev = tlev; in.tsfc = tsfc;
+        in.h2o = h2ovmr; in.o3 = o3vmr; in.co2 = co2vmr; in.ch4 = ch4vmr; in.n2o = n2ovmr; in.o2 = o2vmr;
+        in.cfc11 = cfc11vmr; in.cfc12 = cfc12vmr; in.cfc22 = cfc22vmr; in.ccl4 = ccl4vmr;
+        in.emis = emis; in.tauaer = tauaer;
+        out.uflx = uflx; out.dflx = dflx; out.hr = hr; out.uflxc = uflxc; out.dflxc = dflxc; out.hrc = hrc;
+    }
+    LwDevTables t;
+    t.desc = s->lw.tab;
+    t.bands = (const LwBand *)s->lwd.bands;
+    t.tab = mixed ? s->lwd.tab32 : s->lwd.tab64;
+    t.cst = s->lwd.cst;
+    t.units = s->lwd.units;
+    t.heatfac = s->lw.cst.heatfac;
+    if (ck) return fail(RRTMG_HIP_EDEVICE, "LW: host-to-device copy failed");
+    hipError_t e = launch_lw(in, out, w, t, mixed, st);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
+    s->lw_w = w;
+    s->lw_last_ncol = ncol;
+    s->lw_last_nlay = nlay;
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        ck |= hipMemcpyAsync(uflx, out.uflx, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(dflx, out.dflx, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(uflxc, out.uflxc, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(dflxc, out.dflxc, n3 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(hr, out.hr, n2 * sizeof(double), hipMemcpyDeviceToHost, st);
+        ck |= hipMemcpyAsync(hrc, out.hrc, n2 * sizeof(double), hipMemcpyDeviceToHost, st);
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int rrtmg_hip_debug_indices(int lw, int *jp, int *jt, int *jt1, int *laytrop)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    int ncol = lw ? s->lw_last_ncol : s->sw_last_ncol, nlay = lw ? s->lw_last_nlay : s->sw_last_nlay;
+    if (!ncol) return fail(RRTMG_HIP_ENOINIT, "no previous call");
+    const int *ipk = lw ? s->lw_w.ipk : s->sw_w.ipk;
+    const int *lt = lw ? s->lw_w.laytrop : s->sw_w.laytrop;
+    std::vector<int> h((size_t)ncol * nlay);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(h.data(), ipk, h.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(RRTMG_HIP_EDEVICE, "copy of indices failed");
+    for (size_t i = 0; i < h.size(); ++i) {
+        if (jp) jp[i] = unpack_jp(h[i]);
+        if (jt) jt[i] = unpack_jt(h[i]);
+        if (jt1) jt1[i] = unpack_jt1(h[i]);
+    }
+    if (laytrop && hipMemcpy(laytrop, lt, ncol * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(RRTMG_HIP_EDEVICE, "copy of laytrop failed");
+    return 0;
+}
+
+static SwHost g_host_sw;
+static LwHost g_host_lw;
+static bool g_host_sw_ok = false, g_host_lw_ok = false;
+
+int rrtmg_hip_debug_host_tables(int lw, const char *table_dir, double cpdair)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::string dir = table_dir ? table_dir : default_table_dir();
+    std::string err;
+    bool ok = lw ? build_lw(dir + "/rrtmg_lw.tbl", cpdair, g_host_lw, err)
+                 : build_sw(dir + "/rrtmg_sw.tbl", cpdair, g_host_sw, err);
+    if (!ok) return fail(RRTMG_HIP_ENOENT, err);
+    (lw ? g_host_lw_ok : g_host_sw_ok) = true;
+    return 0;
+}
+
+long rrtmg_hip_debug_table(const char *name, double *out, long n)
+{
+    if (!name) return -1;
+    const bool lw = name[0] == 'l';
+    if ((lw && g_host_lw_ok) || (!lw && g_host_sw_ok)) {
+        const auto &m = lw ? g_host_lw.named : g_host_sw.named;
+        auto it = m.find(name);
+        if (it == m.end()) return -1;
+        long cnt = (long)it->second.size();
+        if (out) std::memcpy(out, it->second.data(), sizeof(double) * (size_t)(n < cnt ? n : cnt));
+        return cnt;
+    }
+    DeviceState *s = state();
+    if (!s) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    const auto &m = lw ? s->lw.named : s->sw.named;
+    auto it = m.find(name);
+    if (it == m.end()) return -1;
+    long cnt = (long)it->second.size();
+    if (out) std::memcpy(out, it->second.data(), sizeof(double) * (size_t)(n < cnt ? n : cnt));
+    return cnt;
+}
+
+}  // extern "C"

@@ -1,0 +1,116 @@
+// Shared host/device data structures of the gfx950 RRTMG library.
+//
+// Work decomposition (both sides).  Columns are cut into tiles of TILE_COLS
+// consecutive columns.  Each one-wave work unit handles one spectral band for
+// up to 64/ng columns of a tile, with lane = (column-in-unit, g-point-in-band),
+// g fastest.  A band's code therefore runs wave-uniformly, table rows of one
+// column are read by ng adjacent lanes (contiguous, g fastest in memory), and
+// the per-column g-point flux sums are segmented sums over ng adjacent lanes.
+// The wave schedule of one tile (which band, which columns) is the same for
+// every tile and is built once at init.
+#pragma once
+#include <cstdint>
+
+namespace rrtmg {
+
+constexpr int TILE_COLS = 60;
+constexpr int NTBL = 10000;
+constexpr int MAX_NLAY = 200;
+constexpr int MAX_WAVES_PER_TILE = 192;
+
+// One work unit of a tile: band index (0-based within the side), first column
+// of the unit within the tile and number of columns it handles.
+struct WaveUnit {
+    int16_t band;
+    int16_t first_col;
+    int16_t ncols;
+    int16_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// SW: per-band offsets into one packed table buffer (element offsets; -1 =
+// absent).  Every 2-D table is stored row-major with the g-point index
+// fastest: t[row * ng + g].
+struct SwBand {
+    int ng, ngs;         // g-points in band, g-points before this band
+    int nspa, nspb;
+    int absa, absb;      // rows = 65*nspa / 235*nspb (equivalence of ka/kb)
+    int selfref, forref; // 10 / nfor rows
+    int sfluxref, nsf;   // nsf rows (1 = no eta dependence)
+    int raylg;           // per-g Rayleigh (bands 23, 25, 26, 27)
+    int rayla, raylb;    // band 24
+    int abso3a, abso3b, absch4, absco2, absh2o;
+    double rayl;         // scalar Rayleigh
+};
+
+struct SwTables {
+    SwBand band[14];
+    int exp_tbl;         // 10001 entries
+    int nwaves;          // wave units per tile
+};
+
+// Per-layer coefficient fields written by the SW prep kernel ([field][lay][col]).
+enum SwField {
+    SWF_FAC00, SWF_FAC01, SWF_FAC10, SWF_FAC11,
+    SWF_COLH2O, SWF_COLCO2, SWF_COLO3, SWF_COLCH4, SWF_COLO2, SWF_COLMOL,
+    SWF_SELFFAC, SWF_SELFFRAC, SWF_FORFAC, SWF_FORFRAC,
+    SWF_COUNT
+};
+
+// ---------------------------------------------------------------------------
+// LW
+struct LwBand {
+    int ng, ngs, nspa, nspb;
+    int absa, absb, selfref, forref;
+    int fracrefa, nfa, fracrefb, nfb;   // rows = eta points (1 if none)
+    int ka_mn2, kb_mn2, ka_mn2o, kb_mn2o, ka_mo3, kb_mo3;
+    int ka_mco2, kb_mco2, ka_mco, ka_mo2, kb_mo2;
+    int ccl4, cfc11adj, cfc12, cfc22adj;
+    double delwave;
+    double a0, a1, a2;                  // diffusivity fit (rrtmg_lw_rtrnmr.f90:246-262)
+};
+
+struct LwTables {
+    LwBand band[16];
+    int exp_tbl, tfn_tbl;     // 10001 entries each
+    int totplnk;              // [band][181]
+    int nwaves;
+};
+
+// host-side FP64 constant tables the LW prep and band kernels index by jp
+struct LwConst {
+    double pref[59], preflog[59], tref[59];
+    double chi_mls[7 * 59];
+    // chi_mls ratios by reference level (rrtmg_lw_setcoef.f90:343-392)
+    double rat_h2oco2[60], rat_h2oo3[60], rat_h2on2o[60], rat_h2och4[60], rat_n2oco2[60], rat_o3co2[60];
+    double heatfac;
+};
+
+enum LwField {
+    LWF_FAC00, LWF_FAC01, LWF_FAC10, LWF_FAC11,
+    LWF_COLH2O, LWF_COLCO2, LWF_COLO3, LWF_COLN2O, LWF_COLCO, LWF_COLCH4, LWF_COLO2, LWF_COLBRD,
+    LWF_SELFFAC, LWF_SELFFRAC, LWF_FORFAC, LWF_FORFRAC,
+    LWF_MINORFRAC, LWF_SCALEMINOR, LWF_SCALEMINORN2, LWF_COLDRY, LWF_PAVEL,
+    LWF_WX1, LWF_WX2, LWF_WX3, LWF_WX4, LWF_TLAYFRAC,
+    LWF_COUNT
+};
+
+// packed per-layer integer word: jp | jt<<6 | jt1<<9 | indself<<12 | indfor<<16 | indminor<<18 | indlay<<23
+__host__ __device__ inline int pack_lw(int jp, int jt, int jt1, int inds, int indf, int indm, int indlay)
+{
+    return jp | (jt << 6) | (jt1 << 9) | (inds << 12) | (indf << 16) | (indm << 18) | (indlay << 23);
+}
+// SW word: jp | jt<<6 | jt1<<9 | indself<<12 | indfor<<16
+__host__ __device__ inline int pack_sw(int jp, int jt, int jt1, int inds, int indf)
+{
+    return jp | (jt << 6) | (jt1 << 9) | (inds << 12) | (indf << 16);
+}
+__host__ __device__ inline int unpack_jp(int w) { return w & 63; }
+__host__ __device__ inline int unpack_jt(int w) { return (w >> 6) & 7; }
+__host__ __device__ inline int unpack_jt1(int w) { return (w >> 9) & 7; }
+__host__ __device__ inline int unpack_inds(int w) { return (w >> 12) & 15; }
+__host__ __device__ inline int unpack_indf(int w) { return (w >> 16) & 3; }
+__host__ __device__ inline int unpack_indm(int w) { return (w >> 18) & 31; }
+__host__ __device__ inline int unpack_indlay(int w) { return (w >> 23) & 255; }
+
+}  // namespace rrtmg

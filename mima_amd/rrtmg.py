@@ -1,0 +1,195 @@
+"""Python interface of the gfx950 RRTMG library (ctypes over include/rrtmg_hip.h).
+
+Mirrors the reference entry points ``rrtmg_sw_ini``/``rrtmg_lw_ini``
+(``rrtmg_*_init.f90:28``) and ``rrtmg_sw``/``rrtmg_lw``
+(``rrtmg_sw_rad.nomcica.f90:78``, ``rrtmg_lw_rad.nomcica.f90:80``): same
+argument names, units, array layout and outputs.  Arrays are level-major
+``(nlay, ncol)`` (the memory layout of the Fortran ``(ncol, nlay)`` arrays);
+they may be float64 numpy arrays (host path, synchronous) or float64 torch
+CUDA tensors (device path, enqueued on torch's current stream).
+
+There is no CPU fallback: importing this module or calling it without the
+built HIP library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "librrtmg_hip.so")
+TABLES = os.path.join(HERE, "tables")
+CPDAIR = 287.04 / (2.0 / 7.0)   # MiMA's cp_air = RDGAS/KAPPA (constants.f90:64-67)
+
+HOST, DEVICE = 0, 1
+PREC = {"fp64": 0, "mixed": 1}
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+
+
+class RRTMGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("rrtmg_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+def lib():
+    """The loaded HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise ImportError("librrtmg_hip.so not built (run __graft_entry__.build() or make -C mima_amd/csrc)")
+        L = ctypes.CDLL(LIBPATH)
+        L.rrtmg_hip_last_error.restype = ctypes.c_char_p
+        L.rrtmg_hip_debug_table.restype = ctypes.c_long
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RRTMGError(rc, lib().rrtmg_hip_last_error().decode())
+
+
+def rrtmg_sw_ini(cpdair=CPDAIR, table_dir=None):
+    _check(lib().rrtmg_hip_sw_init(ctypes.c_double(cpdair), table_dir.encode() if table_dir else None))
+
+
+def rrtmg_lw_ini(cpdair=CPDAIR, table_dir=None):
+    _check(lib().rrtmg_hip_lw_init(ctypes.c_double(cpdair), table_dir.encode() if table_dir else None))
+
+
+def set_precision(mode):
+    """'fp64' or 'mixed'; returns the previous mode name."""
+    prev = lib().rrtmg_hip_set_precision(PREC[mode])
+    if prev < 0:
+        _check(prev)
+    return {v: k for k, v in PREC.items()}[prev]
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+class _Args:
+    """Collects pointers for one call; keeps host copies alive."""
+
+    def __init__(self, device):
+        self.device = device
+        self.keep = []
+
+    def inp(self, x):
+        if x is None:
+            return None
+        if self.device:
+            if not (_is_torch(x) and x.is_cuda and x.dtype.itemsize == 8 and x.is_contiguous()):
+                raise TypeError("device path needs contiguous float64 CUDA tensors")
+            return ctypes.cast(ctypes.c_void_p(x.data_ptr()), _D)
+        a = np.ascontiguousarray(x, dtype=np.float64)
+        self.keep.append(a)
+        return a.ctypes.data_as(_D)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+def _outputs(like, nlay, ncol):
+    if like is not None:
+        import torch
+        mk = lambda n: torch.empty((n, ncol), dtype=torch.float64, device=like.device)
+    else:
+        mk = lambda n: np.empty((n, ncol))
+    return [mk(nlay + 1), mk(nlay + 1), mk(nlay), mk(nlay + 1), mk(nlay + 1), mk(nlay)]
+
+
+def rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+             o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, stream=None):
+    """rrtmg_sw (rrtmg_sw_rad.nomcica.f90:78-88), clear sky.
+
+    Returns (swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc): (nlay+1, ncol)
+    fluxes in W/m2 and (nlay, ncol) heating in K/day."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_sw(
+        int(ncol), int(nlay), int(icld), int(iaer),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             asdir, asdif, aldir, aldif, coszen)],
+        ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon), *optr,
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
+def rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+             o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer=None, stream=None):
+    """rrtmg_lw (rrtmg_lw_rad.nomcica.f90:80-89), clear sky, idrv = 0.
+
+    ``emis`` is (16, ncol); ``tauaer`` (16, nlay, ncol) or None.  Returns
+    (uflx, dflx, hr, uflxc, dflxc, hrc)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_lw(
+        int(ncol), int(nlay), int(icld), int(idrv),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis)],
+        a.inp(tauaer), *optr,
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
+SW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+           "asdir", "asdif", "aldir", "aldif", "coszen")
+LW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+           "cfc11vmr", "cfc12vmr", "cfc22vmr", "ccl4vmr", "emis")
+
+
+def to_device(cols, device="cuda"):
+    """Copy the array fields of a mima_amd.columns dict to float64 CUDA tensors."""
+    import torch
+    out = {}
+    for k, v in cols.items():
+        if isinstance(v, np.ndarray) and v.dtype.kind == "f":
+            out[k] = torch.from_numpy(np.ascontiguousarray(v)).to(device=device, dtype=torch.float64)
+        else:
+            out[k] = v
+    return out
+
+
+def sw_columns(c, stream=None):
+    """rrtmg_sw on a mima_amd.columns dict (numpy: host path; tensors: device path)."""
+    out = rrtmg_sw(c["ncol"], c["nlay"], 0, 0, *[c[k] for k in SW_KEYS], c["adjes"], c["dyofyr"], c["scon"],
+                   stream=stream)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), out))
+
+
+def lw_columns(c, tauaer=None, stream=None):
+    out = rrtmg_lw(c["ncol"], c["nlay"], 0, 0, *[c[k] for k in LW_KEYS], tauaer=tauaer, stream=stream)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
+
+
+def debug_indices(lw, ncol, nlay):
+    jp, jt, jt1 = (np.zeros((nlay, ncol), np.int32) for _ in range(3))
+    lt = np.zeros(ncol, np.int32)
+    P = ctypes.POINTER(ctypes.c_int)
+    _check(lib().rrtmg_hip_debug_indices(int(lw), jp.ctypes.data_as(P), jt.ctypes.data_as(P),
+                                         jt1.ctypes.data_as(P), lt.ctypes.data_as(P)))
+    return dict(jp=jp, jt=jt, jt1=jt1, laytrop=lt)
+
+
+def debug_table(name):
+    n = lib().rrtmg_hip_debug_table(name.encode(), None, 0)
+    if n < 0:
+        raise KeyError(name)
+    out = np.zeros(n)
+    lib().rrtmg_hip_debug_table(name.encode(), out.ctypes.data_as(_D), n)
+    return out
