@@ -92,6 +92,13 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
                  double *uflxc, double *dflxc, double *hrc,
                  int ptr_kind, void *stream);
 
+/* Kernel timing (no reference counterpart).  With timing on, each call
+ * records HIP events on its stream around its three kernels (prep, band,
+ * finish); rrtmg_hip_last_kernel_ms waits for the last call of the given side
+ * on this device and writes the three durations in milliseconds. */
+int rrtmg_hip_set_timing(int on);
+int rrtmg_hip_last_kernel_ms(int lw, float *ms);
+
 /* Message for the last failing call on this host thread ("" if none). */
 const char *rrtmg_hip_last_error(void);
 

@@ -24,6 +24,7 @@ using namespace rrtmg;
 namespace {
 
 thread_local std::string t_err;
+extern bool g_timing;
 
 int fail(int code, const std::string &msg)
 {
@@ -38,8 +39,8 @@ struct Buffer {
     {
         if (bytes <= cap) return 0;
         if (p) {
-            hipDeviceSynchronize();
-            hipFree(p);
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p);
             p = nullptr;
             cap = 0;
         }
@@ -71,10 +72,25 @@ struct DeviceState {
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
     SwWork sw_w;
     LwWork lw_w;
+    hipEvent_t ev[2][4] = {};
+    bool ev_ok = false, ev_recorded[2] = {false, false};
     std::mutex mu;
+    hipEvent_t *events(int side)
+    {
+        if (!g_timing) return nullptr;
+        if (!ev_ok) {
+            for (auto &row : ev)
+                for (auto &e : row)
+                    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev_ok = true;
+        }
+        ev_recorded[side] = true;
+        return ev[side];
+    }
 };
 
 std::mutex g_mu;
+bool g_timing = false;
 std::map<int, DeviceState *> g_dev;
 int g_prec = RRTMG_HIP_PREC_FP64;
 
@@ -213,7 +229,8 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     if (!(cpdair > 0.0) || bad(cpdair)) return fail(RRTMG_HIP_EINVAL, "cpdair must be positive");
     if (s->lwd.ready) {
         s->lw.cst.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);
-        hipMemcpy(s->lwd.cst, &s->lw.cst, sizeof(LwConst), hipMemcpyHostToDevice);
+        if (hipMemcpy(s->lwd.cst, &s->lw.cst, sizeof(LwConst), hipMemcpyHostToDevice) != hipSuccess)
+            return fail(RRTMG_HIP_EDEVICE, "updating heatfac failed");
         s->lwd.cpdair = cpdair;
         return 0;
     }
@@ -347,7 +364,7 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
-    hipError_t e = launch_sw(in, out, w, t, mixed, st);
+    hipError_t e = launch_sw(in, out, w, t, mixed, st, s->events(0));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW kernel launch: ") + hipGetErrorString(e));
     s->sw_w = w;
     s->sw_last_ncol = ncol;
@@ -477,7 +494,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "LW: host-to-device copy failed");
-    hipError_t e = launch_lw(in, out, w, t, mixed, st);
+    hipError_t e = launch_lw(in, out, w, t, mixed, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
     s->lw_w = w;
     s->lw_last_ncol = ncol;
@@ -492,6 +509,27 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
         e = hipStreamSynchronize(st);
         if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW: ") + hipGetErrorString(e));
     }
+    return 0;
+}
+
+int rrtmg_hip_set_timing(int on)
+{
+    int prev = g_timing ? 1 : 0;
+    g_timing = on != 0;
+    return prev;
+}
+
+int rrtmg_hip_last_kernel_ms(int lw, float *ms)
+{
+    DeviceState *s = state();
+    if (!s || !ms) return fail(RRTMG_HIP_EINVAL, "no device or null output");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const int side = lw ? 1 : 0;
+    if (!s->ev_ok || !s->ev_recorded[side]) return fail(RRTMG_HIP_ENOINIT, "no timed call recorded");
+    if (hipEventSynchronize(s->ev[side][3]) != hipSuccess) return fail(RRTMG_HIP_EDEVICE, "event sync failed");
+    for (int k = 0; k < 3; ++k)
+        if (hipEventElapsedTime(&ms[k], s->ev[side][k], s->ev[side][k + 1]) != hipSuccess)
+            return fail(RRTMG_HIP_EDEVICE, "event timing failed");
     return 0;
 }
 

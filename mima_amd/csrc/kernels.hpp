@@ -60,10 +60,12 @@ struct LwDevTables {
     double heatfac;
 };
 
-// mixed = 0: FP64 everywhere; 1: FP32 with FP64 islands
+// mixed = 0: FP64 everywhere; 1: FP32 with FP64 islands.  ev: NULL, or four
+// events recorded on `s` before prep, after prep, after the band kernel and
+// after finish.
 hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t,
-                     int mixed, hipStream_t s);
+                     int mixed, hipStream_t s, hipEvent_t *ev);
 hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t,
-                     int mixed, hipStream_t s);
+                     int mixed, hipStream_t s, hipEvent_t *ev);
 
 }  // namespace rrtmg

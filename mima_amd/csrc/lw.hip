@@ -705,23 +705,28 @@ __global__ __launch_bounds__(256) void lw_finish_kernel(ColumnInputs in, LwWork 
 
 template <typename Real>
 static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t,
-                              double heatfac, hipStream_t s)
+                              double heatfac, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(lw_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, t.cst);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = sizeof(Real) * ((size_t)2 * in.nlay * 64 + 64);
+    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(lw_band_kernel<Real>, dim3(ntiles * t.desc.nwaves), dim3(64), shmem, s, in, w, t.bands,
                        t.desc.nwaves, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk, (const Real *)t.tab, t.cst,
                        t.units);
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, heatfac);
+    if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
 hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t, int mixed,
-                     hipStream_t s)
+                     hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_lw_t<float>(in, out, w, t, t.heatfac, s) : launch_lw_t<double>(in, out, w, t, t.heatfac, s);
+    return mixed ? launch_lw_t<float>(in, out, w, t, t.heatfac, s, ev)
+                 : launch_lw_t<double>(in, out, w, t, t.heatfac, s, ev);
 }
 
 }  // namespace rrtmg

@@ -459,9 +459,17 @@ __device__ __forceinline__ void sw_layer_optics(Real taug, Real taur, Real prmu0
 {
     Real tau = taur + taug;
     Real w = taur / tau;
-    if (sizeof(Real) == 4 && w > (Real)0.999) {
-        // FP64 island: near-conservative scattering (reftra's sqrt(g1^2-g2^2)
-        // and 1-exp cancellations)
+    bool island = false;
+    if (sizeof(Real) == 4) {
+        // FP64 islands of the mixed path: near-conservative scattering
+        // (cancellation in sqrt(g1^2 - g2^2) and 1 - exp) and the removable
+        // 0/0 of the direct-beam solution at k*mu0 = 1, where both the
+        // numerator and zdenr of reftra vanish
+        Real g1 = ((Real)8 - w * (Real)5) * (Real)0.25, g2 = (Real)3 * w * (Real)0.25;
+        Real zrp = sqrt(fmax(g1 * g1 - g2 * g2, (Real)0)) * prmu0;
+        island = w > (Real)0.999 || fabs((Real)1 - zrp) < (Real)0.02;
+    }
+    if (island) {
         double R, RD, T, TD;
         double t64 = (double)taur + (double)taug;
         reftra<double, Real>(t64, (double)taur / t64, (double)prmu0, exp_tbl, R, RD, T, TD);
@@ -624,22 +632,26 @@ __global__ __launch_bounds__(256) void sw_finish_kernel(ColumnInputs in, SwWork 
 
 template <typename Real>
 static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t,
-                              hipStream_t s)
+                              hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
+    if (ev) (void)hipEventRecord(ev[0], s);
     hipLaunchKernelGGL(sw_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, t.preflog, t.tref);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = sizeof(Real) * ((size_t)2 * (in.nlay + 1) * 64 + 64);
+    if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(sw_band_kernel<Real>, dim3(ntiles * t.desc.nwaves), dim3(64), shmem, s, in, w, t.bands,
                        t.desc.nwaves, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+    if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
 hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t, int mixed,
-                     hipStream_t s)
+                     hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_sw_t<float>(in, out, w, t, s) : launch_sw_t<double>(in, out, w, t, s);
+    return mixed ? launch_sw_t<float>(in, out, w, t, s, ev) : launch_sw_t<double>(in, out, w, t, s, ev);
 }
 
 }  // namespace rrtmg
