@@ -1,0 +1,87 @@
+! Test driver of the Fortran drop-in: calls rrtmg_sw_ini/rrtmg_lw_ini and
+! rrtmg_sw/rrtmg_lw the way MiMA's run_rrtmg does (RR/rrtm_radiation.f90:652-749):
+! surface-first arrays built by reversing top-first GCM arrays (non-contiguous
+! actual arguments), expression temporaries such as 10*ones for the cloud
+! radii and MiMA's zero cloud/aerosol arguments.
+!
+! usage: shim_driver <input.bin> <output.bin>
+!   input  (stream): int32 ncol, nlay; then float64 arrays, top-first
+!          (ncol, nlay) or (ncol, nlay+1): play, plev, tlay, tlev; h2o, o3, co2;
+!          (ncol): tsfc, albedo, coszen; scalars solrad(adjes), solr_cnst(scon)
+!   output (stream): swuflx, swdflx, swhr, uflx, dflx, hr, surface-first
+program shim_driver
+  use parkind, only : im => kind_im, rb => kind_rb
+  use parrrtm, only : nbndlw
+  use parrrsw, only : nbndsw
+  use rrtmg_lw_init, only : rrtmg_lw_ini
+  use rrtmg_sw_init, only : rrtmg_sw_ini
+  use rrtmg_lw_rad, only : rrtmg_lw
+  use rrtmg_sw_rad, only : rrtmg_sw
+  implicit none
+  character(len=512) :: fin, fout
+  integer :: u, ncol, nlay, sk
+  integer(kind=im) :: icld, iaer, idrv, dyofyr
+  real(kind=rb), allocatable :: pf(:,:), ph(:,:), tf(:,:), th(:,:), q(:,:), o3(:,:), co2(:,:)
+  real(kind=rb), allocatable :: tsrf(:), albedo(:), cosz(:), zeros(:,:), ones(:,:), emis(:,:)
+  real(kind=rb), allocatable :: taucld(:,:,:), sw_zro(:,:,:), tauaer(:,:,:), zro_sw(:,:,:)
+  real(kind=rb), allocatable :: swuflx(:,:), swdflx(:,:), swhr(:,:), swuflxc(:,:), swdflxc(:,:), swhrc(:,:)
+  real(kind=rb), allocatable :: uflx(:,:), dflx(:,:), hr(:,:), uflxc(:,:), dflxc(:,:), hrc(:,:)
+  real(kind=rb) :: solrad, solr_cnst
+  real(kind=rb), parameter :: cp_air = 287.04_rb / (2._rb / 7._rb)
+
+  call get_command_argument(1, fin)
+  call get_command_argument(2, fout)
+  open(newunit=u, file=trim(fin), access='stream', form='unformatted', status='old')
+  read(u) ncol, nlay
+  sk = nlay
+  allocate(pf(ncol, nlay), ph(ncol, nlay + 1), tf(ncol, nlay), th(ncol, nlay + 1))
+  allocate(q(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay), tsrf(ncol), albedo(ncol), cosz(ncol))
+  read(u) pf, ph, tf, th, q, o3, co2, tsrf, albedo, cosz, solrad, solr_cnst
+  close(u)
+
+  allocate(zeros(ncol, nlay), ones(ncol, nlay), emis(ncol, nbndlw))
+  allocate(taucld(nbndlw, ncol, nlay), sw_zro(nbndsw, ncol, nlay))
+  allocate(tauaer(ncol, nlay, nbndlw), zro_sw(ncol, nlay, nbndsw))
+  zeros = 0._rb
+  ones = 1._rb
+  emis = 1._rb
+  taucld = 0._rb
+  sw_zro = 0._rb
+  tauaer = 0._rb
+  zro_sw = 0._rb
+  allocate(swuflx(ncol, nlay + 1), swdflx(ncol, nlay + 1), swhr(ncol, nlay))
+  allocate(swuflxc(ncol, nlay + 1), swdflxc(ncol, nlay + 1), swhrc(ncol, nlay))
+  allocate(uflx(ncol, nlay + 1), dflx(ncol, nlay + 1), hr(ncol, nlay))
+  allocate(uflxc(ncol, nlay + 1), dflxc(ncol, nlay + 1), hrc(ncol, nlay))
+
+  call rrtmg_lw_ini(cp_air)
+  call rrtmg_sw_ini(cp_air)
+  icld = 0
+  iaer = 0
+  idrv = 0
+  dyofyr = 0
+  ! surface-first views of the top-first state: non-contiguous actuals
+  call rrtmg_sw(ncol, nlay, icld, iaer, &
+                pf(:, sk:1:-1), ph(:, sk+1:1:-1), tf(:, sk:1:-1), th(:, sk+1:1:-1), tsrf, &
+                q(:, sk:1:-1), o3(:, sk:1:-1), co2(:, sk:1:-1), zeros, zeros, zeros, &
+                albedo, albedo, albedo, albedo, &
+                cosz, solrad, dyofyr, solr_cnst, &
+                0, 0, 0, &
+                zeros, taucld(1:nbndsw,:,:), sw_zro, sw_zro, sw_zro, &
+                zeros, zeros, 10*ones, 10*ones, &
+                zro_sw, zro_sw, zro_sw, zro_sw, &
+                swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  call rrtmg_lw(ncol, nlay, icld, idrv, &
+                pf(:, sk:1:-1), ph(:, sk+1:1:-1), tf(:, sk:1:-1), th(:, sk+1:1:-1), tsrf, &
+                q(:, sk:1:-1), o3(:, sk:1:-1), co2(:, sk:1:-1), &
+                zeros, zeros, zeros, &
+                zeros, zeros, zeros, zeros, &
+                emis, 0, 0, 0, &
+                zeros, taucld, zeros, zeros, 10*ones, 10*ones, &
+                tauaer, &
+                uflx, dflx, hr, uflxc, dflxc, hrc)
+
+  open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
+  write(u) swuflx, swdflx, swhr, uflx, dflx, hr
+  close(u)
+end program shim_driver
