@@ -42,6 +42,32 @@ def host_threads():
     return max(1, min(16, n))
 
 
+def copy_probe(torch, mib):
+    """Plain device-to-device copy of a known size: the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters of the band kernels."""
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+
+
+def traffic_from_profile(kernel, ncol, nlay):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (scaled by the same counters read on a plain copy), as recorded in
+    profiles/hbm_traffic_32768x40.txt; None when that workload was not measured."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_32768x40.txt")
+    if (ncol, nlay) != (32768, 40) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +75,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--config", default="c1_mixed_vs_fp64")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the path shards by columns with no "
@@ -56,6 +84,9 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe:
+        copy_probe(torch, args.copy_probe)
+        return
     from mima_amd import rrtmg
     from mima_amd.columns import config_columns
 
@@ -164,7 +195,8 @@ def main():
         "max_abs_dflux_wm2": float(dfl),
         "kernel_ms": {names[k]: float(kt[k[0]][k[1]]) for k in names},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profile(names[dom_key], ncol, nlay),
                      "kernel": names[dom_key], "bytes_per_column": bcol},
         "cpu_baseline": cpu,
     }

@@ -1,12 +1,16 @@
 #!/usr/bin/env bash
-# Build the reference RRTMG_SW Fortran (as MiMA compiles it: default REAL is
+# Build the reference RRTMG Fortran (as MiMA compiles it: default REAL is
 # 8 bytes, -r8 in CMakeLists.txt:90-97) from the sources where they lie in the
-# MiMA reference tree, plus this directory's C-callable driver, into
-# oracle/_ref/librrtmg_sw_ref.so.  Test infrastructure only.
+# MiMA reference tree, plus this directory's C-callable drivers, into
+# oracle/_ref/librrtmg_sw_ref.so and oracle/_ref/librrtmg_lw_ref.so.
+# Test infrastructure only.
 #
-# Only RRTMG_SW is buildable: its k-table data file (rrtmg_sw_k_g.f90) is in
-# the tree.  RRTMG_LW's rrtmg_lw_k_g.f90 is absent, so the LW reference is not
-# built.
+# RRTMG_SW is built whole, its init included: its k-table data file
+# (rrtmg_sw_k_g.f90) is in the tree.  For RRTMG_LW the data file
+# rrtmg_lw_k_g.f90 is absent, so rrtmg_lw_init.f90 (which calls its
+# lw_kgbNN routines) is not built; the LW column code (setcoef, taumol,
+# rtrnmr, rtrn, cldprop, rrtmg_lw) is, and ref_lw_driver.f90 fills the module
+# state that the init would have set (see that file).
 #
 # usage: oracle/build_ref.sh [/root/reference]
 set -euo pipefail
@@ -44,3 +48,25 @@ done
 $FC $FFLAGS -c "$HERE/ref_sw_driver.f90" -o "$OUT/obj/ref_sw_driver.o"
 $FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_sw_ref.so" $OBJS "$OUT/obj/ref_sw_driver.o"
 echo "built $OUT/librrtmg_sw_ref.so"
+
+LW="$REF/src/atmos_param/rrtm_radiation/rrtmg_lw/gcm_model"
+mkdir -p "$OUT/mod_lw" "$OUT/obj_lw"
+LFLAGS="-fdefault-real-8 -fdefault-double-8 -fPIC -O2 -module-dir $OUT/mod_lw -I$OUT/mod_lw -I$HERE"
+LMODS="parkind parrrtm rrlw_cld rrlw_con rrlw_kg01 rrlw_kg02 rrlw_kg03 rrlw_kg04 rrlw_kg05 rrlw_kg06
+       rrlw_kg07 rrlw_kg08 rrlw_kg09 rrlw_kg10 rrlw_kg11 rrlw_kg12 rrlw_kg13 rrlw_kg14 rrlw_kg15
+       rrlw_kg16 rrlw_ref rrlw_tbl rrlw_vsn rrlw_wvn"
+LSRCS="rrtmg_lw_setcoef rrtmg_lw_taumol rrtmg_lw_rtrnmr rrtmg_lw_rtrn rrtmg_lw_cldprop rrtmg_lw_rad.nomcica"
+LOBJS=""
+for m in $LMODS; do
+  $FC $LFLAGS -c "$LW/modules/$m.f90" -o "$OUT/obj_lw/$m.o"
+  LOBJS="$LOBJS $OUT/obj_lw/$m.o"
+done
+for s in $LSRCS; do
+  $FC $LFLAGS -c "$LW/src/$s.f90" -o "$OUT/obj_lw/$s.o"
+  LOBJS="$LOBJS $OUT/obj_lw/$s.o"
+done
+$FC $LFLAGS -c "$HERE/ref_lw_driver.f90" -o "$OUT/obj_lw/ref_lw_driver.o"
+$FC -fdefault-real-8 -fPIC -shared -Wl,--no-undefined -o "$OUT/librrtmg_lw_ref.so" $LOBJS \
+    "$OUT/obj_lw/ref_lw_driver.o" -L"$(dirname "$(command -v $FC)")/../lib" -lflang_rt.runtime 2>/dev/null || \
+  $FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_lw_ref.so" $LOBJS "$OUT/obj_lw/ref_lw_driver.o"
+echo "built $OUT/librrtmg_lw_ref.so"

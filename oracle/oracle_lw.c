@@ -7,7 +7,9 @@
  *   rrtmg_lw_taumol.f90:31-3149 (taugb1..16)
  *   rrtmg_lw_rtrnmr.f90:32-779 (rtrnmr, clear-sky branch: icldlyr = 0)
  * LW k-tables are seeded synthetic values (AER's rrtmg_lw_k_g.f90 is absent);
- * the arithmetic is the reference's regardless of the table values.
+ * the arithmetic is the reference's regardless of the table values.  Pinned
+ * against the reference LW column code itself (inatm/setcoef/taumol/rtrnmr
+ * compiled from its sources, oracle/ref_lw_driver.f90) on those tables.
  */
 #include "oracle_internal.h"
 

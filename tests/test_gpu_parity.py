@@ -116,6 +116,23 @@ def test_sw_gpu_matches_reference_golden(rr):
     assert _maxdiff(gm["swhr"], d["out_swhr"]) < MIXED_ATOL_HR
 
 
+@pytest.mark.parametrize("fixture", ["lw_ref_c0_64col.npz", "lw_ref_l60_16col.npz"])
+def test_lw_gpu_matches_reference_golden(rr, fixture):
+    # the reference Fortran LW column code's own outputs (tests/golden), through the GPU
+    import os
+    d = dict(np.load(os.path.join(os.path.dirname(__file__), "golden", fixture), allow_pickle=False))
+    c = {k: d[k] for k in d if not k.startswith("out_")}
+    for k in ("ncol", "nlay"):
+        c[k] = int(c[k])
+    g = _lw(rr, c, "fp64")
+    for k in ("uflx", "dflx", "hr"):
+        assert _maxdiff(g[k], d["out_" + k]) < FP64_ATOL_FLUX, k
+    gm = _lw(rr, c, "mixed")
+    for k in ("uflx", "dflx"):
+        assert _maxdiff(gm[k], d["out_" + k]) < MIXED_ATOL_FLUX, k
+    assert _maxdiff(gm["hr"], d["out_hr"]) < MIXED_ATOL_HR
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_table_indices_bit_exact(rr, prec):
     c = _sample(512, seed=7)

@@ -23,9 +23,11 @@ def _load(name):
 def _cols_from_fixture(d):
     c = {k: d[k] for k in d if not k.startswith("out_")}
     for k in ("ncol", "nlay", "dyofyr"):
-        c[k] = int(c[k])
+        if k in c:
+            c[k] = int(c[k])
     for k in ("adjes", "scon"):
-        c[k] = float(c[k])
+        if k in c:
+            c[k] = float(c[k])
     return c
 
 
@@ -142,6 +144,46 @@ def test_sw_oracle_reftra_matches_reference():
         r = po.reftra(d["gg"][i:i + 1], float(d["mu0"][i]), d["tau"][i:i + 1], d["omega"][i:i + 1])
         for k in ("pref", "prefd", "ptra", "ptrad"):
             assert abs(r[k][0] - d[k][i]) <= 1e-13 * max(1.0, abs(d[k][i])), (k, i)
+
+
+# oracle vs the reference Fortran's LW column code (inatm, setcoef, taumol,
+# rtrnmr, rrtmg_lw run on the seeded LW tables; tests/golden/make_lw_golden.py)
+
+@pytest.mark.parametrize("fixture", ["lw_ref_c0_64col.npz", "lw_ref_l60_16col.npz"])
+def test_lw_oracle_matches_reference_fluxes(fixture):
+    d = _load(fixture)
+    c = _cols_from_fixture(d)
+    o = po.rrtmg_lw(c)
+    for k in ("uflx", "dflx", "uflxc", "dflxc"):
+        ref = d["out_" + k]
+        assert np.max(np.abs(o[k] - ref)) <= 1e-13 * np.max(np.abs(ref)), k
+    for k in ("hr", "hrc"):
+        assert np.max(np.abs(o[k] - d["out_" + k])) <= 1e-11, k
+    # the fixture exercises the secondary gases (CH4, N2O, O2, CFCs, CCl4) on 4 columns
+    if fixture == "lw_ref_c0_64col.npz":
+        assert np.all(c["cfc11vmr"][:, -4:] > 0) and np.all(c["cfc11vmr"][:, :-4] == 0)
+
+
+def test_lw_oracle_matches_reference_column_intermediates():
+    d = _load("lw_ref_column.npz")
+    gas = ("h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr",
+           "cfc22vmr", "ccl4vmr")
+    for n in range(int(d["ncols"])):
+        c = {k: d["c%d_%s" % (n, k)][:, None] for k in ("play", "plev", "tlay", "tlev") + gas}
+        c["tsfc"] = np.array([float(d["c%d_tsfc" % n])])
+        c["nlay"] = c["play"].shape[0]
+        c["ncol"] = 1
+        sc = po.lw_setcoef(c, 0)
+        assert sc["laytrop"] == int(d["c%d_laytrop" % n])
+        for k in ("jp", "jt", "jt1"):
+            assert np.array_equal(sc[k], d["c%d_%s" % (n, k)]), k
+        for k in ("fac00", "fac01", "fac10", "fac11", "planklay", "planklev", "plankbnd"):
+            np.testing.assert_allclose(sc[k], d["c%d_%s" % (n, k)], rtol=1e-14, atol=0, err_msg=k)
+        tm = po.lw_taumol(c, 0)
+        np.testing.assert_allclose(tm["taug"], d["c%d_taug" % n], rtol=1e-11, atol=1e-300)
+        np.testing.assert_allclose(tm["fracs"], d["c%d_fracs" % n], rtol=1e-14, atol=0)
+    # column 3 carries every secondary gas: the minor-gas and CFC terms are exercised
+    assert float(d["c3_cfc12vmr"][0]) > 0
 
 
 # --------------------------------------------------------------------------
