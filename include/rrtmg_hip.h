@@ -92,12 +92,54 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
                  double *uflxc, double *dflxc, double *hrc,
                  int ptr_kind, void *stream);
 
+/* Backward (vector-Jacobian product) of rrtmg_lw.  No reference counterpart:
+ * the reference's only derivative is idrv = 1, dF_up/dTs of the surface
+ * Planck term (rrtmg_lw/gcm_model/src/rrtmg_lw_rtrnmr.f90:629-746).  Takes
+ * rrtmg_lw's inputs (same layout, icld = idrv = 0) and the cotangents of its
+ * outputs uflx, dflx (ncol, nlay+1) and hr (ncol, nlay), each NULL for zero,
+ * and overwrites the gradients of  sum(uflx_bar*uflx + dflx_bar*dflx +
+ * hr_bar*hr)  with respect to tlay, h2ovmr, o3vmr (ncol, nlay), tlev
+ * (ncol, nlay+1) and tsfc (ncol).  The forward is recomputed inside (no
+ * state is kept from a previous rrtmg_hip_lw call); the arithmetic follows
+ * the precision mode.  Derivatives of table lookups are those of the
+ * tabulated functions; jp/jt/jt1 and the other table indices are held fixed
+ * (piecewise differentiation). */
+int rrtmg_hip_lw_vjp(int ncol, int nlay,
+                     const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *cfc11vmr, const double *cfc12vmr,
+                     const double *cfc22vmr, const double *ccl4vmr,
+                     const double *emis, const double *tauaer,
+                     const double *uflx_bar, const double *dflx_bar, const double *hr_bar,
+                     double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
+                     double *o3vmr_bar, double *tsfc_bar,
+                     int ptr_kind, void *stream);
+
+/* Backward of rrtmg_sw (no reference counterpart), as rrtmg_hip_lw_vjp:
+ * cotangents of swuflx, swdflx, swhr (each NULL for zero); overwrites the
+ * gradients with respect to tlay, h2ovmr and o3vmr (ncol, nlay).  Night
+ * columns (coszen < 1e-10) get zero gradients, as their outputs are zero. */
+int rrtmg_hip_sw_vjp(int ncol, int nlay,
+                     const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc,
+                     const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *asdir, const double *asdif, const double *aldir,
+                     const double *aldif, const double *coszen,
+                     double adjes, int dyofyr, double scon,
+                     const double *swuflx_bar, const double *swdflx_bar, const double *swhr_bar,
+                     double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar,
+                     int ptr_kind, void *stream);
+
 /* Kernel timing (no reference counterpart).  With timing on, each call
  * records HIP events on its stream around its three kernels (prep, band,
- * finish); rrtmg_hip_last_kernel_ms waits for the last call of the given side
- * on this device and writes the three durations in milliseconds. */
+ * finish); rrtmg_hip_last_kernel_ms waits for the last call of the given kind
+ * on this device (0 rrtmg_hip_sw, 1 rrtmg_hip_lw, 2 rrtmg_hip_sw_vjp,
+ * 3 rrtmg_hip_lw_vjp) and writes the three durations in milliseconds. */
 int rrtmg_hip_set_timing(int on);
-int rrtmg_hip_last_kernel_ms(int lw, float *ms);
+int rrtmg_hip_last_kernel_ms(int kind, float *ms);
 
 /* Message for the last failing call on this host thread ("" if none). */
 const char *rrtmg_hip_last_error(void);

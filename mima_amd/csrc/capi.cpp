@@ -69,11 +69,13 @@ struct DeviceState {
     LwHost lw;
     SideTables swd, lwd;
     Buffer sw_work, lw_work, sw_stage, lw_stage, zeros;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage;
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
     SwWork sw_w;
     LwWork lw_w;
-    hipEvent_t ev[2][4] = {};
-    bool ev_ok = false, ev_recorded[2] = {false, false};
+    // timing events per call kind: 0 SW, 1 LW, 2 SW backward, 3 LW backward
+    hipEvent_t ev[4][4] = {};
+    bool ev_ok = false, ev_recorded[4] = {false, false, false, false};
     std::mutex mu;
     hipEvent_t *events(int side)
     {
@@ -181,6 +183,48 @@ int check_lds(size_t bytes)
                                            std::to_string(bytes) + " > " + std::to_string(limit) + " bytes)");
     return 0;
 }
+
+// adjes * scon / 1368.22, or Iqbal's earth_sun factor for dyofyr > 0
+// (rrtmg_sw_rad.nomcica.f90:734-758, 968-972)
+double sw_adjflux(double adjes, int dyofyr, double scon)
+{
+    double adjflx = adjes;
+    if (dyofyr > 0) {
+        const double pi = 2.0 * std::asin(1.0);
+        double gamma = 2.0 * pi * (dyofyr - 1) / 365.0;
+        adjflx = 1.000110 + .034221 * std::cos(gamma) + .001289 * std::sin(gamma) + .000719 * std::cos(2.0 * gamma) +
+                 .000077 * std::sin(2.0 * gamma);
+    }
+    return adjflx * (scon / 1.36822e+03);
+}
+
+// Copies host arrays into a device staging buffer on one stream (host-pointer
+// calls); device-pointer calls pass the caller's pointers through.
+struct Stager {
+    char *p;
+    bool host;
+    hipStream_t st;
+    int ck = 0;
+    const double *in(const double *a, size_t n)
+    {
+        if (!host || !a) return a;
+        double *d = (double *)p;
+        ck |= hipMemcpyAsync(d, a, n * sizeof(double), hipMemcpyHostToDevice, st);
+        p += align_up(n * sizeof(double));
+        return d;
+    }
+    double *out(double *a, size_t n)
+    {
+        if (!host) return a;
+        double *d = (double *)p;
+        p += align_up(n * sizeof(double));
+        return d;
+    }
+    void back(double *h, const double *d, size_t n)
+    {
+        if (host) ck |= hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, st);
+    }
+};
 
 }  // namespace
 
@@ -310,14 +354,7 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     ColumnInputs in{};
     in.ncol = ncol;
     in.nlay = nlay;
-    double adjflx = adjes;
-    if (dyofyr > 0) {  // earth_sun (rrtmg_sw_rad.nomcica.f90:734-758)
-        const double pi = 2.0 * std::asin(1.0);
-        double gamma = 2.0 * pi * (dyofyr - 1) / 365.0;
-        adjflx = 1.000110 + .034221 * std::cos(gamma) + .001289 * std::sin(gamma) + .000719 * std::cos(2.0 * gamma) +
-                 .000077 * std::sin(2.0 * gamma);
-    }
-    in.adjflux = adjflx * (scon / 1.36822e+03);
+    in.adjflux = sw_adjflux(adjes, dyofyr, scon);
     SwOutputs out{};
     double *stage = nullptr;
     if (ptr_kind == RRTMG_HIP_HOST) {
@@ -512,6 +549,207 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     return 0;
 }
 
+int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                     const double *coszen, double adjes, int dyofyr, double scon, const double *swuflx_bar,
+                     const double *swdflx_bar, const double *swhr_bar, double *tlay_bar, double *h2ovmr_bar,
+                     double *o3vmr_bar, int ptr_kind, void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->swd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_sw_vjp called before rrtmg_hip_sw_init");
+    int rc = check_common(ncol, nlay, plev, ptr_kind);
+    if (rc) return rc;
+    const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
+    const double *ins2[] = {play, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr};
+    const char *nm2[] = {"play", "tlay", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr"};
+    const double *ins1[] = {tsfc, asdir, asdif, aldir, aldif, coszen};
+    double *outs[] = {tlay_bar, h2ovmr_bar, o3vmr_bar};
+    for (double *o : outs)
+        if (!o) return fail(RRTMG_HIP_EINVAL, "null gradient array");
+    if (bad(adjes) || bad(scon)) return fail(RRTMG_HIP_EINVAL, "non-finite adjes/scon");
+    const bool host = ptr_kind == RRTMG_HIP_HOST;
+    if (host) {
+        for (int i = 0; i < 8; ++i)
+            if ((rc = check_host_array(ins2[i], n2, nm2[i]))) return rc;
+        if ((rc = check_host_array(plev, n3, "plev")) || (rc = check_host_array(tlev, n3, "tlev"))) return rc;
+        for (int i = 0; i < 6; ++i)
+            if ((rc = check_host_array(ins1[i], ncol, "surface/zenith input"))) return rc;
+        if ((swuflx_bar && (rc = check_host_array(swuflx_bar, n3, "swuflx_bar"))) ||
+            (swdflx_bar && (rc = check_host_array(swdflx_bar, n3, "swdflx_bar"))) ||
+            (swhr_bar && (rc = check_host_array(swhr_bar, n2, "swhr_bar"))))
+            return rc;
+    } else {
+        for (auto p : ins2)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        for (auto p : ins1)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        if (!plev || !tlev) return fail(RRTMG_HIP_EINVAL, "null input array");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    if ((rc = check_lds(sw_vjp_lds_bytes(nlay, rs)))) return rc;
+    size_t off = 0;
+    const size_t o_dc = off; off = align_up(off + rs * SWF_COUNT * 4 * n2);
+    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
+    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
+    const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
+    const size_t o_cot = off; off = align_up(off + rs * 2 * n3);
+    const size_t o_gp = off; off = align_up(off + rs * 14 * 3 * n2);
+    if ((rc = s->sw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)s->sw_vjp_work.p;
+    SwVjpWork w;
+    w.dcoef = wb + o_dc;
+    w.ipk = (int *)(wb + o_ipk);
+    w.laytrop = (int *)(wb + o_lt);
+    w.laysolfr = (int *)(wb + o_lsf);
+    w.cot = wb + o_cot;
+    w.gpart = wb + o_gp;
+    if (host) {
+        const size_t need = 256 * 24 + sizeof(double) * (12 * n2 + 4 * n3 + (size_t)ncol * 6);
+        if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+    }
+    Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
+    ColumnInputs in{};
+    in.ncol = ncol;
+    in.nlay = nlay;
+    in.play = sg.in(play, n2); in.plev = sg.in(plev, n3); in.tlay = sg.in(tlay, n2); in.tlev = sg.in(tlev, n3);
+    in.tsfc = sg.in(tsfc, ncol); in.h2o = sg.in(h2ovmr, n2); in.o3 = sg.in(o3vmr, n2); in.co2 = sg.in(co2vmr, n2);
+    in.ch4 = sg.in(ch4vmr, n2); in.n2o = sg.in(n2ovmr, n2); in.o2 = sg.in(o2vmr, n2);
+    in.asdir = sg.in(asdir, ncol); in.asdif = sg.in(asdif, ncol); in.aldir = sg.in(aldir, ncol);
+    in.aldif = sg.in(aldif, ncol); in.coszen = sg.in(coszen, ncol);
+    in.adjflux = sw_adjflux(adjes, dyofyr, scon);
+    VjpCot cot{sg.in(swuflx_bar, n3), sg.in(swdflx_bar, n3), sg.in(swhr_bar, n2)};
+    SwGrad g{sg.out(tlay_bar, n2), sg.out(h2ovmr_bar, n2), sg.out(o3vmr_bar, n2)};
+    if (sg.ck) return fail(RRTMG_HIP_EDEVICE, "SW backward: host-to-device copy failed");
+    SwDevTables t;
+    t.desc = s->sw.tab;
+    t.bands = (const SwBand *)s->swd.bands;
+    t.tab = mixed ? s->swd.tab32 : s->swd.tab64;
+    t.preflog = s->swd.preflog;
+    t.tref = s->swd.tref;
+    t.units = s->swd.units;
+    t.heatfac = s->sw.heatfac;
+    hipError_t e = launch_sw_vjp(in, cot, g, w, t, mixed, st, s->events(2));
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward launch: ") + hipGetErrorString(e));
+    if (host) {
+        sg.back(tlay_bar, g.tlay, n2);
+        sg.back(h2ovmr_bar, g.h2o, n2);
+        sg.back(o3vmr_bar, g.o3, n2);
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess || sg.ck) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
+int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                     const double *emis, const double *tauaer, const double *uflx_bar, const double *dflx_bar,
+                     const double *hr_bar, double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
+                     double *o3vmr_bar, double *tsfc_bar, int ptr_kind, void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->lwd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_lw_vjp called before rrtmg_hip_lw_init");
+    int rc = check_common(ncol, nlay, plev, ptr_kind);
+    if (rc) return rc;
+    const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
+    const double *ins2[] = {play, tlay, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr};
+    const char *nm2[] = {"play", "tlay", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+                         "cfc11vmr", "cfc12vmr", "cfc22vmr", "ccl4vmr"};
+    double *outs[] = {tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar};
+    for (double *o : outs)
+        if (!o) return fail(RRTMG_HIP_EINVAL, "null gradient array");
+    const bool host = ptr_kind == RRTMG_HIP_HOST;
+    if (host) {
+        for (int i = 0; i < 12; ++i)
+            if ((rc = check_host_array(ins2[i], n2, nm2[i]))) return rc;
+        if ((rc = check_host_array(plev, n3, "plev")) || (rc = check_host_array(tlev, n3, "tlev")) ||
+            (rc = check_host_array(tsfc, ncol, "tsfc")) || (rc = check_host_array(emis, (size_t)ncol * 16, "emis")))
+            return rc;
+        if (tauaer && (rc = check_host_array(tauaer, n2 * 16, "tauaer"))) return rc;
+        if ((uflx_bar && (rc = check_host_array(uflx_bar, n3, "uflx_bar"))) ||
+            (dflx_bar && (rc = check_host_array(dflx_bar, n3, "dflx_bar"))) ||
+            (hr_bar && (rc = check_host_array(hr_bar, n2, "hr_bar"))))
+            return rc;
+    } else {
+        for (auto p : ins2)
+            if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
+        if (!plev || !tlev || !tsfc || !emis) return fail(RRTMG_HIP_EINVAL, "null input array");
+    }
+    hipStream_t st = (hipStream_t)stream;
+    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    if ((rc = check_lds(lw_vjp_lds_bytes(nlay, rs)))) return rc;
+    size_t off = 0;
+    const size_t o_dc = off; off = align_up(off + rs * LWF_COUNT * 4 * n2);
+    const size_t o_tlev = off; off = align_up(off + rs * n3);
+    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
+    const size_t o_ilev = off; off = align_up(off + sizeof(int) * n3);
+    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
+    const size_t o_cs = off; off = align_up(off + sizeof(double) * 2 * ncol);
+    const size_t o_cot = off; off = align_up(off + rs * 2 * n3);
+    const size_t o_gp = off; off = align_up(off + rs * 16 * 3 * n2);
+    const size_t o_gl = off; off = align_up(off + rs * 16 * n3);
+    const size_t o_gs = off; off = align_up(off + rs * 16 * 2 * ncol);
+    if ((rc = s->lw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)s->lw_vjp_work.p;
+    LwVjpWork w;
+    w.dcoef = wb + o_dc;
+    w.tlev = wb + o_tlev;
+    w.ipk = (int *)(wb + o_ipk);
+    w.ilev = (int *)(wb + o_ilev);
+    w.laytrop = (int *)(wb + o_lt);
+    w.colscal = (double *)(wb + o_cs);
+    w.cot = wb + o_cot;
+    w.gpart = wb + o_gp;
+    w.glev = wb + o_gl;
+    w.gsc = wb + o_gs;
+    if (host) {
+        const size_t need = 256 * 24 + sizeof(double) * (16 * n2 + 5 * n3 + (size_t)ncol * 18 + (tauaer ? 16 * n2 : 0));
+        if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+    }
+    Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
+    ColumnInputs in{};
+    in.ncol = ncol;
+    in.nlay = nlay;
+    in.play = sg.in(play, n2); in.plev = sg.in(plev, n3); in.tlay = sg.in(tlay, n2); in.tlev = sg.in(tlev, n3);
+    in.tsfc = sg.in(tsfc, ncol); in.h2o = sg.in(h2ovmr, n2); in.o3 = sg.in(o3vmr, n2); in.co2 = sg.in(co2vmr, n2);
+    in.ch4 = sg.in(ch4vmr, n2); in.n2o = sg.in(n2ovmr, n2); in.o2 = sg.in(o2vmr, n2);
+    in.cfc11 = sg.in(cfc11vmr, n2); in.cfc12 = sg.in(cfc12vmr, n2); in.cfc22 = sg.in(cfc22vmr, n2);
+    in.ccl4 = sg.in(ccl4vmr, n2); in.emis = sg.in(emis, (size_t)ncol * 16); in.tauaer = sg.in(tauaer, n2 * 16);
+    VjpCot cot{sg.in(uflx_bar, n3), sg.in(dflx_bar, n3), sg.in(hr_bar, n2)};
+    LwGrad g{sg.out(tlay_bar, n2), sg.out(tlev_bar, n3), sg.out(h2ovmr_bar, n2), sg.out(o3vmr_bar, n2),
+             sg.out(tsfc_bar, ncol)};
+    if (sg.ck) return fail(RRTMG_HIP_EDEVICE, "LW backward: host-to-device copy failed");
+    LwDevTables t;
+    t.desc = s->lw.tab;
+    t.bands = (const LwBand *)s->lwd.bands;
+    t.tab = mixed ? s->lwd.tab32 : s->lwd.tab64;
+    t.cst = s->lwd.cst;
+    t.units = s->lwd.units;
+    t.heatfac = s->lw.cst.heatfac;
+    hipError_t e = launch_lw_vjp(in, cot, g, w, t, mixed, st, s->events(3));
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
+    if (host) {
+        sg.back(tlay_bar, g.tlay, n2);
+        sg.back(tlev_bar, g.tlev, n3);
+        sg.back(h2ovmr_bar, g.h2o, n2);
+        sg.back(o3vmr_bar, g.o3, n2);
+        sg.back(tsfc_bar, g.tsfc, ncol);
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess || sg.ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward: ") + hipGetErrorString(e));
+    }
+    return 0;
+}
+
 int rrtmg_hip_set_timing(int on)
 {
     int prev = g_timing ? 1 : 0;
@@ -524,7 +762,8 @@ int rrtmg_hip_last_kernel_ms(int lw, float *ms)
     DeviceState *s = state();
     if (!s || !ms) return fail(RRTMG_HIP_EINVAL, "no device or null output");
     std::lock_guard<std::mutex> lk(s->mu);
-    const int side = lw ? 1 : 0;
+    if (lw < 0 || lw > 3) return fail(RRTMG_HIP_EINVAL, "call kind must be 0..3");
+    const int side = lw;
     if (!s->ev_ok || !s->ev_recorded[side]) return fail(RRTMG_HIP_ENOINIT, "no timed call recorded");
     if (hipEventSynchronize(s->ev[side][3]) != hipSuccess) return fail(RRTMG_HIP_EDEVICE, "event sync failed");
     for (int k = 0; k < 3; ++k)

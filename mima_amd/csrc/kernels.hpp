@@ -60,6 +60,46 @@ struct LwDevTables {
     double heatfac;
 };
 
+// ---------------------------------------------------------------------------
+// backward (vector-Jacobian product) of one call
+struct VjpCot {
+    const double *uflx, *dflx, *hr;  // cotangents of the outputs (NULL = zero)
+};
+struct LwGrad {
+    double *tlay, *tlev, *h2o, *o3, *tsfc;  // overwritten
+};
+struct SwGrad {
+    double *tlay, *h2o, *o3;
+};
+struct LwVjpWork {
+    void *dcoef = nullptr;    // [LWF_COUNT][4][nlay][ncol] of Real: value, d/dT, d/dq, d/dO3
+    void *tlev = nullptr;     // [nlay+1][ncol] of Real
+    int *ipk = nullptr;       // [nlay][ncol]
+    int *ilev = nullptr;      // [nlay+1][ncol]
+    int *laytrop = nullptr;   // [ncol]
+    double *colscal = nullptr;// [2][ncol]: pwvcm, tbound
+    void *cot = nullptr;      // [2][nlay+1][ncol] of Real: up / down flux cotangents incl. heating
+    void *gpart = nullptr;    // [16][3][nlay][ncol] of Real: per-band gradient (T, q, O3)
+    void *glev = nullptr;     // [16][nlay+1][ncol] of Real: per-band gradient wrt level temperature
+    void *gsc = nullptr;      // [16][2][ncol] of Real: per-band d/dtsfc and d/dpwvcm
+};
+struct SwVjpWork {
+    void *dcoef = nullptr;    // [SWF_COUNT][4][nlay][ncol] of Real
+    int *ipk = nullptr;
+    int *laytrop = nullptr;
+    int *laysolfr = nullptr;  // [14][ncol]
+    void *cot = nullptr;      // [2][nlay+1][ncol] of Real
+    void *gpart = nullptr;    // [14][3][nlay][ncol] of Real
+};
+// LDS bytes of one backward band workgroup
+inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)3 * nlay * 64 + 4 * 64); }
+inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)5 * (nlay + 1) * 64 + 4 * 64); }
+
+hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
+                         const LwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev);
+hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
+                         const SwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev);
+
 // mixed = 0: FP64 everywhere; 1: FP32 with FP64 islands.  ev: NULL, or four
 // events recorded on `s` before prep, after prep, after the band kernel and
 // after finish.

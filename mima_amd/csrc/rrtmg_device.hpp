@@ -17,6 +17,20 @@ __device__ __forceinline__ int tbl_index(F x)
     return i < 0 ? 0 : (i > 10000 ? 10000 : i);  // keeps NaN/huge inputs inside the table
 }
 
+// XCD-aware block numbering.  The 8 XCDs of the chip each have their own L2
+// and receive consecutive workgroups round-robin (workgroup i runs on XCD
+// i mod 8).  Renumbering i -> (i mod 8) * (grid/8) + i/8 hands each XCD a
+// contiguous range of logical blocks, so the wave units of one column tile,
+// which share that tile's coefficient fields, run behind the same L2.  The
+// grid is padded to a multiple of 8 (xcd_grid); logical blocks >= nblocks
+// have no work.
+__host__ __device__ __forceinline__ int xcd_grid(int nblocks) { return (nblocks + 7) / 8 * 8; }
+__device__ __forceinline__ int xcd_block()
+{
+    const int per = (int)gridDim.x / 8;
+    return ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
+}
+
 // Sum of v over the ng adjacent lanes of each column group of a one-wave
 // block; valid on the group's first lane.  Deterministic (fixed order); FP32
 // sums are compensated (Kahan).  `red` is a 64-entry LDS buffer.

@@ -1,0 +1,414 @@
+// SW optics of one (layer, g-point): per-layer coefficient record, table
+// reader, the 14 band formulas of taumol_sw (rrtmg_sw_taumol.f90), the solar
+// source at the reference layer, reftra_sw and the direct-beam transmittance.
+// Shared by the forward band kernel and the backward (vector-Jacobian)
+// kernel.  `Real` is the table type; `V`/`F` the arithmetic type: Real in the
+// forward, dual numbers in the backward.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dual.hpp"
+#include "rrtmg_device.hpp"
+#include "rrtmg_types.hpp"
+
+namespace rrtmg {
+
+// ---------------------------------------------------------------------------
+// per-layer coefficients of one column, loaded by every lane of that column
+template <typename Real>
+struct SwLayer {
+    Real fac00, fac01, fac10, fac11;
+    Real colh2o, colco2, colo3, colch4, colo2, colmol;
+    Real selffac, selffrac, forfac, forfrac;
+    int jp, jt, jt1, inds, indf;
+};
+
+template <typename Real>
+__device__ __forceinline__ void sw_load_layer(SwLayer<Real> &c, const Real *__restrict__ coef,
+                                              const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
+{
+    const size_t ix = (size_t)lay * ncol + col, fs = (size_t)nlay * ncol;
+    c.fac00 = coef[SWF_FAC00 * fs + ix];
+    c.fac01 = coef[SWF_FAC01 * fs + ix];
+    c.fac10 = coef[SWF_FAC10 * fs + ix];
+    c.fac11 = coef[SWF_FAC11 * fs + ix];
+    c.colh2o = coef[SWF_COLH2O * fs + ix];
+    c.colco2 = coef[SWF_COLCO2 * fs + ix];
+    c.colo3 = coef[SWF_COLO3 * fs + ix];
+    c.colch4 = coef[SWF_COLCH4 * fs + ix];
+    c.colo2 = coef[SWF_COLO2 * fs + ix];
+    c.colmol = coef[SWF_COLMOL * fs + ix];
+    c.selffac = coef[SWF_SELFFAC * fs + ix];
+    c.selffrac = coef[SWF_SELFFRAC * fs + ix];
+    c.forfac = coef[SWF_FORFAC * fs + ix];
+    c.forfrac = coef[SWF_FORFRAC * fs + ix];
+    int w = ipk[ix];
+    c.jp = unpack_jp(w);
+    c.jt = unpack_jt(w);
+    c.jt1 = unpack_jt1(w);
+    c.inds = unpack_inds(w);
+    c.indf = unpack_indf(w);
+}
+
+// Table reader for one lane: row r (1-based, as the reference's ind) of a
+// [rows][ng] table at `off`.
+template <typename Real>
+struct Tab {
+    const Real *__restrict__ t;
+    int ng, g;
+    __device__ __forceinline__ Real operator()(int off, int row) const { return t[off + (row - 1) * ng + g]; }
+};
+
+// binary-species parameter of a band (e.g. rrtmg_sw_taumol.f90:276-283)
+template <typename R>
+__device__ __forceinline__ void sw_eta(R cola, R colb, R strrat, R mult, R &speccomb, int &js, R &fs)
+{
+    speccomb = cola + strrat * colb;
+    R specparm = cola / speccomb;
+    const R oneminus = (R)(1.0 - 1.e-6);
+    if (specparm >= oneminus) specparm = oneminus;
+    R specmult = mult * specparm;
+    js = 1 + (int)specmult;
+    fs = specmult - (R)(int)specmult;
+}
+
+// 8-point (eta, T, p) interpolation; s1 = jt stride (9 lower, 5 upper)
+template <typename Real, typename V>
+__device__ __forceinline__ V sw_maj8(const Tab<Real> &T, int off, int ind0, int ind1, V fs, const SwLayer<V> &c, int s1)
+{
+    V fac000 = ((Real)1 - fs) * c.fac00, fac010 = ((Real)1 - fs) * c.fac10;
+    V fac100 = fs * c.fac00, fac110 = fs * c.fac10;
+    V fac001 = ((Real)1 - fs) * c.fac01, fac011 = ((Real)1 - fs) * c.fac11;
+    V fac101 = fs * c.fac01, fac111 = fs * c.fac11;
+    return fac000 * T(off, ind0) + fac100 * T(off, ind0 + 1) + fac010 * T(off, ind0 + s1) +
+           fac110 * T(off, ind0 + s1 + 1) + fac001 * T(off, ind1) + fac101 * T(off, ind1 + 1) +
+           fac011 * T(off, ind1 + s1) + fac111 * T(off, ind1 + s1 + 1);
+}
+
+template <typename Real, typename V>
+__device__ __forceinline__ V sw_maj4(const Tab<Real> &T, int off, int ind0, int ind1, const SwLayer<V> &c)
+{
+    return c.fac00 * T(off, ind0) + c.fac10 * T(off, ind0 + 1) + c.fac01 * T(off, ind1) + c.fac11 * T(off, ind1 + 1);
+}
+
+template <typename Real, typename V>
+__device__ __forceinline__ V sw_cont(const Tab<Real> &T, const SwBand &B, const SwLayer<V> &c, bool self)
+{
+    V f = c.forfac * (T(B.forref, c.indf) + c.forfrac * (T(B.forref, c.indf + 1) - T(B.forref, c.indf)));
+    if (!self) return f;
+    V s = c.selffac * (T(B.selfref, c.inds) + c.selffrac * (T(B.selfref, c.inds + 1) - T(B.selfref, c.inds)));
+    return s + f;
+}
+
+// Gas and Rayleigh optical depth of one (layer, g) for band jb (16..29),
+// rrtmg_sw_taumol.f90:243-1536.
+template <typename Real, typename V = Real>
+__device__ __forceinline__ void sw_gas(int jb, const SwBand &B, const Tab<Real> &T, const SwLayer<V> &c, bool lower,
+                                       V &taug, V &taur)
+{
+    V speccomb, fs;
+    int js;
+    int ind0, ind1;
+    if (lower) {
+        ind0 = ((c.jp - 1) * 5 + (c.jt - 1)) * B.nspa;
+        ind1 = (c.jp * 5 + (c.jt1 - 1)) * B.nspa;
+    } else {
+        ind0 = ((c.jp - 13) * 5 + (c.jt - 1)) * B.nspb;
+        ind1 = ((c.jp - 12) * 5 + (c.jt1 - 1)) * B.nspb;
+    }
+    Real rayl = (Real)B.rayl;
+    switch (jb) {
+    case 16:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colch4, (Real)252.131, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            taug = c.colch4 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c);
+        }
+        taur = c.colmol * rayl;
+        break;
+    case 17:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colco2, (Real)0.364641, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            sw_eta<V>(c.colh2o, c.colco2, (Real)0.364641, (Real)4, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absb, ind0 + js, ind1 + js, fs, c, 5) + c.colh2o * sw_cont(T, B, c, false);
+        }
+        taur = c.colmol * rayl;
+        break;
+    case 18:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colch4, (Real)38.9589, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            taug = c.colch4 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c);
+        }
+        taur = c.colmol * rayl;
+        break;
+    case 19:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colco2, (Real)5.49281, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            taug = c.colco2 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c);
+        }
+        taur = c.colmol * rayl;
+        break;
+    case 20:
+        if (lower)
+            taug = c.colh2o * (sw_maj4(T, B.absa, ind0 + 1, ind1 + 1, c) + sw_cont(T, B, c, true)) +
+                   c.colch4 * T(B.absch4, 1);
+        else
+            taug = c.colh2o * (sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c) + sw_cont(T, B, c, false)) +
+                   c.colch4 * T(B.absch4, 1);
+        taur = c.colmol * rayl;
+        break;
+    case 21:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colco2, (Real)0.0045321, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            sw_eta<V>(c.colh2o, c.colco2, (Real)0.0045321, (Real)4, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absb, ind0 + js, ind1 + js, fs, c, 5) + c.colh2o * sw_cont(T, B, c, false);
+        }
+        taur = c.colmol * rayl;
+        break;
+    case 22: {
+        V o2cont = (Real)4.35e-4 * c.colo2 / (Real)(350.0 * 2.0);
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colo2, (Real)(1.6 * 0.022708), (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true) +
+                   o2cont;
+        } else {
+            taug = c.colo2 * (Real)1.6 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c) + o2cont;
+        }
+        taur = c.colmol * rayl;
+        break;
+    }
+    case 23:
+        taug = lower ? V(c.colh2o * ((Real)1.029 * sw_maj4(T, B.absa, ind0 + 1, ind1 + 1, c) + sw_cont(T, B, c, true)))
+                     : V((Real)0);
+        taur = c.colmol * T(B.raylg, 1);
+        break;
+    case 24:
+        if (lower) {
+            sw_eta<V>(c.colh2o, c.colo2, (Real)0.124692, (Real)8, speccomb, js, fs);
+            taur = c.colmol * (T(B.rayla, js) + fs * (T(B.rayla, js + 1) - T(B.rayla, js)));
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colo3 * T(B.abso3a, 1) +
+                   c.colh2o * sw_cont(T, B, c, true);
+        } else {
+            taur = c.colmol * T(B.raylb, 1);
+            taug = c.colo2 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c) + c.colo3 * T(B.abso3b, 1);
+        }
+        break;
+    case 25:
+        taur = c.colmol * T(B.raylg, 1);
+        taug = lower ? V(c.colh2o * sw_maj4(T, B.absa, ind0 + 1, ind1 + 1, c) + c.colo3 * T(B.abso3a, 1))
+                     : V(c.colo3 * T(B.abso3b, 1));
+        break;
+    case 26:
+        taug = (Real)0;
+        taur = c.colmol * T(B.raylg, 1);
+        break;
+    case 27:
+        taur = c.colmol * T(B.raylg, 1);
+        taug = lower ? V(c.colo3 * sw_maj4(T, B.absa, ind0 + 1, ind1 + 1, c))
+                     : V(c.colo3 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c));
+        break;
+    case 28:
+        if (lower) {
+            sw_eta<V>(c.colo3, c.colo2, (Real)6.67029e-07, (Real)8, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9);
+        } else {
+            sw_eta<V>(c.colo3, c.colo2, (Real)6.67029e-07, (Real)4, speccomb, js, fs);
+            taug = speccomb * sw_maj8(T, B.absb, ind0 + js, ind1 + js, fs, c, 5);
+        }
+        taur = c.colmol * rayl;
+        break;
+    default:  // 29
+        if (lower)
+            taug = c.colh2o * (sw_maj4(T, B.absa, ind0 + 1, ind1 + 1, c) + sw_cont(T, B, c, true)) +
+                   c.colco2 * T(B.absco2, 1);
+        else
+            taug = c.colco2 * sw_maj4(T, B.absb, ind0 + 1, ind1 + 1, c) + c.colh2o * T(B.absh2o, 1);
+        taur = c.colmol * rayl;
+        break;
+    }
+}
+
+// Solar source of g-point g at the band's reference layer (sfluxzen),
+// evaluated in FP64 from that layer's column amounts.
+template <typename Real>
+__device__ double sw_sfluxzen(int jb, const SwBand &B, const Tab<Real> &T, const SwLayer<Real> &c, bool lower)
+{
+    double sp, fs;
+    int js;
+    auto flux_eta = [&](double cola, double colb, double strrat, double mult) {
+        sw_eta<double>(cola, colb, strrat, mult, sp, js, fs);
+        double a = (double)T(B.sfluxref, js), b = (double)T(B.sfluxref, js + 1);
+        return a + fs * (b - a);
+    };
+    switch (jb) {
+    case 17: return flux_eta(c.colh2o, c.colco2, 0.364641, 4.0);
+    case 18: return flux_eta(c.colh2o, c.colch4, 38.9589, 8.0);
+    case 19: return flux_eta(c.colh2o, c.colco2, 5.49281, 8.0);
+    case 21: return flux_eta(c.colh2o, c.colco2, 0.0045321, 8.0);
+    case 22: return flux_eta(c.colh2o, c.colo2, 1.6 * 0.022708, 8.0);
+    case 24: return flux_eta(c.colh2o, c.colo2, 0.124692, 8.0);
+    case 27: return (50.15 / 48.37) * (double)T(B.sfluxref, 1);
+    case 28: return flux_eta(c.colo3, c.colo2, 6.67029e-07, 4.0);
+    default: return (double)T(B.sfluxref, 1);
+    }
+    (void)lower;
+}
+
+// The same solar source in arithmetic type V (the backward's dual numbers,
+// for its derivative with respect to the reference layer's state).
+template <typename Real, typename V>
+__device__ V sw_sfluxzen_v(int jb, const SwBand &B, const Tab<Real> &T, const SwLayer<V> &c)
+{
+    V sp, fs;
+    int js;
+    auto flux_eta = [&](V cola, V colb, Real strrat, Real mult) {
+        sw_eta<V>(cola, colb, V(strrat), V(mult), sp, js, fs);
+        Real a = T(B.sfluxref, js), b = T(B.sfluxref, js + 1);
+        return a + fs * (b - a);
+    };
+    switch (jb) {
+    case 17: return flux_eta(c.colh2o, c.colco2, (Real)0.364641, (Real)4);
+    case 18: return flux_eta(c.colh2o, c.colch4, (Real)38.9589, (Real)8);
+    case 19: return flux_eta(c.colh2o, c.colco2, (Real)5.49281, (Real)8);
+    case 21: return flux_eta(c.colh2o, c.colco2, (Real)0.0045321, (Real)8);
+    case 22: return flux_eta(c.colh2o, c.colo2, (Real)(1.6 * 0.022708), (Real)8);
+    case 24: return flux_eta(c.colh2o, c.colo2, (Real)0.124692, (Real)8);
+    case 27: return V((Real)(50.15 / 48.37) * T(B.sfluxref, 1));
+    case 28: return flux_eta(c.colo3, c.colo2, (Real)6.67029e-07, (Real)4);
+    default: return V(T(B.sfluxref, 1));
+    }
+}
+
+// exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5); for a dual
+// argument the derivative is that of exp(-x) itself
+template <typename F, typename Real>
+__device__ __forceinline__ F tbl_exp(F x, const Real *__restrict__ tbl)
+{
+    return (F)tbl[tbl_index(x)];
+}
+template <typename F, int N, typename Real>
+__device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *__restrict__ tbl)
+{
+    Dual<F, N> r;
+    r.v = (F)tbl[tbl_index(x.v)];
+    const F s = -exp(-x.v);
+    for (int i = 0; i < N; ++i) r.d[i] = s * x.d[i];
+    return r;
+}
+
+// PIFM two-stream layer reflectance/transmittance, kmodts = 2, asymmetry 0
+// (rrtmg_sw_reftra.f90:127-303).  F is the arithmetic type.
+template <typename F, typename Real, typename S>
+__device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__restrict__ exp_tbl, F &pref, F &prefd,
+                                       F &ptra, F &ptrad)
+{
+    const F eps = (F)1.e-08, od_lo = (F)0.06, zwcrit = (F)0.9999995;
+    const F zg = (F)0, zg3 = (F)3 * zg;
+    F zgamma1 = ((F)8 - zw * ((F)5 + zg3)) * (F)0.25;
+    F zgamma2 = (F)3 * (zw * ((F)1 - zg)) * (F)0.25;
+    F zgamma3 = ((F)2 - zg3 * prmuz) * (F)0.25;
+    F zgamma4 = (F)1 - zgamma3;
+    F zwo = zw;  // zw / (1 - (1 - zw) * (g/(1-g))^2) with g = 0
+    if (zwo >= zwcrit) {
+        F za = zgamma1 * prmuz;
+        F za1 = za - zgamma3;
+        F zgt = zgamma1 * zto1;
+        F ze1 = fmin(zto1 / prmuz, (F)500);
+        F ze2 = ze1 <= od_lo ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+        if (ze2 == (F)1) {
+            pref = (F)0;
+            ptra = (F)1;
+            prefd = (F)0;
+            ptrad = (F)1;
+        } else {
+            pref = (zgt - za1 * ((F)1 - ze2)) / ((F)1 + zgt);
+            ptra = (F)1 - pref;
+            prefd = zgt / ((F)1 + zgt);
+            ptrad = (F)1 - prefd;
+        }
+    } else {
+        F za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+        F za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+        F zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+        F zrp = zrk * prmuz;
+        F zrp1 = (F)1 + zrp, zrm1 = (F)1 - zrp;
+        F zrk2 = (F)2 * zrk;
+        F zrpp = (F)1 - zrp * zrp;
+        F zrkg = zrk + zgamma1;
+        F zr1 = zrm1 * (za2 + zrk * zgamma3);
+        F zr2 = zrp1 * (za2 - zrk * zgamma3);
+        F zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+        F zr4 = zrpp * zrkg;
+        F zr5 = zrpp * (zrk - zgamma1);
+        F zt1 = zrp1 * (za1 + zrk * zgamma4);
+        F zt2 = zrm1 * (za1 - zrk * zgamma4);
+        F zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+        F zbeta = (zgamma1 - zrk) / zrkg;
+        F ze1 = fmin(zrk * zto1, (F)500);
+        F ze2 = fmin(zto1 / prmuz, (F)500);
+        F zem1 = ze1 <= od_lo ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+        F zep1 = (F)1 / zem1;
+        F zem2 = ze2 <= od_lo ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
+        F zep2 = (F)1 / zem2;
+        F zdenr = zr4 * zep1 + zr5 * zem1;
+        F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
+        if (zdenr >= -eps && zdenr <= eps) {
+            pref = eps;
+            ptra = zem2;
+        } else {
+            pref = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) / zdenr;
+            ptra = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) / zdent;
+        }
+        F zemm = zem1 * zem1;
+        F zdend = (F)1 / (((F)1 - zbeta * zemm) * zrkg);
+        prefd = zgamma2 * ((F)1 - zemm) * zdend;
+        ptrad = zrk2 * zem1 * zdend;
+    }
+}
+
+// optics of one (layer, g): total optical depth, reftra and direct-beam
+// transmittance (rrtmg_sw_spcvrt.f90:375-550, clear sky, no aerosol).  V is
+// the arithmetic type in the band kernel's precision and V64 its FP64
+// counterpart for the islands of the FP32-mixed path.
+template <typename Real, typename V = Real, typename V64 = double>
+__device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, const Real *__restrict__ exp_tbl, V &r, V &rd,
+                                                V &t, V &td, V &dbt)
+{
+    V tau = taur + taug;
+    V w = taur / tau;
+    bool island = false;
+    if (sizeof(Real) == 4) {
+        // FP64 islands of the mixed path: near-conservative scattering
+        // (cancellation in sqrt(g1^2 - g2^2) and 1 - exp) and the removable
+        // 0/0 of the direct-beam solution at k*mu0 = 1, where both the
+        // numerator and zdenr of reftra vanish
+        Real wv = val(w);
+        Real g1 = ((Real)8 - wv * (Real)5) * (Real)0.25, g2 = (Real)3 * wv * (Real)0.25;
+        Real zrp = sqrt(fmax(g1 * g1 - g2 * g2, (Real)0)) * prmu0;
+        island = wv > (Real)0.999 || fabs((Real)1 - zrp) < (Real)0.02;
+    }
+    if (island) {
+        V64 R, RD, T, TD;
+        V64 t64 = cast<V64>(taur) + cast<V64>(taug);
+        reftra<V64, Real, double>(t64, cast<V64>(taur) / t64, (double)prmu0, exp_tbl, R, RD, T, TD);
+        r = cast<V>(R);
+        rd = cast<V>(RD);
+        t = cast<V>(T);
+        td = cast<V>(TD);
+    } else {
+        reftra<V, Real, Real>(tau, w, prmu0, exp_tbl, r, rd, t, td);
+    }
+    V ze1 = tau / prmu0;
+    dbt = ze1 <= (Real)0.06 ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+}
+
+}  // namespace rrtmg

@@ -1,0 +1,469 @@
+// Backward (vector-Jacobian product) of the clear-sky SW path on gfx950:
+// given cotangents of swuflx, swdflx (levels) and swhr (layers), the
+// gradients with respect to layer temperature, water vapour and ozone.  No
+// counterpart in the reference; the forward being differentiated is rrtmg_sw
+// (rrtmg_sw_rad.nomcica.f90:78) as the forward kernels implement it.
+//
+// Kernels:
+//  * prep (one thread per column): inatm_sw + setcoef_sw in FP64 dual numbers
+//    seeded with (tlay, h2o, o3) of each layer ([field][value, d/dT, d/dq,
+//    d/dO3][lay][col]), the solar-source reference layers, and the level
+//    cotangents after folding the heating-rate cotangent into the fluxes.
+//  * band (one wave per work unit, lane = (column, g) as in the forward):
+//    pass 1, the adding method's bottom-up sweep (values; reflectances of the
+//    atmosphere below each level kept in LDS); pass 2, top-down: the
+//    layer's optics are recomputed as dual numbers, the downward sweep runs on
+//    their values (its states kept in LDS), the level fluxes and their local
+//    adjoints are formed, and the reverse of the bottom-up sweep is carried
+//    downward, contracting each layer's adjoint with its optics derivatives;
+//    pass 3, bottom-up: reverse of the downward sweep, contracting again, plus
+//    the solar source's dependence on its reference layer.  Per-lane gradients
+//    are reduced over the band's g-points.
+//  * finish (one thread per column): sums over bands in FP64.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "rrtmg_device.hpp"
+#include "sw_optics.hpp"
+
+namespace rrtmg {
+
+using D3 = Dual<double, 3>;
+
+template <typename V, typename Real>
+struct SwDLoad;
+template <typename Real>
+struct SwDLoad<Real, Real> {
+    __device__ static Real get(const Real *__restrict__ p, size_t) { return p[0]; }
+};
+template <typename Real>
+struct SwDLoad<Dual<Real, 3>, Real> {
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, size_t cs)
+    {
+        Dual<Real, 3> r;
+        r.v = p[0];
+        r.d[0] = p[cs];
+        r.d[1] = p[2 * cs];
+        r.d[2] = p[3 * cs];
+        return r;
+    }
+};
+
+template <typename V, typename Real>
+__device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__restrict__ dco,
+                                                const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
+{
+    const size_t fs = (size_t)nlay * ncol, ix = (size_t)lay * ncol + col;
+#define LDD(f, F) c.f = SwDLoad<V, Real>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
+    LDD(fac00, SWF_FAC00); LDD(fac01, SWF_FAC01); LDD(fac10, SWF_FAC10); LDD(fac11, SWF_FAC11);
+    LDD(colh2o, SWF_COLH2O); LDD(colco2, SWF_COLCO2); LDD(colo3, SWF_COLO3); LDD(colch4, SWF_COLCH4);
+    LDD(colo2, SWF_COLO2); LDD(colmol, SWF_COLMOL); LDD(selffac, SWF_SELFFAC); LDD(selffrac, SWF_SELFFRAC);
+    LDD(forfac, SWF_FORFAC); LDD(forfrac, SWF_FORFRAC);
+#undef LDD
+    const int w = ipk[ix];
+    c.jp = unpack_jp(w);
+    c.jt = unpack_jt(w);
+    c.jt1 = unpack_jt1(w);
+    c.inds = unpack_inds(w);
+    c.indf = unpack_indf(w);
+}
+
+// ---------------------------------------------------------------------------
+// prep: the forward prep's arithmetic (sw.hip, rrtmg_sw_rad.nomcica.f90:933-1047,
+// rrtmg_sw_setcoef.f90:129-284) on dual numbers
+template <typename Real>
+__global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCot cot, SwVjpWork w,
+                                                          const double *__restrict__ preflog,
+                                                          const double *__restrict__ tref, double heatfac)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    Real *dco = (Real *)w.dcoef;
+    const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
+    const double avogad = 6.02214199e+23, grav = 9.8066;
+    const double stpfac = 296.0 / 1013.0;
+    const size_t fs = (size_t)nlay * ncol;
+    auto put = [&](int F, size_t ix, const D3 &x) {
+        const size_t b = (size_t)F * 4 * fs + ix;
+        dco[b] = (Real)x.v;
+        dco[b + fs] = (Real)x.d[0];
+        dco[b + 2 * fs] = (Real)x.d[1];
+        dco[b + 3 * fs] = (Real)x.d[2];
+    };
+    int laytrop = 0;
+    int jp_store[MAX_NLAY + 1];
+    for (int lay = 0; lay < nlay; ++lay) {
+        const size_t ix = (size_t)lay * ncol + col;
+        const double pavel = in.play[ix];
+        const D3 tavel = D3::seed(in.tlay[ix], 0);
+        const D3 h2o = D3::seed(in.h2o[ix], 1);
+        const D3 o3 = D3::seed(in.o3[ix], 2);
+        const double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
+        const D3 wkl1 = (h2o / (1.0 - h2o)) * amdw;
+        const D3 amm = (1.0 - wkl1) * amd + wkl1 * amw;
+        const D3 coldry = ((pz0 - pz1) * 1.e3 * avogad) / (1.e2 * grav * amm * (1.0 + wkl1));
+        const D3 colh2o = 1.e-20 * (coldry * wkl1);
+        D3 colco2 = 1.e-20 * (coldry * in.co2[ix]);
+        const D3 colo3 = 1.e-20 * (coldry * (o3 * amdo));
+        D3 colch4 = 1.e-20 * (coldry * in.ch4[ix]);
+        D3 colo2 = 1.e-20 * (coldry * in.o2[ix]);
+        const D3 colmol = 1.e-20 * coldry + colh2o;
+        if (colco2.v == 0.0) colco2 = 1.e-32 * coldry;
+        if (colch4.v == 0.0) colch4 = 1.e-32 * coldry;
+        if (colo2.v == 0.0) colo2 = 1.e-32 * coldry;
+
+        const double plog = log(pavel);
+        const int jp = clampi((int)(36.0 - 5 * (plog + 0.04)), 1, 58);
+        const double fp = 5.0 * (preflog[jp - 1] - plog);
+        const D3 dt0 = (tavel - tref[jp - 1]) / 15.0;
+        const int jt = clampi((int)(3.0 + dt0.v), 1, 4);
+        const D3 ft = dt0 - (double)(jt - 3);
+        const D3 dt1 = (tavel - tref[jp]) / 15.0;
+        const int jt1 = clampi((int)(3.0 + dt1.v), 1, 4);
+        const D3 ft1 = dt1 - (double)(jt1 - 3);
+        const D3 water = (coldry * wkl1) / coldry;
+        const D3 scalefac = pavel * stpfac / tavel;
+        D3 forfac, forfrac, selffac, selffrac;
+        int indfor, indself;
+        if (plog > 4.56) {
+            ++laytrop;
+            forfac = scalefac / (1. + water);
+            D3 factor = (332.0 - tavel) / 36.0;
+            indfor = clampi((int)factor.v, 1, 2);
+            forfrac = factor - (double)indfor;
+            selffac = water * forfac;
+            factor = (tavel - 188.0) / 7.2;
+            indself = clampi((int)factor.v - 7, 1, 9);
+            selffrac = factor - (double)(indself + 7);
+        } else {
+            forfac = scalefac / (1. + water);
+            const D3 factor = (tavel - 188.0) / 36.0;
+            indfor = 3;
+            forfrac = factor - 1.0;
+            selffac = D3(0.0);
+            selffrac = D3(0.0);
+            indself = 0;
+        }
+        const double compfp = 1.0 - fp;
+        put(SWF_FAC10, ix, compfp * ft);
+        put(SWF_FAC00, ix, compfp * (1.0 - ft));
+        put(SWF_FAC11, ix, fp * ft1);
+        put(SWF_FAC01, ix, fp * (1.0 - ft1));
+        put(SWF_COLH2O, ix, colh2o);
+        put(SWF_COLCO2, ix, colco2);
+        put(SWF_COLO3, ix, colo3);
+        put(SWF_COLCH4, ix, colch4);
+        put(SWF_COLO2, ix, colo2);
+        put(SWF_COLMOL, ix, colmol);
+        put(SWF_SELFFAC, ix, selffac);
+        put(SWF_SELFFRAC, ix, selffrac);
+        put(SWF_FORFAC, ix, forfac);
+        put(SWF_FORFRAC, ix, forfrac);
+        w.ipk[ix] = pack_sw(jp, jt, jt1, indself, indfor);
+        jp_store[lay] = jp;
+    }
+    w.laytrop[col] = laytrop;
+    // solar-source reference layers, as the forward prep
+    const int lowref[14] = {0, 0, 6, 3, 3, 8, 2, 6, 1, 2, 0, 0, 0, 0};
+    const int upref[14] = {18, 30, 0, 0, 0, 0, 0, 0, 0, 0, 0, 32, 58, 49};
+    for (int b = 0; b < 14; ++b) {
+        int lsf;
+        if (upref[b] > 0) {
+            lsf = nlay;
+            for (int lay = max(laytrop + 1, 2); lay <= nlay; ++lay)
+                if (jp_store[lay - 2] < upref[b] && jp_store[lay - 1] >= upref[b]) lsf = lay;
+        } else {
+            lsf = laytrop;
+            if (lowref[b] > 0)
+                for (int lay = 1; lay <= laytrop; ++lay) {
+                    const int jpn = lay < nlay ? jp_store[lay] : jp_store[nlay - 1];
+                    if (jp_store[lay - 1] < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
+                }
+        }
+        w.laysolfr[(size_t)b * ncol + col] = max(lsf, 1);
+    }
+    // level cotangents: swhr(k) = (net(k+1) - net(k)) heatfac / (p(k) - p(k+1)),
+    // net = dn - up, top layer zero (rrtmg_sw_rad.nomcica.f90:713-726)
+    Real *cu = (Real *)w.cot, *cd = cu + (size_t)(nlay + 1) * ncol;
+    for (int k = 0; k <= nlay; ++k) {
+        const size_t ix = (size_t)k * ncol + col;
+        double nbar = 0.0;
+        if (cot.hr) {
+            if (k >= 1 && k <= nlay - 1) nbar += cot.hr[ix - ncol] * heatfac / (in.plev[ix - ncol] - in.plev[ix]);
+            if (k <= nlay - 2) nbar -= cot.hr[ix] * heatfac / (in.plev[ix] - in.plev[ix + ncol]);
+        }
+        cu[ix] = (Real)((cot.uflx ? cot.uflx[ix] : 0.0) - nbar);
+        cd[ix] = (Real)((cot.dflx ? cot.dflx[ix] : 0.0) + nbar);
+    }
+}
+
+template <int N, typename Real>
+__device__ __forceinline__ void sw_seg_sum_n(Real (&v)[N], int lane, int ng, Real *red)
+{
+    for (int i = 0; i < N; ++i) red[i * 64 + lane] = v[i];
+    __syncthreads();
+    if (lane % ng == 0 && lane + ng <= 64) {
+        for (int i = 0; i < N; ++i) {
+            Real s = (Real)0;
+            for (int j = 0; j < ng; ++j) s += red[i * 64 + lane + j];
+            v[i] = s;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename Real>
+__global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpWork w,
+                                                         const SwBand *__restrict__ bands, int nwaves, int nblocks,
+                                                         int exp_off, const Real *__restrict__ tab,
+                                                         const WaveUnit *__restrict__ units)
+{
+    using V3 = Dual<Real, 3>;
+    extern __shared__ unsigned char smem_raw[];
+    Real *smem = (Real *)smem_raw;
+    const int ncol = in.ncol, nlay = in.nlay;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int tile = blk / nwaves;
+    const WaveUnit u = units[blk % nwaves];
+    const SwBand B = bands[u.band];
+    const int jb = u.band + 16;
+    const int lane = threadIdx.x;
+    const int ng = B.ng;
+    const int cl = lane / ng, g = lane - cl * ng;
+    const int col0 = tile * TILE_COLS + u.first_col + cl;
+    const bool active = cl < u.ncols && col0 < ncol && in.coszen[min(col0, ncol - 1)] >= 1.e-10;
+    const int col = active ? col0 : 0;
+    const size_t L1 = (size_t)(nlay + 1) * 64;
+    Real *s_prup = smem, *s_prupd = smem + L1;                  // bottom-up sweep
+    Real *s_ptdbt = smem + 2 * L1, *s_ztdn = smem + 3 * L1, *s_prdnd = smem + 4 * L1;  // downward sweep
+    Real *red = smem + 5 * L1;                                   // [4][64]
+    const Real *dco = (const Real *)w.dcoef;
+    const Real *exp_tbl = tab + exp_off;
+    Tab<Real> T{tab, ng, g};
+    const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
+    Real *gpart = (Real *)w.gpart + (size_t)u.band * 3 * n2;
+    const Real *cu = (const Real *)w.cot, *cd = cu + n3;
+
+    Real prmu0 = (Real)1, zincflx = (Real)0, albp = (Real)0, albd = (Real)0;
+    int laytrop = 0, lsf = 1;
+    if (active) {
+        prmu0 = (Real)in.coszen[col];
+        laytrop = w.laytrop[col];
+        const bool nir = (jb <= 24) || jb == 29;
+        albp = (Real)(nir ? in.aldir[col] : in.asdir[col]);
+        albd = (Real)(nir ? in.aldif[col] : in.asdif[col]);
+        lsf = w.laysolfr[(size_t)u.band * ncol + col];
+        SwLayer<Real> c;
+        sw_load_layer_d<Real, Real>(c, dco, w.ipk, lsf - 1, col, ncol, nlay);
+        const double sfz = sw_sfluxzen(jb, B, T, c, lsf <= laytrop);
+        zincflx = (Real)(in.adjflux * sfz * (double)prmu0);
+    }
+
+    // pass 1: bottom-up sweep (values)
+    Real prup_n = albp, prupd_n = albd;
+    s_prup[(size_t)nlay * 64 + lane] = albp;
+    s_prupd[(size_t)nlay * 64 + lane] = albd;
+    for (int lay = 1; lay <= nlay; ++lay) {
+        const int jk = nlay + 1 - lay;
+        Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
+        if (active) {
+            SwLayer<Real> c;
+            sw_load_layer_d<Real, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+            Real taug, taur;
+            sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
+            sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
+        }
+        const Real zreflect = (Real)1 / ((Real)1 - prupd_n * rd);
+        const Real prup = r + (td * ((t - dbt) * prupd_n + dbt * prup_n)) * zreflect;
+        const Real prupd = rd + td * td * prupd_n * zreflect;
+        s_prup[(size_t)(jk - 1) * 64 + lane] = prup;
+        s_prupd[(size_t)(jk - 1) * 64 + lane] = prupd;
+        prup_n = prup;
+        prupd_n = prupd;
+    }
+
+    // layer optics of layer lay as dual numbers in (T, q, O3) of that layer
+    auto optics3 = [&](int lay, V3 &r3, V3 &rd3, V3 &t3, V3 &td3, V3 &dbt3) {
+        SwLayer<V3> c;
+        sw_load_layer_d<V3, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+        V3 tg3, tr3;
+        sw_gas<Real, V3>(jb, B, T, c, lay <= laytrop, tg3, tr3);
+        sw_layer_optics<Real, V3, Dual<double, 3>>(tg3, tr3, prmu0, exp_tbl, r3, rd3, t3, td3, dbt3);
+    };
+    auto contract = [](Real (&v)[3], Real ar, Real ard, Real at, Real atd, Real adbt, const V3 &r3, const V3 &rd3,
+                       const V3 &t3, const V3 &td3, const V3 &dbt3) {
+        for (int i = 0; i < 3; ++i)
+            v[i] = ar * r3.d[i] + ard * rd3.d[i] + at * t3.d[i] + atd * td3.d[i] + adbt * dbt3.d[i];
+    };
+
+    // pass 2: top-down
+    Real ztdn = (Real)1, prdnd = (Real)0, ptdbt = (Real)1;
+    Real a_pu = (Real)0, a_pud = (Real)0, a_zinc = (Real)0;
+    for (int jk = 1; jk <= nlay + 1; ++jk) {
+        Real c_pu = (Real)0, c_pud = (Real)0;
+        const Real pu = s_prup[(size_t)(jk - 1) * 64 + lane], pud = s_prupd[(size_t)(jk - 1) * 64 + lane];
+        if (jk > 1) {
+            const int lay = nlay + 2 - jk;
+            V3 r3(0), rd3(0), t3(1), td3(1), dbt3(1);
+            if (active) optics3(lay, r3, rd3, t3, td3, dbt3);
+            const Real r = r3.v, rd = rd3.v, t = t3.v, td = td3.v, dbt = dbt3.v;
+            // reverse of the bottom-up step that formed prup/prupd(jk-1) from level jk
+            const Real Z = (Real)1 / ((Real)1 - pud * rd);
+            const Real A = (t - dbt) * pud + dbt * pu;
+            const Real ar = a_pu;
+            const Real atd = a_pu * A * Z + a_pud * (Real)2 * td * pud * Z;
+            const Real at = a_pu * td * pud * Z;
+            const Real adbt = a_pu * td * (pu - pud) * Z;
+            const Real ard = a_pu * td * A * pud * Z * Z + a_pud * ((Real)1 + td * td * pud * pud * Z * Z);
+            c_pu = a_pu * td * dbt * Z;
+            c_pud = a_pu * (td * (t - dbt) * Z + td * A * rd * Z * Z) + a_pud * (td * td * Z + td * td * pud * rd * Z * Z);
+            Real v[3];
+            contract(v, ar, ard, at, atd, adbt, r3, rd3, t3, td3, dbt3);
+            if (!active) v[0] = v[1] = v[2] = (Real)0;
+            sw_seg_sum_n<3>(v, lane, ng, red);
+            if (active && g == 0) {
+                const size_t ix = (size_t)(lay - 1) * ncol + col;
+                gpart[ix] = v[0];
+                gpart[n2 + ix] = v[1];
+                gpart[2 * n2 + ix] = v[2];
+            }
+            // downward step (values), as the forward
+            const Real zreflect = (Real)1 / ((Real)1 - rd * prdnd);
+            const Real ztdn_n = ptdbt * t + (td * ((ztdn - ptdbt) + ptdbt * r * prdnd)) * zreflect;
+            const Real prdnd_n = rd + td * td * prdnd * zreflect;
+            ptdbt = dbt * ptdbt;
+            ztdn = ztdn_n;
+            prdnd = prdnd_n;
+        }
+        s_ptdbt[(size_t)(jk - 1) * 64 + lane] = ptdbt;
+        s_ztdn[(size_t)(jk - 1) * 64 + lane] = ztdn;
+        s_prdnd[(size_t)(jk - 1) * 64 + lane] = prdnd;
+        // level fluxes and their local adjoints with respect to prup/prupd(jk)
+        const Real zr = (Real)1 / ((Real)1 - prdnd * pud);
+        const Real pfu = (ptdbt * pu + (ztdn - ptdbt) * pud) * zr;
+        const Real pfd = ptdbt + (ztdn - ptdbt + ptdbt * pu * prdnd) * zr;
+        const int lev = nlay + 1 - jk;
+        const Real gu = active ? cu[(size_t)lev * ncol + col] : (Real)0;
+        const Real gd = active ? cd[(size_t)lev * ncol + col] : (Real)0;
+        const Real a_pfu = zincflx * gu, a_pfd = zincflx * gd;
+        a_zinc += gu * pfu + gd * pfd;
+        a_pu = c_pu + a_pfu * ptdbt * zr + a_pfd * ptdbt * prdnd * zr;
+        a_pud = c_pud + a_pfu * ((ztdn - ptdbt) * zr + pfu * prdnd * zr) + a_pfd * (pfd - ptdbt) * prdnd * zr;
+    }
+
+    // the solar source's dependence on its reference layer
+    const Real a_sfz = a_zinc * (Real)(in.adjflux * (double)prmu0);
+    V3 sfz3(0);
+    if (active) {
+        SwLayer<V3> c;
+        sw_load_layer_d<V3, Real>(c, dco, w.ipk, lsf - 1, col, ncol, nlay);
+        sfz3 = sw_sfluxzen_v<Real, V3>(jb, B, T, c);
+    }
+
+    // pass 3: bottom-up reverse of the downward sweep
+    Real A_b = (Real)0, A_z = (Real)0, A_p = (Real)0;  // adjoints of ptdbt, ztdn, prdnd at level jk
+    for (int jk = nlay + 1; jk >= 2; --jk) {
+        {
+            // local flux adjoints at level jk
+            const size_t k = (size_t)(jk - 1) * 64 + lane;
+            const Real pu = s_prup[k], pud = s_prupd[k];
+            const Real pb = s_ptdbt[k], pz = s_ztdn[k], pp = s_prdnd[k];
+            const Real zr = (Real)1 / ((Real)1 - pp * pud);
+            const Real pfu = (pb * pu + (pz - pb) * pud) * zr;
+            const Real pfd = pb + (pz - pb + pb * pu * pp) * zr;
+            const int lev = nlay + 1 - jk;
+            const Real gu = active ? cu[(size_t)lev * ncol + col] : (Real)0;
+            const Real gd = active ? cd[(size_t)lev * ncol + col] : (Real)0;
+            const Real a_pfu = zincflx * gu, a_pfd = zincflx * gd;
+            A_b += a_pfu * (pu - pud) * zr + a_pfd * ((Real)1 + ((Real)-1 + pu * pp) * zr);
+            A_z += a_pfu * pud * zr + a_pfd * zr;
+            A_p += a_pfu * pfu * pud * zr + a_pfd * (pb * pu * zr + (pfd - pb) * pud * zr);
+        }
+        const int lay = nlay + 2 - jk;
+        V3 r3(0), rd3(0), t3(1), td3(1), dbt3(1);
+        if (active) optics3(lay, r3, rd3, t3, td3, dbt3);
+        const Real r = r3.v, rd = rd3.v, t = t3.v, td = td3.v, dbt = dbt3.v;
+        const size_t k0 = (size_t)(jk - 2) * 64 + lane;  // level jk-1
+        const Real pb = s_ptdbt[k0], pz = s_ztdn[k0], pp = s_prdnd[k0];
+        const Real Zr = (Real)1 / ((Real)1 - rd * pp);
+        const Real Bz = (pz - pb) + pb * r * pp;
+        const Real at = A_z * pb;
+        const Real atd = A_z * Bz * Zr + A_p * (Real)2 * td * pp * Zr;
+        const Real ar = A_z * td * pb * pp * Zr;
+        const Real ard = A_z * td * Bz * pp * Zr * Zr + A_p * ((Real)1 + td * td * pp * pp * Zr * Zr);
+        const Real adbt = A_b * pb;
+        const Real nA_b = A_z * (t + td * ((Real)-1 + r * pp) * Zr) + A_b * dbt;
+        const Real nA_z = A_z * td * Zr;
+        const Real nA_p = A_z * (td * pb * r * Zr + td * Bz * rd * Zr * Zr) + A_p * (td * td * Zr + td * td * pp * rd * Zr * Zr);
+        A_b = nA_b;
+        A_z = nA_z;
+        A_p = nA_p;
+        Real v[3];
+        contract(v, ar, ard, at, atd, adbt, r3, rd3, t3, td3, dbt3);
+        if (lay == lsf)
+            for (int i = 0; i < 3; ++i) v[i] += a_sfz * sfz3.d[i];
+        if (!active) v[0] = v[1] = v[2] = (Real)0;
+        sw_seg_sum_n<3>(v, lane, ng, red);
+        if (active && g == 0) {
+            const size_t ix = (size_t)(lay - 1) * ncol + col;
+            gpart[ix] += v[0];
+            gpart[n2 + ix] += v[1];
+            gpart[2 * n2 + ix] += v[2];
+        }
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void sw_vjp_finish_kernel(ColumnInputs in, SwVjpWork w, SwGrad out)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    const size_t n2 = (size_t)nlay * ncol;
+    const Real *gpart = (const Real *)w.gpart;
+    const bool day = in.coszen[col] >= 1.e-10;
+    for (int lay = 0; lay < nlay; ++lay) {
+        const size_t ix = (size_t)lay * ncol + col;
+        double gt = 0.0, gq = 0.0, go = 0.0;
+        if (day)
+            for (int b = 0; b < 14; ++b) {
+                const Real *gp = gpart + (size_t)b * 3 * n2;
+                gt += (double)gp[ix];
+                gq += (double)gp[n2 + ix];
+                go += (double)gp[2 * n2 + ix];
+            }
+        out.tlay[ix] = gt;
+        out.h2o[ix] = gq;
+        out.o3[ix] = go;
+    }
+}
+
+template <typename Real>
+static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
+                                  const SwDevTables &t, hipStream_t s, hipEvent_t *ev)
+{
+    const int nb = (in.ncol + 255) / 256;
+    if (ev) (void)hipEventRecord(ev[0], s);
+    hipLaunchKernelGGL(sw_vjp_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, cot, w, t.preflog, t.tref, t.heatfac);
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    const size_t shmem = sw_vjp_lds_bytes(in.nlay, sizeof(Real));
+    const int nblocks = ntiles * t.desc.nwaves;
+    if (ev) (void)hipEventRecord(ev[1], s);
+    hipLaunchKernelGGL(sw_vjp_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
+                       t.desc.nwaves, nblocks, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    if (ev) (void)hipEventRecord(ev[2], s);
+    hipLaunchKernelGGL(sw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
+                         const SwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev)
+{
+    return mixed ? launch_sw_vjp_t<float>(in, cot, out, w, t, s, ev)
+                 : launch_sw_vjp_t<double>(in, cot, out, w, t, s, ev);
+}
+
+}  // namespace rrtmg

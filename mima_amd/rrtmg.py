@@ -40,6 +40,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise ImportError("librrtmg_hip.so not built (run __graft_entry__.build() or make -C mima_amd/csrc)")
+        # torch ships its own HIP runtime under the same soname; load it first so
+        # that this library binds to that one instance (a second runtime loaded
+        # before torch leaves torch without devices)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         L.rrtmg_hip_last_error.restype = ctypes.c_char_p
         L.rrtmg_hip_debug_table.restype = ctypes.c_long
@@ -147,6 +154,57 @@ def rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr
     return tuple(outs)
 
 
+def _grad_outputs(dev, like, shapes):
+    if dev:
+        import torch
+        return [torch.empty(sh, dtype=torch.float64, device=like.device) for sh in shapes]
+    return [np.empty(sh) for sh in shapes]
+
+
+def rrtmg_lw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                 cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, uflx_bar=None, dflx_bar=None, hr_bar=None,
+                 tauaer=None, stream=None):
+    """Backward of rrtmg_lw (no reference counterpart; C ABI rrtmg_hip_lw_vjp).
+
+    Takes rrtmg_lw's inputs and the cotangents of uflx, dflx ((nlay+1, ncol))
+    and hr ((nlay, ncol)), each None for zero.  Returns the gradients
+    (tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar) of
+    sum(uflx_bar*uflx + dflx_bar*dflx + hr_bar*hr)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _grad_outputs(dev, play, [(nlay, ncol), (nlay + 1, ncol), (nlay, ncol), (nlay, ncol), (ncol,)])
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_lw_vjp(
+        int(ncol), int(nlay),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar)],
+        *optr, DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
+def rrtmg_sw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                 asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx_bar=None, swdflx_bar=None,
+                 swhr_bar=None, stream=None):
+    """Backward of rrtmg_sw (no reference counterpart; C ABI rrtmg_hip_sw_vjp).
+
+    Returns (tlay_bar, h2ovmr_bar, o3vmr_bar) of
+    sum(swuflx_bar*swuflx + swdflx_bar*swdflx + swhr_bar*swhr)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _grad_outputs(dev, play, [(nlay, ncol), (nlay, ncol), (nlay, ncol)])
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_sw_vjp(
+        int(ncol), int(nlay),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             asdir, asdif, aldir, aldif, coszen)],
+        ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon),
+        *[a.inp(x) for x in (swuflx_bar, swdflx_bar, swhr_bar)],
+        *optr, DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
 SW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
            "asdir", "asdif", "aldir", "aldif", "coszen")
 LW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
@@ -175,6 +233,20 @@ def sw_columns(c, stream=None):
 def lw_columns(c, tauaer=None, stream=None):
     out = rrtmg_lw(c["ncol"], c["nlay"], 0, 0, *[c[k] for k in LW_KEYS], tauaer=tauaer, stream=stream)
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
+
+
+def lw_vjp_columns(c, uflx_bar=None, dflx_bar=None, hr_bar=None, tauaer=None, stream=None):
+    """rrtmg_lw_vjp on a mima_amd.columns dict; returns a dict of gradients."""
+    out = rrtmg_lw_vjp(c["ncol"], c["nlay"], *[c[k] for k in LW_KEYS], uflx_bar=uflx_bar, dflx_bar=dflx_bar,
+                       hr_bar=hr_bar, tauaer=tauaer, stream=stream)
+    return dict(zip(("tlay", "tlev", "h2ovmr", "o3vmr", "tsfc"), out))
+
+
+def sw_vjp_columns(c, swuflx_bar=None, swdflx_bar=None, swhr_bar=None, stream=None):
+    """rrtmg_sw_vjp on a mima_amd.columns dict; returns a dict of gradients."""
+    out = rrtmg_sw_vjp(c["ncol"], c["nlay"], *[c[k] for k in SW_KEYS], c["adjes"], c["dyofyr"], c["scon"],
+                       swuflx_bar=swuflx_bar, swdflx_bar=swdflx_bar, swhr_bar=swhr_bar, stream=stream)
+    return dict(zip(("tlay", "h2ovmr", "o3vmr"), out))
 
 
 def debug_indices(lw, ncol, nlay):

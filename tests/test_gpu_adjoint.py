@@ -1,0 +1,267 @@
+"""Backward (vector-Jacobian product) kernels against central finite
+differences of the FP64 GPU forward.
+
+Workload of the checks: 64 seeded-random columns of the 128 x 64 x 40 grid
+(day and night), cotangents N(0, 1) from a seed; central differences with
+h = 1e-3 K for temperatures and 1e-3 relative for water vapour and ozone.
+The forward's transmittance lookups are tables while the backward
+differentiates the functions they tabulate, so the finite differences carry
+table-quantisation noise; the tolerances below are stated relative to the
+size of the derivative being checked.
+"""
+import numpy as np
+import pytest
+
+from mima_amd.columns import make_columns, subset
+
+pytestmark = pytest.mark.gpu
+
+H_T = 1e-3     # K
+H_REL = 1e-3   # relative, water vapour and ozone
+
+
+@pytest.fixture(scope="module")
+def rr():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from mima_amd import rrtmg
+    rrtmg.rrtmg_sw_ini()
+    rrtmg.rrtmg_lw_ini()
+    yield rrtmg
+    rrtmg.set_precision("fp64")
+
+
+def _cols(n=64, seed=3):
+    full = make_columns(128, 64, 40, seed=0)
+    idx = np.sort(np.random.default_rng(seed).choice(full["ncol"], n, replace=False))
+    return subset(full, idx)
+
+
+def _dev(rr, c):
+    return rr.to_device(c)
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _perturbed(c, dT=None, dtlev=None, dq=None, do3=None, dts=None, s=1.0):
+    p = dict(c)
+    if dT is not None:
+        p["tlay"] = c["tlay"] + s * H_T * dT
+    if dtlev is not None:
+        p["tlev"] = c["tlev"] + s * H_T * dtlev
+    if dq is not None:
+        p["h2ovmr"] = c["h2ovmr"] * (1.0 + s * H_REL * dq)
+    if do3 is not None:
+        p["o3vmr"] = c["o3vmr"] * (1.0 + s * H_REL * do3)
+    if dts is not None:
+        p["tsfc"] = c["tsfc"] + s * H_T * dts
+    return p
+
+
+def _lw_J(rr, c, cot):
+    rr.set_precision("fp64")
+    o = rr.lw_columns(_dev(rr, c))
+    return sum(float(np.sum(cot[k] * _np(o[k]))) for k in ("uflx", "dflx", "hr"))
+
+
+def _sw_J(rr, c, cot):
+    rr.set_precision("fp64")
+    o = rr.sw_columns(_dev(rr, c))
+    return sum(float(np.sum(cot[k] * _np(o[k]))) for k in ("swuflx", "swdflx", "swhr"))
+
+
+def _lw_cot(c, rng):
+    L, n = c["nlay"], c["ncol"]
+    return {"uflx": rng.standard_normal((L + 1, n)) * 0.1, "dflx": rng.standard_normal((L + 1, n)) * 0.1,
+            "hr": rng.standard_normal((L, n))}
+
+
+def _sw_cot(c, rng):
+    L, n = c["nlay"], c["ncol"]
+    return {"swuflx": rng.standard_normal((L + 1, n)) * 0.1, "swdflx": rng.standard_normal((L + 1, n)) * 0.1,
+            "swhr": rng.standard_normal((L, n))}
+
+
+def _lw_grad(rr, c, cot, prec="fp64"):
+    import torch
+    rr.set_precision(prec)
+    d = _dev(rr, c)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+    g = rr.lw_vjp_columns(d, uflx_bar=t["uflx"], dflx_bar=t["dflx"], hr_bar=t["hr"])
+    return {k: _np(v) for k, v in g.items()}
+
+
+def _sw_grad(rr, c, cot, prec="fp64"):
+    import torch
+    rr.set_precision(prec)
+    d = _dev(rr, c)
+    t = {k: torch.from_numpy(np.ascontiguousarray(v)).cuda() for k, v in cot.items()}
+    g = rr.sw_vjp_columns(d, swuflx_bar=t["swuflx"], swdflx_bar=t["swdflx"], swhr_bar=t["swhr"])
+    return {k: _np(v) for k, v in g.items()}
+
+
+LW_DIRS = ["tlay", "tlev", "h2ovmr", "o3vmr", "tsfc"]
+SW_DIRS = ["tlay", "h2ovmr", "o3vmr"]
+
+
+def _directional(c, g, which, delta):
+    if which in ("tlay", "tlev", "tsfc"):
+        return float(np.sum(g[which] * delta))
+    return float(np.sum(g[which] * c[which] * delta))
+
+
+def _kw(which, delta):
+    return {"tlay": {"dT": delta}, "tlev": {"dtlev": delta}, "h2ovmr": {"dq": delta}, "o3vmr": {"do3": delta},
+            "tsfc": {"dts": delta}}[which]
+
+
+@pytest.mark.parametrize("which", LW_DIRS)
+def test_lw_vjp_directional_derivative_matches_central_differences(rr, which):
+    c = _cols()
+    rng = np.random.default_rng(11)
+    cot = _lw_cot(c, rng)
+    delta = rng.standard_normal(c[which].shape)
+    g = _lw_grad(rr, c, cot)
+    ad = _directional(c, g, which, delta)
+    fd = (_lw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0), cot) -
+          _lw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0), cot)) / (2.0 * (H_T if which in ("tlay", "tlev", "tsfc") else H_REL))
+    print("LW %s: ad %.6e fd %.6e rel %.2e" % (which, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
+    assert abs(ad - fd) <= 5e-3 * abs(fd) + 1e-9, (which, ad, fd)
+
+
+@pytest.mark.parametrize("which", SW_DIRS)
+def test_sw_vjp_directional_derivative_matches_central_differences(rr, which):
+    c = _cols()
+    rng = np.random.default_rng(12)
+    cot = _sw_cot(c, rng)
+    delta = rng.standard_normal(c[which].shape)
+    g = _sw_grad(rr, c, cot)
+    ad = _directional(c, g, which, delta)
+    fd = (_sw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0), cot) -
+          _sw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0), cot)) / (2.0 * (H_T if which == "tlay" else H_REL))
+    print("SW %s: ad %.6e fd %.6e rel %.2e" % (which, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
+    assert abs(ad - fd) <= 5e-3 * abs(fd) + 1e-9, (which, ad, fd)
+
+
+def test_sw_vjp_zero_on_night_columns(rr):
+    c = _cols()
+    rng = np.random.default_rng(13)
+    g = _sw_grad(rr, c, _sw_cot(c, rng))
+    night = c["coszen"] < 1e-10
+    assert night.any() and (~night).any()
+    for k in SW_DIRS:
+        assert np.all(g[k][:, night] == 0.0)
+        assert np.any(g[k][:, ~night] != 0.0)
+
+
+@pytest.mark.parametrize("side", ["lw", "sw"])
+def test_heating_jacobian_rows_match_finite_differences(rr, side):
+    """Full d hr / d T of two columns: rows from the backward (one unit
+    cotangent per replicated column) against columns of the central-difference
+    Jacobian (one +-h perturbation of one layer per replicated column)."""
+    c0 = _cols()
+    day = np.where(c0["coszen"] > 0.3)[0][:2]
+    L = c0["nlay"]
+    errs = []
+    for i in day:
+        one = subset(c0, [i])
+        rep = subset(one, np.zeros(L, dtype=int))
+        # backward: row k of J = gradient of hr[k]
+        hr_bar = np.eye(L)
+        if side == "lw":
+            zl = np.zeros((L + 1, L))
+            g = _lw_grad(rr, rep, {"uflx": zl, "dflx": zl, "hr": hr_bar})["tlay"]
+        else:
+            zl = np.zeros((L + 1, L))
+            g = _sw_grad(rr, rep, {"swuflx": zl, "swdflx": zl, "swhr": hr_bar})["tlay"]
+        J_ad = g.T  # J_ad[k, j] = d hr[k] / d T[j]
+        # central differences: column j perturbs T[j]
+        rep2 = subset(one, np.zeros(2 * L, dtype=int))
+        dT = np.zeros((L, 2 * L))
+        dT[np.arange(L), np.arange(L)] = 1.0
+        dT[np.arange(L), L + np.arange(L)] = -1.0
+        rep2["tlay"] = rep2["tlay"] + H_T * dT
+        rr.set_precision("fp64")
+        d2 = rr.to_device(rep2)
+        hr = _np((rr.lw_columns(d2) if side == "lw" else rr.sw_columns(d2))["hr" if side == "lw" else "swhr"])
+        J_fd = (hr[:, :L] - hr[:, L:]) / (2.0 * H_T)
+        scale = np.max(np.abs(J_fd))
+        err = np.max(np.abs(J_ad - J_fd)) / scale
+        errs.append(err)
+        print("%s column %d: max |J_ad - J_fd| / max |J| = %.2e (max |J| = %.3e K/day/K)" % (side, i, err, scale))
+    assert max(errs) < 2e-2, errs
+
+
+@pytest.mark.parametrize("side", ["lw", "sw"])
+def test_mixed_precision_vjp_close_to_fp64(rr, side):
+    c = _cols()
+    rng = np.random.default_rng(14)
+    if side == "lw":
+        cot = _lw_cot(c, rng)
+        g64, g32 = _lw_grad(rr, c, cot, "fp64"), _lw_grad(rr, c, cot, "mixed")
+        keys = LW_DIRS
+    else:
+        cot = _sw_cot(c, rng)
+        g64, g32 = _sw_grad(rr, c, cot, "fp64"), _sw_grad(rr, c, cot, "mixed")
+        keys = SW_DIRS
+    for k in keys:
+        a, b = g64[k], g32[k]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(a), 1e-300)
+        print("%s %s: |g32 - g64| / |g64| = %.2e" % (side, k, rel))
+        assert rel < 1e-2, (side, k, rel)
+
+
+def test_vjp_host_and_device_paths_identical(rr):
+    c = _cols(16, seed=5)
+    rng = np.random.default_rng(15)
+    cot = _lw_cot(c, rng)
+    rr.set_precision("fp64")
+    gh = rr.lw_vjp_columns(c, uflx_bar=cot["uflx"], dflx_bar=cot["dflx"], hr_bar=cot["hr"])
+    gd = _lw_grad(rr, c, cot)
+    for k in LW_DIRS:
+        assert np.array_equal(gh[k], gd[k]), k
+    scot = _sw_cot(c, rng)
+    sh = rr.sw_vjp_columns(c, swuflx_bar=scot["swuflx"], swdflx_bar=scot["swdflx"], swhr_bar=scot["swhr"])
+    sd = _sw_grad(rr, c, scot)
+    for k in SW_DIRS:
+        assert np.array_equal(sh[k], sd[k]), k
+
+
+def test_torch_autograd_matches_finite_differences(rr):
+    """radiation(): heating rates and surface net fluxes as torch functions of
+    (T, q, O3, Ts), with tlev built from T and Ts; the gradient of a random
+    functional against a central difference along a random direction."""
+    import torch
+    from mima_amd.autograd import radiation
+    c = _cols(32, seed=7)
+    rr.set_precision("fp64")
+    d = rr.to_device(c)
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    L, n = c["nlay"], c["ncol"]
+    w = [torch.randn(L, n, generator=gen, dtype=torch.float64).cuda(),
+         torch.randn(L, n, generator=gen, dtype=torch.float64).cuda(),
+         torch.randn(n, generator=gen, dtype=torch.float64).cuda(),
+         torch.randn(n, generator=gen, dtype=torch.float64).cuda()]
+    x = [d["tlay"].clone().requires_grad_(True), d["h2ovmr"].clone().requires_grad_(True),
+         d["o3vmr"].clone().requires_grad_(True), d["tsfc"].clone().requires_grad_(True)]
+
+    def J(t, q, o, ts):
+        out = radiation(t, q, o, ts, d)
+        return sum((wi * oi).sum() for wi, oi in zip(w, out))
+
+    J(*x).backward()
+    dirs = [torch.randn(L, n, generator=gen, dtype=torch.float64).cuda(),
+            torch.randn(L, n, generator=gen, dtype=torch.float64).cuda(),
+            torch.randn(L, n, generator=gen, dtype=torch.float64).cuda(),
+            torch.randn(n, generator=gen, dtype=torch.float64).cuda()]
+    scales = [H_T, H_REL * d["h2ovmr"], H_REL * d["o3vmr"], H_T]
+    steps = [s * di for s, di in zip(scales, dirs)]
+    ad = sum(float((xi.grad * st).sum()) for xi, st in zip(x, steps))
+    with torch.no_grad():
+        jp = float(J(*[xi + st for xi, st in zip(x, steps)]))
+        jm = float(J(*[xi - st for xi, st in zip(x, steps)]))
+    fd = (jp - jm) / 2.0
+    print("autograd: ad %.6e fd %.6e" % (ad, fd))
+    assert abs(ad - fd) <= 5e-3 * abs(fd), (ad, fd)

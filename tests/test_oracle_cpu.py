@@ -304,3 +304,15 @@ def test_library_table_reduction_matches_oracle():
                 np.testing.assert_array_equal(a, po.reduced(key), err_msg=key)
                 n += 1
     assert n > 150
+
+
+def test_autograd_level_temperature_rule_matches_generator():
+    # interface temperatures built from layer temperatures and Ts in torch (the
+    # differentiable step in front of the backward kernels) equal the
+    # generator's interp_temp construction
+    import torch
+    from mima_amd.autograd import mima_tlev
+    c = make_columns(16, 8, 40, seed=4)
+    t = mima_tlev(torch.from_numpy(c["tlay"]), torch.from_numpy(c["tsfc"]), torch.from_numpy(c["play"]),
+                  torch.from_numpy(c["plev"])).numpy()
+    np.testing.assert_allclose(t, c["tlev"], rtol=1e-12, atol=0)
