@@ -68,6 +68,108 @@ def traffic_from_profile(kernel, ncol, nlay):
     return None
 
 
+def adjoint_timing(torch, rrtmg, config_columns, args):
+    """Backward (VJP) of LW+SW heating rates and surface net fluxes on the
+    adjoint workload (128 x 64 = 8192 columns x 40 levels, FP32-mixed):
+    columns/s of one LW+SW backward call pair, and its cost relative to the
+    forward pair on the same columns."""
+    cols = config_columns("c2_adjoint", seed=0)
+    ncol, nlay = cols["ncol"], cols["nlay"]
+    d = rrtmg.to_device(cols)
+    gen = torch.Generator(device="cpu").manual_seed(0)
+    rnd = lambda *sh: torch.randn(*sh, generator=gen, dtype=torch.float64).cuda()  # noqa: E731
+    hr_lw, hr_sw = rnd(nlay, ncol), rnd(nlay, ncol)
+    f_lw, f_sw = rnd(ncol), rnd(ncol)
+    z = lambda: torch.zeros(nlay + 1, ncol, dtype=torch.float64, device="cuda")  # noqa: E731
+    ub, db, sub, sdb = z(), z(), z(), z()
+    ub[0], db[0], sub[0], sdb[0] = -f_lw, f_lw, -f_sw, f_sw
+    rrtmg.set_precision("mixed")
+
+    def fwd():
+        rrtmg.sw_columns(d)
+        rrtmg.lw_columns(d)
+
+    def bwd():
+        rrtmg.sw_vjp_columns(d, swuflx_bar=sub, swdflx_bar=sdb, swhr_bar=hr_sw)
+        rrtmg.lw_vjp_columns(d, uflx_bar=ub, dflx_bar=db, hr_bar=hr_lw)
+
+    res = {}
+    for name, fn in (("forward", fwd), ("backward", bwd)):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        n = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        res[name] = (time.perf_counter() - t0) / n
+    print("[bench] adjoint workload %d x %d: forward %.3f ms, backward %.3f ms" %
+          (ncol, nlay, res["forward"] * 1e3, res["backward"] * 1e3), file=sys.stderr, flush=True)
+    return {"workload": "backward of LW+SW heating rates and surface net fluxes wrt T, q, O3, Ts (and level T); "
+                        "128x64 grid = %d columns x %d levels, FP32-mixed" % (ncol, nlay),
+            "value": ncol / res["backward"], "unit": "columns/s", "ms_per_step": res["backward"] * 1e3,
+            "forward_ms_per_step": res["forward"] * 1e3,
+            "backward_vs_forward_cost": res["backward"] / res["forward"]}
+
+
+def surrogate_timing(torch, rrtmg, config_columns, args):
+    """MLP surrogate: trained on the GPU kernels on 262144 columns (512 x 512,
+    lapse rate U(4, 9) K/km, RH U(0.2, 1)) labelled by the exact FP32-mixed
+    kernels; inference on 131072 columns (512 x 256) of the same generator
+    with a different seed.  Reports surrogate-only inference columns/s, the
+    hybrid call (out-of-envelope fallback + exact check on 1024 columns), the
+    exact kernels' columns/s on the same columns, the error against them, and
+    the MFMA rate of the inference GEMMs."""
+    from mima_amd.surrogate import Surrogate, exact_heating, hybrid_heating
+    rrtmg.set_precision("mixed")
+    tr = rrtmg.to_device(config_columns("c3_train", seed=0))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hl, hs = exact_heating(tr)
+    torch.cuda.synchronize()
+    t_label = time.perf_counter() - t0
+    s = Surrogate(nlay=tr["nlay"], seed=0)
+    t0 = time.perf_counter()
+    hist = s.fit(tr, hl, hs, epochs=args.surrogate_epochs, batch=1024, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    ntrain = tr["ncol"]
+    del tr, hl, hs
+    inf = rrtmg.to_device(config_columns("c3_infer", seed=1))
+    ncol = inf["ncol"]
+
+    def timed(fn, n):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / n
+
+    t_pred = timed(lambda: s.predict(inf), 20)
+    t_hyb = timed(lambda: hybrid_heating(s, inf, check=1024), 10)
+    t_exact = timed(lambda: exact_heating(inf), 5)
+    _, _, rep = hybrid_heating(s, inf, check=8192, seed=3)
+    kin = [s.ldx] + list(s.widths[:-1])
+    flop_col = sum(2 * i * o for i, o in zip(kin, s.widths))
+    tflops = flop_col * ncol / t_pred / 1e12
+    print("[bench] surrogate: train %.1f s (%d epochs), predict %.3f ms, hybrid %.3f ms, exact %.3f ms, rmse lw %.3f "
+          "sw %.3f K/day" % (t_train, args.surrogate_epochs, t_pred * 1e3, t_hyb * 1e3, t_exact * 1e3,
+                             rep["rmse_lw_kday"], rep["rmse_sw_kday"]), file=sys.stderr, flush=True)
+    return {"workload": "MLP %d-%s heating-rate surrogate, inference on 512x256 = %d columns x %d levels"
+                        % (s.nin, "-".join(str(w) for w in s.widths), ncol, inf["nlay"]),
+            "value": ncol / t_pred, "unit": "columns/s", "ms_per_call": t_pred * 1e3,
+            "hybrid_ms_per_call": t_hyb * 1e3, "hybrid_value": ncol / t_hyb,
+            "exact_ms_same_columns": t_exact * 1e3, "speedup_vs_exact": t_exact / t_pred,
+            "error_vs_exact": rep, "train_columns": ntrain, "train_epochs": args.surrogate_epochs,
+            "train_s": t_train, "label_s": t_label, "final_train_loss": hist[-1],
+            "mfma": {"achieved": tflops, "peak": 2500.0, "unit": "TFLOP/s", "frac": tflops / 2500.0,
+                     "flop_per_column": flop_col, "dtype": "bf16 x bf16 -> fp32"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,6 +177,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--config", default="c1_mixed_vs_fp64")
+    ap.add_argument("--surrogate-epochs", type=int, default=10,
+                    help="training epochs of the MLP surrogate (0 skips the surrogate leg)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -147,6 +251,9 @@ def main():
     bcol = 8 * (12 * nlay + 10)
     achieved = bcol * ncol / (dom_ms * 1e-3) / 1e9
 
+    adj = adjoint_timing(torch, rrtmg, config_columns, args)
+    sur = surrogate_timing(torch, rrtmg, config_columns, args) if args.surrogate_epochs > 0 else None
+
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -199,6 +306,8 @@ def main():
                      "traffic": traffic_from_profile(names[dom_key], ncol, nlay),
                      "kernel": names[dom_key], "bytes_per_column": bcol},
         "cpu_baseline": cpu,
+        "adjoint": adj,
+        "surrogate": sur,
     }
     print(json.dumps(out))
 

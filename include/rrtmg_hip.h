@@ -36,6 +36,10 @@ extern "C" {
 
 #define RRTMG_HIP_PREC_FP64 0  /* every kernel in FP64 (the GPU oracle path) */
 #define RRTMG_HIP_PREC_MIXED 1 /* FP32 gas optics/transfer with FP64 islands  */
+#define RRTMG_HIP_PREC_FP64_SMOOTH 2 /* FP64 with exp(-x) and the tau transition
+                                      * function evaluated instead of read from
+                                      * the 10000-point tables: smooth in the
+                                      * inputs, for finite-difference checks */
 
 #define RRTMG_HIP_OK 0
 #define RRTMG_HIP_EINVAL (-22)   /* bad size, NaN/Inf input, non-monotone plev, coszen < 0 */
@@ -132,6 +136,76 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay,
                      const double *swuflx_bar, const double *swdflx_bar, const double *swhr_bar,
                      double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar,
                      int ptr_kind, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * MLP surrogate of the LW+SW heating-rate profile (no reference counterpart;
+ * it learns what rrtmg_hip_sw/rrtmg_hip_lw compute and is judged against
+ * them).  All pointers are device pointers; work is enqueued on `stream`.
+ * BF16 arrays are passed as void* (IEEE bfloat16 bit patterns). */
+#define RRTMG_HIP_MLP_MAX_LAYERS 8
+
+/* A trained network: layer l maps width_{l-1} -> width[l] (width_{-1} = ldx,
+ * the padded feature count), weights w[l] BF16 row-major [width[l]][width_{l-1}],
+ * biases b[l] FP32 [width[l]]; ReLU after every layer but the last.  Features
+ * of a column: tlay (nlay), ln h2ovmr (nlay), o3vmr x 1e6 (nlay), tsfc, coszen,
+ * normalised as (x - in_mean) * in_istd; outputs are (hr_lw, hr_sw) normalised
+ * as (hr - out_mean) / out_std.  in_lo/in_hi bound the training envelope of
+ * the raw features (NULL: no envelope test). */
+typedef struct {
+    int nlay, nin, nout, ldx, nlayers;
+    int width[RRTMG_HIP_MLP_MAX_LAYERS];
+    const void *w[RRTMG_HIP_MLP_MAX_LAYERS];
+    const float *b[RRTMG_HIP_MLP_MAX_LAYERS];
+    const float *in_mean, *in_istd, *in_lo, *in_hi;
+    const float *out_mean, *out_std;
+} RrtmgHipMlp;
+
+/* Surrogate inference: heating rates (ncol, nlay) in K/day of the columns
+ * (tlay, h2ovmr, o3vmr as rrtmg_sw/rrtmg_lw take them; tsfc, coszen (ncol))
+ * and out_of_envelope[ncol] = 1 where some feature lies outside
+ * [in_lo, in_hi] (the caller recomputes those columns with the exact entry
+ * points).  Uses a library-owned workspace of ncol x (2 max width BF16 +
+ * output FP32). */
+int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2ovmr,
+                        const double *o3vmr, const double *tsfc, const double *coszen,
+                        const RrtmgHipMlp *net, double *hr_lw, double *hr_sw,
+                        int *out_of_envelope, void *stream);
+
+/* Training kernels.  C = op(A) op(B) in BF16 with FP32 accumulation:
+ * op(A)(m,k) = A[m*lda + k] (transA = 0) or A[k*lda + m]; op(B)(k,n) =
+ * B[n*ldb + k] (transB = 0, the weight layout) or B[k*ldb + n].  Epilogue:
+ * v = C + bias[n] (bias may be NULL); out_pre = v; v = max(v, 0) if relu;
+ * out_f32 = v (+= with accumulate); out_bf16 = bf16(v); outputs row pitch ldc,
+ * each may be NULL. */
+int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA,
+                        const void *B, int ldb, int transB, const float *bias, int relu,
+                        float *out_pre, float *out_f32, void *out_bf16, int ldc,
+                        int accumulate, void *stream);
+/* dz = (pre[i] > 0 ? dy[i] : 0), i < n, into dz_bf16 and dz_f32 (may be NULL) */
+int rrtmg_hip_mlp_relu_backward(long n, const float *dy, const float *pre, void *dz_bf16, float *dz_f32,
+                                void *stream);
+/* out[n] = sum_m x[m*ldx + n] */
+int rrtmg_hip_mlp_col_sum(int M, int N, const float *x, int ldx, float *out, void *stream);
+/* loss_partial[b] = per-block partial sums of sum (y - t)^2 over M x N (nblocks
+ * blocks); dy (FP32) and dy_bf16 (each may be NULL) = 2 (y - t) / (M N) */
+int rrtmg_hip_mlp_mse(int M, int N, const float *y, int ldy, const float *t, int ldt, float *dy,
+                      void *dy_bf16, int ldd, float *loss_partial, int nblocks, void *stream);
+/* one Adam step (bias-corrected, step >= 1) on FP32 master weights w with
+ * moments m, v and gradient g; writes the BF16 copy w_bf16 (may be NULL) */
+int rrtmg_hip_mlp_adam(long n, float *w, float *m, float *v, const float *g, void *w_bf16,
+                       float lr, float beta1, float beta2, float eps, float weight_decay,
+                       int step, void *stream);
+/* features of columns as rrtmg_hip_surrogate forms them: x_bf16 [ncol][ldx]
+ * (normalised when mean/istd are given), xraw [ncol][ldx] FP32 raw features
+ * (may be NULL), out_of_envelope (may be NULL; needs lo/hi) */
+int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double *h2ovmr,
+                           const double *o3vmr, const double *tsfc, const double *coszen,
+                           const float *mean, const float *istd, const float *lo, const float *hi,
+                           int ldx, void *x_bf16, float *xraw, int *out_of_envelope, void *stream);
+/* training targets t [ncol][ldt] FP32 from heating rates (ncol, nlay),
+ * normalised when mean/istd are given */
+int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw,
+                          const float *mean, const float *istd, int ldt, float *t, void *stream);
 
 /* Kernel timing (no reference counterpart).  With timing on, each call
  * records HIP events on its stream around its three kernels (prep, band,

@@ -6,6 +6,7 @@
 #include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 
 #include "../../include/rrtmg_hip.h"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "tables.hpp"
 
 using namespace rrtmg;
@@ -69,7 +71,7 @@ struct DeviceState {
     LwHost lw;
     SideTables swd, lwd;
     Buffer sw_work, lw_work, sw_stage, lw_stage, zeros;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work;
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
     SwWork sw_w;
     LwWork lw_w;
@@ -234,7 +236,8 @@ const char *rrtmg_hip_last_error(void) { return t_err.c_str(); }
 
 int rrtmg_hip_set_precision(int mode)
 {
-    if (mode != RRTMG_HIP_PREC_FP64 && mode != RRTMG_HIP_PREC_MIXED) return fail(RRTMG_HIP_EINVAL, "bad precision");
+    if (mode != RRTMG_HIP_PREC_FP64 && mode != RRTMG_HIP_PREC_MIXED && mode != RRTMG_HIP_PREC_FP64_SMOOTH)
+        return fail(RRTMG_HIP_EINVAL, "bad precision");
     int prev = g_prec;
     g_prec = mode;
     return prev;
@@ -401,7 +404,7 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
-    hipError_t e = launch_sw(in, out, w, t, mixed, st, s->events(0));
+    hipError_t e = launch_sw(in, out, w, t, g_prec, st, s->events(0));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW kernel launch: ") + hipGetErrorString(e));
     s->sw_w = w;
     s->sw_last_ncol = ncol;
@@ -531,7 +534,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "LW: host-to-device copy failed");
-    hipError_t e = launch_lw(in, out, w, t, mixed, st, s->events(1));
+    hipError_t e = launch_lw(in, out, w, t, g_prec, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
     s->lw_w = w;
     s->lw_last_ncol = ncol;
@@ -634,7 +637,7 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     t.tref = s->swd.tref;
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
-    hipError_t e = launch_sw_vjp(in, cot, g, w, t, mixed, st, s->events(2));
+    hipError_t e = launch_sw_vjp(in, cot, g, w, t, g_prec, st, s->events(2));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward launch: ") + hipGetErrorString(e));
     if (host) {
         sg.back(tlay_bar, g.tlay, n2);
@@ -736,7 +739,7 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     t.cst = s->lwd.cst;
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
-    hipError_t e = launch_lw_vjp(in, cot, g, w, t, mixed, st, s->events(3));
+    hipError_t e = launch_lw_vjp(in, cot, g, w, t, g_prec, st, s->events(3));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
     if (host) {
         sg.back(tlay_bar, g.tlay, n2);
@@ -748,6 +751,137 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
         if (e != hipSuccess || sg.ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+// ---------------------------------------------------------------------------
+// MLP surrogate kernels (device pointers, enqueued on `stream`)
+
+int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA, const void *B, int ldb, int transB,
+                        const float *bias, int relu, float *out_pre, float *out_f32, void *out_bf16, int ldc,
+                        int accumulate, void *stream)
+{
+    if (M <= 0 || N <= 0 || K <= 0) return fail(RRTMG_HIP_EINVAL, "gemm: sizes must be positive");
+    if (!A || !B) return fail(RRTMG_HIP_EINVAL, "gemm: null operand");
+    if (!out_pre && !out_f32 && !out_bf16) return fail(RRTMG_HIP_EINVAL, "gemm: no output");
+    if (lda < (transA ? M : K) || ldb < (transB ? N : K) || ldc < N)
+        return fail(RRTMG_HIP_EINVAL, "gemm: leading dimension smaller than the matrix");
+    GemmEpilogue ep;
+    ep.bias = bias;
+    ep.relu = relu;
+    ep.out_pre = out_pre;
+    ep.out_f32 = out_f32;
+    ep.out_bf16 = (uint16_t *)out_bf16;
+    ep.ldc = ldc;
+    ep.accumulate = accumulate;
+    hipError_t e = launch_gemm_bf16(M, N, K, (const uint16_t *)A, lda, transA, (const uint16_t *)B, ldb, transB, ep,
+                                    (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_relu_backward(long n, const float *dy, const float *pre, void *dz_bf16, float *dz_f32, void *stream)
+{
+    if (n <= 0 || !dy || !pre || !dz_bf16) return fail(RRTMG_HIP_EINVAL, "relu_backward: bad arguments");
+    hipError_t e = launch_relu_backward((size_t)n, dy, pre, (uint16_t *)dz_bf16, dz_f32, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_col_sum(int M, int N, const float *x, int ldx, float *out, void *stream)
+{
+    if (M <= 0 || N <= 0 || ldx < N || !x || !out) return fail(RRTMG_HIP_EINVAL, "col_sum: bad arguments");
+    hipError_t e = launch_col_sum(M, N, x, ldx, out, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_mse(int M, int N, const float *y, int ldy, const float *t, int ldt, float *dy, void *dy_bf16,
+                      int ldd, float *loss_partial, int nblocks, void *stream)
+{
+    if (M <= 0 || N <= 0 || !y || !t || !loss_partial || nblocks <= 0 || ldy < N || ldt < N ||
+        ((dy || dy_bf16) && ldd < N))
+        return fail(RRTMG_HIP_EINVAL, "mse: bad arguments");
+    hipError_t e = launch_mse(M, N, y, ldy, t, ldt, dy, (uint16_t *)dy_bf16, ldd, loss_partial, nblocks,
+                              (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_adam(long n, float *w, float *m, float *v, const float *g, void *w_bf16, float lr, float beta1,
+                       float beta2, float eps, float weight_decay, int step, void *stream)
+{
+    if (n <= 0 || !w || !m || !v || !g || step < 1) return fail(RRTMG_HIP_EINVAL, "adam: bad arguments");
+    AdamParams p{lr, beta1, beta2, eps, weight_decay, (float)(1.0 / (1.0 - std::pow((double)beta1, step))),
+                 (float)(1.0 / (1.0 - std::pow((double)beta2, step)))};
+    hipError_t e = launch_adam((size_t)n, w, m, v, g, (uint16_t *)w_bf16, p, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double *h2ovmr, const double *o3vmr,
+                           const double *tsfc, const double *coszen, const float *mean, const float *istd,
+                           const float *lo, const float *hi, int ldx, void *x_bf16, float *xraw, int *out_of_envelope,
+                           void *stream)
+{
+    if (ncol <= 0 || nlay < 2 || ldx < 3 * nlay + 2 || !tlay || !h2ovmr || !o3vmr || !tsfc || !coszen || !x_bf16 ||
+        (!mean) != (!istd) || (!lo) != (!hi))
+        return fail(RRTMG_HIP_EINVAL, "mlp_features: bad arguments");
+    hipError_t e = launch_mlp_features(ncol, nlay, tlay, h2ovmr, o3vmr, tsfc, coszen, mean, istd, lo, hi, ldx,
+                                       (uint16_t *)x_bf16, xraw, out_of_envelope, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw, const float *mean,
+                          const float *istd, int ldt, float *t, void *stream)
+{
+    if (ncol <= 0 || nlay < 2 || ldt < 2 * nlay || !hr_lw || !hr_sw || !t || (!mean) != (!istd))
+        return fail(RRTMG_HIP_EINVAL, "mlp_targets: bad arguments");
+    hipError_t e = launch_mlp_targets(ncol, nlay, hr_lw, hr_sw, mean, istd, ldt, t, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
+}
+
+int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2ovmr, const double *o3vmr,
+                        const double *tsfc, const double *coszen, const RrtmgHipMlp *net, double *hr_lw,
+                        double *hr_sw, int *out_of_envelope, void *stream)
+{
+    if (ncol <= 0 || !net || !hr_lw || !hr_sw || !tlay || !h2ovmr || !o3vmr || !tsfc || !coszen)
+        return fail(RRTMG_HIP_EINVAL, "surrogate: bad arguments");
+    if (nlay != net->nlay || net->nin != 3 * nlay + 2 || net->nout != 2 * nlay || net->nlayers < 1 ||
+        net->nlayers > RRTMG_HIP_MLP_MAX_LAYERS)
+        return fail(RRTMG_HIP_EINVAL, "surrogate: network shape does not match nlay");
+    int width[RRTMG_HIP_MLP_MAX_LAYERS + 1];
+    width[0] = net->ldx;
+    int wmax = net->ldx;
+    for (int l = 0; l < net->nlayers; ++l) {
+        width[l + 1] = net->width[l];
+        if (!net->w[l] || !net->b[l] || net->width[l] <= 0) return fail(RRTMG_HIP_EINVAL, "surrogate: missing layer");
+        wmax = std::max(wmax, net->width[l]);
+    }
+    if (width[net->nlayers] < net->nout) return fail(RRTMG_HIP_EINVAL, "surrogate: output layer too narrow");
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    const size_t act = align_up((size_t)ncol * wmax * sizeof(uint16_t));
+    const size_t outb = align_up((size_t)ncol * width[net->nlayers] * sizeof(float));
+    int rc;
+    if ((rc = s->mlp_work.ensure(2 * act + outb))) return fail(rc, "surrogate workspace allocation failed");
+    char *wb = (char *)s->mlp_work.p;
+    uint16_t *buf[2] = {(uint16_t *)wb, (uint16_t *)(wb + act)};
+    float *y = (float *)(wb + 2 * act);
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = launch_mlp_features(ncol, nlay, tlay, h2ovmr, o3vmr, tsfc, coszen, net->in_mean, net->in_istd,
+                                       net->in_lo, net->in_hi, net->ldx, buf[0], nullptr, out_of_envelope, st);
+    int cur = 0;
+    for (int l = 0; l < net->nlayers && e == hipSuccess; ++l) {
+        GemmEpilogue ep;
+        ep.bias = net->b[l];
+        const bool last = l == net->nlayers - 1;
+        ep.relu = !last;
+        ep.ldc = width[l + 1];
+        if (last) ep.out_f32 = y;
+        else ep.out_bf16 = buf[1 - cur];
+        e = launch_gemm_bf16(ncol, width[l + 1], width[l], buf[cur], width[l], 0, (const uint16_t *)net->w[l], width[l],
+                             0, ep, st);
+        cur = 1 - cur;
+    }
+    if (e == hipSuccess)
+        e = launch_mlp_decode(ncol, nlay, y, width[net->nlayers], net->out_mean, net->out_std, hr_lw, hr_sw, st);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("surrogate: ") + hipGetErrorString(e));
 }
 
 int rrtmg_hip_set_timing(int on)

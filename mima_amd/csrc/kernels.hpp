@@ -96,16 +96,18 @@ inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * 
 inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)5 * (nlay + 1) * 64 + 4 * 64); }
 
 hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                         const LwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev);
+                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
 hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                         const SwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev);
+                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
 
-// mixed = 0: FP64 everywhere; 1: FP32 with FP64 islands.  ev: NULL, or four
+// mode (the C ABI's precision): 0 FP64 everywhere; 1 FP32 with FP64 islands;
+// 2 FP64 with exp(-x) and the tau transition function evaluated directly
+// instead of read from the 10000-point tables (smooth in the inputs).  ev: NULL, or four
 // events recorded on `s` before prep, after prep, after the band kernel and
 // after finish.
 hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t,
-                     int mixed, hipStream_t s, hipEvent_t *ev);
+                     int mode, hipStream_t s, hipEvent_t *ev);
 hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t,
-                     int mixed, hipStream_t s, hipEvent_t *ev);
+                     int mode, hipStream_t s, hipEvent_t *ev);
 
 }  // namespace rrtmg

@@ -171,7 +171,8 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
     Real *s_red = smem + (size_t)2 * nlay * 64;    // [64]
     const Real *coef = (const Real *)w.coef;
     const Real *tlevf = (const Real *)w.tlev;
-    const Real *exp_tbl = tab + exp_off, *tfn_tbl = tab + tfn_off;
+    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
+    const Real *tfn_tbl = tfn_off >= 0 ? tab + tfn_off : nullptr;
     const Real *plk = tab + plk_off + (BAND - 1) * 181;  // totplnk(:, BAND)
     LTab<Real> T{tab, NG, g};
     Real *bf_up = (Real *)w.bflux + (size_t)(BAND - 1) * 2 * (nlay + 1) * ncol;
@@ -217,15 +218,20 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         Real odepth = secdiff * taug;
         if (odepth < (Real)0) odepth = (Real)0;
         Real atrans, bbd, bbugas;
-        if (odepth <= (Real)0.06) {
+        if (!exp_tbl) {  // smooth mode: the tabulated functions at every depth
+            const LwRT<Real> rt = lw_rt_exact(odepth, false);
+            atrans = rt.atrans;
+            bbd = plfrac * (blay + rt.tf * dplankdn);
+            bbugas = plfrac * (blay + rt.tf * dplankup);
+        } else if (odepth <= (Real)0.06) {
             atrans = odepth - (Real)0.5 * odepth * odepth;
             odepth = (Real)0.166667 * odepth;
             bbd = plfrac * (blay + dplankdn * odepth);
             bbugas = plfrac * (blay + dplankup * odepth);
         } else {
-            int itr = tbl_index(odepth);
+            const int itr = tbl_index(odepth);
             atrans = (Real)1 - exp_tbl[itr];
-            Real tausfac = tfn_tbl[itr];
+            const Real tausfac = tfn_tbl[itr];
             bbd = plfrac * (blay + tausfac * dplankdn);
             bbugas = plfrac * (blay + tausfac * dplankup);
         }
@@ -315,7 +321,7 @@ __global__ __launch_bounds__(256) void lw_finish_kernel(ColumnInputs in, LwWork 
 
 template <typename Real>
 static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t,
-                              double heatfac, hipStream_t s, hipEvent_t *ev)
+                              double heatfac, bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -325,7 +331,8 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     if (ev) (void)hipEventRecord(ev[1], s);
     const int nblocks = ntiles * t.desc.nwaves;
     hipLaunchKernelGGL(lw_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk, (const Real *)t.tab, t.cst,
+                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
+                       t.desc.totplnk, (const Real *)t.tab, t.cst,
                        t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, heatfac);
@@ -333,11 +340,11 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     return hipGetLastError();
 }
 
-hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t, int mixed,
+hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t, int mode,
                      hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_lw_t<float>(in, out, w, t, t.heatfac, s, ev)
-                 : launch_lw_t<double>(in, out, w, t, t.heatfac, s, ev);
+    return mode == 1 ? launch_lw_t<float>(in, out, w, t, t.heatfac, false, s, ev)
+                     : launch_lw_t<double>(in, out, w, t, t.heatfac, mode == 2, s, ev);
 }
 
 }  // namespace rrtmg

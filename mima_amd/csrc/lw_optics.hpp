@@ -151,6 +151,36 @@ __device__ __forceinline__ V lw_adjcol(V col, V coldry, S chiref, S thresh, S ba
     return col;
 }
 
+// Transmittance 1 - exp(-tau) and the tau transition function
+// tfn(tau) = 1 - 2 (1/tau - e/(1 - e)), e = exp(-tau) (what the 10000-point
+// tables of rrtmg_lw_init.f90:106-123 tabulate), evaluated directly for the
+// smooth FP64 mode, with their derivatives.  Below tau = 0.1 tfn and its
+// derivative use the series tau/6 - tau^3/360 + tau^5/15120 (from
+// 1/(e^tau - 1) = 1/tau - 1/2 + tau/12 - tau^3/720 + tau^5/30240 - ...),
+// whose omitted term is below 1e-11 there.
+template <typename Real>
+struct LwRT {
+    Real atrans, tf, datr, dtf;
+};
+template <typename Real>
+__device__ __forceinline__ LwRT<Real> lw_rt_exact(Real od, bool derivs)
+{
+    LwRT<Real> r;
+    const Real e = exp(-od);
+    r.atrans = -expm1(-od);
+    r.datr = e;
+    if (od < (Real)0.1) {
+        const Real o2 = od * od;
+        r.tf = od * ((Real)1 / (Real)6 - o2 * ((Real)1 / (Real)360 - o2 / (Real)15120));
+        r.dtf = (Real)1 / (Real)6 - o2 * ((Real)1 / (Real)120 - o2 / (Real)3024);
+    } else {
+        const Real ome = -expm1(-od);
+        r.tf = (Real)1 - (Real)2 * ((Real)1 / od - e / ome);
+        r.dtf = derivs ? (Real)2 / (od * od) - (Real)2 * e / (ome * ome) : (Real)0;
+    }
+    return r;
+}
+
 #define CHI(i, j) cst->chi_mls[((j)-1) * 7 + ((i)-1)]
 
 // Optical depth and Planck fraction of one (layer, g) of band BAND,

@@ -38,39 +38,49 @@ namespace rrtmg {
 using D3 = Dual<double, 3>;
 
 // value (V = Real) or value and three derivatives (V = Dual<Real, 3>) of one
-// field of a [field][4][lay][col] array; cs = stride between the components
-template <typename V, typename Real>
+// field of a [field][4][lay][col] array; cs = stride between the components.
+// MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3) names the derivatives the field can
+// have; the others are known zeros and are not loaded, so the dual arithmetic
+// on them folds away at compile time.
+enum : unsigned { DT = 1u, DQ = 2u, DO = 4u };
+template <typename V, typename Real, unsigned MASK>
 struct DLoad;
-template <typename Real>
-struct DLoad<Real, Real> {
+template <typename Real, unsigned MASK>
+struct DLoad<Real, Real, MASK> {
     __device__ static Real get(const Real *__restrict__ p, size_t) { return p[0]; }
 };
-template <typename Real>
-struct DLoad<Dual<Real, 3>, Real> {
+template <typename Real, unsigned MASK>
+struct DLoad<Dual<Real, 3>, Real, MASK> {
     __device__ static Dual<Real, 3> get(const Real *__restrict__ p, size_t cs)
     {
         Dual<Real, 3> r;
         r.v = p[0];
-        r.d[0] = p[cs];
-        r.d[1] = p[2 * cs];
-        r.d[2] = p[3 * cs];
+        r.d[0] = (MASK & DT) ? p[cs] : (Real)0;
+        r.d[1] = (MASK & DQ) ? p[2 * cs] : (Real)0;
+        r.d[2] = (MASK & DO) ? p[3 * cs] : (Real)0;
         return r;
     }
 };
 
+// Which inputs each coefficient depends on (the prep's formulas): fac*, the
+// interpolation fractions and the pressure/temperature scalings on T; column
+// amounts on q through the dry-air column (and O3's on O3); the broadening
+// column on q and O3 (it excludes the listed gases); selffac/forfac (stored
+// multiplied by the water column) on T and q.
 template <typename V, typename Real>
 __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__restrict__ dco,
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
     const size_t fs = (size_t)nlay * ncol, ix = (size_t)lay * ncol + col;
-#define LDD(f, F) c.f = DLoad<V, Real>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
-    LDD(fac00, LWF_FAC00); LDD(fac01, LWF_FAC01); LDD(fac10, LWF_FAC10); LDD(fac11, LWF_FAC11);
-    LDD(colh2o, LWF_COLH2O); LDD(colco2, LWF_COLCO2); LDD(colo3, LWF_COLO3); LDD(coln2o, LWF_COLN2O);
-    LDD(colco, LWF_COLCO); LDD(colch4, LWF_COLCH4); LDD(colo2, LWF_COLO2); LDD(colbrd, LWF_COLBRD);
-    LDD(selffac, LWF_SELFFAC); LDD(selffrac, LWF_SELFFRAC); LDD(forfac, LWF_FORFAC); LDD(forfrac, LWF_FORFRAC);
-    LDD(minorfrac, LWF_MINORFRAC); LDD(scaleminor, LWF_SCALEMINOR); LDD(scaleminorn2, LWF_SCALEMINORN2);
-    LDD(coldry, LWF_COLDRY); LDD(pavel, LWF_PAVEL); LDD(wx1, LWF_WX1); LDD(wx2, LWF_WX2); LDD(wx3, LWF_WX3);
-    LDD(wx4, LWF_WX4); LDD(tlayfrac, LWF_TLAYFRAC);
+#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
+    LDD(fac00, LWF_FAC00, DT); LDD(fac01, LWF_FAC01, DT); LDD(fac10, LWF_FAC10, DT); LDD(fac11, LWF_FAC11, DT);
+    LDD(colh2o, LWF_COLH2O, DQ); LDD(colco2, LWF_COLCO2, DQ); LDD(colo3, LWF_COLO3, DQ | DO);
+    LDD(coln2o, LWF_COLN2O, DQ); LDD(colco, LWF_COLCO, DQ); LDD(colch4, LWF_COLCH4, DQ); LDD(colo2, LWF_COLO2, DQ);
+    LDD(colbrd, LWF_COLBRD, DQ | DO); LDD(selffac, LWF_SELFFAC, DT | DQ); LDD(selffrac, LWF_SELFFRAC, DT);
+    LDD(forfac, LWF_FORFAC, DT | DQ); LDD(forfrac, LWF_FORFRAC, DT); LDD(minorfrac, LWF_MINORFRAC, DT);
+    LDD(scaleminor, LWF_SCALEMINOR, DT); LDD(scaleminorn2, LWF_SCALEMINORN2, DT | DQ | DO);
+    LDD(coldry, LWF_COLDRY, DQ); LDD(pavel, LWF_PAVEL, 0u); LDD(wx1, LWF_WX1, DQ); LDD(wx2, LWF_WX2, DQ);
+    LDD(wx3, LWF_WX3, DQ); LDD(wx4, LWF_WX4, DQ); LDD(tlayfrac, LWF_TLAYFRAC, DT);
 #undef LDD
     const int w = ipk[ix];
     c.jp = unpack_jp(w);
@@ -259,7 +269,9 @@ __device__ __forceinline__ void seg_sum_n(Real (&v)[N], int lane, int ng, Real *
 }
 
 // transmittance and sources of one layer (rrtmg_lw_rtrnmr.f90:591-606) and,
-// for the backward, their derivatives with respect to the optical depth
+// for the backward, their derivatives with respect to the optical depth: the
+// reference's thin-layer series below 0.06 and the tables above, or (smooth
+// mode, tables NULL) the tabulated functions themselves at every depth
 template <typename Real>
 struct LwLayerRT {
     Real od, atrans, tf, datr, dtf;
@@ -272,7 +284,13 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
     Real od = secdiff * taug;
     if (od < (Real)0) od = (Real)0;
     r.od = od;
-    if (od <= (Real)0.06) {
+    if (!exp_tbl) {
+        const LwRT<Real> x = lw_rt_exact(od, derivs);
+        r.atrans = x.atrans;
+        r.tf = x.tf;
+        r.datr = x.datr;
+        r.dtf = x.dtf;
+    } else if (od <= (Real)0.06) {
         r.atrans = od - (Real)0.5 * od * od;
         r.tf = (Real)0.166667 * od;
         r.datr = (Real)1 - od;
@@ -293,7 +311,7 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
 }
 
 template <int BAND, typename Real>
-__device__ void lw_vjp_band_body(const ColumnInputs &in, const LwVjpWork &w, const LwBand &B,
+__device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const LwVjpWork &w, const LwBand &B,
                                  const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
                                  const LwConst *__restrict__ cst, WaveUnit u, int tile, Real *smem)
 {
@@ -311,7 +329,8 @@ __device__ void lw_vjp_band_body(const ColumnInputs &in, const LwVjpWork &w, con
     Real *red = smem + (size_t)3 * nlay * 64; // [4][64]
     const Real *dco = (const Real *)w.dcoef;
     const Real *tlevf = (const Real *)w.tlev;
-    const Real *exp_tbl = tab + exp_off, *tfn_tbl = tab + tfn_off;
+    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
+    const Real *tfn_tbl = tfn_off >= 0 ? tab + tfn_off : nullptr;
     const Real *plk = tab + plk_off + (BAND - 1) * 181;
     LTab<Real> T{tab, NG, g};
     const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
@@ -542,7 +561,7 @@ __global__ __launch_bounds__(256) void lw_vjp_finish_kernel(ColumnInputs in, LwV
 
 template <typename Real>
 static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                                  const LwDevTables &t, hipStream_t s, hipEvent_t *ev)
+                                  const LwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -552,7 +571,8 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
     const int nblocks = ntiles * t.desc.nwaves;
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(lw_vjp_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk, (const Real *)t.tab,
+                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
+                       t.desc.totplnk, (const Real *)t.tab,
                        t.cst, t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
@@ -561,10 +581,10 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
 }
 
 hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                         const LwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev)
+                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_lw_vjp_t<float>(in, cot, out, w, t, s, ev)
-                 : launch_lw_vjp_t<double>(in, cot, out, w, t, s, ev);
+    return mode == 1 ? launch_lw_vjp_t<float>(in, cot, out, w, t, false, s, ev)
+                     : launch_lw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev);
 }
 
 }  // namespace rrtmg

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     Real *s_red = smem + (size_t)2 * (nlay + 1) * 64;  // [64]
     const Real *coef = (const Real *)w.coef;
     Real *bflux = (Real *)w.bflux;
-    const Real *exp_tbl = tab + exp_off;
+    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     Tab<Real> T{tab, ng, g};
 
     Real prmu0 = (Real)1, zincflx = (Real)0, albp = (Real)0, albd = (Real)0;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void sw_finish_kernel(ColumnInputs in, SwWork 
 
 template <typename Real>
 static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t,
-                              hipStream_t s, hipEvent_t *ev)
+                              bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -289,17 +289,18 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
     if (ev) (void)hipEventRecord(ev[1], s);
     const int nblocks = ntiles * t.desc.nwaves;
     hipLaunchKernelGGL(sw_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
-hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t, int mixed,
+hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t, int mode,
                      hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_sw_t<float>(in, out, w, t, s, ev) : launch_sw_t<double>(in, out, w, t, s, ev);
+    return mode == 1 ? launch_sw_t<float>(in, out, w, t, false, s, ev)
+                     : launch_sw_t<double>(in, out, w, t, mode == 2, s, ev);
 }
 
 }  // namespace rrtmg

@@ -288,18 +288,20 @@ __device__ V sw_sfluxzen_v(int jb, const SwBand &B, const Tab<Real> &T, const Sw
     }
 }
 
-// exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5); for a dual
-// argument the derivative is that of exp(-x) itself
+// exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5), or exp(-x)
+// itself when tbl is NULL (the smooth FP64 mode, which also drops the
+// small-argument series 1 - x + x^2/2 that the table path uses below 0.06);
+// for a dual argument the derivative is that of exp(-x)
 template <typename F, typename Real>
 __device__ __forceinline__ F tbl_exp(F x, const Real *__restrict__ tbl)
 {
-    return (F)tbl[tbl_index(x)];
+    return tbl ? (F)tbl[tbl_index(x)] : exp(-x);
 }
 template <typename F, int N, typename Real>
 __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *__restrict__ tbl)
 {
     Dual<F, N> r;
-    r.v = (F)tbl[tbl_index(x.v)];
+    r.v = tbl ? (F)tbl[tbl_index(x.v)] : exp(-x.v);
     const F s = -exp(-x.v);
     for (int i = 0; i < N; ++i) r.d[i] = s * x.d[i];
     return r;
@@ -323,7 +325,7 @@ __device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__rest
         F za1 = za - zgamma3;
         F zgt = zgamma1 * zto1;
         F ze1 = fmin(zto1 / prmuz, (F)500);
-        F ze2 = ze1 <= od_lo ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+        F ze2 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
         if (ze2 == (F)1) {
             pref = (F)0;
             ptra = (F)1;
@@ -355,9 +357,9 @@ __device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__rest
         F zbeta = (zgamma1 - zrk) / zrkg;
         F ze1 = fmin(zrk * zto1, (F)500);
         F ze2 = fmin(zto1 / prmuz, (F)500);
-        F zem1 = ze1 <= od_lo ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+        F zem1 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
         F zep1 = (F)1 / zem1;
-        F zem2 = ze2 <= od_lo ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
+        F zem2 = (exp_tbl && ze2 <= od_lo) ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
         F zep2 = (F)1 / zem2;
         F zdenr = zr4 * zep1 + zr5 * zem1;
         F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
@@ -408,7 +410,7 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
         reftra<V, Real, Real>(tau, w, prmu0, exp_tbl, r, rd, t, td);
     }
     V ze1 = tau / prmu0;
-    dbt = ze1 <= (Real)0.06 ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+    dbt = (exp_tbl && ze1 <= (Real)0.06) ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
 }
 
 }  // namespace rrtmg

@@ -30,21 +30,25 @@ namespace rrtmg {
 
 using D3 = Dual<double, 3>;
 
-template <typename V, typename Real>
+// value (V = Real) or value and three derivatives (V = Dual<Real, 3>) of one
+// field; MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3) names the derivatives the field
+// can have, the others are known zeros (not loaded; folded at compile time)
+enum : unsigned { SDT = 1u, SDQ = 2u, SDO = 4u };
+template <typename V, typename Real, unsigned MASK>
 struct SwDLoad;
-template <typename Real>
-struct SwDLoad<Real, Real> {
+template <typename Real, unsigned MASK>
+struct SwDLoad<Real, Real, MASK> {
     __device__ static Real get(const Real *__restrict__ p, size_t) { return p[0]; }
 };
-template <typename Real>
-struct SwDLoad<Dual<Real, 3>, Real> {
+template <typename Real, unsigned MASK>
+struct SwDLoad<Dual<Real, 3>, Real, MASK> {
     __device__ static Dual<Real, 3> get(const Real *__restrict__ p, size_t cs)
     {
         Dual<Real, 3> r;
         r.v = p[0];
-        r.d[0] = p[cs];
-        r.d[1] = p[2 * cs];
-        r.d[2] = p[3 * cs];
+        r.d[0] = (MASK & SDT) ? p[cs] : (Real)0;
+        r.d[1] = (MASK & SDQ) ? p[2 * cs] : (Real)0;
+        r.d[2] = (MASK & SDO) ? p[3 * cs] : (Real)0;
         return r;
     }
 };
@@ -54,11 +58,12 @@ __device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__res
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
     const size_t fs = (size_t)nlay * ncol, ix = (size_t)lay * ncol + col;
-#define LDD(f, F) c.f = SwDLoad<V, Real>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
-    LDD(fac00, SWF_FAC00); LDD(fac01, SWF_FAC01); LDD(fac10, SWF_FAC10); LDD(fac11, SWF_FAC11);
-    LDD(colh2o, SWF_COLH2O); LDD(colco2, SWF_COLCO2); LDD(colo3, SWF_COLO3); LDD(colch4, SWF_COLCH4);
-    LDD(colo2, SWF_COLO2); LDD(colmol, SWF_COLMOL); LDD(selffac, SWF_SELFFAC); LDD(selffrac, SWF_SELFFRAC);
-    LDD(forfac, SWF_FORFAC); LDD(forfrac, SWF_FORFRAC);
+#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
+    LDD(fac00, SWF_FAC00, SDT); LDD(fac01, SWF_FAC01, SDT); LDD(fac10, SWF_FAC10, SDT); LDD(fac11, SWF_FAC11, SDT);
+    LDD(colh2o, SWF_COLH2O, SDQ); LDD(colco2, SWF_COLCO2, SDQ); LDD(colo3, SWF_COLO3, SDQ | SDO);
+    LDD(colch4, SWF_COLCH4, SDQ); LDD(colo2, SWF_COLO2, SDQ); LDD(colmol, SWF_COLMOL, SDQ);
+    LDD(selffac, SWF_SELFFAC, SDT | SDQ); LDD(selffrac, SWF_SELFFRAC, SDT); LDD(forfac, SWF_FORFAC, SDT | SDQ);
+    LDD(forfrac, SWF_FORFRAC, SDT);
 #undef LDD
     const int w = ipk[ix];
     c.jp = unpack_jp(w);
@@ -240,7 +245,7 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     Real *s_ptdbt = smem + 2 * L1, *s_ztdn = smem + 3 * L1, *s_prdnd = smem + 4 * L1;  // downward sweep
     Real *red = smem + 5 * L1;                                   // [4][64]
     const Real *dco = (const Real *)w.dcoef;
-    const Real *exp_tbl = tab + exp_off;
+    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     Tab<Real> T{tab, ng, g};
     const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
     Real *gpart = (Real *)w.gpart + (size_t)u.band * 3 * n2;
@@ -442,7 +447,7 @@ __global__ __launch_bounds__(256) void sw_vjp_finish_kernel(ColumnInputs in, SwV
 
 template <typename Real>
 static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                                  const SwDevTables &t, hipStream_t s, hipEvent_t *ev)
+                                  const SwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -452,7 +457,7 @@ static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
     const int nblocks = ntiles * t.desc.nwaves;
     if (ev) (void)hipEventRecord(ev[1], s);
     hipLaunchKernelGGL(sw_vjp_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
     if (ev) (void)hipEventRecord(ev[3], s);
@@ -460,10 +465,10 @@ static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
 }
 
 hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                         const SwDevTables &t, int mixed, hipStream_t s, hipEvent_t *ev)
+                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev)
 {
-    return mixed ? launch_sw_vjp_t<float>(in, cot, out, w, t, s, ev)
-                 : launch_sw_vjp_t<double>(in, cot, out, w, t, s, ev);
+    return mode == 1 ? launch_sw_vjp_t<float>(in, cot, out, w, t, false, s, ev)
+                     : launch_sw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev);
 }
 
 }  // namespace rrtmg

@@ -88,6 +88,36 @@ module rrtmg_hip_capi
       real(c_double), intent(out) :: uflx(*), dflx(*), hr(*), uflxc(*), dflxc(*), hrc(*)
       type(c_ptr), value :: stream
     end function
+    integer(c_int) function c_lw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                                     h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                                     cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &
+                                     uflx_bar, dflx_bar, hr_bar, tlay_bar, tlev_bar, h2ovmr_bar, &
+                                     o3vmr_bar, tsfc_bar, ptr_kind, stream) bind(C, name="rrtmg_hip_lw_vjp")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: ncol, nlay, ptr_kind
+      real(c_double), intent(in) :: play(*), plev(*), tlay(*), tlev(*), tsfc(*)
+      real(c_double), intent(in) :: h2ovmr(*), o3vmr(*), co2vmr(*), ch4vmr(*), n2ovmr(*), o2vmr(*)
+      real(c_double), intent(in) :: cfc11vmr(*), cfc12vmr(*), cfc22vmr(*), ccl4vmr(*), emis(*)
+      type(c_ptr), value :: tauaer
+      real(c_double), intent(in) :: uflx_bar(*), dflx_bar(*), hr_bar(*)
+      real(c_double), intent(out) :: tlay_bar(*), tlev_bar(*), h2ovmr_bar(*), o3vmr_bar(*), tsfc_bar(*)
+      type(c_ptr), value :: stream
+    end function
+    integer(c_int) function c_sw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                                     h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                                     asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                                     swuflx_bar, swdflx_bar, swhr_bar, tlay_bar, h2ovmr_bar, o3vmr_bar, &
+                                     ptr_kind, stream) bind(C, name="rrtmg_hip_sw_vjp")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: ncol, nlay, dyofyr, ptr_kind
+      real(c_double), intent(in) :: play(*), plev(*), tlay(*), tlev(*), tsfc(*)
+      real(c_double), intent(in) :: h2ovmr(*), o3vmr(*), co2vmr(*), ch4vmr(*), n2ovmr(*), o2vmr(*)
+      real(c_double), intent(in) :: asdir(*), asdif(*), aldir(*), aldif(*), coszen(*)
+      real(c_double), value :: adjes, scon
+      real(c_double), intent(in) :: swuflx_bar(*), swdflx_bar(*), swhr_bar(*)
+      real(c_double), intent(out) :: tlay_bar(*), h2ovmr_bar(*), o3vmr_bar(*)
+      type(c_ptr), value :: stream
+    end function
   end interface
 contains
   subroutine check(rc, where)
@@ -211,3 +241,56 @@ contains
                     swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_sw')
   end subroutine rrtmg_sw
 end module rrtmg_sw_rad
+
+! Adjoint entry points (no counterpart in the reference): the backward of
+! rrtmg_lw / rrtmg_sw, for Fortran callers of the same arrays.  Given the
+! cotangents of the outputs (uflx, dflx, hr or swuflx, swdflx, swhr), they
+! return the gradients of sum(cotangent * output) with respect to the layer
+! temperature, water vapour and ozone (and, for LW, level and surface
+! temperature).  Clear sky without aerosols, as MiMA calls rrtmg_lw/rrtmg_sw.
+module rrtmg_adjoint
+  use parkind, only : im => kind_im, rb => kind_rb
+  use parrrtm, only : nbndlw
+  implicit none
+contains
+  subroutine rrtmg_lw_adj(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                          h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                          cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, &
+                          uflx_bar, dflx_bar, hr_bar, &
+                          tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar)
+    use iso_c_binding
+    use rrtmg_hip_capi
+    integer(kind=im), intent(in) :: ncol, nlay
+    real(kind=rb), intent(in) :: play(:,:), plev(:,:), tlay(:,:), tlev(:,:), tsfc(:)
+    real(kind=rb), intent(in) :: h2ovmr(:,:), o3vmr(:,:), co2vmr(:,:), ch4vmr(:,:), n2ovmr(:,:), o2vmr(:,:)
+    real(kind=rb), intent(in) :: cfc11vmr(:,:), cfc12vmr(:,:), cfc22vmr(:,:), ccl4vmr(:,:), emis(:,:)
+    real(kind=rb), intent(in) :: uflx_bar(:,:), dflx_bar(:,:), hr_bar(:,:)
+    real(kind=rb), intent(out) :: tlay_bar(:,:), tlev_bar(:,:), h2ovmr_bar(:,:), o3vmr_bar(:,:), tsfc_bar(:)
+    call check(c_lw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                        h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                        cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, c_null_ptr, &
+                        uflx_bar, dflx_bar, hr_bar, tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar, &
+                        RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_lw_adj')
+  end subroutine rrtmg_lw_adj
+
+  subroutine rrtmg_sw_adj(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                          h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                          asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                          swuflx_bar, swdflx_bar, swhr_bar, &
+                          tlay_bar, h2ovmr_bar, o3vmr_bar)
+    use iso_c_binding
+    use rrtmg_hip_capi
+    integer(kind=im), intent(in) :: ncol, nlay, dyofyr
+    real(kind=rb), intent(in) :: play(:,:), plev(:,:), tlay(:,:), tlev(:,:), tsfc(:)
+    real(kind=rb), intent(in) :: h2ovmr(:,:), o3vmr(:,:), co2vmr(:,:), ch4vmr(:,:), n2ovmr(:,:), o2vmr(:,:)
+    real(kind=rb), intent(in) :: asdir(:), asdif(:), aldir(:), aldif(:), coszen(:)
+    real(kind=rb), intent(in) :: adjes, scon
+    real(kind=rb), intent(in) :: swuflx_bar(:,:), swdflx_bar(:,:), swhr_bar(:,:)
+    real(kind=rb), intent(out) :: tlay_bar(:,:), h2ovmr_bar(:,:), o3vmr_bar(:,:)
+    call check(c_sw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, &
+                        h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                        asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                        swuflx_bar, swdflx_bar, swhr_bar, tlay_bar, h2ovmr_bar, o3vmr_bar, &
+                        RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_sw_adj')
+  end subroutine rrtmg_sw_adj
+end module rrtmg_adjoint

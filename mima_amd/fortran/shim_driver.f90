@@ -8,7 +8,11 @@
 !   input  (stream): int32 ncol, nlay; then float64 arrays, top-first
 !          (ncol, nlay) or (ncol, nlay+1): play, plev, tlay, tlev; h2o, o3, co2;
 !          (ncol): tsfc, albedo, coszen; scalars solrad(adjes), solr_cnst(scon)
-!   output (stream): swuflx, swdflx, swhr, uflx, dflx, hr, surface-first
+!   output (stream): swuflx, swdflx, swhr, uflx, dflx, hr, surface-first; then
+!          the adjoint entry points' gradients for cotangents hr_bar = swhr_bar =
+!          1 and uflx_bar = dflx_bar = swuflx_bar = swdflx_bar = 0:
+!          LW tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar; SW tlay_bar,
+!          h2ovmr_bar, o3vmr_bar
 program shim_driver
   use parkind, only : im => kind_im, rb => kind_rb
   use parrrtm, only : nbndlw
@@ -17,6 +21,7 @@ program shim_driver
   use rrtmg_sw_init, only : rrtmg_sw_ini
   use rrtmg_lw_rad, only : rrtmg_lw
   use rrtmg_sw_rad, only : rrtmg_sw
+  use rrtmg_adjoint, only : rrtmg_lw_adj, rrtmg_sw_adj
   implicit none
   character(len=512) :: fin, fout
   integer :: u, ncol, nlay, sk
@@ -26,6 +31,8 @@ program shim_driver
   real(kind=rb), allocatable :: taucld(:,:,:), sw_zro(:,:,:), tauaer(:,:,:), zro_sw(:,:,:)
   real(kind=rb), allocatable :: swuflx(:,:), swdflx(:,:), swhr(:,:), swuflxc(:,:), swdflxc(:,:), swhrc(:,:)
   real(kind=rb), allocatable :: uflx(:,:), dflx(:,:), hr(:,:), uflxc(:,:), dflxc(:,:), hrc(:,:)
+  real(kind=rb), allocatable :: zl(:,:), gtl(:,:), gtv(:,:), gq(:,:), go(:,:), gts(:), sgt(:,:), sgq(:,:), sgo(:,:)
+  real(kind=rb), allocatable :: sf(:,:), sh(:,:), sq(:,:), so(:,:), tcopy(:)
   real(kind=rb) :: solrad, solr_cnst
   real(kind=rb), parameter :: cp_air = 287.04_rb / (2._rb / 7._rb)
 
@@ -81,7 +88,26 @@ program shim_driver
                 tauaer, &
                 uflx, dflx, hr, uflxc, dflxc, hrc)
 
+  ! adjoint entry points on the same surface-first state (contiguous copies)
+  allocate(zl(ncol, nlay + 1), gtl(ncol, nlay), gtv(ncol, nlay + 1), gq(ncol, nlay), go(ncol, nlay), gts(ncol))
+  allocate(sgt(ncol, nlay), sgq(ncol, nlay), sgo(ncol, nlay))
+  allocate(sf(ncol, nlay), sh(ncol, nlay + 1), sq(ncol, nlay), so(ncol, nlay), tcopy(ncol))
+  zl = 0._rb
+  sf = tf(:, sk:1:-1)
+  sh = th(:, sk+1:1:-1)
+  sq = q(:, sk:1:-1)
+  so = o3(:, sk:1:-1)
+  tcopy = tsrf
+  call rrtmg_lw_adj(ncol, nlay, pf(:, sk:1:-1), ph(:, sk+1:1:-1), sf, sh, tcopy, &
+                    sq, so, co2(:, sk:1:-1), zeros, zeros, zeros, zeros, zeros, zeros, zeros, emis, &
+                    zl, zl, ones, gtl, gtv, gq, go, gts)
+  call rrtmg_sw_adj(ncol, nlay, pf(:, sk:1:-1), ph(:, sk+1:1:-1), sf, sh, tcopy, &
+                    sq, so, co2(:, sk:1:-1), zeros, zeros, zeros, &
+                    albedo, albedo, albedo, albedo, cosz, solrad, dyofyr, solr_cnst, &
+                    zl, zl, ones, sgt, sgq, sgo)
+
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) swuflx, swdflx, swhr, uflx, dflx, hr
+  write(u) gtl, gtv, gq, go, gts, sgt, sgq, sgo
   close(u)
 end program shim_driver

@@ -22,7 +22,7 @@ TABLES = os.path.join(HERE, "tables")
 CPDAIR = 287.04 / (2.0 / 7.0)   # MiMA's cp_air = RDGAS/KAPPA (constants.f90:64-67)
 
 HOST, DEVICE = 0, 1
-PREC = {"fp64": 0, "mixed": 1}
+PREC = {"fp64": 0, "mixed": 1, "smooth": 2}
 
 _lib = None
 _D = ctypes.POINTER(ctypes.c_double)
@@ -68,7 +68,8 @@ def rrtmg_lw_ini(cpdair=CPDAIR, table_dir=None):
 
 
 def set_precision(mode):
-    """'fp64' or 'mixed'; returns the previous mode name."""
+    """'fp64', 'mixed' or 'smooth' (FP64 without the lookup tables); returns
+    the previous mode name."""
     prev = lib().rrtmg_hip_set_precision(PREC[mode])
     if prev < 0:
         _check(prev)

@@ -4,10 +4,21 @@ differences of the FP64 GPU forward.
 Workload of the checks: 64 seeded-random columns of the 128 x 64 x 40 grid
 (day and night), cotangents N(0, 1) from a seed; central differences with
 h = 1e-3 K for temperatures and 1e-3 relative for water vapour and ozone.
-The forward's transmittance lookups are tables while the backward
-differentiates the functions they tabulate, so the finite differences carry
-table-quantisation noise; the tolerances below are stated relative to the
-size of the derivative being checked.
+
+The finite differences run on the smooth FP64 mode ('smooth': exp(-x) and
+the tau transition function evaluated at every optical depth instead of read
+from the 10000-point tables or, below 0.06, replaced by the reference's
+thin-layer series), whose derivatives are the ones the backward computes.
+With the tables the forward is piecewise constant in each layer's optical
+depth at the scale of an h = 1e-3 K step (an index change moves a
+transmittance by ~50 times the smooth change), and the series/table switch
+at 0.06 is a jump of ~3.5e-5 in transmittance; either would swamp the check.
+What remains non-smooth in the smooth mode is the reference's piecewise
+linear interpolation (Planck table in 1 K cells, temperature and eta nodes),
+whose kinks a step can straddle: the checks run at the stated step (1e-3 K,
+1e-3 relative) with a 2e-3 tolerance and at 1e-5 with 1e-4.  The table
+mode's backward is then tied to the smooth mode's, and the FP32-mixed
+backward to the FP64 one.
 """
 import numpy as np
 import pytest
@@ -45,29 +56,35 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _perturbed(c, dT=None, dtlev=None, dq=None, do3=None, dts=None, s=1.0):
+def _perturbed(c, dT=None, dtlev=None, dq=None, do3=None, dts=None, s=1.0, h=H_T):
     p = dict(c)
     if dT is not None:
-        p["tlay"] = c["tlay"] + s * H_T * dT
+        p["tlay"] = c["tlay"] + s * h * dT
     if dtlev is not None:
-        p["tlev"] = c["tlev"] + s * H_T * dtlev
+        p["tlev"] = c["tlev"] + s * h * dtlev
     if dq is not None:
-        p["h2ovmr"] = c["h2ovmr"] * (1.0 + s * H_REL * dq)
+        p["h2ovmr"] = c["h2ovmr"] * (1.0 + s * h * dq)
     if do3 is not None:
-        p["o3vmr"] = c["o3vmr"] * (1.0 + s * H_REL * do3)
+        p["o3vmr"] = c["o3vmr"] * (1.0 + s * h * do3)
     if dts is not None:
-        p["tsfc"] = c["tsfc"] + s * H_T * dts
+        p["tsfc"] = c["tsfc"] + s * h * dts
     return p
 
 
+STEPS = [(1e-3, 2e-3), (1e-5, 1e-4)]   # (h in K or relative, tolerance)
+
+
+FD_PREC = "smooth"
+
+
 def _lw_J(rr, c, cot):
-    rr.set_precision("fp64")
+    rr.set_precision(FD_PREC)
     o = rr.lw_columns(_dev(rr, c))
     return sum(float(np.sum(cot[k] * _np(o[k]))) for k in ("uflx", "dflx", "hr"))
 
 
 def _sw_J(rr, c, cot):
-    rr.set_precision("fp64")
+    rr.set_precision(FD_PREC)
     o = rr.sw_columns(_dev(rr, c))
     return sum(float(np.sum(cot[k] * _np(o[k]))) for k in ("swuflx", "swdflx", "swhr"))
 
@@ -117,32 +134,34 @@ def _kw(which, delta):
             "tsfc": {"dts": delta}}[which]
 
 
+@pytest.mark.parametrize("h,tol", STEPS)
 @pytest.mark.parametrize("which", LW_DIRS)
-def test_lw_vjp_directional_derivative_matches_central_differences(rr, which):
+def test_lw_vjp_directional_derivative_matches_central_differences(rr, which, h, tol):
     c = _cols()
     rng = np.random.default_rng(11)
     cot = _lw_cot(c, rng)
     delta = rng.standard_normal(c[which].shape)
-    g = _lw_grad(rr, c, cot)
+    g = _lw_grad(rr, c, cot, FD_PREC)
     ad = _directional(c, g, which, delta)
-    fd = (_lw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0), cot) -
-          _lw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0), cot)) / (2.0 * (H_T if which in ("tlay", "tlev", "tsfc") else H_REL))
-    print("LW %s: ad %.6e fd %.6e rel %.2e" % (which, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
-    assert abs(ad - fd) <= 5e-3 * abs(fd) + 1e-9, (which, ad, fd)
+    fd = (_lw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0, h=h), cot) -
+          _lw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0, h=h), cot)) / (2.0 * h)
+    print("LW %s h=%g: ad %.8e fd %.8e rel %.2e" % (which, h, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
+    assert abs(ad - fd) <= tol * abs(fd) + 1e-9, (which, ad, fd)
 
 
+@pytest.mark.parametrize("h,tol", STEPS)
 @pytest.mark.parametrize("which", SW_DIRS)
-def test_sw_vjp_directional_derivative_matches_central_differences(rr, which):
+def test_sw_vjp_directional_derivative_matches_central_differences(rr, which, h, tol):
     c = _cols()
     rng = np.random.default_rng(12)
     cot = _sw_cot(c, rng)
     delta = rng.standard_normal(c[which].shape)
-    g = _sw_grad(rr, c, cot)
+    g = _sw_grad(rr, c, cot, FD_PREC)
     ad = _directional(c, g, which, delta)
-    fd = (_sw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0), cot) -
-          _sw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0), cot)) / (2.0 * (H_T if which == "tlay" else H_REL))
-    print("SW %s: ad %.6e fd %.6e rel %.2e" % (which, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
-    assert abs(ad - fd) <= 5e-3 * abs(fd) + 1e-9, (which, ad, fd)
+    fd = (_sw_J(rr, _perturbed(c, **_kw(which, delta), s=1.0, h=h), cot) -
+          _sw_J(rr, _perturbed(c, **_kw(which, delta), s=-1.0, h=h), cot)) / (2.0 * h)
+    print("SW %s h=%g: ad %.8e fd %.8e rel %.2e" % (which, h, ad, fd, abs(ad - fd) / max(abs(fd), 1e-30)))
+    assert abs(ad - fd) <= tol * abs(fd) + 1e-9, (which, ad, fd)
 
 
 def test_sw_vjp_zero_on_night_columns(rr):
@@ -170,12 +189,11 @@ def test_heating_jacobian_rows_match_finite_differences(rr, side):
         rep = subset(one, np.zeros(L, dtype=int))
         # backward: row k of J = gradient of hr[k]
         hr_bar = np.eye(L)
+        zl = np.zeros((L + 1, L))
         if side == "lw":
-            zl = np.zeros((L + 1, L))
-            g = _lw_grad(rr, rep, {"uflx": zl, "dflx": zl, "hr": hr_bar})["tlay"]
+            g = _lw_grad(rr, rep, {"uflx": zl, "dflx": zl, "hr": hr_bar}, FD_PREC)["tlay"]
         else:
-            zl = np.zeros((L + 1, L))
-            g = _sw_grad(rr, rep, {"swuflx": zl, "swdflx": zl, "swhr": hr_bar})["tlay"]
+            g = _sw_grad(rr, rep, {"swuflx": zl, "swdflx": zl, "swhr": hr_bar}, FD_PREC)["tlay"]
         J_ad = g.T  # J_ad[k, j] = d hr[k] / d T[j]
         # central differences: column j perturbs T[j]
         rep2 = subset(one, np.zeros(2 * L, dtype=int))
@@ -183,7 +201,7 @@ def test_heating_jacobian_rows_match_finite_differences(rr, side):
         dT[np.arange(L), np.arange(L)] = 1.0
         dT[np.arange(L), L + np.arange(L)] = -1.0
         rep2["tlay"] = rep2["tlay"] + H_T * dT
-        rr.set_precision("fp64")
+        rr.set_precision(FD_PREC)
         d2 = rr.to_device(rep2)
         hr = _np((rr.lw_columns(d2) if side == "lw" else rr.sw_columns(d2))["hr" if side == "lw" else "swhr"])
         J_fd = (hr[:, :L] - hr[:, L:]) / (2.0 * H_T)
@@ -191,7 +209,30 @@ def test_heating_jacobian_rows_match_finite_differences(rr, side):
         err = np.max(np.abs(J_ad - J_fd)) / scale
         errs.append(err)
         print("%s column %d: max |J_ad - J_fd| / max |J| = %.2e (max |J| = %.3e K/day/K)" % (side, i, err, scale))
-    assert max(errs) < 2e-2, errs
+    assert max(errs) < 2e-3, errs
+
+
+@pytest.mark.parametrize("side", ["lw", "sw"])
+def test_table_vjp_close_to_smooth_vjp(rr, side):
+    """The default FP64 mode (table lookups) differentiates the same smooth
+    functions at table-quantised values: its gradients stay within the tables'
+    resolution of the smooth mode's."""
+    c = _cols()
+    rng = np.random.default_rng(16)
+    if side == "lw":
+        cot = _lw_cot(c, rng)
+        ga, gb = _lw_grad(rr, c, cot, "fp64"), _lw_grad(rr, c, cot, "smooth")
+        keys = LW_DIRS
+    else:
+        cot = _sw_cot(c, rng)
+        ga, gb = _sw_grad(rr, c, cot, "fp64"), _sw_grad(rr, c, cot, "smooth")
+        keys = SW_DIRS
+    for k in keys:
+        rel = np.linalg.norm(ga[k] - gb[k]) / max(np.linalg.norm(gb[k]), 1e-300)
+        print("%s %s: |g_table - g_smooth| / |g_smooth| = %.2e" % (side, k, rel))
+        # the SW temperature gradient is a small residual (T enters the SW only
+        # through gas absorption) and amplifies the tables' relative resolution
+        assert rel < (5e-2 if (side, k) == ("sw", "tlay") else 1e-2), (side, k, rel)
 
 
 @pytest.mark.parametrize("side", ["lw", "sw"])
@@ -236,7 +277,7 @@ def test_torch_autograd_matches_finite_differences(rr):
     import torch
     from mima_amd.autograd import radiation
     c = _cols(32, seed=7)
-    rr.set_precision("fp64")
+    rr.set_precision(FD_PREC)
     d = rr.to_device(c)
     gen = torch.Generator(device="cpu").manual_seed(5)
     L, n = c["nlay"], c["ncol"]
@@ -263,5 +304,5 @@ def test_torch_autograd_matches_finite_differences(rr):
         jp = float(J(*[xi + st for xi, st in zip(x, steps)]))
         jm = float(J(*[xi - st for xi, st in zip(x, steps)]))
     fd = (jp - jm) / 2.0
-    print("autograd: ad %.6e fd %.6e" % (ad, fd))
-    assert abs(ad - fd) <= 5e-3 * abs(fd), (ad, fd)
+    print("autograd: ad %.8e fd %.8e" % (ad, fd))
+    assert abs(ad - fd) <= 2e-3 * abs(fd), (ad, fd)

@@ -1,6 +1,6 @@
 """The Fortran drop-in (mima_amd/fortran/rrtmg_hip_shim.f90), called the way
 MiMA's run_rrtmg calls RRTMG, gives bit-identical results to the direct C ABI
-path."""
+path; so do its adjoint entry points (rrtmg_lw_adj, rrtmg_sw_adj)."""
 import os
 import subprocess
 
@@ -34,7 +34,8 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, dtype=np.float64)
-    sizes = [(nlay + 1) * ncol, (nlay + 1) * ncol, nlay * ncol] * 2
+    n2, n3 = nlay * ncol, (nlay + 1) * ncol
+    sizes = [n3, n3, n2] * 2 + [n2, n3, n2, n2, ncol, n2, n2, n2]
     parts, o = [], 0
     for s in sizes:
         parts.append(raw[o:o + s])
@@ -44,3 +45,12 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     for got, key, ref in zip(parts, ("swuflx", "swdflx", "swhr", "uflx", "dflx", "hr"),
                              (sw["swuflx"], sw["swdflx"], sw["swhr"], lw["uflx"], lw["dflx"], lw["hr"])):
         np.testing.assert_array_equal(got.reshape(ref.shape), ref, err_msg=key)
+    # adjoint entry points: cotangent 1 on every heating rate, 0 on the fluxes
+    z3, one = np.zeros((nlay + 1, ncol)), np.ones((nlay, ncol))
+    gl = rrtmg.lw_vjp_columns(c, uflx_bar=z3, dflx_bar=z3, hr_bar=one)
+    gs = rrtmg.sw_vjp_columns(c, swuflx_bar=z3, swdflx_bar=z3, swhr_bar=one)
+    refs = [gl["tlay"], gl["tlev"], gl["h2ovmr"], gl["o3vmr"], gl["tsfc"], gs["tlay"], gs["h2ovmr"], gs["o3vmr"]]
+    names = ["lw tlay", "lw tlev", "lw h2o", "lw o3", "lw tsfc", "sw tlay", "sw h2o", "sw o3"]
+    for got, key, ref in zip(parts[6:], names, refs):
+        np.testing.assert_array_equal(got.reshape(ref.shape), ref, err_msg=key)
+    assert np.any(refs[0] != 0) and np.any(refs[5] != 0)

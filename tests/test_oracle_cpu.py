@@ -316,3 +316,57 @@ def test_autograd_level_temperature_rule_matches_generator():
     t = mima_tlev(torch.from_numpy(c["tlay"]), torch.from_numpy(c["tsfc"]), torch.from_numpy(c["play"]),
                   torch.from_numpy(c["plev"])).numpy()
     np.testing.assert_allclose(t, c["tlev"], rtol=1e-12, atol=0)
+
+
+def test_mlp_oracle_gradients_match_finite_differences():
+    # the CPU reference of the surrogate's backward: reverse-mode gradients of
+    # the MSE loss against central differences, on the network without the
+    # BF16 rounding of hidden activations (rounding makes the loss piecewise
+    # constant at the scale of the finite-difference step)
+    import mlp_oracle as mo
+    rng = np.random.default_rng(0)
+    sizes = [(12, 8), (10, 12), (6, 10)]
+    W = [mo.bf16_round(rng.standard_normal(s).astype(np.float32) * 0.5).astype(np.float64) for s in sizes]
+    b = [rng.standard_normal(s[0]) * 0.1 for s in sizes]
+    x = mo.bf16_round(rng.standard_normal((16, 8)).astype(np.float32)).astype(np.float64)
+    t = rng.standard_normal((16, 6))
+
+    def loss_exact(W, b):
+        # same network without the BF16 rounding of hidden activations
+        h = x
+        for l in range(len(W)):
+            z = h @ W[l].T + b[l]
+            h = np.maximum(z, 0.0) if l < len(W) - 1 else z
+        return float(np.mean((h - t) ** 2))
+
+    # gradients of the unrounded network by the oracle's recurrences
+    saved = mo.bf16_round
+    mo.bf16_round = lambda a: np.asarray(a, dtype=np.float64)
+    try:
+        _, gw, gb = mo.mse_and_grads(x, t, W, b)
+    finally:
+        mo.bf16_round = saved
+    h = 1e-6
+    for l in range(len(W)):
+        for (i, j) in [(0, 0), (1, 2), (W[l].shape[0] - 1, W[l].shape[1] - 1)]:
+            Wp = [w.copy() for w in W]; Wm = [w.copy() for w in W]
+            Wp[l][i, j] += h; Wm[l][i, j] -= h
+            fd = (loss_exact(Wp, b) - loss_exact(Wm, b)) / (2 * h)
+            assert abs(fd - gw[l][i, j]) <= 1e-6 + 1e-5 * abs(fd)
+        bp = [v.copy() for v in b]; bm = [v.copy() for v in b]
+        bp[l][0] += h; bm[l][0] -= h
+        fd = (loss_exact(W, bp) - loss_exact(W, bm)) / (2 * h)
+        assert abs(fd - gb[l][0]) <= 1e-6 + 1e-5 * abs(fd)
+
+
+def test_mlp_oracle_bf16_rounding_and_adam():
+    import mlp_oracle as mo
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -3.14159, 0.0], dtype=np.float32)
+    r = mo.bf16_round(x)
+    # 1 + 2^-8 is halfway between 1 and 1 + 2^-7: ties to even -> 1
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 2 ** -7)
+    assert np.array_equal(mo.bits_to_f32(mo.bf16_bits(x)), r)
+    # Adam's first step moves every weight by lr against the gradient's sign
+    w, m, v = np.ones(3), np.zeros(3), np.zeros(3)
+    w1, _, _ = mo.adam_step(w, m, v, np.array([2.0, -0.5, 1e-3]), 1, lr=0.1)
+    np.testing.assert_allclose(w1, [0.9, 1.1, 0.9], rtol=0, atol=1e-6)
