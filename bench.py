@@ -170,6 +170,81 @@ def surrogate_timing(torch, rrtmg, config_columns, args):
                      "flop_per_column": flop_col, "dtype": "bf16 x bf16 -> fp32"}}
 
 
+def large_batch_timing(torch, rrtmg, config_columns, args):
+    """Large batch: 512 x 256 = 131072 columns x 60 levels, CO2 log-uniform on
+    280-1120 ppmv per column, FP32-mixed; forward plus backward of LW+SW
+    heating rates and surface net fluxes (cotangents N(0, 1) from a seed).
+    Three ways: the four entry points one after another (forward, then
+    backward), the one-pass entry points (forward formed inside the backward)
+    on one stream, and the one-pass entry points captured in a HIP graph with
+    SW and LW on two streams (the reported value)."""
+    cols = config_columns("c4_large", seed=0)
+    ncol, nlay = cols["ncol"], cols["nlay"]
+    d = rrtmg.to_device(cols)
+    del cols
+    gen = torch.Generator(device="cpu").manual_seed(4)
+    rnd = lambda *sh: torch.randn(*sh, generator=gen, dtype=torch.float64).cuda()  # noqa: E731
+    hr_lw, hr_sw, f_lw, f_sw = rnd(nlay, ncol), rnd(nlay, ncol), rnd(ncol), rnd(ncol)
+    z = lambda: torch.zeros(nlay + 1, ncol, dtype=torch.float64, device="cuda")  # noqa: E731
+    ub, db, sub, sdb = z(), z(), z(), z()
+    ub[0], db[0], sub[0], sdb[0] = -f_lw, f_lw, -f_sw, f_sw
+    rrtmg.set_precision("mixed")
+
+    def separate():
+        rrtmg.sw_columns(d)
+        rrtmg.lw_columns(d)
+        rrtmg.sw_vjp_columns(d, swuflx_bar=sub, swdflx_bar=sdb, swhr_bar=hr_sw)
+        rrtmg.lw_vjp_columns(d, uflx_bar=ub, dflx_bar=db, hr_bar=hr_lw)
+
+    side = torch.cuda.Stream()
+
+    def one_pass(two_streams=False):
+        cur = torch.cuda.current_stream()
+        if two_streams:
+            side.wait_stream(cur)
+        sw = rrtmg.sw_vjp_columns(d, swuflx_bar=sub, swdflx_bar=sdb, swhr_bar=hr_sw, forward=True)
+        if two_streams:
+            with torch.cuda.stream(side):
+                lw = rrtmg.lw_vjp_columns(d, uflx_bar=ub, dflx_bar=db, hr_bar=hr_lw, forward=True)
+            cur.wait_stream(side)
+        else:
+            lw = rrtmg.lw_vjp_columns(d, uflx_bar=ub, dflx_bar=db, hr_bar=hr_lw, forward=True)
+        return sw, lw
+
+    n = max(2, args.large_steps)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    t_sep = timed(separate)
+    t_one = timed(one_pass)
+    main_s = torch.cuda.Stream()
+    with torch.cuda.stream(main_s):
+        one_pass(True)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=main_s):
+            one_pass(True)
+    t_graph = timed(graph.replay)
+    print("[bench] large batch %d x %d: separate %.2f ms, one pass %.2f ms, one pass in a graph (2 streams) %.2f ms"
+          % (ncol, nlay, t_sep * 1e3, t_one * 1e3, t_graph * 1e3), file=sys.stderr, flush=True)
+    del graph
+    return {"workload": "forward + backward of LW+SW heating rates and surface net fluxes wrt T, q, O3, Ts; "
+                        "512x256 grid = %d columns x %d levels, CO2 log-uniform 280-1120 ppmv, FP32-mixed"
+                        % (ncol, nlay),
+            "value": ncol / t_graph, "unit": "columns/s", "ms_per_step": t_graph * 1e3, "steps": n,
+            "how": "one-pass forward+backward entry points (rrtmg_hip_sw_fwd_vjp, rrtmg_hip_lw_fwd_vjp) "
+                   "captured in one HIP graph, SW and LW on two streams",
+            "one_pass_eager_ms": t_one * 1e3, "separate_calls_ms": t_sep * 1e3,
+            "speedup_vs_separate": t_sep / t_graph}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +254,8 @@ def main():
     ap.add_argument("--config", default="c1_mixed_vs_fp64")
     ap.add_argument("--surrogate-epochs", type=int, default=10,
                     help="training epochs of the MLP surrogate (0 skips the surrogate leg)")
+    ap.add_argument("--large-steps", type=int, default=5,
+                    help="timed steps of the 131072 x 60 forward+backward leg (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB 5 times (HBM counter calibration)")
     args = ap.parse_args()
@@ -253,6 +330,7 @@ def main():
 
     adj = adjoint_timing(torch, rrtmg, config_columns, args)
     sur = surrogate_timing(torch, rrtmg, config_columns, args) if args.surrogate_epochs > 0 else None
+    large = large_batch_timing(torch, rrtmg, config_columns, args) if args.large_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -308,6 +386,7 @@ def main():
         "cpu_baseline": cpu,
         "adjoint": adj,
         "surrogate": sur,
+        "large_batch": large,
     }
     print(json.dumps(out))
 

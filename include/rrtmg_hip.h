@@ -124,7 +124,9 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay,
 /* Backward of rrtmg_sw (no reference counterpart), as rrtmg_hip_lw_vjp:
  * cotangents of swuflx, swdflx, swhr (each NULL for zero); overwrites the
  * gradients with respect to tlay, h2ovmr and o3vmr (ncol, nlay).  Night
- * columns (coszen < 1e-10) get zero gradients, as their outputs are zero. */
+ * columns (coszen < 1e-10) get zero gradients, as their outputs are zero.
+ * The exp(-x) table's derivative is minus the tabulated value (exact
+ * log-derivative of each transmittance). */
 int rrtmg_hip_sw_vjp(int ncol, int nlay,
                      const double *play, const double *plev, const double *tlay,
                      const double *tlev, const double *tsfc,
@@ -136,6 +138,42 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay,
                      const double *swuflx_bar, const double *swdflx_bar, const double *swhr_bar,
                      double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar,
                      int ptr_kind, void *stream);
+
+/* Forward and backward in one pass (no reference counterpart): as
+ * rrtmg_hip_lw_vjp / rrtmg_hip_sw_vjp, and also the forward's outputs (uflx,
+ * dflx (ncol, nlay+1), hr (ncol, nlay); the clear-sky copies of rrtmg_lw /
+ * rrtmg_sw are the same values), taken from the forward that the backward
+ * recomputes anyway.  For callers that know the cotangents before the
+ * forward, e.g. the sensitivity of a fixed functional of the heating rates;
+ * equal to rrtmg_hip_lw + rrtmg_hip_lw_vjp (SW likewise) at about the cost of
+ * the backward alone.  Device-pointer calls launch kernels only (no
+ * allocation once the workspace has grown to the call's size, no memset,
+ * no synchronisation), so a warmed-up call can be captured in a HIP graph. */
+int rrtmg_hip_lw_fwd_vjp(int ncol, int nlay,
+                         const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc,
+                         const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                         const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *cfc11vmr, const double *cfc12vmr,
+                         const double *cfc22vmr, const double *ccl4vmr,
+                         const double *emis, const double *tauaer,
+                         const double *uflx_bar, const double *dflx_bar, const double *hr_bar,
+                         double *uflx, double *dflx, double *hr,
+                         double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
+                         double *o3vmr_bar, double *tsfc_bar,
+                         int ptr_kind, void *stream);
+int rrtmg_hip_sw_fwd_vjp(int ncol, int nlay,
+                         const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc,
+                         const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                         const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *asdir, const double *asdif, const double *aldir,
+                         const double *aldif, const double *coszen,
+                         double adjes, int dyofyr, double scon,
+                         const double *swuflx_bar, const double *swdflx_bar, const double *swhr_bar,
+                         double *swuflx, double *swdflx, double *swhr,
+                         double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar,
+                         int ptr_kind, void *stream);
 
 /* ---------------------------------------------------------------------------
  * MLP surrogate of the LW+SW heating-rate profile (no reference counterpart;

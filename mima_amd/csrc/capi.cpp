@@ -552,13 +552,15 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     return 0;
 }
 
-int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
-                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
-                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
-                     const double *asdir, const double *asdif, const double *aldir, const double *aldif,
-                     const double *coszen, double adjes, int dyofyr, double scon, const double *swuflx_bar,
-                     const double *swdflx_bar, const double *swhr_bar, double *tlay_bar, double *h2ovmr_bar,
-                     double *o3vmr_bar, int ptr_kind, void *stream)
+// backward of rrtmg_sw; with swuflx/swdflx/swhr given, also the forward's
+// outputs, formed from the forward that the backward band kernel recomputes
+static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                       const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                       const double *coszen, double adjes, int dyofyr, double scon, const double *swuflx_bar,
+                       const double *swdflx_bar, const double *swhr_bar, double *tlay_bar, double *h2ovmr_bar,
+                       double *o3vmr_bar, double *swuflx, double *swdflx, double *swhr, int ptr_kind, void *stream)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
@@ -573,6 +575,8 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     double *outs[] = {tlay_bar, h2ovmr_bar, o3vmr_bar};
     for (double *o : outs)
         if (!o) return fail(RRTMG_HIP_EINVAL, "null gradient array");
+    const bool fwd = swuflx || swdflx || swhr;
+    if (fwd && !(swuflx && swdflx && swhr)) return fail(RRTMG_HIP_EINVAL, "null forward output array");
     if (bad(adjes) || bad(scon)) return fail(RRTMG_HIP_EINVAL, "non-finite adjes/scon");
     const bool host = ptr_kind == RRTMG_HIP_HOST;
     if (host) {
@@ -603,6 +607,9 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
     const size_t o_cot = off; off = align_up(off + rs * 2 * n3);
     const size_t o_gp = off; off = align_up(off + rs * 14 * 3 * n2);
+    const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves, sw_vjp_state_bytes(nlay, rs));
+    const size_t o_ds = off; off = align_up(off + (size_t)chunk * sw_vjp_state_bytes(nlay, rs));
+    const size_t o_bf = off; off = align_up(off + (fwd ? rs * 14 * 2 * n3 : 0));
     if ((rc = s->sw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->sw_vjp_work.p;
     SwVjpWork w;
@@ -612,8 +619,11 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     w.laysolfr = (int *)(wb + o_lsf);
     w.cot = wb + o_cot;
     w.gpart = wb + o_gp;
+    w.dstate = wb + o_ds;
+    w.chunk = chunk;
+    w.bflux = fwd ? wb + o_bf : nullptr;
     if (host) {
-        const size_t need = 256 * 24 + sizeof(double) * (12 * n2 + 4 * n3 + (size_t)ncol * 6);
+        const size_t need = 256 * 27 + sizeof(double) * (12 * n2 + 4 * n3 + (size_t)ncol * 6 + (fwd ? n2 + 2 * n3 : 0));
         if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
     }
     Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
@@ -628,6 +638,12 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     in.adjflux = sw_adjflux(adjes, dyofyr, scon);
     VjpCot cot{sg.in(swuflx_bar, n3), sg.in(swdflx_bar, n3), sg.in(swhr_bar, n2)};
     SwGrad g{sg.out(tlay_bar, n2), sg.out(h2ovmr_bar, n2), sg.out(o3vmr_bar, n2)};
+    SwOutputs fo{};
+    if (fwd) {  // clear sky: the clear-sky outputs are the same arrays
+        fo.uflx = fo.uflxc = sg.out(swuflx, n3);
+        fo.dflx = fo.dflxc = sg.out(swdflx, n3);
+        fo.hr = fo.hrc = sg.out(swhr, n2);
+    }
     if (sg.ck) return fail(RRTMG_HIP_EDEVICE, "SW backward: host-to-device copy failed");
     SwDevTables t;
     t.desc = s->sw.tab;
@@ -637,25 +653,32 @@ int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev,
     t.tref = s->swd.tref;
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
-    hipError_t e = launch_sw_vjp(in, cot, g, w, t, g_prec, st, s->events(2));
+    hipError_t e = launch_sw_vjp(in, cot, g, w, t, g_prec, st, s->events(2), fwd ? &fo : nullptr);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward launch: ") + hipGetErrorString(e));
     if (host) {
         sg.back(tlay_bar, g.tlay, n2);
         sg.back(h2ovmr_bar, g.h2o, n2);
         sg.back(o3vmr_bar, g.o3, n2);
+        if (fwd) {
+            sg.back(swuflx, fo.uflx, n3);
+            sg.back(swdflx, fo.dflx, n3);
+            sg.back(swhr, fo.hr, n2);
+        }
         e = hipStreamSynchronize(st);
         if (e != hipSuccess || sg.ck) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward: ") + hipGetErrorString(e));
     }
     return 0;
 }
 
-int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
-                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
-                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
-                     const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
-                     const double *emis, const double *tauaer, const double *uflx_bar, const double *dflx_bar,
-                     const double *hr_bar, double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
-                     double *o3vmr_bar, double *tsfc_bar, int ptr_kind, void *stream)
+// backward of rrtmg_lw; with uflx/dflx/hr given, also the forward's outputs
+static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                       const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                       const double *emis, const double *tauaer, const double *uflx_bar, const double *dflx_bar,
+                       const double *hr_bar, double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
+                       double *o3vmr_bar, double *tsfc_bar, double *uflx, double *dflx, double *hr, int ptr_kind,
+                       void *stream)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
@@ -670,6 +693,8 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     double *outs[] = {tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar};
     for (double *o : outs)
         if (!o) return fail(RRTMG_HIP_EINVAL, "null gradient array");
+    const bool fwd = uflx || dflx || hr;
+    if (fwd && !(uflx && dflx && hr)) return fail(RRTMG_HIP_EINVAL, "null forward output array");
     const bool host = ptr_kind == RRTMG_HIP_HOST;
     if (host) {
         for (int i = 0; i < 12; ++i)
@@ -702,6 +727,9 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     const size_t o_gp = off; off = align_up(off + rs * 16 * 3 * n2);
     const size_t o_gl = off; off = align_up(off + rs * 16 * n3);
     const size_t o_gs = off; off = align_up(off + rs * 16 * 2 * ncol);
+    const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->lw.tab.nwaves, lw_vjp_state_bytes(nlay, rs));
+    const size_t o_ds = off; off = align_up(off + (size_t)chunk * lw_vjp_state_bytes(nlay, rs));
+    const size_t o_bf = off; off = align_up(off + (fwd ? rs * 16 * 2 * n3 : 0));
     if ((rc = s->lw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->lw_vjp_work.p;
     LwVjpWork w;
@@ -715,8 +743,12 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     w.gpart = wb + o_gp;
     w.glev = wb + o_gl;
     w.gsc = wb + o_gs;
+    w.dstate = wb + o_ds;
+    w.chunk = chunk;
+    w.bflux = fwd ? wb + o_bf : nullptr;
     if (host) {
-        const size_t need = 256 * 24 + sizeof(double) * (16 * n2 + 5 * n3 + (size_t)ncol * 18 + (tauaer ? 16 * n2 : 0));
+        const size_t need = 256 * 27 + sizeof(double) * (16 * n2 + 5 * n3 + (size_t)ncol * 18 + (tauaer ? 16 * n2 : 0) +
+                                                         (fwd ? n2 + 2 * n3 : 0));
         if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
     }
     Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
@@ -731,6 +763,12 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     VjpCot cot{sg.in(uflx_bar, n3), sg.in(dflx_bar, n3), sg.in(hr_bar, n2)};
     LwGrad g{sg.out(tlay_bar, n2), sg.out(tlev_bar, n3), sg.out(h2ovmr_bar, n2), sg.out(o3vmr_bar, n2),
              sg.out(tsfc_bar, ncol)};
+    LwOutputs fo{};
+    if (fwd) {  // clear sky: the clear-sky outputs are the same arrays
+        fo.uflx = fo.uflxc = sg.out(uflx, n3);
+        fo.dflx = fo.dflxc = sg.out(dflx, n3);
+        fo.hr = fo.hrc = sg.out(hr, n2);
+    }
     if (sg.ck) return fail(RRTMG_HIP_EDEVICE, "LW backward: host-to-device copy failed");
     LwDevTables t;
     t.desc = s->lw.tab;
@@ -739,7 +777,7 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
     t.cst = s->lwd.cst;
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
-    hipError_t e = launch_lw_vjp(in, cot, g, w, t, g_prec, st, s->events(3));
+    hipError_t e = launch_lw_vjp(in, cot, g, w, t, g_prec, st, s->events(3), fwd ? &fo : nullptr);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
     if (host) {
         sg.back(tlay_bar, g.tlay, n2);
@@ -747,10 +785,71 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev,
         sg.back(h2ovmr_bar, g.h2o, n2);
         sg.back(o3vmr_bar, g.o3, n2);
         sg.back(tsfc_bar, g.tsfc, ncol);
+        if (fwd) {
+            sg.back(uflx, fo.uflx, n3);
+            sg.back(dflx, fo.dflx, n3);
+            sg.back(hr, fo.hr, n2);
+        }
         e = hipStreamSynchronize(st);
         if (e != hipSuccess || sg.ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+int rrtmg_hip_sw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                     const double *coszen, double adjes, int dyofyr, double scon, const double *swuflx_bar,
+                     const double *swdflx_bar, const double *swhr_bar, double *tlay_bar, double *h2ovmr_bar,
+                     double *o3vmr_bar, int ptr_kind, void *stream)
+{
+    return sw_vjp_impl(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, asdir,
+                       asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx_bar, swdflx_bar, swhr_bar, tlay_bar,
+                       h2ovmr_bar, o3vmr_bar, nullptr, nullptr, nullptr, ptr_kind, stream);
+}
+
+int rrtmg_hip_sw_fwd_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                         const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                         const double *coszen, double adjes, int dyofyr, double scon, const double *swuflx_bar,
+                         const double *swdflx_bar, const double *swhr_bar, double *swuflx, double *swdflx,
+                         double *swhr, double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar, int ptr_kind,
+                         void *stream)
+{
+    if (!swuflx || !swdflx || !swhr) return fail(RRTMG_HIP_EINVAL, "null forward output array");
+    return sw_vjp_impl(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, asdir,
+                       asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx_bar, swdflx_bar, swhr_bar, tlay_bar,
+                       h2ovmr_bar, o3vmr_bar, swuflx, swdflx, swhr, ptr_kind, stream);
+}
+
+int rrtmg_hip_lw_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                     const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                     const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                     const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                     const double *emis, const double *tauaer, const double *uflx_bar, const double *dflx_bar,
+                     const double *hr_bar, double *tlay_bar, double *tlev_bar, double *h2ovmr_bar,
+                     double *o3vmr_bar, double *tsfc_bar, int ptr_kind, void *stream)
+{
+    return lw_vjp_impl(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                       cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar, tlay_bar,
+                       tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar, nullptr, nullptr, nullptr, ptr_kind, stream);
+}
+
+int rrtmg_hip_lw_fwd_vjp(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                         const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr,
+                         const double *ccl4vmr, const double *emis, const double *tauaer, const double *uflx_bar,
+                         const double *dflx_bar, const double *hr_bar, double *uflx, double *dflx, double *hr,
+                         double *tlay_bar, double *tlev_bar, double *h2ovmr_bar, double *o3vmr_bar,
+                         double *tsfc_bar, int ptr_kind, void *stream)
+{
+    if (!uflx || !dflx || !hr) return fail(RRTMG_HIP_EINVAL, "null forward output array");
+    return lw_vjp_impl(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                       cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar, tlay_bar,
+                       tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar, uflx, dflx, hr, ptr_kind, stream);
 }
 
 // ---------------------------------------------------------------------------

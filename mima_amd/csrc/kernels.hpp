@@ -82,6 +82,9 @@ struct LwVjpWork {
     void *gpart = nullptr;    // [16][3][nlay][ncol] of Real: per-band gradient (T, q, O3)
     void *glev = nullptr;     // [16][nlay+1][ncol] of Real: per-band gradient wrt level temperature
     void *gsc = nullptr;      // [16][2][ncol] of Real: per-band d/dtsfc and d/dpwvcm
+    void *dstate = nullptr;   // [chunk][nlay][64] of Real: incoming downward radiance of each wave of a launch
+    int chunk = 0;            // wave units per backward band-kernel launch (rows of dstate)
+    void *bflux = nullptr;    // [16][2][nlay+1][ncol] of Real: band fluxes of the recomputed forward, or NULL
 };
 struct SwVjpWork {
     void *dcoef = nullptr;    // [SWF_COUNT][4][nlay][ncol] of Real
@@ -90,15 +93,45 @@ struct SwVjpWork {
     int *laysolfr = nullptr;  // [14][ncol]
     void *cot = nullptr;      // [2][nlay+1][ncol] of Real
     void *gpart = nullptr;    // [14][3][nlay][ncol] of Real
+    void *dstate = nullptr;   // [chunk][3 (nlay+1) + 5 nlay][64] of Real: downward-sweep states and layer adjoints
+    int chunk = 0;            // wave units per backward band-kernel launch (rows of dstate)
+    void *bflux = nullptr;    // [14][2][nlay+1][ncol] of Real: band fluxes of the recomputed forward, or NULL
 };
+// The backward band kernels keep the states of the sweep that runs in the
+// other direction from its reverse in a global scratch row per wave unit
+// (LDS keeps the rest); a launch covers at most `chunk` wave units so the
+// scratch stays within VJP_SCRATCH_BYTES, and longer grids are launched in
+// chunks that reuse the rows.
+constexpr size_t VJP_SCRATCH_BYTES = (size_t)512 << 20;
+inline int vjp_chunk(int units, size_t bytes_per_unit)
+{
+    size_t c = VJP_SCRATCH_BYTES / bytes_per_unit;
+    if (c < 8) c = 8;
+    return units < (int)c ? units : (int)c;
+}
+inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
+{
+    return real_size * (3 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;
+}
+inline size_t lw_vjp_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)nlay * 64; }
 // LDS bytes of one backward band workgroup
-inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)3 * nlay * 64 + 4 * 64); }
-inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)5 * (nlay + 1) * 64 + 4 * 64); }
+inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)2 * nlay * 64 + 4 * 64); }
+inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)2 * (nlay + 1) * 64 + 4 * 64); }
 
+// fwd: NULL, or the forward's outputs, formed from w.bflux (which the
+// backward band kernel then fills from the forward it recomputes) by the
+// forward's finish kernel
 hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
+                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev,
+                         const LwOutputs *fwd = nullptr);
 hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
+                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev,
+                         const SwOutputs *fwd = nullptr);
+// the forward's finish kernels alone, on band fluxes [band][2][nlay+1][ncol]
+hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *bflux, const LwDevTables &t,
+                            int mode, hipStream_t s);
+hipError_t launch_sw_finish(const ColumnInputs &in, const SwOutputs &out, void *bflux, const SwDevTables &t,
+                            int mode, hipStream_t s);
 
 // mode (the C ABI's precision): 0 FP64 everywhere; 1 FP32 with FP64 islands;
 // 2 FP64 with exp(-x) and the tau transition function evaluated directly

@@ -340,6 +340,19 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     return hipGetLastError();
 }
 
+hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *bflux, const LwDevTables &t,
+                            int mode, hipStream_t s)
+{
+    const int nb = (in.ncol + 255) / 256;
+    LwWork w;
+    w.bflux = bflux;
+    if (mode == 1)
+        hipLaunchKernelGGL(lw_finish_kernel<float>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
+    else
+        hipLaunchKernelGGL(lw_finish_kernel<double>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
+    return hipGetLastError();
+}
+
 hipError_t launch_lw(const ColumnInputs &in, const LwOutputs &out, const LwWork &w, const LwDevTables &t, int mode,
                      hipStream_t s, hipEvent_t *ev)
 {

@@ -13,8 +13,9 @@
 //    d/dO3][lay][col]); level Planck indices; and the per-level radiance
 //    cotangents after folding the heating-rate cotangent into the fluxes.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
-//    pass A, downward sweep (values), storing per layer the transmittance,
-//    the upward source and the incoming downward radiance in LDS; pass B,
+//    pass A, downward sweep (values), storing per layer the transmittance
+//    and the upward source in LDS and the incoming downward radiance in a
+//    per-wave global scratch row (so LDS does not cap occupancy); pass B,
 //    upward sweep, replacing the source by (source - incoming upward
 //    radiance); pass C, reverse of the upward sweep, top-down, leaving the
 //    adjoints of the upward source and of the transmittance (upward part) in
@@ -28,6 +29,8 @@
 //  * finish (one thread per column): sums over bands in FP64 and the
 //    precipitable-water chain of the diffusivity angle.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 #include "lw_optics.hpp"
@@ -313,7 +316,8 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
 template <int BAND, typename Real>
 __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const LwVjpWork &w, const LwBand &B,
                                  const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
-                                 const LwConst *__restrict__ cst, WaveUnit u, int tile, Real *smem)
+                                 const LwConst *__restrict__ cst, WaveUnit u, int tile, Real *smem,
+                                 Real *__restrict__ s3)
 {
     constexpr int NG = lw_ng(BAND);
     using V3 = Dual<Real, 3>;
@@ -325,8 +329,9 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     const int col = active ? col0 : 0;
     Real *s1 = smem;                          // [nlay][64]
     Real *s2 = smem + (size_t)nlay * 64;      // [nlay][64]
-    Real *s3 = smem + (size_t)2 * nlay * 64;  // [nlay][64]
-    Real *red = smem + (size_t)3 * nlay * 64; // [4][64]
+    Real *red = smem + (size_t)2 * nlay * 64; // [4][64]
+    // s3: incoming downward radiance per layer, [nlay][64] in this wave's
+    // global scratch row (written in pass A, read back in pass D)
     const Real *dco = (const Real *)w.dcoef;
     const Real *tlevf = (const Real *)w.tlev;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
@@ -375,6 +380,12 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     auto ru = [&](int k) { return active ? cu[(size_t)k * ncol + col] * cband : (Real)0; };
     auto rd = [&](int k) { return active ? cd[(size_t)k * ncol + col] * cband : (Real)0; };
 
+    // band fluxes of the recomputed forward, when asked for (as the forward
+    // band kernel writes them; uniform branch)
+    Real *bf_up = w.bflux ? (Real *)w.bflux + (size_t)(BAND - 1) * 2 * n3 : nullptr;
+    Real *bf_dn = bf_up ? bf_up + n3 : nullptr;
+    if (bf_dn && active && g == 0) bf_dn[(size_t)nlay * ncol + col] = (Real)0;
+
     // pass A: downward sweep
     Real radld = (Real)0, fracs1 = (Real)0;
     Real plev_up = planklev(nlay);
@@ -394,17 +405,29 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
             s2[(size_t)(lev - 1) * 64 + lane] = bbu;
             s3[(size_t)(lev - 1) * 64 + lane] = radld;
             radld = radld + (bbd - radld) * rt.atrans;
+            if (bf_dn) {
+                const Real sd = seg_sum(active ? radld : (Real)0, lane, NG, red);
+                if (active && g == 0) bf_dn[(size_t)(lev - 1) * ncol + col] = sd;
+            }
             plev_up = plev_dn;
             if (lev == 1) fracs1 = plfrac;
         }
     }
     // surface and pass B: upward sweep; keep source minus incoming radiance
     Real radlu = fracs1 * plankbnd + ((Real)1 - semiss) * radld;
+    if (bf_up) {
+        const Real su = seg_sum(active ? radlu : (Real)0, lane, NG, red);
+        if (active && g == 0) bf_up[col] = su;
+    }
     for (int lev = 1; lev <= nlay; ++lev) {
         const Real atr = s1[(size_t)(lev - 1) * 64 + lane];
         const Real dsrc = s2[(size_t)(lev - 1) * 64 + lane] - radlu;
         s2[(size_t)(lev - 1) * 64 + lane] = dsrc;
         radlu = radlu + dsrc * atr;
+        if (bf_up) {
+            const Real su = seg_sum(active ? radlu : (Real)0, lane, NG, red);
+            if (active && g == 0) bf_up[(size_t)lev * ncol + col] = su;
+        }
     }
     // pass C: reverse of the upward sweep (top-down)
     Real a_ru = ru(nlay);
@@ -479,20 +502,22 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
 
 template <typename Real>
 __global__ __launch_bounds__(64) void lw_vjp_band_kernel(ColumnInputs in, LwVjpWork w, const LwBand *__restrict__ bands,
-                                                         int nwaves, int nblocks, int exp_off, int tfn_off, int plk_off,
-                                                         const Real *__restrict__ tab, const LwConst *__restrict__ cst,
+                                                         int nwaves, int blk0, int nblocks, int exp_off, int tfn_off,
+                                                         int plk_off, const Real *__restrict__ tab,
+                                                         const LwConst *__restrict__ cst,
                                                          const WaveUnit *__restrict__ units)
 {
     extern __shared__ unsigned char smem_raw[];
     Real *smem = (Real *)smem_raw;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int tile = blk / nwaves;
-    const WaveUnit u = units[blk % nwaves];
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
     const LwBand B = bands[u.band];
+    Real *s3 = (Real *)w.dstate + (size_t)blk * in.nlay * 64;
 #define BODY(b)                                                                                          \
     case b - 1:                                                                                          \
-        lw_vjp_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem);          \
+        lw_vjp_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem, s3);      \
         break;
     switch (u.band) {
         BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
@@ -561,7 +586,8 @@ __global__ __launch_bounds__(256) void lw_vjp_finish_kernel(ColumnInputs in, LwV
 
 template <typename Real>
 static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                                  const LwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev)
+                                  const LwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev,
+                                  const LwOutputs *fwd, int mode)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -570,21 +596,29 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
     const size_t shmem = lw_vjp_lds_bytes(in.nlay, sizeof(Real));
     const int nblocks = ntiles * t.desc.nwaves;
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(lw_vjp_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
-                       t.desc.totplnk, (const Real *)t.tab,
-                       t.cst, t.units);
+    // launches of at most w.chunk wave units, each reusing the scratch rows
+    for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
+        const int nb_c = std::min(w.chunk, nblocks - b0);
+        hipLaunchKernelGGL(lw_vjp_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), shmem, s, in, w, t.bands,
+                           t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
+                           t.desc.totplnk, (const Real *)t.tab, t.cst, t.units);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
+    if (fwd) {
+        const hipError_t e = launch_lw_finish(in, *fwd, w.bflux, t, mode, s);
+        if (e != hipSuccess) return e;
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
 hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad &out, const LwVjpWork &w,
-                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev)
+                         const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev, const LwOutputs *fwd)
 {
-    return mode == 1 ? launch_lw_vjp_t<float>(in, cot, out, w, t, false, s, ev)
-                     : launch_lw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev);
+    if (fwd && !w.bflux) return hipErrorInvalidValue;
+    return mode == 1 ? launch_lw_vjp_t<float>(in, cot, out, w, t, false, s, ev, fwd, mode)
+                     : launch_lw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev, fwd, mode);
 }
 
 }  // namespace rrtmg

@@ -296,6 +296,19 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
     return hipGetLastError();
 }
 
+hipError_t launch_sw_finish(const ColumnInputs &in, const SwOutputs &out, void *bflux, const SwDevTables &t,
+                            int mode, hipStream_t s)
+{
+    const int nb = (in.ncol + 255) / 256;
+    SwWork w;
+    w.bflux = bflux;
+    if (mode == 1)
+        hipLaunchKernelGGL(sw_finish_kernel<float>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+    else
+        hipLaunchKernelGGL(sw_finish_kernel<double>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+    return hipGetLastError();
+}
+
 hipError_t launch_sw(const ColumnInputs &in, const SwOutputs &out, const SwWork &w, const SwDevTables &t, int mode,
                      hipStream_t s, hipEvent_t *ev)
 {

@@ -291,7 +291,11 @@ __device__ V sw_sfluxzen_v(int jb, const SwBand &B, const Tab<Real> &T, const Sw
 // exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5), or exp(-x)
 // itself when tbl is NULL (the smooth FP64 mode, which also drops the
 // small-argument series 1 - x + x^2/2 that the table path uses below 0.06);
-// for a dual argument the derivative is that of exp(-x)
+// for a dual argument the derivative is minus the value returned, the
+// derivative of exp(-x') at the x' whose exponential the table holds: it keeps
+// d ln E / dx = -1 exactly, so a product of transmittances (the direct beam
+// through many layers at low sun, where the table's spacing in x reaches tens
+// of percent of a transmittance) has the log-derivative of the exact product
 template <typename F, typename Real>
 __device__ __forceinline__ F tbl_exp(F x, const Real *__restrict__ tbl)
 {
@@ -302,8 +306,7 @@ __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *_
 {
     Dual<F, N> r;
     r.v = tbl ? (F)tbl[tbl_index(x.v)] : exp(-x.v);
-    const F s = -exp(-x.v);
-    for (int i = 0; i < N; ++i) r.d[i] = s * x.d[i];
+    for (int i = 0; i < N; ++i) r.d[i] = -r.v * x.d[i];
     return r;
 }
 

@@ -11,16 +11,20 @@
 //    cotangents after folding the heating-rate cotangent into the fluxes.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
 //    pass 1, the adding method's bottom-up sweep (values; reflectances of the
-//    atmosphere below each level kept in LDS); pass 2, top-down: the
-//    layer's optics are recomputed as dual numbers, the downward sweep runs on
-//    their values (its states kept in LDS), the level fluxes and their local
-//    adjoints are formed, and the reverse of the bottom-up sweep is carried
-//    downward, contracting each layer's adjoint with its optics derivatives;
-//    pass 3, bottom-up: reverse of the downward sweep, contracting again, plus
-//    the solar source's dependence on its reference layer.  Per-lane gradients
-//    are reduced over the band's g-points.
+//    atmosphere below each level kept in LDS); pass 2, top-down (values): the
+//    downward sweep (its states kept in a per-wave global scratch row), the
+//    level fluxes and their local adjoints, and the reverse of the bottom-up
+//    sweep, whose adjoints of each layer's (r, rd, t, td, dbt) go to the
+//    scratch row; pass 3, bottom-up: the reverse of the downward sweep adds
+//    its own layer adjoints, and the sum is projected once onto the layer's
+//    (T, q, O3) through its optics recomputed as dual numbers: gas/Rayleigh
+//    optical depths in (T, q, O3), two-stream optics in those two optical
+//    depths; plus the solar source's dependence on its reference layer.
+//    Per-lane gradients are reduced over the band's g-points.
 //  * finish (one thread per column): sums over bands in FP64.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 #include "rrtmg_device.hpp"
@@ -220,8 +224,8 @@ __device__ __forceinline__ void sw_seg_sum_n(Real (&v)[N], int lane, int ng, Rea
 
 template <typename Real>
 __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpWork w,
-                                                         const SwBand *__restrict__ bands, int nwaves, int nblocks,
-                                                         int exp_off, const Real *__restrict__ tab,
+                                                         const SwBand *__restrict__ bands, int nwaves, int blk0,
+                                                         int nblocks, int exp_off, const Real *__restrict__ tab,
                                                          const WaveUnit *__restrict__ units)
 {
     using V3 = Dual<Real, 3>;
@@ -230,8 +234,8 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     const int ncol = in.ncol, nlay = in.nlay;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int tile = blk / nwaves;
-    const WaveUnit u = units[blk % nwaves];
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
     const SwBand B = bands[u.band];
     const int jb = u.band + 16;
     const int lane = threadIdx.x;
@@ -241,9 +245,18 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     const bool active = cl < u.ncols && col0 < ncol && in.coszen[min(col0, ncol - 1)] >= 1.e-10;
     const int col = active ? col0 : 0;
     const size_t L1 = (size_t)(nlay + 1) * 64;
-    Real *s_prup = smem, *s_prupd = smem + L1;                  // bottom-up sweep
-    Real *s_ptdbt = smem + 2 * L1, *s_ztdn = smem + 3 * L1, *s_prdnd = smem + 4 * L1;  // downward sweep
-    Real *red = smem + 5 * L1;                                   // [4][64]
+    Real *s_prup = smem, *s_prupd = smem + L1;  // bottom-up sweep, LDS
+    Real *red = smem + 2 * L1;                   // [4][64]
+    // downward-sweep states, written top-down in pass 2 and read bottom-up in
+    // pass 3: this wave's row of the global scratch (LDS holds only the
+    // bottom-up states, so that LDS does not cap occupancy below the VGPRs)
+    Real *__restrict__ d_ptdbt = (Real *)w.dstate + (size_t)blk * (3 * L1 + (size_t)5 * nlay * 64);
+    Real *__restrict__ d_ztdn = d_ptdbt + L1, *__restrict__ d_prdnd = d_ptdbt + 2 * L1;
+    // adjoints of each layer's (r, rd, t, td, dbt) from the reverse of the
+    // bottom-up sweep (pass 2), [5][nlay][64]: pass 3 adds the reverse of the
+    // downward sweep's and projects the sum once onto (T, q, O3)
+    Real *__restrict__ d_adj = d_ptdbt + 3 * L1;
+    const size_t LA = (size_t)nlay * 64;
     const Real *dco = (const Real *)w.dcoef;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     Tab<Real> T{tab, ng, g};
@@ -289,19 +302,42 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         prupd_n = prupd;
     }
 
-    // layer optics of layer lay as dual numbers in (T, q, O3) of that layer
-    auto optics3 = [&](int lay, V3 &r3, V3 &rd3, V3 &t3, V3 &td3, V3 &dbt3) {
+    // Layer lay's gas and Rayleigh optical depths as dual numbers in (T, q,
+    // O3) of that layer, and its two-stream optics as dual numbers in those
+    // two optical depths: the optics' adjoint is projected onto (taug, taur)
+    // and then onto (T, q, O3), cheaper than carrying three directions
+    // through reftra.
+    using V2 = Dual<Real, 2>;
+    struct LayerD {
+        V2 r, rd, t, td, dbt;
+        V3 tg, tr;
+    };
+    auto optics_d = [&](int lay, LayerD &o) {
         SwLayer<V3> c;
         sw_load_layer_d<V3, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
-        V3 tg3, tr3;
-        sw_gas<Real, V3>(jb, B, T, c, lay <= laytrop, tg3, tr3);
-        sw_layer_optics<Real, V3, Dual<double, 3>>(tg3, tr3, prmu0, exp_tbl, r3, rd3, t3, td3, dbt3);
+        sw_gas<Real, V3>(jb, B, T, c, lay <= laytrop, o.tg, o.tr);
+        sw_layer_optics<Real, V2, Dual<double, 2>>(V2::seed(o.tg.v, 0), V2::seed(o.tr.v, 1), prmu0, exp_tbl, o.r,
+                                                   o.rd, o.t, o.td, o.dbt);
     };
-    auto contract = [](Real (&v)[3], Real ar, Real ard, Real at, Real atd, Real adbt, const V3 &r3, const V3 &rd3,
-                       const V3 &t3, const V3 &td3, const V3 &dbt3) {
-        for (int i = 0; i < 3; ++i)
-            v[i] = ar * r3.d[i] + ard * rd3.d[i] + at * t3.d[i] + atd * td3.d[i] + adbt * dbt3.d[i];
+    auto inactive_d = [](LayerD &o) {
+        o.r = V2((Real)0);
+        o.rd = V2((Real)0);
+        o.t = V2((Real)1);
+        o.td = V2((Real)1);
+        o.dbt = V2((Real)1);
+        o.tg = V3((Real)0);
+        o.tr = V3((Real)0);
     };
+    auto contract = [](Real (&v)[3], Real ar, Real ard, Real at, Real atd, Real adbt, const LayerD &o) {
+        Real a_tg = ar * o.r.d[0] + ard * o.rd.d[0] + at * o.t.d[0] + atd * o.td.d[0] + adbt * o.dbt.d[0];
+        Real a_tr = ar * o.r.d[1] + ard * o.rd.d[1] + at * o.t.d[1] + atd * o.td.d[1] + adbt * o.dbt.d[1];
+        for (int i = 0; i < 3; ++i) v[i] = a_tg * o.tg.d[i] + a_tr * o.tr.d[i];
+    };
+
+    // band fluxes of the recomputed forward, when asked for (as the forward
+    // band kernel writes them; uniform branch)
+    Real *bf_up = w.bflux ? (Real *)w.bflux + (size_t)u.band * 2 * n3 : nullptr;
+    Real *bf_dn = bf_up ? bf_up + n3 : nullptr;
 
     // pass 2: top-down
     Real ztdn = (Real)1, prdnd = (Real)0, ptdbt = (Real)1;
@@ -311,9 +347,14 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         const Real pu = s_prup[(size_t)(jk - 1) * 64 + lane], pud = s_prupd[(size_t)(jk - 1) * 64 + lane];
         if (jk > 1) {
             const int lay = nlay + 2 - jk;
-            V3 r3(0), rd3(0), t3(1), td3(1), dbt3(1);
-            if (active) optics3(lay, r3, rd3, t3, td3, dbt3);
-            const Real r = r3.v, rd = rd3.v, t = t3.v, td = td3.v, dbt = dbt3.v;
+            Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
+            if (active) {
+                SwLayer<Real> c;
+                sw_load_layer_d<Real, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+                Real taug, taur;
+                sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
+                sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
+            }
             // reverse of the bottom-up step that formed prup/prupd(jk-1) from level jk
             const Real Z = (Real)1 / ((Real)1 - pud * rd);
             const Real A = (t - dbt) * pud + dbt * pu;
@@ -324,16 +365,12 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
             const Real ard = a_pu * td * A * pud * Z * Z + a_pud * ((Real)1 + td * td * pud * pud * Z * Z);
             c_pu = a_pu * td * dbt * Z;
             c_pud = a_pu * (td * (t - dbt) * Z + td * A * rd * Z * Z) + a_pud * (td * td * Z + td * td * pud * rd * Z * Z);
-            Real v[3];
-            contract(v, ar, ard, at, atd, adbt, r3, rd3, t3, td3, dbt3);
-            if (!active) v[0] = v[1] = v[2] = (Real)0;
-            sw_seg_sum_n<3>(v, lane, ng, red);
-            if (active && g == 0) {
-                const size_t ix = (size_t)(lay - 1) * ncol + col;
-                gpart[ix] = v[0];
-                gpart[n2 + ix] = v[1];
-                gpart[2 * n2 + ix] = v[2];
-            }
+            const size_t ka = (size_t)(lay - 1) * 64 + lane;
+            d_adj[ka] = ar;
+            d_adj[LA + ka] = ard;
+            d_adj[2 * LA + ka] = at;
+            d_adj[3 * LA + ka] = atd;
+            d_adj[4 * LA + ka] = adbt;
             // downward step (values), as the forward
             const Real zreflect = (Real)1 / ((Real)1 - rd * prdnd);
             const Real ztdn_n = ptdbt * t + (td * ((ztdn - ptdbt) + ptdbt * r * prdnd)) * zreflect;
@@ -342,9 +379,9 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
             ztdn = ztdn_n;
             prdnd = prdnd_n;
         }
-        s_ptdbt[(size_t)(jk - 1) * 64 + lane] = ptdbt;
-        s_ztdn[(size_t)(jk - 1) * 64 + lane] = ztdn;
-        s_prdnd[(size_t)(jk - 1) * 64 + lane] = prdnd;
+        d_ptdbt[(size_t)(jk - 1) * 64 + lane] = ptdbt;
+        d_ztdn[(size_t)(jk - 1) * 64 + lane] = ztdn;
+        d_prdnd[(size_t)(jk - 1) * 64 + lane] = prdnd;
         // level fluxes and their local adjoints with respect to prup/prupd(jk)
         const Real zr = (Real)1 / ((Real)1 - prdnd * pud);
         const Real pfu = (ptdbt * pu + (ztdn - ptdbt) * pud) * zr;
@@ -354,6 +391,14 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         const Real gd = active ? cd[(size_t)lev * ncol + col] : (Real)0;
         const Real a_pfu = zincflx * gu, a_pfd = zincflx * gd;
         a_zinc += gu * pfu + gd * pfd;
+        if (bf_up) {
+            const Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, ng, red);
+            const Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, ng, red);
+            if (active && g == 0) {
+                bf_up[(size_t)lev * ncol + col] = su;
+                bf_dn[(size_t)lev * ncol + col] = sd;
+            }
+        }
         a_pu = c_pu + a_pfu * ptdbt * zr + a_pfd * ptdbt * prdnd * zr;
         a_pud = c_pud + a_pfu * ((ztdn - ptdbt) * zr + pfu * prdnd * zr) + a_pfd * (pfd - ptdbt) * prdnd * zr;
     }
@@ -369,12 +414,18 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
 
     // pass 3: bottom-up reverse of the downward sweep
     Real A_b = (Real)0, A_z = (Real)0, A_p = (Real)0;  // adjoints of ptdbt, ztdn, prdnd at level jk
+    // downward-sweep states at level jk, carried from the previous iteration
+    Real pb_c = d_ptdbt[(size_t)nlay * 64 + lane], pz_c = d_ztdn[(size_t)nlay * 64 + lane],
+         pp_c = d_prdnd[(size_t)nlay * 64 + lane];
     for (int jk = nlay + 1; jk >= 2; --jk) {
+        // level jk-1's states, loaded ahead of their use in the reverse step
+        const size_t k0 = (size_t)(jk - 2) * 64 + lane;
+        const Real pb = d_ptdbt[k0], pz = d_ztdn[k0], pp = d_prdnd[k0];
         {
             // local flux adjoints at level jk
             const size_t k = (size_t)(jk - 1) * 64 + lane;
             const Real pu = s_prup[k], pud = s_prupd[k];
-            const Real pb = s_ptdbt[k], pz = s_ztdn[k], pp = s_prdnd[k];
+            const Real pb = pb_c, pz = pz_c, pp = pp_c;
             const Real zr = (Real)1 / ((Real)1 - pp * pud);
             const Real pfu = (pb * pu + (pz - pb) * pud) * zr;
             const Real pfd = pb + (pz - pb + pb * pu * pp) * zr;
@@ -387,11 +438,13 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
             A_p += a_pfu * pfu * pud * zr + a_pfd * (pb * pu * zr + (pfd - pb) * pud * zr);
         }
         const int lay = nlay + 2 - jk;
-        V3 r3(0), rd3(0), t3(1), td3(1), dbt3(1);
-        if (active) optics3(lay, r3, rd3, t3, td3, dbt3);
-        const Real r = r3.v, rd = rd3.v, t = t3.v, td = td3.v, dbt = dbt3.v;
-        const size_t k0 = (size_t)(jk - 2) * 64 + lane;  // level jk-1
-        const Real pb = s_ptdbt[k0], pz = s_ztdn[k0], pp = s_prdnd[k0];
+        LayerD o;
+        if (active) optics_d(lay, o);
+        else inactive_d(o);
+        const Real r = o.r.v, rd = o.rd.v, t = o.t.v, td = o.td.v, dbt = o.dbt.v;
+        pb_c = pb;
+        pz_c = pz;
+        pp_c = pp;
         const Real Zr = (Real)1 / ((Real)1 - rd * pp);
         const Real Bz = (pz - pb) + pb * r * pp;
         const Real at = A_z * pb;
@@ -399,6 +452,9 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         const Real ar = A_z * td * pb * pp * Zr;
         const Real ard = A_z * td * Bz * pp * Zr * Zr + A_p * ((Real)1 + td * td * pp * pp * Zr * Zr);
         const Real adbt = A_b * pb;
+        const size_t ka = (size_t)(lay - 1) * 64 + lane;
+        const Real ar2 = d_adj[ka], ard2 = d_adj[LA + ka], at2 = d_adj[2 * LA + ka], atd2 = d_adj[3 * LA + ka],
+                   adbt2 = d_adj[4 * LA + ka];
         const Real nA_b = A_z * (t + td * ((Real)-1 + r * pp) * Zr) + A_b * dbt;
         const Real nA_z = A_z * td * Zr;
         const Real nA_p = A_z * (td * pb * r * Zr + td * Bz * rd * Zr * Zr) + A_p * (td * td * Zr + td * td * pp * rd * Zr * Zr);
@@ -406,16 +462,16 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         A_z = nA_z;
         A_p = nA_p;
         Real v[3];
-        contract(v, ar, ard, at, atd, adbt, r3, rd3, t3, td3, dbt3);
+        contract(v, ar + ar2, ard + ard2, at + at2, atd + atd2, adbt + adbt2, o);
         if (lay == lsf)
             for (int i = 0; i < 3; ++i) v[i] += a_sfz * sfz3.d[i];
         if (!active) v[0] = v[1] = v[2] = (Real)0;
         sw_seg_sum_n<3>(v, lane, ng, red);
         if (active && g == 0) {
             const size_t ix = (size_t)(lay - 1) * ncol + col;
-            gpart[ix] += v[0];
-            gpart[n2 + ix] += v[1];
-            gpart[2 * n2 + ix] += v[2];
+            gpart[ix] = v[0];
+            gpart[n2 + ix] = v[1];
+            gpart[2 * n2 + ix] = v[2];
         }
     }
 }
@@ -447,7 +503,8 @@ __global__ __launch_bounds__(256) void sw_vjp_finish_kernel(ColumnInputs in, SwV
 
 template <typename Real>
 static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                                  const SwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev)
+                                  const SwDevTables &t, bool smooth, hipStream_t s, hipEvent_t *ev,
+                                  const SwOutputs *fwd, int mode)
 {
     const int nb = (in.ncol + 255) / 256;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -456,19 +513,28 @@ static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
     const size_t shmem = sw_vjp_lds_bytes(in.nlay, sizeof(Real));
     const int nblocks = ntiles * t.desc.nwaves;
     if (ev) (void)hipEventRecord(ev[1], s);
-    hipLaunchKernelGGL(sw_vjp_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    // launches of at most w.chunk wave units, each reusing the scratch rows
+    for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
+        const int nb_c = std::min(w.chunk, nblocks - b0);
+        hipLaunchKernelGGL(sw_vjp_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), shmem, s, in, w, t.bands,
+                           t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
+    if (fwd) {
+        const hipError_t e = launch_sw_finish(in, *fwd, w.bflux, t, mode, s);
+        if (e != hipSuccess) return e;
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
 
 hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
-                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev)
+                         const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev, const SwOutputs *fwd)
 {
-    return mode == 1 ? launch_sw_vjp_t<float>(in, cot, out, w, t, false, s, ev)
-                     : launch_sw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev);
+    if (fwd && !w.bflux) return hipErrorInvalidValue;
+    return mode == 1 ? launch_sw_vjp_t<float>(in, cot, out, w, t, false, s, ev, fwd, mode)
+                     : launch_sw_vjp_t<double>(in, cot, out, w, t, mode == 2, s, ev, fwd, mode);
 }
 
 }  // namespace rrtmg

@@ -162,48 +162,58 @@ def _grad_outputs(dev, like, shapes):
     return [np.empty(sh) for sh in shapes]
 
 
+def _ptrs(dev, arrays):
+    return [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in arrays]
+
+
 def rrtmg_lw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
                  cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, uflx_bar=None, dflx_bar=None, hr_bar=None,
-                 tauaer=None, stream=None):
+                 tauaer=None, stream=None, forward=False):
     """Backward of rrtmg_lw (no reference counterpart; C ABI rrtmg_hip_lw_vjp).
 
     Takes rrtmg_lw's inputs and the cotangents of uflx, dflx ((nlay+1, ncol))
     and hr ((nlay, ncol)), each None for zero.  Returns the gradients
     (tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar) of
-    sum(uflx_bar*uflx + dflx_bar*dflx + hr_bar*hr)."""
+    sum(uflx_bar*uflx + dflx_bar*dflx + hr_bar*hr).  With forward=True
+    (C ABI rrtmg_hip_lw_fwd_vjp) the tuple starts with the forward's (uflx,
+    dflx, hr), computed in the same pass."""
     dev = _is_torch(play) and play.is_cuda
     a = _Args(dev)
     outs = _grad_outputs(dev, play, [(nlay, ncol), (nlay + 1, ncol), (nlay, ncol), (nlay, ncol), (ncol,)])
-    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
-    rc = lib().rrtmg_hip_lw_vjp(
-        int(ncol), int(nlay),
-        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
-                             cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar)],
-        *optr, DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    fwd = _grad_outputs(dev, play, [(nlay + 1, ncol), (nlay + 1, ncol), (nlay, ncol)]) if forward else []
+    args = [int(ncol), int(nlay),
+            *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                                 cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar)],
+            *_ptrs(dev, fwd), *_ptrs(dev, outs), DEVICE if dev else HOST,
+            _stream_ptr(stream) if dev else ctypes.c_void_p(0)]
+    rc = (lib().rrtmg_hip_lw_fwd_vjp if forward else lib().rrtmg_hip_lw_vjp)(*args)
     _check(rc)
-    return tuple(outs)
+    return tuple(fwd + outs)
 
 
 def rrtmg_sw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
                  asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx_bar=None, swdflx_bar=None,
-                 swhr_bar=None, stream=None):
+                 swhr_bar=None, stream=None, forward=False):
     """Backward of rrtmg_sw (no reference counterpart; C ABI rrtmg_hip_sw_vjp).
 
     Returns (tlay_bar, h2ovmr_bar, o3vmr_bar) of
-    sum(swuflx_bar*swuflx + swdflx_bar*swdflx + swhr_bar*swhr)."""
+    sum(swuflx_bar*swuflx + swdflx_bar*swdflx + swhr_bar*swhr).  With
+    forward=True (C ABI rrtmg_hip_sw_fwd_vjp) the tuple starts with the
+    forward's (swuflx, swdflx, swhr), computed in the same pass."""
     dev = _is_torch(play) and play.is_cuda
     a = _Args(dev)
     outs = _grad_outputs(dev, play, [(nlay, ncol), (nlay, ncol), (nlay, ncol)])
-    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
-    rc = lib().rrtmg_hip_sw_vjp(
-        int(ncol), int(nlay),
-        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
-                             asdir, asdif, aldir, aldif, coszen)],
-        ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon),
-        *[a.inp(x) for x in (swuflx_bar, swdflx_bar, swhr_bar)],
-        *optr, DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    fwd = _grad_outputs(dev, play, [(nlay + 1, ncol), (nlay + 1, ncol), (nlay, ncol)]) if forward else []
+    args = [int(ncol), int(nlay),
+            *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                                 asdir, asdif, aldir, aldif, coszen)],
+            ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon),
+            *[a.inp(x) for x in (swuflx_bar, swdflx_bar, swhr_bar)],
+            *_ptrs(dev, fwd), *_ptrs(dev, outs), DEVICE if dev else HOST,
+            _stream_ptr(stream) if dev else ctypes.c_void_p(0)]
+    rc = (lib().rrtmg_hip_sw_fwd_vjp if forward else lib().rrtmg_hip_sw_vjp)(*args)
     _check(rc)
-    return tuple(outs)
+    return tuple(fwd + outs)
 
 
 SW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
@@ -236,18 +246,21 @@ def lw_columns(c, tauaer=None, stream=None):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
 
 
-def lw_vjp_columns(c, uflx_bar=None, dflx_bar=None, hr_bar=None, tauaer=None, stream=None):
-    """rrtmg_lw_vjp on a mima_amd.columns dict; returns a dict of gradients."""
+def lw_vjp_columns(c, uflx_bar=None, dflx_bar=None, hr_bar=None, tauaer=None, stream=None, forward=False):
+    """rrtmg_lw_vjp on a mima_amd.columns dict; returns a dict of gradients
+    (and, with forward=True, of the forward's uflx, dflx, hr)."""
     out = rrtmg_lw_vjp(c["ncol"], c["nlay"], *[c[k] for k in LW_KEYS], uflx_bar=uflx_bar, dflx_bar=dflx_bar,
-                       hr_bar=hr_bar, tauaer=tauaer, stream=stream)
-    return dict(zip(("tlay", "tlev", "h2ovmr", "o3vmr", "tsfc"), out))
+                       hr_bar=hr_bar, tauaer=tauaer, stream=stream, forward=forward)
+    return dict(zip((("uflx", "dflx", "hr") if forward else ()) + ("tlay", "tlev", "h2ovmr", "o3vmr", "tsfc"), out))
 
 
-def sw_vjp_columns(c, swuflx_bar=None, swdflx_bar=None, swhr_bar=None, stream=None):
-    """rrtmg_sw_vjp on a mima_amd.columns dict; returns a dict of gradients."""
+def sw_vjp_columns(c, swuflx_bar=None, swdflx_bar=None, swhr_bar=None, stream=None, forward=False):
+    """rrtmg_sw_vjp on a mima_amd.columns dict; returns a dict of gradients
+    (and, with forward=True, of the forward's swuflx, swdflx, swhr)."""
     out = rrtmg_sw_vjp(c["ncol"], c["nlay"], *[c[k] for k in SW_KEYS], c["adjes"], c["dyofyr"], c["scon"],
-                       swuflx_bar=swuflx_bar, swdflx_bar=swdflx_bar, swhr_bar=swhr_bar, stream=stream)
-    return dict(zip(("tlay", "h2ovmr", "o3vmr"), out))
+                       swuflx_bar=swuflx_bar, swdflx_bar=swdflx_bar, swhr_bar=swhr_bar, stream=stream,
+                       forward=forward)
+    return dict(zip((("swuflx", "swdflx", "swhr") if forward else ()) + ("tlay", "h2ovmr", "o3vmr"), out))
 
 
 def debug_indices(lw, ncol, nlay):
