@@ -105,9 +105,10 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
  * hr_bar*hr)  with respect to tlay, h2ovmr, o3vmr (ncol, nlay), tlev
  * (ncol, nlay+1) and tsfc (ncol).  The forward is recomputed inside (no
  * state is kept from a previous rrtmg_hip_lw call); the arithmetic follows
- * the precision mode.  Derivatives of table lookups are those of the
- * tabulated functions; jp/jt/jt1 and the other table indices are held fixed
- * (piecewise differentiation). */
+ * the precision mode.  The derivatives are those of the smooth functions
+ * that the exp and tau-transition tables and the thin-layer series stand
+ * for, evaluated along the forward's values; jp/jt/jt1 and the other table
+ * indices are held fixed (piecewise differentiation). */
 int rrtmg_hip_lw_vjp(int ncol, int nlay,
                      const double *play, const double *plev, const double *tlay,
                      const double *tlev, const double *tsfc,
@@ -124,9 +125,7 @@ int rrtmg_hip_lw_vjp(int ncol, int nlay,
 /* Backward of rrtmg_sw (no reference counterpart), as rrtmg_hip_lw_vjp:
  * cotangents of swuflx, swdflx, swhr (each NULL for zero); overwrites the
  * gradients with respect to tlay, h2ovmr and o3vmr (ncol, nlay).  Night
- * columns (coszen < 1e-10) get zero gradients, as their outputs are zero.
- * The exp(-x) table's derivative is minus the tabulated value (exact
- * log-derivative of each transmittance). */
+ * columns (coszen < 1e-10) get zero gradients, as their outputs are zero. */
 int rrtmg_hip_sw_vjp(int ncol, int nlay,
                      const double *play, const double *plev, const double *tlay,
                      const double *tlev, const double *tsfc,

@@ -722,7 +722,7 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
     const size_t o_ilev = off; off = align_up(off + sizeof(int) * n3);
     const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
-    const size_t o_cs = off; off = align_up(off + sizeof(double) * 2 * ncol);
+    const size_t o_cs = off; off = align_up(off + sizeof(double) * 4 * ncol);
     const size_t o_cot = off; off = align_up(off + rs * 2 * n3);
     const size_t o_gp = off; off = align_up(off + rs * 16 * 3 * n2);
     const size_t o_gl = off; off = align_up(off + rs * 16 * n3);

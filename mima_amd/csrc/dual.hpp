@@ -145,10 +145,12 @@ DUAL_T __host__ __device__ __forceinline__ D operator/(const D &a, F b)
 }
 DUAL_T __host__ __device__ __forceinline__ D operator/(F a, const D &b)
 {
+    // -(a/b) (db/b), associated so that no intermediate overflows where the
+    // result does not (a/b^2 does for b = 1e-20 in FP32)
     D r;
     r.v = a / b.v;
-    const F s = -r.v / b.v;
-    for (int i = 0; i < N; ++i) r.d[i] = s * b.d[i];
+    const F ib = (F)1 / b.v;
+    for (int i = 0; i < N; ++i) r.d[i] = -r.v * (b.d[i] * ib);
     return r;
 }
 DUAL_T __host__ __device__ __forceinline__ D &operator+=(D &a, const D &b) { return a = a + b; }

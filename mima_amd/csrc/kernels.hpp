@@ -77,7 +77,7 @@ struct LwVjpWork {
     int *ipk = nullptr;       // [nlay][ncol]
     int *ilev = nullptr;      // [nlay+1][ncol]
     int *laytrop = nullptr;   // [ncol]
-    double *colscal = nullptr;// [2][ncol]: pwvcm, tbound
+    double *colscal = nullptr;// [4][ncol]: pwvcm, tbound, column water and dry-air totals
     void *cot = nullptr;      // [2][nlay+1][ncol] of Real: up / down flux cotangents incl. heating
     void *gpart = nullptr;    // [16][3][nlay][ncol] of Real: per-band gradient (T, q, O3)
     void *glev = nullptr;     // [16][nlay+1][ncol] of Real: per-band gradient wrt level temperature
@@ -100,9 +100,9 @@ struct SwVjpWork {
 // The backward band kernels keep the states of the sweep that runs in the
 // other direction from its reverse in a global scratch row per wave unit
 // (LDS keeps the rest); a launch covers at most `chunk` wave units so the
-// scratch stays within VJP_SCRATCH_BYTES, and longer grids are launched in
-// chunks that reuse the rows.
-constexpr size_t VJP_SCRATCH_BYTES = (size_t)512 << 20;
+// scratch stays within VJP_SCRATCH_BYTES (2 GiB), and longer grids are
+// launched in chunks that reuse the rows.
+constexpr size_t VJP_SCRATCH_BYTES = (size_t)2 << 30;
 inline int vjp_chunk(int units, size_t bytes_per_unit)
 {
     size_t c = VJP_SCRATCH_BYTES / bytes_per_unit;

@@ -97,14 +97,18 @@ __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__res
 
 // ---------------------------------------------------------------------------
 // prep: the forward prep's arithmetic (lw.hip, rrtmg_lw_rad.nomcica.f90:729-852,
-// rrtmg_lw_setcoef.f90:150-413) on dual numbers
+// rrtmg_lw_setcoef.f90:150-413) on dual numbers, one thread per (layer,
+// column): the layer's coefficients with their derivatives, the Planck index
+// of the level above it (and of the surface level), and the radiance
+// cotangents of its lower level (and of the top level)
 template <typename Real>
 __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCot cot, LwVjpWork w,
                                                           const LwConst *__restrict__ cst, double heatfac)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     Real *dco = (Real *)w.dcoef;
     Real *tlevf = (Real *)w.tlev;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
@@ -118,15 +122,13 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
         dco[b + 2 * fs] = (Real)x.d[1];
         dco[b + 3 * fs] = (Real)x.d[2];
     };
-    double amttl = 0.0, wvttl = 0.0;
-    int laytrop = 0;
-    {
+    if (lay == 0) {
         double tz0 = in.tlev[col];
         int i0 = clampi((int)(tz0 - 159.0), 1, 180);
         w.ilev[col] = i0;
         tlevf[col] = (Real)(tz0 - 159.0 - (double)i0);
     }
-    for (int lay = 0; lay < nlay; ++lay) {
+    {
         const size_t ix = (size_t)lay * ncol + col;
         const double pavel = in.play[ix];
         const D3 tavel = D3::seed(in.tlay[ix], 0);
@@ -147,8 +149,6 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
         summol = summol + wkl7;
         const D3 wbrodl = coldry * (1.0 - summol);
         wkl1 = coldry * wkl1;
-        amttl = amttl + coldry.v + wkl1.v;
-        wvttl = wvttl + wkl1.v;
         const D3 wx1 = coldry * in.ccl4[ix] * 1.e-20, wx2 = coldry * in.cfc11[ix] * 1.e-20;
         const D3 wx3 = coldry * in.cfc12[ix] * 1.e-20, wx4 = coldry * in.cfc22[ix] * 1.e-20;
 
@@ -174,7 +174,6 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
         D3 forfrac, selffac = water * forfac, selffrac(0.0);
         int indfor, indself = 0;
         if (plog > 4.56) {
-            ++laytrop;
             D3 factor = (332.0 - tavel) / 36.0;
             indfor = clampi((int)factor.v, 1, 2);
             forfrac = factor - (double)indfor;
@@ -233,16 +232,12 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
         put(LWF_TLAYFRAC, ix, tlayfrac);
         w.ipk[ix] = pack_lw(jp, jt, jt1, indself, indfor, indminor, indlay);
     }
-    const double wvsh = (amw * wvttl) / (amd * amttl);
-    w.laytrop[col] = laytrop;
-    w.colscal[col] = wvsh * (1.e3 * in.plev[col]) / (1.e2 * grav);
-    w.colscal[(size_t)ncol + col] = in.tsfc[col];
 
     // level radiance cotangents: fluxes' own cotangents plus the heating rate
     // hr(k) = heatfac (fnet(k) - fnet(k+1)) / (p(k) - p(k+1)), fnet = up - dn
-    // (rrtmg_lw_rtrnmr.f90:767-772)
+    // (rrtmg_lw_rtrnmr.f90:767-772); this thread's level lay, and the top
     Real *cu = (Real *)w.cot, *cd = cu + (size_t)(nlay + 1) * ncol;
-    for (int k = 0; k <= nlay; ++k) {
+    for (int k = lay; k <= (lay == nlay - 1 ? nlay : lay); ++k) {
         const size_t ix = (size_t)k * ncol + col;
         double fbar = 0.0;
         if (cot.hr) {
@@ -252,6 +247,36 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
         cu[ix] = (Real)((cot.uflx ? cot.uflx[ix] : 0.0) + fbar);
         cd[ix] = (Real)((cot.dflx ? cot.dflx[ix] : 0.0) - fbar);
     }
+}
+
+// per-column part of the prep (one thread per column): laytrop, the column
+// water and dry-air totals (the forward's arithmetic and summation order),
+// precipitable water and surface temperature
+__global__ __launch_bounds__(256) void lw_vjp_column_kernel(ColumnInputs in, LwVjpWork w)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, avogad = 6.02214199e+23, grav = 9.8066;
+    double amttl = 0.0, wvttl = 0.0;
+    int laytrop = 0;
+    for (int lay = 0; lay < nlay; ++lay) {
+        const size_t ix = (size_t)lay * ncol + col;
+        const double h = in.h2o[ix], pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
+        double wkl1 = (h / (1.0 - h)) * amdw;
+        const double amm = (1.0 - wkl1) * amd + wkl1 * amw;
+        const double coldry = ((pz0 - pz1) * 1.e3 * avogad) / (1.e2 * grav * amm * (1.0 + wkl1));
+        wkl1 = coldry * wkl1;
+        amttl = amttl + coldry + wkl1;
+        wvttl = wvttl + wkl1;
+        if (log(in.play[ix]) > 4.56) ++laytrop;
+    }
+    const double wvsh = (amw * wvttl) / (amd * amttl);
+    w.laytrop[col] = laytrop;
+    w.colscal[col] = wvsh * (1.e3 * in.plev[col]) / (1.e2 * grav);
+    w.colscal[(size_t)ncol + col] = in.tsfc[col];
+    w.colscal[(size_t)2 * ncol + col] = wvttl;
+    w.colscal[(size_t)3 * ncol + col] = amttl;
 }
 
 // sum of N values over the ng adjacent lanes of each column group (valid on
@@ -272,9 +297,10 @@ __device__ __forceinline__ void seg_sum_n(Real (&v)[N], int lane, int ng, Real *
 }
 
 // transmittance and sources of one layer (rrtmg_lw_rtrnmr.f90:591-606) and,
-// for the backward, their derivatives with respect to the optical depth: the
-// reference's thin-layer series below 0.06 and the tables above, or (smooth
-// mode, tables NULL) the tabulated functions themselves at every depth
+// for the backward, their derivatives with respect to the optical depth.
+// Values: the reference's thin-layer series below 0.06 and the tables above,
+// or (smooth mode, tables NULL) the tabulated functions themselves at every
+// depth; derivatives: always those of the smooth functions
 template <typename Real>
 struct LwLayerRT {
     Real od, atrans, tf, datr, dtf;
@@ -293,21 +319,23 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
         r.tf = x.tf;
         r.datr = x.datr;
         r.dtf = x.dtf;
-    } else if (od <= (Real)0.06) {
-        r.atrans = od - (Real)0.5 * od * od;
-        r.tf = (Real)0.166667 * od;
-        r.datr = (Real)1 - od;
-        r.dtf = (Real)0.166667;
     } else {
-        const int itr = tbl_index(od);
-        r.atrans = (Real)1 - exp_tbl[itr];
-        r.tf = tfn_tbl[itr];
+        if (od <= (Real)0.06) {
+            r.atrans = od - (Real)0.5 * od * od;
+            r.tf = (Real)0.166667 * od;
+        } else {
+            const int itr = tbl_index(od);
+            r.atrans = (Real)1 - exp_tbl[itr];
+            r.tf = tfn_tbl[itr];
+        }
         r.datr = r.dtf = (Real)0;
         if (derivs) {
-            // d/dtau of 1 - exp(-tau) and of 1 - 2 (1/tau - e/(1 - e)), e = exp(-tau)
-            const Real e = exp(-od), ome = (Real)1 - e;
-            r.datr = e;
-            r.dtf = (Real)2 / (od * od) - (Real)2 * e / (ome * ome);
+            // the derivatives of the smooth functions that the series and the
+            // tables stand for, 1 - exp(-tau) and the tau transition
+            // function, at the same optical depth
+            const LwRT<Real> x = lw_rt_exact(od, true);
+            r.datr = x.datr;
+            r.dtf = x.dtf;
         }
     }
     return r;
@@ -527,13 +555,16 @@ __global__ __launch_bounds__(64) void lw_vjp_band_kernel(ColumnInputs in, LwVjpW
 }
 
 // finish: band sums and the precipitable-water chain of the diffusivity
-// angle (pwvcm depends on the water vapour of every layer)
+// angle (pwvcm depends on the water vapour of every layer), one thread per
+// (layer, column); the thread of layer 0 also writes the surface gradient and
+// that of the top layer the top level's
 template <typename Real>
 __global__ __launch_bounds__(256) void lw_vjp_finish_kernel(ColumnInputs in, LwVjpWork w, LwGrad out)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
     const Real *gpart = (const Real *)w.gpart, *glev = (const Real *)w.glev, *gsc = (const Real *)w.gsc;
     double a_pw = 0.0, gts = 0.0;
@@ -541,8 +572,8 @@ __global__ __launch_bounds__(256) void lw_vjp_finish_kernel(ColumnInputs in, LwV
         gts += (double)gsc[(size_t)b * 2 * ncol + col];
         a_pw += (double)gsc[(size_t)b * 2 * ncol + ncol + col];
     }
-    out.tsfc[col] = gts;
-    for (int k = 0; k <= nlay; ++k) {
+    if (lay == 0) out.tsfc[col] = gts;
+    for (int k = lay; k <= (lay == nlay - 1 ? nlay : lay); ++k) {
         double s = 0.0;
         for (int b = 0; b < 16; ++b) s += (double)glev[(size_t)b * n3 + (size_t)k * ncol + col];
         out.tlev[(size_t)k * ncol + col] = s;
@@ -551,37 +582,26 @@ __global__ __launch_bounds__(256) void lw_vjp_finish_kernel(ColumnInputs in, LwV
     // coldry wkl1, amttl = sum coldry (1 + wkl1) (rrtmg_lw_rad.nomcica.f90:845-852)
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, avogad = 6.02214199e+23, grav = 9.8066;
     using D1 = Dual<double, 1>;
-    double wvttl = 0.0, amttl = 0.0;
-    for (int lay = 0; lay < nlay; ++lay) {
-        const size_t ix = (size_t)lay * ncol + col;
-        const double h = in.h2o[ix], pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
-        const double wkl1 = (h / (1.0 - h)) * amdw;
-        const double amm = (1.0 - wkl1) * amd + wkl1 * amw;
-        const double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1.0 + wkl1));
-        wvttl += coldry * wkl1;
-        amttl += coldry + coldry * wkl1;
-    }
+    const double wvttl = w.colscal[(size_t)2 * ncol + col], amttl = w.colscal[(size_t)3 * ncol + col];
     const double cpw = (amw / amd) * (1.e3 * in.plev[col]) / (1.e2 * grav);
-    for (int lay = 0; lay < nlay; ++lay) {
-        const size_t ix = (size_t)lay * ncol + col;
-        const D1 h = D1::seed(in.h2o[ix], 0);
-        const double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
-        const D1 wkl1 = (h / (1.0 - h)) * amdw;
-        const D1 amm = (1.0 - wkl1) * amd + wkl1 * amw;
-        const D1 coldry = ((pz0 - pz1) * 1.e3 * avogad) / (1.e2 * grav * amm * (1.0 + wkl1));
-        const D1 wv = coldry * wkl1, am = coldry + coldry * wkl1;
-        const double dpw = cpw * (wv.d[0] * amttl - wvttl * am.d[0]) / (amttl * amttl);
-        double gt = 0.0, gq = 0.0, go = 0.0;
-        for (int b = 0; b < 16; ++b) {
-            const Real *gp = gpart + (size_t)b * 3 * n2;
-            gt += (double)gp[ix];
-            gq += (double)gp[n2 + ix];
-            go += (double)gp[2 * n2 + ix];
-        }
-        out.tlay[ix] = gt;
-        out.h2o[ix] = gq + a_pw * dpw;
-        out.o3[ix] = go;
+    const size_t ix = (size_t)lay * ncol + col;
+    const D1 h = D1::seed(in.h2o[ix], 0);
+    const double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
+    const D1 wkl1 = (h / (1.0 - h)) * amdw;
+    const D1 amm = (1.0 - wkl1) * amd + wkl1 * amw;
+    const D1 coldry = ((pz0 - pz1) * 1.e3 * avogad) / (1.e2 * grav * amm * (1.0 + wkl1));
+    const D1 wv = coldry * wkl1, am = coldry + coldry * wkl1;
+    const double dpw = cpw * (wv.d[0] * amttl - wvttl * am.d[0]) / (amttl * amttl);
+    double gt = 0.0, gq = 0.0, go = 0.0;
+    for (int b = 0; b < 16; ++b) {
+        const Real *gp = gpart + (size_t)b * 3 * n2;
+        gt += (double)gp[ix];
+        gq += (double)gp[n2 + ix];
+        go += (double)gp[2 * n2 + ix];
     }
+    out.tlay[ix] = gt;
+    out.h2o[ix] = gq + a_pw * dpw;
+    out.o3[ix] = go;
 }
 
 template <typename Real>
@@ -590,8 +610,10 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
                                   const LwOutputs *fwd, int mode)
 {
     const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(lw_vjp_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, cot, w, t.cst, t.heatfac);
+    hipLaunchKernelGGL(lw_vjp_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cot, w, t.cst, t.heatfac);
+    hipLaunchKernelGGL(lw_vjp_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = lw_vjp_lds_bytes(in.nlay, sizeof(Real));
     const int nblocks = ntiles * t.desc.nwaves;
@@ -604,7 +626,7 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
                            t.desc.totplnk, (const Real *)t.tab, t.cst, t.units);
     }
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(lw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
+    hipLaunchKernelGGL(lw_vjp_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out);
     if (fwd) {
         const hipError_t e = launch_lw_finish(in, *fwd, w.bflux, t, mode, s);
         if (e != hipSuccess) return e;

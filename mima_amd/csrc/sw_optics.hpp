@@ -288,25 +288,26 @@ __device__ V sw_sfluxzen_v(int jb, const SwBand &B, const Tab<Real> &T, const Sw
     }
 }
 
-// exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5), or exp(-x)
-// itself when tbl is NULL (the smooth FP64 mode, which also drops the
-// small-argument series 1 - x + x^2/2 that the table path uses below 0.06);
-// for a dual argument the derivative is minus the value returned, the
-// derivative of exp(-x') at the x' whose exponential the table holds: it keeps
-// d ln E / dx = -1 exactly, so a product of transmittances (the direct beam
-// through many layers at low sun, where the table's spacing in x reaches tens
-// of percent of a transmittance) has the log-derivative of the exact product
+// exp(-x) from the lookup table at int(1e4 x/(bpade + x) + 0.5), or, when tbl
+// is NULL, the function the table holds: exp(-x) floored at 1e-20 as the
+// table's entries are (rrtmg_sw_init.f90:94-104; the floor also keeps 1/E
+// finite in FP32).  The smooth FP64 mode and the backward's derivatives use
+// the latter, the smooth mode also without the small-argument series
+// 1 - x + x^2/2 that the table path uses below 0.06.  For a dual argument the
+// derivative is -E, zero on the floor.
 template <typename F, typename Real>
 __device__ __forceinline__ F tbl_exp(F x, const Real *__restrict__ tbl)
 {
-    return tbl ? (F)tbl[tbl_index(x)] : exp(-x);
+    return tbl ? (F)tbl[tbl_index(x)] : fmax(exp(-x), (F)1.e-20);
 }
 template <typename F, int N, typename Real>
 __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *__restrict__ tbl)
 {
     Dual<F, N> r;
-    r.v = tbl ? (F)tbl[tbl_index(x.v)] : exp(-x.v);
-    for (int i = 0; i < N; ++i) r.d[i] = -r.v * x.d[i];
+    const F e = tbl ? (F)tbl[tbl_index(x.v)] : exp(-x.v);
+    r.v = tbl ? e : fmax(e, (F)1.e-20);
+    const F s = e > (F)1.e-20 ? -e : (F)0;
+    for (int i = 0; i < N; ++i) r.d[i] = s * x.d[i];
     return r;
 }
 

@@ -79,15 +79,18 @@ __device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__res
 
 // ---------------------------------------------------------------------------
 // prep: the forward prep's arithmetic (sw.hip, rrtmg_sw_rad.nomcica.f90:933-1047,
-// rrtmg_sw_setcoef.f90:129-284) on dual numbers
+// rrtmg_sw_setcoef.f90:129-284) on dual numbers, one thread per (layer,
+// column): the layer's coefficients with their derivatives and the flux
+// cotangents of its lower level (and of the top level)
 template <typename Real>
 __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCot cot, SwVjpWork w,
                                                           const double *__restrict__ preflog,
                                                           const double *__restrict__ tref, double heatfac)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     Real *dco = (Real *)w.dcoef;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
@@ -100,9 +103,7 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
         dco[b + 2 * fs] = (Real)x.d[1];
         dco[b + 3 * fs] = (Real)x.d[2];
     };
-    int laytrop = 0;
-    int jp_store[MAX_NLAY + 1];
-    for (int lay = 0; lay < nlay; ++lay) {
+    {
         const size_t ix = (size_t)lay * ncol + col;
         const double pavel = in.play[ix];
         const D3 tavel = D3::seed(in.tlay[ix], 0);
@@ -136,7 +137,6 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
         D3 forfac, forfrac, selffac, selffrac;
         int indfor, indself;
         if (plog > 4.56) {
-            ++laytrop;
             forfac = scalefac / (1. + water);
             D3 factor = (332.0 - tavel) / 36.0;
             indfor = clampi((int)factor.v, 1, 2);
@@ -170,32 +170,12 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
         put(SWF_FORFAC, ix, forfac);
         put(SWF_FORFRAC, ix, forfrac);
         w.ipk[ix] = pack_sw(jp, jt, jt1, indself, indfor);
-        jp_store[lay] = jp;
-    }
-    w.laytrop[col] = laytrop;
-    // solar-source reference layers, as the forward prep
-    const int lowref[14] = {0, 0, 6, 3, 3, 8, 2, 6, 1, 2, 0, 0, 0, 0};
-    const int upref[14] = {18, 30, 0, 0, 0, 0, 0, 0, 0, 0, 0, 32, 58, 49};
-    for (int b = 0; b < 14; ++b) {
-        int lsf;
-        if (upref[b] > 0) {
-            lsf = nlay;
-            for (int lay = max(laytrop + 1, 2); lay <= nlay; ++lay)
-                if (jp_store[lay - 2] < upref[b] && jp_store[lay - 1] >= upref[b]) lsf = lay;
-        } else {
-            lsf = laytrop;
-            if (lowref[b] > 0)
-                for (int lay = 1; lay <= laytrop; ++lay) {
-                    const int jpn = lay < nlay ? jp_store[lay] : jp_store[nlay - 1];
-                    if (jp_store[lay - 1] < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
-                }
-        }
-        w.laysolfr[(size_t)b * ncol + col] = max(lsf, 1);
     }
     // level cotangents: swhr(k) = (net(k+1) - net(k)) heatfac / (p(k) - p(k+1)),
-    // net = dn - up, top layer zero (rrtmg_sw_rad.nomcica.f90:713-726)
+    // net = dn - up, top layer zero (rrtmg_sw_rad.nomcica.f90:713-726); this
+    // thread's level lay, and the top
     Real *cu = (Real *)w.cot, *cd = cu + (size_t)(nlay + 1) * ncol;
-    for (int k = 0; k <= nlay; ++k) {
+    for (int k = lay; k <= (lay == nlay - 1 ? nlay : lay); ++k) {
         const size_t ix = (size_t)k * ncol + col;
         double nbar = 0.0;
         if (cot.hr) {
@@ -204,6 +184,39 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
         }
         cu[ix] = (Real)((cot.uflx ? cot.uflx[ix] : 0.0) - nbar);
         cd[ix] = (Real)((cot.dflx ? cot.dflx[ix] : 0.0) + nbar);
+    }
+}
+
+// per-column part of the prep (one thread per column, after the per-layer
+// part, whose packed indices hold jp): laytrop and the solar-source reference
+// layers, as the forward prep
+__global__ __launch_bounds__(256) void sw_vjp_column_kernel(ColumnInputs in, SwVjpWork w)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    int laytrop = 0;
+    for (int lay = 0; lay < nlay; ++lay)
+        if (log(in.play[(size_t)lay * ncol + col]) > 4.56) ++laytrop;
+    w.laytrop[col] = laytrop;
+    auto jp_store = [&](int lay) { return unpack_jp(w.ipk[(size_t)lay * ncol + col]); };
+    const int lowref[14] = {0, 0, 6, 3, 3, 8, 2, 6, 1, 2, 0, 0, 0, 0};
+    const int upref[14] = {18, 30, 0, 0, 0, 0, 0, 0, 0, 0, 0, 32, 58, 49};
+    for (int b = 0; b < 14; ++b) {
+        int lsf;
+        if (upref[b] > 0) {
+            lsf = nlay;
+            for (int lay = max(laytrop + 1, 2); lay <= nlay; ++lay)
+                if (jp_store(lay - 2) < upref[b] && jp_store(lay - 1) >= upref[b]) lsf = lay;
+        } else {
+            lsf = laytrop;
+            if (lowref[b] > 0)
+                for (int lay = 1; lay <= laytrop; ++lay) {
+                    const int jpn = lay < nlay ? jp_store(lay) : jp_store(nlay - 1);
+                    if (jp_store(lay - 1) < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
+                }
+        }
+        w.laysolfr[(size_t)b * ncol + col] = max(lsf, 1);
     }
 }
 
@@ -316,8 +329,12 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         SwLayer<V3> c;
         sw_load_layer_d<V3, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
         sw_gas<Real, V3>(jb, B, T, c, lay <= laytrop, o.tg, o.tr);
-        sw_layer_optics<Real, V2, Dual<double, 2>>(V2::seed(o.tg.v, 0), V2::seed(o.tr.v, 1), prmu0, exp_tbl, o.r,
-                                                   o.rd, o.t, o.td, o.dbt);
+        // exp(-x) itself rather than the table: the backward differentiates
+        // the smooth functions the table stands for (the table's neighbouring
+        // entries are not consistent to the derivative's accuracy where
+        // reftra's k mu0 = 1 quotient is nearly 0/0)
+        sw_layer_optics<Real, V2, Dual<double, 2>>(V2::seed(o.tg.v, 0), V2::seed(o.tr.v, 1), prmu0,
+                                                   (const Real *)nullptr, o.r, o.rd, o.t, o.td, o.dbt);
     };
     auto inactive_d = [](LayerD &o) {
         o.r = V2((Real)0);
@@ -476,29 +493,27 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     }
 }
 
+// finish: band sums in FP64, one thread per (layer, column)
 template <typename Real>
 __global__ __launch_bounds__(256) void sw_vjp_finish_kernel(ColumnInputs in, SwVjpWork w, SwGrad out)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
     const size_t n2 = (size_t)nlay * ncol;
+    const size_t ix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= n2) return;
+    const int col = (int)(ix % ncol);
     const Real *gpart = (const Real *)w.gpart;
-    const bool day = in.coszen[col] >= 1.e-10;
-    for (int lay = 0; lay < nlay; ++lay) {
-        const size_t ix = (size_t)lay * ncol + col;
-        double gt = 0.0, gq = 0.0, go = 0.0;
-        if (day)
-            for (int b = 0; b < 14; ++b) {
-                const Real *gp = gpart + (size_t)b * 3 * n2;
-                gt += (double)gp[ix];
-                gq += (double)gp[n2 + ix];
-                go += (double)gp[2 * n2 + ix];
-            }
-        out.tlay[ix] = gt;
-        out.h2o[ix] = gq;
-        out.o3[ix] = go;
-    }
+    double gt = 0.0, gq = 0.0, go = 0.0;
+    if (in.coszen[col] >= 1.e-10)
+        for (int b = 0; b < 14; ++b) {
+            const Real *gp = gpart + (size_t)b * 3 * n2;
+            gt += (double)gp[ix];
+            gq += (double)gp[n2 + ix];
+            go += (double)gp[2 * n2 + ix];
+        }
+    out.tlay[ix] = gt;
+    out.h2o[ix] = gq;
+    out.o3[ix] = go;
 }
 
 template <typename Real>
@@ -507,8 +522,11 @@ static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
                                   const SwOutputs *fwd, int mode)
 {
     const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(sw_vjp_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, cot, w, t.preflog, t.tref, t.heatfac);
+    hipLaunchKernelGGL(sw_vjp_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cot, w, t.preflog, t.tref,
+                       t.heatfac);
+    hipLaunchKernelGGL(sw_vjp_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = sw_vjp_lds_bytes(in.nlay, sizeof(Real));
     const int nblocks = ntiles * t.desc.nwaves;
@@ -520,7 +538,7 @@ static hipError_t launch_sw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
                            t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
     }
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(sw_vjp_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out);
+    hipLaunchKernelGGL(sw_vjp_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out);
     if (fwd) {
         const hipError_t e = launch_sw_finish(in, *fwd, w.bflux, t, mode, s);
         if (e != hipSuccess) return e;

@@ -1,0 +1,103 @@
+"""The reference RRTMG Fortran, compiled from the MiMA sources by
+``oracle/build_ref.sh`` into ``oracle/_ref/``, driven from Python.
+
+Test infrastructure: the golden-vector scripts (``tests/golden/``) write their
+fixtures with it, and bench.py's CPU-baseline leg times it.  Nothing in the
+library imports it.
+
+* SW: ``librrtmg_sw_ref.so`` is the whole reference RRTMG_SW (init included)
+  plus ``ref_sw_driver.f90``; ``ref_sw_run`` calls ``rrtmg_sw``
+  (``rrtmg_sw_rad.nomcica.f90:78``) on ``(ncol, nlay)`` arrays.
+* LW: ``librrtmg_lw_ref.so`` is the reference LW column code plus
+  ``ref_lw_driver.f90``.  The reference's LW init cannot run (its AER k-table
+  data file is absent), so ``load_lw`` fills the module state it would set:
+  reduced k-tables from the oracle's reduction of the committed seeded LW
+  tables, band bookkeeping from the same file, then ``ref_lw_init`` (the
+  reference's own lwatmref/lwavplank).
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SW_LIB = os.path.join(HERE, "_ref", "librrtmg_sw_ref.so")
+LW_LIB = os.path.join(HERE, "_ref", "librrtmg_lw_ref.so")
+CPDAIR = 287.04 / (2.0 / 7.0)   # MiMA's cp_air (constants.F90: rdgas / kappa)
+D = ctypes.POINTER(ctypes.c_double)
+I = ctypes.POINTER(ctypes.c_int)
+SW_IN = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr",
+         "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")
+LW_IN = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr",
+         "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr", "ccl4vmr", "emis")
+# reduced-table names of the oracle -> the reference LW modules' names
+LW_RENAME = {"kao": "ka", "kbo": "kb", "selfrefo": "selfref", "forrefo": "forref", "fracrefao": "fracrefa",
+             "fracrefbo": "fracrefb", "ccl4o": "ccl4", "cfc11adjo": "cfc11adj", "cfc12o": "cfc12",
+             "cfc22adjo": "cfc22adj"}
+
+
+def _p(a):
+    return a.ctypes.data_as(D)
+
+
+def available():
+    return os.path.exists(SW_LIB) and os.path.exists(LW_LIB)
+
+
+def load_sw():
+    lib = ctypes.CDLL(SW_LIB)
+    lib.ref_sw_init(ctypes.c_double(CPDAIR))
+    return lib
+
+
+def run_sw(lib, c):
+    """rrtmg_sw of the reference on a mima_amd.columns dict (numpy arrays)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_sw_run(n, L, *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
+                   ctypes.c_double(c["scon"]), *[_p(a) for a in outs])
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
+def load_lw():
+    """Load the reference LW column code and fill its module state; returns
+    (lib, number of reduced tables set)."""
+    sys.path.insert(0, HERE)
+    sys.path.insert(0, ROOT)
+    import pyoracle as po
+    from mima_amd.tables.extract_tables import read_table
+    lib = ctypes.CDLL(LW_LIB)
+    lib.ref_lw_set.restype = ctypes.c_int
+    po.lib()  # oracle init: reduces the seeded tables
+    tabs, _, _ = read_table(po.LW_TABLE)
+    i32 = lambda k: np.ascontiguousarray(tabs[k], dtype=np.int32)  # noqa: E731
+    lib.ref_lw_set_bands(i32("lw/ngc").ctypes.data_as(I), i32("lw/nspa").ctypes.data_as(I),
+                         i32("lw/nspb").ctypes.data_as(I), _p(np.ascontiguousarray(tabs["lw/delwave"], np.float64)))
+    nset = 0
+    for name in sorted(tabs):
+        if not name.startswith("lw/b"):
+            continue
+        band, oname = int(name[4:6]), name[7:]
+        rname = LW_RENAME.get(oname) or oname.replace("kao_", "ka_").replace("kbo_", "kb_")
+        red = po.reduced("lw/b%02d/%s" % (band, rname))
+        rc = lib.ref_lw_set(band, rname.encode(), _p(red), red.size)
+        if rc != 0:
+            raise RuntimeError("ref_lw_set(%d, %s) = %d" % (band, rname, rc))
+        nset += 1
+    lib.ref_lw_init(ctypes.c_double(po.CPDAIR))
+    return lib, nset
+
+
+def run_lw(lib, c, tauaer=None):
+    """rrtmg_lw of the reference (clear sky, idrv = 0) on a columns dict."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in LW_IN]
+    tau = np.zeros((16, L, n)) if tauaer is None else np.ascontiguousarray(tauaer, dtype=np.float64)
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_lw_run(n, L, *[_p(a) for a in ins], _p(tau), *[_p(a) for a in outs])
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))

@@ -34,59 +34,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
-import pyoracle as po  # noqa: E402
 from mima_amd.columns import make_columns, subset  # noqa: E402
-from mima_amd.tables.extract_tables import read_table  # noqa: E402
+from refrun import LW_IN, load_lw as load_reference, run_lw  # noqa: E402
 
-REF = os.path.join(ROOT, "oracle", "_ref", "librrtmg_lw_ref.so")
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
-LW_IN = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr",
-         "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr", "ccl4vmr", "emis")
 GAS = ("h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr",
        "ccl4vmr")
-RENAME = {"kao": "ka", "kbo": "kb", "selfrefo": "selfref", "forrefo": "forref", "fracrefao": "fracrefa",
-          "fracrefbo": "fracrefb", "ccl4o": "ccl4", "cfc11adjo": "cfc11adj", "cfc12o": "cfc12",
-          "cfc22adjo": "cfc22adj"}
 
 
 def _p(a):
     return a.ctypes.data_as(D)
-
-
-def load_reference():
-    """Load the reference LW column code and fill its module state."""
-    lib = ctypes.CDLL(REF)
-    lib.ref_lw_set.restype = ctypes.c_int
-    po.lib()  # oracle init: reduces the seeded tables
-    tabs, _, _ = read_table(po.LW_TABLE)
-    i32 = lambda k: np.ascontiguousarray(tabs[k], dtype=np.int32)  # noqa: E731
-    lib.ref_lw_set_bands(i32("lw/ngc").ctypes.data_as(I), i32("lw/nspa").ctypes.data_as(I),
-                         i32("lw/nspb").ctypes.data_as(I), _p(np.ascontiguousarray(tabs["lw/delwave"], np.float64)))
-    nset = 0
-    for name in sorted(tabs):
-        if not name.startswith("lw/b"):
-            continue
-        band, oname = int(name[4:6]), name[7:]
-        rname = RENAME.get(oname) or oname.replace("kao_", "ka_").replace("kbo_", "kb_")
-        red = po.reduced("lw/b%02d/%s" % (band, rname))
-        rc = lib.ref_lw_set(band, rname.encode(), _p(red), red.size)
-        if rc != 0:
-            raise RuntimeError("ref_lw_set(%d, %s) = %d" % (band, rname, rc))
-        nset += 1
-    lib.ref_lw_init(ctypes.c_double(po.CPDAIR))
-    return lib, nset
-
-
-def run_lw(lib, c, tauaer=None):
-    n, L = c["ncol"], c["nlay"]
-    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in LW_IN]
-    # emis is (16, ncol) level-major in the columns dict: Fortran (ncol, 16)
-    tau = np.zeros((16, L, n)) if tauaer is None else np.ascontiguousarray(tauaer, dtype=np.float64)
-    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
-            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
-    lib.ref_lw_run(n, L, *[_p(a) for a in ins], _p(tau), *[_p(a) for a in outs])
-    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
 def pack(c, out):

@@ -29,26 +29,16 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 from mima_amd.columns import make_columns, subset  # noqa: E402
 
-REF = os.path.join(ROOT, "oracle", "_ref", "librrtmg_sw_ref.so")
-CPDAIR = 287.04 / (2.0 / 7.0)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from refrun import SW_IN, run_sw  # noqa: E402
+import refrun  # noqa: E402
+
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
-SW_IN = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr",
-         "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")
 
 
 def _p(a):
     return a.ctypes.data_as(D)
-
-
-def run_sw(lib, c):
-    n, L = c["ncol"], c["nlay"]
-    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
-    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
-            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
-    lib.ref_sw_run(n, L, *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
-                   ctypes.c_double(c["scon"]), *[_p(a) for a in outs])
-    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
 
 
 def pack(c, out):
@@ -60,8 +50,7 @@ def pack(c, out):
 
 
 def main():
-    lib = ctypes.CDLL(REF)
-    lib.ref_sw_init(ctypes.c_double(CPDAIR))
+    lib = refrun.load_sw()
 
     # C0 grid, 64 random columns (day and night) + 4 with secondary gases
     full = make_columns(128, 64, 40, seed=0)
