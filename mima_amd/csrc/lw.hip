@@ -19,30 +19,31 @@
 namespace rrtmg {
 
 // ---------------------------------------------------------------------------
-// prep: one thread per column, FP64 (rrtmg_lw_rad.nomcica.f90:729-852 and
-// rrtmg_lw_setcoef.f90:150-413)
+// prep: FP64 (rrtmg_lw_rad.nomcica.f90:729-852 and rrtmg_lw_setcoef.f90:150-413),
+// one thread per (layer, column): the layer's coefficients in the band
+// kernel's precision, its integer table indices and the Planck index of the
+// level above it (and of the surface level)
 template <typename Real>
 __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w, const LwConst *__restrict__ cst)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     Real *coef = (Real *)w.coef;
     Real *tlevf = (Real *)w.tlev;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
     const size_t fs = (size_t)nlay * ncol;
-    double amttl = 0.0, wvttl = 0.0;
-    int laytrop = 0;
     // level 0 Planck index (setcoef: indlev0, t0frac)
-    {
+    if (lay == 0) {
         double tz0 = in.tlev[col];
         int i0 = clampi((int)(tz0 - 159.0), 1, 180);
         w.ilev[col] = i0;
         tlevf[col] = (Real)(tz0 - 159.0 - (double)i0);
     }
-    for (int lay = 0; lay < nlay; ++lay) {
+    {
         const size_t ix = (size_t)lay * ncol + col;
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
@@ -60,8 +61,6 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         summol = summol + wkl7;
         double wbrodl = coldry * (1.0 - summol);
         wkl1 = coldry * wkl1;
-        amttl = amttl + coldry + wkl1;
-        wvttl = wvttl + wkl1;
         double wx1 = coldry * in.ccl4[ix] * 1.e-20, wx2 = coldry * in.cfc11[ix] * 1.e-20;
         double wx3 = coldry * in.cfc12[ix] * 1.e-20, wx4 = coldry * in.cfc22[ix] * 1.e-20;
 
@@ -87,7 +86,6 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         double forfac = scalefac / (1. + water), forfrac, selffac = water * forfac, selffrac = 0.0;
         int indfor, indself = 0;
         if (plog > 4.56) {
-            ++laytrop;
             double factor = (332.0 - tavel) / 36.0;
             indfor = clampi((int)factor, 1, 2);
             forfrac = factor - (double)indfor;
@@ -145,6 +143,31 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         coef[LWF_WX4 * fs + ix] = (Real)wx4;
         coef[LWF_TLAYFRAC * fs + ix] = (Real)tlayfrac;
         w.ipk[ix] = pack_lw(jp, jt, jt1, indself, indfor, indminor, indlay);
+    }
+}
+
+// per-column part of the prep (one thread per column): laytrop and the
+// precipitable water from the column's water and dry-air totals, summed in
+// the reference's order (rrtmg_lw_rad.nomcica.f90:845-852)
+__global__ __launch_bounds__(256) void lw_column_kernel(ColumnInputs in, LwWork w)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, avogad = 6.02214199e+23, grav = 9.8066;
+    double amttl = 0.0, wvttl = 0.0;
+    int laytrop = 0;
+    for (int lay = 0; lay < nlay; ++lay) {
+        const size_t ix = (size_t)lay * ncol + col;
+        double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
+        double h2o = in.h2o[ix];
+        double wkl1 = (h2o / (1.0 - h2o)) * amdw;
+        double amm = (1.0 - wkl1) * amd + wkl1 * amw;
+        double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1.0 + wkl1));
+        wkl1 = coldry * wkl1;
+        amttl = amttl + coldry + wkl1;
+        wvttl = wvttl + wkl1;
+        if (log(in.play[ix]) > 4.56) ++laytrop;
     }
     double wvsh = (amw * wvttl) / (amd * amttl);
     double pwvcm = wvsh * (1.e3 * in.plev[col]) / (1.e2 * grav);
@@ -282,19 +305,23 @@ __global__ __launch_bounds__(64) void lw_band_kernel(ColumnInputs in, LwWork w, 
 #undef BODY
 }
 
-// finish: band sums, fluxfac, net flux and heating (rrtmg_lw_rtrnmr.f90:719-777)
+// finish: band sums, fluxfac, net flux and heating (rrtmg_lw_rtrnmr.f90:719-777),
+// one thread per (layer, column): the fluxes of the layer's lower level (and
+// of the top level) and its heating rate from the net fluxes of both levels
 template <typename Real>
 __global__ __launch_bounds__(256) void lw_finish_kernel(ColumnInputs in, LwWork w, LwOutputs out,
                                                         const LwBand *__restrict__ bands, double heatfac)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     const Real *bflux = (const Real *)w.bflux;
     const size_t bstride = (size_t)(nlay + 1) * ncol;
     const double fluxfac = 2.0 * asin(1.0) * 2.e4;
-    double fnet_below = 0.0;
-    for (int k = 0; k <= nlay; ++k) {
+    double fnet[2];
+    for (int j = 0; j < 2; ++j) {
+        const int k = lay + j;
         double up = 0.0, dn = 0.0;
         for (int b = 0; b < 16; ++b) {
             double dw = bands[b].delwave;
@@ -303,20 +330,19 @@ __global__ __launch_bounds__(256) void lw_finish_kernel(ColumnInputs in, LwWork 
         }
         up = up * fluxfac;
         dn = dn * fluxfac;
-        size_t ix = (size_t)k * ncol + col;
-        out.uflx[ix] = up;
-        out.dflx[ix] = dn;
-        out.uflxc[ix] = up;
-        out.dflxc[ix] = dn;
-        double fnet = up - dn;
-        if (k > 0) {
-            size_t il = (size_t)(k - 1) * ncol + col;
-            double h = heatfac * (fnet_below - fnet) / (in.plev[il] - in.plev[il + ncol]);
-            out.hr[il] = h;
-            out.hrc[il] = h;
+        if (j == 0 || lay == nlay - 1) {
+            size_t ix = (size_t)k * ncol + col;
+            out.uflx[ix] = up;
+            out.dflx[ix] = dn;
+            out.uflxc[ix] = up;
+            out.dflxc[ix] = dn;
         }
-        fnet_below = fnet;
+        fnet[j] = up - dn;
     }
+    size_t il = (size_t)lay * ncol + col;
+    double h = heatfac * (fnet[0] - fnet[1]) / (in.plev[il] - in.plev[il + ncol]);
+    out.hr[il] = h;
+    out.hrc[il] = h;
 }
 
 template <typename Real>
@@ -324,8 +350,10 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
                               double heatfac, bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(lw_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, t.cst);
+    hipLaunchKernelGGL(lw_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, t.cst);
+    hipLaunchKernelGGL(lw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = sizeof(Real) * ((size_t)2 * in.nlay * 64 + 64);
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -335,7 +363,7 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
                        t.desc.totplnk, (const Real *)t.tab, t.cst,
                        t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, heatfac);
+    hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out, t.bands, heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
@@ -343,13 +371,13 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
 hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *bflux, const LwDevTables &t,
                             int mode, hipStream_t s)
 {
-    const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     LwWork w;
     w.bflux = bflux;
     if (mode == 1)
-        hipLaunchKernelGGL(lw_finish_kernel<float>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
+        hipLaunchKernelGGL(lw_finish_kernel<float>, dim3(nbl), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
     else
-        hipLaunchKernelGGL(lw_finish_kernel<double>, dim3(nb), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
+        hipLaunchKernelGGL(lw_finish_kernel<double>, dim3(nbl), dim3(256), 0, s, in, w, out, t.bands, t.heatfac);
     return hipGetLastError();
 }
 

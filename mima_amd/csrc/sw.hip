@@ -18,23 +18,22 @@
 namespace rrtmg {
 
 // ---------------------------------------------------------------------------
-// prep: one thread per column, FP64 (rrtmg_sw_rad.nomcica.f90:933-1047 and
-// rrtmg_sw_setcoef.f90:129-284); writes the per-layer coefficients in the
-// band kernel's precision and the integer table indices.
+// prep: FP64 (rrtmg_sw_rad.nomcica.f90:933-1047 and rrtmg_sw_setcoef.f90:129-284),
+// one thread per (layer, column): the layer's coefficients in the band
+// kernel's precision and its integer table indices.
 template <typename Real>
 __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w, const double *__restrict__ preflog,
                                                       const double *__restrict__ tref)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     Real *coef = (Real *)w.coef;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    int laytrop = 0;
-    int jp_store[MAX_NLAY + 1];
-    for (int lay = 0; lay < nlay; ++lay) {
+    {
         const size_t ix = (size_t)lay * ncol + col;
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
@@ -66,7 +65,6 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
         double forfac, forfrac, selffac, selffrac;
         int indfor, indself;
         if (plog > 4.56) {
-            ++laytrop;
             forfac = scalefac / (1. + water);
             double factor = (332.0 - tavel) / 36.0;
             indfor = clampi((int)factor, 1, 2);
@@ -101,9 +99,22 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
         coef[SWF_FORFAC * fs + ix] = (Real)forfac;
         coef[SWF_FORFRAC * fs + ix] = (Real)forfrac;
         w.ipk[ix] = pack_sw(jp, jt, jt1, indself, indfor);
-        jp_store[lay] = jp;
     }
+}
+
+// per-column part of the prep (one thread per column, after the per-layer
+// part, whose packed indices hold jp): laytrop and the solar-source
+// reference layers
+__global__ __launch_bounds__(256) void sw_column_kernel(ColumnInputs in, SwWork w)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ncol = in.ncol, nlay = in.nlay;
+    if (col >= ncol) return;
+    int laytrop = 0;
+    for (int lay = 0; lay < nlay; ++lay)
+        if (log(in.play[(size_t)lay * ncol + col]) > 4.56) ++laytrop;
     w.laytrop[col] = laytrop;
+    auto jp_store = [&](int lay) { return unpack_jp(w.ipk[(size_t)lay * ncol + col]); };
     // reference layers for the solar source (taumol16..29): bands searching
     // the lower atmosphere (18-25) take min(lay+1, laytrop) at the last jp
     // crossing of layreffr, default laytrop; bands searching aloft (16, 17,
@@ -116,13 +127,13 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
         if (upref[b] > 0) {
             lsf = nlay;
             for (int lay = max(laytrop + 1, 2); lay <= nlay; ++lay)
-                if (jp_store[lay - 2] < upref[b] && jp_store[lay - 1] >= upref[b]) lsf = lay;
+                if (jp_store(lay - 2) < upref[b] && jp_store(lay - 1) >= upref[b]) lsf = lay;
         } else {
             lsf = laytrop;
             if (lowref[b] > 0)
                 for (int lay = 1; lay <= laytrop; ++lay) {
-                    int jpn = lay < nlay ? jp_store[lay] : jp_store[nlay - 1];
-                    if (jp_store[lay - 1] < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
+                    int jpn = lay < nlay ? jp_store(lay) : jp_store(nlay - 1);
+                    if (jp_store(lay - 1) < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
                 }
         }
         w.laysolfr[(size_t)b * ncol + col] = max(lsf, 1);
@@ -230,51 +241,52 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     }
 }
 
-// finish: band sums, net flux and heating (rrtmg_sw_rad.nomcica.f90:700-726)
+// finish: band sums, net flux and heating (rrtmg_sw_rad.nomcica.f90:700-726),
+// one thread per (layer, column): the fluxes of the layer's lower level (and
+// of the top level) and its heating rate, zero in the top layer
 template <typename Real>
 __global__ __launch_bounds__(256) void sw_finish_kernel(ColumnInputs in, SwWork w, SwOutputs out, double heatfac)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
     const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
     const Real *bflux = (const Real *)w.bflux;
     const size_t bstride = (size_t)(nlay + 1) * ncol;
+    const size_t il = (size_t)lay * ncol + col;
     if (in.coszen[col] < 1.e-10) {
-        for (int k = 0; k <= nlay; ++k) {
+        for (int k = lay; k <= (lay == nlay - 1 ? nlay : lay); ++k) {
             size_t ix = (size_t)k * ncol + col;
             out.uflx[ix] = 0.0;
             out.dflx[ix] = 0.0;
             out.uflxc[ix] = 0.0;
             out.dflxc[ix] = 0.0;
-            if (k < nlay) {
-                out.hr[ix] = 0.0;
-                out.hrc[ix] = 0.0;
-            }
         }
+        out.hr[il] = 0.0;
+        out.hrc[il] = 0.0;
         return;
     }
-    double net_below = 0.0;
-    for (int k = 0; k <= nlay; ++k) {
+    double net[2];
+    for (int j = 0; j < 2; ++j) {
+        const int k = lay + j;
         double up = 0.0, dn = 0.0;
         for (int b = 0; b < 14; ++b) {
             up += (double)bflux[(size_t)b * 2 * bstride + (size_t)k * ncol + col];
             dn += (double)bflux[(size_t)b * 2 * bstride + bstride + (size_t)k * ncol + col];
         }
-        size_t ix = (size_t)k * ncol + col;
-        out.uflx[ix] = up;
-        out.dflx[ix] = dn;
-        out.uflxc[ix] = up;
-        out.dflxc[ix] = dn;
-        double net = dn - up;
-        if (k > 0) {
-            size_t il = (size_t)(k - 1) * ncol + col;
-            double pdp = in.plev[il] - in.plev[il + ncol];
-            double h = (k == nlay) ? 0.0 : (net - net_below) * (heatfac / pdp);
-            out.hr[il] = h;
-            out.hrc[il] = h;
+        if (j == 0 || lay == nlay - 1) {
+            size_t ix = (size_t)k * ncol + col;
+            out.uflx[ix] = up;
+            out.dflx[ix] = dn;
+            out.uflxc[ix] = up;
+            out.dflxc[ix] = dn;
         }
-        net_below = net;
+        net[j] = dn - up;
     }
+    double pdp = in.plev[il] - in.plev[il + ncol];
+    double h = (lay == nlay - 1) ? 0.0 : (net[1] - net[0]) * (heatfac / pdp);
+    out.hr[il] = h;
+    out.hrc[il] = h;
 }
 
 template <typename Real>
@@ -282,8 +294,10 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
                               bool smooth, hipStream_t s, hipEvent_t *ev)
 {
     const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     if (ev) (void)hipEventRecord(ev[0], s);
-    hipLaunchKernelGGL(sw_prep_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, t.preflog, t.tref);
+    hipLaunchKernelGGL(sw_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, t.preflog, t.tref);
+    hipLaunchKernelGGL(sw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const size_t shmem = sizeof(Real) * ((size_t)2 * (in.nlay + 1) * 64 + 64);
     if (ev) (void)hipEventRecord(ev[1], s);
@@ -291,7 +305,7 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
     hipLaunchKernelGGL(sw_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
                        t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipLaunchKernelGGL(sw_finish_kernel<Real>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+    hipLaunchKernelGGL(sw_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out, t.heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);
     return hipGetLastError();
 }
@@ -299,13 +313,13 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
 hipError_t launch_sw_finish(const ColumnInputs &in, const SwOutputs &out, void *bflux, const SwDevTables &t,
                             int mode, hipStream_t s)
 {
-    const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     SwWork w;
     w.bflux = bflux;
     if (mode == 1)
-        hipLaunchKernelGGL(sw_finish_kernel<float>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+        hipLaunchKernelGGL(sw_finish_kernel<float>, dim3(nbl), dim3(256), 0, s, in, w, out, t.heatfac);
     else
-        hipLaunchKernelGGL(sw_finish_kernel<double>, dim3(nb), dim3(256), 0, s, in, w, out, t.heatfac);
+        hipLaunchKernelGGL(sw_finish_kernel<double>, dim3(nbl), dim3(256), 0, s, in, w, out, t.heatfac);
     return hipGetLastError();
 }
 
