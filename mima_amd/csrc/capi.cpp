@@ -335,7 +335,7 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     hipStream_t st = (hipStream_t)stream;
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
-    if ((rc = check_lds(rs * ((size_t)2 * (nlay + 1) * 64 + 64)))) return rc;
+    if ((rc = check_lds(sw_fwd_lds_bytes(nlay, rs)))) return rc;
     int ck = 0;
 
     // workspace
@@ -345,6 +345,8 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
     const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
     const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
+    const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves, sw_fwd_state_bytes(nlay, rs));
+    const size_t o_ds = off; off = align_up(off + (size_t)chunk * sw_fwd_state_bytes(nlay, rs));
     if ((rc = s->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->sw_work.p;
     SwWork w;
@@ -353,6 +355,8 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     w.laytrop = (int *)(wb + o_lt);
     w.laysolfr = (int *)(wb + o_lsf);
     w.bflux = wb + o_bf;
+    w.dstate = wb + o_ds;
+    w.chunk = chunk;
 
     ColumnInputs in{};
     in.ncol = ncol;
@@ -459,7 +463,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     hipStream_t st = (hipStream_t)stream;
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
-    if ((rc = check_lds(rs * ((size_t)2 * nlay * 64 + 64)))) return rc;
+    if ((rc = check_lds(lw_fwd_lds_bytes(nlay, rs)))) return rc;
     int ck = 0;
     size_t off = 0;
     const size_t o_coef = off; off = align_up(off + rs * LWF_COUNT * n2);
@@ -469,6 +473,8 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
     const size_t o_cs = off; off = align_up(off + sizeof(double) * 3 * ncol);
     const size_t o_bf = off; off = align_up(off + rs * 16 * 2 * n3);
+    const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->lw.tab.nwaves, lw_fwd_state_bytes(nlay, rs));
+    const size_t o_ds = off; off = align_up(off + (size_t)chunk * lw_fwd_state_bytes(nlay, rs));
     if ((rc = s->lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->lw_work.p;
     LwWork w;
@@ -479,6 +485,8 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     w.laytrop = (int *)(wb + o_lt);
     w.colscal = (double *)(wb + o_cs);
     w.bflux = wb + o_bf;
+    w.dstate = wb + o_ds;
+    w.chunk = chunk;
 
     ColumnInputs in{};
     in.ncol = ncol;

@@ -32,6 +32,8 @@ struct SwWork {
     int *laytrop = nullptr;   // [ncol]
     int *laysolfr = nullptr;  // [14][ncol]
     void *bflux = nullptr;    // [14][2][nlay+1][ncol] of Real
+    void *dstate = nullptr;   // [chunk][nlay+1][64] of Real: prupd of each wave of a band-kernel launch
+    int chunk = 0;            // wave units per band-kernel launch (rows of dstate)
 };
 struct LwWork {
     void *coef = nullptr;     // [LWF_COUNT][nlay][ncol] of Real
@@ -41,6 +43,8 @@ struct LwWork {
     int *laytrop = nullptr;   // [ncol]
     double *colscal = nullptr;// [3][ncol]: pwvcm, tbound, (spare)
     void *bflux = nullptr;    // [16][2][nlay+1][ncol] of Real
+    void *dstate = nullptr;   // [chunk][nlay][64] of Real: upward source of each wave of a band-kernel launch
+    int chunk = 0;            // wave units per band-kernel launch (rows of dstate)
 };
 
 struct SwDevTables {
@@ -97,9 +101,9 @@ struct SwVjpWork {
     int chunk = 0;            // wave units per backward band-kernel launch (rows of dstate)
     void *bflux = nullptr;    // [14][2][nlay+1][ncol] of Real: band fluxes of the recomputed forward, or NULL
 };
-// The backward band kernels keep the states of the sweep that runs in the
-// other direction from its reverse in a global scratch row per wave unit
-// (LDS keeps the rest); a launch covers at most `chunk` wave units so the
+// The band kernels keep part of the per-level state that one sweep writes and
+// the next reads in the other direction in a global scratch row per wave
+// unit, the rest in LDS, so that LDS does not cap occupancy below the VGPRs; a launch covers at most `chunk` wave units so the
 // scratch stays within VJP_SCRATCH_BYTES (2 GiB), and longer grids are
 // launched in chunks that reuse the rows.
 constexpr size_t VJP_SCRATCH_BYTES = (size_t)2 << 30;
@@ -109,6 +113,10 @@ inline int vjp_chunk(int units, size_t bytes_per_unit)
     if (c < 8) c = 8;
     return units < (int)c ? units : (int)c;
 }
+inline size_t sw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)(nlay + 1) * 64; }
+inline size_t lw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)nlay * 64; }
+inline size_t sw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)(nlay + 1) * 64 + 64); }
+inline size_t lw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 64); }
 inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
 {
     return real_size * (3 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;

@@ -12,6 +12,8 @@
 // (segmented sum over ng adjacent lanes) into per-band level fluxes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 #include "lw_optics.hpp"
 #include "rrtmg_device.hpp"
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void lw_column_kernel(ColumnInputs in, LwWork 
 template <int BAND, typename Real>
 __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBand &B, const Real *__restrict__ tab,
                              int exp_off, int tfn_off, int plk_off, const LwConst *__restrict__ cst, WaveUnit u,
-                             int tile, Real *smem)
+                             int tile, Real *smem, Real *__restrict__ s_bbu)
 {
     constexpr int NG = lw_ng(BAND);
     const int ncol = in.ncol, nlay = in.nlay;
@@ -189,9 +191,9 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol;
     const int col = active ? col0 : 0;
-    Real *s_atr = smem;                            // [nlay][64]
-    Real *s_bbu = smem + (size_t)nlay * 64;        // [nlay][64]
-    Real *s_red = smem + (size_t)2 * nlay * 64;    // [64]
+    Real *s_atr = smem;                            // [nlay][64], LDS
+    Real *s_red = smem + (size_t)nlay * 64;        // [64]
+    // s_bbu: upward source per layer, [nlay][64] in this wave's global scratch row
     const Real *coef = (const Real *)w.coef;
     const Real *tlevf = (const Real *)w.tlev;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
@@ -272,9 +274,11 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         Real s = seg_sum(active ? radlu : (Real)0, lane, NG, s_red);
         if (active && g == 0) bf_up[col] = s;
     }
+    Real bbu_next = s_bbu[lane];
     for (int lev = 1; lev <= nlay; ++lev) {
+        const Real bbugas = bbu_next;
+        if (lev < nlay) bbu_next = s_bbu[(size_t)lev * 64 + lane];  // one level ahead
         Real atrans = s_atr[(size_t)(lev - 1) * 64 + lane];
-        Real bbugas = s_bbu[(size_t)(lev - 1) * 64 + lane];
         radlu = radlu + (bbugas - radlu) * atrans;
         Real s = seg_sum(active ? radlu : (Real)0, lane, NG, s_red);
         if (active && g == 0) bf_up[(size_t)lev * ncol + col] = s;
@@ -283,20 +287,21 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
 
 template <typename Real>
 __global__ __launch_bounds__(64) void lw_band_kernel(ColumnInputs in, LwWork w, const LwBand *__restrict__ bands,
-                                                     int nwaves, int nblocks, int exp_off, int tfn_off, int plk_off,
-                                                     const Real *__restrict__ tab, const LwConst *__restrict__ cst,
-                                                     const WaveUnit *__restrict__ units)
+                                                     int nwaves, int blk0, int nblocks, int exp_off, int tfn_off,
+                                                     int plk_off, const Real *__restrict__ tab,
+                                                     const LwConst *__restrict__ cst, const WaveUnit *__restrict__ units)
 {
     extern __shared__ unsigned char smem_raw[];
     Real *smem = (Real *)smem_raw;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int tile = blk / nwaves;
-    const WaveUnit u = units[blk % nwaves];
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
     const LwBand B = bands[u.band];
+    Real *s_bbu = (Real *)w.dstate + (size_t)blk * in.nlay * 64;
 #define BODY(b)                                                                                          \
     case b - 1:                                                                                          \
-        lw_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem);              \
+        lw_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem, s_bbu);       \
         break;
     switch (u.band) {
         BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
@@ -355,13 +360,16 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     hipLaunchKernelGGL(lw_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, t.cst);
     hipLaunchKernelGGL(lw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
-    const size_t shmem = sizeof(Real) * ((size_t)2 * in.nlay * 64 + 64);
+    const size_t shmem = lw_fwd_lds_bytes(in.nlay, sizeof(Real));
     if (ev) (void)hipEventRecord(ev[1], s);
     const int nblocks = ntiles * t.desc.nwaves;
-    hipLaunchKernelGGL(lw_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
-                       t.desc.totplnk, (const Real *)t.tab, t.cst,
-                       t.units);
+    // launches of at most w.chunk wave units, each reusing the scratch rows
+    for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
+        const int nb_c = std::min(w.chunk, nblocks - b0);
+        hipLaunchKernelGGL(lw_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), shmem, s, in, w, t.bands,
+                           t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,
+                           t.desc.totplnk, (const Real *)t.tab, t.cst, t.units);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out, t.bands, heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);

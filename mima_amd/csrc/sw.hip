@@ -5,11 +5,14 @@
 //
 // Band kernel layout: one wave per work unit (band b, up to 64/ng columns of a
 // tile); lane = (column, g).  Each lane runs the adding method for its
-// g-point: a bottom-up sweep keeps prup/prupd of every level in LDS, a
+// g-point: a bottom-up sweep keeps prup/prupd of every level (prup in LDS,
+// prupd in a per-wave global scratch row), a
 // top-down sweep recomputes the layer optics, carries ztdn/prdnd/ptdbt and
 // produces level fluxes, which are summed over the band's g-points of a
 // column (segmented sum over ng adjacent lanes) into per-band fluxes.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "kernels.hpp"
 #include "rrtmg_device.hpp"
@@ -142,16 +145,16 @@ __global__ __launch_bounds__(256) void sw_column_kernel(ColumnInputs in, SwWork 
 
 template <typename Real>
 __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
-                                                     int nwaves, int nblocks, int exp_off, const Real *__restrict__ tab,
-                                                     const WaveUnit *__restrict__ units)
+                                                     int nwaves, int blk0, int nblocks, int exp_off,
+                                                     const Real *__restrict__ tab, const WaveUnit *__restrict__ units)
 {
     extern __shared__ unsigned char smem_raw[];
     Real *smem = (Real *)smem_raw;
     const int ncol = in.ncol, nlay = in.nlay;
     const int blk = xcd_block();
     if (blk >= nblocks) return;
-    const int tile = blk / nwaves;
-    const WaveUnit u = units[blk % nwaves];
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
     const SwBand B = bands[u.band];
     const int jb = u.band + 16;
     const int lane = threadIdx.x;
@@ -159,9 +162,10 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     const int cl = lane / ng, g = lane - cl * ng;
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
-    Real *s_prup = smem;                           // [(nlay+1)][64]
-    Real *s_prupd = smem + (size_t)(nlay + 1) * 64;
-    Real *s_red = smem + (size_t)2 * (nlay + 1) * 64;  // [64]
+    Real *s_prup = smem;                           // [(nlay+1)][64], LDS
+    Real *s_red = smem + (size_t)(nlay + 1) * 64;  // [64]
+    // prupd of every level in this wave's global scratch row, [(nlay+1)][64]
+    Real *__restrict__ s_prupd = (Real *)w.dstate + (size_t)blk * (nlay + 1) * 64;
     const Real *coef = (const Real *)w.coef;
     Real *bflux = (Real *)w.bflux;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
@@ -209,6 +213,7 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     const size_t lvstride = ncol, bstride = (size_t)(nlay + 1) * ncol;
     Real *bf_up = bflux + (size_t)u.band * 2 * bstride;
     Real *bf_dn = bf_up + bstride;
+    Real prupd_next = s_prupd[lane];
     for (int jk = 1; jk <= nlay + 1; ++jk) {
         if (jk > 1) {
             const int lay = nlay + 2 - jk;  // layer between levels jk-1 and jk
@@ -227,7 +232,8 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
             prdnd = prdnd_n;
         }
         Real prup = s_prup[(size_t)(jk - 1) * 64 + lane];
-        Real prupd = s_prupd[(size_t)(jk - 1) * 64 + lane];
+        const Real prupd = prupd_next;
+        if (jk <= nlay) prupd_next = s_prupd[(size_t)jk * 64 + lane];  // one level ahead
         Real zreflect = (Real)1 / ((Real)1 - prdnd * prupd);
         Real pfu = (ptdbt * prup + (ztdn - ptdbt) * prupd) * zreflect;
         Real pfd = ptdbt + (ztdn - ptdbt + ptdbt * prup * prdnd) * zreflect;
@@ -299,11 +305,15 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
     hipLaunchKernelGGL(sw_prep_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, t.preflog, t.tref);
     hipLaunchKernelGGL(sw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
-    const size_t shmem = sizeof(Real) * ((size_t)2 * (in.nlay + 1) * 64 + 64);
+    const size_t shmem = sw_fwd_lds_bytes(in.nlay, sizeof(Real));
     if (ev) (void)hipEventRecord(ev[1], s);
     const int nblocks = ntiles * t.desc.nwaves;
-    hipLaunchKernelGGL(sw_band_kernel<Real>, dim3(xcd_grid(nblocks)), dim3(64), shmem, s, in, w, t.bands,
-                       t.desc.nwaves, nblocks, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    // launches of at most w.chunk wave units, each reusing the scratch rows
+    for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
+        const int nb_c = std::min(w.chunk, nblocks - b0);
+        hipLaunchKernelGGL(sw_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), shmem, s, in, w, t.bands,
+                           t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out, t.heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);
