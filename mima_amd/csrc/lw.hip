@@ -263,7 +263,7 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         radld = radld + (bbd - radld) * atrans;
         s_atr[(size_t)(lev - 1) * 64 + lane] = atrans;
         s_bbu[(size_t)(lev - 1) * 64 + lane] = bbugas;
-        Real s = seg_sum(active ? radld : (Real)0, lane, NG, s_red);
+        Real s = seg_sum(active ? radld : (Real)0, lane, g, NG, s_red);
         if (active && g == 0) bf_dn[(size_t)(lev - 1) * ncol + col] = s;
         plev_up = plev_dn;
         if (lev == 1) fracs1 = plfrac;
@@ -271,7 +271,7 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
     // surface emission and reflection, upward sweep
     Real radlu = fracs1 * plankbnd + ((Real)1 - semiss) * radld;
     {
-        Real s = seg_sum(active ? radlu : (Real)0, lane, NG, s_red);
+        Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
         if (active && g == 0) bf_up[col] = s;
     }
     Real bbu_next = s_bbu[lane];
@@ -280,7 +280,7 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         if (lev < nlay) bbu_next = s_bbu[(size_t)lev * 64 + lane];  // one level ahead
         Real atrans = s_atr[(size_t)(lev - 1) * 64 + lane];
         radlu = radlu + (bbugas - radlu) * atrans;
-        Real s = seg_sum(active ? radlu : (Real)0, lane, NG, s_red);
+        Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
         if (active && g == 0) bf_up[(size_t)lev * ncol + col] = s;
     }
 }

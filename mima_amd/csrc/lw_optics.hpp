@@ -52,7 +52,13 @@ template <typename Real>
 struct LTab {
     const Real *__restrict__ t;
     int ng, g;
-    __device__ __forceinline__ Real operator()(int off, int row) const { return t[off + (row - 1) * ng + g]; }
+    // a 32-bit unsigned byte offset from the uniform base: one load with an
+    // SGPR base and a VGPR offset, no 64-bit address arithmetic per gather
+    __device__ __forceinline__ Real operator()(int off, int row) const
+    {
+        const unsigned byte = (unsigned)(off + (row - 1) * ng + g) * (unsigned)sizeof(Real);
+        return *(const Real *)((const char *)t + byte);
+    }
 };
 
 template <typename R>
@@ -67,7 +73,7 @@ __device__ __forceinline__ Eta<V> lw_eta(V cola, V colb, S rat, S mult)
 {
     Eta<V> e;
     e.speccomb = cola + rat * colb;
-    e.specparm = cola / e.speccomb;
+    e.specparm = fdiv(cola, e.speccomb);
     const S oneminus = (S)(1.0 - 1.e-6);
     if (e.specparm >= oneminus) e.specparm = V(oneminus);
     V specmult = mult * e.specparm;
@@ -146,7 +152,7 @@ __device__ __forceinline__ V lw_pfrac(const LTab<Real> &T, int off, const Eta<V>
 template <typename V, typename S>
 __device__ __forceinline__ V lw_adjcol(V col, V coldry, S chiref, S thresh, S base, S ex)
 {
-    V rat = (S)1.e20 * (col / coldry) / chiref;
+    V rat = fdiv((S)1.e20 * fdiv(col, coldry), chiref);
     if (rat > thresh) return (base + pow(rat - base, ex)) * chiref * coldry * (S)1.e-20;
     return col;
 }
@@ -202,20 +208,20 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
     auto R = [](double x) { return (Real)x; };
     if constexpr (BAND == 1) {
         if (lower) {
-            V corradj = c.pavel < (Real)250 ? V((Real)1 - (Real)0.15 * ((Real)250 - c.pavel) / (Real)154.4) : V((Real)1);
+            V corradj = c.pavel < (Real)250 ? V((Real)1 - fdiv((Real)0.15 * ((Real)250 - c.pavel), (Real)154.4)) : V((Real)1);
             V taun2 = c.colbrd * c.scaleminorn2 * lw_minor1(T, B.ka_mn2, c);
             taug = corradj * (c.colh2o * lw_four(T, B.absa, ind0 + 1, ind1 + 1, c) + lw_self(T, B, c) +
                               lw_for(T, B, c) + taun2);
             fracs = T(B.fracrefa, 1);
         } else {
-            V corradj = (Real)1 - (Real)0.15 * (c.pavel / (Real)95.6);
+            V corradj = (Real)1 - (Real)0.15 * fdiv(c.pavel, (Real)95.6);
             V taun2 = c.colbrd * c.scaleminorn2 * lw_minor1(T, B.kb_mn2, c);
             taug = corradj * (c.colh2o * lw_four(T, B.absb, ind0 + 1, ind1 + 1, c) + lw_for(T, B, c) + taun2);
             fracs = T(B.fracrefb, 1);
         }
     } else if constexpr (BAND == 2) {
         if (lower) {
-            V corradj = (Real)1 - (Real).05 * (c.pavel - (Real)100) / (Real)900;
+            V corradj = (Real)1 - fdiv((Real).05 * (c.pavel - (Real)100), (Real)900);
             taug = corradj * (c.colh2o * lw_four(T, B.absa, ind0 + 1, ind1 + 1, c) + lw_self(T, B, c) + lw_for(T, B, c));
             fracs = T(B.fracrefa, 1);
         } else {

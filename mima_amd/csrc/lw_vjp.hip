@@ -434,7 +434,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
             s3[(size_t)(lev - 1) * 64 + lane] = radld;
             radld = radld + (bbd - radld) * rt.atrans;
             if (bf_dn) {
-                const Real sd = seg_sum(active ? radld : (Real)0, lane, NG, red);
+                const Real sd = seg_sum(active ? radld : (Real)0, lane, g, NG, red);
                 if (active && g == 0) bf_dn[(size_t)(lev - 1) * ncol + col] = sd;
             }
             plev_up = plev_dn;
@@ -444,7 +444,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     // surface and pass B: upward sweep; keep source minus incoming radiance
     Real radlu = fracs1 * plankbnd + ((Real)1 - semiss) * radld;
     if (bf_up) {
-        const Real su = seg_sum(active ? radlu : (Real)0, lane, NG, red);
+        const Real su = seg_sum(active ? radlu : (Real)0, lane, g, NG, red);
         if (active && g == 0) bf_up[col] = su;
     }
     for (int lev = 1; lev <= nlay; ++lev) {
@@ -453,7 +453,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
         s2[(size_t)(lev - 1) * 64 + lane] = dsrc;
         radlu = radlu + dsrc * atr;
         if (bf_up) {
-            const Real su = seg_sum(active ? radlu : (Real)0, lane, NG, red);
+            const Real su = seg_sum(active ? radlu : (Real)0, lane, g, NG, red);
             if (active && g == 0) bf_up[(size_t)lev * ncol + col] = su;
         }
     }

@@ -9,11 +9,23 @@ __host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v
 // Index of the 10000-interval lookup tables for optical depth x:
 // int(tblint * x/(bpade + x) + 0.5)  (rrtmg_lw_rtrnmr.f90:597-598,
 // rrtmg_sw_reftra.f90:182-183)
+// a / b in the arithmetic of the path: FP64 exactly (the reference's
+// division); FP32 as a times the hardware reciprocal (1 ulp), the FP32-mixed
+// path's division (an IEEE FP32 division is a ten-instruction sequence);
+// anything else (dual numbers) with its own operator.
+__device__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+template <typename A, typename B>
+__device__ __forceinline__ auto fdiv(const A &a, const B &b) -> decltype(a / b)
+{
+    return a / b;
+}
+
 template <typename F>
 __device__ __forceinline__ int tbl_index(F x)
 {
     const F bpade = (F)(1.0 / 0.278);
-    int i = (int)((F)1.e4 * (x / (bpade + x)) + (F)0.5);
+    int i = (int)((F)1.e4 * fdiv(x, bpade + x) + (F)0.5);
     return i < 0 ? 0 : (i > 10000 ? 10000 : i);  // keeps NaN/huge inputs inside the table
 }
 
@@ -32,29 +44,29 @@ __device__ __forceinline__ int xcd_block()
 }
 
 // Sum of v over the ng adjacent lanes of each column group of a one-wave
-// block; valid on the group's first lane.  Deterministic (fixed order); FP32
-// sums are compensated (Kahan).  `red` is a 64-entry LDS buffer.
+// block (g = the lane's position in its group); valid on the group's first
+// lane.  Deterministic (fixed order).  FP64: sequential through LDS, the
+// reference's order over g-points; FP32: a tree of cross-lane shuffles
+// (log2 ng steps; pairwise summation, error ~log2(ng) ulp instead of ng).
+// `red` is a 64-entry LDS buffer (FP64 only).  Called by all 64 lanes.
 template <typename Real>
-__device__ __forceinline__ Real seg_sum(Real v, int lane, int ng, Real *red)
+__device__ __forceinline__ Real seg_sum(Real v, int lane, int g, int ng, Real *red)
 {
-    red[lane] = v;
-    __syncthreads();
-    Real s = (Real)0;
-    if (lane % ng == 0 && lane + ng <= 64) {
-        if (sizeof(Real) == 4) {
-            Real comp = (Real)0;
-            for (int j = 0; j < ng; ++j) {
-                Real y = red[lane + j] - comp;
-                Real t = s + y;
-                comp = (t - s) - y;
-                s = t;
-            }
-        } else {
-            for (int j = 0; j < ng; ++j) s += red[lane + j];
+    if constexpr (sizeof(Real) == 4) {
+        for (int s = 1; s < ng; s <<= 1) {
+            const Real o = __shfl_down(v, s);
+            if (g + s < ng) v += o;
         }
+        return g == 0 ? v : (Real)0;
+    } else {
+        red[lane] = v;
+        __syncthreads();
+        Real s = (Real)0;
+        if (g == 0 && lane + ng <= 64)
+            for (int j = 0; j < ng; ++j) s += red[lane + j];
+        __syncthreads();
+        return s;
     }
-    __syncthreads();
-    return s;
 }
 
 }  // namespace rrtmg

@@ -199,7 +199,7 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
             sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
             sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
         }
-        Real zreflect = (Real)1 / ((Real)1 - prupd_n * rd);
+        Real zreflect = fdiv((Real)1, (Real)1 - prupd_n * rd);
         Real prup = r + (td * ((t - dbt) * prupd_n + dbt * prup_n)) * zreflect;
         Real prupd = rd + td * td * prupd_n * zreflect;
         s_prup[(size_t)(jk - 1) * 64 + lane] = prup;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
                 sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
                 sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
             }
-            Real zreflect = (Real)1 / ((Real)1 - rd * prdnd);
+            Real zreflect = fdiv((Real)1, (Real)1 - rd * prdnd);
             Real ztdn_n = ptdbt * t + (td * ((ztdn - ptdbt) + ptdbt * r * prdnd)) * zreflect;
             Real prdnd_n = rd + td * td * prdnd * zreflect;
             ptdbt = dbt * ptdbt;
@@ -234,12 +234,12 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
         Real prup = s_prup[(size_t)(jk - 1) * 64 + lane];
         const Real prupd = prupd_next;
         if (jk <= nlay) prupd_next = s_prupd[(size_t)jk * 64 + lane];  // one level ahead
-        Real zreflect = (Real)1 / ((Real)1 - prdnd * prupd);
+        Real zreflect = fdiv((Real)1, (Real)1 - prdnd * prupd);
         Real pfu = (ptdbt * prup + (ztdn - ptdbt) * prupd) * zreflect;
         Real pfd = ptdbt + (ztdn - ptdbt + ptdbt * prup * prdnd) * zreflect;
         const int lev = nlay + 1 - jk;  // bottom-up level index
-        Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, ng, s_red);
-        Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, ng, s_red);
+        Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, g, ng, s_red);
+        Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, g, ng, s_red);
         if (active && g == 0) {
             bf_up[(size_t)lev * lvstride + col] = su;
             bf_dn[(size_t)lev * lvstride + col] = sd;

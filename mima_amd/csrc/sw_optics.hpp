@@ -56,7 +56,13 @@ template <typename Real>
 struct Tab {
     const Real *__restrict__ t;
     int ng, g;
-    __device__ __forceinline__ Real operator()(int off, int row) const { return t[off + (row - 1) * ng + g]; }
+    // a 32-bit unsigned byte offset from the uniform base: one load with an
+    // SGPR base and a VGPR offset, no 64-bit address arithmetic per gather
+    __device__ __forceinline__ Real operator()(int off, int row) const
+    {
+        const unsigned byte = (unsigned)(off + (row - 1) * ng + g) * (unsigned)sizeof(Real);
+        return *(const Real *)((const char *)t + byte);
+    }
 };
 
 // binary-species parameter of a band (e.g. rrtmg_sw_taumol.f90:276-283)
@@ -64,7 +70,7 @@ template <typename R>
 __device__ __forceinline__ void sw_eta(R cola, R colb, R strrat, R mult, R &speccomb, int &js, R &fs)
 {
     speccomb = cola + strrat * colb;
-    R specparm = cola / speccomb;
+    R specparm = fdiv(cola, speccomb);
     const R oneminus = (R)(1.0 - 1.e-6);
     if (specparm >= oneminus) specparm = oneminus;
     R specmult = mult * specparm;
@@ -175,7 +181,7 @@ __device__ __forceinline__ void sw_gas(int jb, const SwBand &B, const Tab<Real> 
         taur = c.colmol * rayl;
         break;
     case 22: {
-        V o2cont = (Real)4.35e-4 * c.colo2 / (Real)(350.0 * 2.0);
+        V o2cont = fdiv((Real)4.35e-4 * c.colo2, (Real)(350.0 * 2.0));
         if (lower) {
             sw_eta<V>(c.colh2o, c.colo2, (Real)(1.6 * 0.022708), (Real)8, speccomb, js, fs);
             taug = speccomb * sw_maj8(T, B.absa, ind0 + js, ind1 + js, fs, c, 9) + c.colh2o * sw_cont(T, B, c, true) +
@@ -312,10 +318,12 @@ __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *_
 }
 
 // PIFM two-stream layer reflectance/transmittance, kmodts = 2, asymmetry 0
-// (rrtmg_sw_reftra.f90:127-303).  F is the arithmetic type.
+// (rrtmg_sw_reftra.f90:127-303).  F is the arithmetic type.  zomw = 1 - zw
+// formed without cancellation (gas over total optical depth); the FP32
+// arithmetic uses it for the eigenvalue, see below.
 template <typename F, typename Real, typename S>
-__device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__restrict__ exp_tbl, F &pref, F &prefd,
-                                       F &ptra, F &ptrad)
+__device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real *__restrict__ exp_tbl, F &pref,
+                                       F &prefd, F &ptra, F &ptrad)
 {
     const F eps = (F)1.e-08, od_lo = (F)0.06, zwcrit = (F)0.9999995;
     const F zg = (F)0, zg3 = (F)3 * zg;
@@ -328,7 +336,7 @@ __device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__rest
         F za = zgamma1 * prmuz;
         F za1 = za - zgamma3;
         F zgt = zgamma1 * zto1;
-        F ze1 = fmin(zto1 / prmuz, (F)500);
+        F ze1 = fmin(fdiv(zto1, (F)prmuz), (F)500);
         F ze2 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
         if (ze2 == (F)1) {
             pref = (F)0;
@@ -336,15 +344,22 @@ __device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__rest
             prefd = (F)0;
             ptrad = (F)1;
         } else {
-            pref = (zgt - za1 * ((F)1 - ze2)) / ((F)1 + zgt);
+            pref = fdiv(zgt - za1 * ((F)1 - ze2), (F)1 + zgt);
             ptra = (F)1 - pref;
-            prefd = zgt / ((F)1 + zgt);
+            prefd = fdiv(zgt, (F)1 + zgt);
             ptrad = (F)1 - prefd;
         }
     } else {
         F za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
         F za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
-        F zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+        // k = sqrt(g1^2 - g2^2); with g = 0, g1 - g2 = 2 (1 - w) exactly, so
+        // in FP32 k = sqrt(2 (1 - w) (g1 + g2)), free of the cancellation of
+        // the difference of squares as w -> 1; FP64 keeps the reference's form
+        F zrk;
+        if (sizeof(val(zto1)) == 4)
+            zrk = sqrt((F)2 * zomw * (zgamma1 + zgamma2));
+        else
+            zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
         F zrp = zrk * prmuz;
         F zrp1 = (F)1 + zrp, zrm1 = (F)1 - zrp;
         F zrk2 = (F)2 * zrk;
@@ -358,24 +373,24 @@ __device__ __forceinline__ void reftra(F zto1, F zw, S prmuz, const Real *__rest
         F zt1 = zrp1 * (za1 + zrk * zgamma4);
         F zt2 = zrm1 * (za1 - zrk * zgamma4);
         F zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-        F zbeta = (zgamma1 - zrk) / zrkg;
+        F zbeta = fdiv(zgamma1 - zrk, zrkg);
         F ze1 = fmin(zrk * zto1, (F)500);
-        F ze2 = fmin(zto1 / prmuz, (F)500);
+        F ze2 = fmin(fdiv(zto1, (F)prmuz), (F)500);
         F zem1 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
-        F zep1 = (F)1 / zem1;
+        F zep1 = fdiv((F)1, zem1);
         F zem2 = (exp_tbl && ze2 <= od_lo) ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
-        F zep2 = (F)1 / zem2;
+        F zep2 = fdiv((F)1, zem2);
         F zdenr = zr4 * zep1 + zr5 * zem1;
         F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
         if (zdenr >= -eps && zdenr <= eps) {
             pref = eps;
             ptra = zem2;
         } else {
-            pref = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) / zdenr;
-            ptra = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) / zdent;
+            pref = fdiv(zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2), zdenr);
+            ptra = zem2 - fdiv(zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2), zdent);
         }
         F zemm = zem1 * zem1;
-        F zdend = (F)1 / (((F)1 - zbeta * zemm) * zrkg);
+        F zdend = fdiv((F)1, ((F)1 - zbeta * zemm) * zrkg);
         prefd = zgamma2 * ((F)1 - zemm) * zdend;
         ptrad = zrk2 * zem1 * zdend;
     }
@@ -390,30 +405,34 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
                                                 V &t, V &td, V &dbt)
 {
     V tau = taur + taug;
-    V w = taur / tau;
+    V w = fdiv(taur, tau);
+    V omw = fdiv(taug, tau);
     bool island = false;
     if (sizeof(Real) == 4) {
-        // FP64 islands of the mixed path: near-conservative scattering
-        // (cancellation in sqrt(g1^2 - g2^2) and 1 - exp) and the removable
-        // 0/0 of the direct-beam solution at k*mu0 = 1, where both the
-        // numerator and zdenr of reftra vanish
-        Real wv = val(w);
-        Real g1 = ((Real)8 - wv * (Real)5) * (Real)0.25, g2 = (Real)3 * wv * (Real)0.25;
-        Real zrp = sqrt(fmax(g1 * g1 - g2 * g2, (Real)0)) * prmu0;
-        island = wv > (Real)0.999 || fabs((Real)1 - zrp) < (Real)0.02;
+        // FP64 islands of the mixed path: the removable 0/0 of the direct-beam
+        // solution at k mu0 = 1, where both the numerator and zdenr of reftra
+        // vanish.  Near-conservative scattering needs none for the values
+        // (the eigenvalue is formed from 1 - w, see reftra) but does for the
+        // derivatives (dual V, the backward): with w > 0.999 the derivative
+        // with respect to the gas optical depth is a (1 - w)-sized difference
+        // of O(1) terms, which FP32 resolves to a few percent only
+        Real wv = val(w), ov = val(omw);
+        Real zrp = sqrt((Real)2 * ov * ((Real)2 - (Real)0.5 * wv)) * prmu0;
+        island = fabs((Real)1 - zrp) < (Real)0.02 || (sizeof(V) != sizeof(Real) && wv > (Real)0.999);
     }
     if (island) {
         V64 R, RD, T, TD;
         V64 t64 = cast<V64>(taur) + cast<V64>(taug);
-        reftra<V64, Real, double>(t64, cast<V64>(taur) / t64, (double)prmu0, exp_tbl, R, RD, T, TD);
+        reftra<V64, Real, double>(t64, cast<V64>(taur) / t64, cast<V64>(taug) / t64, (double)prmu0, exp_tbl, R, RD,
+                                  T, TD);
         r = cast<V>(R);
         rd = cast<V>(RD);
         t = cast<V>(T);
         td = cast<V>(TD);
     } else {
-        reftra<V, Real, Real>(tau, w, prmu0, exp_tbl, r, rd, t, td);
+        reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td);
     }
-    V ze1 = tau / prmu0;
+    V ze1 = fdiv(tau, V(prmu0));
     dbt = (exp_tbl && ze1 <= (Real)0.06) ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
 }
 

@@ -409,8 +409,8 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         const Real a_pfu = zincflx * gu, a_pfd = zincflx * gd;
         a_zinc += gu * pfu + gd * pfd;
         if (bf_up) {
-            const Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, ng, red);
-            const Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, ng, red);
+            const Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, g, ng, red);
+            const Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, g, ng, red);
             if (active && g == 0) {
                 bf_up[(size_t)lev * ncol + col] = su;
                 bf_dn[(size_t)lev * ncol + col] = sd;
