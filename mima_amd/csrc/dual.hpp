@@ -25,6 +25,11 @@ using ::log;
 using ::pow;
 using ::sqrt;
 
+// a / b for the parts of a dual number: FP64 exactly; FP32 as a times the
+// hardware reciprocal (1 ulp), the FP32 forward's division
+__host__ __device__ __forceinline__ double dual_div(double a, double b) { return a / b; }
+__device__ __forceinline__ float dual_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 template <typename F, int N>
 struct Dual {
     F v;
@@ -95,8 +100,8 @@ DUAL_T __host__ __device__ __forceinline__ D operator*(const D &a, const D &b)
 DUAL_T __host__ __device__ __forceinline__ D operator/(const D &a, const D &b)
 {
     D r;
-    r.v = a.v / b.v;
-    const F ib = (F)1 / b.v;
+    r.v = dual_div(a.v, b.v);
+    const F ib = dual_div((F)1, b.v);
     for (int i = 0; i < N; ++i) r.d[i] = (a.d[i] - r.v * b.d[i]) * ib;
     return r;
 }
@@ -138,8 +143,8 @@ DUAL_T __host__ __device__ __forceinline__ D operator*(F a, const D &b)
 DUAL_T __host__ __device__ __forceinline__ D operator/(const D &a, F b)
 {
     D r;
-    r.v = a.v / b;
-    const F ib = (F)1 / b;
+    r.v = dual_div(a.v, b);
+    const F ib = dual_div((F)1, b);
     for (int i = 0; i < N; ++i) r.d[i] = a.d[i] * ib;
     return r;
 }
@@ -148,8 +153,8 @@ DUAL_T __host__ __device__ __forceinline__ D operator/(F a, const D &b)
     // -(a/b) (db/b), associated so that no intermediate overflows where the
     // result does not (a/b^2 does for b = 1e-20 in FP32)
     D r;
-    r.v = a / b.v;
-    const F ib = (F)1 / b.v;
+    r.v = dual_div(a, b.v);
+    const F ib = dual_div((F)1, b.v);
     for (int i = 0; i < N; ++i) r.d[i] = -r.v * (b.d[i] * ib);
     return r;
 }
@@ -176,7 +181,7 @@ DUAL_T __device__ __forceinline__ D sqrt(const D &a)
 {
     D r;
     r.v = sqrt(a.v);
-    const F s = r.v > (F)0 ? (F)0.5 / r.v : (F)0;
+    const F s = r.v > (F)0 ? dual_div((F)0.5, r.v) : (F)0;
     for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i];
     return r;
 }
@@ -191,7 +196,7 @@ DUAL_T __device__ __forceinline__ D log(const D &a)
 {
     D r;
     r.v = log(a.v);
-    const F s = (F)1 / a.v;
+    const F s = dual_div((F)1, a.v);
     for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i];
     return r;
 }
@@ -199,7 +204,7 @@ DUAL_T __device__ __forceinline__ D pow(const D &a, F e)
 {
     D r;
     r.v = pow(a.v, e);
-    const F s = a.v != (F)0 ? e * r.v / a.v : (F)0;
+    const F s = a.v != (F)0 ? dual_div(e * r.v, a.v) : (F)0;
     for (int i = 0; i < N; ++i) r.d[i] = s * a.d[i];
     return r;
 }

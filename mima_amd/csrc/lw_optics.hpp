@@ -181,8 +181,9 @@ __device__ __forceinline__ LwRT<Real> lw_rt_exact(Real od, bool derivs)
         r.dtf = (Real)1 / (Real)6 - o2 * ((Real)1 / (Real)120 - o2 / (Real)3024);
     } else {
         const Real ome = -expm1(-od);
-        r.tf = (Real)1 - (Real)2 * ((Real)1 / od - e / ome);
-        r.dtf = derivs ? (Real)2 / (od * od) - (Real)2 * e / (ome * ome) : (Real)0;
+        const Real iod = fdiv((Real)1, od), iome = fdiv((Real)1, ome);
+        r.tf = (Real)1 - (Real)2 * (iod - e * iome);
+        r.dtf = derivs ? (Real)2 * iod * iod - (Real)2 * e * iome * iome : (Real)0;
     }
     return r;
 }

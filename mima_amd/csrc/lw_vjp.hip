@@ -279,23 +279,6 @@ __global__ __launch_bounds__(256) void lw_vjp_column_kernel(ColumnInputs in, LwV
     w.colscal[(size_t)3 * ncol + col] = amttl;
 }
 
-// sum of N values over the ng adjacent lanes of each column group (valid on
-// the group's first lane); red holds N x 64 entries
-template <int N, typename Real>
-__device__ __forceinline__ void seg_sum_n(Real (&v)[N], int lane, int ng, Real *red)
-{
-    for (int i = 0; i < N; ++i) red[i * 64 + lane] = v[i];
-    __syncthreads();
-    if (lane % ng == 0 && lane + ng <= 64) {
-        for (int i = 0; i < N; ++i) {
-            Real s = (Real)0;
-            for (int j = 0; j < ng; ++j) s += red[i * 64 + lane + j];
-            v[i] = s;
-        }
-    }
-    __syncthreads();
-}
-
 // transmittance and sources of one layer (rrtmg_lw_rtrnmr.f90:591-606) and,
 // for the backward, their derivatives with respect to the optical depth.
 // Values: the reference's thin-layer series below 0.06 and the tables above,
@@ -505,7 +488,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
         v[3] = (pend_up + a_pdn) * plkslope(lev - 1);  // level lev-1 is complete
         pend_up = a_pup;
         if (!active) v[0] = v[1] = v[2] = v[3] = (Real)0;
-        seg_sum_n<4>(v, lane, NG, red);
+        seg_sum_n<4>(v, lane, g, NG, red);
         if (active && g == 0) {
             const size_t ix = (size_t)(lev - 1) * ncol + col;
             gpart[ix] = v[0];
@@ -519,7 +502,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
         v[0] = active ? pend_up * plkslope(nlay) : (Real)0;
         v[1] = active ? a_tb : (Real)0;
         v[2] = active ? a_sec * (Real)dsd : (Real)0;
-        seg_sum_n<3>(v, lane, NG, red);
+        seg_sum_n<3>(v, lane, g, NG, red);
         if (active && g == 0) {
             glev[(size_t)nlay * ncol + col] = v[0];
             gsc[col] = v[1];

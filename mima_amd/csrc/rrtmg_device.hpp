@@ -69,4 +69,29 @@ __device__ __forceinline__ Real seg_sum(Real v, int lane, int g, int ng, Real *r
     }
 }
 
+// seg_sum of N values at once; v[i] is valid on the group's first lane.
+// `red` holds N x 64 entries (FP64 only).
+template <int N, typename Real>
+__device__ __forceinline__ void seg_sum_n(Real (&v)[N], int lane, int g, int ng, Real *red)
+{
+    if constexpr (sizeof(Real) == 4) {
+        for (int s = 1; s < ng; s <<= 1)
+            for (int i = 0; i < N; ++i) {
+                const Real o = __shfl_down(v[i], s);
+                if (g + s < ng) v[i] += o;
+            }
+    } else {
+        for (int i = 0; i < N; ++i) red[i * 64 + lane] = v[i];
+        __syncthreads();
+        if (g == 0 && lane + ng <= 64) {
+            for (int i = 0; i < N; ++i) {
+                Real s = (Real)0;
+                for (int j = 0; j < ng; ++j) s += red[i * 64 + lane + j];
+                v[i] = s;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 }  // namespace rrtmg

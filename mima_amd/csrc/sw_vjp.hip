@@ -220,21 +220,6 @@ __global__ __launch_bounds__(256) void sw_vjp_column_kernel(ColumnInputs in, SwV
     }
 }
 
-template <int N, typename Real>
-__device__ __forceinline__ void sw_seg_sum_n(Real (&v)[N], int lane, int ng, Real *red)
-{
-    for (int i = 0; i < N; ++i) red[i * 64 + lane] = v[i];
-    __syncthreads();
-    if (lane % ng == 0 && lane + ng <= 64) {
-        for (int i = 0; i < N; ++i) {
-            Real s = (Real)0;
-            for (int j = 0; j < ng; ++j) s += red[i * 64 + lane + j];
-            v[i] = s;
-        }
-    }
-    __syncthreads();
-}
-
 template <typename Real>
 __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpWork w,
                                                          const SwBand *__restrict__ bands, int nwaves, int blk0,
@@ -483,7 +468,7 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         if (lay == lsf)
             for (int i = 0; i < 3; ++i) v[i] += a_sfz * sfz3.d[i];
         if (!active) v[0] = v[1] = v[2] = (Real)0;
-        sw_seg_sum_n<3>(v, lane, ng, red);
+        seg_sum_n<3>(v, lane, g, ng, red);
         if (active && g == 0) {
             const size_t ix = (size_t)(lay - 1) * ncol + col;
             gpart[ix] = v[0];
