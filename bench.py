@@ -287,6 +287,9 @@ def main():
     ap.add_argument("--config", default="c1_mixed_vs_fp64")
     ap.add_argument("--surrogate-epochs", type=int, default=10,
                     help="training epochs of the MLP surrogate (0 skips the surrogate leg)")
+    ap.add_argument("--legs", default="forward,adjoint,surrogate,large",
+                    help="comma-separated legs to run (forward, adjoint, surrogate, large); the JSON line is the "
+                         "benchmark line only when forward is among them (the others are for profiling one leg)")
     ap.add_argument("--large-steps", type=int, default=5,
                     help="timed steps of the 131072 x 60 forward+backward leg (0 skips it)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -306,6 +309,17 @@ def main():
 
     rrtmg.rrtmg_sw_ini()
     rrtmg.rrtmg_lw_ini()
+    legs = set(args.legs.split(","))
+    if "forward" not in legs:
+        out = {"legs": sorted(legs)}
+        if "adjoint" in legs:
+            out["adjoint"] = adjoint_timing(torch, rrtmg, config_columns, args)
+        if "surrogate" in legs and args.surrogate_epochs > 0:
+            out["surrogate"] = surrogate_timing(torch, rrtmg, config_columns, args)
+        if "large" in legs and args.large_steps > 0:
+            out["large_batch"] = large_batch_timing(torch, rrtmg, config_columns, args)
+        print(json.dumps(out))
+        return
     cols = config_columns(args.config, seed=0)
     ncol, nlay = cols["ncol"], cols["nlay"]
     d = rrtmg.to_device(cols)
@@ -361,9 +375,9 @@ def main():
     bcol = 8 * (12 * nlay + 10)
     achieved = bcol * ncol / (dom_ms * 1e-3) / 1e9
 
-    adj = adjoint_timing(torch, rrtmg, config_columns, args)
-    sur = surrogate_timing(torch, rrtmg, config_columns, args) if args.surrogate_epochs > 0 else None
-    large = large_batch_timing(torch, rrtmg, config_columns, args) if args.large_steps > 0 else None
+    adj = adjoint_timing(torch, rrtmg, config_columns, args) if "adjoint" in legs else None
+    sur = surrogate_timing(torch, rrtmg, config_columns, args) if "surrogate" in legs and args.surrogate_epochs > 0 else None
+    large = large_batch_timing(torch, rrtmg, config_columns, args) if "large" in legs and args.large_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:

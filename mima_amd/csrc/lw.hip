@@ -274,10 +274,16 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
         if (active && g == 0) bf_up[col] = s;
     }
-    Real bbu_next = s_bbu[lane];
+    // the scratch row is read four levels ahead (rotating registers), so a
+    // level's load has three levels of work to complete behind
+    auto bbu_at = [&](int i) { return i < nlay ? s_bbu[(size_t)i * 64 + lane] : (Real)0; };
+    Real b0 = bbu_at(0), b1 = bbu_at(1), b2 = bbu_at(2), b3 = bbu_at(3);
     for (int lev = 1; lev <= nlay; ++lev) {
-        const Real bbugas = bbu_next;
-        if (lev < nlay) bbu_next = s_bbu[(size_t)lev * 64 + lane];  // one level ahead
+        const Real bbugas = b0;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = bbu_at(lev + 3);
         Real atrans = s_atr[(size_t)(lev - 1) * 64 + lane];
         radlu = radlu + (bbugas - radlu) * atrans;
         Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
