@@ -232,14 +232,15 @@ int rrtmg_hip_mlp_mse(int M, int N, const float *y, int ldy, const float *t, int
 int rrtmg_hip_mlp_adam(long n, float *w, float *m, float *v, const float *g, void *w_bf16,
                        float lr, float beta1, float beta2, float eps, float weight_decay,
                        int step, void *stream);
-/* features of columns as rrtmg_hip_surrogate forms them: x_bf16 [ncol][ldx]
+/* features of columns as rrtmg_hip_surrogate forms them (ldx even, at most
+ * 240; the same bound applies to RrtmgHipMlp.ldx, and nlay <= 120 there): x_bf16 [ncol][ldx]
  * (normalised when mean/istd are given), xraw [ncol][ldx] FP32 raw features
  * (may be NULL), out_of_envelope (may be NULL; needs lo/hi) */
 int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double *h2ovmr,
                            const double *o3vmr, const double *tsfc, const double *coszen,
                            const float *mean, const float *istd, const float *lo, const float *hi,
                            int ldx, void *x_bf16, float *xraw, int *out_of_envelope, void *stream);
-/* training targets t [ncol][ldt] FP32 from heating rates (ncol, nlay),
+/* training targets t [ncol][ldt] (ldt at most 240) FP32 from heating rates (ncol, nlay),
  * normalised when mean/istd are given */
 int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw,
                           const float *mean, const float *istd, int ldt, float *t, void *stream);

@@ -925,7 +925,8 @@ int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double 
                            const float *lo, const float *hi, int ldx, void *x_bf16, float *xraw, int *out_of_envelope,
                            void *stream)
 {
-    if (ncol <= 0 || nlay < 2 || ldx < 3 * nlay + 2 || !tlay || !h2ovmr || !o3vmr || !tsfc || !coszen || !x_bf16 ||
+    if (ncol <= 0 || nlay < 2 || ldx < 3 * nlay + 2 || ldx % 2 || ldx > 240 || !tlay || !h2ovmr || !o3vmr || !tsfc ||
+        !coszen || !x_bf16 ||
         (!mean) != (!istd) || (!lo) != (!hi))
         return fail(RRTMG_HIP_EINVAL, "mlp_features: bad arguments");
     hipError_t e = launch_mlp_features(ncol, nlay, tlay, h2ovmr, o3vmr, tsfc, coszen, mean, istd, lo, hi, ldx,
@@ -936,7 +937,7 @@ int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double 
 int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw, const float *mean,
                           const float *istd, int ldt, float *t, void *stream)
 {
-    if (ncol <= 0 || nlay < 2 || ldt < 2 * nlay || !hr_lw || !hr_sw || !t || (!mean) != (!istd))
+    if (ncol <= 0 || nlay < 2 || ldt < 2 * nlay || ldt > 240 || !hr_lw || !hr_sw || !t || (!mean) != (!istd))
         return fail(RRTMG_HIP_EINVAL, "mlp_targets: bad arguments");
     hipError_t e = launch_mlp_targets(ncol, nlay, hr_lw, hr_sw, mean, istd, ldt, t, (hipStream_t)stream);
     return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, hipGetErrorString(e));
@@ -951,6 +952,8 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
     if (nlay != net->nlay || net->nin != 3 * nlay + 2 || net->nout != 2 * nlay || net->nlayers < 1 ||
         net->nlayers > RRTMG_HIP_MLP_MAX_LAYERS)
         return fail(RRTMG_HIP_EINVAL, "surrogate: network shape does not match nlay");
+    if (net->ldx % 2 || net->ldx > 240 || nlay > 120)  // the layout kernels' 64-column LDS tiles
+        return fail(RRTMG_HIP_EINVAL, "surrogate: padded feature count must be even and at most 240, nlay at most 120");
     int width[RRTMG_HIP_MLP_MAX_LAYERS + 1];
     width[0] = net->ldx;
     int wmax = net->ldx;

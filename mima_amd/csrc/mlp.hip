@@ -93,9 +93,109 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
             }
 }
 
+// ---------------------------------------------------------------------------
+// C = A B^T with both operands K-contiguous (A [M][K] activations, B [N][K]
+// weights): the network's forward GEMM and dX = dY W^T's layout after
+// transposition is not needed.  Workgroup tile 128 x 128, four waves of
+// 64 x 64 (4 x 4 MFMA 16x16x16 tiles, 64 accumulator registers per lane);
+// K in steps of 32 staged through LDS with 16-byte global loads, the next
+// step's loads issued before the current step's MFMAs.
+constexpr int NB = 128;       // output tile
+constexpr int NK = 32;        // K step
+constexpr int NLD = NK + 8;   // LDS row pitch (bf16): 80 bytes, rows on different banks
+
+typedef __attribute__((__vector_size__(4 * sizeof(unsigned)))) unsigned u32x4;
+
+__global__ __launch_bounds__(256) void gemm_bf16_nn_kernel(int M, int N, int K, const uint16_t *__restrict__ A,
+                                                           int lda, const uint16_t *__restrict__ B, int ldb,
+                                                           GemmEpilogue ep)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t As[NB * NLD];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[NB * NLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * NB, n0 = blockIdx.x * NB;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    f32x4 acc[4][4];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    // each thread moves 2 chunks of 8 bf16 per operand and step: chunk c =
+    // tid + 256 q -> row c / 4, k offset 8 (c % 4)
+    u32x4 ra[2], rb[2];
+    // 8 bf16 of row `row` of a K-contiguous operand from k, zero beyond the
+    // matrix (the vector load when the whole chunk is inside)
+    auto chunk = [&](const uint16_t *__restrict__ X, int ldx_, int rows, int row, int k) -> u32x4 {
+        if (row < rows && k + 8 <= K) return *(const u32x4 *)(X + (size_t)row * ldx_ + k);
+        u32x4 v = (u32x4){0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const unsigned h = (row < rows && k + e < K) ? X[(size_t)row * ldx_ + k + e] : 0u;
+            v[e >> 1] |= h << (16 * (e & 1));
+        }
+        return v;
+    };
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = tid + 256 * q, row = c >> 2, kc = k0 + 8 * (c & 3);
+            ra[q] = chunk(A, lda, M, m0 + row, kc);
+            rb[q] = chunk(B, ldb, N, n0 + row, kc);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < K; k0 += NK) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int c = tid + 256 * q, row = c >> 2, kk = 8 * (c & 3);
+            *(u32x4 *)(As + row * NLD + kk) = ra[q];
+            *(u32x4 *)(Bs + row * NLD + kk) = rb[q];
+        }
+        __syncthreads();
+        if (k0 + NK < K) load(k0 + NK);  // next step's loads in flight during the MFMAs
+#pragma unroll
+        for (int kq = 0; kq < NK; kq += 16) {
+            const int ko = kq + 4 * (lane >> 4);
+            bf16x4 a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = *(const bf16x4 *)(As + (wm + 16 * i + (lane & 15)) * NLD + ko);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = *(const bf16x4 *)(Bs + (wn + 16 * j + (lane & 15)) * NLD + ko);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // C lane layout of the 16x16 MFMA tile: row 4*(lane/16) + r, column lane%16
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m >= M || n >= N) continue;
+                float v = acc[i][j][r];
+                if (ep.bias) v += ep.bias[n];
+                const size_t o = (size_t)m * ep.ldc + n;
+                if (ep.out_pre) ep.out_pre[o] = v;
+                if (ep.relu) v = v > 0.f ? v : 0.f;
+                if (ep.out_f32) ep.out_f32[o] = ep.accumulate ? ep.out_f32[o] + v : v;
+                if (ep.out_bf16) ep.out_bf16[o] = f2bf(v);
+            }
+}
+
 hipError_t launch_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
                             int transB, const GemmEpilogue &ep, hipStream_t s)
 {
+    // both operands K-contiguous with 16-byte aligned rows: the tiled kernel
+    if (!transA && !transB && lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0) {
+        dim3 grid((N + NB - 1) / NB, (M + NB - 1) / NB), block(256);
+        hipLaunchKernelGGL(gemm_bf16_nn_kernel, grid, block, 0, s, M, N, K, A, lda, B, ldb, ep);
+        return hipGetLastError();
+    }
     dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT), block(256);
     if (transA) {
         if (transB) hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, block, 0, s, M, N, K, A, lda, B, ldb, ep);
@@ -122,13 +222,24 @@ __global__ void relu_backward_kernel(size_t n, const float *__restrict__ dy, con
 }
 
 // per-output-column sum over rows (bias gradient): one thread per column
-__global__ void col_sum_kernel(int M, int N, const float *__restrict__ x, int ldx, float *__restrict__ out)
+__global__ __launch_bounds__(1024) void col_sum_kernel(int M, int N, const float *__restrict__ x, int ldx,
+                                                       float *__restrict__ out)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+    // 64 columns per block, 16 row groups: thread (c, r) sums rows r, r + 16, ...
+    // of column c; the 16 partial sums are added in a fixed order (deterministic)
+    __shared__ float part[16][64];
+    const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int m = 0; m < M; ++m) s += x[(size_t)m * ldx + n];
-    out[n] = s;
+    if (n < N)
+        for (int m = r; m < M; m += 16) s += x[(size_t)m * ldx + n];
+    part[r][c] = s;
+    __syncthreads();
+    if (r == 0 && n < N) {
+        float t = 0.f;
+        for (int j = 0; j < 16; ++j) t += part[j][c];
+        out[n] = t;
+    }
 }
 
 // mean squared error over M x N (row pitch ld) and its gradient 2 (y - t) / (M N)
@@ -178,62 +289,166 @@ __global__ void adam_kernel(size_t n, float *__restrict__ w, float *__restrict__
 // features of a column: T (nlay), ln q (nlay), O3 (nlay), Ts, cos zenith,
 // normalised with per-feature (mean, 1/std), BF16, row-major [ncol][ldx]
 // with zero padding; envelope flag = some feature outside [lo, hi]
-__global__ void mlp_features_kernel(int ncol, int nlay, const double *__restrict__ tlay,
-                                    const double *__restrict__ h2o, const double *__restrict__ o3,
-                                    const double *__restrict__ tsfc, const double *__restrict__ coszen,
-                                    const float *__restrict__ mean, const float *__restrict__ istd,
-                                    const float *__restrict__ lo, const float *__restrict__ hi, int ldx,
-                                    uint16_t *__restrict__ x, float *__restrict__ xraw, int *__restrict__ flag)
+// The three layout kernels below move between the library's level-major
+// column arrays ([level][column], FP64) and the network's row-major
+// [column][feature] arrays through an LDS tile of 64 columns, so that both
+// the reads and the writes are coalesced.  Block: 256 threads = 64 columns x 4.
+
+// features of 64 columns: T, ln q, O3 x 1e6 (nlay each), Ts, cos zenith;
+// x_bf16 [ncol][ldx] (normalised when mean/istd are given), xraw (FP32, may be
+// NULL), flag = 1 where some raw feature lies outside [lo, hi]
+__global__ __launch_bounds__(256) void mlp_features_kernel(int ncol, int nlay, const double *__restrict__ tlay,
+                                                           const double *__restrict__ h2o,
+                                                           const double *__restrict__ o3,
+                                                           const double *__restrict__ tsfc,
+                                                           const double *__restrict__ coszen,
+                                                           const float *__restrict__ mean,
+                                                           const float *__restrict__ istd,
+                                                           const float *__restrict__ lo, const float *__restrict__ hi,
+                                                           int ldx, uint16_t *__restrict__ x, float *__restrict__ xraw,
+                                                           int *__restrict__ flag)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncol) return;
+    extern __shared__ float tile[];  // [64][ldx + 1], then lo, hi, mean, istd [ldx] each
+    __shared__ int fl[64];
+    const int ld = ldx + 1;
+    const int col0 = blockIdx.x * 64;
+    const int t = threadIdx.x, c = t & 63, q = t >> 6;
+    const int col = col0 + c;
     const int nf = 3 * nlay + 2;
-    int out = 0;
-    for (int f = 0; f < ldx; ++f) {
-        float r = 0.f;
-        if (f < nf) {
-            double v;
-            if (f < nlay) v = tlay[(size_t)f * ncol + col];
-            else if (f < 2 * nlay) v = log(h2o[(size_t)(f - nlay) * ncol + col]);
-            else if (f < 3 * nlay) v = o3[(size_t)(f - 2 * nlay) * ncol + col] * 1.e6;
-            else if (f == 3 * nlay) v = tsfc[col];
-            else v = coszen[col];
-            r = (float)v;
-            if (lo && (r < lo[f] || r > hi[f])) out = 1;
-        }
-        if (xraw) xraw[(size_t)col * ldx + f] = r;
-        const float z = f < nf && mean ? (r - mean[f]) * istd[f] : (f < nf ? r : 0.f);
-        x[(size_t)col * ldx + f] = f2bf(z);
+    // the per-feature bounds and scalings in LDS: read per feature below,
+    // from global memory each would be a serial round trip
+    float *s_lo = tile + 64 * ld, *s_hi = s_lo + ldx, *s_mean = s_hi + ldx, *s_istd = s_mean + ldx;
+    for (int f = t; f < ldx; f += 256) {
+        s_lo[f] = lo ? lo[f] : 0.f;
+        s_hi[f] = lo ? hi[f] : 0.f;
+        s_mean[f] = mean ? mean[f] : 0.f;
+        s_istd[f] = mean ? istd[f] : 1.f;
     }
-    if (flag) flag[col] = out;
+    if (t < 64) fl[t] = 0;
+    __syncthreads();
+    // raw inputs of 8 features are loaded before any is transformed, so each
+    // wave keeps 8 loads in flight instead of one
+    // (branch-free: source array and offset are selected, then one load, so
+    // the 8 loads of a batch are not separated by waits at branch joins)
+    auto raw = [&](int f) -> double {
+        const int part = f < nlay ? 0 : f < 2 * nlay ? 1 : f < 3 * nlay ? 2 : f == 3 * nlay ? 3 : 4;
+        const double *base = part == 0 ? tlay : part == 1 ? h2o : part == 2 ? o3 : part == 3 ? tsfc : coszen;
+        const size_t off = part < 3 ? (size_t)(f - part * nlay) * ncol + col : (size_t)col;
+        const bool ok = f < nf && col < ncol;
+        return base[ok ? off : 0];
+    };
+    int out = 0;
+    for (int f0 = q; f0 < ldx; f0 += 32) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = raw(f0 + 4 * u);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int f = f0 + 4 * u;
+            if (f >= ldx) break;
+            float r = 0.f;
+            if (f < nf && col < ncol) {
+                const double x = f < nlay ? v[u] : f < 2 * nlay ? log(v[u]) : f < 3 * nlay ? v[u] * 1.e6 : v[u];
+                r = (float)x;
+                if (lo && (r < s_lo[f] || r > s_hi[f])) out = 1;
+            }
+            tile[c * ld + f] = r;
+        }
+    }
+    if (out) atomicOr(&fl[c], 1);
+    __syncthreads();
+    // rows out, two BF16 features per 32-bit word (ldx is even)
+    const int wpr = ldx / 2;
+    uint32_t *x32 = (uint32_t *)x;
+    for (int w = t; w < 64 * wpr; w += 256) {
+        const int cc = w / wpr, j = w - cc * wpr, f0 = 2 * j;
+        if (col0 + cc >= ncol) continue;
+        const float r0 = tile[cc * ld + f0], r1 = tile[cc * ld + f0 + 1];
+        const float z0 = f0 < nf ? (r0 - s_mean[f0]) * s_istd[f0] : 0.f;
+        const float z1 = f0 + 1 < nf ? (r1 - s_mean[f0 + 1]) * s_istd[f0 + 1] : 0.f;
+        x32[(size_t)(col0 + cc) * wpr + j] = (uint32_t)f2bf(z0) | ((uint32_t)f2bf(z1) << 16);
+        if (xraw) {
+            xraw[(size_t)(col0 + cc) * ldx + f0] = r0;
+            xraw[(size_t)(col0 + cc) * ldx + f0 + 1] = r1;
+        }
+    }
+    if (flag && t < 64 && col < ncol) flag[col] = fl[t];
 }
 
-// targets: (hr_lw, hr_sw) of a column normalised per output, FP32 [ncol][ldt]
-__global__ void mlp_targets_kernel(int ncol, int nlay, const double *__restrict__ hr_lw,
-                                   const double *__restrict__ hr_sw, const float *__restrict__ mean,
-                                   const float *__restrict__ istd, int ldt, float *__restrict__ t)
+// training targets: (hr_lw, hr_sw) of a column normalised per output, FP32 [ncol][ldt]
+__global__ __launch_bounds__(256) void mlp_targets_kernel(int ncol, int nlay, const double *__restrict__ hr_lw,
+                                                          const double *__restrict__ hr_sw,
+                                                          const float *__restrict__ mean,
+                                                          const float *__restrict__ istd, int ldt,
+                                                          float *__restrict__ t)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= ncol) return;
-    for (int f = 0; f < ldt; ++f) {
-        float v = 0.f;
-        if (f < 2 * nlay) {
-            const double h = f < nlay ? hr_lw[(size_t)f * ncol + col] : hr_sw[(size_t)(f - nlay) * ncol + col];
-            v = mean ? ((float)h - mean[f]) * istd[f] : (float)h;
+    extern __shared__ float tile[];  // [64][ldt + 1], then mean, istd [ldt]
+    const int ld = ldt + 1;
+    const int col0 = blockIdx.x * 64;
+    const int th = threadIdx.x, c = th & 63, q = th >> 6;
+    const int col = col0 + c;
+    float *s_mean = tile + 64 * ld, *s_istd = s_mean + ldt;
+    for (int f = th; f < ldt; f += 256) {
+        s_mean[f] = mean ? mean[f] : 0.f;
+        s_istd[f] = mean ? istd[f] : 1.f;
+    }
+    __syncthreads();
+    for (int f0 = q; f0 < ldt; f0 += 32) {
+        double h[8];  // 8 loads in flight per wave
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int f = f0 + 4 * u;
+            h[u] = f < 2 * nlay && col < ncol
+                       ? (f < nlay ? hr_lw[(size_t)f * ncol + col] : hr_sw[(size_t)(f - nlay) * ncol + col])
+                       : 0.0;
         }
-        t[(size_t)col * ldt + f] = v;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int f = f0 + 4 * u;
+            if (f >= ldt) break;
+            tile[c * ld + f] = f < 2 * nlay ? ((float)h[u] - s_mean[f]) * s_istd[f] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int w = th; w < 64 * ldt; w += 256) {
+        const int cc = w / ldt, f = w - cc * ldt;
+        if (col0 + cc < ncol) t[(size_t)(col0 + cc) * ldt + f] = tile[cc * ld + f];
     }
 }
 
 // network outputs back to heating rates (level-major FP64, the library's layout)
-__global__ void mlp_decode_kernel(int ncol, int nlay, const float *__restrict__ y, int ldy,
-                                  const float *__restrict__ mean, const float *__restrict__ std_,
-                                  double *__restrict__ hr_lw, double *__restrict__ hr_sw)
+__global__ __launch_bounds__(256) void mlp_decode_kernel(int ncol, int nlay, const float *__restrict__ y, int ldy,
+                                                         const float *__restrict__ mean,
+                                                         const float *__restrict__ std_, double *__restrict__ hr_lw,
+                                                         double *__restrict__ hr_sw)
 {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    extern __shared__ float tile[];  // [64][2 nlay + 1], then mean, std [2 nlay]
+    const int nout = 2 * nlay, ld = nout + 1;
+    const int col0 = blockIdx.x * 64;
+    const int t = threadIdx.x, c = t & 63, q = t >> 6;
+    float *s_mean = tile + 64 * ld, *s_std = s_mean + nout;
+    for (int f = t; f < nout; f += 256) {
+        s_mean[f] = mean[f];
+        s_std[f] = std_[f];
+    }
+    for (int w0 = t; w0 < 64 * nout; w0 += 256 * 8) {
+        float v[8];  // 8 loads in flight per wave
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int w = w0 + 256 * u, cc = w / nout, f = w - cc * nout;
+            v[u] = w < 64 * nout && col0 + cc < ncol ? y[(size_t)(col0 + cc) * ldy + f] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int w = w0 + 256 * u, cc = w / nout, f = w - cc * nout;
+            if (w < 64 * nout) tile[cc * ld + f] = v[u];
+        }
+    }
+    __syncthreads();
+    const int col = col0 + c;
     if (col >= ncol) return;
-    for (int f = 0; f < 2 * nlay; ++f) {
-        const double h = (double)(y[(size_t)col * ldy + f] * std_[f] + mean[f]);
+    for (int f = q; f < nout; f += 4) {
+        const double h = (double)(tile[c * ld + f] * s_std[f] + s_mean[f]);
         if (f < nlay) hr_lw[(size_t)f * ncol + col] = h;
         else hr_sw[(size_t)(f - nlay) * ncol + col] = h;
     }
@@ -246,7 +461,7 @@ hipError_t launch_relu_backward(size_t n, const float *dy, const float *pre, uin
 }
 hipError_t launch_col_sum(int M, int N, const float *x, int ldx, float *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(col_sum_kernel, dim3((N + 63) / 64), dim3(64), 0, s, M, N, x, ldx, out);
+    hipLaunchKernelGGL(col_sum_kernel, dim3((N + 63) / 64), dim3(1024), 0, s, M, N, x, ldx, out);
     return hipGetLastError();
 }
 hipError_t launch_mse(int M, int N, const float *y, int ldy, const float *t, int ldt, float *dy, uint16_t *dy_bf16,
@@ -266,22 +481,23 @@ hipError_t launch_mlp_features(int ncol, int nlay, const double *tlay, const dou
                                const float *lo, const float *hi, int ldx, uint16_t *x, float *xraw, int *flag,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL(mlp_features_kernel, dim3((ncol + 127) / 128), dim3(128), 0, s, ncol, nlay, tlay, h2o, o3, tsfc,
-                       coszen, mean, istd, lo, hi, ldx, x, xraw, flag);
+    if (ldx % 2) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mlp_features_kernel, dim3((ncol + 63) / 64), dim3(256), sizeof(float) * (64 * (ldx + 1) + 4 * ldx), s, ncol,
+                       nlay, tlay, h2o, o3, tsfc, coszen, mean, istd, lo, hi, ldx, x, xraw, flag);
     return hipGetLastError();
 }
 hipError_t launch_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw, const float *mean,
                               const float *istd, int ldt, float *t, hipStream_t s)
 {
-    hipLaunchKernelGGL(mlp_targets_kernel, dim3((ncol + 127) / 128), dim3(128), 0, s, ncol, nlay, hr_lw, hr_sw, mean,
-                       istd, ldt, t);
+    hipLaunchKernelGGL(mlp_targets_kernel, dim3((ncol + 63) / 64), dim3(256), sizeof(float) * (64 * (ldt + 1) + 2 * ldt), s, ncol,
+                       nlay, hr_lw, hr_sw, mean, istd, ldt, t);
     return hipGetLastError();
 }
 hipError_t launch_mlp_decode(int ncol, int nlay, const float *y, int ldy, const float *mean, const float *std_,
                              double *hr_lw, double *hr_sw, hipStream_t s)
 {
-    hipLaunchKernelGGL(mlp_decode_kernel, dim3((ncol + 127) / 128), dim3(128), 0, s, ncol, nlay, y, ldy, mean, std_,
-                       hr_lw, hr_sw);
+    hipLaunchKernelGGL(mlp_decode_kernel, dim3((ncol + 63) / 64), dim3(256), sizeof(float) * (64 * (2 * nlay + 1) + 4 * nlay), s,
+                       ncol, nlay, y, ldy, mean, std_, hr_lw, hr_sw);
     return hipGetLastError();
 }
 

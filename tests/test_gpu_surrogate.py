@@ -61,6 +61,30 @@ def test_gemm_bf16_matches_reference(env, ta, tb):
     _ = ctypes
 
 
+@pytest.mark.parametrize("M,N,K,lda,ldb", [(300, 200, 136, 136, 136), (257, 80, 76, 80, 88), (64, 256, 256, 256, 256)])
+def test_gemm_bf16_tiled_path_matches_reference(env, M, N, K, lda, ldb):
+    """Both operands K-contiguous with leading dimensions a multiple of 8: the
+    128 x 128 tiled kernel; sizes off the tile grid, a K tail inside a 16-byte
+    chunk, padded leading dimensions."""
+    rr, torch = env
+    from mima_amd.surrogate import _p, _stream
+    rng = np.random.default_rng(M + N + K)
+    a = rng.standard_normal((M, lda)).astype(np.float32)
+    b = rng.standard_normal((N, ldb)).astype(np.float32)
+    A, a64 = _bf(torch, a)
+    B, b64 = _bf(torch, b)
+    bias = rng.standard_normal(N).astype(np.float32)
+    bias_t = torch.from_numpy(bias).cuda()
+    pre = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    outb = torch.empty((M, N), dtype=torch.bfloat16, device="cuda")
+    rc = rr.lib().rrtmg_hip_gemm_bf16(M, N, K, _p(A), lda, 0, _p(B), ldb, 0, _p(bias_t), 1, _p(pre), None, _p(outb), N,
+                                      0, _stream())
+    assert rc == 0
+    ref = a64[:, :K] @ b64[:, :K].T + bias
+    np.testing.assert_allclose(pre.cpu().numpy(), ref, rtol=1e-5, atol=1e-4)
+    np.testing.assert_array_equal(outb.float().cpu().numpy(), mo.bf16_round(np.maximum(pre.cpu().numpy(), 0)))
+
+
 def test_training_step_gradients_and_adam_match_cpu_reference(env):
     rr, torch = env
     from mima_amd.surrogate import Surrogate
