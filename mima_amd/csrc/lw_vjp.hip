@@ -7,11 +7,13 @@
 // rrtmg_lw (rrtmg_lw_rad.nomcica.f90:80) as the forward kernels implement it.
 //
 // Kernels:
-//  * prep (one thread per column): inatm + setcoef in FP64 dual numbers
-//    seeded with (tlay, h2o, o3) of each layer, writing every per-layer
-//    coefficient with its three derivatives ([field][value, d/dT, d/dq,
-//    d/dO3][lay][col]); level Planck indices; and the per-level radiance
-//    cotangents after folding the heating-rate cotangent into the fluxes.
+//  * prep (one thread per (layer, column)): inatm + setcoef in FP64 dual
+//    numbers seeded with (tlay, h2o, o3) of each layer, writing every
+//    per-layer coefficient with its three derivatives ([field][value, d/dT,
+//    d/dq, d/dO3][lay][col]); level Planck indices; and the per-level
+//    radiance cotangents after folding the heating-rate cotangent into the
+//    fluxes; then a per-column kernel for laytrop, the column water and
+//    dry-air totals, precipitable water and surface temperature.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
 //    pass A, downward sweep (values), storing per layer the transmittance
 //    and the upward source in LDS and the incoming downward radiance in a
@@ -26,8 +28,8 @@
 //    fraction and Planck integrals are analytic (for table lookups, those of
 //    the functions the tables tabulate), and the per-lane gradient is reduced
 //    over the band's g-points.
-//  * finish (one thread per column): sums over bands in FP64 and the
-//    precipitable-water chain of the diffusivity angle.
+//  * finish (one thread per (layer, column)): sums over bands in FP64 and
+//    the precipitable-water chain of the diffusivity angle.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -5,10 +5,11 @@
 // (rrtmg_sw_rad.nomcica.f90:78) as the forward kernels implement it.
 //
 // Kernels:
-//  * prep (one thread per column): inatm_sw + setcoef_sw in FP64 dual numbers
-//    seeded with (tlay, h2o, o3) of each layer ([field][value, d/dT, d/dq,
-//    d/dO3][lay][col]), the solar-source reference layers, and the level
-//    cotangents after folding the heating-rate cotangent into the fluxes.
+//  * prep (one thread per (layer, column)): inatm_sw + setcoef_sw in FP64
+//    dual numbers seeded with (tlay, h2o, o3) of each layer ([field][value,
+//    d/dT, d/dq, d/dO3][lay][col]) and the level cotangents after folding the
+//    heating-rate cotangent into the fluxes; then a per-column kernel for
+//    laytrop and the solar-source reference layers.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
 //    pass 1, the adding method's bottom-up sweep (values; reflectances of the
 //    atmosphere below each level kept in LDS); pass 2, top-down (values): the
@@ -20,8 +21,10 @@
 //    (T, q, O3) through its optics recomputed as dual numbers: gas/Rayleigh
 //    optical depths in (T, q, O3), two-stream optics in those two optical
 //    depths; plus the solar source's dependence on its reference layer.
-//    Per-lane gradients are reduced over the band's g-points.
-//  * finish (one thread per column): sums over bands in FP64.
+//    The derivatives are those of the smooth functions behind the exp table
+//    (exp(-x), floored at 1e-20) in every precision mode.  Per-lane gradients
+//    are reduced over the band's g-points.
+//  * finish (one thread per (layer, column)): sums over bands in FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
