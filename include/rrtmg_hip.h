@@ -58,7 +58,10 @@ extern "C" {
 int rrtmg_hip_sw_init(double cpdair, const char *table_dir);
 
 /* Replaces rrtmg_lw_ini(cpdair) (rrtmg_lw/gcm_model/src/rrtmg_lw_init.f90:28):
- * 256 -> 140 g-points, tau/exp/tfn tables, Planck table.  Idempotent. */
+ * 256 -> 140 g-points, tau/exp/tfn tables, Planck table.  Idempotent.
+ * Environment RRTMG_HIP_LW_LDS=0, read at the first call on a device, selects
+ * the one-wave FP32 band kernel instead of the LDS-table kernel (bitwise the
+ * same results). */
 int rrtmg_hip_lw_init(double cpdair, const char *table_dir);
 
 /* Selects the arithmetic of subsequent calls (RRTMG_HIP_PREC_*); returns the

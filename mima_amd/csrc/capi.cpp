@@ -63,6 +63,9 @@ struct SideTables {
     WaveUnit *units = nullptr;
     double *preflog = nullptr, *tref = nullptr;  // SW
     LwConst *cst = nullptr;                      // LW
+    LwBand *bands_lds = nullptr;                 // LW: offsets relative to each band's first k-table element
+    LwLdsPlan plan{};                            // LW: schedule of the LDS-table kernel
+    bool lds = false;                            // LW: that kernel fits this device
     double cpdair = 0.0;
 };
 
@@ -154,6 +157,9 @@ int check_common(int ncol, int nlay, const double *plev, int ptr_kind)
 {
     if (ncol <= 0) return fail(RRTMG_HIP_EINVAL, "ncol must be positive");
     if (nlay < 2 || nlay > RRTMG_HIP_MAX_NLAY) return fail(RRTMG_HIP_EINVAL, "nlay out of range [2, 200]");
+    // the band kernels address each per-level field with 32-bit byte offsets
+    if ((size_t)(nlay + 1) * (size_t)ncol * sizeof(double) >= ((size_t)1 << 32))
+        return fail(RRTMG_HIP_ENOTSUP, "ncol x (nlay+1) too large for one call; split the columns");
     if (ptr_kind != RRTMG_HIP_HOST && ptr_kind != RRTMG_HIP_DEVICE) return fail(RRTMG_HIP_EINVAL, "bad ptr_kind");
     if (ptr_kind == RRTMG_HIP_HOST && plev) {
         for (int i = 0; i < ncol; ++i)
@@ -268,6 +274,64 @@ int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
     return 0;
 }
 
+// The LDS-table schedule of the FP32 LW band kernel: each band's k-tables
+// are one contiguous range of the packed buffer (tables.cpp packs them band
+// by band, before the transmittance tables); every k-table offset of the band
+// must lie in it, and is rebased to its start for the LDS copy.
+static bool make_lw_plan(const LwHost &h, LwLdsPlan &p, LwBand *rebased, std::string &err)
+{
+    for (int b = 0; b < 16; ++b) {
+        const LwBand &d = h.tab.band[b];
+        const int offs[] = {d.absa,   d.absb,   d.selfref, d.forref,  d.fracrefa, d.fracrefb, d.ka_mn2,
+                            d.kb_mn2, d.ka_mn2o, d.kb_mn2o, d.ka_mo3, d.kb_mo3,   d.ka_mco2,  d.kb_mco2,
+                            d.ka_mco, d.ka_mo2, d.kb_mo2,  d.ccl4,   d.cfc11adj, d.cfc12,   d.cfc22adj};
+        int lo = 1 << 30;
+        for (int o : offs)
+            if (o >= 0) lo = std::min(lo, o);
+        p.tab0[b] = lo;
+    }
+    p.tab0[16] = h.tab.exp_tbl;
+    p.lds_bytes = 0;
+    for (int b = 0; b < 16; ++b) {
+        const int lo = p.tab0[b], hi = p.tab0[b + 1];
+        if (!(lo < hi)) {
+            err = "LW k-tables are not packed band by band";
+            return false;
+        }
+        p.lds_bytes = std::max(p.lds_bytes, (int)sizeof(float) * (hi - lo));
+        LwBand r = h.tab.band[b];
+        int *fields[] = {&r.absa,   &r.absb,   &r.selfref, &r.forref,  &r.fracrefa, &r.fracrefb, &r.ka_mn2,
+                         &r.kb_mn2, &r.ka_mn2o, &r.kb_mn2o, &r.ka_mo3, &r.kb_mo3,   &r.ka_mco2,  &r.kb_mco2,
+                         &r.ka_mco, &r.ka_mo2, &r.kb_mo2,  &r.ccl4,   &r.cfc11adj, &r.cfc12,   &r.cfc22adj};
+        for (int *f : fields) {
+            if (*f < 0) continue;
+            if (*f < lo || *f >= hi) {
+                err = "LW k-table offset outside its band's range";
+                return false;
+            }
+            *f -= lo;
+        }
+        rebased[b] = r;
+        p.ubeg[b] = -1;
+        p.upt[b] = 0;
+    }
+    for (int i = 0; i < (int)h.units.size(); ++i) {
+        const int b = h.units[i].band;
+        if (p.ubeg[b] < 0) p.ubeg[b] = i;
+        if (p.ubeg[b] + p.upt[b] != i) {
+            err = "LW wave units of a band are not contiguous";
+            return false;
+        }
+        ++p.upt[b];
+    }
+    for (int b = 0; b < 16; ++b)
+        if (p.upt[b] < 1) {
+            err = "LW band without wave units";
+            return false;
+        }
+    return true;
+}
+
 int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
 {
     DeviceState *s = state();
@@ -288,7 +352,20 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     if (!rc) rc = upload((LwBand **)&s->lwd.bands, s->lw.tab.band, sizeof s->lw.tab.band);
     if (!rc) rc = upload(&s->lwd.units, s->lw.units.data(), s->lw.units.size() * sizeof(WaveUnit));
     if (!rc) rc = upload(&s->lwd.cst, &s->lw.cst, sizeof(LwConst));
+    LwBand rebased[16];
+    if (!rc && !make_lw_plan(s->lw, s->lwd.plan, rebased, err)) return fail(RRTMG_HIP_ENOENT, err);
+    if (!rc) rc = upload(&s->lwd.bands_lds, rebased, sizeof rebased);
     if (rc) return fail(rc, "uploading LW tables failed");
+    {
+        int dev = 0, ncu = 0, lds = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            return fail(RRTMG_HIP_EDEVICE, "device attribute query failed");
+        s->lwd.plan.ncu = std::max(ncu, 1);
+        const char *env = std::getenv("RRTMG_HIP_LW_LDS");
+        s->lwd.lds = s->lwd.plan.lds_bytes <= lds && !(env && env[0] == '0');
+    }
     s->lwd.cpdair = cpdair;
     s->lwd.ready = true;
     return 0;
@@ -474,7 +551,11 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     const size_t o_cs = off; off = align_up(off + sizeof(double) * 3 * ncol);
     const size_t o_bf = off; off = align_up(off + rs * 16 * 2 * n3);
     const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->lw.tab.nwaves, lw_fwd_state_bytes(nlay, rs));
-    const size_t o_ds = off; off = align_up(off + (size_t)chunk * lw_fwd_state_bytes(nlay, rs));
+    const bool lds = mixed && s->lwd.lds;
+    const size_t ds_bytes = lds ? (size_t)lw_lds_grid(s->lwd.plan, ncol, s->lw.tab.nwaves) * LW_WGW *
+                                      lw_lds_state_bytes(nlay)
+                                : (size_t)chunk * lw_fwd_state_bytes(nlay, rs);
+    const size_t o_ds = off; off = align_up(off + ds_bytes);
     if ((rc = s->lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->lw_work.p;
     LwWork w;
@@ -541,6 +622,9 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     t.cst = s->lwd.cst;
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
+    t.bands_lds = s->lwd.bands_lds;
+    t.plan = s->lwd.plan;
+    t.lds = lds;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "LW: host-to-device copy failed");
     hipError_t e = launch_lw(in, out, w, t, g_prec, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));

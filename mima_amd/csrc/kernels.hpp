@@ -55,6 +55,28 @@ struct SwDevTables {
     const WaveUnit *units;
     double heatfac;
 };
+// Schedule of the FP32 LW band kernel that keeps one band's k-tables in LDS:
+// persistent workgroups of LW_WGW waves; work item = (group of LW_WGW tiles,
+// unit r of a tile's unit list), whose LW_WGW waves take the group's units of
+// the band of unit r.
+constexpr int LW_WGW = 16;
+struct LwLdsPlan {
+    int tab0[17];   // first element of each band's k-tables in the packed buffer; [16]: end of band 16's
+    int ubeg[16];   // first unit of each band in a tile's unit list
+    int upt[16];    // units of each band per tile
+    int lds_bytes;  // largest band's k-tables in FP32
+    int ncu;        // compute units: workgroups of one launch
+};
+// rows of per-wave global scratch (transmittance and upward source, nlay x 64
+// each) the LDS-table kernel uses: one per wave of its persistent grid
+inline int lw_lds_grid(const LwLdsPlan &p, int ncol, int nwaves)
+{
+    const int ntiles = (ncol + TILE_COLS - 1) / TILE_COLS;
+    const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * nwaves;
+    return nitems < p.ncu ? nitems : p.ncu;
+}
+inline size_t lw_lds_state_bytes(int nlay) { return sizeof(float) * 2 * (size_t)nlay * 64; }
+
 struct LwDevTables {
     LwTables desc;
     const LwBand *bands;      // device copy of desc.band
@@ -62,6 +84,9 @@ struct LwDevTables {
     const LwConst *cst;       // device copy
     const WaveUnit *units;
     double heatfac;
+    const LwBand *bands_lds = nullptr;  // desc.band with k-table offsets relative to the band's first element
+    LwLdsPlan plan{};
+    bool lds = false;                   // FP32 forward uses the LDS-table kernel
 };
 
 // ---------------------------------------------------------------------------

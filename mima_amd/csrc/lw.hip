@@ -10,6 +10,13 @@
 // (atrans, bbugas) per layer in LDS; the surface term and an upward sweep
 // follow.  Radiances are summed over the band's g-points of a column
 // (segmented sum over ng adjacent lanes) into per-band level fluxes.
+// FP32 runs the same body in lw_band_lds_kernel: persistent 16-wave
+// workgroups that copy one band's k-tables (at most 128 KB) into LDS, so the
+// ~30 k-table gathers per layer take LDS latency rather than L2 latency
+// (64-lane gathers across four columns miss the 32 KB L1); a 16-wave
+// workgroup also holds the kernel at 128 VGPRs, four waves per SIMD.  FP64
+// (whose bands 3-5 exceed 160 KB of LDS) and RRTMG_HIP_LW_LDS=0 use
+// lw_band_kernel, one wave per workgroup.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -178,28 +185,30 @@ __global__ __launch_bounds__(256) void lw_column_kernel(ColumnInputs in, LwWork 
     w.colscal[(size_t)ncol + col] = in.tsfc[col];
 }
 
-// One band of one work unit (rrtmg_lw_rtrnmr.f90:483-716, clear sky)
+// One band of one work unit (rrtmg_lw_rtrnmr.f90:483-716, clear sky).
+// ktab: the k-tables (B's offsets index it; the packed global buffer, or the
+// band's copy in LDS with rebased offsets); tab: the packed global buffer
+// (transmittance and Planck tables).  s_atr, s_bbu: transmittance and upward
+// source per layer, [nlay][64] each (LDS or this wave's global scratch row);
+// s_red: 64 LDS entries for the FP64 g-sum.
 template <int BAND, typename Real>
-__device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBand &B, const Real *__restrict__ tab,
-                             int exp_off, int tfn_off, int plk_off, const LwConst *__restrict__ cst, WaveUnit u,
-                             int tile, Real *smem, Real *__restrict__ s_bbu)
+__device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBand &B, const Real *__restrict__ ktab,
+                             const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
+                             const LwConst *__restrict__ cst, WaveUnit u, int tile, int lane,
+                             Real *__restrict__ s_atr, Real *__restrict__ s_bbu, Real *s_red)
 {
     constexpr int NG = lw_ng(BAND);
     const int ncol = in.ncol, nlay = in.nlay;
-    const int lane = threadIdx.x;
     const int cl = lane / NG, g = lane - cl * NG;
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol;
     const int col = active ? col0 : 0;
-    Real *s_atr = smem;                            // [nlay][64], LDS
-    Real *s_red = smem + (size_t)nlay * 64;        // [64]
-    // s_bbu: upward source per layer, [nlay][64] in this wave's global scratch row
     const Real *coef = (const Real *)w.coef;
     const Real *tlevf = (const Real *)w.tlev;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     const Real *tfn_tbl = tfn_off >= 0 ? tab + tfn_off : nullptr;
     const Real *plk = tab + plk_off + (BAND - 1) * 181;  // totplnk(:, BAND)
-    LTab<Real> T{tab, NG, g};
+    LTab<Real> T{ktab, NG, g};
     Real *bf_up = (Real *)w.bflux + (size_t)(BAND - 1) * 2 * (nlay + 1) * ncol;
     Real *bf_dn = bf_up + (size_t)(nlay + 1) * ncol;
 
@@ -221,16 +230,16 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
     const Real plankbnd =
         (Real)((double)semiss * ((double)plk[indbound - 1] + tbndfrac * ((double)plk[indbound] - (double)plk[indbound - 1])));
     auto planklev = [&](int lev) {
-        int i = w.ilev[(size_t)lev * ncol + col];
-        Real f = tlevf[(size_t)lev * ncol + col];
-        return plk[i - 1] + f * (plk[i] - plk[i - 1]);
+        int i = at32(w.ilev, (unsigned)(lev * ncol + col));
+        Real f = at32(tlevf, (unsigned)(lev * ncol + col));
+        return at32(plk, (unsigned)(i - 1)) + f * (at32(plk, (unsigned)i) - at32(plk, (unsigned)(i - 1)));
     };
     const double *tauaer = in.tauaer;
 
     // downward sweep
     Real radld = (Real)0, fracs1 = (Real)0;
     Real plev_up = planklev(nlay);
-    if (active && g == 0) bf_dn[(size_t)nlay * ncol + col] = (Real)0;  // drad(nlayers) stays 0
+    if (active && g == 0) at32(bf_dn, (unsigned)(nlay * ncol + col)) = (Real)0;  // drad(nlayers) stays 0
     LwLayer<Real> c;
     for (int lev = nlay; lev >= 1; --lev) {
         lw_load_layer(c, coef, w.ipk, lev - 1, col, ncol, nlay);
@@ -238,7 +247,8 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
         lw_gas<BAND, Real>(B, T, cst, c, lev <= laytrop, taug, plfrac);
         if (tauaer) taug = taug + (Real)tauaer[((size_t)(BAND - 1) * nlay + (lev - 1)) * ncol + col];
         Real plev_dn = planklev(lev - 1);
-        Real blay = plk[c.indlay - 1] + c.tlayfrac * (plk[c.indlay] - plk[c.indlay - 1]);
+        Real blay = at32(plk, (unsigned)(c.indlay - 1)) +
+                    c.tlayfrac * (at32(plk, (unsigned)c.indlay) - at32(plk, (unsigned)(c.indlay - 1)));
         Real dplankup = plev_up - blay, dplankdn = plev_dn - blay;
         Real odepth = secdiff * taug;
         if (odepth < (Real)0) odepth = (Real)0;
@@ -255,16 +265,16 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
             bbugas = plfrac * (blay + dplankup * odepth);
         } else {
             const int itr = tbl_index(odepth);
-            atrans = (Real)1 - exp_tbl[itr];
-            const Real tausfac = tfn_tbl[itr];
+            atrans = (Real)1 - at32(exp_tbl, (unsigned)itr);
+            const Real tausfac = at32(tfn_tbl, (unsigned)itr);
             bbd = plfrac * (blay + tausfac * dplankdn);
             bbugas = plfrac * (blay + tausfac * dplankup);
         }
         radld = radld + (bbd - radld) * atrans;
-        s_atr[(size_t)(lev - 1) * 64 + lane] = atrans;
-        s_bbu[(size_t)(lev - 1) * 64 + lane] = bbugas;
+        at32(s_atr, (unsigned)((lev - 1) * 64 + lane)) = atrans;
+        at32(s_bbu, (unsigned)((lev - 1) * 64 + lane)) = bbugas;
         Real s = seg_sum(active ? radld : (Real)0, lane, g, NG, s_red);
-        if (active && g == 0) bf_dn[(size_t)(lev - 1) * ncol + col] = s;
+        if (active && g == 0) at32(bf_dn, (unsigned)((lev - 1) * ncol + col)) = s;
         plev_up = plev_dn;
         if (lev == 1) fracs1 = plfrac;
     }
@@ -272,22 +282,26 @@ __device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBa
     Real radlu = fracs1 * plankbnd + ((Real)1 - semiss) * radld;
     {
         Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
-        if (active && g == 0) bf_up[col] = s;
+        if (active && g == 0) at32(bf_up, (unsigned)col) = s;
     }
-    // the scratch row is read four levels ahead (rotating registers), so a
-    // level's load has three levels of work to complete behind
-    auto bbu_at = [&](int i) { return i < nlay ? s_bbu[(size_t)i * 64 + lane] : (Real)0; };
-    Real b0 = bbu_at(0), b1 = bbu_at(1), b2 = bbu_at(2), b3 = bbu_at(3);
+    // the per-layer states are read four levels ahead (rotating registers),
+    // so a level's load has three levels of work to complete behind
+    auto at = [&](const Real *p, int i) { return i < nlay ? at32(p, (unsigned)(i * 64 + lane)) : (Real)0; };
+    Real b0 = at(s_bbu, 0), b1 = at(s_bbu, 1), b2 = at(s_bbu, 2), b3 = at(s_bbu, 3);
+    Real a0 = at(s_atr, 0), a1 = at(s_atr, 1), a2 = at(s_atr, 2), a3 = at(s_atr, 3);
     for (int lev = 1; lev <= nlay; ++lev) {
-        const Real bbugas = b0;
+        const Real bbugas = b0, atrans = a0;
         b0 = b1;
         b1 = b2;
         b2 = b3;
-        b3 = bbu_at(lev + 3);
-        Real atrans = s_atr[(size_t)(lev - 1) * 64 + lane];
+        b3 = at(s_bbu, lev + 3);
+        a0 = a1;
+        a1 = a2;
+        a2 = a3;
+        a3 = at(s_atr, lev + 3);
         radlu = radlu + (bbugas - radlu) * atrans;
         Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
-        if (active && g == 0) bf_up[(size_t)lev * ncol + col] = s;
+        if (active && g == 0) at32(bf_up, (unsigned)(lev * ncol + col)) = s;
     }
 }
 
@@ -305,15 +319,73 @@ __global__ __launch_bounds__(64) void lw_band_kernel(ColumnInputs in, LwWork w, 
     const WaveUnit u = units[(blk0 + blk) % nwaves];
     const LwBand B = bands[u.band];
     Real *s_bbu = (Real *)w.dstate + (size_t)blk * in.nlay * 64;
-#define BODY(b)                                                                                          \
-    case b - 1:                                                                                          \
-        lw_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem, s_bbu);       \
+    Real *s_atr = smem, *s_red = smem + (size_t)in.nlay * 64;
+#define BODY(b)                                                                                               \
+    case b - 1:                                                                                               \
+        lw_band_body<b, Real>(in, w, B, tab, tab, exp_off, tfn_off, plk_off, cst, u, tile, (int)threadIdx.x, s_atr, \
+                              s_bbu, s_red);                                                                  \
         break;
     switch (u.band) {
         BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
         BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
     }
 #undef BODY
+}
+
+// FP32 band kernel with the band's k-tables in LDS.  Persistent workgroups of
+// LW_WGW waves run work items item = blockIdx.x, blockIdx.x + gridDim.x, ...;
+// item = (group of LW_WGW tiles, unit r of a tile's unit list) covers the
+// group's LW_WGW x upt units of the band of unit r, one per wave.  Before an
+// item of another band the workgroup copies that band's k-tables (at most
+// plan.lds_bytes) into LDS; gathers from them then take LDS latency instead
+// of the L2 latency of 64-lane gathers that miss the 32 KB L1.  The
+// transmittance and upward source of each wave go to its global scratch row
+// (2 x nlay x 64; rows = waves of the grid, so they stay in the last-level
+// cache).
+template <typename Real>
+__global__ __launch_bounds__(64 * LW_WGW) void lw_band_lds_kernel(ColumnInputs in, LwWork w,
+                                                                  const LwBand *__restrict__ bands, LwLdsPlan plan,
+                                                                  int nwaves, int nitems, int exp_off, int tfn_off,
+                                                                  int plk_off, const Real *__restrict__ tab,
+                                                                  const LwConst *__restrict__ cst,
+                                                                  const WaveUnit *__restrict__ units)
+{
+    static_assert(sizeof(Real) == 4, "FP32 only: the FP64 g-sum synchronises the workgroup");
+    extern __shared__ unsigned char smem_raw[];
+    Real *ktab = (Real *)smem_raw;
+    const int nlay = in.nlay;
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    Real *s_atr = (Real *)w.dstate + ((size_t)blockIdx.x * LW_WGW + wave) * 2 * nlay * 64;
+    Real *s_bbu = s_atr + (size_t)nlay * 64;
+    int loaded = -1;
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int group = item / nwaves, r = item - group * nwaves;
+        const int b = units[r].band;
+        if (b != loaded) {
+            __syncthreads();  // every wave is done with the previous band's tables
+            const int t0 = plan.tab0[b], n = plan.tab0[b + 1] - t0;
+#pragma unroll 8
+            for (int i = threadIdx.x; i < n; i += 64 * LW_WGW) ktab[i] = tab[t0 + i];
+            __syncthreads();
+            loaded = b;
+        }
+        const int k = (r - plan.ubeg[b]) * LW_WGW + wave;
+        const int tile = group * LW_WGW + k / plan.upt[b];
+        if (tile >= ntiles) continue;
+        const WaveUnit u = units[plan.ubeg[b] + k % plan.upt[b]];
+        const LwBand B = bands[b];
+#define BODY(bb)                                                                                              \
+    case bb - 1:                                                                                              \
+        lw_band_body<bb, Real>(in, w, B, ktab, tab, exp_off, tfn_off, plk_off, cst, u, tile, lane, s_atr, s_bbu, \
+                               nullptr);                                                                      \
+        break;
+        switch (b) {
+            BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
+            BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
+        }
+#undef BODY
+    }
 }
 
 // finish: band sums, fluxfac, net flux and heating (rrtmg_lw_rtrnmr.f90:719-777),
@@ -369,6 +441,19 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     const size_t shmem = lw_fwd_lds_bytes(in.nlay, sizeof(Real));
     if (ev) (void)hipEventRecord(ev[1], s);
     const int nblocks = ntiles * t.desc.nwaves;
+    if constexpr (sizeof(Real) == 4) {
+        if (t.lds && !smooth) {
+            const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * t.desc.nwaves;
+            hipLaunchKernelGGL(lw_band_lds_kernel<Real>, dim3(lw_lds_grid(t.plan, in.ncol, t.desc.nwaves)),
+                               dim3(64 * LW_WGW), (size_t)t.plan.lds_bytes, s, in, w, t.bands_lds, t.plan,
+                               t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,
+                               (const Real *)t.tab, t.cst, t.units);
+            if (ev) (void)hipEventRecord(ev[2], s);
+            hipLaunchKernelGGL(lw_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, w, out, t.bands, heatfac);
+            if (ev) (void)hipEventRecord(ev[3], s);
+            return hipGetLastError();
+        }
+    }
     // launches of at most w.chunk wave units, each reusing the scratch rows
     for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
         const int nb_c = std::min(w.chunk, nblocks - b0);

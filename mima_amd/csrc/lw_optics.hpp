@@ -28,8 +28,9 @@ template <typename Real>
 __device__ __forceinline__ void lw_load_layer(LwLayer<Real> &c, const Real *__restrict__ coef,
                                               const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t ix = (size_t)lay * ncol + col, fs = (size_t)nlay * ncol;
-#define LD(f, F) c.f = coef[F * fs + ix]
+    const unsigned ix = (unsigned)(lay * ncol + col);
+    const size_t fs = (size_t)nlay * ncol;
+#define LD(f, F) c.f = at32(coef + F * fs, ix)
     LD(fac00, LWF_FAC00); LD(fac01, LWF_FAC01); LD(fac10, LWF_FAC10); LD(fac11, LWF_FAC11);
     LD(colh2o, LWF_COLH2O); LD(colco2, LWF_COLCO2); LD(colo3, LWF_COLO3); LD(coln2o, LWF_COLN2O);
     LD(colco, LWF_COLCO); LD(colch4, LWF_COLCH4); LD(colo2, LWF_COLO2); LD(colbrd, LWF_COLBRD);
@@ -38,7 +39,7 @@ __device__ __forceinline__ void lw_load_layer(LwLayer<Real> &c, const Real *__re
     LD(coldry, LWF_COLDRY); LD(pavel, LWF_PAVEL); LD(wx1, LWF_WX1); LD(wx2, LWF_WX2); LD(wx3, LWF_WX3);
     LD(wx4, LWF_WX4); LD(tlayfrac, LWF_TLAYFRAC);
 #undef LD
-    int w = ipk[ix];
+    int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
     c.jt = unpack_jt(w);
     c.jt1 = unpack_jt1(w);
@@ -188,7 +189,7 @@ __device__ __forceinline__ LwRT<Real> lw_rt_exact(Real od, bool derivs)
     return r;
 }
 
-#define CHI(i, j) cst->chi_mls[((j)-1) * 7 + ((i)-1)]
+#define CHI(i, j) at32(cst->chi_mls, (unsigned)(((j)-1) * 7 + ((i)-1)))
 
 // Optical depth and Planck fraction of one (layer, g) of band BAND,
 // rrtmg_lw_taumol.f90:280-3147.
@@ -231,8 +232,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 3) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 3) / CHI(2, 3)), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 9) / CHI(2, 9)), (Real)8);
             V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
@@ -241,8 +242,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
                    adj * lw_minor2(T, B.ka_mn2o, 9, em, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp]), (Real)4);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp + 1]), (Real)4);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)4);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)4);
             Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 13) / CHI(2, 13)), (Real)4);
             Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 13) / CHI(2, 13)), (Real)4);
             V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
@@ -253,15 +254,15 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 4) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 11) / CHI(2, 11)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colo3, c.colco2, R(cst->rat_o3co2[jp]), (Real)4);
-            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(cst->rat_o3co2[jp + 1]), (Real)4);
+            Eta<V> e = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp))), (Real)4);
+            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp + 1))), (Real)4);
             Eta<V> ep = lw_eta(c.colo3, c.colco2, R(CHI(3, 13) / CHI(2, 13)), (Real)4);
             taug = lw_major_b(T, B.absb, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major_b(T, B.absb, e1, ind1 + e1.js, c.fac01, c.fac11);
@@ -275,8 +276,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 5) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 7) / CHI(2, 7)), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 5) / CHI(2, 5)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
@@ -284,8 +285,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
                    lw_minor2(T, B.ka_mo3, 9, em, c) * c.colo3 + c.wx1 * T(B.ccl4, 1);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colo3, c.colco2, R(cst->rat_o3co2[jp]), (Real)4);
-            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(cst->rat_o3co2[jp + 1]), (Real)4);
+            Eta<V> e = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp))), (Real)4);
+            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp + 1))), (Real)4);
             Eta<V> ep = lw_eta(c.colo3, c.colco2, R(CHI(3, 43) / CHI(2, 43)), (Real)4);
             taug = lw_major_b(T, B.absb, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major_b(T, B.absb, e1, ind1 + e1.js, c.fac01, c.fac11) + c.wx1 * T(B.ccl4, 1);
@@ -302,8 +303,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         fracs = T(B.fracrefa, 1);
     } else if constexpr (BAND == 7) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colo3, R(cst->rat_h2oo3[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colo3, R(cst->rat_h2oo3[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colo3, R(at32(cst->rat_h2oo3, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colo3, R(at32(cst->rat_h2oo3, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.colh2o, c.colo3, R(CHI(1, 3) / CHI(3, 3)), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colo3, R(CHI(1, 3) / CHI(3, 3)), (Real)8);
             V adj = lw_adjcol(c.colco2, c.coldry, R(CHI(2, jp + 1)), (Real)3, (Real)3, (Real)0.79);
@@ -337,8 +338,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
     } else if constexpr (BAND == 9) {
         V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(cst->rat_h2och4[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(cst->rat_h2och4[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.colh2o, c.colch4, R(CHI(1, 3) / CHI(6, 3)), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colch4, R(CHI(1, 9) / CHI(6, 9)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
@@ -370,8 +371,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 12) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(cst->rat_h2oco2[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 10) / CHI(2, 10)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);
@@ -382,8 +383,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 13) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.coln2o, R(cst->rat_h2on2o[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.coln2o, R(cst->rat_h2on2o[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.coln2o, R(at32(cst->rat_h2on2o, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.coln2o, R(at32(cst->rat_h2on2o, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 1) / CHI(4, 1)), (Real)8);
             Eta<V> emco = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 3) / CHI(4, 3)), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 5) / CHI(4, 5)), (Real)8);
@@ -406,8 +407,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 15) {
         if (lower) {
-            Eta<V> e = lw_eta(c.coln2o, c.colco2, R(cst->rat_n2oco2[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.coln2o, c.colco2, R(cst->rat_n2oco2[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.coln2o, c.colco2, R(at32(cst->rat_n2oco2, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.coln2o, c.colco2, R(at32(cst->rat_n2oco2, (unsigned)(jp + 1))), (Real)8);
             Eta<V> em = lw_eta(c.coln2o, c.colco2, R(CHI(4, 1) / CHI(2, 1)), (Real)8);
             Eta<V> ep = lw_eta(c.coln2o, c.colco2, R(CHI(4, 1) / CHI(2, 1)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
@@ -420,8 +421,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else {  // 16; aloft nspb(16) = 0, so ind0 = ind1 = 1 (rrtmg_lw_taumol.f90:3135-3136)
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(cst->rat_h2och4[jp]), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(cst->rat_h2och4[jp + 1]), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp))), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp + 1))), (Real)8);
             Eta<V> ep = lw_eta(c.colh2o, c.colch4, R(CHI(1, 6) / CHI(6, 6)), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);

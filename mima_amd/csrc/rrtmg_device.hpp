@@ -43,6 +43,21 @@ __device__ __forceinline__ int xcd_block()
     return ((int)blockIdx.x % 8) * per + (int)blockIdx.x / 8;
 }
 
+// Element i of an array whose uniform base sits in SGPRs, addressed by a
+// 32-bit unsigned byte offset (one global access with a VGPR offset and no
+// per-lane 64-bit address arithmetic).  Valid for arrays below 4 GiB, which
+// the C ABI checks for the per-field arrays it passes to the band kernels.
+template <typename T>
+__device__ __forceinline__ T &at32(T *base, unsigned i)
+{
+    return *(T *)((char *)base + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ const T &at32(const T *base, unsigned i)
+{
+    return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
+}
+
 // Sum of v over the ng adjacent lanes of each column group of a one-wave
 // block (g = the lane's position in its group); valid on the group's first
 // lane.  Deterministic (fixed order).  FP64: sequential through LDS, the
