@@ -249,6 +249,18 @@ int rrtmg_hip_set_precision(int mode)
     return prev;
 }
 
+// Device attributes of the LDS-table kernels: compute units and the LDS of
+// one workgroup.
+static int lds_device(int *ncu, int *lds)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipDeviceGetAttribute(lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        return RRTMG_HIP_EDEVICE;
+    *ncu = std::max(*ncu, 1);
+    return 0;
+}
+
 int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
 {
     DeviceState *s = state();
@@ -357,12 +369,9 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     if (!rc) rc = upload(&s->lwd.bands_lds, rebased, sizeof rebased);
     if (rc) return fail(rc, "uploading LW tables failed");
     {
-        int dev = 0, ncu = 0, lds = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            return fail(RRTMG_HIP_EDEVICE, "device attribute query failed");
-        s->lwd.plan.ncu = std::max(ncu, 1);
+        int ncu = 0, lds = 0;
+        if (lds_device(&ncu, &lds)) return fail(RRTMG_HIP_EDEVICE, "device attribute query failed");
+        s->lwd.plan.ncu = ncu;
         const char *env = std::getenv("RRTMG_HIP_LW_LDS");
         s->lwd.lds = s->lwd.plan.lds_bytes <= lds && !(env && env[0] == '0');
     }
@@ -556,6 +565,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
                                       lw_lds_state_bytes(nlay)
                                 : (size_t)chunk * lw_fwd_state_bytes(nlay, rs);
     const size_t o_ds = off; off = align_up(off + ds_bytes);
+    const size_t o_it = off; off = align_up(off + sizeof(int));
     if ((rc = s->lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->lw_work.p;
     LwWork w;
@@ -568,6 +578,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     w.bflux = wb + o_bf;
     w.dstate = wb + o_ds;
     w.chunk = chunk;
+    w.items = (int *)(wb + o_it);
 
     ColumnInputs in{};
     in.ncol = ncol;

@@ -45,6 +45,7 @@ struct LwWork {
     void *bflux = nullptr;    // [16][2][nlay+1][ncol] of Real
     void *dstate = nullptr;   // [chunk][nlay][64] of Real: upward source of each wave of a band-kernel launch
     int chunk = 0;            // wave units per band-kernel launch (rows of dstate)
+    int *items = nullptr;     // work-item counter of the LDS-table band kernel
 };
 
 struct SwDevTables {

@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void lw_column_kernel(ColumnInputs in, LwWork 
 // source per layer, [nlay][64] each (LDS or this wave's global scratch row);
 // s_red: 64 LDS entries for the FP64 g-sum.
 template <int BAND, typename Real>
-__device__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBand &B, const Real *__restrict__ ktab,
+__device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWork &w, const LwBand &B, const Real *__restrict__ ktab,
                              const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
                              const LwConst *__restrict__ cst, WaveUnit u, int tile, int lane,
                              Real *__restrict__ s_atr, Real *__restrict__ s_bbu, Real *s_red)
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(64) void lw_band_kernel(ColumnInputs in, LwWork w, 
 }
 
 // FP32 band kernel with the band's k-tables in LDS.  Persistent workgroups of
-// LW_WGW waves run work items item = blockIdx.x, blockIdx.x + gridDim.x, ...;
+// LW_WGW waves claim work items from a global counter (next_item);
 // item = (group of LW_WGW tiles, unit r of a tile's unit list) covers the
 // group's LW_WGW x upt units of the band of unit r, one per wave.  Before an
 // item of another band the workgroup copies that band's k-tables (at most
@@ -358,12 +358,14 @@ __global__ __launch_bounds__(64 * LW_WGW) void lw_band_lds_kernel(ColumnInputs i
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
     Real *s_atr = (Real *)w.dstate + ((size_t)blockIdx.x * LW_WGW + wave) * 2 * nlay * 64;
     Real *s_bbu = s_atr + (size_t)nlay * 64;
+    __shared__ int s_item;
     int loaded = -1;
-    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+    for (;;) {
+        const int item = next_item(w.items, &s_item);
+        if (item >= nitems) break;
         const int group = item / nwaves, r = item - group * nwaves;
         const int b = units[r].band;
         if (b != loaded) {
-            __syncthreads();  // every wave is done with the previous band's tables
             const int t0 = plan.tab0[b], n = plan.tab0[b + 1] - t0;
 #pragma unroll 8
             for (int i = threadIdx.x; i < n; i += 64 * LW_WGW) ktab[i] = tab[t0 + i];
@@ -444,6 +446,7 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     if constexpr (sizeof(Real) == 4) {
         if (t.lds && !smooth) {
             const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * t.desc.nwaves;
+            hipLaunchKernelGGL(reset_items_kernel, dim3(1), dim3(1), 0, s, w.items);
             hipLaunchKernelGGL(lw_band_lds_kernel<Real>, dim3(lw_lds_grid(t.plan, in.ncol, t.desc.nwaves)),
                                dim3(64 * LW_WGW), (size_t)t.plan.lds_bytes, s, in, w, t.bands_lds, t.plan,
                                t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,

@@ -310,8 +310,8 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
             r.tf = (Real)0.166667 * od;
         } else {
             const int itr = tbl_index(od);
-            r.atrans = (Real)1 - exp_tbl[itr];
-            r.tf = tfn_tbl[itr];
+            r.atrans = (Real)1 - at32(exp_tbl, (unsigned)itr);
+            r.tf = at32(tfn_tbl, (unsigned)itr);
         }
         r.datr = r.dtf = (Real)0;
         if (derivs) {

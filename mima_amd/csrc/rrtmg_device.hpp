@@ -58,6 +58,22 @@ __device__ __forceinline__ const T &at32(const T *base, unsigned i)
     return *(const T *)((const char *)base + i * (unsigned)sizeof(T));
 }
 
+// Zeroes the work-item counter of a persistent band kernel (a kernel, not a
+// memset, so that the sequence can be captured in a graph).
+[[maybe_unused]] static __global__ void reset_items_kernel(int *items) { *items = 0; }
+
+// Next work item of a persistent workgroup, claimed from a global counter
+// (dynamic balance: items differ in cost, e.g. night columns in SW); every
+// wave of the workgroup gets the same item.  The leading barrier also
+// guarantees that all waves are done with the previous item.
+__device__ __forceinline__ int next_item(int *items, int *s_item)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) *s_item = atomicAdd(items, 1);
+    __syncthreads();
+    return *s_item;
+}
+
 // Sum of v over the ng adjacent lanes of each column group of a one-wave
 // block (g = the lane's position in its group); valid on the group's first
 // lane.  Deterministic (fixed order).  FP64: sequential through LDS, the

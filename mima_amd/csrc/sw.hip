@@ -4,7 +4,9 @@
 // MiMA's RRTMG_SW (file:line cited per function).
 //
 // Band kernel layout: one wave per work unit (band b, up to 64/ng columns of a
-// tile); lane = (column, g).  Each lane runs the adding method for its
+// tile); lane = (column, g); each band is a compile-time instantiation of the
+// band body (straight-line band code, registers sized per band).  Each lane
+// runs the adding method for its
 // g-point: a bottom-up sweep keeps prup/prupd of every level (prup in LDS,
 // prupd in a per-wave global scratch row), a
 // top-down sweep recomputes the layer optics, carries ztdn/prdnd/ptdbt and
@@ -143,33 +145,26 @@ __global__ __launch_bounds__(256) void sw_column_kernel(ColumnInputs in, SwWork 
     }
 }
 
-template <typename Real>
-__global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
-                                                     int nwaves, int blk0, int nblocks, int exp_off,
-                                                     const Real *__restrict__ tab, const WaveUnit *__restrict__ units)
+// One band JB of one work unit (rrtmg_sw_spcvrt.f90:302-600 with
+// rrtmg_sw_vrtqdr.f90:28-152, clear sky).  ktab: the k-tables (B's offsets index
+// it: the packed global buffer, or the band's copy in LDS with rebased
+// offsets); exp_tbl: the exp table (global) or NULL for the smooth function.
+// s_prup, s_prupd: reflectances of the atmosphere below each level,
+// [(nlay+1)][64] each (LDS or this wave's global scratch row); s_red: 64 LDS
+// entries for the FP64 g-sum.
+template <int JB, typename Real>
+__device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWork &w, const SwBand &B, const Real *__restrict__ ktab,
+                             const Real *__restrict__ exp_tbl, WaveUnit u, int tile, int lane,
+                             Real *__restrict__ s_prup, Real *__restrict__ s_prupd, Real *s_red)
 {
-    extern __shared__ unsigned char smem_raw[];
-    Real *smem = (Real *)smem_raw;
     const int ncol = in.ncol, nlay = in.nlay;
-    const int blk = xcd_block();
-    if (blk >= nblocks) return;
-    const int tile = (blk0 + blk) / nwaves;
-    const WaveUnit u = units[(blk0 + blk) % nwaves];
-    const SwBand B = bands[u.band];
-    const int jb = u.band + 16;
-    const int lane = threadIdx.x;
     const int ng = B.ng;
     const int cl = lane / ng, g = lane - cl * ng;
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
-    Real *s_prup = smem;                           // [(nlay+1)][64], LDS
-    Real *s_red = smem + (size_t)(nlay + 1) * 64;  // [64]
-    // prupd of every level in this wave's global scratch row, [(nlay+1)][64]
-    Real *__restrict__ s_prupd = (Real *)w.dstate + (size_t)blk * (nlay + 1) * 64;
     const Real *coef = (const Real *)w.coef;
     Real *bflux = (Real *)w.bflux;
-    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
-    Tab<Real> T{tab, ng, g};
+    Tab<Real> T{ktab, ng, g};
 
     Real prmu0 = (Real)1, zincflx = (Real)0, albp = (Real)0, albd = (Real)0;
     int laytrop = 0;
@@ -177,43 +172,43 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     if (active) {
         prmu0 = (Real)in.coszen[col];
         laytrop = w.laytrop[col];
-        const bool nir = (jb <= 24) || jb == 29;  // rrtmg_sw_rad.nomcica.f90:567-577
+        const bool nir = (JB <= 24) || JB == 29;  // rrtmg_sw_rad.nomcica.f90:567-577
         albp = (Real)(nir ? in.aldir[col] : in.asdir[col]);
         albd = (Real)(nir ? in.aldif[col] : in.asdif[col]);
-        int lsf = w.laysolfr[(size_t)u.band * ncol + col];
+        int lsf = w.laysolfr[(size_t)(JB - 16) * ncol + col];
         sw_load_layer(c, coef, w.ipk, lsf - 1, col, ncol, nlay);
-        double sfz = sw_sfluxzen(jb, B, T, c, lsf <= laytrop);
+        double sfz = sw_sfluxzen(JB, B, T, c, lsf <= laytrop);
         zincflx = (Real)(in.adjflux * sfz * (double)prmu0);
     }
 
     // bottom-up sweep: reflectances of the atmosphere below each level
     Real prup_n = albp, prupd_n = albd;
-    s_prup[(size_t)nlay * 64 + lane] = albp;
-    s_prupd[(size_t)nlay * 64 + lane] = albd;
+    at32(s_prup, (unsigned)(nlay * 64 + lane)) = albp;
+    at32(s_prupd, (unsigned)(nlay * 64 + lane)) = albd;
     for (int lay = 1; lay <= nlay; ++lay) {
         const int jk = nlay + 1 - lay;  // top-down index of the layer
         Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
         if (active) {
             sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
             Real taug, taur;
-            sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
+            sw_gas(JB, B, T, c, lay <= laytrop, taug, taur);
             sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
         }
         Real zreflect = fdiv((Real)1, (Real)1 - prupd_n * rd);
         Real prup = r + (td * ((t - dbt) * prupd_n + dbt * prup_n)) * zreflect;
         Real prupd = rd + td * td * prupd_n * zreflect;
-        s_prup[(size_t)(jk - 1) * 64 + lane] = prup;
-        s_prupd[(size_t)(jk - 1) * 64 + lane] = prupd;
+        at32(s_prup, (unsigned)((jk - 1) * 64 + lane)) = prup;
+        at32(s_prupd, (unsigned)((jk - 1) * 64 + lane)) = prupd;
         prup_n = prup;
         prupd_n = prupd;
     }
 
     // top-down sweep with level fluxes
     Real ztdn = (Real)1, prdnd = (Real)0, ptdbt = (Real)1;
-    const size_t lvstride = ncol, bstride = (size_t)(nlay + 1) * ncol;
-    Real *bf_up = bflux + (size_t)u.band * 2 * bstride;
+    const size_t bstride = (size_t)(nlay + 1) * ncol;
+    Real *bf_up = bflux + (size_t)(JB - 16) * 2 * bstride;
     Real *bf_dn = bf_up + bstride;
-    Real prupd_next = s_prupd[lane];
+    Real prupd_next = at32(s_prupd, (unsigned)lane), prup_next = at32(s_prup, (unsigned)lane);
     for (int jk = 1; jk <= nlay + 1; ++jk) {
         if (jk > 1) {
             const int lay = nlay + 2 - jk;  // layer between levels jk-1 and jk
@@ -221,7 +216,7 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
             if (active) {
                 sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
                 Real taug, taur;
-                sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
+                sw_gas(JB, B, T, c, lay <= laytrop, taug, taur);
                 sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
             }
             Real zreflect = fdiv((Real)1, (Real)1 - rd * prdnd);
@@ -231,9 +226,12 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
             ztdn = ztdn_n;
             prdnd = prdnd_n;
         }
-        Real prup = s_prup[(size_t)(jk - 1) * 64 + lane];
-        const Real prupd = prupd_next;
-        if (jk <= nlay) prupd_next = s_prupd[(size_t)jk * 64 + lane];  // one level ahead
+        // the reflectances of the level below are read one level ahead
+        const Real prup = prup_next, prupd = prupd_next;
+        if (jk <= nlay) {
+            prupd_next = at32(s_prupd, (unsigned)(jk * 64 + lane));
+            prup_next = at32(s_prup, (unsigned)(jk * 64 + lane));
+        }
         Real zreflect = fdiv((Real)1, (Real)1 - prdnd * prupd);
         Real pfu = (ptdbt * prup + (ztdn - ptdbt) * prupd) * zreflect;
         Real pfd = ptdbt + (ztdn - ptdbt + ptdbt * prup * prdnd) * zreflect;
@@ -241,10 +239,38 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
         Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, g, ng, s_red);
         Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, g, ng, s_red);
         if (active && g == 0) {
-            bf_up[(size_t)lev * lvstride + col] = su;
-            bf_dn[(size_t)lev * lvstride + col] = sd;
+            at32(bf_up, (unsigned)(lev * ncol + col)) = su;
+            at32(bf_dn, (unsigned)(lev * ncol + col)) = sd;
         }
     }
+}
+
+#define SW_BANDS(X) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29)
+
+template <typename Real>
+__global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
+                                                     int nwaves, int blk0, int nblocks, int exp_off,
+                                                     const Real *__restrict__ tab, const WaveUnit *__restrict__ units)
+{
+    extern __shared__ unsigned char smem_raw[];
+    Real *smem = (Real *)smem_raw;
+    const int nlay = in.nlay;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
+    const SwBand B = bands[u.band];
+    Real *s_prup = smem;                           // [(nlay+1)][64], LDS
+    Real *s_red = smem + (size_t)(nlay + 1) * 64;  // [64]
+    // prupd of every level in this wave's global scratch row, [(nlay+1)][64]
+    Real *s_prupd = (Real *)w.dstate + (size_t)blk * (nlay + 1) * 64;
+    const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
+#define BODY(jb)                                                                                              \
+    case jb - 16:                                                                                             \
+        sw_band_body<jb, Real>(in, w, B, tab, exp_tbl, u, tile, (int)threadIdx.x, s_prup, s_prupd, s_red);      \
+        break;
+    switch (u.band) { SW_BANDS(BODY) }
+#undef BODY
 }
 
 // finish: band sums, net flux and heating (rrtmg_sw_rad.nomcica.f90:700-726),

@@ -27,22 +27,15 @@ template <typename Real>
 __device__ __forceinline__ void sw_load_layer(SwLayer<Real> &c, const Real *__restrict__ coef,
                                               const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t ix = (size_t)lay * ncol + col, fs = (size_t)nlay * ncol;
-    c.fac00 = coef[SWF_FAC00 * fs + ix];
-    c.fac01 = coef[SWF_FAC01 * fs + ix];
-    c.fac10 = coef[SWF_FAC10 * fs + ix];
-    c.fac11 = coef[SWF_FAC11 * fs + ix];
-    c.colh2o = coef[SWF_COLH2O * fs + ix];
-    c.colco2 = coef[SWF_COLCO2 * fs + ix];
-    c.colo3 = coef[SWF_COLO3 * fs + ix];
-    c.colch4 = coef[SWF_COLCH4 * fs + ix];
-    c.colo2 = coef[SWF_COLO2 * fs + ix];
-    c.colmol = coef[SWF_COLMOL * fs + ix];
-    c.selffac = coef[SWF_SELFFAC * fs + ix];
-    c.selffrac = coef[SWF_SELFFRAC * fs + ix];
-    c.forfac = coef[SWF_FORFAC * fs + ix];
-    c.forfrac = coef[SWF_FORFRAC * fs + ix];
-    int w = ipk[ix];
+    const unsigned ix = (unsigned)(lay * ncol + col);
+    const size_t fs = (size_t)nlay * ncol;
+#define LD(f, F) c.f = at32(coef + F * fs, ix)
+    LD(fac00, SWF_FAC00); LD(fac01, SWF_FAC01); LD(fac10, SWF_FAC10); LD(fac11, SWF_FAC11);
+    LD(colh2o, SWF_COLH2O); LD(colco2, SWF_COLCO2); LD(colo3, SWF_COLO3); LD(colch4, SWF_COLCH4);
+    LD(colo2, SWF_COLO2); LD(colmol, SWF_COLMOL);
+    LD(selffac, SWF_SELFFAC); LD(selffrac, SWF_SELFFRAC); LD(forfac, SWF_FORFAC); LD(forfrac, SWF_FORFRAC);
+#undef LD
+    int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
     c.jt = unpack_jt(w);
     c.jt1 = unpack_jt1(w);
@@ -304,13 +297,13 @@ __device__ V sw_sfluxzen_v(int jb, const SwBand &B, const Tab<Real> &T, const Sw
 template <typename F, typename Real>
 __device__ __forceinline__ F tbl_exp(F x, const Real *__restrict__ tbl)
 {
-    return tbl ? (F)tbl[tbl_index(x)] : fmax(exp(-x), (F)1.e-20);
+    return tbl ? (F)at32(tbl, (unsigned)tbl_index(x)) : fmax(exp(-x), (F)1.e-20);
 }
 template <typename F, int N, typename Real>
 __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *__restrict__ tbl)
 {
     Dual<F, N> r;
-    const F e = tbl ? (F)tbl[tbl_index(x.v)] : exp(-x.v);
+    const F e = tbl ? (F)at32(tbl, (unsigned)tbl_index(x.v)) : exp(-x.v);
     r.v = tbl ? e : fmax(e, (F)1.e-20);
     const F s = e > (F)1.e-20 ? -e : (F)0;
     for (int i = 0; i < N; ++i) r.d[i] = s * x.d[i];

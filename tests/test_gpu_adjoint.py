@@ -248,7 +248,12 @@ def test_heating_jacobian_rows_match_finite_differences(rr, side):
     cotangent per replicated column) against columns of the central-difference
     Jacobian of the FP64 forward (one +-h perturbation of one layer per
     replicated column), at h = 1e-3 K and 1e-5 K; entries whose two
-    differences disagree straddle a kink of the forward and are reported."""
+    differences disagree straddle a kink of the forward and are reported.
+    Elsewhere the backward must match the better of the two steps: SW heating
+    is a difference of nearly equal level fluxes, so its central difference
+    carries roundoff ~1e-10 |hr| / h (measured: the SW error grows as 1/h,
+    8e-9 of max |J| at h = 1e-2, 1e-7 at 1e-3, 1e-5 at 1e-5), while LW is
+    truncation-limited at 1e-3 and roundoff-limited below 1e-4."""
     c0 = _shift_off_nodes(_cols())
     day = np.where(c0["coszen"] > 0.3)[0][:2]
     L = c0["nlay"]
@@ -275,7 +280,7 @@ def test_heating_jacobian_rows_match_finite_differences(rr, side):
             J_fd[h] = (hr[:, :L] - hr[:, L:]) / (2.0 * h)
         scale = np.max(np.abs(J_fd[1e-5]))
         kink = np.abs(J_fd[1e-3] - J_fd[1e-5]) > 1e-4 * scale
-        err = np.abs(J_ad - J_fd[1e-5])[~kink].max() / scale
+        err = np.minimum(np.abs(J_ad - J_fd[1e-3]), np.abs(J_ad - J_fd[1e-5]))[~kink].max() / scale
         print("%s column %d: max |J_ad - J_fd| / max |J| = %.2e (max |J| = %.3e K/day/K), %d of %d entries "
               "within a step of a kink" % (side, i, err, scale, int(kink.sum()), kink.size))
         assert kink.mean() < 0.03
