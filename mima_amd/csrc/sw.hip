@@ -245,8 +245,6 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
     }
 }
 
-#define SW_BANDS(X) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29)
-
 template <typename Real>
 __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
                                                      int nwaves, int blk0, int nblocks, int exp_off,

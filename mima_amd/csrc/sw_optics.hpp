@@ -43,6 +43,9 @@ __device__ __forceinline__ void sw_load_layer(SwLayer<Real> &c, const Real *__re
     c.indf = unpack_indf(w);
 }
 
+// X(jb) for every SW band, for switches over compile-time band bodies
+#define SW_BANDS(X) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29)
+
 // Table reader for one lane: row r (1-based, as the reference's ind) of a
 // [rows][ng] table at `off`.
 template <typename Real>
