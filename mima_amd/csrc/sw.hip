@@ -162,6 +162,9 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
     const int cl = lane / ng, g = lane - cl * ng;
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
+    // a wave whose columns are all at night writes nothing: the finish
+    // kernel sets night columns to zero without reading the band fluxes
+    if (!__any(active)) return;
     const Real *coef = (const Real *)w.coef;
     Real *bflux = (Real *)w.bflux;
     Tab<Real> T{ktab, ng, g};

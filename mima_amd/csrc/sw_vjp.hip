@@ -244,6 +244,9 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     const int cl = lane / ng, g = lane - cl * ng;
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol && in.coszen[min(col0, ncol - 1)] >= 1.e-10;
+    // a wave whose columns are all at night writes nothing: the finish
+    // kernels set night columns to zero without reading its outputs
+    if (!__any(active)) return;
     const int col = active ? col0 : 0;
     const size_t L1 = (size_t)(nlay + 1) * 64;
     Real *s_prup = smem, *s_prupd = smem + L1;  // bottom-up sweep, LDS
