@@ -831,7 +831,12 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const size_t o_gl = off; off = align_up(off + rs * 16 * n3);
     const size_t o_gs = off; off = align_up(off + rs * 16 * 2 * ncol);
     const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->lw.tab.nwaves, lw_vjp_state_bytes(nlay, rs));
-    const size_t o_ds = off; off = align_up(off + (size_t)chunk * lw_vjp_state_bytes(nlay, rs));
+    const bool lds = mixed && s->lwd.lds;
+    const size_t ds_bytes = lds ? (size_t)lw_vjp_lds_grid(s->lwd.plan, ncol, s->lw.tab.nwaves) * LWV_WGW *
+                                      lw_vjp_lds_state_bytes(nlay)
+                                : (size_t)chunk * lw_vjp_state_bytes(nlay, rs);
+    const size_t o_ds = off; off = align_up(off + ds_bytes);
+    const size_t o_it = off; off = align_up(off + sizeof(int));
     const size_t o_bf = off; off = align_up(off + (fwd ? rs * 16 * 2 * n3 : 0));
     if ((rc = s->lw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)s->lw_vjp_work.p;
@@ -848,6 +853,7 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     w.gsc = wb + o_gs;
     w.dstate = wb + o_ds;
     w.chunk = chunk;
+    w.items = (int *)(wb + o_it);
     w.bflux = fwd ? wb + o_bf : nullptr;
     if (host) {
         const size_t need = 256 * 27 + sizeof(double) * (16 * n2 + 5 * n3 + (size_t)ncol * 18 + (tauaer ? 16 * n2 : 0) +
@@ -880,6 +886,9 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     t.cst = s->lwd.cst;
     t.units = s->lwd.units;
     t.heatfac = s->lw.cst.heatfac;
+    t.bands_lds = s->lwd.bands_lds;
+    t.plan = s->lwd.plan;
+    t.lds = lds;
     hipError_t e = launch_lw_vjp(in, cot, g, w, t, g_prec, st, s->events(3), fwd ? &fo : nullptr);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
     if (host) {

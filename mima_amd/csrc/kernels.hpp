@@ -77,6 +77,16 @@ inline int lw_lds_grid(const LwLdsPlan &p, int ncol, int nwaves)
     return nitems < p.ncu ? nitems : p.ncu;
 }
 inline size_t lw_lds_state_bytes(int nlay) { return sizeof(float) * 2 * (size_t)nlay * 64; }
+// the backward's LDS-table kernel: LWV_WGW waves per workgroup (at most 168
+// VGPRs, three waves per SIMD), per-wave rows of 3 x nlay x 64
+constexpr int LWV_WGW = 12;
+inline int lw_vjp_lds_grid(const LwLdsPlan &p, int ncol, int nwaves)
+{
+    const int ntiles = (ncol + TILE_COLS - 1) / TILE_COLS;
+    const int nitems = (ntiles + LWV_WGW - 1) / LWV_WGW * nwaves;
+    return nitems < p.ncu ? nitems : p.ncu;
+}
+inline size_t lw_vjp_lds_state_bytes(int nlay) { return sizeof(float) * 3 * (size_t)nlay * 64; }
 
 struct LwDevTables {
     LwTables desc;
@@ -112,9 +122,11 @@ struct LwVjpWork {
     void *gpart = nullptr;    // [16][3][nlay][ncol] of Real: per-band gradient (T, q, O3)
     void *glev = nullptr;     // [16][nlay+1][ncol] of Real: per-band gradient wrt level temperature
     void *gsc = nullptr;      // [16][2][ncol] of Real: per-band d/dtsfc and d/dpwvcm
-    void *dstate = nullptr;   // [chunk][nlay][64] of Real: incoming downward radiance of each wave of a launch
+    void *dstate = nullptr;   // [rows][2][nlay][64] of Real: incoming downward radiance and upward source per wave
+                              // ([rows][3][nlay][64] with the transmittance for the LDS-table kernel)
     int chunk = 0;            // wave units per backward band-kernel launch (rows of dstate)
     void *bflux = nullptr;    // [16][2][nlay+1][ncol] of Real: band fluxes of the recomputed forward, or NULL
+    int *items = nullptr;     // work-item counter of the LDS-table band kernel
 };
 struct SwVjpWork {
     void *dcoef = nullptr;    // [SWF_COUNT][4][nlay][ncol] of Real
@@ -147,9 +159,9 @@ inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
 {
     return real_size * (3 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;
 }
-inline size_t lw_vjp_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)nlay * 64; }
+inline size_t lw_vjp_state_bytes(int nlay, size_t real_size) { return real_size * 2 * (size_t)nlay * 64; }
 // LDS bytes of one backward band workgroup
-inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)2 * nlay * 64 + 4 * 64); }
+inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 4 * 64); }
 inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)2 * (nlay + 1) * 64 + 4 * 64); }
 
 // fwd: NULL, or the forward's outputs, formed from w.bflux (which the

@@ -16,12 +16,12 @@
 //    dry-air totals, precipitable water and surface temperature.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
 //    pass A, downward sweep (values), storing per layer the transmittance
-//    and the upward source in LDS and the incoming downward radiance in a
+//    in LDS and the upward source and the incoming downward radiance in a
 //    per-wave global scratch row (so LDS does not cap occupancy); pass B,
 //    upward sweep, replacing the source by (source - incoming upward
 //    radiance); pass C, reverse of the upward sweep, top-down, leaving the
 //    adjoints of the upward source and of the transmittance (upward part) in
-//    LDS; pass D, reverse of the surface term and of the downward sweep,
+//    place of those two; pass D, reverse of the surface term and of the downward sweep,
 //    bottom-up: the layer's optical depth and Planck fraction are recomputed
 //    as dual numbers (no g-point-by-level tape is kept), the local derivatives
 //    of transmittance and source with respect to optical depth, Planck
@@ -50,19 +50,20 @@ using D3 = Dual<double, 3>;
 enum : unsigned { DT = 1u, DQ = 2u, DO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct DLoad;
+// (p: the field's uniform base; ix: the element, a 32-bit offset)
 template <typename Real, unsigned MASK>
 struct DLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p, size_t) { return p[0]; }
+    __device__ static Real get(const Real *__restrict__ p, unsigned ix, size_t) { return at32(p, ix); }
 };
 template <typename Real, unsigned MASK>
 struct DLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, size_t cs)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, unsigned ix, size_t cs)
     {
         Dual<Real, 3> r;
-        r.v = p[0];
-        r.d[0] = (MASK & DT) ? p[cs] : (Real)0;
-        r.d[1] = (MASK & DQ) ? p[2 * cs] : (Real)0;
-        r.d[2] = (MASK & DO) ? p[3 * cs] : (Real)0;
+        r.v = at32(p, ix);
+        r.d[0] = (MASK & DT) ? at32(p + cs, ix) : (Real)0;
+        r.d[1] = (MASK & DQ) ? at32(p + 2 * cs, ix) : (Real)0;
+        r.d[2] = (MASK & DO) ? at32(p + 3 * cs, ix) : (Real)0;
         return r;
     }
 };
@@ -76,8 +77,9 @@ template <typename V, typename Real>
 __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__restrict__ dco,
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t fs = (size_t)nlay * ncol, ix = (size_t)lay * ncol + col;
-#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
+    const size_t fs = (size_t)nlay * ncol;
+    const unsigned ix = (unsigned)(lay * ncol + col);
+#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs, ix, fs)
     LDD(fac00, LWF_FAC00, DT); LDD(fac01, LWF_FAC01, DT); LDD(fac10, LWF_FAC10, DT); LDD(fac11, LWF_FAC11, DT);
     LDD(colh2o, LWF_COLH2O, DQ); LDD(colco2, LWF_COLCO2, DQ); LDD(colo3, LWF_COLO3, DQ | DO);
     LDD(coln2o, LWF_COLN2O, DQ); LDD(colco, LWF_COLCO, DQ); LDD(colch4, LWF_COLCH4, DQ); LDD(colo2, LWF_COLO2, DQ);
@@ -87,7 +89,7 @@ __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__res
     LDD(coldry, LWF_COLDRY, DQ); LDD(pavel, LWF_PAVEL, 0u); LDD(wx1, LWF_WX1, DQ); LDD(wx2, LWF_WX2, DQ);
     LDD(wx3, LWF_WX3, DQ); LDD(wx4, LWF_WX4, DQ); LDD(tlayfrac, LWF_TLAYFRAC, DT);
 #undef LDD
-    const int w = ipk[ix];
+    const int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
     c.jt = unpack_jt(w);
     c.jt1 = unpack_jt1(w);
@@ -326,23 +328,26 @@ __device__ __forceinline__ LwLayerRT<Real> lw_layer_rt(Real secdiff, Real taug, 
     return r;
 }
 
+// One band of one work unit of the backward (passes A-D, described at the top
+// of this file).  ktab: the k-tables (the packed global buffer, or the band's
+// copy in LDS with rebased offsets in B); tab: the packed global buffer
+// (transmittance and Planck tables).  s1: transmittance per layer, [nlay][64]
+// (LDS or global); s3: this wave's scratch row, incoming downward radiance
+// and upward source, [2][nlay][64]; red: [4][64] LDS entries for FP64 g-sums.
 template <int BAND, typename Real>
 __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const LwVjpWork &w, const LwBand &B,
-                                 const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
-                                 const LwConst *__restrict__ cst, WaveUnit u, int tile, Real *smem,
-                                 Real *__restrict__ s3)
+                                 const Real *__restrict__ ktab, const Real *__restrict__ tab, int exp_off,
+                                 int tfn_off, int plk_off, const LwConst *__restrict__ cst, WaveUnit u, int tile,
+                                 int lane, Real *__restrict__ s1, Real *__restrict__ s3, Real *red)
 {
     constexpr int NG = lw_ng(BAND);
     using V3 = Dual<Real, 3>;
     const int ncol = in.ncol, nlay = in.nlay;
-    const int lane = threadIdx.x;
     const int cl = lane / NG, g = lane - cl * NG;
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol;
     const int col = active ? col0 : 0;
-    Real *s1 = smem;                          // [nlay][64]
-    Real *s2 = smem + (size_t)nlay * 64;      // [nlay][64]
-    Real *red = smem + (size_t)2 * nlay * 64; // [4][64]
+    Real *__restrict__ s2 = s3 + (size_t)nlay * 64;
     // s3: incoming downward radiance per layer, [nlay][64] in this wave's
     // global scratch row (written in pass A, read back in pass D)
     const Real *dco = (const Real *)w.dcoef;
@@ -350,7 +355,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     const Real *tfn_tbl = tfn_off >= 0 ? tab + tfn_off : nullptr;
     const Real *plk = tab + plk_off + (BAND - 1) * 181;
-    LTab<Real> T{tab, NG, g};
+    LTab<Real> T{ktab, NG, g};
     const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
     Real *gpart = (Real *)w.gpart + (size_t)(BAND - 1) * 3 * n2;
     Real *glev = (Real *)w.glev + (size_t)(BAND - 1) * n3;
@@ -414,9 +419,9 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
             const LwLayerRT<Real> rt = lw_layer_rt(secdiff, taug, exp_tbl, tfn_tbl, false);
             const Real bbd = plfrac * (blay + rt.tf * (plev_dn - blay));
             const Real bbu = plfrac * (blay + rt.tf * (plev_up - blay));
-            s1[(size_t)(lev - 1) * 64 + lane] = rt.atrans;
-            s2[(size_t)(lev - 1) * 64 + lane] = bbu;
-            s3[(size_t)(lev - 1) * 64 + lane] = radld;
+            at32(s1, (unsigned)((lev - 1) * 64 + lane)) = rt.atrans;
+            at32(s2, (unsigned)((lev - 1) * 64 + lane)) = bbu;
+            at32(s3, (unsigned)((lev - 1) * 64 + lane)) = radld;
             radld = radld + (bbd - radld) * rt.atrans;
             if (bf_dn) {
                 const Real sd = seg_sum(active ? radld : (Real)0, lane, g, NG, red);
@@ -432,10 +437,16 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
         const Real su = seg_sum(active ? radlu : (Real)0, lane, g, NG, red);
         if (active && g == 0) bf_up[col] = su;
     }
+    // the per-layer states are read one level ahead
+    Real s1n = at32(s1, (unsigned)lane), s2n = at32(s2, (unsigned)lane);
     for (int lev = 1; lev <= nlay; ++lev) {
-        const Real atr = s1[(size_t)(lev - 1) * 64 + lane];
-        const Real dsrc = s2[(size_t)(lev - 1) * 64 + lane] - radlu;
-        s2[(size_t)(lev - 1) * 64 + lane] = dsrc;
+        const Real atr = s1n;
+        const Real dsrc = s2n - radlu;
+        if (lev < nlay) {
+            s1n = at32(s1, (unsigned)(lev * 64 + lane));
+            s2n = at32(s2, (unsigned)(lev * 64 + lane));
+        }
+        at32(s2, (unsigned)((lev - 1) * 64 + lane)) = dsrc;
         radlu = radlu + dsrc * atr;
         if (bf_up) {
             const Real su = seg_sum(active ? radlu : (Real)0, lane, g, NG, red);
@@ -444,11 +455,17 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     }
     // pass C: reverse of the upward sweep (top-down)
     Real a_ru = ru(nlay);
+    s1n = at32(s1, (unsigned)((nlay - 1) * 64 + lane));
+    s2n = at32(s2, (unsigned)((nlay - 1) * 64 + lane));
     for (int lev = nlay; lev >= 1; --lev) {
-        const size_t k = (size_t)(lev - 1) * 64 + lane;
-        const Real atr = s1[k], dsrc = s2[k];
-        s1[k] = a_ru * atr;   // adjoint of the upward source of layer lev
-        s2[k] = a_ru * dsrc;  // adjoint of its transmittance, upward part
+        const unsigned k = (unsigned)((lev - 1) * 64 + lane);
+        const Real atr = s1n, dsrc = s2n;
+        if (lev > 1) {
+            s1n = at32(s1, (unsigned)((lev - 2) * 64 + lane));
+            s2n = at32(s2, (unsigned)((lev - 2) * 64 + lane));
+        }
+        at32(s1, k) = a_ru * atr;   // adjoint of the upward source of layer lev
+        at32(s2, k) = a_ru * dsrc;  // adjoint of its transmittance, upward part
         a_ru = a_ru * ((Real)1 - atr) + ru(lev - 1);
     }
     // reverse of the surface term
@@ -472,8 +489,8 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
         const Real bbd = pf * (blay + rt.tf * dpdn);
         const size_t k = (size_t)(lev - 1) * 64 + lane;
         const Real a_bbd = a_rd * rt.atrans;
-        const Real a_atr = a_rd * (bbd - s3[k]) + s2[k];
-        const Real a_bbu = s1[k];
+        const Real a_atr = a_rd * (bbd - at32(s3, (unsigned)k)) + at32(s2, (unsigned)k);
+        const Real a_bbu = at32(s1, (unsigned)k);
         a_rd = a_rd * ((Real)1 - rt.atrans) + (lev < nlay ? rd(lev) : (Real)0);
         Real a_pf = a_bbd * (blay + rt.tf * dpdn) + a_bbu * (blay + rt.tf * dpup);
         if (lev == 1) a_pf += a_fracs1;
@@ -514,7 +531,8 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
 }
 
 template <typename Real>
-__global__ __launch_bounds__(64) void lw_vjp_band_kernel(ColumnInputs in, LwVjpWork w, const LwBand *__restrict__ bands,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 3 : 1)))
+void lw_vjp_band_kernel(ColumnInputs in, LwVjpWork w, const LwBand *__restrict__ bands,
                                                          int nwaves, int blk0, int nblocks, int exp_off, int tfn_off,
                                                          int plk_off, const Real *__restrict__ tab,
                                                          const LwConst *__restrict__ cst,
@@ -527,16 +545,72 @@ __global__ __launch_bounds__(64) void lw_vjp_band_kernel(ColumnInputs in, LwVjpW
     const int tile = (blk0 + blk) / nwaves;
     const WaveUnit u = units[(blk0 + blk) % nwaves];
     const LwBand B = bands[u.band];
-    Real *s3 = (Real *)w.dstate + (size_t)blk * in.nlay * 64;
+    Real *s3 = (Real *)w.dstate + (size_t)blk * 2 * in.nlay * 64;
+    Real *s1 = smem, *red = smem + (size_t)in.nlay * 64;
 #define BODY(b)                                                                                          \
     case b - 1:                                                                                          \
-        lw_vjp_band_body<b, Real>(in, w, B, tab, exp_off, tfn_off, plk_off, cst, u, tile, smem, s3);      \
+        lw_vjp_band_body<b, Real>(in, w, B, tab, tab, exp_off, tfn_off, plk_off, cst, u, tile, (int)threadIdx.x, \
+                                  s1, s3, red);                                                          \
         break;
     switch (u.band) {
         BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
         BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
     }
 #undef BODY
+}
+
+// FP32 backward band kernel with the band's k-tables in LDS, scheduled as the
+// forward's lw_band_lds_kernel (persistent workgroups of LWV_WGW waves that
+// claim work items from a global counter and copy a band's k-tables into LDS
+// before its first item).  All three per-layer states of a wave live in its
+// global scratch row, [3][nlay][64].
+template <typename Real>
+__global__ __launch_bounds__(64 * LWV_WGW) void lw_vjp_band_lds_kernel(ColumnInputs in, LwVjpWork w,
+                                                                       const LwBand *__restrict__ bands,
+                                                                       LwLdsPlan plan, int nwaves, int nitems,
+                                                                       int exp_off, int tfn_off, int plk_off,
+                                                                       const Real *__restrict__ tab,
+                                                                       const LwConst *__restrict__ cst,
+                                                                       const WaveUnit *__restrict__ units)
+{
+    static_assert(sizeof(Real) == 4, "FP32 only: the FP64 g-sum synchronises the workgroup");
+    extern __shared__ unsigned char smem_raw[];
+    Real *ktab = (Real *)smem_raw;
+    const int nlay = in.nlay;
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    Real *s3 = (Real *)w.dstate + ((size_t)blockIdx.x * LWV_WGW + wave) * 3 * nlay * 64;
+    Real *s1 = s3 + (size_t)2 * nlay * 64;
+    __shared__ int s_item;
+    int loaded = -1;
+    for (;;) {
+        const int item = next_item(w.items, &s_item);
+        if (item >= nitems) break;
+        const int group = item / nwaves, r = item - group * nwaves;
+        const int b = units[r].band;
+        if (b != loaded) {
+            const int t0 = plan.tab0[b], n = plan.tab0[b + 1] - t0;
+#pragma unroll 8
+            for (int i = threadIdx.x; i < n; i += 64 * LWV_WGW) ktab[i] = tab[t0 + i];
+            __syncthreads();
+            loaded = b;
+        }
+        const int k = (r - plan.ubeg[b]) * LWV_WGW + wave;
+        const int tile = group * LWV_WGW + k / plan.upt[b];
+        if (tile >= ntiles) continue;
+        const WaveUnit u = units[plan.ubeg[b] + k % plan.upt[b]];
+        const LwBand B = bands[b];
+#define BODY(bb)                                                                                              \
+    case bb - 1:                                                                                              \
+        lw_vjp_band_body<bb, Real>(in, w, B, ktab, tab, exp_off, tfn_off, plk_off, cst, u, tile, lane, s1, s3,  \
+                                   nullptr);                                                                  \
+        break;
+        switch (b) {
+            BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
+            BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
+        }
+#undef BODY
+    }
 }
 
 // finish: band sums and the precipitable-water chain of the diffusivity
@@ -603,8 +677,20 @@ static hipError_t launch_lw_vjp_t(const ColumnInputs &in, const VjpCot &cot, con
     const size_t shmem = lw_vjp_lds_bytes(in.nlay, sizeof(Real));
     const int nblocks = ntiles * t.desc.nwaves;
     if (ev) (void)hipEventRecord(ev[1], s);
-    // launches of at most w.chunk wave units, each reusing the scratch rows
-    for (int b0 = 0; b0 < nblocks; b0 += w.chunk) {
+    bool done = false;
+    if constexpr (sizeof(Real) == 4) {
+        if (t.lds && !smooth) {
+            const int nitems = (ntiles + LWV_WGW - 1) / LWV_WGW * t.desc.nwaves;
+            hipLaunchKernelGGL(reset_items_kernel, dim3(1), dim3(1), 0, s, w.items);
+            hipLaunchKernelGGL(lw_vjp_band_lds_kernel<Real>, dim3(lw_vjp_lds_grid(t.plan, in.ncol, t.desc.nwaves)),
+                               dim3(64 * LWV_WGW), (size_t)t.plan.lds_bytes, s, in, w, t.bands_lds, t.plan,
+                               t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,
+                               (const Real *)t.tab, t.cst, t.units);
+            done = true;
+        }
+    }
+    // otherwise launches of at most w.chunk wave units, each reusing the scratch rows
+    for (int b0 = 0; !done && b0 < nblocks; b0 += w.chunk) {
         const int nb_c = std::min(w.chunk, nblocks - b0);
         hipLaunchKernelGGL(lw_vjp_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), shmem, s, in, w, t.bands,
                            t.desc.nwaves, b0, nb_c, smooth ? -1 : t.desc.exp_tbl, smooth ? -1 : t.desc.tfn_tbl,

@@ -43,19 +43,20 @@ using D3 = Dual<double, 3>;
 enum : unsigned { SDT = 1u, SDQ = 2u, SDO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct SwDLoad;
+// (p: the field's uniform base; ix: the element, a 32-bit offset)
 template <typename Real, unsigned MASK>
 struct SwDLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p, size_t) { return p[0]; }
+    __device__ static Real get(const Real *__restrict__ p, unsigned ix, size_t) { return at32(p, ix); }
 };
 template <typename Real, unsigned MASK>
 struct SwDLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, size_t cs)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, unsigned ix, size_t cs)
     {
         Dual<Real, 3> r;
-        r.v = p[0];
-        r.d[0] = (MASK & SDT) ? p[cs] : (Real)0;
-        r.d[1] = (MASK & SDQ) ? p[2 * cs] : (Real)0;
-        r.d[2] = (MASK & SDO) ? p[3 * cs] : (Real)0;
+        r.v = at32(p, ix);
+        r.d[0] = (MASK & SDT) ? at32(p + cs, ix) : (Real)0;
+        r.d[1] = (MASK & SDQ) ? at32(p + 2 * cs, ix) : (Real)0;
+        r.d[2] = (MASK & SDO) ? at32(p + 3 * cs, ix) : (Real)0;
         return r;
     }
 };
@@ -64,15 +65,16 @@ template <typename V, typename Real>
 __device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__restrict__ dco,
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t fs = (size_t)nlay * ncol, ix = (size_t)lay * ncol + col;
-#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs + ix, fs)
+    const size_t fs = (size_t)nlay * ncol;
+    const unsigned ix = (unsigned)(lay * ncol + col);
+#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs, ix, fs)
     LDD(fac00, SWF_FAC00, SDT); LDD(fac01, SWF_FAC01, SDT); LDD(fac10, SWF_FAC10, SDT); LDD(fac11, SWF_FAC11, SDT);
     LDD(colh2o, SWF_COLH2O, SDQ); LDD(colco2, SWF_COLCO2, SDQ); LDD(colo3, SWF_COLO3, SDQ | SDO);
     LDD(colch4, SWF_COLCH4, SDQ); LDD(colo2, SWF_COLO2, SDQ); LDD(colmol, SWF_COLMOL, SDQ);
     LDD(selffac, SWF_SELFFAC, SDT | SDQ); LDD(selffrac, SWF_SELFFRAC, SDT); LDD(forfac, SWF_FORFAC, SDT | SDQ);
     LDD(forfrac, SWF_FORFRAC, SDT);
 #undef LDD
-    const int w = ipk[ix];
+    const int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
     c.jt = unpack_jt(w);
     c.jt1 = unpack_jt1(w);
