@@ -43,6 +43,46 @@ __device__ __forceinline__ void sw_load_layer(SwLayer<Real> &c, const Real *__re
     c.indf = unpack_indf(w);
 }
 
+// Per-column part of the SW prep for column col (after the per-layer part,
+// whose packed indices ipk hold jp): laytrop and the solar-source reference
+// layer of each band, laysolfr[b * ncol + col].  Bands searching the lower
+// atmosphere (18-25) take min(lay+1, laytrop) at the last jp crossing of
+// layreffr, default laytrop; bands searching aloft (16, 17, 27, 28, 29) take
+// the last crossing above laytrop, default nlayers; band 26 uses laytrop
+// (e.g. rrtmg_sw_taumol.f90:292-295, 391-394, 494-496).  One pass over the
+// layers tests every band's crossing.
+__device__ __forceinline__ void sw_column_refs(const double *__restrict__ play, const int *__restrict__ ipk,
+                                               int ncol, int nlay, int col, int *__restrict__ laytrop_out,
+                                               int *__restrict__ laysolfr)
+{
+    int laytrop = 0;
+    for (int lay = 0; lay < nlay; ++lay)
+        if (log(play[(size_t)lay * ncol + col]) > 4.56) ++laytrop;
+    *laytrop_out = laytrop;
+    constexpr int lowref[14] = {0, 0, 6, 3, 3, 8, 2, 6, 1, 2, 0, 0, 0, 0};
+    constexpr int upref[14] = {18, 30, 0, 0, 0, 0, 0, 0, 0, 0, 0, 32, 58, 49};
+    int lsf[14];
+#pragma unroll
+    for (int b = 0; b < 14; ++b) lsf[b] = upref[b] > 0 ? nlay : laytrop;
+    const int uplo = max(laytrop + 1, 2);
+    int jp2 = 0, jp1 = unpack_jp(ipk[col]);  // jp of layers lay-2 and lay-1
+    for (int lay = 1; lay <= nlay; ++lay) {
+        const int jpc = lay < nlay ? unpack_jp(ipk[(size_t)lay * ncol + col]) : jp1;  // jp(lay), jp(nlay-1) on top
+#pragma unroll
+        for (int b = 0; b < 14; ++b) {
+            if (upref[b] > 0) {
+                if (lay >= uplo && jp2 < upref[b] && jp1 >= upref[b]) lsf[b] = lay;
+            } else if (lowref[b] > 0) {
+                if (lay <= laytrop && jp1 < lowref[b] && jpc >= lowref[b]) lsf[b] = min(lay + 1, laytrop);
+            }
+        }
+        jp2 = jp1;
+        jp1 = jpc;
+    }
+#pragma unroll
+    for (int b = 0; b < 14; ++b) laysolfr[(size_t)b * ncol + col] = max(lsf[b], 1);
+}
+
 // X(jb) for every SW band, for switches over compile-time band bodies
 #define SW_BANDS(X) X(16) X(17) X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29)
 

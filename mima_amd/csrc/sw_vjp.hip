@@ -198,31 +198,8 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
 __global__ __launch_bounds__(256) void sw_vjp_column_kernel(ColumnInputs in, SwVjpWork w)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ncol = in.ncol, nlay = in.nlay;
-    if (col >= ncol) return;
-    int laytrop = 0;
-    for (int lay = 0; lay < nlay; ++lay)
-        if (log(in.play[(size_t)lay * ncol + col]) > 4.56) ++laytrop;
-    w.laytrop[col] = laytrop;
-    auto jp_store = [&](int lay) { return unpack_jp(w.ipk[(size_t)lay * ncol + col]); };
-    const int lowref[14] = {0, 0, 6, 3, 3, 8, 2, 6, 1, 2, 0, 0, 0, 0};
-    const int upref[14] = {18, 30, 0, 0, 0, 0, 0, 0, 0, 0, 0, 32, 58, 49};
-    for (int b = 0; b < 14; ++b) {
-        int lsf;
-        if (upref[b] > 0) {
-            lsf = nlay;
-            for (int lay = max(laytrop + 1, 2); lay <= nlay; ++lay)
-                if (jp_store(lay - 2) < upref[b] && jp_store(lay - 1) >= upref[b]) lsf = lay;
-        } else {
-            lsf = laytrop;
-            if (lowref[b] > 0)
-                for (int lay = 1; lay <= laytrop; ++lay) {
-                    const int jpn = lay < nlay ? jp_store(lay) : jp_store(nlay - 1);
-                    if (jp_store(lay - 1) < lowref[b] && jpn >= lowref[b]) lsf = min(lay + 1, laytrop);
-                }
-        }
-        w.laysolfr[(size_t)b * ncol + col] = max(lsf, 1);
-    }
+    if (col >= in.ncol) return;
+    sw_column_refs(in.play, w.ipk, in.ncol, in.nlay, col, w.laytrop + col, w.laysolfr);
 }
 
 template <typename Real>
