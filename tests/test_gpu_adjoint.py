@@ -384,3 +384,24 @@ def test_torch_autograd_matches_finite_differences(rr):
     fd = (jp - jm) / 2.0
     print("autograd: ad %.8e fd %.8e" % (ad, fd))
     assert abs(ad - fd) <= 2e-3 * abs(fd), (ad, fd)
+
+
+def test_sw_vjp_large_call_matches_split_calls(rr):
+    """16384 columns x 40 levels in FP32: one row of backward scratch per wave
+    unit would exceed 2 GiB (82.7 KB x 30688 units), so the call runs the band
+    kernel in two chunked launches that reuse the rows; the same columns in
+    two 8192-column calls run one launch each.  Every column is computed
+    independently of the schedule: the gradients must agree bitwise."""
+    full = make_columns(128, 128, 40, seed=5)
+    n = full["ncol"]
+    assert n == 16384
+    rng = np.random.default_rng(17)
+    cot = _sw_cot(full, rng)
+    g_all = _sw_grad(rr, full, cot, "mixed")
+    halves = [np.arange(0, n // 2), np.arange(n // 2, n)]
+    for idx in halves:
+        part = subset(full, idx)
+        g = _sw_grad(rr, part, {k: v[:, idx] for k, v in cot.items()}, "mixed")
+        for k in SW_DIRS:
+            assert np.array_equal(g[k], g_all[k][:, idx]), k
+    assert any(np.any(g_all[k] != 0.0) for k in SW_DIRS)
