@@ -216,7 +216,10 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
  * B[n*ldb + k] (transB = 0, the weight layout) or B[k*ldb + n].  Epilogue:
  * v = C + bias[n] (bias may be NULL); out_pre = v; v = max(v, 0) if relu;
  * out_f32 = v (+= with accumulate); out_bf16 = bf16(v); outputs row pitch ldc,
- * each may be NULL. */
+ * each may be NULL.  With both operands transposed and few 64 x 64 output
+ * tiles (the weight gradient dY^T X) K is split over a library-owned
+ * workspace of up to 256 x 64 x 64 floats per device; like the other
+ * workspaces, calls that overlap on one device race on it. */
 int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA,
                         const void *B, int ldb, int transB, const float *bias, int relu,
                         float *out_pre, float *out_f32, void *out_bf16, int ldc,

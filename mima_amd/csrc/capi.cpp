@@ -74,7 +74,7 @@ struct DeviceState {
     LwHost lw;
     SideTables swd, lwd;
     Buffer sw_work, lw_work, sw_stage, lw_stage, zeros;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws;
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
     SwWork sw_w;
     LwWork lw_w;
@@ -984,8 +984,19 @@ int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA,
     ep.out_bf16 = (uint16_t *)out_bf16;
     ep.ldc = ldc;
     ep.accumulate = accumulate;
+    // A^T B with few output tiles: split K over the device's split-K
+    // workspace (falls back to the direct kernel if it cannot be allocated)
+    int ks = gemm_splitk_parts(M, N, K, transA, transB, lda, ldb, A, B);
+    float *part = nullptr;
+    if (ks) {
+        DeviceState *s = state();
+        if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->gemm_ws.ensure((size_t)ks * M * N * sizeof(float)) == 0) part = (float *)s->gemm_ws.p;
+        else ks = 0;
+    }
     hipError_t e = launch_gemm_bf16(M, N, K, (const uint16_t *)A, lda, transA, (const uint16_t *)B, ldb, transB, ep,
-                                    (hipStream_t)stream);
+                                    (hipStream_t)stream, part, ks);
     return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("gemm launch: ") + hipGetErrorString(e));
 }
 

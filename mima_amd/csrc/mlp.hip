@@ -13,6 +13,7 @@
 // stored [row][k], which absorbs the transposes at load time.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mlp.hpp"
@@ -187,9 +188,116 @@ __global__ __launch_bounds__(256) void gemm_bf16_nn_kernel(int M, int N, int K, 
             }
 }
 
-hipError_t launch_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
-                            int transB, const GemmEpilogue &ep, hipStream_t s)
+// ---------------------------------------------------------------------------
+// C = A^T B with both operands stored [k][*] (A(m, k) = A[k*lda + m], B(k, n)
+// = B[k*ldb + n]): the weight gradient dW = dY^T X, whose K is the batch and
+// whose 64 x 64 output tiles are too few to fill the chip (16 for 256 x 256).
+// K is split over grid.z; each workgroup accumulates its K range of one tile
+// (16-byte loads along m and n, K in steps of 32 through LDS) and writes the
+// FP32 partial to part[z][m][n]; splitk_reduce_kernel sums the partials in z
+// order (deterministic) and applies the epilogue.
+constexpr int TK = 32;        // K step
+constexpr int TLD = TK + 8;   // LDS row pitch (bf16)
+
+__global__ __launch_bounds__(256) void gemm_bf16_tt_splitk_kernel(int M, int N, int K, int kchunk,
+                                                                  const uint16_t *__restrict__ A, int lda,
+                                                                  const uint16_t *__restrict__ B, int ldb,
+                                                                  float *__restrict__ part)
 {
+    __shared__ uint16_t As[GT * TLD];
+    __shared__ uint16_t Bs[GT * TLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int kbeg = blockIdx.z * kchunk, kend = min(K, kbeg + kchunk);
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    // loader: thread t moves 8 consecutive m (or n) of row k = t / 8 of the step
+    const int lk = tid >> 3, lc = (tid & 7) * 8;
+    f32x4 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        const int gk = k0 + lk;
+        u32x4 va = {0u, 0u, 0u, 0u}, vb = {0u, 0u, 0u, 0u};
+        if (gk < kend) {
+            if (m0 + lc < M) va = *(const u32x4 *)(A + (size_t)gk * lda + m0 + lc);
+            if (n0 + lc < N) vb = *(const u32x4 *)(B + (size_t)gk * ldb + n0 + lc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            As[(lc + 2 * q) * TLD + lk] = (uint16_t)(va[q] & 0xffffu);
+            As[(lc + 2 * q + 1) * TLD + lk] = (uint16_t)(va[q] >> 16);
+            Bs[(lc + 2 * q) * TLD + lk] = (uint16_t)(vb[q] & 0xffffu);
+            Bs[(lc + 2 * q + 1) * TLD + lk] = (uint16_t)(vb[q] >> 16);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TK; kk += 16) {
+            const int kq = kk + 4 * (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const uint16_t *ap = As + (wm + 16 * i + (lane & 15)) * TLD + kq;
+                bf16x4 a = {(short)ap[0], (short)ap[1], (short)ap[2], (short)ap[3]};
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    const uint16_t *bp = Bs + (wn + 16 * j + (lane & 15)) * TLD + kq;
+                    bf16x4 b = {(short)bp[0], (short)bp[1], (short)bp[2], (short)bp[3]};
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    float *pz = part + (size_t)blockIdx.z * M * N;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + 16 * i + 4 * (lane >> 4) + r;
+                const int n = n0 + wn + 16 * j + (lane & 15);
+                if (m < M && n < N) pz[(size_t)m * N + n] = acc[i][j][r];
+            }
+}
+
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(int M, int N, int ks, const float *__restrict__ part,
+                                                            GemmEpilogue ep)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)M * N) return;
+    const int m = (int)(i / N), n = (int)(i - (size_t)m * N);
+    float v = 0.f;
+    for (int z = 0; z < ks; ++z) v += part[(size_t)z * M * N + i];
+    if (ep.bias) v += ep.bias[n];
+    const size_t o = (size_t)m * ep.ldc + n;
+    if (ep.out_pre) ep.out_pre[o] = v;
+    if (ep.relu) v = v > 0.f ? v : 0.f;
+    if (ep.out_f32) ep.out_f32[o] = ep.accumulate ? ep.out_f32[o] + v : v;
+    if (ep.out_bf16) ep.out_bf16[o] = f2bf(v);
+}
+
+int gemm_splitk_parts(int M, int N, int K, int transA, int transB, int lda, int ldb, const void *A, const void *B)
+{
+    if (!transA || !transB || M % 8 || N % 8 || lda % 8 || ldb % 8 || ((uintptr_t)A % 16) || ((uintptr_t)B % 16))
+        return 0;
+    const int tiles = ((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+    int ks = 256 / tiles;                      // about one workgroup per CU
+    ks = std::min(ks, (K + 2 * TK - 1) / (2 * TK));  // at least two K steps per split
+    return ks >= 2 ? ks : 0;
+}
+
+hipError_t launch_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
+                            int transB, const GemmEpilogue &ep, hipStream_t s, float *part, int ks)
+{
+    if (ks >= 2 && part) {
+        const int kchunk = ((K + ks - 1) / ks + TK - 1) / TK * TK;
+        const int kz = (K + kchunk - 1) / kchunk;
+        dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, kz), block(256);
+        hipLaunchKernelGGL(gemm_bf16_tt_splitk_kernel, grid, block, 0, s, M, N, K, kchunk, A, lda, B, ldb, part);
+        const int nb = (int)(((size_t)M * N + 255) / 256);
+        hipLaunchKernelGGL(splitk_reduce_kernel, dim3(nb), dim3(256), 0, s, M, N, kz, part, ep);
+        return hipGetLastError();
+    }
     // both operands K-contiguous with 16-byte aligned rows: the tiled kernel
     if (!transA && !transB && lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)A % 16) == 0 && ((uintptr_t)B % 16) == 0) {
         dim3 grid((N + NB - 1) / NB, (M + NB - 1) / NB), block(256);

@@ -24,8 +24,11 @@ struct AdamParams {
     float bc1, bc2;  // 1/(1 - beta1^t), 1/(1 - beta2^t)
 };
 
+// split-K factor of the A^T B kernel for this call (0: the direct kernels);
+// the caller provides part, ks x M x N floats of workspace
+int gemm_splitk_parts(int M, int N, int K, int transA, int transB, int lda, int ldb, const void *A, const void *B);
 hipError_t launch_gemm_bf16(int M, int N, int K, const uint16_t *A, int lda, int transA, const uint16_t *B, int ldb,
-                            int transB, const GemmEpilogue &ep, hipStream_t s);
+                            int transB, const GemmEpilogue &ep, hipStream_t s, float *part = nullptr, int ks = 0);
 hipError_t launch_relu_backward(size_t n, const float *dy, const float *pre, uint16_t *dz, float *dz32, hipStream_t s);
 hipError_t launch_col_sum(int M, int N, const float *x, int ldx, float *out, hipStream_t s);
 hipError_t launch_mse(int M, int N, const float *y, int ldy, const float *t, int ldt, float *dy, uint16_t *dy_bf16,

@@ -159,3 +159,30 @@ def test_surrogate_learns_and_falls_back_outside_envelope(env):
     _, _, rep = hybrid_heating(s, te, check=512)
     print(rep)
     assert rep["checked_columns"] == 512 and rep["rmse_lw_kday"] < 2.0 * rmse + 0.05
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 256, 1024), (80, 128, 1000), (256, 128, 96)])
+def test_gemm_bf16_splitk_path_matches_reference(env, M, N, K):
+    """A^T B with both operands stored [k][*] and few 64 x 64 output tiles (the
+    weight gradient dY^T X): K is split over workgroups and the FP32 partials
+    are summed in a fixed order.  Checked against the FP64 product of the same
+    BF16 values, with the accumulate epilogue, and bitwise reproducible."""
+    rr, torch = env
+    from mima_amd.surrogate import _p, _stream
+    rng = np.random.default_rng(M + N + K)
+    a = rng.standard_normal((K, M)).astype(np.float32)
+    b = rng.standard_normal((K, N)).astype(np.float32)
+    A, a64 = _bf(torch, a)
+    B, b64 = _bf(torch, b)
+    ref = a64.T @ b64
+    base = rng.standard_normal((M, N)).astype(np.float32)
+    outs = []
+    for _ in range(2):
+        out = torch.from_numpy(base.copy()).cuda()
+        rc = rr.lib().rrtmg_hip_gemm_bf16(M, N, K, _p(A), M, 1, _p(B), N, 1, None, 0, None, _p(out), None, N, 1,
+                                          _stream())
+        assert rc == 0
+        torch.cuda.synchronize()
+        outs.append(out.cpu().numpy())
+    np.testing.assert_allclose(outs[0], base.astype(np.float64) + ref, rtol=1e-5, atol=2e-4 * np.sqrt(K))
+    assert np.array_equal(outs[0], outs[1])
