@@ -12,7 +12,9 @@ roofline: the dominant kernel (longest of the band kernels) timed with HIP
 events on its own stream inside the library; achieved = algorithmic interface
 bytes per column (8 * (12 L + 10) B: play, tlay, q, O3, plev, tlev, Ts,
 coszen, albedo, CO2 in; LW/SW heating and up/down fluxes out, FP64) times the
-columns of one launch, divided by that kernel's mean duration.
+columns of one launch, divided by that kernel's mean duration.  The band
+kernels are bound by VALU issue and load latency, not by HBM: valu_issue_frac
+is the kernel's VALU issue fraction from the committed rocprofv3 PMC profile.
 
 cpu_baseline: the reference RRTMG Fortran itself (rrtmg_sw + rrtmg_lw,
 compiled from the MiMA sources into oracle/_ref/; LW on the seeded k-tables)
@@ -85,6 +87,22 @@ def copy_probe(torch, mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
+
+
+def valu_issue_from_profile(kernel, ncol, nlay):
+    """VALU issue fraction of `kernel`'s FP32 instantiation, 4 SQ_INSTS_VALU /
+    (1024 SIMDs x cycles), as measured with rocprofv3 --pmc and recorded in
+    profiles/valu_32768x40.txt; None when that workload was not measured.
+    The band kernels are bound by VALU issue and load latency, not by HBM."""
+    path = os.path.join(ROOT, "profiles", "valu_32768x40.txt")
+    names = {"lw_band": "lw_band_lds_kernel<float>", "sw_band": "sw_band_kernel<float>"}
+    if (ncol, nlay) != (32768, 40) or kernel not in names or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = [x.strip() for x in line.split("|")]
+        if len(f) > 9 and f[1] == names[kernel] and f[9].endswith("%"):
+            return float(f[9][:-1]) / 100.0
+    return None
 
 
 def traffic_from_profile(kernel, ncol, nlay):
@@ -431,7 +449,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profile(names[dom_key], ncol, nlay),
-                     "kernel": names[dom_key], "bytes_per_column": bcol},
+                     "kernel": names[dom_key], "bytes_per_column": bcol,
+                     "valu_issue_frac": valu_issue_from_profile(names[dom_key], ncol, nlay)},
         "cpu_baseline": cpu,
         "adjoint": adj,
         "surrogate": sur,
