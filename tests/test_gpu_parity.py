@@ -262,3 +262,35 @@ def test_error_contract(rr):
     # outputs are untouched on failure and the library keeps working
     ok = rr.sw_columns(c)
     assert np.all(np.isfinite(ok["swdflx"]))
+
+
+def test_lw_one_wave_kernel_matches_lds_kernel(rr, tmp_path):
+    """The FP32 LW forward runs lw_band_lds_kernel (k-tables in LDS, persistent
+    workgroups); RRTMG_HIP_LW_LDS=0 at init selects lw_band_kernel (one wave
+    per unit, k-tables from L2), the path for devices whose LDS cannot hold a
+    band's tables.  Same arithmetic, same order: the fluxes must be bitwise
+    equal.  The other kernel runs in a child process (the choice is made at
+    init), on the same columns."""
+    import os
+    import subprocess
+    import sys
+    c = _sample(600, seed=21)
+    np.savez(tmp_path / "cols.npz", **{k: np.asarray(v) for k, v in c.items()})
+    here = _lw(rr, c, "mixed")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    child = (
+        "import sys, numpy as np\n"
+        "sys.path.insert(0, %r)\n"
+        "from mima_amd import rrtmg\n"
+        "z = np.load(%r)\n"
+        "c = {k: (z[k] if z[k].ndim else z[k].item()) for k in z.files}\n"
+        "rrtmg.rrtmg_lw_ini(); rrtmg.set_precision('mixed')\n"
+        "out = rrtmg.lw_columns(rrtmg.to_device(c))\n"
+        "np.savez(%r, **{k: v.cpu().numpy() for k, v in out.items()})\n"
+    ) % (root, str(tmp_path / "cols.npz"), str(tmp_path / "one_wave.npz"))
+    env = dict(os.environ, RRTMG_HIP_LW_LDS="0")
+    r = subprocess.run([sys.executable, "-c", child], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    other = np.load(tmp_path / "one_wave.npz")
+    for k in here:
+        assert np.array_equal(here[k], other[k]), k
