@@ -426,7 +426,7 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
 
     // workspace
     size_t off = 0;
-    const size_t o_coef = off; off = align_up(off + rs * SWF_COUNT * n2);
+    const size_t o_coef = off; off = align_up(off + rs * SWF_STRIDE * n2);
     const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
     const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
     const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
@@ -552,7 +552,7 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     if ((rc = check_lds(lw_fwd_lds_bytes(nlay, rs)))) return rc;
     int ck = 0;
     size_t off = 0;
-    const size_t o_coef = off; off = align_up(off + rs * LWF_COUNT * n2);
+    const size_t o_coef = off; off = align_up(off + rs * LWF_STRIDE * n2);
     const size_t o_tlev = off; off = align_up(off + rs * n3);
     const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
     const size_t o_ilev = off; off = align_up(off + sizeof(int) * n3);

@@ -27,7 +27,7 @@ struct LwOutputs {
 
 // Per-call device workspace (owned by the library, grown on demand).
 struct SwWork {
-    void *coef = nullptr;     // [SWF_COUNT][nlay][ncol] of Real
+    void *coef = nullptr;     // [nlay][ncol][SWF_STRIDE] of Real (one record per layer and column)
     int *ipk = nullptr;       // [nlay][ncol]
     int *laytrop = nullptr;   // [ncol]
     int *laysolfr = nullptr;  // [14][ncol]
@@ -36,7 +36,7 @@ struct SwWork {
     int chunk = 0;            // wave units per band-kernel launch (rows of dstate)
 };
 struct LwWork {
-    void *coef = nullptr;     // [LWF_COUNT][nlay][ncol] of Real
+    void *coef = nullptr;     // [nlay][ncol][LWF_STRIDE] of Real (one record per layer and column)
     void *tlev = nullptr;     // [nlay+1][ncol] of Real (level Planck interpolation fraction)
     int *ipk = nullptr;       // [nlay][ncol]
     int *ilev = nullptr;      // [nlay+1][ncol] Planck table index of levels
@@ -112,7 +112,7 @@ struct SwGrad {
     double *tlay, *h2o, *o3;
 };
 struct LwVjpWork {
-    void *dcoef = nullptr;    // [LWF_COUNT][4][nlay][ncol] of Real: value, d/dT, d/dq, d/dO3
+    void *dcoef = nullptr;    // [nlay][ncol][LWF_COUNT][4] of Real: value and d/dT, d/dq, d/dO3 per field
     void *tlev = nullptr;     // [nlay+1][ncol] of Real
     int *ipk = nullptr;       // [nlay][ncol]
     int *ilev = nullptr;      // [nlay+1][ncol]
@@ -129,7 +129,7 @@ struct LwVjpWork {
     int *items = nullptr;     // work-item counter of the LDS-table band kernel
 };
 struct SwVjpWork {
-    void *dcoef = nullptr;    // [SWF_COUNT][4][nlay][ncol] of Real
+    void *dcoef = nullptr;    // [nlay][ncol][SWF_COUNT][4] of Real: value and d/dT, d/dq, d/dO3 per field
     int *ipk = nullptr;
     int *laytrop = nullptr;
     int *laysolfr = nullptr;  // [14][ncol]

@@ -44,7 +44,6 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    const size_t fs = (size_t)nlay * ncol;
     // level 0 Planck index (setcoef: indlev0, t0frac)
     if (lay == 0) {
         double tz0 = in.tlev[col];
@@ -125,32 +124,32 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         if (colch4 == 0.0) colch4 = 1.e-32 * coldry;
         double colbrd = 1.e-20 * wbrodl;
         double compfp = 1. - fp;
-        coef[LWF_FAC10 * fs + ix] = (Real)(compfp * ft);
-        coef[LWF_FAC00 * fs + ix] = (Real)(compfp * (1.0 - ft));
-        coef[LWF_FAC11 * fs + ix] = (Real)(fp * ft1);
-        coef[LWF_FAC01 * fs + ix] = (Real)(fp * (1.0 - ft1));
-        coef[LWF_COLH2O * fs + ix] = (Real)colh2o;
-        coef[LWF_COLCO2 * fs + ix] = (Real)colco2;
-        coef[LWF_COLO3 * fs + ix] = (Real)colo3;
-        coef[LWF_COLN2O * fs + ix] = (Real)coln2o;
-        coef[LWF_COLCO * fs + ix] = (Real)colco;
-        coef[LWF_COLCH4 * fs + ix] = (Real)colch4;
-        coef[LWF_COLO2 * fs + ix] = (Real)colo2;
-        coef[LWF_COLBRD * fs + ix] = (Real)colbrd;
-        coef[LWF_SELFFAC * fs + ix] = (Real)(colh2o * selffac);
-        coef[LWF_SELFFRAC * fs + ix] = (Real)selffrac;
-        coef[LWF_FORFAC * fs + ix] = (Real)(colh2o * forfac);
-        coef[LWF_FORFRAC * fs + ix] = (Real)forfrac;
-        coef[LWF_MINORFRAC * fs + ix] = (Real)minorfrac;
-        coef[LWF_SCALEMINOR * fs + ix] = (Real)scaleminor;
-        coef[LWF_SCALEMINORN2 * fs + ix] = (Real)scaleminorn2;
-        coef[LWF_COLDRY * fs + ix] = (Real)coldry;
-        coef[LWF_PAVEL * fs + ix] = (Real)pavel;
-        coef[LWF_WX1 * fs + ix] = (Real)wx1;
-        coef[LWF_WX2 * fs + ix] = (Real)wx2;
-        coef[LWF_WX3 * fs + ix] = (Real)wx3;
-        coef[LWF_WX4 * fs + ix] = (Real)wx4;
-        coef[LWF_TLAYFRAC * fs + ix] = (Real)tlayfrac;
+        coef[ix * LWF_STRIDE + LWF_FAC10] = (Real)(compfp * ft);
+        coef[ix * LWF_STRIDE + LWF_FAC00] = (Real)(compfp * (1.0 - ft));
+        coef[ix * LWF_STRIDE + LWF_FAC11] = (Real)(fp * ft1);
+        coef[ix * LWF_STRIDE + LWF_FAC01] = (Real)(fp * (1.0 - ft1));
+        coef[ix * LWF_STRIDE + LWF_COLH2O] = (Real)colh2o;
+        coef[ix * LWF_STRIDE + LWF_COLCO2] = (Real)colco2;
+        coef[ix * LWF_STRIDE + LWF_COLO3] = (Real)colo3;
+        coef[ix * LWF_STRIDE + LWF_COLN2O] = (Real)coln2o;
+        coef[ix * LWF_STRIDE + LWF_COLCO] = (Real)colco;
+        coef[ix * LWF_STRIDE + LWF_COLCH4] = (Real)colch4;
+        coef[ix * LWF_STRIDE + LWF_COLO2] = (Real)colo2;
+        coef[ix * LWF_STRIDE + LWF_COLBRD] = (Real)colbrd;
+        coef[ix * LWF_STRIDE + LWF_SELFFAC] = (Real)(colh2o * selffac);
+        coef[ix * LWF_STRIDE + LWF_SELFFRAC] = (Real)selffrac;
+        coef[ix * LWF_STRIDE + LWF_FORFAC] = (Real)(colh2o * forfac);
+        coef[ix * LWF_STRIDE + LWF_FORFRAC] = (Real)forfrac;
+        coef[ix * LWF_STRIDE + LWF_MINORFRAC] = (Real)minorfrac;
+        coef[ix * LWF_STRIDE + LWF_SCALEMINOR] = (Real)scaleminor;
+        coef[ix * LWF_STRIDE + LWF_SCALEMINORN2] = (Real)scaleminorn2;
+        coef[ix * LWF_STRIDE + LWF_COLDRY] = (Real)coldry;
+        coef[ix * LWF_STRIDE + LWF_PAVEL] = (Real)pavel;
+        coef[ix * LWF_STRIDE + LWF_WX1] = (Real)wx1;
+        coef[ix * LWF_STRIDE + LWF_WX2] = (Real)wx2;
+        coef[ix * LWF_STRIDE + LWF_WX3] = (Real)wx3;
+        coef[ix * LWF_STRIDE + LWF_WX4] = (Real)wx4;
+        coef[ix * LWF_STRIDE + LWF_TLAYFRAC] = (Real)tlayfrac;
         w.ipk[ix] = pack_lw(jp, jt, jt1, indself, indfor, indminor, indlay);
     }
 }

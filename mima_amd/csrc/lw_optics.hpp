@@ -29,8 +29,10 @@ __device__ __forceinline__ void lw_load_layer(LwLayer<Real> &c, const Real *__re
                                               const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
     const unsigned ix = (unsigned)(lay * ncol + col);
-    const size_t fs = (size_t)nlay * ncol;
-#define LD(f, F) c.f = at32(coef + F * fs, ix)
+    // the (layer, column) record, [lay][col][LWF_STRIDE], 16-byte aligned
+    const Real *rec = (const Real *)__builtin_assume_aligned(coef + (size_t)ix * LWF_STRIDE, 16);
+    (void)nlay;
+#define LD(f, F) c.f = rec[F]
     LD(fac00, LWF_FAC00); LD(fac01, LWF_FAC01); LD(fac10, LWF_FAC10); LD(fac11, LWF_FAC11);
     LD(colh2o, LWF_COLH2O); LD(colco2, LWF_COLCO2); LD(colo3, LWF_COLO3); LD(coln2o, LWF_COLN2O);
     LD(colco, LWF_COLCO); LD(colch4, LWF_COLCH4); LD(colo2, LWF_COLO2); LD(colbrd, LWF_COLBRD);

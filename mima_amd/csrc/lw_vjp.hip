@@ -50,20 +50,21 @@ using D3 = Dual<double, 3>;
 enum : unsigned { DT = 1u, DQ = 2u, DO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct DLoad;
-// (p: the field's uniform base; ix: the element, a 32-bit offset)
+// (p: the field's [value, d/dT, d/dq, d/dO3] in the cell's record, 16-byte
+// aligned, so the components load together)
 template <typename Real, unsigned MASK>
 struct DLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p, unsigned ix, size_t) { return at32(p, ix); }
+    __device__ static Real get(const Real *__restrict__ p) { return p[0]; }
 };
 template <typename Real, unsigned MASK>
 struct DLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, unsigned ix, size_t cs)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ p)
     {
         Dual<Real, 3> r;
-        r.v = at32(p, ix);
-        r.d[0] = (MASK & DT) ? at32(p + cs, ix) : (Real)0;
-        r.d[1] = (MASK & DQ) ? at32(p + 2 * cs, ix) : (Real)0;
-        r.d[2] = (MASK & DO) ? at32(p + 3 * cs, ix) : (Real)0;
+        r.v = p[0];
+        r.d[0] = (MASK & DT) ? p[1] : (Real)0;
+        r.d[1] = (MASK & DQ) ? p[2] : (Real)0;
+        r.d[2] = (MASK & DO) ? p[3] : (Real)0;
         return r;
     }
 };
@@ -77,9 +78,11 @@ template <typename V, typename Real>
 __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__restrict__ dco,
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t fs = (size_t)nlay * ncol;
+    // the cell's record, [lay][col][LWF_COUNT][4], 16-byte aligned
+    const Real *rec = (const Real *)__builtin_assume_aligned(dco + (size_t)(lay * ncol + col) * LWF_COUNT * 4, 16);
     const unsigned ix = (unsigned)(lay * ncol + col);
-#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs, ix, fs)
+    (void)nlay;
+#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(rec + (F) * 4)
     LDD(fac00, LWF_FAC00, DT); LDD(fac01, LWF_FAC01, DT); LDD(fac10, LWF_FAC10, DT); LDD(fac11, LWF_FAC11, DT);
     LDD(colh2o, LWF_COLH2O, DQ); LDD(colco2, LWF_COLCO2, DQ); LDD(colo3, LWF_COLO3, DQ | DO);
     LDD(coln2o, LWF_COLN2O, DQ); LDD(colco, LWF_COLCO, DQ); LDD(colch4, LWF_COLCH4, DQ); LDD(colo2, LWF_COLO2, DQ);
@@ -118,13 +121,13 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    const size_t fs = (size_t)nlay * ncol;
+    // one record per (layer, column): [LWF_COUNT][value, d/dT, d/dq, d/dO3]
     auto put = [&](int F, size_t ix, const D3 &x) {
-        const size_t b = (size_t)F * 4 * fs + ix;
+        const size_t b = (ix * LWF_COUNT + (size_t)F) * 4;
         dco[b] = (Real)x.v;
-        dco[b + fs] = (Real)x.d[0];
-        dco[b + 2 * fs] = (Real)x.d[1];
-        dco[b + 3 * fs] = (Real)x.d[2];
+        dco[b + 1] = (Real)x.d[0];
+        dco[b + 2] = (Real)x.d[1];
+        dco[b + 3] = (Real)x.d[2];
     };
     if (lay == 0) {
         double tz0 = in.tlev[col];

@@ -56,6 +56,9 @@ enum SwField {
     SWF_SELFFAC, SWF_SELFFRAC, SWF_FORFAC, SWF_FORFRAC,
     SWF_COUNT
 };
+// the forward prep's coefficient record of one (layer, column), padded to a
+// 16-byte multiple (as LWF_STRIDE)
+constexpr int SWF_STRIDE = (SWF_COUNT + 3) / 4 * 4;
 
 // ---------------------------------------------------------------------------
 // LW
@@ -94,6 +97,10 @@ enum LwField {
     LWF_WX1, LWF_WX2, LWF_WX3, LWF_WX4, LWF_TLAYFRAC,
     LWF_COUNT
 };
+// the forward prep's coefficient record of one (layer, column): LWF_COUNT
+// fields padded to a 16-byte multiple, so a band kernel lane loads its
+// column's fields with a few 16-byte loads
+constexpr int LWF_STRIDE = (LWF_COUNT + 3) / 4 * 4;
 
 // packed per-layer integer word: jp | jt<<6 | jt1<<9 | indself<<12 | indfor<<16 | indminor<<18 | indlay<<23
 __host__ __device__ inline int pack_lw(int jp, int jt, int jt1, int inds, int indf, int indm, int indlay)

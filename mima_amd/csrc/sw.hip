@@ -88,21 +88,20 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
             indself = 0;
         }
         double compfp = 1.0 - fp;
-        const size_t fs = (size_t)nlay * ncol;
-        coef[SWF_FAC10 * fs + ix] = (Real)(compfp * ft);
-        coef[SWF_FAC00 * fs + ix] = (Real)(compfp * (1.0 - ft));
-        coef[SWF_FAC11 * fs + ix] = (Real)(fp * ft1);
-        coef[SWF_FAC01 * fs + ix] = (Real)(fp * (1.0 - ft1));
-        coef[SWF_COLH2O * fs + ix] = (Real)colh2o;
-        coef[SWF_COLCO2 * fs + ix] = (Real)colco2;
-        coef[SWF_COLO3 * fs + ix] = (Real)colo3;
-        coef[SWF_COLCH4 * fs + ix] = (Real)colch4;
-        coef[SWF_COLO2 * fs + ix] = (Real)colo2;
-        coef[SWF_COLMOL * fs + ix] = (Real)colmol;
-        coef[SWF_SELFFAC * fs + ix] = (Real)selffac;
-        coef[SWF_SELFFRAC * fs + ix] = (Real)selffrac;
-        coef[SWF_FORFAC * fs + ix] = (Real)forfac;
-        coef[SWF_FORFRAC * fs + ix] = (Real)forfrac;
+        coef[ix * SWF_STRIDE + SWF_FAC10] = (Real)(compfp * ft);
+        coef[ix * SWF_STRIDE + SWF_FAC00] = (Real)(compfp * (1.0 - ft));
+        coef[ix * SWF_STRIDE + SWF_FAC11] = (Real)(fp * ft1);
+        coef[ix * SWF_STRIDE + SWF_FAC01] = (Real)(fp * (1.0 - ft1));
+        coef[ix * SWF_STRIDE + SWF_COLH2O] = (Real)colh2o;
+        coef[ix * SWF_STRIDE + SWF_COLCO2] = (Real)colco2;
+        coef[ix * SWF_STRIDE + SWF_COLO3] = (Real)colo3;
+        coef[ix * SWF_STRIDE + SWF_COLCH4] = (Real)colch4;
+        coef[ix * SWF_STRIDE + SWF_COLO2] = (Real)colo2;
+        coef[ix * SWF_STRIDE + SWF_COLMOL] = (Real)colmol;
+        coef[ix * SWF_STRIDE + SWF_SELFFAC] = (Real)selffac;
+        coef[ix * SWF_STRIDE + SWF_SELFFRAC] = (Real)selffrac;
+        coef[ix * SWF_STRIDE + SWF_FORFAC] = (Real)forfac;
+        coef[ix * SWF_STRIDE + SWF_FORFRAC] = (Real)forfrac;
         w.ipk[ix] = pack_sw(jp, jt, jt1, indself, indfor);
     }
 }

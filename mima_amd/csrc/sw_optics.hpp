@@ -28,8 +28,10 @@ __device__ __forceinline__ void sw_load_layer(SwLayer<Real> &c, const Real *__re
                                               const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
     const unsigned ix = (unsigned)(lay * ncol + col);
-    const size_t fs = (size_t)nlay * ncol;
-#define LD(f, F) c.f = at32(coef + F * fs, ix)
+    // the (layer, column) record, [lay][col][SWF_STRIDE], 16-byte aligned
+    const Real *rec = (const Real *)__builtin_assume_aligned(coef + (size_t)ix * SWF_STRIDE, 16);
+    (void)nlay;
+#define LD(f, F) c.f = rec[F]
     LD(fac00, SWF_FAC00); LD(fac01, SWF_FAC01); LD(fac10, SWF_FAC10); LD(fac11, SWF_FAC11);
     LD(colh2o, SWF_COLH2O); LD(colco2, SWF_COLCO2); LD(colo3, SWF_COLO3); LD(colch4, SWF_COLCH4);
     LD(colo2, SWF_COLO2); LD(colmol, SWF_COLMOL);

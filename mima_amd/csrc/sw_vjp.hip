@@ -43,20 +43,21 @@ using D3 = Dual<double, 3>;
 enum : unsigned { SDT = 1u, SDQ = 2u, SDO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct SwDLoad;
-// (p: the field's uniform base; ix: the element, a 32-bit offset)
+// (p: the field's [value, d/dT, d/dq, d/dO3] in the cell's record, 16-byte
+// aligned, so the components load together)
 template <typename Real, unsigned MASK>
 struct SwDLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p, unsigned ix, size_t) { return at32(p, ix); }
+    __device__ static Real get(const Real *__restrict__ p) { return p[0]; }
 };
 template <typename Real, unsigned MASK>
 struct SwDLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p, unsigned ix, size_t cs)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ p)
     {
         Dual<Real, 3> r;
-        r.v = at32(p, ix);
-        r.d[0] = (MASK & SDT) ? at32(p + cs, ix) : (Real)0;
-        r.d[1] = (MASK & SDQ) ? at32(p + 2 * cs, ix) : (Real)0;
-        r.d[2] = (MASK & SDO) ? at32(p + 3 * cs, ix) : (Real)0;
+        r.v = p[0];
+        r.d[0] = (MASK & SDT) ? p[1] : (Real)0;
+        r.d[1] = (MASK & SDQ) ? p[2] : (Real)0;
+        r.d[2] = (MASK & SDO) ? p[3] : (Real)0;
         return r;
     }
 };
@@ -65,9 +66,11 @@ template <typename V, typename Real>
 __device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__restrict__ dco,
                                                 const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
 {
-    const size_t fs = (size_t)nlay * ncol;
+    // the cell's record, [lay][col][SWF_COUNT][4], 16-byte aligned
+    const Real *rec = (const Real *)__builtin_assume_aligned(dco + (size_t)(lay * ncol + col) * SWF_COUNT * 4, 16);
     const unsigned ix = (unsigned)(lay * ncol + col);
-#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(dco + (size_t)(F) * 4 * fs, ix, fs)
+    (void)nlay;
+#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(rec + (F) * 4)
     LDD(fac00, SWF_FAC00, SDT); LDD(fac01, SWF_FAC01, SDT); LDD(fac10, SWF_FAC10, SDT); LDD(fac11, SWF_FAC11, SDT);
     LDD(colh2o, SWF_COLH2O, SDQ); LDD(colco2, SWF_COLCO2, SDQ); LDD(colo3, SWF_COLO3, SDQ | SDO);
     LDD(colch4, SWF_COLCH4, SDQ); LDD(colo2, SWF_COLO2, SDQ); LDD(colmol, SWF_COLMOL, SDQ);
@@ -100,13 +103,13 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    const size_t fs = (size_t)nlay * ncol;
+    // one record per (layer, column): [SWF_COUNT][value, d/dT, d/dq, d/dO3]
     auto put = [&](int F, size_t ix, const D3 &x) {
-        const size_t b = (size_t)F * 4 * fs + ix;
+        const size_t b = (ix * SWF_COUNT + (size_t)F) * 4;
         dco[b] = (Real)x.v;
-        dco[b + fs] = (Real)x.d[0];
-        dco[b + 2 * fs] = (Real)x.d[1];
-        dco[b + 3 * fs] = (Real)x.d[2];
+        dco[b + 1] = (Real)x.d[0];
+        dco[b + 2] = (Real)x.d[1];
+        dco[b + 3] = (Real)x.d[2];
     };
     {
         const size_t ix = (size_t)lay * ncol + col;
