@@ -157,12 +157,12 @@ inline size_t sw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * 
 inline size_t lw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 64); }
 inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
 {
-    return real_size * (3 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;
+    return real_size * (4 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;
 }
 inline size_t lw_vjp_state_bytes(int nlay, size_t real_size) { return real_size * 2 * (size_t)nlay * 64; }
 // LDS bytes of one backward band workgroup
 inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 4 * 64); }
-inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)2 * (nlay + 1) * 64 + 4 * 64); }
+inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)(nlay + 1) * 64 + 4 * 64); }
 
 // fwd: NULL, or the forward's outputs, formed from w.bflux (which the
 // backward band kernel then fills from the forward it recomputes) by the

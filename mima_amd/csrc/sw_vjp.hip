@@ -231,18 +231,20 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     if (!__any(active)) return;
     const int col = active ? col0 : 0;
     const size_t L1 = (size_t)(nlay + 1) * 64;
-    Real *s_prup = smem, *s_prupd = smem + L1;  // bottom-up sweep, LDS
-    Real *red = smem + 2 * L1;                   // [4][64]
-    // downward-sweep states, written top-down in pass 2 and read bottom-up in
-    // pass 3: this wave's row of the global scratch (LDS holds only the
-    // bottom-up states, so that LDS does not cap occupancy below the VGPRs)
-    Real *__restrict__ d_ptdbt = (Real *)w.dstate + (size_t)blk * (3 * L1 + (size_t)5 * nlay * 64);
+    const size_t LA = (size_t)nlay * 64;
+    Real *s_prup = smem;   // bottom-up sweep's prup, LDS
+    Real *red = smem + L1; // [4][64]
+    // this wave's row of the global scratch (LDS holds only prup, so that LDS
+    // does not cap occupancy below the VGPRs): downward-sweep states, written
+    // top-down in pass 2 and read bottom-up in pass 3;
+    Real *__restrict__ d_ptdbt = (Real *)w.dstate + (size_t)blk * (4 * L1 + 5 * LA);
     Real *__restrict__ d_ztdn = d_ptdbt + L1, *__restrict__ d_prdnd = d_ptdbt + 2 * L1;
     // adjoints of each layer's (r, rd, t, td, dbt) from the reverse of the
     // bottom-up sweep (pass 2), [5][nlay][64]: pass 3 adds the reverse of the
-    // downward sweep's and projects the sum once onto (T, q, O3)
+    // downward sweep's and projects the sum once onto (T, q, O3);
     Real *__restrict__ d_adj = d_ptdbt + 3 * L1;
-    const size_t LA = (size_t)nlay * 64;
+    // the bottom-up sweep's prupd (read one level ahead)
+    Real *__restrict__ s_prupd = d_adj + 5 * LA;
     const Real *dco = (const Real *)w.dcoef;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     Tab<Real> T{tab, ng, g};
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         const Real prup = r + (td * ((t - dbt) * prupd_n + dbt * prup_n)) * zreflect;
         const Real prupd = rd + td * td * prupd_n * zreflect;
         s_prup[(size_t)(jk - 1) * 64 + lane] = prup;
-        s_prupd[(size_t)(jk - 1) * 64 + lane] = prupd;
+        at32(s_prupd, (unsigned)((jk - 1) * 64 + lane)) = prupd;
         prup_n = prup;
         prupd_n = prupd;
     }
@@ -332,9 +334,11 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     // pass 2: top-down
     Real ztdn = (Real)1, prdnd = (Real)0, ptdbt = (Real)1;
     Real a_pu = (Real)0, a_pud = (Real)0, a_zinc = (Real)0;
+    Real pud_n = at32(s_prupd, (unsigned)lane);
     for (int jk = 1; jk <= nlay + 1; ++jk) {
         Real c_pu = (Real)0, c_pud = (Real)0;
-        const Real pu = s_prup[(size_t)(jk - 1) * 64 + lane], pud = s_prupd[(size_t)(jk - 1) * 64 + lane];
+        const Real pu = s_prup[(size_t)(jk - 1) * 64 + lane], pud = pud_n;
+        if (jk <= nlay) pud_n = at32(s_prupd, (unsigned)(jk * 64 + lane));
         if (jk > 1) {
             const int lay = nlay + 2 - jk;
             Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
@@ -406,15 +410,16 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
     Real A_b = (Real)0, A_z = (Real)0, A_p = (Real)0;  // adjoints of ptdbt, ztdn, prdnd at level jk
     // downward-sweep states at level jk, carried from the previous iteration
     Real pb_c = d_ptdbt[(size_t)nlay * 64 + lane], pz_c = d_ztdn[(size_t)nlay * 64 + lane],
-         pp_c = d_prdnd[(size_t)nlay * 64 + lane];
+         pp_c = d_prdnd[(size_t)nlay * 64 + lane], pud_c = at32(s_prupd, (unsigned)(nlay * 64 + lane));
     for (int jk = nlay + 1; jk >= 2; --jk) {
         // level jk-1's states, loaded ahead of their use in the reverse step
         const size_t k0 = (size_t)(jk - 2) * 64 + lane;
         const Real pb = d_ptdbt[k0], pz = d_ztdn[k0], pp = d_prdnd[k0];
+        const Real pud_next = at32(s_prupd, (unsigned)k0);
         {
             // local flux adjoints at level jk
             const size_t k = (size_t)(jk - 1) * 64 + lane;
-            const Real pu = s_prup[k], pud = s_prupd[k];
+            const Real pu = s_prup[k], pud = pud_c;
             const Real pb = pb_c, pz = pz_c, pp = pp_c;
             const Real zr = (Real)1 / ((Real)1 - pp * pud);
             const Real pfu = (pb * pu + (pz - pb) * pud) * zr;
@@ -435,6 +440,7 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         pb_c = pb;
         pz_c = pz;
         pp_c = pp;
+        pud_c = pud_next;
         const Real Zr = (Real)1 / ((Real)1 - rd * pp);
         const Real Bz = (pz - pb) + pb * r * pp;
         const Real at = A_z * pb;
