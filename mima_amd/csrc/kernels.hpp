@@ -151,9 +151,9 @@ inline int vjp_chunk(int units, size_t bytes_per_unit)
     if (c < 8) c = 8;
     return units < (int)c ? units : (int)c;
 }
-inline size_t sw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)(nlay + 1) * 64; }
+inline size_t sw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * 2 * (size_t)(nlay + 1) * 64; }
 inline size_t lw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)nlay * 64; }
-inline size_t sw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)(nlay + 1) * 64 + 64); }
+inline size_t sw_fwd_lds_bytes(int, size_t real_size) { return real_size * 64; }
 inline size_t lw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 64); }
 inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
 {

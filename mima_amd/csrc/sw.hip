@@ -7,8 +7,8 @@
 // tile); lane = (column, g); each band is a compile-time instantiation of the
 // band body (straight-line band code, registers sized per band).  Each lane
 // runs the adding method for its
-// g-point: a bottom-up sweep keeps prup/prupd of every level (prup in LDS,
-// prupd in a per-wave global scratch row), a
+// g-point: a bottom-up sweep keeps prup/prupd of every level (in a per-wave
+// global scratch row), a
 // top-down sweep recomputes the layer optics, carries ztdn/prdnd/ptdbt and
 // produces level fluxes, which are summed over the band's g-points of a
 // column (segmented sum over ng adjacent lanes) into per-band fluxes.
@@ -232,10 +232,11 @@ __global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, 
     const int tile = (blk0 + blk) / nwaves;
     const WaveUnit u = units[(blk0 + blk) % nwaves];
     const SwBand B = bands[u.band];
-    Real *s_prup = smem;                           // [(nlay+1)][64], LDS
-    Real *s_red = smem + (size_t)(nlay + 1) * 64;  // [64]
-    // prupd of every level in this wave's global scratch row, [(nlay+1)][64]
-    Real *s_prupd = (Real *)w.dstate + (size_t)blk * (nlay + 1) * 64;
+    Real *s_red = smem;  // [64], LDS (FP64 g-sums)
+    // prupd and prup of every level in this wave's global scratch row,
+    // [2][(nlay+1)][64] (in LDS they would cap occupancy below the VGPRs)
+    Real *s_prupd = (Real *)w.dstate + (size_t)blk * 2 * (nlay + 1) * 64;
+    Real *s_prup = s_prupd + (size_t)(nlay + 1) * 64;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
 #define BODY(jb)                                                                                              \
     case jb - 16:                                                                                             \
