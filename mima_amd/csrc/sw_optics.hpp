@@ -434,11 +434,40 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
     }
 }
 
+// reftra in FP64 for an FP32 layer (the islands below): r, rd, t, td as
+// plain FP32 values or as FP32 dual numbers.  Out of line on purpose: the
+// islands are rare, and inlined, their FP64 temporaries would set the
+// register count of the whole FP32 band kernel (forward: 107 VGPRs and 4
+// waves per SIMD, against 80 and 6 with the call).
+template <typename V>
+struct LayerOptics {
+    V r, rd, t, td;
+};
+template <typename V, typename V64>
+__device__ __forceinline__ LayerOptics<V> reftra_fp64(V taug, V taur, float prmu0, const float *__restrict__ exp_tbl)
+{
+    V64 R, RD, T, TD;
+    const V64 t64 = cast<V64>(taur) + cast<V64>(taug);
+    reftra<V64, float, double>(t64, cast<V64>(taur) / t64, cast<V64>(taug) / t64, (double)prmu0, exp_tbl, R, RD, T,
+                               TD);
+    return {cast<V>(R), cast<V>(RD), cast<V>(T), cast<V>(TD)};
+}
+[[maybe_unused]] static __device__ __attribute__((noinline)) LayerOptics<float>
+reftra_island(float taug, float taur, float prmu0, const float *__restrict__ exp_tbl)
+{
+    return reftra_fp64<float, double>(taug, taur, prmu0, exp_tbl);
+}
+[[maybe_unused]] static __device__ __attribute__((noinline)) LayerOptics<Dual<float, 2>>
+reftra_island(Dual<float, 2> taug, Dual<float, 2> taur, float prmu0, const float *__restrict__ exp_tbl)
+{
+    return reftra_fp64<Dual<float, 2>, Dual<double, 2>>(taug, taur, prmu0, exp_tbl);
+}
+
 // optics of one (layer, g): total optical depth, reftra and direct-beam
 // transmittance (rrtmg_sw_spcvrt.f90:375-550, clear sky, no aerosol).  V is
-// the arithmetic type in the band kernel's precision and V64 its FP64
-// counterpart for the islands of the FP32-mixed path.
-template <typename Real, typename V = Real, typename V64 = double>
+// the arithmetic type in the band kernel's precision (plain, or dual numbers
+// in the backward).
+template <typename Real, typename V = Real>
 __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, const Real *__restrict__ exp_tbl, V &r, V &rd,
                                                 V &t, V &td, V &dbt)
 {
@@ -449,24 +478,30 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
     if (sizeof(Real) == 4) {
         // FP64 islands of the mixed path: the removable 0/0 of the direct-beam
         // solution at k mu0 = 1, where both the numerator and zdenr of reftra
-        // vanish.  Near-conservative scattering needs none for the values
-        // (the eigenvalue is formed from 1 - w, see reftra) but does for the
-        // derivatives (dual V, the backward): with w > 0.999 the derivative
-        // with respect to the gas optical depth is a (1 - w)-sized difference
-        // of O(1) terms, which FP32 resolves to a few percent only
+        // vanish.  Both are O(d), d = |1 - k mu0|, and carry absolute errors
+        // of the FP32 epsilon, so FP32 values are good to ~6e-8/d relative:
+        // the island covers d < 0.002 (relative error above 3e-5 outside it).
+        // The derivatives (dual V, the backward) divide once more by d and
+        // keep d < 0.02.  Near-conservative scattering needs no island for the
+        // values (the eigenvalue is formed from 1 - w, see reftra) but does
+        // for the derivatives: with w > 0.999 the derivative with respect to
+        // the gas optical depth is a (1 - w)-sized difference of O(1) terms,
+        // which FP32 resolves to a few percent only
         Real wv = val(w), ov = val(omw);
         Real zrp = sqrt((Real)2 * ov * ((Real)2 - (Real)0.5 * wv)) * prmu0;
-        island = fabs((Real)1 - zrp) < (Real)0.02 || (sizeof(V) != sizeof(Real) && wv > (Real)0.999);
+        const bool dual = sizeof(V) != sizeof(Real);
+        island = fabs((Real)1 - zrp) < (dual ? (Real)0.02 : (Real)0.002) || (dual && wv > (Real)0.999);
     }
-    if (island) {
-        V64 R, RD, T, TD;
-        V64 t64 = cast<V64>(taur) + cast<V64>(taug);
-        reftra<V64, Real, double>(t64, cast<V64>(taur) / t64, cast<V64>(taug) / t64, (double)prmu0, exp_tbl, R, RD,
-                                  T, TD);
-        r = cast<V>(R);
-        rd = cast<V>(RD);
-        t = cast<V>(T);
-        td = cast<V>(TD);
+    if constexpr (sizeof(Real) == 4) {
+        if (island) {
+            const LayerOptics<V> o = reftra_island(taug, taur, prmu0, exp_tbl);
+            r = o.r;
+            rd = o.rd;
+            t = o.t;
+            td = o.td;
+        } else {
+            reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td);
+        }
     } else {
         reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td);
     }

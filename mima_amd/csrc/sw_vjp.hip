@@ -308,8 +308,8 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
         // the smooth functions the table stands for (the table's neighbouring
         // entries are not consistent to the derivative's accuracy where
         // reftra's k mu0 = 1 quotient is nearly 0/0)
-        sw_layer_optics<Real, V2, Dual<double, 2>>(V2::seed(o.tg.v, 0), V2::seed(o.tr.v, 1), prmu0,
-                                                   (const Real *)nullptr, o.r, o.rd, o.t, o.td, o.dbt);
+        sw_layer_optics<Real, V2>(V2::seed(o.tg.v, 0), V2::seed(o.tr.v, 1), prmu0, (const Real *)nullptr, o.r, o.rd,
+                                  o.t, o.td, o.dbt);
     };
     auto inactive_d = [](LayerD &o) {
         o.r = V2((Real)0);
