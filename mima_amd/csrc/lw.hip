@@ -37,21 +37,23 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
 {
     const int ncol = in.ncol, nlay = in.nlay;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)nlay * ncol) return;
+    const size_t n = (size_t)nlay * ncol, iblk = (size_t)blockIdx.x * blockDim.x;
     const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
-    Real *coef = (Real *)w.coef;
+    // this thread's coefficient record, staged for store_records
+    __shared__ __align__(16) Real srec[256 * LWF_STRIDE];
+    Real *rec = srec + threadIdx.x * LWF_STRIDE;
     Real *tlevf = (Real *)w.tlev;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
     // level 0 Planck index (setcoef: indlev0, t0frac)
-    if (lay == 0) {
+    if (i < n && lay == 0) {
         double tz0 = in.tlev[col];
         int i0 = clampi((int)(tz0 - 159.0), 1, 180);
         w.ilev[col] = i0;
         tlevf[col] = (Real)(tz0 - 159.0 - (double)i0);
     }
-    {
+    if (i < n) {
         const size_t ix = (size_t)lay * ncol + col;
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
@@ -124,34 +126,37 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         if (colch4 == 0.0) colch4 = 1.e-32 * coldry;
         double colbrd = 1.e-20 * wbrodl;
         double compfp = 1. - fp;
-        coef[ix * LWF_STRIDE + LWF_FAC10] = (Real)(compfp * ft);
-        coef[ix * LWF_STRIDE + LWF_FAC00] = (Real)(compfp * (1.0 - ft));
-        coef[ix * LWF_STRIDE + LWF_FAC11] = (Real)(fp * ft1);
-        coef[ix * LWF_STRIDE + LWF_FAC01] = (Real)(fp * (1.0 - ft1));
-        coef[ix * LWF_STRIDE + LWF_COLH2O] = (Real)colh2o;
-        coef[ix * LWF_STRIDE + LWF_COLCO2] = (Real)colco2;
-        coef[ix * LWF_STRIDE + LWF_COLO3] = (Real)colo3;
-        coef[ix * LWF_STRIDE + LWF_COLN2O] = (Real)coln2o;
-        coef[ix * LWF_STRIDE + LWF_COLCO] = (Real)colco;
-        coef[ix * LWF_STRIDE + LWF_COLCH4] = (Real)colch4;
-        coef[ix * LWF_STRIDE + LWF_COLO2] = (Real)colo2;
-        coef[ix * LWF_STRIDE + LWF_COLBRD] = (Real)colbrd;
-        coef[ix * LWF_STRIDE + LWF_SELFFAC] = (Real)(colh2o * selffac);
-        coef[ix * LWF_STRIDE + LWF_SELFFRAC] = (Real)selffrac;
-        coef[ix * LWF_STRIDE + LWF_FORFAC] = (Real)(colh2o * forfac);
-        coef[ix * LWF_STRIDE + LWF_FORFRAC] = (Real)forfrac;
-        coef[ix * LWF_STRIDE + LWF_MINORFRAC] = (Real)minorfrac;
-        coef[ix * LWF_STRIDE + LWF_SCALEMINOR] = (Real)scaleminor;
-        coef[ix * LWF_STRIDE + LWF_SCALEMINORN2] = (Real)scaleminorn2;
-        coef[ix * LWF_STRIDE + LWF_COLDRY] = (Real)coldry;
-        coef[ix * LWF_STRIDE + LWF_PAVEL] = (Real)pavel;
-        coef[ix * LWF_STRIDE + LWF_WX1] = (Real)wx1;
-        coef[ix * LWF_STRIDE + LWF_WX2] = (Real)wx2;
-        coef[ix * LWF_STRIDE + LWF_WX3] = (Real)wx3;
-        coef[ix * LWF_STRIDE + LWF_WX4] = (Real)wx4;
-        coef[ix * LWF_STRIDE + LWF_TLAYFRAC] = (Real)tlayfrac;
+        rec[LWF_FAC10] = (Real)(compfp * ft);
+        rec[LWF_FAC00] = (Real)(compfp * (1.0 - ft));
+        rec[LWF_FAC11] = (Real)(fp * ft1);
+        rec[LWF_FAC01] = (Real)(fp * (1.0 - ft1));
+        rec[LWF_COLH2O] = (Real)colh2o;
+        rec[LWF_COLCO2] = (Real)colco2;
+        rec[LWF_COLO3] = (Real)colo3;
+        rec[LWF_COLN2O] = (Real)coln2o;
+        rec[LWF_COLCO] = (Real)colco;
+        rec[LWF_COLCH4] = (Real)colch4;
+        rec[LWF_COLO2] = (Real)colo2;
+        rec[LWF_COLBRD] = (Real)colbrd;
+        rec[LWF_SELFFAC] = (Real)(colh2o * selffac);
+        rec[LWF_SELFFRAC] = (Real)selffrac;
+        rec[LWF_FORFAC] = (Real)(colh2o * forfac);
+        rec[LWF_FORFRAC] = (Real)forfrac;
+        rec[LWF_MINORFRAC] = (Real)minorfrac;
+        rec[LWF_SCALEMINOR] = (Real)scaleminor;
+        rec[LWF_SCALEMINORN2] = (Real)scaleminorn2;
+        rec[LWF_COLDRY] = (Real)coldry;
+        rec[LWF_PAVEL] = (Real)pavel;
+        rec[LWF_WX1] = (Real)wx1;
+        rec[LWF_WX2] = (Real)wx2;
+        rec[LWF_WX3] = (Real)wx3;
+        rec[LWF_WX4] = (Real)wx4;
+        rec[LWF_TLAYFRAC] = (Real)tlayfrac;
         w.ipk[ix] = pack_lw(jp, jt, jt1, indself, indfor, indminor, indlay);
+        for (int f = LWF_COUNT; f < LWF_STRIDE; ++f) rec[f] = (Real)0;
     }
+    const int nrec = (int)(n - iblk < blockDim.x ? n - iblk : blockDim.x);
+    store_records<Real, LWF_STRIDE>((Real *)w.coef + iblk * LWF_STRIDE, srec, nrec);
 }
 
 // per-column part of the prep (one thread per column): laytrop and the

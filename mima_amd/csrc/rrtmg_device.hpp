@@ -29,6 +29,23 @@ __device__ __forceinline__ int tbl_index(F x)
     return i < 0 ? 0 : (i > 10000 ? 10000 : i);  // keeps NaN/huge inputs inside the table
 }
 
+// Stores the records of a block's threads, staged in LDS (thread t's record
+// of R elements at srec[t * R]), to global memory, where the records of the
+// block's first nrec threads are contiguous from dst: consecutive threads
+// store consecutive 16-byte vectors.  From each thread directly, the stores
+// would be 16-byte pieces at the record's stride, one L2 request each.  Every
+// thread of the block calls it, after writing its own record.
+template <typename Real, int R>
+__device__ __forceinline__ void store_records(Real *__restrict__ dst, const Real *srec, int nrec)
+{
+    static_assert(R * sizeof(Real) % 16 == 0, "records are whole 16-byte vectors");
+    __syncthreads();
+    const int nvec = nrec * (int)(R * sizeof(Real) / 16);
+    const uint4 *s = (const uint4 *)srec;
+    uint4 *d = (uint4 *)dst;
+    for (int k = (int)threadIdx.x; k < nvec; k += (int)blockDim.x) d[k] = s[k];
+}
+
 // XCD-aware block numbering.  The 8 XCDs of the chip each have their own L2
 // and receive consecutive workgroups round-robin (workgroup i runs on XCD
 // i mod 8).  Renumbering i -> (i mod 8) * (grid/8) + i/8 hands each XCD a

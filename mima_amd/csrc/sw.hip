@@ -32,13 +32,15 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
 {
     const int ncol = in.ncol, nlay = in.nlay;
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)nlay * ncol) return;
+    const size_t n = (size_t)nlay * ncol, iblk = (size_t)blockIdx.x * blockDim.x;
     const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
-    Real *coef = (Real *)w.coef;
+    // this thread's coefficient record, staged for store_records
+    __shared__ __align__(16) Real srec[256 * SWF_STRIDE];
+    Real *rec = srec + threadIdx.x * SWF_STRIDE;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    {
+    if (i < n) {
         const size_t ix = (size_t)lay * ncol + col;
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
@@ -88,22 +90,25 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
             indself = 0;
         }
         double compfp = 1.0 - fp;
-        coef[ix * SWF_STRIDE + SWF_FAC10] = (Real)(compfp * ft);
-        coef[ix * SWF_STRIDE + SWF_FAC00] = (Real)(compfp * (1.0 - ft));
-        coef[ix * SWF_STRIDE + SWF_FAC11] = (Real)(fp * ft1);
-        coef[ix * SWF_STRIDE + SWF_FAC01] = (Real)(fp * (1.0 - ft1));
-        coef[ix * SWF_STRIDE + SWF_COLH2O] = (Real)colh2o;
-        coef[ix * SWF_STRIDE + SWF_COLCO2] = (Real)colco2;
-        coef[ix * SWF_STRIDE + SWF_COLO3] = (Real)colo3;
-        coef[ix * SWF_STRIDE + SWF_COLCH4] = (Real)colch4;
-        coef[ix * SWF_STRIDE + SWF_COLO2] = (Real)colo2;
-        coef[ix * SWF_STRIDE + SWF_COLMOL] = (Real)colmol;
-        coef[ix * SWF_STRIDE + SWF_SELFFAC] = (Real)selffac;
-        coef[ix * SWF_STRIDE + SWF_SELFFRAC] = (Real)selffrac;
-        coef[ix * SWF_STRIDE + SWF_FORFAC] = (Real)forfac;
-        coef[ix * SWF_STRIDE + SWF_FORFRAC] = (Real)forfrac;
+        rec[SWF_FAC10] = (Real)(compfp * ft);
+        rec[SWF_FAC00] = (Real)(compfp * (1.0 - ft));
+        rec[SWF_FAC11] = (Real)(fp * ft1);
+        rec[SWF_FAC01] = (Real)(fp * (1.0 - ft1));
+        rec[SWF_COLH2O] = (Real)colh2o;
+        rec[SWF_COLCO2] = (Real)colco2;
+        rec[SWF_COLO3] = (Real)colo3;
+        rec[SWF_COLCH4] = (Real)colch4;
+        rec[SWF_COLO2] = (Real)colo2;
+        rec[SWF_COLMOL] = (Real)colmol;
+        rec[SWF_SELFFAC] = (Real)selffac;
+        rec[SWF_SELFFRAC] = (Real)selffrac;
+        rec[SWF_FORFAC] = (Real)forfac;
+        rec[SWF_FORFRAC] = (Real)forfrac;
         w.ipk[ix] = pack_sw(jp, jt, jt1, indself, indfor);
+        for (int f = SWF_COUNT; f < SWF_STRIDE; ++f) rec[f] = (Real)0;
     }
+    const int nrec = (int)(n - iblk < blockDim.x ? n - iblk : blockDim.x);
+    store_records<Real, SWF_STRIDE>((Real *)w.coef + iblk * SWF_STRIDE, srec, nrec);
 }
 
 // per-column part of the prep (one thread per column, after the per-layer
