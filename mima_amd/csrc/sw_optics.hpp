@@ -358,10 +358,12 @@ __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *_
 // PIFM two-stream layer reflectance/transmittance, kmodts = 2, asymmetry 0
 // (rrtmg_sw_reftra.f90:127-303).  F is the arithmetic type.  zomw = 1 - zw
 // formed without cancellation (gas over total optical depth); the FP32
-// arithmetic uses it for the eigenvalue, see below.
+// arithmetic uses it for the eigenvalue, see below.  pdbt, when given,
+// receives the direct-beam transmittance exp(-min(zto1/prmuz, 500)) that
+// reftra looks up on the way.
 template <typename F, typename Real, typename S>
 __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real *__restrict__ exp_tbl, F &pref,
-                                       F &prefd, F &ptra, F &ptrad)
+                                       F &prefd, F &ptra, F &ptrad, F *pdbt = nullptr)
 {
     const F eps = (F)1.e-08, od_lo = (F)0.06, zwcrit = (F)0.9999995;
     const F zg = (F)0, zg3 = (F)3 * zg;
@@ -376,6 +378,7 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
         F zgt = zgamma1 * zto1;
         F ze1 = fmin(fdiv(zto1, (F)prmuz), (F)500);
         F ze2 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
+        if (pdbt) *pdbt = ze2;
         if (ze2 == (F)1) {
             pref = (F)0;
             ptra = (F)1;
@@ -418,6 +421,7 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
         F zep1 = fdiv((F)1, zem1);
         F zem2 = (exp_tbl && ze2 <= od_lo) ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
         F zep2 = fdiv((F)1, zem2);
+        if (pdbt) *pdbt = zem2;
         F zdenr = zr4 * zep1 + zr5 * zem1;
         F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
         if (zdenr >= -eps && zdenr <= eps) {
@@ -492,6 +496,11 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
         const bool dual = sizeof(V) != sizeof(Real);
         island = fabs((Real)1 - zrp) < (dual ? (Real)0.02 : (Real)0.002) || (dual && wv > (Real)0.999);
     }
+    // direct-beam transmittance (rrtmg_sw_spcvrt.f90:536-546): the lookup of
+    // exp(-tau/mu0) that reftra makes for zem2 (rrtmg_sw_reftra.f90:237-270)
+    // with the same operands, so it is taken from there.  Where reftra clamps
+    // tau/mu0 at 500, both table entries are the 1e-20 floor
+    // (rrtmg_sw_init.f90:96-104; past 46 in the smooth function).
     if constexpr (sizeof(Real) == 4) {
         if (island) {
             const LayerOptics<V> o = reftra_island(taug, taur, prmu0, exp_tbl);
@@ -499,14 +508,14 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
             rd = o.rd;
             t = o.t;
             td = o.td;
+            V ze1 = fdiv(tau, V(prmu0));
+            dbt = (exp_tbl && ze1 <= (Real)0.06) ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
         } else {
-            reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td);
+            reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td, &dbt);
         }
     } else {
-        reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td);
+        reftra<V, Real, Real>(tau, w, omw, prmu0, exp_tbl, r, rd, t, td, &dbt);
     }
-    V ze1 = fdiv(tau, V(prmu0));
-    dbt = (exp_tbl && ze1 <= (Real)0.06) ? V((Real)1 - ze1 + (Real)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
 }
 
 }  // namespace rrtmg
