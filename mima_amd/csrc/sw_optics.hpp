@@ -418,20 +418,36 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
         F ze1 = fmin(zrk * zto1, (F)500);
         F ze2 = fmin(fdiv(zto1, (F)prmuz), (F)500);
         F zem1 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
-        F zep1 = fdiv((F)1, zem1);
         F zem2 = (exp_tbl && ze2 <= od_lo) ? F((S)1 - ze2 + (S)0.5 * ze2 * ze2) : tbl_exp(ze2, exp_tbl);
-        F zep2 = fdiv((F)1, zem2);
         if (pdbt) *pdbt = zem2;
-        F zdenr = zr4 * zep1 + zr5 * zem1;
-        F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
-        if (zdenr >= -eps && zdenr <= eps) {
-            pref = eps;
-            ptra = zem2;
-        } else {
-            pref = fdiv(zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2), zdenr);
-            ptra = zem2 - fdiv(zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2), zdent);
-        }
         F zemm = zem1 * zem1;
+        if (sizeof(val(zto1)) == 4) {
+            // FP32: numerators and denominator multiplied by zem1, which
+            // leaves no 1/zem1 and, with zem2 zep2 = 1, no 1/zem2:
+            // zdenr zem1 = zr4 + zr5 zem1^2, and the zdenr test scales with it
+            const F den = zr4 + zr5 * zemm;
+            const F lim = eps * zem1;
+            if (den >= -lim && den <= lim) {
+                pref = eps;
+                ptra = zem2;
+            } else {
+                const F rden = fdiv((F)1, den);
+                pref = zw * (zr1 - zr2 * zemm - zr3 * (zem2 * zem1)) * rden;
+                ptra = zem2 - zw * (zem2 * (zt1 - zt2 * zemm) - zt3 * zem1) * rden;
+            }
+        } else {
+            F zep1 = fdiv((F)1, zem1);
+            F zep2 = fdiv((F)1, zem2);
+            F zdenr = zr4 * zep1 + zr5 * zem1;
+            F zdent = zr4 * zep1 + zr5 * zem1;  // zt4 = zr4, zt5 = zr5
+            if (zdenr >= -eps && zdenr <= eps) {
+                pref = eps;
+                ptra = zem2;
+            } else {
+                pref = fdiv(zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2), zdenr);
+                ptra = zem2 - fdiv(zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2), zdent);
+            }
+        }
         F zdend = fdiv((F)1, ((F)1 - zbeta * zemm) * zrkg);
         prefd = zgamma2 * ((F)1 - zemm) * zdend;
         ptrad = zrk2 * zem1 * zdend;
