@@ -355,6 +355,19 @@ __device__ __forceinline__ Dual<F, N> tbl_exp(const Dual<F, N> &x, const Real *_
     return r;
 }
 
+// The FP32 two-stream eigenvalue for asymmetry 0: with g = 0, gamma1 - gamma2
+// = 2 (1 - w) exactly, so k = sqrt(gamma1^2 - gamma2^2) = sqrt(2 (1 - w)
+// (gamma1 + gamma2)), formed from zomw = 1 - w without the cancellation of
+// the difference of squares as w -> 1 (rrtmg_sw_reftra.f90:211 has the
+// general form).  One function for reftra and the island test below, so the
+// two evaluate it identically and share it.
+template <typename F>
+__device__ __forceinline__ F eigen_k32(F zw, F zomw)
+{
+    const F zgamma1 = ((F)8 - zw * (F)5) * (F)0.25, zgamma2 = (F)3 * zw * (F)0.25;
+    return sqrt((F)2 * zomw * (zgamma1 + zgamma2));
+}
+
 // PIFM two-stream layer reflectance/transmittance, kmodts = 2, asymmetry 0
 // (rrtmg_sw_reftra.f90:127-303).  F is the arithmetic type.  zomw = 1 - zw
 // formed without cancellation (gas over total optical depth); the FP32
@@ -398,7 +411,7 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
         // the difference of squares as w -> 1; FP64 keeps the reference's form
         F zrk;
         if (sizeof(val(zto1)) == 4)
-            zrk = sqrt((F)2 * zomw * (zgamma1 + zgamma2));
+            zrk = eigen_k32(zw, zomw);
         else
             zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
         F zrp = zrk * prmuz;
@@ -414,7 +427,6 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
         F zt1 = zrp1 * (za1 + zrk * zgamma4);
         F zt2 = zrm1 * (za1 - zrk * zgamma4);
         F zt3 = zrk2 * (zgamma4 + za1 * prmuz);
-        F zbeta = fdiv(zgamma1 - zrk, zrkg);
         F ze1 = fmin(zrk * zto1, (F)500);
         F ze2 = fmin(fdiv(zto1, (F)prmuz), (F)500);
         F zem1 = (exp_tbl && ze1 <= od_lo) ? F((S)1 - ze1 + (S)0.5 * ze1 * ze1) : tbl_exp(ze1, exp_tbl);
@@ -448,7 +460,15 @@ __device__ __forceinline__ void reftra(F zto1, F zw, F zomw, S prmuz, const Real
                 ptra = zem2 - fdiv(zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2), zdent);
             }
         }
-        F zdend = fdiv((F)1, ((F)1 - zbeta * zemm) * zrkg);
+        // (1 - zbeta zemm) zrkg with zbeta = (zgamma1 - zrk) / zrkg; FP32 forms
+        // it without the quotient
+        F zdend;
+        if (sizeof(val(zto1)) == 4) {
+            zdend = fdiv((F)1, zrkg - (zgamma1 - zrk) * zemm);
+        } else {
+            F zbeta = fdiv(zgamma1 - zrk, zrkg);
+            zdend = fdiv((F)1, ((F)1 - zbeta * zemm) * zrkg);
+        }
         prefd = zgamma2 * ((F)1 - zemm) * zdend;
         ptrad = zrk2 * zem1 * zdend;
     }
@@ -508,7 +528,7 @@ __device__ __forceinline__ void sw_layer_optics(V taug, V taur, Real prmu0, cons
         // the gas optical depth is a (1 - w)-sized difference of O(1) terms,
         // which FP32 resolves to a few percent only
         Real wv = val(w), ov = val(omw);
-        Real zrp = sqrt((Real)2 * ov * ((Real)2 - (Real)0.5 * wv)) * prmu0;
+        Real zrp = eigen_k32(wv, ov) * prmu0;
         const bool dual = sizeof(V) != sizeof(Real);
         island = fabs((Real)1 - zrp) < (dual ? (Real)0.02 : (Real)0.002) || (dual && wv > (Real)0.999);
     }
