@@ -191,7 +191,28 @@ __device__ __forceinline__ LwRT<Real> lw_rt_exact(Real od, bool derivs)
     return r;
 }
 
-#define CHI(i, j) at32(cst->chi_mls, (unsigned)(((j)-1) * 7 + ((i)-1)))
+// a LwConst table in the band kernel's precision: the FP64 values or their
+// FP32 copies (so the FP32 kernels gather floats and convert nothing)
+template <typename Real>
+__device__ __forceinline__ const Real *lw_cst(const double *d, const float *f)
+{
+    if constexpr (sizeof(Real) == 4)
+        return f;
+    else
+        return d;
+}
+template <int NUM, int DEN, int LEV>
+constexpr int lw_chi_ratio_k()
+{
+    constexpr int k = lw_chi_ratio_index(NUM, DEN, LEV);
+    static_assert(k >= 0, "chi_mls ratio missing from LW_CHI_RATIOS");
+    return k;
+}
+// chi_mls(i, j); the rat_* table NAME at reference level e; the fixed-level
+// quotient chi_mls(a, j) / chi_mls(b, j), computed once at init
+#define CHI(i, j) at32(lw_cst<Real>(cst->chi_mls, cst->chi_mls_f), (unsigned)(((j)-1) * 7 + ((i)-1)))
+#define RAT(NAME, e) at32(lw_cst<Real>(cst->NAME, cst->NAME##_f), (unsigned)(e))
+#define CHIQ(a, b, j) lw_cst<Real>(cst->chi_ratio, cst->chi_ratio_f)[lw_chi_ratio_k<a, b, j>()]
 
 // Optical depth and Planck fraction of one (layer, g) of band BAND,
 // rrtmg_lw_taumol.f90:280-3147.
@@ -209,7 +230,6 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         ind1 = ((c.jp - 12) * 5 + (c.jt1 - 1)) * B.nspb;
     }
     const int jp = c.jp;
-    auto R = [](double x) { return (Real)x; };
     if constexpr (BAND == 1) {
         if (lower) {
             V corradj = c.pavel < (Real)250 ? V((Real)1 - fdiv((Real)0.15 * ((Real)250 - c.pavel), (Real)154.4)) : V((Real)1);
@@ -234,21 +254,21 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 3) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 3) / CHI(2, 3)), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 9) / CHI(2, 9)), (Real)8);
-            V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 3), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 9), (Real)8);
+            V adj = lw_adjcol(c.coln2o, c.coldry, CHI(4, jp + 1), (Real)1.5, (Real)0.5, (Real)0.65);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
                    adj * lw_minor2(T, B.ka_mn2o, 9, em, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)4);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)4);
-            Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 13) / CHI(2, 13)), (Real)4);
-            Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 13) / CHI(2, 13)), (Real)4);
-            V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp), (Real)4);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp + 1), (Real)4);
+            Eta<V> em = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 13), (Real)4);
+            Eta<V> ep = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 13), (Real)4);
+            V adj = lw_adjcol(c.coln2o, c.coldry, CHI(4, jp + 1), (Real)1.5, (Real)0.5, (Real)0.65);
             taug = lw_major_b(T, B.absb, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major_b(T, B.absb, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_for(T, B, c) +
                    adj * lw_minor2(T, B.kb_mn2o, 5, em, c);
@@ -256,16 +276,16 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 4) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 11) / CHI(2, 11)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp + 1), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 11), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp))), (Real)4);
-            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp + 1))), (Real)4);
-            Eta<V> ep = lw_eta(c.colo3, c.colco2, R(CHI(3, 13) / CHI(2, 13)), (Real)4);
+            Eta<V> e = lw_eta(c.colo3, c.colco2, RAT(rat_o3co2, jp), (Real)4);
+            Eta<V> e1 = lw_eta(c.colo3, c.colco2, RAT(rat_o3co2, jp + 1), (Real)4);
+            Eta<V> ep = lw_eta(c.colo3, c.colco2, CHIQ(3, 2, 13), (Real)4);
             taug = lw_major_b(T, B.absb, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major_b(T, B.absb, e1, ind1 + e1.js, c.fac01, c.fac11);
             // empirical g-point adjustments aloft (rrtmg_lw_taumol.f90:1011-1017)
@@ -278,25 +298,25 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 5) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.colh2o, c.colco2, R(CHI(1, 7) / CHI(2, 7)), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 5) / CHI(2, 5)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 7), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 5), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
                    lw_minor2(T, B.ka_mo3, 9, em, c) * c.colo3 + c.wx1 * T(B.ccl4, 1);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            Eta<V> e = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp))), (Real)4);
-            Eta<V> e1 = lw_eta(c.colo3, c.colco2, R(at32(cst->rat_o3co2, (unsigned)(jp + 1))), (Real)4);
-            Eta<V> ep = lw_eta(c.colo3, c.colco2, R(CHI(3, 43) / CHI(2, 43)), (Real)4);
+            Eta<V> e = lw_eta(c.colo3, c.colco2, RAT(rat_o3co2, jp), (Real)4);
+            Eta<V> e1 = lw_eta(c.colo3, c.colco2, RAT(rat_o3co2, jp + 1), (Real)4);
+            Eta<V> ep = lw_eta(c.colo3, c.colco2, CHIQ(3, 2, 43), (Real)4);
             taug = lw_major_b(T, B.absb, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major_b(T, B.absb, e1, ind1 + e1.js, c.fac01, c.fac11) + c.wx1 * T(B.ccl4, 1);
             fracs = lw_pfrac(T, B.fracrefb, ep);
         }
     } else if constexpr (BAND == 6) {
         if (lower) {
-            V adj = lw_adjcol(c.colco2, c.coldry, R(CHI(2, jp + 1)), (Real)3, (Real)2, (Real)0.77);
+            V adj = lw_adjcol(c.colco2, c.coldry, CHI(2, jp + 1), (Real)3, (Real)2, (Real)0.77);
             taug = c.colh2o * lw_four(T, B.absa, ind0 + 1, ind1 + 1, c) + lw_self(T, B, c) + lw_for(T, B, c) +
                    adj * lw_minor1(T, B.ka_mco2, c) + c.wx2 * T(B.cfc11adj, 1) + c.wx3 * T(B.cfc12, 1);
         } else {
@@ -305,17 +325,17 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         fracs = T(B.fracrefa, 1);
     } else if constexpr (BAND == 7) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colo3, R(at32(cst->rat_h2oo3, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colo3, R(at32(cst->rat_h2oo3, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.colh2o, c.colo3, R(CHI(1, 3) / CHI(3, 3)), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colo3, R(CHI(1, 3) / CHI(3, 3)), (Real)8);
-            V adj = lw_adjcol(c.colco2, c.coldry, R(CHI(2, jp + 1)), (Real)3, (Real)3, (Real)0.79);
+            Eta<V> e = lw_eta(c.colh2o, c.colo3, RAT(rat_h2oo3, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colo3, RAT(rat_h2oo3, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.colh2o, c.colo3, CHIQ(1, 3, 3), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colo3, CHIQ(1, 3, 3), (Real)8);
+            V adj = lw_adjcol(c.colco2, c.coldry, CHI(2, jp + 1), (Real)3, (Real)3, (Real)0.79);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
                    adj * lw_minor2(T, B.ka_mco2, 9, em, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
         } else {
-            V adj = lw_adjcol(c.colco2, c.coldry, R(CHI(2, jp + 1)), (Real)3, (Real)2, (Real)0.79);
+            V adj = lw_adjcol(c.colco2, c.coldry, CHI(2, jp + 1), (Real)3, (Real)2, (Real)0.79);
             taug = c.colo3 * lw_four(T, B.absb, ind0 + 1, ind1 + 1, c) + adj * lw_minor1(T, B.kb_mco2, c);
             // rrtmg_lw_taumol.f90:1646-1651
             if (g1 >= 6 && g1 <= 11) {
@@ -326,7 +346,7 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
             fracs = T(B.fracrefb, 1);
         }
     } else if constexpr (BAND == 8) {
-        V adj = lw_adjcol(c.colco2, c.coldry, R(CHI(2, jp + 1)), (Real)3, (Real)2, (Real)0.65);
+        V adj = lw_adjcol(c.colco2, c.coldry, CHI(2, jp + 1), (Real)3, (Real)2, (Real)0.65);
         if (lower) {
             taug = c.colh2o * lw_four(T, B.absa, ind0 + 1, ind1 + 1, c) + lw_self(T, B, c) + lw_for(T, B, c) +
                    adj * lw_minor1(T, B.ka_mco2, c) + c.colo3 * lw_minor1(T, B.ka_mo3, c) +
@@ -338,12 +358,12 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
             fracs = T(B.fracrefb, 1);
         }
     } else if constexpr (BAND == 9) {
-        V adj = lw_adjcol(c.coln2o, c.coldry, R(CHI(4, jp + 1)), (Real)1.5, (Real)0.5, (Real)0.65);
+        V adj = lw_adjcol(c.coln2o, c.coldry, CHI(4, jp + 1), (Real)1.5, (Real)0.5, (Real)0.65);
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.colh2o, c.colch4, R(CHI(1, 3) / CHI(6, 3)), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colch4, R(CHI(1, 9) / CHI(6, 9)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colch4, RAT(rat_h2och4, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, RAT(rat_h2och4, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.colh2o, c.colch4, CHIQ(1, 6, 3), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colch4, CHIQ(1, 6, 9), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
                    adj * lw_minor2(T, B.ka_mn2o, 9, em, c);
@@ -373,9 +393,9 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 12) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, R(at32(cst->rat_h2oco2, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colco2, R(CHI(1, 10) / CHI(2, 10)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colco2, RAT(rat_h2oco2, jp + 1), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colco2, CHIQ(1, 2, 10), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
@@ -385,11 +405,11 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 13) {
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.coln2o, R(at32(cst->rat_h2on2o, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.coln2o, R(at32(cst->rat_h2on2o, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 1) / CHI(4, 1)), (Real)8);
-            Eta<V> emco = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 3) / CHI(4, 3)), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.coln2o, R(CHI(1, 5) / CHI(4, 5)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.coln2o, RAT(rat_h2on2o, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.coln2o, RAT(rat_h2on2o, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.colh2o, c.coln2o, CHIQ(1, 4, 1), (Real)8);
+            Eta<V> emco = lw_eta(c.colh2o, c.coln2o, CHIQ(1, 4, 3), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.coln2o, CHIQ(1, 4, 5), (Real)8);
             V adj = lw_adjcol(c.colco2, c.coldry, (Real)3.55e-4, (Real)3, (Real)2, (Real)0.68);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
@@ -409,10 +429,10 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else if constexpr (BAND == 15) {
         if (lower) {
-            Eta<V> e = lw_eta(c.coln2o, c.colco2, R(at32(cst->rat_n2oco2, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.coln2o, c.colco2, R(at32(cst->rat_n2oco2, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> em = lw_eta(c.coln2o, c.colco2, R(CHI(4, 1) / CHI(2, 1)), (Real)8);
-            Eta<V> ep = lw_eta(c.coln2o, c.colco2, R(CHI(4, 1) / CHI(2, 1)), (Real)8);
+            Eta<V> e = lw_eta(c.coln2o, c.colco2, RAT(rat_n2oco2, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.coln2o, c.colco2, RAT(rat_n2oco2, jp + 1), (Real)8);
+            Eta<V> em = lw_eta(c.coln2o, c.colco2, CHIQ(4, 2, 1), (Real)8);
+            Eta<V> ep = lw_eta(c.coln2o, c.colco2, CHIQ(4, 2, 1), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c) +
                    c.colbrd * c.scaleminor * lw_minor2(T, B.ka_mn2, 9, em, c);
@@ -423,9 +443,9 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
         }
     } else {  // 16; aloft nspb(16) = 0, so ind0 = ind1 = 1 (rrtmg_lw_taumol.f90:3135-3136)
         if (lower) {
-            Eta<V> e = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp))), (Real)8);
-            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, R(at32(cst->rat_h2och4, (unsigned)(jp + 1))), (Real)8);
-            Eta<V> ep = lw_eta(c.colh2o, c.colch4, R(CHI(1, 6) / CHI(6, 6)), (Real)8);
+            Eta<V> e = lw_eta(c.colh2o, c.colch4, RAT(rat_h2och4, jp), (Real)8);
+            Eta<V> e1 = lw_eta(c.colh2o, c.colch4, RAT(rat_h2och4, jp + 1), (Real)8);
+            Eta<V> ep = lw_eta(c.colh2o, c.colch4, CHIQ(1, 6, 6), (Real)8);
             taug = lw_major3(T, B.absa, e, ind0 + e.js, c.fac00, c.fac10) +
                    lw_major3(T, B.absa, e1, ind1 + e1.js, c.fac01, c.fac11) + lw_self(T, B, c) + lw_for(T, B, c);
             fracs = lw_pfrac(T, B.fracrefa, ep);
@@ -436,6 +456,8 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
     }
 }
 #undef CHI
+#undef RAT
+#undef CHIQ
 
 // ng of each band (parrrtm.f90)
 __host__ __device__ constexpr int lw_ng(int band)

@@ -81,12 +81,35 @@ struct LwTables {
 };
 
 // host-side FP64 constant tables the LW prep and band kernels index by jp
+// chi_mls ratios at fixed reference levels that LW taumol uses as the eta
+// weights of species pairs (e.g. rrtmg_lw_taumol.f90:487-495): numerator
+// species, denominator species and level, 1-based as chi_mls(i, j)
+struct LwChiRatio {
+    int num, den, lev;
+};
+constexpr LwChiRatio LW_CHI_RATIOS[] = {{1, 2, 3}, {1, 2, 5}, {1, 2, 7},  {1, 2, 9},  {1, 2, 10}, {1, 2, 11},
+                                        {1, 2, 13}, {3, 2, 13}, {3, 2, 43}, {1, 3, 3}, {1, 4, 1},  {1, 4, 3},
+                                        {1, 4, 5}, {4, 2, 1},  {1, 6, 3},  {1, 6, 6}, {1, 6, 9}};
+constexpr int LW_NCHI_RATIO = (int)(sizeof(LW_CHI_RATIOS) / sizeof(LW_CHI_RATIOS[0]));
+constexpr int lw_chi_ratio_index(int num, int den, int lev)
+{
+    for (int k = 0; k < LW_NCHI_RATIO; ++k)
+        if (LW_CHI_RATIOS[k].num == num && LW_CHI_RATIOS[k].den == den && LW_CHI_RATIOS[k].lev == lev) return k;
+    return -1;
+}
+
 struct LwConst {
     double pref[59], preflog[59], tref[59];
     double chi_mls[7 * 59];
     // chi_mls ratios by reference level (rrtmg_lw_setcoef.f90:343-392)
     double rat_h2oco2[60], rat_h2oo3[60], rat_h2on2o[60], rat_h2och4[60], rat_n2oco2[60], rat_o3co2[60];
     double heatfac;
+    // the LW_CHI_RATIOS quotients, each one FP64 division
+    double chi_ratio[LW_NCHI_RATIO];
+    // FP32 copies (each FP64 value rounded once) for the FP32 band kernels
+    float chi_mls_f[7 * 59];
+    float rat_h2oco2_f[60], rat_h2oo3_f[60], rat_h2on2o_f[60], rat_h2och4_f[60], rat_n2oco2_f[60], rat_o3co2_f[60];
+    float chi_ratio_f[LW_NCHI_RATIO];
 };
 
 enum LwField {

@@ -323,6 +323,20 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     }
     c.rat_h2oco2[0] = c.rat_h2oo3[0] = c.rat_h2on2o[0] = c.rat_h2och4[0] = c.rat_n2oco2[0] = c.rat_o3co2[0] = 0.0;
     c.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);  // lwdatinit, rrtmg_lw_init.f90:279
+    for (int k = 0; k < LW_NCHI_RATIO; ++k) {
+        const LwChiRatio &r = LW_CHI_RATIOS[k];
+        c.chi_ratio[k] = CHI(r.num, r.lev) / CHI(r.den, r.lev);
+        c.chi_ratio_f[k] = (float)c.chi_ratio[k];
+    }
+    for (int i = 0; i < 7 * 59; ++i) c.chi_mls_f[i] = (float)c.chi_mls[i];
+    for (int jp = 0; jp < 60; ++jp) {
+        c.rat_h2oco2_f[jp] = (float)c.rat_h2oco2[jp];
+        c.rat_h2oo3_f[jp] = (float)c.rat_h2oo3[jp];
+        c.rat_h2on2o_f[jp] = (float)c.rat_h2on2o[jp];
+        c.rat_h2och4_f[jp] = (float)c.rat_h2och4[jp];
+        c.rat_n2oco2_f[jp] = (float)c.rat_n2oco2[jp];
+        c.rat_o3co2_f[jp] = (float)c.rat_o3co2[jp];
+    }
     std::vector<double> rwgt = make_rwgt(16, ngc, ngn, ngm, wt);
     h.named["lw/rwgt"] = rwgt;
     std::vector<double> &buf = h.buf;
