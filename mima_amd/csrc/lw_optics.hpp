@@ -193,10 +193,10 @@ __device__ __forceinline__ LwRT<Real> lw_rt_exact(Real od, bool derivs)
 
 // a LwConst table in the band kernel's precision: the FP64 values or their
 // FP32 copies (so the FP32 kernels gather floats and convert nothing)
-template <typename Real>
-__device__ __forceinline__ const Real *lw_cst(const double *d, const float *f)
+template <typename V>
+__device__ __forceinline__ auto lw_cst(const double *d, const float *f)
 {
-    if constexpr (sizeof(Real) == 4)
+    if constexpr (sizeof(V) == 4)
         return f;
     else
         return d;
@@ -210,9 +210,9 @@ constexpr int lw_chi_ratio_k()
 }
 // chi_mls(i, j); the rat_* table NAME at reference level e; the fixed-level
 // quotient chi_mls(a, j) / chi_mls(b, j), computed once at init
-#define CHI(i, j) at32(lw_cst<Real>(cst->chi_mls, cst->chi_mls_f), (unsigned)(((j)-1) * 7 + ((i)-1)))
-#define RAT(NAME, e) at32(lw_cst<Real>(cst->NAME, cst->NAME##_f), (unsigned)(e))
-#define CHIQ(a, b, j) lw_cst<Real>(cst->chi_ratio, cst->chi_ratio_f)[lw_chi_ratio_k<a, b, j>()]
+#define CHI(i, j) ((Real)at32(lw_cst<V>(cst->chi_mls, cst->chi_mls_f), (unsigned)(((j)-1) * 7 + ((i)-1))))
+#define RAT(NAME, e) ((Real)at32(lw_cst<V>(cst->NAME, cst->NAME##_f), (unsigned)(e)))
+#define CHIQ(a, b, j) ((Real)lw_cst<V>(cst->chi_ratio, cst->chi_ratio_f)[lw_chi_ratio_k<a, b, j>()])
 
 // Optical depth and Planck fraction of one (layer, g) of band BAND,
 // rrtmg_lw_taumol.f90:280-3147.
