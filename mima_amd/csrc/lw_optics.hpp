@@ -459,12 +459,4 @@ __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, con
 #undef RAT
 #undef CHIQ
 
-// ng of each band (parrrtm.f90)
-__host__ __device__ constexpr int lw_ng(int band)
-{
-    return band == 1 ? 10 : band == 2 ? 12 : band == 3 ? 16 : band == 4 ? 14 : band == 5 ? 16 : band == 6 ? 8
-         : band == 7 ? 12 : band == 8 ? 8 : band == 9 ? 12 : band == 10 ? 6 : band == 11 ? 8 : band == 12 ? 8
-         : band == 13 ? 4 : 2;
-}
-
 }  // namespace rrtmg

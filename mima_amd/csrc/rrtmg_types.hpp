@@ -81,6 +81,22 @@ struct LwTables {
 };
 
 // host-side FP64 constant tables the LW prep and band kernels index by jp
+// reduced g-points of each band (parrrtm.f90, parrrsw.f90:53-66): the band
+// kernels are compiled for them, and the table loader checks the table file
+// against them
+constexpr int lw_ng(int band)
+{
+    return band == 1 ? 10 : band == 2 ? 12 : band == 3 ? 16 : band == 4 ? 14 : band == 5 ? 16 : band == 6 ? 8
+         : band == 7 ? 12 : band == 8 ? 8 : band == 9 ? 12 : band == 10 ? 6 : band == 11 ? 8 : band == 12 ? 8
+         : band == 13 ? 4 : 2;
+}
+constexpr int sw_ng(int jb)
+{
+    return jb == 16 ? 6 : jb == 17 ? 12 : jb == 18 ? 8 : jb == 19 ? 8 : jb == 20 ? 10 : jb == 21 ? 10
+         : jb == 22 ? 2 : jb == 23 ? 10 : jb == 24 ? 8 : jb == 25 ? 6 : jb == 26 ? 6 : jb == 27 ? 8
+         : jb == 28 ? 6 : 12;
+}
+
 // chi_mls ratios at fixed reference levels that LW taumol uses as the eta
 // weights of species pairs (e.g. rrtmg_lw_taumol.f90:487-495): numerator
 // species, denominator species and level, 1-based as chi_mls(i, j)

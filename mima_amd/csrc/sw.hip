@@ -134,7 +134,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
                              Real *__restrict__ s_prup, Real *__restrict__ s_prupd, Real *s_red)
 {
     const int ncol = in.ncol, nlay = in.nlay;
-    const int ng = B.ng;
+    constexpr int ng = sw_ng(JB);  // B.ng, checked at init
     const int cl = lane / ng, g = lane - cl * ng;
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
@@ -225,7 +225,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
 }
 
 template <typename Real>
-__global__ __launch_bounds__(64) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(sizeof(Real) == 4 ? 6 : 1))) void sw_band_kernel(ColumnInputs in, SwWork w, const SwBand *__restrict__ bands,
                                                      int nwaves, int blk0, int nblocks, int exp_off,
                                                      const Real *__restrict__ tab, const WaveUnit *__restrict__ units)
 {

@@ -229,6 +229,10 @@ bool build_sw(const std::string &path, double cpdair, SwHost &h, std::string &er
         int jb = b + 16;
         SwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);  // offsets -1
+        if (ngc[b] != sw_ng(jb)) {
+            err = "SW table file's g-point counts differ from the compiled band kernels";
+            return false;
+        }
         d.ng = ngc[b];
         ngv[b] = d.ng;
         d.ngs = b == 0 ? 0 : ngs[b - 1];
@@ -351,6 +355,10 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     for (int b = 0; b < 16; ++b) {
         LwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);
+        if (ngc[b] != lw_ng(b + 1)) {
+            err = "LW table file's g-point counts differ from the compiled band kernels";
+            return false;
+        }
         d.ng = ngc[b];
         ngv[b] = d.ng;
         d.ngs = b == 0 ? 0 : ngs[b - 1];
