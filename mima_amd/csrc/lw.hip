@@ -248,7 +248,7 @@ __device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWor
     for (int lev = nlay; lev >= 1; --lev) {
         lw_load_layer(c, coef, w.ipk, lev - 1, col, ncol, nlay);
         Real taug, plfrac;
-        lw_gas<BAND, Real>(B, T, cst, c, lev <= laytrop, taug, plfrac);
+        lw_gas<BAND, Real, Real, true>(B, T, cst, c, lev <= laytrop, taug, plfrac);
         if (tauaer) taug = taug + (Real)tauaer[((size_t)(BAND - 1) * nlay + (lev - 1)) * ncol + col];
         Real plev_dn = planklev(lev - 1);
         Real blay = at32(plk, (unsigned)(c.indlay - 1)) +

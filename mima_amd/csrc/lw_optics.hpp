@@ -216,18 +216,21 @@ constexpr int lw_chi_ratio_k()
 
 // Optical depth and Planck fraction of one (layer, g) of band BAND,
 // rrtmg_lw_taumol.f90:280-3147.
-template <int BAND, typename Real, typename V = Real>
+// CONST_NSP: the species-parameter counts as compile-time constants (the
+// forward band kernel; the backward's kernels keep the loaded values, as
+// constants they ran 2% slower).
+template <int BAND, typename Real, typename V = Real, bool CONST_NSP = false>
 __device__ __forceinline__ void lw_gas(const LwBand &B, const LTab<Real> &T, const LwConst *__restrict__ cst,
                                        const LwLayer<V> &c, bool lower, V &taug, V &fracs)
 {
     const int g1 = T.g + 1;
     int ind0, ind1;
     if (lower) {
-        ind0 = ((c.jp - 1) * 5 + (c.jt - 1)) * B.nspa;
-        ind1 = (c.jp * 5 + (c.jt1 - 1)) * B.nspa;
+        ind0 = ((c.jp - 1) * 5 + (c.jt - 1)) * (CONST_NSP ? LW_NSPA[BAND - 1] : B.nspa);
+        ind1 = (c.jp * 5 + (c.jt1 - 1)) * (CONST_NSP ? LW_NSPA[BAND - 1] : B.nspa);
     } else {
-        ind0 = ((c.jp - 13) * 5 + (c.jt - 1)) * B.nspb;
-        ind1 = ((c.jp - 12) * 5 + (c.jt1 - 1)) * B.nspb;
+        ind0 = ((c.jp - 13) * 5 + (c.jt - 1)) * (CONST_NSP ? LW_NSPB[BAND - 1] : B.nspb);
+        ind1 = ((c.jp - 12) * 5 + (c.jt1 - 1)) * (CONST_NSP ? LW_NSPB[BAND - 1] : B.nspb);
     }
     const int jp = c.jp;
     if constexpr (BAND == 1) {

@@ -97,6 +97,14 @@ constexpr int sw_ng(int jb)
          : jb == 28 ? 6 : 12;
 }
 
+// species-parameter counts of the k-tables per band, lower/upper atmosphere
+// (rrtmg_lw_init.f90:208-209, rrtmg_sw_init.f90:183-184); checked against
+// the table file like the g-point counts
+constexpr int LW_NSPA[16] = {1, 1, 9, 9, 9, 1, 9, 1, 9, 1, 1, 9, 9, 1, 9, 9};
+constexpr int LW_NSPB[16] = {1, 1, 5, 5, 5, 0, 1, 1, 1, 1, 1, 0, 0, 1, 0, 0};
+constexpr int SW_NSPA[14] = {9, 9, 9, 9, 1, 9, 9, 1, 9, 1, 0, 1, 9, 1};
+constexpr int SW_NSPB[14] = {1, 5, 1, 1, 1, 5, 1, 0, 1, 0, 0, 1, 5, 1};
+
 // chi_mls ratios at fixed reference levels that LW taumol uses as the eta
 // weights of species pairs (e.g. rrtmg_lw_taumol.f90:487-495): numerator
 // species, denominator species and level, 1-based as chi_mls(i, j)

@@ -135,6 +135,10 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
 {
     const int ncol = in.ncol, nlay = in.nlay;
     constexpr int ng = sw_ng(JB);  // B.ng, checked at init
+    // the band record with its species-parameter counts as constants
+    SwBand Bc = B;
+    Bc.nspa = SW_NSPA[JB - 16];
+    Bc.nspb = SW_NSPB[JB - 16];
     const int cl = lane / ng, g = lane - cl * ng;
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
@@ -156,7 +160,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
         albd = (Real)(nir ? in.aldif[col] : in.asdif[col]);
         int lsf = w.laysolfr[(size_t)(JB - 16) * ncol + col];
         sw_load_layer(c, coef, w.ipk, lsf - 1, col, ncol, nlay);
-        double sfz = sw_sfluxzen(JB, B, T, c, lsf <= laytrop);
+        double sfz = sw_sfluxzen(JB, Bc, T, c, lsf <= laytrop);
         zincflx = (Real)(in.adjflux * sfz * (double)prmu0);
     }
 
@@ -170,7 +174,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
         if (active) {
             sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
             Real taug, taur;
-            sw_gas(JB, B, T, c, lay <= laytrop, taug, taur);
+            sw_gas(JB, Bc, T, c, lay <= laytrop, taug, taur);
             sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
         }
         Real zreflect = fdiv((Real)1, (Real)1 - prupd_n * rd);
@@ -195,7 +199,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
             if (active) {
                 sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
                 Real taug, taur;
-                sw_gas(JB, B, T, c, lay <= laytrop, taug, taur);
+                sw_gas(JB, Bc, T, c, lay <= laytrop, taug, taur);
                 sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
             }
             Real zreflect = fdiv((Real)1, (Real)1 - rd * prdnd);

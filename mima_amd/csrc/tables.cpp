@@ -229,8 +229,8 @@ bool build_sw(const std::string &path, double cpdair, SwHost &h, std::string &er
         int jb = b + 16;
         SwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);  // offsets -1
-        if (ngc[b] != sw_ng(jb)) {
-            err = "SW table file's g-point counts differ from the compiled band kernels";
+        if (ngc[b] != sw_ng(jb) || nspa[b] != SW_NSPA[b] || nspb[b] != SW_NSPB[b]) {
+            err = "SW table file's g-point or species-parameter counts differ from the compiled band kernels";
             return false;
         }
         d.ng = ngc[b];
@@ -355,8 +355,8 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     for (int b = 0; b < 16; ++b) {
         LwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);
-        if (ngc[b] != lw_ng(b + 1)) {
-            err = "LW table file's g-point counts differ from the compiled band kernels";
+        if (ngc[b] != lw_ng(b + 1) || nspa[b] != LW_NSPA[b] || nspb[b] != LW_NSPB[b]) {
+            err = "LW table file's g-point or species-parameter counts differ from the compiled band kernels";
             return false;
         }
         d.ng = ngc[b];
