@@ -1163,6 +1163,7 @@ int rrtmg_hip_debug_host_tables(int lw, const char *table_dir, double cpdair)
     std::lock_guard<std::mutex> lk(g_mu);
     std::string dir = table_dir ? table_dir : default_table_dir();
     std::string err;
+    (lw ? g_host_lw_ok : g_host_sw_ok) = false;  // a failed load leaves no half-built tables readable
     bool ok = lw ? build_lw(dir + "/rrtmg_lw.tbl", cpdair, g_host_lw, err)
                  : build_sw(dir + "/rrtmg_sw.tbl", cpdair, g_host_sw, err);
     if (!ok) return fail(RRTMG_HIP_ENOENT, err);

@@ -207,10 +207,16 @@ bool build_sw(const std::string &path, double cpdair, SwHost &h, std::string &er
     std::vector<int> ngc = t.ints("sw/ngc"), ngs = t.ints("sw/ngs"), ngn = t.ints("sw/ngn"), ngm = t.ints("sw/ngm");
     std::vector<int> nspa = t.ints("sw/nspa"), nspb = t.ints("sw/nspb");
     std::vector<double> wt = t.doubles("sw/wt");
-    if (ngc.size() != 14 || ngn.size() != 112 || ngm.size() != 224 || wt.size() != 16 || nspa.size() != 14) {
+    if (ngc.size() != 14 || ngn.size() != 112 || ngm.size() != 224 || wt.size() != 16 || nspa.size() != 14 ||
+        nspb.size() != 14) {
         err = "SW table file lacks the g-point bookkeeping";
         return false;
     }
+    for (int b = 0; b < 14; ++b)
+        if (ngc[b] != sw_ng(b + 16) || nspa[b] != SW_NSPA[b] || nspb[b] != SW_NSPB[b]) {
+            err = "SW table file's g-point or species-parameter counts differ from the compiled band kernels";
+            return false;
+        }
     h.pref = t.doubles("sw/pref");
     h.preflog = t.doubles("sw/preflog");
     h.tref = t.doubles("sw/tref");
@@ -229,10 +235,6 @@ bool build_sw(const std::string &path, double cpdair, SwHost &h, std::string &er
         int jb = b + 16;
         SwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);  // offsets -1
-        if (ngc[b] != sw_ng(jb) || nspa[b] != SW_NSPA[b] || nspb[b] != SW_NSPB[b]) {
-            err = "SW table file's g-point or species-parameter counts differ from the compiled band kernels";
-            return false;
-        }
         d.ng = ngc[b];
         ngv[b] = d.ng;
         d.ngs = b == 0 ? 0 : ngs[b - 1];
@@ -307,10 +309,16 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     std::vector<double> pref = t.doubles("lw/pref"), preflog = t.doubles("lw/preflog"), tref = t.doubles("lw/tref");
     std::vector<double> chi = t.doubles("lw/chi_mls"), totplnk = t.doubles("lw/totplnk");
     if (ngc.size() != 16 || ngn.size() != 140 || ngm.size() != 256 || wt.size() != 16 || a0.size() != 16 ||
-        chi.size() != 7 * 59 || totplnk.size() != 181 * 16 || preflog.size() != 59) {
+        chi.size() != 7 * 59 || totplnk.size() != 181 * 16 || preflog.size() != 59 || nspa.size() != 16 ||
+        nspb.size() != 16) {
         err = "LW table file lacks bookkeeping, reference atmosphere or Planck data";
         return false;
     }
+    for (int b = 0; b < 16; ++b)
+        if (ngc[b] != lw_ng(b + 1) || nspa[b] != LW_NSPA[b] || nspb[b] != LW_NSPB[b]) {
+            err = "LW table file's g-point or species-parameter counts differ from the compiled band kernels";
+            return false;
+        }
     LwConst &c = h.cst;
     std::memcpy(c.pref, pref.data(), sizeof c.pref);
     std::memcpy(c.preflog, preflog.data(), sizeof c.preflog);
@@ -355,10 +363,6 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     for (int b = 0; b < 16; ++b) {
         LwBand &d = h.tab.band[b];
         std::memset(&d, 0xff, sizeof d);
-        if (ngc[b] != lw_ng(b + 1) || nspa[b] != LW_NSPA[b] || nspb[b] != LW_NSPB[b]) {
-            err = "LW table file's g-point or species-parameter counts differ from the compiled band kernels";
-            return false;
-        }
         d.ng = ngc[b];
         ngv[b] = d.ng;
         d.ngs = b == 0 ? 0 : ngs[b - 1];

@@ -283,6 +283,34 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, n), n
 
 
+@pytest.mark.parametrize("side,key,band", [("sw", "sw/ngc", 3), ("sw", "sw/nspa", 0), ("sw", "sw/nspb", 1),
+                                           ("lw", "lw/ngc", 5), ("lw", "lw/nspa", 2), ("lw", "lw/nspb", 14)])
+def test_table_loader_rejects_band_layout_mismatch(tmp_path, side, key, band):
+    """The band kernels are compiled for the reference's per-band g-point and
+    species-parameter counts (parrrsw.f90, rrtmg_*_init.f90); a table file
+    with other counts is refused at load with ENOENT and a message, before any
+    table is reduced with them."""
+    import importlib.util
+    from mima_amd import rrtmg
+    spec = importlib.util.spec_from_file_location(
+        "extract_tables", os.path.join(ROOT, "mima_amd", "tables", "extract_tables.py"))
+    et = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(et)
+    L = rrtmg.lib()
+    fname = "rrtmg_%s.tbl" % side
+    arrays, bounds, flags = et.read_table(os.path.join(ROOT, "mima_amd", "tables", fname))
+    entries = {n: (np.array(a), bounds[n]) for n, a in arrays.items()}
+    entries[key][0][band] += 1
+    et.write_table(str(tmp_path / fname), entries, flags)
+    lw = 1 if side == "lw" else 0
+    try:
+        rc = L.rrtmg_hip_debug_host_tables(lw, str(tmp_path).encode(), ctypes.c_double(rrtmg.CPDAIR))
+        assert rc == -3
+        assert b"counts differ from the compiled band kernels" in L.rrtmg_hip_last_error()
+    finally:
+        assert L.rrtmg_hip_debug_host_tables(lw, None, ctypes.c_double(rrtmg.CPDAIR)) == 0
+
+
 def test_library_table_reduction_matches_oracle():
     from mima_amd import rrtmg
     L = rrtmg.lib()
