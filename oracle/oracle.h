@@ -87,6 +87,14 @@ int oracle_lw_taumol(int nlay, const double *play, const double *plev,
                      const double *cfc11vmr, const double *cfc12vmr,
                      const double *cfc22vmr, const double *ccl4vmr,
                      double *taug, double *fracs);
+/* inatm of one column: coldry (nlay), molecule columns wkl (7, nlay) in
+ * molecules/cm2 and precipitable water pwvcm (cm). */
+int oracle_lw_inatm(int nlay, const double *play, const double *plev,
+                    const double *tlay, const double *tlev, double tsfc,
+                    const double *h2ovmr, const double *o3vmr,
+                    const double *co2vmr, const double *ch4vmr,
+                    const double *n2ovmr, const double *o2vmr,
+                    double *coldry, double *wkl, double *pwvcm);
 /* reftra_sw for nlay layers (rrtmg_sw_reftra.f90:29-305), all layers active. */
 void oracle_sw_reftra(int nlay, const double *gg, double prmuz, const double *tau,
                       const double *w, double *pref, double *prefd,

@@ -1177,3 +1177,26 @@ int oracle_lw_taumol(int nlay, const double *play, const double *plev, const dou
     free(c);
     return lt;
 }
+
+/* inatm of one column (rrtmg_lw_rad.nomcica.f90:729-852): dry-air column
+ * coldry (nlay) and the molecule columns wkl(1..7) (7 x nlay, molecules/cm2),
+ * in molec/cm2, and the precipitable water pwvcm.  For the hydrostatic and
+ * unit checks of the CPU tests. */
+int oracle_lw_inatm(int nlay, const double *play, const double *plev, const double *tlay,
+                    const double *tlev, double tsfc, const double *h2ovmr, const double *o3vmr,
+                    const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                    const double *o2vmr, double *coldry, double *wkl, double *pwvcm)
+{
+    if (!L.ready || nlay > MAXLAY) return -1;
+    lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+    if (!c) return -3;
+    one_lw(c, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, NULL, NULL,
+           NULL, NULL);
+    for (int k = 1; k <= nlay; ++k) {
+        coldry[k - 1] = c->coldry[k];
+        for (int m = 1; m <= 7; ++m) wkl[(m - 1) * nlay + (k - 1)] = c->wkl[m][k];
+    }
+    *pwvcm = c->pwvcm;
+    free(c);
+    return 0;
+}

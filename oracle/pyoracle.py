@@ -163,3 +163,37 @@ def reduced(name):
 
 def heatfac(lw):
     return lib().oracle_heatfac(1 if lw else 0)
+
+
+def lw_inatm(c, i):
+    """inatm of column i: coldry (nlay), wkl (7, nlay) molecules/cm2, pwvcm (cm)."""
+    L = lib()
+    n = c["nlay"]
+    coldry = np.zeros(n)
+    wkl = np.zeros((7, n))
+    pw = ctypes.c_double(0.0)
+    ins = [_col(c, i, k) for k in ("play", "plev", "tlay", "tlev")]
+    gas = [_col(c, i, k) for k in ("h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr")]
+    rc = L.oracle_lw_inatm(n, *[_p(a) for a in ins], ctypes.c_double(c["tsfc"][i]), *[_p(a) for a in gas],
+                           _p(coldry), _p(wkl), ctypes.byref(pw))
+    if rc != 0:
+        raise RuntimeError("oracle_lw_inatm returned %d" % rc)
+    return dict(coldry=coldry, wkl=wkl, pwvcm=pw.value)
+
+
+class lw_tables:
+    """Context manager: the LW oracle on another RRTMGTB1 table file, the
+    package's LW tables again on exit."""
+
+    def __init__(self, path):
+        self.path = str(path)
+
+    def __enter__(self):
+        if lib().oracle_lw_init(self.path.encode(), ctypes.c_double(CPDAIR)) != 0:
+            raise RuntimeError("oracle_lw_init(%s) failed" % self.path)
+        return self
+
+    def __exit__(self, *exc):
+        if lib().oracle_lw_init(LW_TABLE.encode(), ctypes.c_double(CPDAIR)) != 0:
+            raise RuntimeError("oracle_lw_init failed")
+        return False
