@@ -16,6 +16,8 @@
 #include <cstring>
 #include <fstream>
 
+#include "table_shapes.hpp"
+
 namespace rrtmg {
 
 bool TableFile::load(const std::string &path, std::string &err)
@@ -80,6 +82,63 @@ std::vector<int> TableFile::ints(const std::string &name) const
     std::vector<int> v(e->nbytes / 4);
     std::memcpy(v.data(), raw_.data() + e->off, e->nbytes);
     return v;
+}
+
+// Checks that a table file holds every band array with the shape the band
+// kernels index it with (table_shapes.hpp) and reduction bookkeeping that
+// make_rwgt and reduce can use: ngs the running sum of ngc, each band's ngn
+// counts summing to its 16 original g-points, and ngm numbering every
+// original g-point by the reduced point those counts assign it (the
+// reference's lwcmbdat/swcmbdat literals satisfy all three,
+// rrtmg_lw_init.f90:284-363, rrtmg_sw_init.f90:263-367).
+static bool check_layout(const TableFile &t, const TableShape *shapes, int nshapes, int nb,
+                         const std::vector<int> &ngc, const std::vector<int> &ngs, const std::vector<int> &ngn,
+                         const std::vector<int> &ngm, const char *side, std::string &err)
+{
+    for (int i = 0; i < nshapes; ++i) {
+        const TableShape &sh = shapes[i];
+        const TableFile::Entry *e = t.find(sh.name);
+        bool ok = e && e->dtype == 0 && (int)e->ndim == sh.ndim;
+        uint64_t n = 1;
+        for (int d = 0; ok && d < 4; ++d) {
+            ok = e->ext[d] == sh.ext[d];
+            n *= (uint64_t)sh.ext[d];
+        }
+        if (!ok || e->nbytes != 8 * n) {
+            err = std::string("table array ") + sh.name + " is missing or does not have the shape the band kernels use";
+            return false;
+        }
+    }
+    if ((int)ngs.size() != nb || (int)ngm.size() != 16 * nb) {
+        err = std::string(side) + " table file: ngs/ngm have the wrong length";
+        return false;
+    }
+    int first = 0;
+    for (int b = 0; b < nb; ++b) {
+        if (ngs[b] != first + ngc[b]) {
+            err = std::string(side) + " table file: ngs is not the running sum of ngc";
+            return false;
+        }
+        int g = 0;
+        for (int r = 0; r < ngc[b]; ++r) {
+            const int k = ngn[first + r];
+            if (k < 1 || g + k > 16) {
+                err = std::string(side) + " table file: a band's ngn counts do not cover its 16 g-points";
+                return false;
+            }
+            for (int j = 0; j < k; ++j, ++g)
+                if (ngm[16 * b + g] != r + 1) {
+                    err = std::string(side) + " table file: ngm disagrees with ngn";
+                    return false;
+                }
+        }
+        if (g != 16) {
+            err = std::string(side) + " table file: a band's ngn counts do not cover its 16 g-points";
+            return false;
+        }
+        first += ngc[b];
+    }
+    return true;
 }
 
 // rwgt for nb bands of 16 original g-points
@@ -217,10 +276,11 @@ bool build_sw(const std::string &path, double cpdair, SwHost &h, std::string &er
             err = "SW table file's g-point or species-parameter counts differ from the compiled band kernels";
             return false;
         }
+    if (!check_layout(t, SW_SHAPES, SW_NSHAPES, 14, ngc, ngs, ngn, ngm, "SW", err)) return false;
     h.pref = t.doubles("sw/pref");
     h.preflog = t.doubles("sw/preflog");
     h.tref = t.doubles("sw/tref");
-    if (h.preflog.size() != 59 || h.tref.size() != 59) {
+    if (h.pref.size() != 59 || h.preflog.size() != 59 || h.tref.size() != 59) {
         err = "SW table file lacks the reference atmosphere";
         return false;
     }
@@ -309,8 +369,9 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
     std::vector<double> pref = t.doubles("lw/pref"), preflog = t.doubles("lw/preflog"), tref = t.doubles("lw/tref");
     std::vector<double> chi = t.doubles("lw/chi_mls"), totplnk = t.doubles("lw/totplnk");
     if (ngc.size() != 16 || ngn.size() != 140 || ngm.size() != 256 || wt.size() != 16 || a0.size() != 16 ||
-        chi.size() != 7 * 59 || totplnk.size() != 181 * 16 || preflog.size() != 59 || nspa.size() != 16 ||
-        nspb.size() != 16) {
+        a1.size() != 16 || a2.size() != 16 || delwave.size() != 16 || chi.size() != 7 * 59 ||
+        totplnk.size() != 181 * 16 || pref.size() != 59 || preflog.size() != 59 || tref.size() != 59 ||
+        nspa.size() != 16 || nspb.size() != 16) {
         err = "LW table file lacks bookkeeping, reference atmosphere or Planck data";
         return false;
     }
@@ -319,6 +380,7 @@ bool build_lw(const std::string &path, double cpdair, LwHost &h, std::string &er
             err = "LW table file's g-point or species-parameter counts differ from the compiled band kernels";
             return false;
         }
+    if (!check_layout(t, LW_SHAPES, LW_NSHAPES, 16, ngc, ngs, ngn, ngm, "LW", err)) return false;
     LwConst &c = h.cst;
     std::memcpy(c.pref, pref.data(), sizeof c.pref);
     std::memcpy(c.preflog, preflog.data(), sizeof c.preflog);

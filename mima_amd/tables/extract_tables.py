@@ -455,12 +455,44 @@ def read_table(path):
     return arrays, bounds, flags
 
 
+def write_shapes_header(path, table_dir=HERE):
+    """C++ header listing the shape of every band array of the two table
+    files (the reference's module declarations, as the files hold them): the
+    loader refuses a table file whose arrays differ, since the band kernels
+    index them with those extents."""
+    lines = ["// Generated by mima_amd/tables/extract_tables.py --shapes-header from the",
+             "// committed table files: name, rank and extents (first index fastest) of",
+             "// every band array, as declared by the reference's rrlw_kgNN / rrsw_kgNN",
+             "// modules.  The table loader refuses a file whose arrays differ.",
+             "#pragma once", "", "namespace rrtmg {", "",
+             "struct TableShape {", "    const char *name;", "    int ndim, ext[4];", "};", ""]
+    for side in ("sw", "lw"):
+        arrays, _, _ = read_table(os.path.join(table_dir, "rrtmg_%s.tbl" % side))
+        names = sorted(n for n in arrays if n.count("/") == 2)
+        lines.append("constexpr TableShape %s_SHAPES[] = {" % side.upper())
+        for n in names:
+            ext = list(arrays[n].shape) + [1] * (4 - arrays[n].ndim)
+            lines.append('    {"%s", %d, {%s}},' % (n, arrays[n].ndim, ", ".join(str(e) for e in ext)))
+        lines.append("};")
+        lines.append("constexpr int %s_NSHAPES = (int)(sizeof(%s_SHAPES) / sizeof(%s_SHAPES[0]));"
+                     % ((side.upper(),) * 3))
+        lines.append("")
+    lines.append("}  // namespace rrtmg")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--out", default=HERE)
     ap.add_argument("--lw-seed", type=int, default=20261202)
+    ap.add_argument("--shapes-header", default=None,
+                    help="only write the band-array shape header (C++) of the committed tables to this path")
     args = ap.parse_args()
+    if args.shapes_header:
+        write_shapes_header(args.shapes_header)
+        return 0
 
     sw, prov_sw = extract_sw(args.ref)
     write_table(os.path.join(args.out, "rrtmg_sw.tbl"), sw, 0)
@@ -484,6 +516,7 @@ def main():
     }
     with open(os.path.join(args.out, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
+    write_shapes_header(os.path.join(HERE, "..", "csrc", "table_shapes.hpp"), args.out)
     print("wrote", len(sw), "SW entries and", len(lw), "LW entries")
 
 

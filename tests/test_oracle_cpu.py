@@ -311,6 +311,71 @@ def test_table_loader_rejects_band_layout_mismatch(tmp_path, side, key, band):
         assert L.rrtmg_hip_debug_host_tables(lw, None, ctypes.c_double(rrtmg.CPDAIR)) == 0
 
 
+def _bad_table(tmp_path, side, edit):
+    """Write the package's table file of `side` with edit(entries) applied to
+    tmp_path; returns the loader's return code and message."""
+    import importlib.util
+    from mima_amd import rrtmg
+    spec = importlib.util.spec_from_file_location(
+        "extract_tables", os.path.join(ROOT, "mima_amd", "tables", "extract_tables.py"))
+    et = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(et)
+    L = rrtmg.lib()
+    fname = "rrtmg_%s.tbl" % side
+    arrays, bounds, flags = et.read_table(os.path.join(ROOT, "mima_amd", "tables", fname))
+    entries = {n: (np.array(a), bounds[n]) for n, a in arrays.items()}
+    edit(entries)
+    et.write_table(str(tmp_path / fname), entries, flags)
+    lw = 1 if side == "lw" else 0
+    try:
+        rc = L.rrtmg_hip_debug_host_tables(lw, str(tmp_path).encode(), ctypes.c_double(rrtmg.CPDAIR))
+        msg = L.rrtmg_hip_last_error().decode()
+    finally:
+        assert L.rrtmg_hip_debug_host_tables(lw, None, ctypes.c_double(rrtmg.CPDAIR)) == 0
+    return rc, msg
+
+
+@pytest.mark.parametrize("side", ["sw", "lw"])
+def test_table_loader_rejects_ngn_not_covering_the_band(tmp_path, side):
+    """make_rwgt walks wt[] by the ngn counts: a band whose counts do not sum
+    to 16 would read past its 16 weights; the loader refuses it first."""
+    def edit(e):  # the first band's last reduced point takes one g-point more
+        e[side + "/ngn"][0][int(e[side + "/ngc"][0][0]) - 1] += 1
+    rc, msg = _bad_table(tmp_path, side, edit)
+    assert rc == -3 and "ngn counts do not cover" in msg
+
+
+@pytest.mark.parametrize("side", ["sw", "lw"])
+def test_table_loader_rejects_ngm_out_of_range(tmp_path, side):
+    """ngm indexes the per-band weight sums: a value outside 1..ngc (or one
+    that disagrees with ngn) is refused."""
+    def edit(e):
+        e[side + "/ngm"][0][15] = 17
+    rc, msg = _bad_table(tmp_path, side, edit)
+    assert rc == -3 and "ngm disagrees with ngn" in msg
+
+
+@pytest.mark.parametrize("side,key", [("sw", "sw/b17/kao"), ("lw", "lw/b03/kbo"), ("lw", "lw/b05/ccl4o")])
+def test_table_loader_rejects_band_array_shape(tmp_path, side, key):
+    """The band kernels index every k-table with the reference's declared
+    extents; an array of another shape (here one row short) is refused
+    instead of being read past its end."""
+    def edit(e):
+        a, lb = e[key]
+        e[key] = (np.asfortranarray(a[:-1] if a.ndim == 1 else a[:-1, ...]), lb)
+    rc, msg = _bad_table(tmp_path, side, edit)
+    assert rc == -3 and key in msg and "shape" in msg
+
+
+@pytest.mark.parametrize("side,key", [("lw", "lw/delwave"), ("lw", "lw/tref"), ("sw", "sw/pref"),
+                                      ("lw", "lw/b07/kao_mco2")])
+def test_table_loader_rejects_missing_arrays(tmp_path, side, key):
+    def edit(e):
+        del e[key]
+    rc, msg = _bad_table(tmp_path, side, edit)
+    assert rc == -3 and msg
+
+
 def test_library_table_reduction_matches_oracle():
     from mima_amd import rrtmg
     L = rrtmg.lib()
