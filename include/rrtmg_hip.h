@@ -42,7 +42,7 @@ extern "C" {
                                       * inputs, for finite-difference checks */
 
 #define RRTMG_HIP_OK 0
-#define RRTMG_HIP_EINVAL (-22)   /* bad size, NaN/Inf input, non-monotone plev, coszen < 0 */
+#define RRTMG_HIP_EINVAL (-22)   /* bad size, null array, NaN/Inf input, non-monotone plev */
 #define RRTMG_HIP_ENOTSUP (-95)  /* outside the clear-sky contract */
 #define RRTMG_HIP_ENOINIT (-2)   /* called before the matching *_init */
 #define RRTMG_HIP_ENOMEM (-12)   /* device allocation failed */
@@ -70,7 +70,9 @@ int rrtmg_hip_set_precision(int mode);
 
 /* Replaces rrtmg_sw (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.nomcica.f90:78-88).
  * Same leading arguments; the cloud and aerosol arrays of the reference are
- * not read because icld = iaer = 0 (the reference ignores them then too). */
+ * not read because icld = iaer = 0 (the reference ignores them then too).
+ * Columns with coszen < 1e-10 (negative values included) are night columns
+ * with all outputs zero, as in the reference (:502-510). */
 int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer,
                  const double *play, const double *plev, const double *tlay,
                  const double *tlev, const double *tsfc,
@@ -218,8 +220,7 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
  * out_f32 = v (+= with accumulate); out_bf16 = bf16(v); outputs row pitch ldc,
  * each may be NULL.  With both operands transposed and few 64 x 64 output
  * tiles (the weight gradient dY^T X) K is split over a library-owned
- * workspace of up to 256 x 64 x 64 floats per device; like the other
- * workspaces, calls that overlap on one device race on it. */
+ * workspace of up to 256 x 64 x 64 floats per device and stream. */
 int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA,
                         const void *B, int ldb, int transB, const float *bias, int relu,
                         float *out_pre, float *out_f32, void *out_bf16, int ldc,
@@ -251,6 +252,17 @@ int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double 
 int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw,
                           const float *mean, const float *istd, int ldt, float *t, void *stream);
 
+/* Input checks of device-pointer calls (no reference counterpart; the
+ * reference checks nothing).  Host-pointer calls always check that every
+ * input is finite and that plev decreases strictly upward with a positive
+ * top, and return RRTMG_HIP_EINVAL naming the array otherwise.  With checks on
+ * (on != 0), device-pointer calls of rrtmg_hip_sw, rrtmg_hip_lw and the
+ * backward entry points make the same checks on the device before any
+ * kernel of the call runs; the call then waits for them, so it is no longer
+ * asynchronous and cannot be captured in a graph.  Off by default; returns
+ * the previous setting. */
+int rrtmg_hip_set_checks(int on);
+
 /* Kernel timing (no reference counterpart).  With timing on, each call
  * records HIP events on its stream around its three kernels (prep, band,
  * finish); rrtmg_hip_last_kernel_ms waits for the last call of the given kind
@@ -258,6 +270,12 @@ int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double 
  * 3 rrtmg_hip_lw_vjp) and writes the three durations in milliseconds. */
 int rrtmg_hip_set_timing(int on);
 int rrtmg_hip_last_kernel_ms(int kind, float *ms);
+
+/* Workspaces (all entry points): the library keeps its device workspaces per
+ * device and per stream, allocated on first use and grown on demand; calls
+ * on one stream reuse them in stream order, calls on different streams use
+ * different workspaces and may overlap.  Growing a workspace synchronises
+ * the device. */
 
 /* Message for the last failing call on this host thread ("" if none). */
 const char *rrtmg_hip_last_error(void);

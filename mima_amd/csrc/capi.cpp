@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -69,12 +70,20 @@ struct SideTables {
     double cpdair = 0.0;
 };
 
+// The workspaces of the calls on one stream.  Calls on one stream run in
+// stream order and may share buffers; calls on different streams get
+// different buffers, so they can overlap on the device.
+struct StreamWork {
+    Buffer sw_work, lw_work, sw_stage, lw_stage;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws, flag;
+};
+
 struct DeviceState {
     SwHost sw;
     LwHost lw;
     SideTables swd, lwd;
-    Buffer sw_work, lw_work, sw_stage, lw_stage, zeros;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws;
+    std::map<hipStream_t, StreamWork> ws;
+    StreamWork &work(hipStream_t st) { return ws[st]; }
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
     SwWork sw_w;
     LwWork lw_w;
@@ -98,6 +107,7 @@ struct DeviceState {
 
 std::mutex g_mu;
 bool g_timing = false;
+bool g_checks = false;
 std::map<int, DeviceState *> g_dev;
 int g_prec = RRTMG_HIP_PREC_FP64;
 
@@ -173,6 +183,31 @@ int check_common(int ncol, int nlay, const double *plev, int ptr_kind)
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Device-pointer inputs checked on the device when rrtmg_hip_set_checks is
+// on: the host-pointer path's checks (finite inputs, plev decreasing upward
+// and positive at the top).  The call waits for the verdict on its stream.
+struct NamedArray {
+    const char *name;
+    const double *p;
+    size_t n;
+};
+int device_checks(StreamWork &W, hipStream_t st, const std::vector<NamedArray> &a, const double *plev, int ncol,
+                  int nlay)
+{
+    int rc;
+    if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
+    std::vector<DeviceArray> d;
+    for (const NamedArray &x : a) d.push_back({x.p, x.n});
+    int *flag = (int *)W.flag.p, h = 0;
+    hipError_t e = launch_input_checks(d.data(), (int)d.size(), plev, ncol, nlay, flag, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("input checks: ") + hipGetErrorString(e));
+    if (h == INT_MAX) return 0;
+    if (h == (int)a.size()) return fail(RRTMG_HIP_EINVAL, "plev must decrease strictly upward and be positive at the top");
+    return fail(RRTMG_HIP_EINVAL, std::string("non-finite value in ") + a[h].name);
+}
 
 // The band kernels keep two per-level arrays per lane in LDS; refuse a layer
 // count whose footprint exceeds the device's per-workgroup limit.
@@ -419,6 +454,16 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
             if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
     }
     hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if (ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
+        std::vector<NamedArray> a;
+        for (int i = 0; i < 8; ++i) a.push_back({nm2[i], ins2[i], n2});
+        a.push_back({"plev", plev, n3});
+        a.push_back({"tlev", tlev, n3});
+        const char *nm1[] = {"tsfc", "asdir", "asdif", "aldir", "aldif", "coszen"};
+        for (int i = 0; i < 6; ++i) a.push_back({nm1[i], ins1[i], (size_t)ncol});
+        if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
+    }
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(sw_fwd_lds_bytes(nlay, rs)))) return rc;
@@ -433,8 +478,8 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
     const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves, sw_fwd_state_bytes(nlay, rs));
     const size_t o_ds = off; off = align_up(off + (size_t)chunk * sw_fwd_state_bytes(nlay, rs));
-    if ((rc = s->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
-    char *wb = (char *)s->sw_work.p;
+    if ((rc = W.sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)W.sw_work.p;
     SwWork w;
     w.coef = wb + o_coef;
     w.ipk = (int *)(wb + o_ipk);
@@ -452,8 +497,8 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     double *stage = nullptr;
     if (ptr_kind == RRTMG_HIP_HOST) {
         const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + 4 * n3 + 2 * n2);
-        if ((rc = s->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
-        stage = (double *)s->sw_stage.p;
+        if ((rc = W.sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        stage = (double *)W.sw_stage.p;
         double *p = stage;
         const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4, &in.n2o, &in.o2};
         for (int i = 0; i < 8; ++i) {
@@ -547,6 +592,17 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
         if (!plev || !tlev || !tsfc || !emis) return fail(RRTMG_HIP_EINVAL, "null input array");
     }
     hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if (ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
+        std::vector<NamedArray> a;
+        for (int i = 0; i < 12; ++i) a.push_back({nm2[i], ins2[i], n2});
+        a.push_back({"plev", plev, n3});
+        a.push_back({"tlev", tlev, n3});
+        a.push_back({"tsfc", tsfc, (size_t)ncol});
+        a.push_back({"emis", emis, (size_t)ncol * 16});
+        if (tauaer) a.push_back({"tauaer", tauaer, n2 * 16});
+        if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
+    }
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(lw_fwd_lds_bytes(nlay, rs)))) return rc;
@@ -566,8 +622,8 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
                                 : (size_t)chunk * lw_fwd_state_bytes(nlay, rs);
     const size_t o_ds = off; off = align_up(off + ds_bytes);
     const size_t o_it = off; off = align_up(off + sizeof(int));
-    if ((rc = s->lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
-    char *wb = (char *)s->lw_work.p;
+    if ((rc = W.lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)W.lw_work.p;
     LwWork w;
     w.coef = wb + o_coef;
     w.tlev = wb + o_tlev;
@@ -587,8 +643,8 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     if (ptr_kind == RRTMG_HIP_HOST) {
         const size_t need = sizeof(double) * (12 * n2 + 2 * n3 + (size_t)ncol * 17 + (tauaer ? 16 * n2 : 0) +
                                               4 * n3 + 2 * n2);
-        if ((rc = s->lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
-        double *p = (double *)s->lw_stage.p;
+        if ((rc = W.lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        double *p = (double *)W.lw_stage.p;
         const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4,
                                  &in.n2o, &in.o2, &in.cfc11, &in.cfc12, &in.cfc22, &in.ccl4};
         for (int i = 0; i < 12; ++i) {
@@ -700,6 +756,19 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
         if (!plev || !tlev) return fail(RRTMG_HIP_EINVAL, "null input array");
     }
     hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if (!host && g_checks) {
+        std::vector<NamedArray> a;
+        for (int i = 0; i < 8; ++i) a.push_back({nm2[i], ins2[i], n2});
+        a.push_back({"plev", plev, n3});
+        a.push_back({"tlev", tlev, n3});
+        const char *nm1[] = {"tsfc", "asdir", "asdif", "aldir", "aldif", "coszen"};
+        for (int i = 0; i < 6; ++i) a.push_back({nm1[i], ins1[i], (size_t)ncol});
+        if (swuflx_bar) a.push_back({"swuflx_bar", swuflx_bar, n3});
+        if (swdflx_bar) a.push_back({"swdflx_bar", swdflx_bar, n3});
+        if (swhr_bar) a.push_back({"swhr_bar", swhr_bar, n2});
+        if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
+    }
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(sw_vjp_lds_bytes(nlay, rs)))) return rc;
@@ -713,8 +782,8 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves, sw_vjp_state_bytes(nlay, rs));
     const size_t o_ds = off; off = align_up(off + (size_t)chunk * sw_vjp_state_bytes(nlay, rs));
     const size_t o_bf = off; off = align_up(off + (fwd ? rs * 14 * 2 * n3 : 0));
-    if ((rc = s->sw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
-    char *wb = (char *)s->sw_vjp_work.p;
+    if ((rc = W.sw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)W.sw_vjp_work.p;
     SwVjpWork w;
     w.dcoef = wb + o_dc;
     w.ipk = (int *)(wb + o_ipk);
@@ -727,9 +796,9 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     w.bflux = fwd ? wb + o_bf : nullptr;
     if (host) {
         const size_t need = 256 * 27 + sizeof(double) * (12 * n2 + 4 * n3 + (size_t)ncol * 6 + (fwd ? n2 + 2 * n3 : 0));
-        if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        if ((rc = W.vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
     }
-    Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
+    Stager sg{host ? (char *)W.vjp_stage.p : nullptr, host, st};
     ColumnInputs in{};
     in.ncol = ncol;
     in.nlay = nlay;
@@ -816,6 +885,20 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
         if (!plev || !tlev || !tsfc || !emis) return fail(RRTMG_HIP_EINVAL, "null input array");
     }
     hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if (!host && g_checks) {
+        std::vector<NamedArray> a;
+        for (int i = 0; i < 12; ++i) a.push_back({nm2[i], ins2[i], n2});
+        a.push_back({"plev", plev, n3});
+        a.push_back({"tlev", tlev, n3});
+        a.push_back({"tsfc", tsfc, (size_t)ncol});
+        a.push_back({"emis", emis, (size_t)ncol * 16});
+        if (tauaer) a.push_back({"tauaer", tauaer, n2 * 16});
+        if (uflx_bar) a.push_back({"uflx_bar", uflx_bar, n3});
+        if (dflx_bar) a.push_back({"dflx_bar", dflx_bar, n3});
+        if (hr_bar) a.push_back({"hr_bar", hr_bar, n2});
+        if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
+    }
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(lw_vjp_lds_bytes(nlay, rs)))) return rc;
@@ -838,8 +921,8 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const size_t o_ds = off; off = align_up(off + ds_bytes);
     const size_t o_it = off; off = align_up(off + sizeof(int));
     const size_t o_bf = off; off = align_up(off + (fwd ? rs * 16 * 2 * n3 : 0));
-    if ((rc = s->lw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
-    char *wb = (char *)s->lw_vjp_work.p;
+    if ((rc = W.lw_vjp_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+    char *wb = (char *)W.lw_vjp_work.p;
     LwVjpWork w;
     w.dcoef = wb + o_dc;
     w.tlev = wb + o_tlev;
@@ -858,9 +941,9 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     if (host) {
         const size_t need = 256 * 27 + sizeof(double) * (16 * n2 + 5 * n3 + (size_t)ncol * 18 + (tauaer ? 16 * n2 : 0) +
                                                          (fwd ? n2 + 2 * n3 : 0));
-        if ((rc = s->vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        if ((rc = W.vjp_stage.ensure(need))) return fail(rc, "device staging allocation failed");
     }
-    Stager sg{host ? (char *)s->vjp_stage.p : nullptr, host, st};
+    Stager sg{host ? (char *)W.vjp_stage.p : nullptr, host, st};
     ColumnInputs in{};
     in.ncol = ncol;
     in.nlay = nlay;
@@ -992,7 +1075,8 @@ int rrtmg_hip_gemm_bf16(int M, int N, int K, const void *A, int lda, int transA,
         DeviceState *s = state();
         if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
         std::lock_guard<std::mutex> lk(s->mu);
-        if (s->gemm_ws.ensure((size_t)ks * M * N * sizeof(float)) == 0) part = (float *)s->gemm_ws.p;
+        StreamWork &W = s->work((hipStream_t)stream);
+        if (W.gemm_ws.ensure((size_t)ks * M * N * sizeof(float)) == 0) part = (float *)W.gemm_ws.p;
         else ks = 0;
     }
     hipError_t e = launch_gemm_bf16(M, N, K, (const uint16_t *)A, lda, transA, (const uint16_t *)B, ldb, transB, ep,
@@ -1081,14 +1165,15 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
     std::lock_guard<std::mutex> lk(s->mu);
+    hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
     const size_t act = align_up((size_t)ncol * wmax * sizeof(uint16_t));
     const size_t outb = align_up((size_t)ncol * width[net->nlayers] * sizeof(float));
     int rc;
-    if ((rc = s->mlp_work.ensure(2 * act + outb))) return fail(rc, "surrogate workspace allocation failed");
-    char *wb = (char *)s->mlp_work.p;
+    if ((rc = W.mlp_work.ensure(2 * act + outb))) return fail(rc, "surrogate workspace allocation failed");
+    char *wb = (char *)W.mlp_work.p;
     uint16_t *buf[2] = {(uint16_t *)wb, (uint16_t *)(wb + act)};
     float *y = (float *)(wb + 2 * act);
-    hipStream_t st = (hipStream_t)stream;
     hipError_t e = launch_mlp_features(ncol, nlay, tlay, h2ovmr, o3vmr, tsfc, coszen, net->in_mean, net->in_istd,
                                        net->in_lo, net->in_hi, net->ldx, buf[0], nullptr, out_of_envelope, st);
     int cur = 0;
@@ -1107,6 +1192,13 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
     if (e == hipSuccess)
         e = launch_mlp_decode(ncol, nlay, y, width[net->nlayers], net->out_mean, net->out_std, hr_lw, hr_sw, st);
     return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("surrogate: ") + hipGetErrorString(e));
+}
+
+int rrtmg_hip_set_checks(int on)
+{
+    int prev = g_checks ? 1 : 0;
+    g_checks = on != 0;
+    return prev;
 }
 
 int rrtmg_hip_set_timing(int on)

@@ -179,6 +179,18 @@ hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *
 hipError_t launch_sw_finish(const ColumnInputs &in, const SwOutputs &out, void *bflux, const SwDevTables &t,
                             int mode, hipStream_t s);
 
+// one device array of n doubles, for launch_input_checks
+struct DeviceArray {
+    const double *p;
+    size_t n;
+};
+// Checks on the device that every array is finite and, if plev is given,
+// that plev (ncol, nlay+1) decreases strictly upward and is positive at the
+// top.  flag (device int) ends as INT_MAX, the index of the first non-finite
+// array, or narr for plev.
+hipError_t launch_input_checks(const DeviceArray *arrays, int narr, const double *plev, int ncol, int nlay, int *flag,
+                               hipStream_t s);
+
 // mode (the C ABI's precision): 0 FP64 everywhere; 1 FP32 with FP64 islands;
 // 2 FP64 with exp(-x) and the tau transition function evaluated directly
 // instead of read from the 10000-point tables (smooth in the inputs).  ev: NULL, or four

@@ -76,6 +76,13 @@ def set_precision(mode):
     return {v: k for k, v in PREC.items()}[prev]
 
 
+def set_checks(on):
+    """Input checks of device-pointer calls on the device (finite inputs, plev
+    decreasing upward); the calls then wait for the verdict.  Returns the
+    previous setting."""
+    return bool(lib().rrtmg_hip_set_checks(1 if on else 0))
+
+
 def _is_torch(x):
     return type(x).__module__.startswith("torch")
 

@@ -179,8 +179,19 @@ def test_full_size_mixed_vs_fp64(rr):
         assert _maxdiff(l32[k], l64[k]) < MIXED_ATOL_FLUX, k
     assert _maxdiff(s32["swhr"], s64["swhr"]) < MIXED_ATOL_HR
     assert _maxdiff(l32["hr"], l64["hr"]) < MIXED_ATOL_HR
-    # energy bookkeeping: heating integrates to the net-flux divergence
-    assert np.all(np.isfinite(s32["swhr"])) and np.all(np.isfinite(l32["hr"]))
+    # energy bookkeeping in both precisions: each layer's heating rate is the
+    # divergence of the returned net fluxes across it
+    # (rrtmg_lw_rtrnmr.f90:771-772, rrtmg_sw_rad.nomcica.f90:713-726)
+    heatfac = 9.8066 * 8.64e4 / (rr.CPDAIR * 1e2)
+    dp = c["plev"][:-1] - c["plev"][1:]
+    for s_, l_ in ((s64, l64), (s32, l32)):
+        fn = l_["uflx"] - l_["dflx"]
+        np.testing.assert_allclose(l_["hr"], heatfac * (fn[:-1] - fn[1:]) / dp, rtol=0, atol=1e-9)
+        sn = s_["swdflx"] - s_["swuflx"]
+        hsw = heatfac * (sn[1:] - sn[:-1]) / dp
+        hsw[-1] = 0.0   # the reference sets the top layer's SW heating to zero
+        np.testing.assert_allclose(s_["swhr"], hsw, rtol=0, atol=1e-9)
+        assert np.all(np.isfinite(s_["swhr"])) and np.all(np.isfinite(l_["hr"]))
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
