@@ -277,6 +277,33 @@ int rrtmg_hip_last_kernel_ms(int kind, float *ms);
  * different workspaces and may overlap.  Growing a workspace synchronises
  * the device. */
 
+/* rrtmg_hip_sw without device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as rrtmg_hip_sw with ptr_kind = RRTMG_HIP_DEVICE (bitwise the same
+ * outputs: the same kernels), work enqueued on `stream`.  The workspace is a
+ * fixed device array of RRTMG_HIP_NOALLOC_BYTES per GPU, held from
+ * rrtmg_hip_sw_init until the process ends; the call allocates nothing.  Any
+ * ncol x (nlay + 1) <= RRTMG_HIP_NOALLOC_MAX_CELLS fits in every precision
+ * mode; a call that does not fit returns RRTMG_HIP_EINVAL naming the size.
+ * Every call on one GPU shares that GPU's array, whatever its stream: calls
+ * that overlap on one GPU (two streams, two host threads) race, so order them.
+ * With rrtmg_hip_set_checks on, the input checks run as for rrtmg_hip_sw
+ * (their flag is in the same array; the call then waits for them). */
+#define RRTMG_HIP_NOALLOC_BYTES (128u << 20)
+#define RRTMG_HIP_NOALLOC_MAX_CELLS 262144
+int rrtmg_hip_sw_noalloc(int ncol, int nlay, int icld, int iaer,
+                         const double *play, const double *plev, const double *tlay,
+                         const double *tlev, const double *tsfc,
+                         const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                         const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                         const double *asdir, const double *asdif, const double *aldir,
+                         const double *aldif, const double *coszen,
+                         double adjes, int dyofyr, double scon,
+                         double *swuflx, double *swdflx, double *swhr,
+                         double *swuflxc, double *swdflxc, double *swhrc,
+                         void *stream);
+
 /* Message for the last failing call on this host thread ("" if none). */
 const char *rrtmg_hip_last_error(void);
 

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/rrtmg_hip.h"
+#include "capi_internal.hpp"
 #include "kernels.hpp"
 #include "mlp.hpp"
 #include "tables.hpp"
@@ -192,14 +193,12 @@ struct NamedArray {
     const double *p;
     size_t n;
 };
-int device_checks(StreamWork &W, hipStream_t st, const std::vector<NamedArray> &a, const double *plev, int ncol,
+int device_checks(int *flag, hipStream_t st, const std::vector<NamedArray> &a, const double *plev, int ncol,
                   int nlay)
 {
-    int rc;
-    if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
     std::vector<DeviceArray> d;
     for (const NamedArray &x : a) d.push_back({x.p, x.n});
-    int *flag = (int *)W.flag.p, h = 0;
+    int h = 0;
     hipError_t e = launch_input_checks(d.data(), (int)d.size(), plev, ncol, nlay, flag, st);
     if (e == hipSuccess) e = hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -211,6 +210,14 @@ int device_checks(StreamWork &W, hipStream_t st, const std::vector<NamedArray> &
 
 // The band kernels keep two per-level arrays per lane in LDS; refuse a layer
 // count whose footprint exceeds the device's per-workgroup limit.
+int device_checks(StreamWork &W, hipStream_t st, const std::vector<NamedArray> &a, const double *plev, int ncol,
+                  int nlay)
+{
+    int rc;
+    if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
+    return device_checks((int *)W.flag.p, st, a, plev, ncol, nlay);
+}
+
 int check_lds(size_t bytes)
 {
     static int limit = -1;
@@ -305,6 +312,7 @@ int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
     if (s->swd.ready) {
         s->sw.heatfac = 9.8066 * 8.6400e4 / (cpdair * 1.e2);
         s->swd.cpdair = cpdair;
+        (void)noalloc_attach();
         return 0;
     }
     std::string dir = table_dir ? table_dir : default_table_dir();
@@ -318,6 +326,9 @@ int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
     if (rc) return fail(rc, "uploading SW tables failed");
     s->swd.cpdair = cpdair;
     s->swd.ready = true;
+    // the no-allocation entry point's scratch address on this device; a
+    // failure is reported by that entry point (EDEVICE), not here
+    (void)noalloc_attach();
     return 0;
 }
 
@@ -415,17 +426,31 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     return 0;
 }
 
-int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev, const double *tlay,
-                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
-                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
-                 const double *asdir, const double *asdif, const double *aldir, const double *aldif,
-                 const double *coszen, double adjes, int dyofyr, double scon, double *swuflx, double *swdflx,
-                 double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind, void *stream)
+}  // extern "C"
+
+namespace rrtmg {
+
+// rrtmg_hip_sw, with its workspace from the stream's growing buffer (fx NULL)
+// or carved from the caller's fixed device region fx (device pointers only;
+// nothing is allocated: the band kernel's scratch rows are reused over as
+// many launches as the region requires, which leaves every result bitwise
+// unchanged because wave units are independent).
+int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev, const double *tlay,
+               const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+               const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+               const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+               const double *coszen, double adjes, int dyofyr, double scon, double *swuflx, double *swdflx,
+               double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind, void *stream,
+               const FixedWorkspace *fx)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
     std::lock_guard<std::mutex> lk(s->mu);
     if (!s->swd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_sw called before rrtmg_hip_sw_init");
+    if (fx && !fx->p)
+        return fail(RRTMG_HIP_EDEVICE, "the no-allocation scratch array has no device address on this device");
+    if (fx && ptr_kind != RRTMG_HIP_DEVICE)
+        return fail(RRTMG_HIP_EINVAL, "the no-allocation entry point takes device pointers only");
     if (icld != 0) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: only the clear-sky path is implemented");
     if (iaer != 0) return fail(RRTMG_HIP_ENOTSUP, "iaer != 0: SW aerosol optics are not implemented");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
@@ -454,7 +479,39 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
             if (!p) return fail(RRTMG_HIP_EINVAL, "null input array");
     }
     hipStream_t st = (hipStream_t)stream;
-    StreamWork &W = s->work(st);
+    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
+    const size_t rs = mixed ? sizeof(float) : sizeof(double);
+    if ((rc = check_lds(sw_fwd_lds_bytes(nlay, rs)))) return rc;
+
+    // workspace: [input-check flag (fixed region only)] coefficient records,
+    // packed indices, laytrop, laysolfr, band fluxes, scratch rows
+    size_t off = 0;
+    const size_t o_coef = off; off = align_up(off + rs * SWF_STRIDE * n2);
+    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
+    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
+    const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
+    const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
+    const size_t o_ds = off;
+    const int units = (ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves;
+    const size_t unit_bytes = sw_fwd_state_bytes(nlay, rs);
+    int chunk;
+    char *wb;
+    int *flag = nullptr;
+    StreamWork *W = nullptr;
+    if (fx) {
+        const size_t avail = fx->cap > 256 ? fx->cap - 256 : 0;
+        const size_t rows = avail > o_ds ? (avail - o_ds) / unit_bytes : 0;
+        if (rows < (size_t)std::min(units, 8))
+            return fail(RRTMG_HIP_EINVAL, "ncol x nlay too large for the no-allocation entry point (needs more than " +
+                                              std::to_string(o_ds + 8 * unit_bytes + 256) + " bytes of its " +
+                                              std::to_string(fx->cap) + "-byte scratch array); call rrtmg_hip_sw");
+        chunk = (int)std::min((size_t)units, rows);
+        flag = (int *)fx->p;
+        wb = (char *)fx->p + 256;
+    } else {
+        W = &s->work(st);
+        chunk = vjp_chunk(units, unit_bytes);
+    }
     if (ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
         std::vector<NamedArray> a;
         for (int i = 0; i < 8; ++i) a.push_back({nm2[i], ins2[i], n2});
@@ -462,24 +519,15 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
         a.push_back({"tlev", tlev, n3});
         const char *nm1[] = {"tsfc", "asdir", "asdif", "aldir", "aldif", "coszen"};
         for (int i = 0; i < 6; ++i) a.push_back({nm1[i], ins1[i], (size_t)ncol});
-        if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
+        if ((rc = flag ? device_checks(flag, st, a, plev, ncol, nlay) : device_checks(*W, st, a, plev, ncol, nlay)))
+            return rc;
     }
-    const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
-    const size_t rs = mixed ? sizeof(float) : sizeof(double);
-    if ((rc = check_lds(sw_fwd_lds_bytes(nlay, rs)))) return rc;
     int ck = 0;
-
-    // workspace
-    size_t off = 0;
-    const size_t o_coef = off; off = align_up(off + rs * SWF_STRIDE * n2);
-    const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
-    const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
-    const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
-    const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
-    const int chunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves, sw_fwd_state_bytes(nlay, rs));
-    const size_t o_ds = off; off = align_up(off + (size_t)chunk * sw_fwd_state_bytes(nlay, rs));
-    if ((rc = W.sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
-    char *wb = (char *)W.sw_work.p;
+    off = align_up(o_ds + (size_t)chunk * unit_bytes);
+    if (W) {
+        if ((rc = W->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
+        wb = (char *)W->sw_work.p;
+    }
     SwWork w;
     w.coef = wb + o_coef;
     w.ipk = (int *)(wb + o_ipk);
@@ -497,8 +545,8 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
     double *stage = nullptr;
     if (ptr_kind == RRTMG_HIP_HOST) {
         const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + 4 * n3 + 2 * n2);
-        if ((rc = W.sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
-        stage = (double *)W.sw_stage.p;
+        if ((rc = W->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
+        stage = (double *)W->sw_stage.p;
         double *p = stage;
         const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4, &in.n2o, &in.o2};
         for (int i = 0; i < 8; ++i) {
@@ -555,6 +603,22 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
         if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("SW: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+}  // namespace rrtmg
+
+extern "C" {
+
+int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *asdir, const double *asdif, const double *aldir, const double *aldif,
+                 const double *coszen, double adjes, int dyofyr, double scon, double *swuflx, double *swdflx,
+                 double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind, void *stream)
+{
+    return sw_forward(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr, swuflxc,
+                      swdflxc, swhrc, ptr_kind, stream, nullptr);
 }
 
 int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,
