@@ -312,6 +312,13 @@ const char *rrtmg_hip_last_error(void);
  * copied to host int arrays of (ncol, nlay): jp, jt, jt1 (1-based, as the
  * reference's setcoef), and laytrop per column.  0 on success. */
 int rrtmg_hip_debug_indices(int lw, int *jp, int *jt, int *jt1, int *laytrop);
+/* The smooth functions that the lookup tables stand for, at n host
+ * arguments x (optical depths), as the smooth FP64 mode and the backward
+ * evaluate them on the device: out[6 i + 0..5] = LW 1 - exp(-x), LW tau
+ * transition function tfn(x) (rrtmg_lw_init.f90:106-123), their derivatives,
+ * SW exp(-x) floored at 1e-20 (rrtmg_sw_init.f90:94-104) and its derivative.
+ * Host arrays; synchronous. */
+int rrtmg_hip_debug_smooth(int n, const double *x, double *out);
 /* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...), Fortran
  * order, FP64, as built at init; returns the element count or -1. */
 long rrtmg_hip_debug_table(const char *name, double *out, long n);

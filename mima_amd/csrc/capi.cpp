@@ -1327,6 +1327,22 @@ int rrtmg_hip_debug_host_tables(int lw, const char *table_dir, double cpdair)
     return 0;
 }
 
+int rrtmg_hip_debug_smooth(int n, const double *x, double *out)
+{
+    if (n < 0 || (n > 0 && (!x || !out))) return fail(RRTMG_HIP_EINVAL, "bad arguments");
+    if (n == 0) return 0;
+    double *dx = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc((void **)&dx, sizeof(double) * n);
+    if (e == hipSuccess) e = hipMalloc((void **)&dout, sizeof(double) * 6 * n);
+    if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_smooth_functions(n, dx, dout, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * 6 * n, hipMemcpyDeviceToHost);
+    (void)hipFree(dx);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("smooth functions: ") + hipGetErrorString(e));
+    return 0;
+}
+
 long rrtmg_hip_debug_table(const char *name, double *out, long n)
 {
     if (!name) return -1;

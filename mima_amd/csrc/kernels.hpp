@@ -191,6 +191,11 @@ struct DeviceArray {
 hipError_t launch_input_checks(const DeviceArray *arrays, int narr, const double *plev, int ncol, int nlay, int *flag,
                                hipStream_t s);
 
+// x[n] (device) -> out[n][6] (device): LW 1 - exp(-x), tfn(x), their
+// derivatives, SW exp(-x) floored at 1e-20 and its derivative, as the smooth
+// mode and the backward evaluate them
+hipError_t launch_smooth_functions(int n, const double *x, double *out, hipStream_t s);
+
 // mode (the C ABI's precision): 0 FP64 everywhere; 1 FP32 with FP64 islands;
 // 2 FP64 with exp(-x) and the tau transition function evaluated directly
 // instead of read from the 10000-point tables (smooth in the inputs).  ev: NULL, or four

@@ -123,3 +123,116 @@ def test_graph_replay_reproduces_eager(rr):
         _fused(rr, d, cot)
         torch.cuda.synchronize()
     print("graph: %d outputs bitwise equal to eager on 3 replays" % len(ref))
+
+
+# ---------------------------------------------------------------------------
+# the large-batch workload itself: 512 x 256 = 131072 columns x 60 levels, CO2
+# log-uniform on 280-1120 ppmv per column
+
+@pytest.fixture(scope="module")
+def c4(rr):
+    import torch
+    c = make_columns(512, 256, 60, seed=41, co2_logrange=(280.0, 1120.0))
+    d = rr.to_device(c)
+    L, n = c["nlay"], c["ncol"]
+    g = torch.Generator(device="cpu").manual_seed(41)
+    rnd = lambda *sh: torch.randn(*sh, generator=g, dtype=torch.float64).cuda()  # noqa: E731
+    cot = dict(ub=rnd(L + 1, n) * 0.1, db=rnd(L + 1, n) * 0.1, hr=rnd(L, n),
+               sub=rnd(L + 1, n) * 0.1, sdb=rnd(L + 1, n) * 0.1, shr=rnd(L, n))
+    return c, d, cot
+
+
+def _graph_step(rr, d, cot, side):
+    import torch
+    cur = torch.cuda.current_stream()
+    side.wait_stream(cur)
+    sw = rr.sw_vjp_columns(d, swuflx_bar=cot["sub"], swdflx_bar=cot["sdb"], swhr_bar=cot["shr"], forward=True)
+    with torch.cuda.stream(side):
+        lw = rr.lw_vjp_columns(d, uflx_bar=cot["ub"], dflx_bar=cot["db"], hr_bar=cot["hr"], forward=True)
+    cur.wait_stream(side)
+    return sw, lw
+
+
+def test_large_batch_graph_matches_separate_calls(rr, c4):
+    """FP32-mixed forward + backward of all 131072 x 60 columns captured in a
+    HIP graph (SW and LW on two streams), replayed twice: gradients bitwise
+    those of the separate backward calls, fluxes and heating within 0.01 of
+    the forward calls, night columns exactly zero."""
+    import torch
+    c, d, cot = c4
+    rr.set_precision("mixed")
+    print("large batch %d x %d, graph of the one-pass calls" % (c["ncol"], c["nlay"]), flush=True)
+    gs = rr.sw_vjp_columns(d, swuflx_bar=cot["sub"], swdflx_bar=cot["sdb"], swhr_bar=cot["shr"])
+    gl = rr.lw_vjp_columns(d, uflx_bar=cot["ub"], dflx_bar=cot["db"], hr_bar=cot["hr"])
+    fs, fl = rr.sw_columns(d), rr.lw_columns(d)
+    torch.cuda.synchronize()
+    main, side = torch.cuda.Stream(), torch.cuda.Stream()
+    with torch.cuda.stream(main):
+        _graph_step(rr, d, cot, side)          # grows the workspaces of both streams
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=main):
+            gsw, glw = _graph_step(rr, d, cot, side)
+    for rep in range(2):
+        for v in list(gsw.values()) + list(glw.values()):
+            v.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        for k, v in gs.items():
+            assert torch.equal(gsw[k], v), (rep, "SW", k)
+        for k, v in gl.items():
+            assert torch.equal(glw[k], v), (rep, "LW", k)
+        for k in ("swuflx", "swdflx", "swhr"):
+            err = float((gsw[k] - fs[k]).abs().max())
+            assert err <= 1e-2, (rep, k, err)
+        for k in ("uflx", "dflx", "hr"):
+            err = float((glw[k] - fl[k]).abs().max())
+            assert err <= 1e-2, (rep, k, err)
+    night = torch.from_numpy(c["coszen"] < 1e-10).cuda()
+    assert int(night.sum()) > 0
+    for k in ("swuflx", "swhr", "tlay", "h2ovmr", "o3vmr"):
+        assert bool((gsw[k][:, night] == 0).all()), k
+
+
+def _subset_np(c, idx):
+    out = {}
+    for k, v in c.items():
+        if isinstance(v, np.ndarray) and v.ndim == 2 and v.shape[1] == c["ncol"]:
+            out[k] = np.ascontiguousarray(v[:, idx])
+        elif isinstance(v, np.ndarray) and v.ndim == 1 and v.shape[0] == c["ncol"]:
+            out[k] = np.ascontiguousarray(v[idx])
+        else:
+            out[k] = v
+    out["ncol"] = len(idx)
+    return out
+
+
+def test_large_batch_forward_matches_oracle_and_fp64(rr, c4):
+    """The one-pass FP64 forward of the 131072 x 60 columns against the CPU
+    oracle on 512 seeded columns (1e-7 W/m2 and K/day), and the FP32-mixed
+    one-pass forward against it on every column (0.05 W/m2 and K/day)."""
+    import pyoracle as po
+    import torch
+    c, d, cot = c4
+    rr.set_precision("fp64")
+    sw64, lw64 = _fused(rr, d, cot)
+    rr.set_precision("mixed")
+    swm, lwm = _fused(rr, d, cot)
+    torch.cuda.synchronize()
+    for a, b in ((sw64, swm), (lw64, lwm)):
+        for k in [k for k in a if k in ("swuflx", "swdflx", "swhr", "uflx", "dflx", "hr")]:
+            err = float((a[k] - b[k]).abs().max())
+            print("mixed vs FP64 %s: %.2e" % (k, err))
+            assert err <= 0.05, (k, err)
+    idx = np.sort(np.random.default_rng(41).choice(c["ncol"], 512, replace=False))
+    sub = _subset_np(c, idx)
+    osw, olw = po.rrtmg_sw(sub), po.rrtmg_lw(sub)
+    it = torch.from_numpy(idx).cuda()
+    for k in ("swuflx", "swdflx", "swhr"):
+        err = float(np.abs(sw64[k][:, it].cpu().numpy() - osw[k]).max())
+        print("FP64 vs oracle SW %s: %.2e" % (k, err))
+        assert err <= 1e-7, (k, err)
+    for k in ("uflx", "dflx", "hr"):
+        err = float(np.abs(lw64[k][:, it].cpu().numpy() - olw[k]).max())
+        print("FP64 vs oracle LW %s: %.2e" % (k, err))
+        assert err <= 1e-7, (k, err)

@@ -85,10 +85,14 @@ def test_gemm_bf16_tiled_path_matches_reference(env, M, N, K, lda, ldb):
     np.testing.assert_array_equal(outb.float().cpu().numpy(), mo.bf16_round(np.maximum(pre.cpu().numpy(), 0)))
 
 
-def test_training_step_gradients_and_adam_match_cpu_reference(env):
+@pytest.mark.parametrize("hidden", [(64, 48), (256, 256, 256)])
+def test_training_step_gradients_and_adam_match_cpu_reference(env, hidden):
+    """One training step of a small network and of the configured
+    122-256-256-256-80 network: loss, every layer's weight and bias gradient
+    and one Adam update against the FP64 reference on the same BF16 values."""
     rr, torch = env
     from mima_amd.surrogate import Surrogate
-    s = Surrogate(nlay=40, hidden=(64, 48), seed=3)
+    s = Surrogate(nlay=40, hidden=hidden, seed=3)
     rng = np.random.default_rng(2)
     B = 192
     x = rng.standard_normal((B, s.ldx)).astype(np.float32)
@@ -122,16 +126,50 @@ def test_training_step_gradients_and_adam_match_cpu_reference(env):
     np.testing.assert_array_equal(s.wbf[0].float().cpu().numpy(), mo.bf16_round(s.w32[0].cpu().numpy()))
 
 
+def test_inference_of_configured_network_matches_cpu_reference(env):
+    """rrtmg_hip_surrogate with the configured 122-256-256-256-80 network
+    (features, four BF16 layers with FP32 accumulation, decode) against the
+    FP64 reference forward of the same BF16 features, weights and biases;
+    the raw features against their definition (T, ln q, O3 in ppmv by mass
+    mixing ratio x 1e6, Ts, cos zenith)."""
+    rr, torch = env
+    from mima_amd.surrogate import Surrogate, exact_heating
+    c = make_columns(64, 32, 40, seed=13, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0))
+    d = rr.to_device(c)
+    hl, hs = exact_heating(d)
+    s = Surrogate(nlay=40, seed=4)
+    s.set_statistics(d, hl, hs)
+    for l in range(len(s.b32)):                      # non-zero biases exercise the epilogue
+        s.b32[l].copy_(0.1 * torch.randn(s.b32[l].shape, generator=torch.Generator().manual_seed(l)).cuda())
+    pl, ps, fl = s.predict(d)
+    x, xr, _ = s.features(d, raw=True)
+    torch.cuda.synchronize()
+    L, n = 40, c["ncol"]
+    want = np.concatenate([c["tlay"], np.log(c["h2ovmr"]), c["o3vmr"] * 1e6, c["tsfc"][None], c["coszen"][None]]).T
+    np.testing.assert_allclose(xr[:, :3 * L + 2].cpu().numpy(), want, rtol=2e-6, atol=1e-6)
+    x64 = x.float().cpu().numpy().astype(np.float64)
+    W = [w.float().cpu().numpy().astype(np.float64) for w in s.wbf]
+    b = [bb.cpu().numpy().astype(np.float64) for bb in s.b32]
+    _, _, y = mo.forward(x64, W, b)
+    om, osd = s.out_mean.cpu().numpy().astype(np.float64), s.out_std.cpu().numpy().astype(np.float64)
+    got = np.concatenate([pl.cpu().numpy(), ps.cpu().numpy()]).T        # [ncol][80], K/day
+    yg = (got - om) / osd
+    err = np.linalg.norm(yg - y) / np.linalg.norm(y)
+    print("inference vs reference: relative L2 %.2e, max %.2e (standardised units)" % (err, np.abs(yg - y).max()))
+    assert err < 1e-3 and np.abs(yg - y).max() < 1e-2, err
+    assert int(fl.sum()) == 0
+
+
 def test_surrogate_learns_and_falls_back_outside_envelope(env):
-    """Train on 65536 columns (lapse rate U(4, 9) K/km, RH U(0.2, 1.0)) labelled
-    by the exact FP32-mixed kernels; on 8192 held-out columns the network's
-    heating-rate RMSE must beat the method without it (each layer's training
-    mean) several times over, and columns pushed outside the training
-    envelope must be flagged and recomputed exactly."""
+    """Train on 262144 columns (lapse rate U(4, 9) K/km, RH U(0.2, 1.0))
+    labelled by the exact FP32-mixed kernels; on 131072 held-out columns the
+    network's heating-rate RMSE must beat the method without it (each layer's
+    training mean) several times over, and columns pushed outside the
+    training envelope must be flagged and recomputed exactly."""
     rr, torch = env
     from mima_amd.surrogate import Surrogate, exact_heating, hybrid_heating
-    tr = rr.to_device(make_columns(256, 256, 40, seed=11, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0)))
-    te = rr.to_device(make_columns(128, 64, 40, seed=12, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0)))
+    tr = rr.to_device(make_columns(512, 512, 40, seed=11, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0)))
+    te = rr.to_device(make_columns(512, 256, 40, seed=12, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0)))
     hl, hs = exact_heating(tr)
     s = Surrogate(nlay=40, seed=0)
     hist = s.fit(tr, hl, hs, epochs=12, batch=1024, lr=2e-3, log=print)
