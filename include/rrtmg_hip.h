@@ -252,6 +252,80 @@ int rrtmg_hip_mlp_features(int ncol, int nlay, const double *tlay, const double 
 int rrtmg_hip_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw,
                           const float *mean, const float *istd, int ldt, float *t, void *stream);
 
+/* ---------------------------------------------------------------------------
+ * MiMA's radiation driver around rrtmg_sw/rrtmg_lw, on the device, so that a
+ * model whose state lives on the GPU calls the radiation without host copies.
+ * GCM arrays are FMS physics-window arrays (nlon, nlat, nlev) in Fortran
+ * order (longitude fastest), level 1 at the top of the atmosphere; RRTM
+ * arrays are the (ncol, nlay) arrays rrtmg_hip_sw/rrtmg_hip_lw take, ncol =
+ * nlon*nlat/lonstep.  All array arguments are device pointers (FP64); work is
+ * enqueued on `stream`.  Results are the reference's arithmetic operation by
+ * operation (bitwise for the pre- and post-processing; compute_zenith up to
+ * the last-place differences of the device's trigonometric functions). */
+
+/* Replaces compute_zenith(Time, equinox_day, dt, lat, lon, cosz, dyofyr)
+ * (RR/astro.f90:59-248, RR = src/atmos_param/rrtm_radiation): cosz[n] at lat,
+ * lon (radians).  Time as get_time returns it (seconds of the day, days),
+ * daysperyear = length_of_year()'s days, obliq in degrees (astro_nml).  dt = 0:
+ * instantaneous; 0 < dt < 86400: averaged over [t, t + dt] seconds (the
+ * reference's cases assume dt <= 43200, as its comments state); dt >= 86400:
+ * daily mean.  *dyofyr (host int) receives the day of the year. */
+int rrtmg_hip_compute_zenith(int n, const double *lat, const double *lon, int seconds, int days, int daysperyear,
+                             double equinox_day, double obliq, int dt, double *cosz, int *dyofyr, void *stream);
+
+/* Replaces interp_temp(z_full, z_half, t_surf_rad, t) (RR/rrtm_radiation.f90:
+ * 422-461): interface temperatures t_half (nlon, nlat, nlev+1) from full-level
+ * temperatures t and heights z_full (nlon, nlat, nlev), z_half (nlon, nlat,
+ * nlev+1) in metres; top extrapolated, bottom = t_surf (nlon, nlat). */
+int rrtmg_hip_interp_temp(int nlon, int nlat, int nlev, const double *z_full, const double *z_half,
+                          const double *t_surf, const double *t, double *t_half, void *stream);
+
+/* run_rrtmg's options (rrtm_radiation_nml, RR/rrtm_radiation.f90:111-173)
+ * and the physics window. */
+typedef struct {
+    int nlon, nlat, nlev, lonstep; /* nlon % lonstep == 0 */
+    int do_zm_tracers;             /* q replaced by its zonal mean */
+    int do_fixed_water;            /* q = fixed_water where p_full <= fixed_water_pres*100 and
+                                    * |lat| <= fixed_water_lat (lat in radians against the namelist's
+                                    * number, as the reference compares them) */
+    int do_precip_albedo, num_precip; /* albedo + (precip_albedo - albedo) * zonal mean precip / num_precip */
+    int do_read_ozone;             /* o3 = max(0, o3f * scale_ozone), else o3_val */
+    int do_zm_rad;                 /* zonal-mean heating and surface fluxes */
+    double fixed_water, fixed_water_pres, fixed_water_lat, precip_albedo, precip_lat;
+    double scale_ozone, o3_val, co2ppmv, h2o_lower_limit, temp_lower_limit, temp_upper_limit;
+} RrtmgHipDriverConfig;
+
+/* run_rrtmg from the model state to the RRTM inputs (RR/rrtm_radiation.f90:
+ * 581-677): optional zonal-mean water vapour, fixed water, precipitation
+ * albedo; every lonstep-th longitude, levels reversed to surface-first,
+ * pressures Pa -> hPa, a top interface at or below zero in any column
+ * replaced by half the top full-level pressure in all columns, water vapour
+ * and temperatures clamped.  Inputs: lat (nlon, nlat) radians, p_full, t, q
+ * (and o3f with do_read_ozone) (nlon, nlat, nlev), p_half, t_half
+ * (nlon, nlat, nlev+1), albedo, t_surf, coszen (and precip with
+ * do_precip_albedo: the accumulated precipitation, which the caller resets)
+ * (nlon, nlat).  Outputs: play, tlay, h2ovmr, o3vmr, co2vmr (ncol, nlev),
+ * plev, tlev (ncol, nlev+1), albedo_rr, tsfc, coszen_rr (ncol). */
+int rrtmg_hip_rrtmg_pre(const RrtmgHipDriverConfig *cfg, const double *lat, const double *p_full,
+                        const double *p_half, const double *t, const double *t_half, const double *q,
+                        const double *o3f, const double *albedo, const double *precip, const double *t_surf,
+                        const double *coszen, double *play, double *plev, double *tlay, double *tlev,
+                        double *h2ovmr, double *o3vmr, double *co2vmr, double *albedo_rr, double *tsfc,
+                        double *coszen_rr, void *stream);
+
+/* run_rrtmg from the RRTM outputs back to the model (RR/rrtm_radiation.f90:
+ * 715-826): heating rates (ncol, nlev) in K/day -> K/s, levels reversed,
+ * interpolated linearly in longitude between the RRTM columns (closing
+ * toroidally) or zonally averaged (do_zm_rad), added to tdt (nlon, nlat,
+ * nlev).  Optional outputs (NULL to skip): tdt_rad, tdt_sw, tdt_lw (the
+ * tendency, its SW and LW parts), flux_sw (net down SW at the surface) and
+ * flux_lw (down LW at the surface) -- both or neither, as the reference --,
+ * olr and isr (nlon, nlat). */
+int rrtmg_hip_rrtmg_post(const RrtmgHipDriverConfig *cfg, const double *swhr, const double *hr,
+                         const double *swuflx, const double *swdflx, const double *uflx, const double *dflx,
+                         double *tdt, double *tdt_rad, double *flux_sw, double *flux_lw, double *tdt_sw,
+                         double *tdt_lw, double *olr, double *isr, void *stream);
+
 /* Input checks of device-pointer calls (no reference counterpart; the
  * reference checks nothing).  Host-pointer calls always check that every
  * input is finite and that plev decreases strictly upward with a positive

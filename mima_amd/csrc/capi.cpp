@@ -76,7 +76,7 @@ struct SideTables {
 // different buffers, so they can overlap on the device.
 struct StreamWork {
     Buffer sw_work, lw_work, sw_stage, lw_stage;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws, flag;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws, flag, drv;
 };
 
 struct DeviceState {
@@ -1256,6 +1256,126 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
     if (e == hipSuccess)
         e = launch_mlp_decode(ncol, nlay, y, width[net->nlayers], net->out_mean, net->out_std, hr_lw, hr_sw, st);
     return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("surrogate: ") + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// MiMA's radiation driver around the RRTM calls (kernels in driver.hip)
+
+int rrtmg_hip_compute_zenith(int n, const double *lat, const double *lon, int seconds, int days, int daysperyear,
+                             double equinox_day, double obliq, int dt, double *cosz, int *dyofyr, void *stream)
+{
+    if (n < 0 || !dyofyr || (n > 0 && (!lat || !lon || !cosz)))
+        return fail(RRTMG_HIP_EINVAL, "compute_zenith: bad arguments");
+    if (daysperyear <= 0 || bad(equinox_day) || bad(obliq))
+        return fail(RRTMG_HIP_EINVAL, "compute_zenith: daysperyear must be positive, equinox_day and obliq finite");
+    // the per-call scalars, as RR/astro.f90:100-143 forms them
+    const double PI = 3.141592653589793;  // 4 atan(1), shared/fms/fms.f90:412
+    ZenithScalars z;
+    const double deg2rad = PI / 180.;
+    z.radpersec = 2 * PI / 86400.;
+    z.radsec = seconds * z.radpersec;
+    z.dt_pi = dt * z.radpersec;
+    z.dt = dt;
+    const int d = days - (int)(equinox_day * daysperyear);
+    int dy = d % daysperyear;
+    if (dy < 0) dy += daysperyear;  // Fortran MODULO
+    const double radperday = 2 * PI / daysperyear;
+    const double radday = dy * radperday;
+    z.dec_sin = std::sin(obliq * deg2rad) * std::sin(radday);
+    const double dec = std::asin(z.dec_sin);
+    z.dec_cos = std::cos(dec);
+    z.dec_tan = std::tan(dec);
+    hipError_t e = launch_zenith(n, lat, lon, z, cosz, (hipStream_t)stream);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("compute_zenith: ") + hipGetErrorString(e));
+    *dyofyr = dy;
+    return 0;
+}
+
+int rrtmg_hip_interp_temp(int nlon, int nlat, int nlev, const double *z_full, const double *z_half,
+                          const double *t_surf, const double *t, double *t_half, void *stream)
+{
+    if (nlon <= 0 || nlat <= 0 || nlev < 2 || !z_full || !z_half || !t_surf || !t || !t_half)
+        return fail(RRTMG_HIP_EINVAL, "interp_temp: bad arguments (nlev must be at least 2)");
+    if ((double)nlon * nlat * (nlev + 1) >= 2147483647.0)
+        return fail(RRTMG_HIP_ENOTSUP, "interp_temp: window too large for one call");
+    hipError_t e = launch_interp_temp(nlon, nlat, nlev, z_full, z_half, t_surf, t, t_half, (hipStream_t)stream);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("interp_temp: ") + hipGetErrorString(e));
+}
+
+static int check_driver_cfg(const RrtmgHipDriverConfig *c)
+{
+    if (!c) return fail(RRTMG_HIP_EINVAL, "driver: null configuration");
+    if (c->nlon <= 0 || c->nlat <= 0 || c->nlev < 2 || c->lonstep <= 0 || c->nlon % c->lonstep)
+        return fail(RRTMG_HIP_EINVAL, "driver: nlon, nlat positive, nlev >= 2, lonstep positive and dividing nlon");
+    if (c->num_precip < 0) return fail(RRTMG_HIP_EINVAL, "driver: num_precip must not be negative");
+    const double v[] = {c->fixed_water, c->fixed_water_pres, c->fixed_water_lat, c->precip_albedo, c->precip_lat,
+                        c->scale_ozone,  c->o3_val,          c->co2ppmv,          c->h2o_lower_limit,
+                        c->temp_lower_limit, c->temp_upper_limit};
+    for (double x : v)
+        if (bad(x)) return fail(RRTMG_HIP_EINVAL, "driver: non-finite option");
+    if ((double)c->nlon * c->nlat * (c->nlev + 1) >= 2147483647.0)
+        return fail(RRTMG_HIP_ENOTSUP, "driver: window too large for one call");
+    return 0;
+}
+
+int rrtmg_hip_rrtmg_pre(const RrtmgHipDriverConfig *cfg, const double *lat, const double *p_full,
+                        const double *p_half, const double *t, const double *t_half, const double *q,
+                        const double *o3f, const double *albedo, const double *precip, const double *t_surf,
+                        const double *coszen, double *play, double *plev, double *tlay, double *tlev,
+                        double *h2ovmr, double *o3vmr, double *co2vmr, double *albedo_rr, double *tsfc,
+                        double *coszen_rr, void *stream)
+{
+    int rc = check_driver_cfg(cfg);
+    if (rc) return rc;
+    const double *need[] = {lat, p_full, p_half, t, t_half, q, albedo, t_surf, coszen};
+    double *outs[] = {play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, albedo_rr, tsfc, coszen_rr};
+    for (const double *p : need)
+        if (!p) return fail(RRTMG_HIP_EINVAL, "rrtmg_pre: null input array");
+    for (double *p : outs)
+        if (!p) return fail(RRTMG_HIP_EINVAL, "rrtmg_pre: null output array");
+    if (cfg->do_read_ozone && !o3f) return fail(RRTMG_HIP_EINVAL, "rrtmg_pre: do_read_ozone needs o3f");
+    if (cfg->do_precip_albedo && cfg->num_precip > 0 && !precip)
+        return fail(RRTMG_HIP_EINVAL, "rrtmg_pre: do_precip_albedo needs precip");
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if ((rc = W.drv.ensure(sizeof(double) * driver_work_doubles(cfg->nlat, cfg->nlev))))
+        return fail(rc, "rrtmg_pre: device workspace allocation failed");
+    DriverDims d{cfg->nlon, cfg->nlat, cfg->nlev, cfg->lonstep};
+    DriverPre a;
+    a.cf = *cfg;
+    a.lat = lat; a.p_full = p_full; a.p_half = p_half; a.t = t; a.t_half = t_half; a.q = q; a.o3f = o3f;
+    a.albedo = albedo; a.precip = precip; a.t_surf = t_surf; a.coszen = coszen;
+    a.play = play; a.plev = plev; a.tlay = tlay; a.tlev = tlev; a.h2o = h2ovmr; a.o3 = o3vmr; a.co2 = co2vmr;
+    a.albedo_rr = albedo_rr; a.tsfc = tsfc; a.coszen_rr = coszen_rr;
+    hipError_t e = launch_rrtmg_pre(d, a, (double *)W.drv.p, st);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("rrtmg_pre: ") + hipGetErrorString(e));
+}
+
+int rrtmg_hip_rrtmg_post(const RrtmgHipDriverConfig *cfg, const double *swhr, const double *hr,
+                         const double *swuflx, const double *swdflx, const double *uflx, const double *dflx,
+                         double *tdt, double *tdt_rad, double *flux_sw, double *flux_lw, double *tdt_sw,
+                         double *tdt_lw, double *olr, double *isr, void *stream)
+{
+    int rc = check_driver_cfg(cfg);
+    if (rc) return rc;
+    if (!swhr || !hr || !swuflx || !swdflx || !uflx || !dflx || !tdt)
+        return fail(RRTMG_HIP_EINVAL, "rrtmg_post: null array");
+    if (!flux_sw != !flux_lw) return fail(RRTMG_HIP_EINVAL, "rrtmg_post: flux_sw and flux_lw go together");
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if ((rc = W.drv.ensure(sizeof(double) * driver_work_doubles(cfg->nlat, cfg->nlev))))
+        return fail(rc, "rrtmg_post: device workspace allocation failed");
+    DriverDims d{cfg->nlon, cfg->nlat, cfg->nlev, cfg->lonstep};
+    DriverPost a{cfg->do_zm_rad, swhr, hr, swuflx, swdflx, uflx, dflx, tdt, tdt_rad, flux_sw, flux_lw,
+                 tdt_sw, tdt_lw, olr, isr};
+    hipError_t e = launch_rrtmg_post(d, a, (double *)W.drv.p, st);
+    return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("rrtmg_post: ") + hipGetErrorString(e));
 }
 
 int rrtmg_hip_set_checks(int on)

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "../../include/rrtmg_hip.h"
 #include "rrtmg_types.hpp"
 
 namespace rrtmg {
@@ -195,6 +196,37 @@ hipError_t launch_input_checks(const DeviceArray *arrays, int narr, const double
 // derivatives, SW exp(-x) floored at 1e-20 and its derivative, as the smooth
 // mode and the backward evaluate them
 hipError_t launch_smooth_functions(int n, const double *x, double *out, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// MiMA's radiation driver around the RRTM calls (driver.hip)
+struct ZenithScalars {
+    double radsec, dt_pi, radpersec, dec_sin, dec_cos, dec_tan;  // per call, as compute_zenith forms them
+    int dt;
+};
+struct DriverDims {
+    int nlon, nlat, nlev, lonstep;
+};
+typedef RrtmgHipDriverConfig DriverConfigD;
+struct DriverPre {
+    DriverConfigD cf;
+    const double *lat, *p_full, *p_half, *t, *t_half, *q, *o3f, *albedo, *precip, *t_surf, *coszen;
+    double *play, *plev, *tlay, *tlev, *h2o, *o3, *co2, *albedo_rr, *tsfc, *coszen_rr;
+    double *zq = nullptr, *zp = nullptr;  // workspace: zonal-mean q, zonal precipitation sums
+    int *flag = nullptr;                  // workspace: top interface to be replaced
+};
+struct DriverPost {
+    int do_zm_rad;
+    const double *swhr, *hr, *swuflx, *swdflx, *uflx, *dflx;
+    double *tdt, *tdt_rad, *flux_sw, *flux_lw, *tdt_sw, *tdt_lw, *olr, *isr;
+};
+// workspace doubles of launch_rrtmg_pre / launch_rrtmg_post
+inline size_t driver_work_doubles(int nlat, int nlev) { return (size_t)nlat * (nlev + 3) + 1; }
+hipError_t launch_zenith(int n, const double *lat, const double *lon, const ZenithScalars &z, double *cosz,
+                         hipStream_t s);
+hipError_t launch_interp_temp(int nlon, int nlat, int nlev, const double *z_full, const double *z_half,
+                              const double *t_surf, const double *t, double *t_half, hipStream_t s);
+hipError_t launch_rrtmg_pre(const DriverDims &d, DriverPre a, double *work, hipStream_t s);
+hipError_t launch_rrtmg_post(const DriverDims &d, const DriverPost &a, double *work, hipStream_t s);
 
 // mode (the C ABI's precision): 0 FP64 everywhere; 1 FP32 with FP64 islands;
 // 2 FP64 with exp(-x) and the tau transition function evaluated directly

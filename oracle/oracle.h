@@ -109,6 +109,33 @@ double oracle_heatfac(int lw);
 /* Threads used by the column loops (OpenMP); returns the previous value. */
 int oracle_set_threads(int n);
 
+/* ---- MiMA's radiation driver around rrtmg_sw/rrtmg_lw (oracle_driver.c).
+ * GCM arrays: (nlon, nlat, nlev) Fortran order, longitude fastest, level 1
+ * at the top; RRTMG arrays (ncol, nlay), ncol = nlon*nlat/lonstep. */
+typedef struct {
+    int nlon, nlat, nlev, lonstep;
+    int do_zm_tracers, do_fixed_water, do_precip_albedo, num_precip, do_read_ozone, do_zm_rad;
+    double fixed_water, fixed_water_pres, fixed_water_lat, precip_albedo, precip_lat;
+    double scale_ozone, o3_val, co2ppmv, h2o_lower_limit, temp_lower_limit, temp_upper_limit;
+} OracleDriverConfig;
+
+/* compute_zenith (RR/astro.f90:59-248) */
+int oracle_compute_zenith(int n, const double *lat, const double *lon, int seconds, int days, int daysperyear,
+                          double equinox_day, double obliq, int dt, double *cosz, int *dyofyr);
+/* interp_temp (RR/rrtm_radiation.f90:422-461) */
+int oracle_interp_temp(int nlon, int nlat, int nlev, const double *z_full, const double *z_half,
+                       const double *t_surf, const double *t, double *t_half);
+/* run_rrtmg before the RRTM calls (RR/rrtm_radiation.f90:581-677) */
+int oracle_rrtmg_pre(const OracleDriverConfig *cf, const double *lat, const double *p_full, const double *p_half,
+                     const double *t, const double *t_half, const double *q, const double *o3f,
+                     const double *albedo, const double *precip, const double *t_surf, const double *coszen,
+                     double *play, double *plev, double *tlay, double *tlev, double *h2ovmr, double *o3vmr,
+                     double *co2vmr, double *albedo_rr, double *tsfc, double *coszen_rr);
+/* run_rrtmg after the RRTM calls (RR/rrtm_radiation.f90:715-826) */
+int oracle_rrtmg_post(const OracleDriverConfig *cf, const double *swhr, const double *hr, const double *swuflx,
+                      const double *swdflx, const double *uflx, const double *dflx, double *tdt, double *tdt_rad,
+                      double *flux_sw, double *flux_lw, double *tdt_sw, double *tdt_lw, double *olr, double *isr);
+
 #ifdef __cplusplus
 }
 #endif

@@ -197,3 +197,83 @@ class lw_tables:
         if lib().oracle_lw_init(LW_TABLE.encode(), ctypes.c_double(CPDAIR)) != 0:
             raise RuntimeError("oracle_lw_init failed")
         return False
+
+
+# ---- MiMA's radiation driver (oracle_driver.c) -------------------------------
+
+class DriverConfig(ctypes.Structure):
+    """OracleDriverConfig / RrtmgHipDriverConfig (the same layout)."""
+    _fields_ = [(n, ctypes.c_int) for n in ("nlon", "nlat", "nlev", "lonstep", "do_zm_tracers", "do_fixed_water",
+                                            "do_precip_albedo", "num_precip", "do_read_ozone", "do_zm_rad")] + \
+               [(n, ctypes.c_double) for n in ("fixed_water", "fixed_water_pres", "fixed_water_lat", "precip_albedo",
+                                               "precip_lat", "scale_ozone", "o3_val", "co2ppmv", "h2o_lower_limit",
+                                               "temp_lower_limit", "temp_upper_limit")]
+
+
+def driver_config(nlon, nlat, nlev, **kw):
+    """rrtm_radiation_nml defaults (RR/rrtm_radiation.f90:111-173) with overrides."""
+    d = dict(nlon=nlon, nlat=nlat, nlev=nlev, lonstep=1, do_zm_tracers=0, do_fixed_water=0, do_precip_albedo=0,
+             num_precip=0, do_read_ozone=0, do_zm_rad=0, fixed_water=2.e-06, fixed_water_pres=100.e02,
+             fixed_water_lat=90.0, precip_albedo=0.35, precip_lat=0.0, scale_ozone=1.0, o3_val=0.0, co2ppmv=300.0,
+             h2o_lower_limit=2.e-7, temp_lower_limit=100.0, temp_upper_limit=370.0)
+    d.update(kw)
+    return DriverConfig(**d)
+
+
+def compute_zenith(lat, lon, seconds, days, daysperyear=360, equinox_day=0.25, obliq=23.439, dt=0):
+    lat, lon = _f64(lat), _f64(lon)
+    out = np.zeros(lat.shape)
+    dy = ctypes.c_int(0)
+    rc = lib().oracle_compute_zenith(int(lat.size), _p(lat), _p(lon), int(seconds), int(days), int(daysperyear),
+                                     ctypes.c_double(equinox_day), ctypes.c_double(obliq), int(dt), _p(out),
+                                     ctypes.byref(dy))
+    if rc != 0:
+        raise RuntimeError("oracle_compute_zenith returned %d" % rc)
+    return out, dy.value
+
+
+def interp_temp(z_full, z_half, t_surf, t):
+    """Arrays (nlev, nlat, nlon) (i.e. Fortran (nlon, nlat, nlev)); returns t_half (nlev+1, nlat, nlon)."""
+    nlev, nlat, nlon = t.shape
+    out = np.zeros((nlev + 1, nlat, nlon))
+    rc = lib().oracle_interp_temp(nlon, nlat, nlev, _p(_f64(z_full)), _p(_f64(z_half)), _p(_f64(t_surf)),
+                                  _p(_f64(t)), _p(out))
+    if rc != 0:
+        raise RuntimeError("oracle_interp_temp returned %d" % rc)
+    return out
+
+
+PRE_OUT = ("play", "plev", "tlay", "tlev", "h2ovmr", "o3vmr", "co2vmr", "albedo", "tsfc", "coszen")
+POST_OUT = ("tdt", "tdt_rad", "flux_sw", "flux_lw", "tdt_sw", "tdt_lw", "olr", "isr")
+
+
+def rrtmg_pre(cf, lat, p_full, p_half, t, t_half, q, o3f, albedo, precip, t_surf, coszen):
+    """GCM arrays as numpy (nlev[, +1], nlat, nlon) / (nlat, nlon); returns the RRTM inputs (nlay[+1], ncol)."""
+    ncol = cf.nlon * cf.nlat // cf.lonstep
+    L = cf.nlev
+    outs = [np.zeros((L, ncol)), np.zeros((L + 1, ncol)), np.zeros((L, ncol)), np.zeros((L + 1, ncol)),
+            np.zeros((L, ncol)), np.zeros((L, ncol)), np.zeros((L, ncol)), np.zeros(ncol), np.zeros(ncol),
+            np.zeros(ncol)]
+    opt = lambda a: None if a is None else _p(_f64(a))  # noqa: E731
+    keep = [_f64(a) for a in (lat, p_full, p_half, t, t_half, q, albedo, t_surf, coszen)]
+    rc = lib().oracle_rrtmg_pre(ctypes.byref(cf), *[_p(a) for a in keep[:6]], opt(o3f), _p(keep[6]), opt(precip),
+                                _p(keep[7]), _p(keep[8]), *[_p(o) for o in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_pre returned %d" % rc)
+    return dict(zip(PRE_OUT, outs))
+
+
+def rrtmg_post(cf, swhr, hr, swuflx, swdflx, uflx, dflx, tdt):
+    """RRTM outputs (nlay[+1], ncol); tdt (nlev, nlat, nlon) is not modified: the
+    updated tendency is returned with the other outputs."""
+    L, si, sj = cf.nlev, cf.nlon, cf.nlat
+    out = {"tdt": _f64(tdt).copy()}
+    for k in ("tdt_rad", "tdt_sw", "tdt_lw"):
+        out[k] = np.zeros((L, sj, si))
+    for k in ("flux_sw", "flux_lw", "olr", "isr"):
+        out[k] = np.zeros((sj, si))
+    ins = [_f64(a) for a in (swhr, hr, swuflx, swdflx, uflx, dflx)]
+    rc = lib().oracle_rrtmg_post(ctypes.byref(cf), *[_p(a) for a in ins], *[_p(out[k]) for k in POST_OUT])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_post returned %d" % rc)
+    return out
