@@ -4,10 +4,12 @@
 (K/day, ``(nlay, ncol)``) and the net downward LW and SW surface fluxes (W/m2,
 ``(ncol,)``) as functions of layer temperature, specific humidity, ozone and
 surface temperature.  The interface temperatures are formed from ``tlay`` and
-``tsfc`` the way MiMA's driver forms them (``interp_temp``,
-``RR/rrtm_radiation.f90:422-461``: linear in ln p between full levels, top
+``tsfc`` the way MiMA forms them (``interp_temp``,
+``RR/rrtm_radiation.f90:422-461``, linear in height between full levels on
+the hydrostatic heights of ``compute_geopotential``,
+``atmos_spectral/model/press_and_geopot.f90:267-311``; top
 ``0.5 (3 T_top - T_below)``, surface ``T_s``), in torch, so autograd carries
-that step.  The forward calls the HIP library's rrtmg_sw / rrtmg_lw; the
+that step, including the dependence of the heights on temperature.  The forward calls the HIP library's rrtmg_sw / rrtmg_lw; the
 backward calls its hand-written backward kernels (rrtmg_hip_sw_vjp,
 rrtmg_hip_lw_vjp), which recompute the optics instead of storing a tape.
 
@@ -20,14 +22,21 @@ from . import rrtmg
 
 
 def mima_tlev(tlay, tsfc, play, plev):
-    """Interface temperatures (nlay+1, ncol) from layer temperatures and Ts."""
+    """Interface temperatures (nlay+1, ncol) from layer temperatures and Ts
+    (surface-first arrays, pressures in any unit).
+
+    MiMA's heights are hydrostatic with the dry layer temperature: a layer's
+    full level lies (Rd/g) T ln(p_half_below / p_full) above its lower
+    interface and (Rd/g) T ln(p_full / p_half_above) below its upper one, so
+    interp_temp's weight of each neighbour at an interior interface is the
+    other neighbour's half-layer thickness over their sum (Rd/g cancels)."""
     lpf = torch.log(play)
     lph = torch.log(plev)
     nlay = tlay.shape[0]
-    dz2 = 1.0 / (lpf[:-1] - lpf[1:])
-    w_below = (lph[1:nlay] - lpf[1:]) * dz2
-    w_above = (lpf[:-1] - lph[1:nlay]) * dz2
-    interior = tlay[:-1] * w_below + tlay[1:] * w_above
+    h_below = tlay[:-1] * (lpf[:-1] - lph[1:nlay])   # z_half(k) - z_full(k): layer under the interface
+    h_above = tlay[1:] * (lph[1:nlay] - lpf[1:])     # z_full(k-1) - z_half(k): layer over it
+    dz2 = 1.0 / (h_above + h_below)
+    interior = tlay[:-1] * (h_above * dz2) + tlay[1:] * (h_below * dz2)
     top = 0.5 * (3.0 * tlay[-1] - tlay[-2])
     return torch.cat([tsfc[None], interior, top[None]], 0)
 

@@ -399,16 +399,35 @@ def test_library_table_reduction_matches_oracle():
     assert n > 150
 
 
-def test_autograd_level_temperature_rule_matches_generator():
-    # interface temperatures built from layer temperatures and Ts in torch (the
-    # differentiable step in front of the backward kernels) equal the
-    # generator's interp_temp construction
+def test_autograd_level_temperature_rule_is_interp_temp_on_hydrostatic_heights():
+    """The interface temperatures the autograd path builds in torch (the
+    differentiable step in front of the backward kernels) equal the oracle's
+    interp_temp (RR/rrtm_radiation.f90:422-461) applied to heights built as
+    MiMA's compute_geopotential builds them (press_and_geopot.f90:267-311,
+    dry: z_half(k) = z_half(k+1) + Rd/g T(k) (ln p_half(k+1) - ln p_half(k)),
+    z_full(k) = z_half(k+1) + Rd/g T(k) (ln p_half(k+1) - ln p_full(k))),
+    restated here in GCM order.  Interpolating in -ln p instead differs by
+    up to 0.7 K on these columns, which this check would catch."""
     import torch
     from mima_amd.autograd import mima_tlev
     c = make_columns(16, 8, 40, seed=4)
+    L, n = c["nlay"], c["ncol"]
     t = mima_tlev(torch.from_numpy(c["tlay"]), torch.from_numpy(c["tsfc"]), torch.from_numpy(c["play"]),
                   torch.from_numpy(c["plev"])).numpy()
-    np.testing.assert_allclose(t, c["tlev"], rtol=1e-12, atol=0)
+    # GCM order (level 0 at the top), one latitude row of n columns
+    T = c["tlay"][::-1][:, None, :]
+    lpf = np.log(c["play"][::-1])[:, None, :] + np.log(100.0)
+    lph = np.log(c["plev"][::-1])[:, None, :] + np.log(100.0)
+    rg = 287.04 / 9.80
+    zh = np.zeros((L + 1, 1, n))
+    for k in range(L - 1, 0, -1):
+        zh[k] = zh[k + 1] + rg * T[k] * (lph[k + 1] - lph[k])
+    zf = np.array([zh[k + 1] + rg * T[k] * (lph[k + 1] - lpf[k]) for k in range(L)])
+    want = po.interp_temp(zf, zh, c["tsfc"][None, :], T)[::-1, 0, :]
+    np.testing.assert_allclose(t, want, rtol=1e-12, atol=0)
+    lnp = np.abs(t - c["tlev"]).max()           # the generator's -ln p rule
+    print("height rule vs -ln p rule: max %.3f K" % lnp)
+    assert lnp > 1e-3
 
 
 def test_mlp_oracle_gradients_match_finite_differences():
