@@ -76,7 +76,7 @@ struct SideTables {
 // different buffers, so they can overlap on the device.
 struct StreamWork {
     Buffer sw_work, lw_work, sw_stage, lw_stage;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, gemm_ws, flag, drv;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, mlp_pack, gemm_ws, flag, drv;
 };
 
 struct DeviceState {
@@ -1231,9 +1231,29 @@ int rrtmg_hip_surrogate(int ncol, int nlay, const double *tlay, const double *h2
     std::lock_guard<std::mutex> lk(s->mu);
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
+    int rc;
+    if (!std::getenv("RRTMG_HIP_MLP_UNFUSED") && mlp_fused_fits(net->ldx, net->nlayers, net->width, net->nout)) {
+        // one kernel: features, every layer and the decode, activations in
+        // LDS; the weights first repacked in the order its MFMAs read them
+        size_t wbytes = 0;
+        for (int l = 0; l < net->nlayers; ++l) wbytes += align_up((size_t)width[l] * width[l + 1] * sizeof(uint16_t));
+        if ((rc = W.mlp_pack.ensure(wbytes))) return fail(rc, "surrogate workspace allocation failed");
+        const void *pw[RRTMG_HIP_MLP_MAX_LAYERS];
+        hipError_t e = hipSuccess;
+        char *p = (char *)W.mlp_pack.p;
+        for (int l = 0; l < net->nlayers && e == hipSuccess; ++l) {
+            e = launch_mlp_pack(width[l + 1], width[l], (const uint16_t *)net->w[l], (uint16_t *)p, st);
+            pw[l] = p;
+            p += align_up((size_t)width[l] * width[l + 1] * sizeof(uint16_t));
+        }
+        if (e == hipSuccess)
+            e = launch_mlp_infer_fused(ncol, nlay, tlay, h2ovmr, o3vmr, tsfc, coszen, net->ldx, net->nlayers,
+                                       net->width, pw, net->b, net->in_mean, net->in_istd, net->in_lo, net->in_hi,
+                                       net->out_mean, net->out_std, hr_lw, hr_sw, out_of_envelope, st);
+        return e == hipSuccess ? 0 : fail(RRTMG_HIP_EDEVICE, std::string("surrogate: ") + hipGetErrorString(e));
+    }
     const size_t act = align_up((size_t)ncol * wmax * sizeof(uint16_t));
     const size_t outb = align_up((size_t)ncol * width[net->nlayers] * sizeof(float));
-    int rc;
     if ((rc = W.mlp_work.ensure(2 * act + outb))) return fail(rc, "surrogate workspace allocation failed");
     char *wb = (char *)W.mlp_work.p;
     uint16_t *buf[2] = {(uint16_t *)wb, (uint16_t *)(wb + act)};

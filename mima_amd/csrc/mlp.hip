@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdint>
 
+#include "../../include/rrtmg_hip.h"
 #include "mlp.hpp"
 
 namespace rrtmg {
@@ -30,6 +31,18 @@ __device__ __forceinline__ uint16_t f2bf(float f)
     uint32_t u = __float_as_uint(f);
     u += 0x7fffu + ((u >> 16) & 1u);
     return (uint16_t)(u >> 16);
+}
+
+// raw feature f of a column from its loaded input value: T, ln q, O3 x 1e6,
+// Ts, cos zenith (ln q in FP32: the feature is rounded to FP32 and then to
+// BF16 after normalisation, and the FP32 logarithm is a single instruction
+// sequence where the FP64 one is a long polynomial)
+__device__ __forceinline__ float mlp_feature(int f, int nlay, double v)
+{
+    if (f < nlay) return (float)v;
+    if (f < 2 * nlay) return logf((float)v);
+    if (f < 3 * nlay) return (float)(v * 1.e6);
+    return (float)v;
 }
 
 constexpr int GT = 64;      // output tile
@@ -456,8 +469,7 @@ __global__ __launch_bounds__(256) void mlp_features_kernel(int ncol, int nlay, c
             if (f >= ldx) break;
             float r = 0.f;
             if (f < nf && col < ncol) {
-                const double x = f < nlay ? v[u] : f < 2 * nlay ? log(v[u]) : f < 3 * nlay ? v[u] * 1.e6 : v[u];
-                r = (float)x;
+                r = mlp_feature(f, nlay, v[u]);
                 if (lo && (r < s_lo[f] || r > s_hi[f])) out = 1;
             }
             tile[c * ld + f] = r;
@@ -606,6 +618,252 @@ hipError_t launch_mlp_decode(int ncol, int nlay, const float *y, int ldy, const 
 {
     hipLaunchKernelGGL(mlp_decode_kernel, dim3((ncol + 63) / 64), dim3(256), sizeof(float) * (64 * (2 * nlay + 1) + 4 * nlay), s,
                        ncol, nlay, y, ldy, mean, std_, hr_lw, hr_sw);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Inference of the whole network in one kernel: a workgroup of four waves
+// takes 64 columns, forms their features in LDS (BF16, normalised, with the
+// envelope test), runs every layer with its activations kept in LDS and the
+// weights read from L2, and writes the decoded heating rates.  Nothing but
+// the inputs, the weights and the outputs touches memory.
+//
+// MFMA: gfx950's v_mfma_f32_16x16x32_bf16 (8 BF16 per lane and operand): lane
+// l holds row (A) or column (B) l % 16 and k = 8 (l / 16) .. 8 (l / 16) + 7;
+// the 16 x 16 FP32 result holds column l % 16, rows 4 (l / 16) + 0..3.  Each
+// wave computes all four 16-row tiles of the 64 rows for the output column
+// tiles wave, wave + 4, ... (at most four: widths up to 256), so the weight
+// fragments of a K step are read once per workgroup and the activation
+// fragments once per wave.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+constexpr int FR = 64;            // columns (rows of the GEMMs) per workgroup
+constexpr int FMAXW = 256;        // largest layer width
+constexpr int FLDA = FMAXW + 8;   // activation row pitch (bf16): 528 bytes, 16 lanes on distinct banks
+constexpr int FKS = FMAXW / 32;   // K steps of the widest layer input
+
+struct FusedNet {
+    int nlay, nin, nout, ldx, nlayers;
+    int width[RRTMG_HIP_MLP_MAX_LAYERS + 1];  // width[0] = ldx, width[l + 1] = layer l's outputs
+    const uint16_t *w[RRTMG_HIP_MLP_MAX_LAYERS];
+    const float *b[RRTMG_HIP_MLP_MAX_LAYERS];
+    const float *in_mean, *in_istd, *in_lo, *in_hi, *out_mean, *out_std;
+};
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void mlp_infer_fused_kernel(int ncol, const double *__restrict__ tlay,
+                                                              const double *__restrict__ h2o,
+                                                              const double *__restrict__ o3,
+                                                              const double *__restrict__ tsfc,
+                                                              const double *__restrict__ coszen, FusedNet net,
+                                                              double *__restrict__ hr_lw, double *__restrict__ hr_sw,
+                                                              int *__restrict__ flag)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char fsm[];
+    uint16_t *const act0 = (uint16_t *)fsm;
+    uint16_t *const act1 = act0 + FR * FLDA;
+    float *s_mean = (float *)(fsm + 2 * FR * FLDA * sizeof(uint16_t));
+    float *s_istd = s_mean + FMAXW, *s_lo = s_istd + FMAXW, *s_hi = s_lo + FMAXW;
+    int *fl = (int *)(s_hi + FMAXW);
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int col0 = blockIdx.x * FR;
+    const int nlay = net.nlay, nf = net.nin, ldx = net.ldx;
+    for (int f = t; f < ldx; f += 256) {
+        s_mean[f] = net.in_mean[f];
+        s_istd[f] = net.in_istd[f];
+        s_lo[f] = net.in_lo ? net.in_lo[f] : 0.f;
+        s_hi[f] = net.in_lo ? net.in_hi[f] : 0.f;
+    }
+    if (t < FR) fl[t] = 0;
+    const int lr = lane & 15, lk = 8 * (lane >> 4);
+    // weight fragments of one layer for this wave: column tile j = wave + 4 j'
+    // (tiles beyond the layer read the last tile; their MFMAs are skipped),
+    // K step st; every load unconditional (K steps beyond win re-read the
+    // last one), so the compiler waits for them only at their first use
+    bf16x8 bw[4][FKS];
+    auto wload = [&](int l) {
+        // net.w[l] holds the layer's weights in fragment order (pack kernel
+        // below): [column tile][K step][lane][8], one contiguous 1 KB per
+        // wave and load
+        const int nst = net.width[l] / 32, nct = net.width[l + 1] / 16;
+        const bf16x8 *__restrict__ P = (const bf16x8 *)net.w[l];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int ct = min(wave + 4 * j, nct - 1);
+#pragma unroll
+            for (int st = 0; st < FKS; ++st) bw[j][st] = P[((size_t)ct * nst + min(st, nst - 1)) * 64 + lane];
+        }
+    };
+    __syncthreads();
+    // features (as mlp_features_kernel forms them): thread (c, q) takes
+    // features q, q + 4, ...; eight loads in flight before any transform
+    {
+        const int c = t & 63, q = t >> 6, col = col0 + c;
+        const bool live = col < ncol;
+        auto raw = [&](int f) -> double {
+            const int part = f < nlay ? 0 : f < 2 * nlay ? 1 : f < 3 * nlay ? 2 : f == 3 * nlay ? 3 : 4;
+            const double *base = part == 0 ? tlay : part == 1 ? h2o : part == 2 ? o3 : part == 3 ? tsfc : coszen;
+            const size_t off = part < 3 ? (size_t)(f - part * nlay) * ncol + col : (size_t)col;
+            return base[f < nf && live ? off : 0];
+        };
+        int out = 0;
+        // a thread's features q, q + 4, ..., all 32 loads in flight at once
+        // (ldx <= 128; wider feature sets in further batches)
+        for (int f0 = q; f0 < ldx; f0 += 128) {
+            double v[32];
+#pragma unroll
+            for (int u = 0; u < 32; ++u) v[u] = raw(f0 + 4 * u);
+#pragma unroll
+            for (int u = 0; u < 32; ++u) {
+                const int f = f0 + 4 * u;
+                if (f >= ldx) continue;
+                float z = 0.f;
+                if (f < nf && live) {
+                    const float r = mlp_feature(f, nlay, v[u]);
+                    if (net.in_lo && (r < s_lo[f] || r > s_hi[f])) out = 1;
+                    z = (r - s_mean[f]) * s_istd[f];
+                }
+                act0[c * FLDA + f] = f2bf(z);
+            }
+        }
+        if (out) atomicOr(&fl[c], 1);
+    }
+    wload(0);
+    __syncthreads();
+    int cur = 0;
+    float *y = nullptr;  // the last layer's FP32 outputs, [FR][nout + 1], in the buffer it does not read
+    const int ldy = net.nout + 1;
+    for (int l = 0; l < net.nlayers; ++l) {
+        const int win = net.width[l], wout = net.width[l + 1], nct = wout / 16;
+        const bool last = l == net.nlayers - 1;
+        const uint16_t *in = cur ? act1 : act0;
+        f32x4v acc[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[j][r] = (f32x4v){0.f, 0.f, 0.f, 0.f};
+        // the wave's weight fragments of the whole layer are in registers
+        // (loaded before the previous layer's epilogue, or before the
+        // features for layer 0): one L2 round trip per layer
+#pragma unroll
+        for (int st = 0; st < FKS; ++st) {
+            if (st * 32 >= win) break;  // uniform
+            bf16x8 a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = *(const bf16x8 *)(in + (r * 16 + lr) * FLDA + st * 32 + lk);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (wave + 4 * j >= nct) break;  // uniform per wave
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[j][r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r], bw[j][st], acc[j][r], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave is done reading `in` (the next buffer may alias it)
+        uint16_t *outb = cur ? act0 : act1;
+        if (last) y = (float *)outb;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int ct = wave + 4 * j;
+            if (ct >= nct) break;
+            const int n = ct * 16 + lr;
+            const float bias = net.b[l][n];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int m = r * 16 + 4 * (lane >> 4) + e;
+                    float v = acc[j][r][e] + bias;
+                    if (last) {
+                        y[m * ldy + n] = v;
+                    } else {
+                        v = v > 0.f ? v : 0.f;
+                        outb[m * FLDA + n] = f2bf(v);
+                    }
+                }
+        }
+        if (!last) wload(l + 1);  // the next layer's weights, in flight across the barrier
+        __syncthreads();
+        cur = 1 - cur;
+    }
+    // decode: (hr_lw, hr_sw) level-major FP64; lanes over columns, coalesced
+    {
+        const int c = t & 63, q = t >> 6, col = col0 + c;
+        if (col < ncol) {
+            for (int f = q; f < net.nout; f += 4) {
+                const double h = (double)(y[c * ldy + f] * net.out_std[f] + net.out_mean[f]);
+                if (f < nlay) hr_lw[(size_t)f * ncol + col] = h;
+                else hr_sw[(size_t)(f - nlay) * ncol + col] = h;
+            }
+            if (flag && q == 0) flag[col] = fl[c];
+        }
+    }
+}
+
+// W [wout][win] (row-major, K-contiguous) -> P [wout/16][win/32][64][8]:
+// element e of lane l of (column tile ct, K step st) is
+// W[ct*16 + l%16][st*32 + 8*(l/16) + e], the B operand of the fused kernel's
+// v_mfma_f32_16x16x32_bf16 for that tile and step
+__global__ __launch_bounds__(256) void mlp_pack_kernel(int wout, int win, const uint16_t *__restrict__ W,
+                                                       uint16_t *__restrict__ P)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // one 8-element fragment row per thread
+    const int nst = win / 32;
+    if (i >= (size_t)(wout / 16) * nst * 64) return;
+    const int lane = (int)(i % 64), st = (int)((i / 64) % nst), ct = (int)(i / 64 / nst);
+    const uint16_t *src = W + (size_t)(ct * 16 + lane % 16) * win + st * 32 + 8 * (lane / 16);
+    uint16_t *dst = P + i * 8;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) dst[e] = src[e];
+}
+
+hipError_t launch_mlp_pack(int wout, int win, const uint16_t *W, uint16_t *P, hipStream_t s)
+{
+    const size_t n = (size_t)(wout / 16) * (win / 32) * 64;
+    hipLaunchKernelGGL(mlp_pack_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, wout, win, W, P);
+    return hipGetLastError();
+}
+
+bool mlp_fused_fits(int ldx, int nlayers, const int *width, int nout)
+{
+    // every layer's input width a multiple of 32 (whole K steps), every
+    // output width a multiple of 16 (whole column tiles), all at most 256
+    if (nlayers < 1 || nlayers > RRTMG_HIP_MLP_MAX_LAYERS || ldx % 32 || ldx > FMAXW) return false;
+    for (int l = 0; l < nlayers; ++l)
+        if (width[l] < 16 || width[l] % (l + 1 < nlayers ? 32 : 16) || width[l] > FMAXW) return false;
+    // the last layer's FP32 outputs, [FR][nout + 1], reuse one BF16 activation buffer
+    return nout <= width[nlayers - 1] && (size_t)FR * (nout + 1) * sizeof(float) <= (size_t)FR * FLDA * 2;
+}
+
+size_t mlp_fused_lds_bytes() { return 2 * (size_t)FR * FLDA * sizeof(uint16_t) + 4 * FMAXW * sizeof(float) + FR * sizeof(int); }
+
+hipError_t launch_mlp_infer_fused(int ncol, int nlay, const double *tlay, const double *h2o, const double *o3,
+                                  const double *tsfc, const double *coszen, int ldx, int nlayers, const int *width,
+                                  const void *const *w, const float *const *b, const float *in_mean,
+                                  const float *in_istd, const float *in_lo, const float *in_hi,
+                                  const float *out_mean, const float *out_std, double *hr_lw, double *hr_sw,
+                                  int *flag, hipStream_t s)
+{
+    FusedNet n{};
+    n.nlay = nlay;
+    n.nin = 3 * nlay + 2;
+    n.nout = 2 * nlay;
+    n.ldx = ldx;
+    n.nlayers = nlayers;
+    n.width[0] = ldx;
+    for (int l = 0; l < nlayers; ++l) {
+        n.width[l + 1] = width[l];
+        n.w[l] = (const uint16_t *)w[l];
+        n.b[l] = b[l];
+    }
+    n.in_mean = in_mean;
+    n.in_istd = in_istd;
+    n.in_lo = in_lo;
+    n.in_hi = in_hi;
+    n.out_mean = out_mean;
+    n.out_std = out_std;
+    hipLaunchKernelGGL(mlp_infer_fused_kernel, dim3((ncol + FR - 1) / FR), dim3(256), mlp_fused_lds_bytes(), s, ncol,
+                       tlay, h2o, o3, tsfc, coszen, n, hr_lw, hr_sw, flag);
     return hipGetLastError();
 }
 

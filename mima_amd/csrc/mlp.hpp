@@ -41,6 +41,17 @@ hipError_t launch_mlp_features(int ncol, int nlay, const double *tlay, const dou
                                hipStream_t s);
 hipError_t launch_mlp_targets(int ncol, int nlay, const double *hr_lw, const double *hr_sw, const float *mean,
                               const float *istd, int ldt, float *t, hipStream_t s);
+// the whole inference in one kernel (features, every layer, decode) for
+// networks whose layer inputs are multiples of 32 and outputs multiples of 16,
+// all at most 256; its weights w[l] in fragment order (launch_mlp_pack)
+bool mlp_fused_fits(int ldx, int nlayers, const int *width, int nout);
+hipError_t launch_mlp_pack(int wout, int win, const uint16_t *W, uint16_t *P, hipStream_t s);
+hipError_t launch_mlp_infer_fused(int ncol, int nlay, const double *tlay, const double *h2o, const double *o3,
+                                  const double *tsfc, const double *coszen, int ldx, int nlayers, const int *width,
+                                  const void *const *w, const float *const *b, const float *in_mean,
+                                  const float *in_istd, const float *in_lo, const float *in_hi,
+                                  const float *out_mean, const float *out_std, double *hr_lw, double *hr_sw,
+                                  int *flag, hipStream_t s);
 hipError_t launch_mlp_decode(int ncol, int nlay, const float *y, int ldy, const float *mean, const float *std_,
                              double *hr_lw, double *hr_sw, hipStream_t s);
 

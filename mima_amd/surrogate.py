@@ -229,8 +229,14 @@ class Surrogate:
         hl = torch.empty((L, n), dtype=torch.float64, device=self.device)
         hs = torch.empty((L, n), dtype=torch.float64, device=self.device)
         fl = torch.empty(n, dtype=torch.int32, device=self.device)
-        net = self.net()
-        _ck(rrtmg.lib().rrtmg_hip_surrogate(n, L, *[_p(c[k]) for k in ("tlay", "h2ovmr", "o3vmr", "tsfc", "coszen")],
+        return self.predict_into(c, hl, hs, fl)
+
+    def predict_into(self, c, hl, hs, fl, net=None):
+        """predict() into given outputs, with a prebuilt net() (for callers
+        that run inference in a loop)."""
+        net = self.net() if net is None else net
+        _ck(rrtmg.lib().rrtmg_hip_surrogate(c["ncol"], self.nlay,
+                                            *[_p(c[k]) for k in ("tlay", "h2ovmr", "o3vmr", "tsfc", "coszen")],
                                             ctypes.byref(net), _p(hl), _p(hs), _p(fl), _stream()))
         return hl, hs, fl
 
@@ -275,11 +281,13 @@ def _subset(c, idx):
     return out
 
 
-def hybrid_heating(model, c, check=1024, seed=0):
+def hybrid_heating(model, c, check=1024, seed=0, max_err_kday=None):
     """Heating rates of device columns from the surrogate, with out-of-envelope
     columns recomputed by the exact kernels and the surrogate's error measured
-    against the exact kernels on `check` random columns.  Returns
-    (hr_lw, hr_sw, report)."""
+    against the exact kernels on `check` random columns.  With max_err_kday,
+    a call whose checked columns show a larger error (LW or SW, K/day) is
+    answered by the exact kernels for every column instead
+    (report["error_fallback"]).  Returns (hr_lw, hr_sw, report)."""
     hl, hs, fl = model.predict(c)
     out = torch.nonzero(fl).flatten()
     if out.numel():
@@ -293,5 +301,9 @@ def hybrid_heating(model, c, check=1024, seed=0):
     dl, ds = hl[:, pick] - el, hs[:, pick] - es
     report = {"fallback_columns": int(out.numel()), "checked_columns": int(pick.numel()),
               "rmse_lw_kday": float(dl.pow(2).mean().sqrt()), "rmse_sw_kday": float(ds.pow(2).mean().sqrt()),
-              "max_abs_lw_kday": float(dl.abs().max()), "max_abs_sw_kday": float(ds.abs().max())}
+              "max_abs_lw_kday": float(dl.abs().max()), "max_abs_sw_kday": float(ds.abs().max()),
+              "error_fallback": False}
+    if max_err_kday is not None and max(report["max_abs_lw_kday"], report["max_abs_sw_kday"]) > max_err_kday:
+        hl, hs = exact_heating(c)
+        report["error_fallback"] = True
     return hl, hs, report

@@ -197,6 +197,48 @@ def test_surrogate_learns_and_falls_back_outside_envelope(env):
     _, _, rep = hybrid_heating(s, te, check=512)
     print(rep)
     assert rep["checked_columns"] == 512 and rep["rmse_lw_kday"] < 2.0 * rmse + 0.05
+    assert not rep["error_fallback"]
+    # error-based fallback: a bound below the checked error answers every column exactly
+    ql, qs, rep = hybrid_heating(s, te, check=256, max_err_kday=1e-6)
+    assert rep["error_fallback"]
+    xl, xs = exact_heating(te)
+    assert torch.equal(ql, xl) and torch.equal(qs, xs)
+    _, _, rep = hybrid_heating(s, te, check=256, max_err_kday=1e3)
+    assert not rep["error_fallback"]
+
+
+def test_fused_inference_matches_layer_by_layer_kernels(env):
+    """The one-kernel inference (features, every layer and the decode with
+    activations in LDS, gfx950 16x16x32 BF16 MFMA) against the layer-by-layer
+    GEMM kernels on the same network and columns: the same features and
+    flags exactly, heating rates within BF16 activation rounding (FP32
+    accumulation in another order can move an activation by one BF16 unit)."""
+    rr, torch = env
+    import os
+    from mima_amd.surrogate import Surrogate, exact_heating
+    d = rr.to_device(make_columns(96, 70, 40, seed=14, lapse_range=(4.0, 9.0), rh_range=(0.2, 1.0)))
+    hl, hs = exact_heating(d)
+    s = Surrogate(nlay=40, seed=6)
+    s.set_statistics(d, hl, hs)
+    for l in range(len(s.b32)):
+        s.b32[l].copy_(0.1 * torch.randn(s.b32[l].shape, generator=torch.Generator().manual_seed(10 + l)).cuda())
+    hot = dict(d)
+    hot["tlay"] = d["tlay"].clone()
+    hot["tlay"][:, ::7] += 80.0                      # some columns out of the envelope
+    pl, ps, fl = s.predict(hot)
+    os.environ["RRTMG_HIP_MLP_UNFUSED"] = "1"
+    try:
+        ul, us, ufl = s.predict(hot)
+    finally:
+        del os.environ["RRTMG_HIP_MLP_UNFUSED"]
+    torch.cuda.synchronize()
+    assert torch.equal(fl, ufl) and int(fl.sum()) == (d["ncol"] + 6) // 7
+    om, osd = s.out_mean.double(), s.out_std.double()
+    a = (torch.cat([pl, ps]).T - om) / osd
+    b = (torch.cat([ul, us]).T - om) / osd
+    rel = float((a - b).norm() / b.norm())
+    print("fused vs layer-by-layer: relative %.2e, max %.2e (standardised)" % (rel, float((a - b).abs().max())))
+    assert rel < 1e-3 and float((a - b).abs().max()) < 2e-2
 
 
 @pytest.mark.parametrize("M,N,K", [(256, 256, 1024), (80, 128, 1000), (256, 128, 96)])
