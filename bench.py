@@ -89,6 +89,34 @@ def copy_probe(torch, mib):
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}))
 
 
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo), or the platform's."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or platform.machine()
+
+
+def roofline_entry(achieved, kernel, kernel_ms, ncol, nlay, bcol):
+    """The dominant kernel against the HBM roofline with algorithmic bytes
+    (`frac`), beside what binds it: the VALU issue fraction and the HBM
+    traffic the PMC counters measured for it (profiles/), as a rate."""
+    traffic = traffic_from_profile(kernel, ncol, nlay)
+    valu = valu_issue_from_profile(kernel, ncol, nlay)
+    meas = traffic / (kernel_ms * 1e-3) / 1e9 if traffic else None
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel, "bytes_per_column": bcol,
+            "binding": "VALU issue and load latency, not HBM: valu_issue_frac of the issue slots; the measured "
+                       "traffic (L2 misses, mostly served by the last-level cache) as a rate in measured_GBps",
+            "valu_issue_frac": valu,
+            "measured_GBps": meas, "measured_frac": meas / HBM_PEAK_GBS if meas else None}
+
+
 def valu_issue_from_profile(kernel, ncol, nlay):
     """VALU issue fraction of `kernel`'s FP32 instantiation, 4 SQ_INSTS_VALU /
     (1024 SIMDs x cycles), as measured with rocprofv3 --pmc and recorded in
@@ -432,13 +460,13 @@ def main():
                           % (ncol, nlay)}
         if refrun.available():
             v, nproc, v1 = cpu_reference_baseline(cols)
-            cpu = {"value": v, "unit": "columns/s", "cores": nproc, "kind": "reference",
+            cpu = {"value": v, "unit": "columns/s", "cores": nproc, "cpu_model": cpu_model(), "kind": "reference",
                    "sample": "all %d columns x %d levels of the benchmark grid; reference rrtmg_sw + rrtmg_lw "
                              "(Fortran from the MiMA sources, -O2, default real 8 bytes; LW on the seeded k-tables), "
                              "%d processes over column slices, time = slowest slice" % (ncol, nlay, nproc),
                    "value_per_core": v1, "port": port}
         else:
-            cpu = {"value": port["value"], "unit": "columns/s", "cores": nthr, "kind": "port",
+            cpu = {"value": port["value"], "unit": "columns/s", "cores": nthr, "cpu_model": cpu_model(), "kind": "port",
                    "sample": port["sample"]}
         print("[bench] cpu baseline (%s): %.4g columns/s on %d cores; C oracle %.4g columns/s on %d threads" %
               (cpu["kind"], cpu["value"], cpu["cores"], port["value"], nthr), file=sys.stderr, flush=True)
@@ -465,11 +493,7 @@ def main():
         "max_abs_dhr_kday": float(dhr),
         "max_abs_dflux_wm2": float(dfl),
         "kernel_ms": {names[k]: float(kt[k[0]][k[1]]) for k in names},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profile(names[dom_key], ncol, nlay),
-                     "kernel": names[dom_key], "bytes_per_column": bcol,
-                     "valu_issue_frac": valu_issue_from_profile(names[dom_key], ncol, nlay)},
+        "roofline": roofline_entry(achieved, names[dom_key], kt[dom_key[0]][dom_key[1]], ncol, nlay, bcol),
         "cpu_baseline": cpu,
         "adjoint": adj,
         "surrogate": sur,
