@@ -205,24 +205,22 @@ __global__ __launch_bounds__(256) void sw_vjp_column_kernel(ColumnInputs in, SwV
     sw_column_refs(in.play, w.ipk, in.ncol, in.nlay, col, w.laytrop + col, w.laysolfr);
 }
 
-template <typename Real>
-__global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpWork w,
-                                                         const SwBand *__restrict__ bands, int nwaves, int blk0,
-                                                         int nblocks, int exp_off, const Real *__restrict__ tab,
-                                                         const WaveUnit *__restrict__ units)
+// One band JB of one work unit; blk: the wave's row of the scratch.  The
+// band's g-point and species-parameter counts are compile-time constants, as
+// in the forward band kernel.
+template <int JB, typename Real>
+__device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const SwVjpWork &w, const SwBand &B0,
+                                                 const WaveUnit u, int tile, int blk, int exp_off,
+                                                 const Real *__restrict__ tab, Real *smem)
 {
     using V3 = Dual<Real, 3>;
-    extern __shared__ unsigned char smem_raw[];
-    Real *smem = (Real *)smem_raw;
     const int ncol = in.ncol, nlay = in.nlay;
-    const int blk = xcd_block();
-    if (blk >= nblocks) return;
-    const int tile = (blk0 + blk) / nwaves;
-    const WaveUnit u = units[(blk0 + blk) % nwaves];
-    const SwBand B = bands[u.band];
-    const int jb = u.band + 16;
+    constexpr int jb = JB;
+    constexpr int ng = sw_ng(JB);  // B.ng, checked at init
+    SwBand B = B0;
+    B.nspa = SW_NSPA[JB - 16];
+    B.nspb = SW_NSPB[JB - 16];
     const int lane = threadIdx.x;
-    const int ng = B.ng;
     const int cl = lane / ng, g = lane - cl * ng;
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol && in.coszen[min(col0, ncol - 1)] >= 1.e-10;
@@ -470,6 +468,27 @@ __global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpW
             gpart[2 * n2 + ix] = v[2];
         }
     }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(64) void sw_vjp_band_kernel(ColumnInputs in, SwVjpWork w,
+                                                         const SwBand *__restrict__ bands, int nwaves, int blk0,
+                                                         int nblocks, int exp_off, const Real *__restrict__ tab,
+                                                         const WaveUnit *__restrict__ units)
+{
+    extern __shared__ unsigned char smem_raw[];
+    Real *smem = (Real *)smem_raw;
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
+    const SwBand B = bands[u.band];
+#define BODY(jb)                                                                  \
+    case jb - 16:                                                                 \
+        sw_vjp_band_body<jb, Real>(in, w, B, u, tile, blk, exp_off, tab, smem);   \
+        break;
+    switch (u.band) { SW_BANDS(BODY) }
+#undef BODY
 }
 
 // finish: band sums in FP64, one thread per (layer, column)
