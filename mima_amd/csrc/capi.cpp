@@ -837,7 +837,8 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(sw_vjp_lds_bytes(nlay, rs)))) return rc;
     size_t off = 0;
-    const size_t o_dc = off; off = align_up(off + rs * SWF_COUNT * 4 * n2);
+    const size_t o_dc = off; off = align_up(off + rs * SWF_STRIDE * n2);
+    const size_t o_dd = off; off = align_up(off + rs * SW_DSTRIDE * n2);
     const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
     const size_t o_lt = off; off = align_up(off + sizeof(int) * ncol);
     const size_t o_lsf = off; off = align_up(off + sizeof(int) * 14 * ncol);
@@ -850,6 +851,7 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     char *wb = (char *)W.sw_vjp_work.p;
     SwVjpWork w;
     w.dcoef = wb + o_dc;
+    w.dder = wb + o_dd;
     w.ipk = (int *)(wb + o_ipk);
     w.laytrop = (int *)(wb + o_lt);
     w.laysolfr = (int *)(wb + o_lsf);
@@ -967,7 +969,8 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     const size_t rs = mixed ? sizeof(float) : sizeof(double);
     if ((rc = check_lds(lw_vjp_lds_bytes(nlay, rs)))) return rc;
     size_t off = 0;
-    const size_t o_dc = off; off = align_up(off + rs * LWF_COUNT * 4 * n2);
+    const size_t o_dc = off; off = align_up(off + rs * LWF_STRIDE * n2);
+    const size_t o_dd = off; off = align_up(off + rs * LW_DSTRIDE * n2);
     const size_t o_tlev = off; off = align_up(off + rs * n3);
     const size_t o_ipk = off; off = align_up(off + sizeof(int) * n2);
     const size_t o_ilev = off; off = align_up(off + sizeof(int) * n3);
@@ -989,6 +992,7 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     char *wb = (char *)W.lw_vjp_work.p;
     LwVjpWork w;
     w.dcoef = wb + o_dc;
+    w.dder = wb + o_dd;
     w.tlev = wb + o_tlev;
     w.ipk = (int *)(wb + o_ipk);
     w.ilev = (int *)(wb + o_ilev);

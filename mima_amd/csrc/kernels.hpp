@@ -113,7 +113,8 @@ struct SwGrad {
     double *tlay, *h2o, *o3;
 };
 struct LwVjpWork {
-    void *dcoef = nullptr;    // [nlay][ncol][LWF_COUNT][4] of Real: value and d/dT, d/dq, d/dO3 per field
+    void *dcoef = nullptr;    // [nlay][ncol][LWF_STRIDE] of Real: the field values
+    void *dder = nullptr;     // [nlay][ncol][LW_DSTRIDE] of Real: their nonzero derivatives (LW_DMASK)
     void *tlev = nullptr;     // [nlay+1][ncol] of Real
     int *ipk = nullptr;       // [nlay][ncol]
     int *ilev = nullptr;      // [nlay+1][ncol]
@@ -130,7 +131,8 @@ struct LwVjpWork {
     int *items = nullptr;     // work-item counter of the LDS-table band kernel
 };
 struct SwVjpWork {
-    void *dcoef = nullptr;    // [nlay][ncol][SWF_COUNT][4] of Real: value and d/dT, d/dq, d/dO3 per field
+    void *dcoef = nullptr;    // [nlay][ncol][SWF_STRIDE] of Real: the field values
+    void *dder = nullptr;     // [nlay][ncol][SW_DSTRIDE] of Real: their nonzero derivatives (SW_DMASK)
     int *ipk = nullptr;
     int *laytrop = nullptr;
     int *laysolfr = nullptr;  // [14][ncol]

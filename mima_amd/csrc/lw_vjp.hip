@@ -9,8 +9,8 @@
 // Kernels:
 //  * prep (one thread per (layer, column)): inatm + setcoef in FP64 dual
 //    numbers seeded with (tlay, h2o, o3) of each layer, writing every
-//    per-layer coefficient with its three derivatives ([field][value, d/dT,
-//    d/dq, d/dO3][lay][col]); level Planck indices; and the per-level
+//    per-layer coefficient's value (the forward's record) and its nonzero
+//    derivatives (a second record, LW_DMASK); level Planck indices; and the per-level
 //    radiance cotangents after folding the heating-rate cotangent into the
 //    fluxes; then a per-column kernel for laytrop, the column water and
 //    dry-air totals, precipitable water and surface temperature.
@@ -43,54 +43,50 @@ namespace rrtmg {
 using D3 = Dual<double, 3>;
 
 // value (V = Real) or value and three derivatives (V = Dual<Real, 3>) of one
-// field of a [field][4][lay][col] array; cs = stride between the components.
-// MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3) names the derivatives the field can
-// have; the others are known zeros and are not loaded, so the dual arithmetic
-// on them folds away at compile time.
+// field: v, the field in the cell's value record; d, its first derivative in
+// the cell's derivative record.  MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3, the
+// field's LW_DMASK entry) names the derivatives the field can have; the
+// others are known zeros, not stored, so the dual arithmetic on them folds
+// away at compile time.
 enum : unsigned { DT = 1u, DQ = 2u, DO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct DLoad;
-// (p: the field's [value, d/dT, d/dq, d/dO3] in the cell's record, 16-byte
-// aligned, so the components load together)
 template <typename Real, unsigned MASK>
 struct DLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p) { return p[0]; }
+    __device__ static Real get(const Real *__restrict__ v, const Real *__restrict__) { return v[0]; }
 };
 template <typename Real, unsigned MASK>
 struct DLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ v, const Real *__restrict__ d)
     {
         Dual<Real, 3> r;
-        r.v = p[0];
-        r.d[0] = (MASK & DT) ? p[1] : (Real)0;
-        r.d[1] = (MASK & DQ) ? p[2] : (Real)0;
-        r.d[2] = (MASK & DO) ? p[3] : (Real)0;
+        r.v = v[0];
+        r.d[0] = (MASK & DT) ? d[0] : (Real)0;
+        r.d[1] = (MASK & DQ) ? d[dmask_count(MASK & DT)] : (Real)0;
+        r.d[2] = (MASK & DO) ? d[dmask_count(MASK & (DT | DQ))] : (Real)0;
         return r;
     }
 };
 
-// Which inputs each coefficient depends on (the prep's formulas): fac*, the
-// interpolation fractions and the pressure/temperature scalings on T; column
-// amounts on q through the dry-air column (and O3's on O3); the broadening
-// column on q and O3 (it excludes the listed gases); selffac/forfac (stored
-// multiplied by the water column) on T and q.
+// the layer's coefficients from the backward's records (values only when V
+// is Real: the derivative record is not read)
 template <typename V, typename Real>
 __device__ __forceinline__ void lw_load_layer_d(LwLayer<V> &c, const Real *__restrict__ dco,
-                                                const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
+                                                const Real *__restrict__ dde, const int *__restrict__ ipk, int lay,
+                                                int col, int ncol)
 {
-    // the cell's record, [lay][col][LWF_COUNT][4], 16-byte aligned
-    const Real *rec = (const Real *)__builtin_assume_aligned(dco + (size_t)(lay * ncol + col) * LWF_COUNT * 4, 16);
     const unsigned ix = (unsigned)(lay * ncol + col);
-    (void)nlay;
-#define LDD(f, F, M) c.f = DLoad<V, Real, M>::get(rec + (F) * 4)
-    LDD(fac00, LWF_FAC00, DT); LDD(fac01, LWF_FAC01, DT); LDD(fac10, LWF_FAC10, DT); LDD(fac11, LWF_FAC11, DT);
-    LDD(colh2o, LWF_COLH2O, DQ); LDD(colco2, LWF_COLCO2, DQ); LDD(colo3, LWF_COLO3, DQ | DO);
-    LDD(coln2o, LWF_COLN2O, DQ); LDD(colco, LWF_COLCO, DQ); LDD(colch4, LWF_COLCH4, DQ); LDD(colo2, LWF_COLO2, DQ);
-    LDD(colbrd, LWF_COLBRD, DQ | DO); LDD(selffac, LWF_SELFFAC, DT | DQ); LDD(selffrac, LWF_SELFFRAC, DT);
-    LDD(forfac, LWF_FORFAC, DT | DQ); LDD(forfrac, LWF_FORFRAC, DT); LDD(minorfrac, LWF_MINORFRAC, DT);
-    LDD(scaleminor, LWF_SCALEMINOR, DT); LDD(scaleminorn2, LWF_SCALEMINORN2, DT | DQ | DO);
-    LDD(coldry, LWF_COLDRY, DQ); LDD(pavel, LWF_PAVEL, 0u); LDD(wx1, LWF_WX1, DQ); LDD(wx2, LWF_WX2, DQ);
-    LDD(wx3, LWF_WX3, DQ); LDD(wx4, LWF_WX4, DQ); LDD(tlayfrac, LWF_TLAYFRAC, DT);
+    const Real *vr = (const Real *)__builtin_assume_aligned(dco + (size_t)ix * LWF_STRIDE, 16);
+    const Real *dr = (const Real *)__builtin_assume_aligned(dde + (size_t)ix * LW_DSTRIDE, 16);
+#define LDD(f, F) c.f = DLoad<V, Real, LW_DMASK[F]>::get(vr + (F), dr + der_offset(LW_DMASK, F))
+    LDD(fac00, LWF_FAC00); LDD(fac01, LWF_FAC01); LDD(fac10, LWF_FAC10); LDD(fac11, LWF_FAC11);
+    LDD(colh2o, LWF_COLH2O); LDD(colco2, LWF_COLCO2); LDD(colo3, LWF_COLO3);
+    LDD(coln2o, LWF_COLN2O); LDD(colco, LWF_COLCO); LDD(colch4, LWF_COLCH4); LDD(colo2, LWF_COLO2);
+    LDD(colbrd, LWF_COLBRD); LDD(selffac, LWF_SELFFAC); LDD(selffrac, LWF_SELFFRAC);
+    LDD(forfac, LWF_FORFAC); LDD(forfrac, LWF_FORFRAC); LDD(minorfrac, LWF_MINORFRAC);
+    LDD(scaleminor, LWF_SCALEMINOR); LDD(scaleminorn2, LWF_SCALEMINORN2);
+    LDD(coldry, LWF_COLDRY); LDD(pavel, LWF_PAVEL); LDD(wx1, LWF_WX1); LDD(wx2, LWF_WX2);
+    LDD(wx3, LWF_WX3); LDD(wx4, LWF_WX4); LDD(tlayfrac, LWF_TLAYFRAC);
 #undef LDD
     const int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
@@ -116,18 +112,19 @@ __global__ __launch_bounds__(256) void lw_vjp_prep_kernel(ColumnInputs in, VjpCo
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nlay * ncol) return;
     const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
-    Real *dco = (Real *)w.dcoef;
+    Real *dco = (Real *)w.dcoef, *dde = (Real *)w.dder;
     Real *tlevf = (Real *)w.tlev;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    // one record per (layer, column): [LWF_COUNT][value, d/dT, d/dq, d/dO3]
+    // the value record and the derivative record of the (layer, column)
     auto put = [&](int F, size_t ix, const D3 &x) {
-        const size_t b = (ix * LWF_COUNT + (size_t)F) * 4;
-        dco[b] = (Real)x.v;
-        dco[b + 1] = (Real)x.d[0];
-        dco[b + 2] = (Real)x.d[1];
-        dco[b + 3] = (Real)x.d[2];
+        dco[ix * LWF_STRIDE + F] = (Real)x.v;
+        Real *d = dde + ix * LW_DSTRIDE + der_offset(LW_DMASK, F);
+        const unsigned m = LW_DMASK[F];
+        int k = 0;
+        for (int j = 0; j < 3; ++j)
+            if (m & (1u << j)) d[k++] = (Real)x.d[j];
     };
     if (lay == 0) {
         double tz0 = in.tlev[col];
@@ -353,7 +350,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     Real *__restrict__ s2 = s3 + (size_t)nlay * 64;
     // s3: incoming downward radiance per layer, [nlay][64] in this wave's
     // global scratch row (written in pass A, read back in pass D)
-    const Real *dco = (const Real *)w.dcoef;
+    const Real *dco = (const Real *)w.dcoef, *dde = (const Real *)w.dder;
     const Real *tlevf = (const Real *)w.tlev;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     const Real *tfn_tbl = tfn_off >= 0 ? tab + tfn_off : nullptr;
@@ -413,7 +410,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     {
         LwLayer<Real> c;
         for (int lev = nlay; lev >= 1; --lev) {
-            lw_load_layer_d<Real, Real>(c, dco, w.ipk, lev - 1, col, ncol, nlay);
+            lw_load_layer_d<Real, Real>(c, dco, dde, w.ipk, lev - 1, col, ncol);
             Real taug, plfrac;
             lw_gas<BAND, Real>(B, T, cst, c, lev <= laytrop, taug, plfrac);
             taug = taug + tau_a(lev);
@@ -480,7 +477,7 @@ __device__ __forceinline__ void lw_vjp_band_body(const ColumnInputs &in, const L
     // optical depth and Planck fraction recomputed as dual numbers
     for (int lev = 1; lev <= nlay; ++lev) {
         LwLayer<V3> c;
-        lw_load_layer_d<V3, Real>(c, dco, w.ipk, lev - 1, col, ncol, nlay);
+        lw_load_layer_d<V3, Real>(c, dco, dde, w.ipk, lev - 1, col, ncol);
         V3 tg3, pf3;
         lw_gas<BAND, Real, V3>(B, T, cst, c, lev <= laytrop, tg3, pf3);
         const Real tg = tg3.v + tau_a(lev), pf = pf3.v;

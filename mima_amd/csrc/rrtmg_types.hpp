@@ -149,6 +149,33 @@ enum LwField {
 // column's fields with a few 16-byte loads
 constexpr int LWF_STRIDE = (LWF_COUNT + 3) / 4 * 4;
 
+// The backward's coefficient records: per (layer, column) a value record of
+// LWF_STRIDE / SWF_STRIDE fields (the forward's record) and a derivative
+// record holding only the derivatives a field can have.  Which of (d/dT,
+// d/dq, d/dO3) each field depends on (bits 0, 1, 2; the preps' formulas):
+// the interpolation fractions and the temperature scalings on T; column
+// amounts on q through the dry-air column (O3's also on O3); the broadening
+// column and the N2 minor scaling on q and O3 (they exclude the listed gases,
+// the latter also on T); selffac/forfac (stored times the water column) on T
+// and q.  The derivative record lists each field's components in field order,
+// d/dT before d/dq before d/dO3, padded to a 16-byte multiple.
+constexpr unsigned LW_DMASK[LWF_COUNT] = {1, 1, 1, 1, 2, 2, 6, 2, 2, 2, 2, 6, 3,
+                                          1, 3, 1, 1, 1, 7, 2, 0, 2, 2, 2, 2, 1};
+constexpr unsigned SW_DMASK[SWF_COUNT] = {1, 1, 1, 1, 2, 2, 6, 2, 2, 2, 3, 1, 3, 1};
+constexpr int dmask_count(unsigned m) { return (int)(m & 1u) + (int)((m >> 1) & 1u) + (int)((m >> 2) & 1u); }
+// offset of field f's first derivative in the derivative record
+template <int N>
+constexpr int der_offset(const unsigned (&mask)[N], int f)
+{
+    int o = 0;
+    for (int i = 0; i < f; ++i) o += dmask_count(mask[i]);
+    return o;
+}
+constexpr int LW_DSTRIDE = (der_offset(LW_DMASK, LWF_COUNT) + 3) / 4 * 4;
+constexpr int SW_DSTRIDE = (der_offset(SW_DMASK, SWF_COUNT) + 3) / 4 * 4;
+static_assert(der_offset(LW_DMASK, LWF_COUNT) == 31 && LW_DSTRIDE == 32, "LW derivative record");
+static_assert(der_offset(SW_DMASK, SWF_COUNT) == 17 && SW_DSTRIDE == 20, "SW derivative record");
+
 // packed per-layer integer word: jp | jt<<6 | jt1<<9 | indself<<12 | indfor<<16 | indminor<<18 | indlay<<23
 __host__ __device__ inline int pack_lw(int jp, int jt, int jt1, int inds, int indf, int indm, int indlay)
 {

@@ -6,8 +6,9 @@
 //
 // Kernels:
 //  * prep (one thread per (layer, column)): inatm_sw + setcoef_sw in FP64
-//    dual numbers seeded with (tlay, h2o, o3) of each layer ([field][value,
-//    d/dT, d/dq, d/dO3][lay][col]) and the level cotangents after folding the
+//    dual numbers seeded with (tlay, h2o, o3) of each layer (a value record
+//    and a record of the nonzero derivatives, SW_DMASK, per (layer, column))
+//    and the level cotangents after folding the
 //    heating-rate cotangent into the fluxes; then a per-column kernel for
 //    laytrop and the solar-source reference layers.
 //  * band (one wave per work unit, lane = (column, g) as in the forward):
@@ -38,44 +39,46 @@ namespace rrtmg {
 using D3 = Dual<double, 3>;
 
 // value (V = Real) or value and three derivatives (V = Dual<Real, 3>) of one
-// field; MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3) names the derivatives the field
-// can have, the others are known zeros (not loaded; folded at compile time)
+// field: v, the field in the cell's value record; d, its first derivative in
+// the cell's derivative record.  MASK (bit 0: d/dT, 1: d/dq, 2: d/dO3, the
+// field's SW_DMASK entry) names the derivatives the field can have; the
+// others are known zeros (not stored; folded at compile time)
 enum : unsigned { SDT = 1u, SDQ = 2u, SDO = 4u };
 template <typename V, typename Real, unsigned MASK>
 struct SwDLoad;
-// (p: the field's [value, d/dT, d/dq, d/dO3] in the cell's record, 16-byte
-// aligned, so the components load together)
 template <typename Real, unsigned MASK>
 struct SwDLoad<Real, Real, MASK> {
-    __device__ static Real get(const Real *__restrict__ p) { return p[0]; }
+    __device__ static Real get(const Real *__restrict__ v, const Real *__restrict__) { return v[0]; }
 };
 template <typename Real, unsigned MASK>
 struct SwDLoad<Dual<Real, 3>, Real, MASK> {
-    __device__ static Dual<Real, 3> get(const Real *__restrict__ p)
+    __device__ static Dual<Real, 3> get(const Real *__restrict__ v, const Real *__restrict__ d)
     {
         Dual<Real, 3> r;
-        r.v = p[0];
-        r.d[0] = (MASK & SDT) ? p[1] : (Real)0;
-        r.d[1] = (MASK & SDQ) ? p[2] : (Real)0;
-        r.d[2] = (MASK & SDO) ? p[3] : (Real)0;
+        r.v = v[0];
+        r.d[0] = (MASK & SDT) ? d[0] : (Real)0;
+        r.d[1] = (MASK & SDQ) ? d[dmask_count(MASK & SDT)] : (Real)0;
+        r.d[2] = (MASK & SDO) ? d[dmask_count(MASK & (SDT | SDQ))] : (Real)0;
         return r;
     }
 };
 
+// the layer's coefficients from the backward's records (values only when V
+// is Real: the derivative record is not read)
 template <typename V, typename Real>
 __device__ __forceinline__ void sw_load_layer_d(SwLayer<V> &c, const Real *__restrict__ dco,
-                                                const int *__restrict__ ipk, int lay, int col, int ncol, int nlay)
+                                                const Real *__restrict__ dde, const int *__restrict__ ipk, int lay,
+                                                int col, int ncol)
 {
-    // the cell's record, [lay][col][SWF_COUNT][4], 16-byte aligned
-    const Real *rec = (const Real *)__builtin_assume_aligned(dco + (size_t)(lay * ncol + col) * SWF_COUNT * 4, 16);
     const unsigned ix = (unsigned)(lay * ncol + col);
-    (void)nlay;
-#define LDD(f, F, M) c.f = SwDLoad<V, Real, M>::get(rec + (F) * 4)
-    LDD(fac00, SWF_FAC00, SDT); LDD(fac01, SWF_FAC01, SDT); LDD(fac10, SWF_FAC10, SDT); LDD(fac11, SWF_FAC11, SDT);
-    LDD(colh2o, SWF_COLH2O, SDQ); LDD(colco2, SWF_COLCO2, SDQ); LDD(colo3, SWF_COLO3, SDQ | SDO);
-    LDD(colch4, SWF_COLCH4, SDQ); LDD(colo2, SWF_COLO2, SDQ); LDD(colmol, SWF_COLMOL, SDQ);
-    LDD(selffac, SWF_SELFFAC, SDT | SDQ); LDD(selffrac, SWF_SELFFRAC, SDT); LDD(forfac, SWF_FORFAC, SDT | SDQ);
-    LDD(forfrac, SWF_FORFRAC, SDT);
+    const Real *vr = (const Real *)__builtin_assume_aligned(dco + (size_t)ix * SWF_STRIDE, 16);
+    const Real *dr = (const Real *)__builtin_assume_aligned(dde + (size_t)ix * SW_DSTRIDE, 16);
+#define LDD(f, F) c.f = SwDLoad<V, Real, SW_DMASK[F]>::get(vr + (F), dr + der_offset(SW_DMASK, F))
+    LDD(fac00, SWF_FAC00); LDD(fac01, SWF_FAC01); LDD(fac10, SWF_FAC10); LDD(fac11, SWF_FAC11);
+    LDD(colh2o, SWF_COLH2O); LDD(colco2, SWF_COLCO2); LDD(colo3, SWF_COLO3);
+    LDD(colch4, SWF_COLCH4); LDD(colo2, SWF_COLO2); LDD(colmol, SWF_COLMOL);
+    LDD(selffac, SWF_SELFFAC); LDD(selffrac, SWF_SELFFRAC); LDD(forfac, SWF_FORFAC);
+    LDD(forfrac, SWF_FORFRAC);
 #undef LDD
     const int w = at32(ipk, ix);
     c.jp = unpack_jp(w);
@@ -99,17 +102,18 @@ __global__ __launch_bounds__(256) void sw_vjp_prep_kernel(ColumnInputs in, VjpCo
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (size_t)nlay * ncol) return;
     const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
-    Real *dco = (Real *)w.dcoef;
+    Real *dco = (Real *)w.dcoef, *dde = (Real *)w.dder;
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
     const double avogad = 6.02214199e+23, grav = 9.8066;
     const double stpfac = 296.0 / 1013.0;
-    // one record per (layer, column): [SWF_COUNT][value, d/dT, d/dq, d/dO3]
+    // the value record and the derivative record of the (layer, column)
     auto put = [&](int F, size_t ix, const D3 &x) {
-        const size_t b = (ix * SWF_COUNT + (size_t)F) * 4;
-        dco[b] = (Real)x.v;
-        dco[b + 1] = (Real)x.d[0];
-        dco[b + 2] = (Real)x.d[1];
-        dco[b + 3] = (Real)x.d[2];
+        dco[ix * SWF_STRIDE + F] = (Real)x.v;
+        Real *d = dde + ix * SW_DSTRIDE + der_offset(SW_DMASK, F);
+        const unsigned m = SW_DMASK[F];
+        int k = 0;
+        for (int j = 0; j < 3; ++j)
+            if (m & (1u << j)) d[k++] = (Real)x.d[j];
     };
     {
         const size_t ix = (size_t)lay * ncol + col;
@@ -243,7 +247,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
     Real *__restrict__ d_adj = d_ptdbt + 3 * L1;
     // the bottom-up sweep's prupd (read one level ahead)
     Real *__restrict__ s_prupd = d_adj + 5 * LA;
-    const Real *dco = (const Real *)w.dcoef;
+    const Real *dco = (const Real *)w.dcoef, *dde = (const Real *)w.dder;
     const Real *exp_tbl = exp_off >= 0 ? tab + exp_off : nullptr;
     Tab<Real> T{tab, ng, g};
     const size_t n2 = (size_t)nlay * ncol, n3 = (size_t)(nlay + 1) * ncol;
@@ -260,7 +264,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
         albd = (Real)(nir ? in.aldif[col] : in.asdif[col]);
         lsf = w.laysolfr[(size_t)u.band * ncol + col];
         SwLayer<Real> c;
-        sw_load_layer_d<Real, Real>(c, dco, w.ipk, lsf - 1, col, ncol, nlay);
+        sw_load_layer_d<Real, Real>(c, dco, dde, w.ipk, lsf - 1, col, ncol);
         const double sfz = sw_sfluxzen(jb, B, T, c, lsf <= laytrop);
         zincflx = (Real)(in.adjflux * sfz * (double)prmu0);
     }
@@ -274,7 +278,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
         Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
         if (active) {
             SwLayer<Real> c;
-            sw_load_layer_d<Real, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+            sw_load_layer_d<Real, Real>(c, dco, dde, w.ipk, lay - 1, col, ncol);
             Real taug, taur;
             sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
             sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
@@ -300,7 +304,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
     };
     auto optics_d = [&](int lay, LayerD &o) {
         SwLayer<V3> c;
-        sw_load_layer_d<V3, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+        sw_load_layer_d<V3, Real>(c, dco, dde, w.ipk, lay - 1, col, ncol);
         sw_gas<Real, V3>(jb, B, T, c, lay <= laytrop, o.tg, o.tr);
         // exp(-x) itself rather than the table: the backward differentiates
         // the smooth functions the table stands for (the table's neighbouring
@@ -342,7 +346,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
             Real r = 0, rd = 0, t = 1, td = 1, dbt = 1;
             if (active) {
                 SwLayer<Real> c;
-                sw_load_layer_d<Real, Real>(c, dco, w.ipk, lay - 1, col, ncol, nlay);
+                sw_load_layer_d<Real, Real>(c, dco, dde, w.ipk, lay - 1, col, ncol);
                 Real taug, taur;
                 sw_gas(jb, B, T, c, lay <= laytrop, taug, taur);
                 sw_layer_optics(taug, taur, prmu0, exp_tbl, r, rd, t, td, dbt);
@@ -400,7 +404,7 @@ __device__ __forceinline__ void sw_vjp_band_body(const ColumnInputs &in, const S
     V3 sfz3(0);
     if (active) {
         SwLayer<V3> c;
-        sw_load_layer_d<V3, Real>(c, dco, w.ipk, lsf - 1, col, ncol, nlay);
+        sw_load_layer_d<V3, Real>(c, dco, dde, w.ipk, lsf - 1, col, ncol);
         sfz3 = sw_sfluxzen_v<Real, V3>(jb, B, T, c);
     }
 
