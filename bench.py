@@ -174,8 +174,15 @@ def adjoint_timing(torch, rrtmg, config_columns, args):
 
     res = {}
     for name, fn in (("forward", fwd), ("backward", bwd)):
-        for _ in range(args.warmup):
+        # warm-up: at least args.warmup calls and 0.3 s, so that the GPU's
+        # clocks have settled whichever leg runs first
+        t0 = time.perf_counter()
+        k = 0
+        while k < args.warmup or time.perf_counter() - t0 < 0.3:
             fn()
+            k += 1
+            if k % 10 == 0:
+                torch.cuda.synchronize()
         torch.cuda.synchronize()
         n = max(3, args.steps // 2)
         t0 = time.perf_counter()
@@ -272,10 +279,12 @@ def large_batch_timing(torch, rrtmg, config_columns, args):
     """Large batch: 512 x 256 = 131072 columns x 60 levels, CO2 log-uniform on
     280-1120 ppmv per column, FP32-mixed; forward plus backward of LW+SW
     heating rates and surface net fluxes (cotangents N(0, 1) from a seed).
-    Three ways: the four entry points one after another (forward, then
-    backward), the one-pass entry points (forward formed inside the backward)
-    on one stream, and the one-pass entry points captured in a HIP graph with
-    SW and LW on two streams (the reported value)."""
+    Four ways: the four entry points one after another (forward, then
+    backward); the one-pass entry points (forward formed inside the backward)
+    on one stream; the one-pass entry points captured in a HIP graph with SW
+    and LW on two streams (the reported value, the fastest); and the one-pass
+    calls in a fused session (the band work of both sides in one persistent
+    kernel) captured in a HIP graph."""
     cols = config_columns("c4_large", seed=0)
     ncol, nlay = cols["ncol"], cols["nlay"]
     d = rrtmg.to_device(cols)
@@ -330,17 +339,38 @@ def large_batch_timing(torch, rrtmg, config_columns, args):
         with torch.cuda.graph(graph, stream=main_s):
             one_pass(True)
     t_graph = timed(graph.replay)
-    print("[bench] large batch %d x %d: separate %.2f ms, one pass %.2f ms, one pass in a graph (2 streams) %.2f ms"
-          % (ncol, nlay, t_sep * 1e3, t_one * 1e3, t_graph * 1e3), file=sys.stderr, flush=True)
     del graph
+
+    def fused():
+        return rrtmg.fwd_vjp_fused(d, uflx_bar=ub, dflx_bar=db, hr_bar=hr_lw, swuflx_bar=sub, swdflx_bar=sdb,
+                                   swhr_bar=hr_sw)
+
+    with torch.cuda.stream(main_s):
+        fused()
+        one_kernel = bool(rrtmg.lib().rrtmg_hip_fused_last_kernel())
+        torch.cuda.synchronize()
+        fgraph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(fgraph, stream=main_s):
+            fused()
+    t_fused = timed(fgraph.replay)
+    del fgraph
+    print("[bench] large batch %d x %d: separate %.2f ms, one pass %.2f ms, one pass in a graph (2 streams) %.2f ms, "
+          "fused kernel in a graph %.2f ms" % (ncol, nlay, t_sep * 1e3, t_one * 1e3, t_graph * 1e3, t_fused * 1e3),
+          file=sys.stderr, flush=True)
     return {"workload": "forward + backward of LW+SW heating rates and surface net fluxes wrt T, q, O3, Ts; "
                         "512x256 grid = %d columns x %d levels, CO2 log-uniform 280-1120 ppmv, FP32-mixed"
                         % (ncol, nlay),
             "value": ncol / t_graph, "unit": "columns/s", "ms_per_step": t_graph * 1e3, "steps": n,
             "how": "one-pass forward+backward entry points (rrtmg_hip_sw_fwd_vjp, rrtmg_hip_lw_fwd_vjp) "
                    "captured in one HIP graph, SW and LW on two streams",
+            "fused_kernel_graph_ms": t_fused * 1e3,
+            "fused_kernel_how": "a fused session (rrtmg_hip_fused_begin, rrtmg_hip_sw_fwd_vjp, "
+                                "rrtmg_hip_lw_fwd_vjp, rrtmg_hip_fused_end) captured in one HIP graph: the band "
+                                "work of both sides in one persistent kernel (fused_band_kernel, LW and SW work "
+                                "items from one counter)",
+            "fused_kernel_ran": one_kernel,
             "one_pass_eager_ms": t_one * 1e3, "separate_calls_ms": t_sep * 1e3,
-            "speedup_vs_separate": t_sep / t_graph}
+            "speedup_vs_separate": t_sep / t_graph, "fused_kernel_vs_two_streams": t_fused / t_graph}
 
 
 def main():

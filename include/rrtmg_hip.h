@@ -179,6 +179,30 @@ int rrtmg_hip_sw_fwd_vjp(int ncol, int nlay,
                          double *tlay_bar, double *h2ovmr_bar, double *o3vmr_bar,
                          int ptr_kind, void *stream);
 
+/* Fused session: the one-pass forward + backward of both sides with their
+ * band work in one persistent kernel.  No counterpart in the reference (its
+ * forward and backward are separate, and it has no backward).  Between
+ * rrtmg_hip_fused_begin(stream) and rrtmg_hip_fused_end(stream), device-pointer
+ * calls of rrtmg_hip_lw_vjp / rrtmg_hip_lw_fwd_vjp and rrtmg_hip_sw_vjp /
+ * rrtmg_hip_sw_fwd_vjp (at most one per side) on that stream enqueue their
+ * prep kernels only; rrtmg_hip_fused_end enqueues the band work of both and
+ * then their finish kernels, so their outputs are complete once the stream
+ * reaches that point.  With both sides pending in the FP32-mixed precision
+ * the band work of both runs as one persistent kernel whose workgroups claim
+ * LW and then SW work items from one counter; otherwise (one side, FP64, or
+ * LDS too small for the level count) each side's own band kernels run.
+ * The LW results are bitwise those of the separate calls; the SW results
+ * agree with them to FP32 rounding (the fused kernel calls the SW body as a
+ * function where the SW kernel inlines it).  At 131072 x 60 the single
+ * kernel is slower than the two sides' kernels on two streams (88.1 against
+ * 81.2 ms per step).  Host-pointer calls inside a session return
+ * RRTMG_HIP_EINVAL.  Warmed-up sessions allocate nothing, so a session can
+ * be captured in a HIP graph. */
+int rrtmg_hip_fused_begin(void *stream);
+int rrtmg_hip_fused_end(void *stream);
+/* 1 if the last rrtmg_hip_fused_end ran the single fused kernel, else 0 */
+int rrtmg_hip_fused_last_kernel(void);
+
 /* ---------------------------------------------------------------------------
  * MLP surrogate of the LW+SW heating-rate profile (no reference counterpart;
  * it learns what rrtmg_hip_sw/rrtmg_hip_lw compute and is judged against

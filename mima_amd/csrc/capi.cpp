@@ -74,9 +74,36 @@ struct SideTables {
 // The workspaces of the calls on one stream.  Calls on one stream run in
 // stream order and may share buffers; calls on different streams get
 // different buffers, so they can overlap on the device.
+// a backward call of one side whose band and finish phases wait for
+// rrtmg_hip_fused_end
+struct PendingLw {
+    bool on = false, fwd = false;
+    int mode = 0;
+    ColumnInputs in{};
+    LwGrad g{};
+    LwVjpWork w;
+    LwDevTables t;
+    LwOutputs fo{};
+};
+struct PendingSw {
+    bool on = false, fwd = false;
+    int mode = 0;
+    ColumnInputs in{};
+    SwGrad g{};
+    SwVjpWork w;
+    SwDevTables t;
+    SwOutputs fo{};
+};
+struct FusedSession {
+    bool open = false;
+    PendingLw lw;
+    PendingSw sw;
+};
+
 struct StreamWork {
     Buffer sw_work, lw_work, sw_stage, lw_stage;
-    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, mlp_pack, gemm_ws, flag, drv;
+    Buffer sw_vjp_work, lw_vjp_work, vjp_stage, mlp_work, mlp_pack, gemm_ws, flag, drv, fused_rows;
+    FusedSession fused;
 };
 
 struct DeviceState {
@@ -111,6 +138,8 @@ bool g_timing = false;
 bool g_checks = false;
 std::map<int, DeviceState *> g_dev;
 int g_prec = RRTMG_HIP_PREC_FP64;
+// 1 when the last rrtmg_hip_fused_end ran the fused band kernel
+static int g_fused_last = 0;
 
 DeviceState *state()
 {
@@ -821,6 +850,9 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     }
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
+    const bool fused = W.fused.open;
+    if (fused && host) return fail(RRTMG_HIP_EINVAL, "calls inside a fused session take device pointers");
+    if (fused && W.fused.sw.on) return fail(RRTMG_HIP_EINVAL, "one SW backward call per fused session");
     if (!host && g_checks) {
         std::vector<NamedArray> a;
         for (int i = 0; i < 8; ++i) a.push_back({nm2[i], ins2[i], n2});
@@ -891,6 +923,20 @@ static int sw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     t.tref = s->swd.tref;
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
+    if (fused) {
+        const hipError_t e = launch_sw_vjp_prep(in, cot, w, t, g_prec, st);
+        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward launch: ") + hipGetErrorString(e));
+        PendingSw &P = W.fused.sw;
+        P.on = true;
+        P.fwd = fwd;
+        P.mode = g_prec;
+        P.in = in;
+        P.g = g;
+        P.w = w;
+        P.t = t;
+        P.fo = fo;
+        return 0;
+    }
     hipError_t e = launch_sw_vjp(in, cot, g, w, t, g_prec, st, s->events(2), fwd ? &fo : nullptr);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW backward launch: ") + hipGetErrorString(e));
     if (host) {
@@ -952,6 +998,9 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     }
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
+    const bool fused = W.fused.open;
+    if (fused && host) return fail(RRTMG_HIP_EINVAL, "calls inside a fused session take device pointers");
+    if (fused && W.fused.lw.on) return fail(RRTMG_HIP_EINVAL, "one LW backward call per fused session");
     if (!host && g_checks) {
         std::vector<NamedArray> a;
         for (int i = 0; i < 12; ++i) a.push_back({nm2[i], ins2[i], n2});
@@ -1040,6 +1089,20 @@ static int lw_vjp_impl(int ncol, int nlay, const double *play, const double *ple
     t.bands_lds = s->lwd.bands_lds;
     t.plan = s->lwd.plan;
     t.lds = lds;
+    if (fused) {
+        const hipError_t e = launch_lw_vjp_prep(in, cot, w, t, g_prec, st);
+        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
+        PendingLw &P = W.fused.lw;
+        P.on = true;
+        P.fwd = fwd;
+        P.mode = g_prec;
+        P.in = in;
+        P.g = g;
+        P.w = w;
+        P.t = t;
+        P.fo = fo;
+        return 0;
+    }
     hipError_t e = launch_lw_vjp(in, cot, g, w, t, g_prec, st, s->events(3), fwd ? &fo : nullptr);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW backward launch: ") + hipGetErrorString(e));
     if (host) {
@@ -1114,6 +1177,83 @@ int rrtmg_hip_lw_fwd_vjp(int ncol, int nlay, const double *play, const double *p
                        cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, uflx_bar, dflx_bar, hr_bar, tlay_bar,
                        tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar, uflx, dflx, hr, ptr_kind, stream);
 }
+
+int rrtmg_hip_fused_begin(void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    StreamWork &W = s->work((hipStream_t)stream);
+    if (W.fused.open) return fail(RRTMG_HIP_EINVAL, "a fused session is already open on this stream");
+    W.fused.open = true;
+    W.fused.lw.on = W.fused.sw.on = false;
+    return 0;
+}
+
+int rrtmg_hip_fused_end(void *stream)
+{
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if (!W.fused.open) return fail(RRTMG_HIP_EINVAL, "rrtmg_hip_fused_end without rrtmg_hip_fused_begin");
+    W.fused.open = false;
+    const PendingLw L = W.fused.lw;
+    const PendingSw S = W.fused.sw;
+    W.fused.lw.on = W.fused.sw.on = false;
+    bool fuse = L.on && S.on && L.mode == RRTMG_HIP_PREC_MIXED && S.mode == RRTMG_HIP_PREC_MIXED && L.t.lds;
+    if (fuse) {
+        int dev = 0, lim = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            lim = 65536;
+        fuse = fused_lds_bytes(L.t.plan, S.in.nlay) <= (size_t)lim;
+    }
+    hipError_t e = hipSuccess;
+    g_fused_last = fuse ? 1 : 0;
+    if (fuse) {
+        FusedBand a;
+        a.lin = L.in;
+        a.lw = L.w;
+        a.lbands = L.t.bands_lds;
+        a.plan = L.t.plan;
+        a.lnwaves = L.t.desc.nwaves;
+        const int ltiles = (L.in.ncol + TILE_COLS - 1) / TILE_COLS;
+        a.litems = (ltiles + FUSED_WGW - 1) / FUSED_WGW * a.lnwaves;
+        a.exp_off = L.t.desc.exp_tbl;
+        a.tfn_off = L.t.desc.tfn_tbl;
+        a.plk_off = L.t.desc.totplnk;
+        a.ltab = (const float *)L.t.tab;
+        a.cst = L.t.cst;
+        a.lunits = L.t.units;
+        a.sin = S.in;
+        a.sw = S.w;
+        a.sbands = S.t.bands;
+        a.snwaves = S.t.desc.nwaves;
+        a.snunits = (S.in.ncol + TILE_COLS - 1) / TILE_COLS * a.snwaves;
+        a.sexp_off = S.t.desc.exp_tbl;
+        a.stab = (const float *)S.t.tab;
+        a.sunits = S.t.units;
+        a.row_floats = fused_row_floats(L.in.nlay, S.in.nlay);
+        a.items = L.w.items;
+        a.nitems = a.litems + (a.snunits + FUSED_WGW - 1) / FUSED_WGW;
+        const size_t rows = (size_t)fused_grid(a.plan, a.nitems) * FUSED_WGW;
+        int rc = W.fused_rows.ensure(rows * a.row_floats * sizeof(float));
+        if (rc) return fail(rc, "device workspace allocation failed");
+        a.rows = (float *)W.fused_rows.p;
+        e = launch_fused_band(a, st);
+    } else {
+        if (L.on) e = launch_lw_vjp_band(L.in, L.w, L.t, L.mode, st);
+        if (e == hipSuccess && S.on) e = launch_sw_vjp_band(S.in, S.w, S.t, S.mode, st);
+    }
+    if (e == hipSuccess && L.on) e = launch_lw_vjp_finish(L.in, L.g, L.w, L.t, L.mode, st, L.fwd ? &L.fo : nullptr);
+    if (e == hipSuccess && S.on) e = launch_sw_vjp_finish(S.in, S.g, S.w, S.t, S.mode, st, S.fwd ? &S.fo : nullptr);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("fused band launch: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int rrtmg_hip_fused_last_kernel(void) { return g_fused_last; }
 
 // ---------------------------------------------------------------------------
 // MLP surrogate kernels (device pointers, enqueued on `stream`)

@@ -87,7 +87,7 @@ inline int lw_vjp_lds_grid(const LwLdsPlan &p, int ncol, int nwaves)
     const int nitems = (ntiles + LWV_WGW - 1) / LWV_WGW * nwaves;
     return nitems < p.ncu ? nitems : p.ncu;
 }
-inline size_t lw_vjp_lds_state_bytes(int nlay) { return sizeof(float) * 3 * (size_t)nlay * 64; }
+__host__ __device__ inline size_t lw_vjp_lds_state_bytes(int nlay) { return sizeof(float) * 3 * (size_t)nlay * 64; }
 
 struct LwDevTables {
     LwTables desc;
@@ -158,14 +158,14 @@ inline size_t sw_fwd_state_bytes(int nlay, size_t real_size) { return real_size 
 inline size_t lw_fwd_state_bytes(int nlay, size_t real_size) { return real_size * (size_t)nlay * 64; }
 inline size_t sw_fwd_lds_bytes(int, size_t real_size) { return real_size * 64; }
 inline size_t lw_fwd_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 64); }
-inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
+__host__ __device__ inline size_t sw_vjp_state_bytes(int nlay, size_t real_size)
 {
     return real_size * (4 * (size_t)(nlay + 1) + 5 * (size_t)nlay) * 64;
 }
 inline size_t lw_vjp_state_bytes(int nlay, size_t real_size) { return real_size * 2 * (size_t)nlay * 64; }
 // LDS bytes of one backward band workgroup
 inline size_t lw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)nlay * 64 + 4 * 64); }
-inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)(nlay + 1) * 64 + 4 * 64); }
+__host__ __device__ inline size_t sw_vjp_lds_bytes(int nlay, size_t real_size) { return real_size * ((size_t)(nlay + 1) * 64 + 4 * 64); }
 
 // fwd: NULL, or the forward's outputs, formed from w.bflux (which the
 // backward band kernel then fills from the forward it recomputes) by the
@@ -176,6 +176,56 @@ hipError_t launch_lw_vjp(const ColumnInputs &in, const VjpCot &cot, const LwGrad
 hipError_t launch_sw_vjp(const ColumnInputs &in, const VjpCot &cot, const SwGrad &out, const SwVjpWork &w,
                          const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev,
                          const SwOutputs *fwd = nullptr);
+// the three phases of launch_lw_vjp / launch_sw_vjp: prep (prep and column
+// kernels), band (the band kernels), finish (the backward's finish and, with
+// fwd given, the forward's)
+hipError_t launch_lw_vjp_prep(const ColumnInputs &in, const VjpCot &cot, const LwVjpWork &w, const LwDevTables &t,
+                              int mode, hipStream_t s);
+hipError_t launch_lw_vjp_band(const ColumnInputs &in, const LwVjpWork &w, const LwDevTables &t, int mode,
+                              hipStream_t s);
+hipError_t launch_lw_vjp_finish(const ColumnInputs &in, const LwGrad &out, const LwVjpWork &w, const LwDevTables &t,
+                                int mode, hipStream_t s, const LwOutputs *fwd);
+hipError_t launch_sw_vjp_prep(const ColumnInputs &in, const VjpCot &cot, const SwVjpWork &w, const SwDevTables &t,
+                              int mode, hipStream_t s);
+hipError_t launch_sw_vjp_band(const ColumnInputs &in, const SwVjpWork &w, const SwDevTables &t, int mode,
+                              hipStream_t s);
+hipError_t launch_sw_vjp_finish(const ColumnInputs &in, const SwGrad &out, const SwVjpWork &w, const SwDevTables &t,
+                                int mode, hipStream_t s, const SwOutputs *fwd);
+
+// The band phase of one LW and one SW backward (FP32, LW k-tables staged in
+// LDS) as one persistent kernel: workgroups of FUSED_WGW waves claim work
+// items from one counter, first the LW items (as the LW LDS-table kernel's,
+// FUSED_WGW tiles per group), then SW items of FUSED_WGW wave units each.
+// Each wave has one global scratch row (row_floats floats) for either side.
+constexpr int FUSED_WGW = 8;
+struct FusedBand {
+    ColumnInputs lin;
+    LwVjpWork lw;
+    const LwBand *lbands;       // LDS-relative k-table offsets
+    LwLdsPlan plan;
+    int lnwaves, litems;        // LW units per tile; LW items
+    int exp_off, tfn_off, plk_off;
+    const float *ltab;
+    const LwConst *cst;
+    const WaveUnit *lunits;
+    ColumnInputs sin;
+    SwVjpWork sw;
+    const SwBand *sbands;
+    int snwaves, snunits;       // SW units per tile; SW units in all
+    int sexp_off;
+    const float *stab;
+    const WaveUnit *sunits;
+    float *rows;
+    size_t row_floats;
+    int *items;
+    int nitems;
+};
+// LDS bytes and scratch-row floats of the fused kernel, and its grid
+size_t fused_lds_bytes(const LwLdsPlan &plan, int sw_nlay);
+size_t fused_row_floats(int lw_nlay, int sw_nlay);
+int fused_grid(const LwLdsPlan &plan, int nitems);
+hipError_t launch_fused_band(const FusedBand &a, hipStream_t s);
+
 // the forward's finish kernels alone, on band fluxes [band][2][nlay+1][ncol]
 hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *bflux, const LwDevTables &t,
                             int mode, hipStream_t s);

@@ -77,6 +77,15 @@ __device__ __forceinline__ const T &at32(const T *base, unsigned i)
 
 // Zeroes the work-item counter of a persistent band kernel (a kernel, not a
 // memset, so that the sequence can be captured in a graph).
+// Pointers typed by address space (global memory, LDS).  A called function
+// knows of a generic pointer argument only that it is flat: its accesses
+// compile to flat_* instructions, which wait on both the vector-memory and
+// the LDS counters.  Arguments typed by address space keep global_* and ds_*.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+using lptr = __attribute__((address_space(3))) T *;
+
 [[maybe_unused]] static __global__ void reset_items_kernel(int *items) { *items = 0; }
 
 // Next work item of a persistent workgroup, claimed from a global counter

@@ -270,6 +270,37 @@ def sw_vjp_columns(c, swuflx_bar=None, swdflx_bar=None, swhr_bar=None, stream=No
     return dict(zip((("swuflx", "swdflx", "swhr") if forward else ()) + ("tlay", "h2ovmr", "o3vmr"), out))
 
 
+def fused_begin(stream=None):
+    """Open a fused session on `stream` (torch's current stream by default):
+    the device-pointer backward calls that follow on it (at most one per
+    side) enqueue their prep kernels only, until fused_end."""
+    _check(lib().rrtmg_hip_fused_begin(_stream_ptr(stream)))
+
+
+def fused_end(stream=None):
+    """Enqueue the session's band work (one persistent kernel for both sides
+    in the FP32-mixed precision) and its finish kernels; returns True when
+    the single fused kernel ran."""
+    _check(lib().rrtmg_hip_fused_end(_stream_ptr(stream)))
+    return lib().rrtmg_hip_fused_last_kernel() == 1
+
+
+def fwd_vjp_fused(c, uflx_bar=None, dflx_bar=None, hr_bar=None, swuflx_bar=None, swdflx_bar=None, swhr_bar=None,
+                  stream=None):
+    """Forward and backward of both sides on a mima_amd.columns dict of CUDA
+    tensors in one fused session (rrtmg_hip_fused_begin, rrtmg_hip_sw_fwd_vjp,
+    rrtmg_hip_lw_fwd_vjp, rrtmg_hip_fused_end); returns (sw, lw) dicts as
+    sw_vjp_columns / lw_vjp_columns with forward=True."""
+    fused_begin(stream)
+    try:
+        sw = sw_vjp_columns(c, swuflx_bar=swuflx_bar, swdflx_bar=swdflx_bar, swhr_bar=swhr_bar, stream=stream,
+                            forward=True)
+        lw = lw_vjp_columns(c, uflx_bar=uflx_bar, dflx_bar=dflx_bar, hr_bar=hr_bar, stream=stream, forward=True)
+    finally:
+        fused_end(stream)
+    return sw, lw
+
+
 def debug_indices(lw, ncol, nlay):
     jp, jt, jt1 = (np.zeros((nlay, ncol), np.int32) for _ in range(3))
     lt = np.zeros(ncol, np.int32)

@@ -125,6 +125,98 @@ def test_graph_replay_reproduces_eager(rr):
     print("graph: %d outputs bitwise equal to eager on 3 replays" % len(ref))
 
 
+def _session(rr, d, cot):
+    return rr.fwd_vjp_fused(d, uflx_bar=cot["ub"], dflx_bar=cot["db"], hr_bar=cot["hr"], swuflx_bar=cot["sub"],
+                            swdflx_bar=cot["sdb"], swhr_bar=cot["shr"])
+
+
+def _doubled(cot):
+    return {k: 2.0 * v for k, v in cot.items()}
+
+
+def _check_session(rr, d, cot, sw, lw, ref_sw, ref_lw, exact_sw):
+    """LW outputs bitwise the one-pass calls'; SW outputs bitwise too
+    (exact_sw), or, from the fused kernel, whose SW body is compiled as a
+    called function, within twice the FP32-mixed path's own error against
+    FP64 on the same columns, elementwise by layer (the FP64 one-pass
+    results are the reference for both)."""
+    import torch
+    for k, v in ref_lw.items():
+        assert torch.equal(lw[k], v), ("LW", k)
+    if exact_sw:
+        for k, v in ref_sw.items():
+            assert torch.equal(sw[k], v), ("SW", k)
+        return
+    prev = rr.set_precision("fp64")
+    sw64, _ = _fused(rr, d, cot)
+    rr.set_precision(prev)
+    torch.cuda.synchronize()
+    for k, v in ref_sw.items():
+        err_mixed = (v - sw64[k]).abs().amax(dim=-1)
+        diff = (sw[k] - v).abs().amax(dim=-1)
+        bound = 2.0 * err_mixed + 1e-12 * sw64[k].abs().amax()
+        print("SW %s: max |fused - one pass| %.3e, max |one pass - FP64| %.3e"
+              % (k, float(diff.max()), float(err_mixed.max())))
+        assert bool((diff <= bound).all()), ("SW", k)
+
+
+@pytest.mark.parametrize("prec,fused_kernel", [("mixed", True), ("fp64", False)])
+def test_fused_session_matches_one_pass_calls(rr, prec, fused_kernel):
+    """A fused session (rrtmg_hip_fused_begin, both one-pass calls,
+    rrtmg_hip_fused_end) against the one-pass calls: in the FP32-mixed
+    precision through the single persistent kernel that runs the band work
+    of both sides, in FP64 through each side's own band kernels (bitwise).
+    The stream's workspaces first hold the results of doubled cotangents, so
+    work the session skipped would show."""
+    import torch
+    c, d, cot = _setup(rr, seed=8)
+    rr.set_precision(prec)
+    ref_sw, ref_lw = _fused(rr, d, cot)
+    ref_sw = {k: v.clone() for k, v in ref_sw.items()}
+    ref_lw = {k: v.clone() for k, v in ref_lw.items()}
+    _fused(rr, d, _doubled(cot))
+    sw, lw = _session(rr, d, cot)
+    assert (rr.lib().rrtmg_hip_fused_last_kernel() == 1) == fused_kernel
+    torch.cuda.synchronize()
+    _check_session(rr, d, cot, sw, lw, ref_sw, ref_lw, exact_sw=not fused_kernel)
+
+
+def test_fused_session_one_side_and_misuse(rr):
+    """One side alone in a session runs its own band kernels (bitwise the
+    one-pass call); end without begin, a second begin, a second call of one
+    side and host arrays inside a session are refused with EINVAL."""
+    import torch
+    c, d, cot = _setup(rr, seed=9)
+    rr.set_precision("mixed")
+    ref = rr.sw_vjp_columns(d, swuflx_bar=cot["sub"], swdflx_bar=cot["sdb"], swhr_bar=cot["shr"], forward=True)
+    rr.fused_begin()
+    sw = rr.sw_vjp_columns(d, swuflx_bar=cot["sub"], swdflx_bar=cot["sdb"], swhr_bar=cot["shr"], forward=True)
+    assert rr.fused_end() is False
+    torch.cuda.synchronize()
+    for k, v in ref.items():
+        assert torch.equal(sw[k], v), k
+    einval = -22
+    with pytest.raises(rr.RRTMGError) as e:
+        rr.fused_end()
+    assert e.value.code == einval
+    rr.fused_begin()
+    try:
+        with pytest.raises(rr.RRTMGError) as e:
+            rr.fused_begin()
+        assert e.value.code == einval
+        rr.lw_vjp_columns(d, uflx_bar=cot["ub"], dflx_bar=cot["db"], hr_bar=cot["hr"])
+        with pytest.raises(rr.RRTMGError) as e:
+            rr.lw_vjp_columns(d, uflx_bar=cot["ub"], dflx_bar=cot["db"], hr_bar=cot["hr"])
+        assert e.value.code == einval
+        h = {k: v.cpu().numpy() for k, v in cot.items()}
+        with pytest.raises(rr.RRTMGError) as e:
+            rr.sw_vjp_columns(c, swuflx_bar=h["sub"], swdflx_bar=h["sdb"], swhr_bar=h["shr"])
+        assert e.value.code == einval
+    finally:
+        rr.fused_end()
+    torch.cuda.synchronize()
+
+
 # ---------------------------------------------------------------------------
 # the large-batch workload itself: 512 x 256 = 131072 columns x 60 levels, CO2
 # log-uniform on 280-1120 ppmv per column
@@ -236,3 +328,39 @@ def test_large_batch_forward_matches_oracle_and_fp64(rr, c4):
         err = float(np.abs(lw64[k][:, it].cpu().numpy() - olw[k]).max())
         print("FP64 vs oracle LW %s: %.2e" % (k, err))
         assert err <= 1e-7, (k, err)
+
+
+def test_large_batch_fused_graph_matches_one_pass_calls(rr, c4):
+    """The large-batch step through a fused session (the band work of both
+    sides in one persistent kernel), FP32-mixed, all 131072 x 60 columns,
+    captured in a HIP graph and replayed twice, each time after an eager
+    session with doubled cotangents on the same workspaces: LW outputs
+    bitwise those of the one-pass calls, SW outputs within twice the mixed
+    path's own error against FP64."""
+    import torch
+    c, d, cot = c4
+    rr.set_precision("mixed")
+    print("large batch %d x %d, graph of a fused session" % (c["ncol"], c["nlay"]), flush=True)
+    ref_sw, ref_lw = _fused(rr, d, cot)
+    ref_sw = {k: v.clone() for k, v in ref_sw.items()}
+    ref_lw = {k: v.clone() for k, v in ref_lw.items()}
+    torch.cuda.synchronize()
+    main = torch.cuda.Stream()
+    cot2 = _doubled(cot)
+    with torch.cuda.stream(main):
+        _session(rr, d, cot2)                 # grows the workspaces
+        assert rr.lib().rrtmg_hip_fused_last_kernel() == 1
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=main):
+            gsw, glw = _session(rr, d, cot)
+    for rep in range(2):
+        for v in list(gsw.values()) + list(glw.values()):
+            v.fill_(float("nan"))
+        with torch.cuda.stream(main):
+            _session(rr, d, cot2)             # workspaces hold other results before each replay
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        _check_session(rr, d, cot, gsw, glw, ref_sw, ref_lw, exact_sw=False)
+    del g
