@@ -103,18 +103,29 @@ __device__ __forceinline__ int next_item(int *items, int *s_item)
 // Sum of v over the ng adjacent lanes of each column group of a one-wave
 // block (g = the lane's position in its group); valid on the group's first
 // lane.  Deterministic (fixed order).  FP64: sequential through LDS, the
-// reference's order over g-points; FP32: a tree of cross-lane shuffles
-// (log2 ng steps; pairwise summation, error ~log2(ng) ulp instead of ng).
-// `red` is a 64-entry LDS buffer (FP64 only).  Called by all 64 lanes.
+// reference's order over g-points; FP32 (the g-point flux sums): a
+// compensated pairwise tree of cross-lane shuffles, log2 ng steps, each
+// addition's rounding error recovered exactly (Knuth's TwoSum) and the
+// errors summed beside it and added once at the end, so the sum is good to
+// about one rounding instead of log2(ng).  `red` is a 64-entry LDS buffer
+// (FP64 only).  Called by all 64 lanes.
 template <typename Real>
 __device__ __forceinline__ Real seg_sum(Real v, int lane, int g, int ng, Real *red)
 {
     if constexpr (sizeof(Real) == 4) {
+        Real e = (Real)0;
         for (int s = 1; s < ng; s <<= 1) {
             const Real o = __shfl_down(v, s);
-            if (g + s < ng) v += o;
+            const Real oe = __shfl_down(e, s);
+            if (g + s < ng) {
+                const Real t = v + o;
+                const Real bp = t - v;
+                const Real err = (v - (t - bp)) + (o - bp);
+                v = t;
+                e = e + (oe + err);
+            }
         }
-        return g == 0 ? v : (Real)0;
+        return g == 0 ? v + e : (Real)0;
     } else {
         red[lane] = v;
         __syncthreads();
