@@ -204,6 +204,7 @@ __device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWor
     constexpr int NG = lw_ng(BAND);
     const int ncol = in.ncol, nlay = in.nlay;
     const int cl = lane / NG, g = lane - cl * NG;
+    const GSum<Real, NG> gsum(lane, g);
     const int col0 = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col0 < ncol;
     const int col = active ? col0 : 0;
@@ -277,7 +278,7 @@ __device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWor
         radld = radld + (bbd - radld) * atrans;
         at32(s_atr, (unsigned)((lev - 1) * 64 + lane)) = atrans;
         at32(s_bbu, (unsigned)((lev - 1) * 64 + lane)) = bbugas;
-        Real s = seg_sum(active ? radld : (Real)0, lane, g, NG, s_red);
+        Real s = gsum.sum(active ? radld : (Real)0, s_red);
         if (active && g == 0) at32(bf_dn, (unsigned)((lev - 1) * ncol + col)) = s;
         plev_up = plev_dn;
         if (lev == 1) fracs1 = plfrac;
@@ -285,7 +286,7 @@ __device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWor
     // surface emission and reflection, upward sweep
     Real radlu = fracs1 * plankbnd + ((Real)1 - semiss) * radld;
     {
-        Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
+        Real s = gsum.sum(active ? radlu : (Real)0, s_red);
         if (active && g == 0) at32(bf_up, (unsigned)col) = s;
     }
     // the per-layer states are read four levels ahead (rotating registers),
@@ -304,7 +305,7 @@ __device__ __forceinline__ void lw_band_body(const ColumnInputs &in, const LwWor
         a2 = a3;
         a3 = at(s_atr, lev + 3);
         radlu = radlu + (bbugas - radlu) * atrans;
-        Real s = seg_sum(active ? radlu : (Real)0, lane, g, NG, s_red);
+        Real s = gsum.sum(active ? radlu : (Real)0, s_red);
         if (active && g == 0) at32(bf_up, (unsigned)(lev * ncol + col)) = s;
     }
 }

@@ -137,6 +137,47 @@ __device__ __forceinline__ Real seg_sum(Real v, int lane, int g, int ng, Real *r
     }
 }
 
+// seg_sum with the lane arithmetic done once per work unit (the forward band
+// kernels call it for every level): the byte address of each step's
+// partner lane for ds_bpermute and whether the lane adds at that step, and
+// selects instead of a branch.  Bitwise the same sums as seg_sum.  A lane
+// whose partner lies past lane 63 never adds (its group would extend past
+// the wave), so the wrapped address only feeds lanes whose sum is unused.
+template <typename Real, int NG>
+struct GSum {
+    static constexpr int STEPS = NG <= 2 ? 1 : NG <= 4 ? 2 : NG <= 8 ? 3 : 4;
+    int addr[STEPS];
+    bool take[STEPS];
+    int lane, g;
+    __device__ GSum(int lane_, int g_) : lane(lane_), g(g_)
+    {
+#pragma unroll
+        for (int k = 0; k < STEPS; ++k) {
+            addr[k] = ((lane_ + (1 << k)) & 63) << 2;
+            take[k] = g_ + (1 << k) < NG;
+        }
+    }
+    __device__ __forceinline__ Real sum(Real v, Real *red) const
+    {
+        if constexpr (sizeof(Real) == 4) {
+            Real e = (Real)0;
+#pragma unroll
+            for (int k = 0; k < STEPS; ++k) {
+                const Real o = __int_as_float(__builtin_amdgcn_ds_bpermute(addr[k], __float_as_int(v)));
+                const Real oe = __int_as_float(__builtin_amdgcn_ds_bpermute(addr[k], __float_as_int(e)));
+                const Real t = v + o;
+                const Real bp = t - v;
+                const Real err = (v - (t - bp)) + (o - bp);
+                e = take[k] ? e + (oe + err) : e;
+                v = take[k] ? t : v;
+            }
+            return g == 0 ? v + e : (Real)0;
+        } else {
+            return seg_sum(v, lane, g, NG, red);
+        }
+    }
+};
+
 // seg_sum of N values at once; v[i] is valid on the group's first lane.
 // `red` holds N x 64 entries (FP64 only).
 template <int N, typename Real>

@@ -140,6 +140,7 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
     Bc.nspa = SW_NSPA[JB - 16];
     Bc.nspb = SW_NSPB[JB - 16];
     const int cl = lane / ng, g = lane - cl * ng;
+    const GSum<Real, ng> gsum(lane, g);
     const int col = tile * TILE_COLS + u.first_col + cl;
     const bool active = cl < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
     // a wave whose columns are all at night writes nothing: the finish
@@ -219,8 +220,8 @@ __device__ __forceinline__ void sw_band_body(const ColumnInputs &in, const SwWor
         Real pfu = (ptdbt * prup + (ztdn - ptdbt) * prupd) * zreflect;
         Real pfd = ptdbt + (ztdn - ptdbt + ptdbt * prup * prdnd) * zreflect;
         const int lev = nlay + 1 - jk;  // bottom-up level index
-        Real su = seg_sum(active ? zincflx * pfu : (Real)0, lane, g, ng, s_red);
-        Real sd = seg_sum(active ? zincflx * pfd : (Real)0, lane, g, ng, s_red);
+        Real su = gsum.sum(active ? zincflx * pfu : (Real)0, s_red);
+        Real sd = gsum.sum(active ? zincflx * pfd : (Real)0, s_red);
         if (active && g == 0) {
             at32(bf_up, (unsigned)(lev * ncol + col)) = su;
             at32(bf_dn, (unsigned)(lev * ncol + col)) = sd;
