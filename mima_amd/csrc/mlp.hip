@@ -352,8 +352,11 @@ __global__ __launch_bounds__(1024) void col_sum_kernel(int M, int N, const float
     const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
     const int n = blockIdx.x * 64 + c;
     float s = 0.f;
-    if (n < N)
+    // (unrolled: eight loads in flight, the additions in the same order)
+    if (n < N) {
+#pragma unroll 8
         for (int m = r; m < M; m += 16) s += x[(size_t)m * ldx + n];
+    }
     part[r][c] = s;
     __syncthreads();
     if (r == 0 && n < N) {
