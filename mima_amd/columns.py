@@ -174,3 +174,68 @@ def subset(cols, idx):
             out[k] = v
     out["ncol"] = len(idx)
     return out
+
+
+def make_clouds(cols, seed=0, blocks=(0, 3), edge_cases=True):
+    """Seeded cloud profiles for the LW cloudy path (``icld`` 1-3, ``inflglw = 0``).
+
+    Returns ``(cldfr, taucld)``: cloud fraction ``(nlay, ncol)`` and in-cloud
+    optical depth ``(nlay, ncol, 16)``, i.e. the reference's ``cldfr(ncol,
+    nlay)`` and ``taucld(nbndlw, ncol, nlay)`` in memory order.
+
+    Each column gets ``U{blocks}`` blocks of 1-6 adjacent cloudy layers whose
+    fractions are constant, increasing, decreasing, zig-zag or overcast
+    (1.0), drawn in (0.05, 1]; a layer's optical depth is a log-uniform base
+    in [1e-3, 50] times a per-column, per-band factor in [0.5, 1.5].  With
+    ``edge_cases``, column ``i`` additionally gets, by ``i mod 8``: 1 a layer
+    with fraction 1e-8 (at or above cldprop's 1e-20 threshold, below
+    rtrnmr's 1e-6); 2 a cloudy layer with zero optical depth; 3 a cloud in
+    the top layer; 4 a cloud in the surface layer; 5 two adjacent layers of
+    equal fraction; 6 a cloud of optical depth 1e-4 in the top layers
+    (gas-and-cloud depth below 0.06); 7 a cloud-free column.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, L = cols["ncol"], cols["nlay"]
+    cf = np.zeros((L, n))
+    tau = np.zeros((L, n, 16))
+    for i in range(n):
+        spec = rng.uniform(0.5, 1.5, 16)
+        nb = int(rng.integers(blocks[0], blocks[1] + 1))
+        for _ in range(nb):
+            th = int(rng.integers(1, 7))
+            k0 = int(rng.integers(0, max(1, L - th)))
+            kind = int(rng.integers(0, 5))
+            v = np.sort(rng.uniform(0.05, 1.0, th))
+            prof = {0: np.full(th, v[-1]), 1: v, 2: v[::-1], 3: rng.permutation(v), 4: np.ones(th)}[kind]
+            for j in range(th):
+                k = min(k0 + j, L - 1)
+                cf[k, i] = prof[j]
+                tau[k, i] = np.exp(rng.uniform(np.log(1e-3), np.log(50.0))) * spec
+        if not edge_cases:
+            continue
+        case = i % 8
+        if case == 1:
+            k = int(rng.integers(0, L))
+            cf[k, i] = 1e-8
+            tau[k, i] = 2.0 * spec
+        elif case == 2:
+            k = int(rng.integers(0, L))
+            cf[k, i] = 0.5
+            tau[k, i] = 0.0
+        elif case == 3:
+            cf[L - 1, i] = 0.3
+            tau[L - 1, i] = 0.5 * spec
+        elif case == 4:
+            cf[0, i] = 0.8
+            tau[0, i] = 5.0 * spec
+        elif case == 5:
+            k = int(rng.integers(0, L - 1))
+            cf[k:k + 2, i] = 0.6
+            tau[k:k + 2, i] = 3.0 * spec
+        elif case == 6:
+            cf[L - 3:L - 1, i] = (0.4, 0.7)
+            tau[L - 3:L - 1, i] = 1e-4 * spec
+        elif case == 7:
+            cf[:, i] = 0.0
+            tau[:, i] = 0.0
+    return cf, tau

@@ -99,6 +99,20 @@ int oracle_lw_inatm(int nlay, const double *play, const double *plev,
 void oracle_sw_reftra(int nlay, const double *gg, double prmuz, const double *tau,
                       const double *w, double *pref, double *prefd,
                       double *ptra, double *ptrad);
+/* rrtmg_lw with clouds (rrtmg_lw_rad.nomcica.f90:80-569): cldprop with
+ * inflglw = 0, then rtrn (icld = 1) or rtrnmr (icld = 2, 3); idrv = 0.
+ * cldfr, cicewp, cliqwp (ncol, nlay) (NULL water paths = 0); taucld
+ * (16, ncol, nlay) band fastest; tauaer (ncol, nlay, 16) or NULL.  Total-sky
+ * and clear-sky outputs as rrtmg_lw.  0 on success. */
+int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                          const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                          const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                          const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                          const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
+                          const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
+                          double *uflxc, double *dflxc, double *hrc);
+
 /* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...): copies up to
  * n values (Fortran order) into out and returns the element count, or -1. */
 long oracle_reduced(const char *name, double *out, long n);

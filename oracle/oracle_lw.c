@@ -1115,6 +1115,505 @@ int oracle_rrtmg_lw(int ncol, int nlay, const double *play, const double *plev, 
     return err;
 }
 
+/* ---- cloudy sky (icld = 1..3, inflglw = 0) -------------------------------- */
+
+/* cldprop with inflag = 0 (rrtmg_lw_cldprop.f90:154-176): a layer is cloudy
+ * for cldprop when its fraction and its water path or summed optical depth
+ * reach cldmin = 1e-20; its in-cloud optical depths are then the input's,
+ * otherwise zero.  tauc[lay][ib], 1-based. */
+static void cldprop0(int nl, const double *cldfrac, double (*tauc)[17], const double *cwp, double (*taucloud)[17])
+{
+    const double cldmin = 1.e-20;
+    for (int lay = 1; lay <= nl; ++lay) {
+        double tauctot = 0.0;
+        for (int ib = 1; ib <= 16; ++ib) {
+            taucloud[lay][ib] = 0.0;
+            tauctot = tauctot + tauc[lay][ib];
+        }
+        if (cldfrac[lay] >= cldmin && (cwp[lay] >= cldmin || tauctot >= cldmin))
+            for (int ib = 1; ib <= 16; ++ib) taucloud[lay][ib] = tauc[lay][ib];
+    }
+}
+
+/* Maximum-random overlap factors of rtrnmr (rrtmg_lw_rtrnmr.f90:325-476).
+ * cldfrac[0] and cldfrac[nl+1] are 0: the reference reads cldfrac(0) at
+ * lev = 1 only as a factor of faccld2(1) = facclr2(1) = 0, and the driver's
+ * cldfrac(nlay+1) is zero (rrtmg_lw_rad.nomcica.f90:736).  Arrays 0..nl+1. */
+typedef struct {
+    double faccld1[MAXLAY + 2], faccld2[MAXLAY + 2], facclr1[MAXLAY + 2], facclr2[MAXLAY + 2];
+    double faccmb1[MAXLAY + 2], faccmb2[MAXLAY + 2];
+    double faccld1d[MAXLAY + 2], faccld2d[MAXLAY + 2], facclr1d[MAXLAY + 2], facclr2d[MAXLAY + 2];
+    double faccmb1d[MAXLAY + 2], faccmb2d[MAXLAY + 2];
+    int istcld[MAXLAY + 2], istcldd[MAXLAY + 2];
+} overlap_t;
+
+static void overlap_mr(int nl, const double *cldfrac, const int *icldlyr, overlap_t *o)
+{
+    double rat1 = 0.0, rat2 = 0.0, fmax, fmin;
+    memset(o, 0, sizeof *o);
+    o->istcld[1] = 1;
+    o->istcldd[nl] = 1;
+    for (int lev = 1; lev <= nl; ++lev) {
+        if (icldlyr[lev] == 1) {
+            o->istcld[lev + 1] = 0;
+            if (lev == nl) {
+                o->faccld1[lev + 1] = 0.0;
+                o->faccld2[lev + 1] = 0.0;
+                o->facclr1[lev + 1] = 0.0;
+                o->facclr2[lev + 1] = 0.0;
+                o->faccmb1[lev + 1] = 0.0;
+                o->faccmb2[lev + 1] = 0.0;
+            } else if (cldfrac[lev + 1] >= cldfrac[lev]) {
+                o->faccld1[lev + 1] = 0.0;
+                o->faccld2[lev + 1] = 0.0;
+                if (o->istcld[lev] == 1) {
+                    o->facclr1[lev + 1] = 0.0;
+                    o->facclr2[lev + 1] = 0.0;
+                    if (cldfrac[lev] < 1.0) o->facclr2[lev + 1] = (cldfrac[lev + 1] - cldfrac[lev]) / (1.0 - cldfrac[lev]);
+                    o->facclr2[lev] = 0.0;
+                    o->faccld2[lev] = 0.0;
+                } else {
+                    fmax = cldfrac[lev] > cldfrac[lev - 1] ? cldfrac[lev] : cldfrac[lev - 1];
+                    if (cldfrac[lev + 1] > fmax) {
+                        o->facclr1[lev + 1] = rat2;
+                        o->facclr2[lev + 1] = (cldfrac[lev + 1] - fmax) / (1.0 - fmax);
+                    } else if (cldfrac[lev + 1] < fmax) {
+                        o->facclr1[lev + 1] = (cldfrac[lev + 1] - cldfrac[lev]) / (cldfrac[lev - 1] - cldfrac[lev]);
+                        o->facclr2[lev + 1] = 0.0;
+                    } else {
+                        o->facclr1[lev + 1] = rat2;
+                        o->facclr2[lev + 1] = 0.0;
+                    }
+                }
+                if (o->facclr1[lev + 1] > 0.0 || o->facclr2[lev + 1] > 0.0) {
+                    rat1 = 1.0;
+                    rat2 = 0.0;
+                } else {
+                    rat1 = 0.0;
+                    rat2 = 0.0;
+                }
+            } else {
+                o->facclr1[lev + 1] = 0.0;
+                o->facclr2[lev + 1] = 0.0;
+                if (o->istcld[lev] == 1) {
+                    o->faccld1[lev + 1] = 0.0;
+                    o->faccld2[lev + 1] = (cldfrac[lev] - cldfrac[lev + 1]) / cldfrac[lev];
+                    o->facclr2[lev] = 0.0;
+                    o->faccld2[lev] = 0.0;
+                } else {
+                    fmin = cldfrac[lev] < cldfrac[lev - 1] ? cldfrac[lev] : cldfrac[lev - 1];
+                    if (cldfrac[lev + 1] <= fmin) {
+                        o->faccld1[lev + 1] = rat1;
+                        o->faccld2[lev + 1] = (fmin - cldfrac[lev + 1]) / fmin;
+                    } else {
+                        o->faccld1[lev + 1] = (cldfrac[lev] - cldfrac[lev + 1]) / (cldfrac[lev] - fmin);
+                        o->faccld2[lev + 1] = 0.0;
+                    }
+                }
+                if (o->faccld1[lev + 1] > 0.0 || o->faccld2[lev + 1] > 0.0) {
+                    rat1 = 0.0;
+                    rat2 = 1.0;
+                } else {
+                    rat1 = 0.0;
+                    rat2 = 0.0;
+                }
+            }
+            o->faccmb1[lev + 1] = o->facclr1[lev + 1] * o->faccld2[lev] * cldfrac[lev - 1];
+            o->faccmb2[lev + 1] = o->faccld1[lev + 1] * o->facclr2[lev] * (1.0 - cldfrac[lev - 1]);
+        } else {
+            o->istcld[lev + 1] = 1;
+        }
+    }
+    for (int lev = nl; lev >= 1; --lev) {
+        if (icldlyr[lev] == 1) {
+            o->istcldd[lev - 1] = 0;
+            if (lev == 1) {
+                o->faccld1d[lev - 1] = 0.0;
+                o->faccld2d[lev - 1] = 0.0;
+                o->facclr1d[lev - 1] = 0.0;
+                o->facclr2d[lev - 1] = 0.0;
+                o->faccmb1d[lev - 1] = 0.0;
+                o->faccmb2d[lev - 1] = 0.0;
+            } else if (cldfrac[lev - 1] >= cldfrac[lev]) {
+                o->faccld1d[lev - 1] = 0.0;
+                o->faccld2d[lev - 1] = 0.0;
+                if (o->istcldd[lev] == 1) {
+                    o->facclr1d[lev - 1] = 0.0;
+                    o->facclr2d[lev - 1] = 0.0;
+                    if (cldfrac[lev] < 1.0) o->facclr2d[lev - 1] = (cldfrac[lev - 1] - cldfrac[lev]) / (1.0 - cldfrac[lev]);
+                    o->facclr2d[lev] = 0.0;
+                    o->faccld2d[lev] = 0.0;
+                } else {
+                    fmax = cldfrac[lev] > cldfrac[lev + 1] ? cldfrac[lev] : cldfrac[lev + 1];
+                    if (cldfrac[lev - 1] > fmax) {
+                        o->facclr1d[lev - 1] = rat2;
+                        o->facclr2d[lev - 1] = (cldfrac[lev - 1] - fmax) / (1.0 - fmax);
+                    } else if (cldfrac[lev - 1] < fmax) {
+                        o->facclr1d[lev - 1] = (cldfrac[lev - 1] - cldfrac[lev]) / (cldfrac[lev + 1] - cldfrac[lev]);
+                        o->facclr2d[lev - 1] = 0.0;
+                    } else {
+                        o->facclr1d[lev - 1] = rat2;
+                        o->facclr2d[lev - 1] = 0.0;
+                    }
+                }
+                if (o->facclr1d[lev - 1] > 0.0 || o->facclr2d[lev - 1] > 0.0) {
+                    rat1 = 1.0;
+                    rat2 = 0.0;
+                } else {
+                    rat1 = 0.0;
+                    rat2 = 0.0;
+                }
+            } else {
+                o->facclr1d[lev - 1] = 0.0;
+                o->facclr2d[lev - 1] = 0.0;
+                if (o->istcldd[lev] == 1) {
+                    o->faccld1d[lev - 1] = 0.0;
+                    o->faccld2d[lev - 1] = (cldfrac[lev] - cldfrac[lev - 1]) / cldfrac[lev];
+                    o->facclr2d[lev] = 0.0;
+                    o->faccld2d[lev] = 0.0;
+                } else {
+                    fmin = cldfrac[lev] < cldfrac[lev + 1] ? cldfrac[lev] : cldfrac[lev + 1];
+                    if (cldfrac[lev - 1] <= fmin) {
+                        o->faccld1d[lev - 1] = rat1;
+                        o->faccld2d[lev - 1] = (fmin - cldfrac[lev - 1]) / fmin;
+                    } else {
+                        o->faccld1d[lev - 1] = (cldfrac[lev] - cldfrac[lev - 1]) / (cldfrac[lev] - fmin);
+                        o->faccld2d[lev - 1] = 0.0;
+                    }
+                }
+                if (o->faccld1d[lev - 1] > 0.0 || o->faccld2d[lev - 1] > 0.0) {
+                    rat1 = 0.0;
+                    rat2 = 1.0;
+                } else {
+                    rat1 = 0.0;
+                    rat2 = 0.0;
+                }
+            }
+            o->faccmb1d[lev - 1] = o->facclr1d[lev - 1] * o->faccld2d[lev] * cldfrac[lev + 1];
+            o->faccmb2d[lev - 1] = o->faccld1d[lev - 1] * o->facclr2d[lev] * (1.0 - cldfrac[lev + 1]);
+        } else {
+            o->istcldd[lev - 1] = 1;
+        }
+    }
+}
+
+/* One cloudy layer's gas and gas+cloud terms (the three branches shared by
+ * rrtmg_lw_rtrn.f90:349-418 and rrtmg_lw_rtrnmr.f90:497-545). */
+typedef struct {
+    double atrans, atot, gassrc, bbd, bbdtot, bbugas, bbutot;
+} cldlay_t;
+
+static void cloudy_layer(double odepth, double odcld, double plfrac, double blay, double dplankup, double dplankdn,
+                         cldlay_t *r)
+{
+    const double rec_6 = 0.166667;
+    double odtot = odepth + odcld;
+    if (odtot < 0.06) {
+        r->atrans = odepth - 0.5 * odepth * odepth;
+        double odepth_rec = rec_6 * odepth;
+        r->gassrc = plfrac * (blay + dplankdn * odepth_rec) * r->atrans;
+        r->atot = odtot - 0.5 * odtot * odtot;
+        double odtot_rec = rec_6 * odtot;
+        r->bbdtot = plfrac * (blay + dplankdn * odtot_rec);
+        r->bbd = plfrac * (blay + dplankdn * odepth_rec);
+        r->bbugas = plfrac * (blay + dplankup * odepth_rec);
+        r->bbutot = plfrac * (blay + dplankup * odtot_rec);
+    } else if (odepth <= 0.06) {
+        r->atrans = odepth - 0.5 * odepth * odepth;
+        double odepth_rec = rec_6 * odepth;
+        r->gassrc = plfrac * (blay + dplankdn * odepth_rec) * r->atrans;
+        odtot = odepth + odcld;
+        double tblind = odtot / (L.bpade + odtot);
+        int ittot = (int)(1.e4 * tblind + 0.5);
+        double tfactot = L.tfn_tbl[ittot];
+        r->bbdtot = plfrac * (blay + tfactot * dplankdn);
+        r->bbd = plfrac * (blay + dplankdn * odepth_rec);
+        r->atot = 1.0 - L.exp_tbl[ittot];
+        r->bbugas = plfrac * (blay + dplankup * odepth_rec);
+        r->bbutot = plfrac * (blay + tfactot * dplankup);
+    } else {
+        double tblind = odepth / (L.bpade + odepth);
+        int itgas = (int)(1.e4 * tblind + 0.5);
+        odepth = L.tau_tbl[itgas];
+        r->atrans = 1.0 - L.exp_tbl[itgas];
+        double tfacgas = L.tfn_tbl[itgas];
+        r->gassrc = r->atrans * plfrac * (blay + tfacgas * dplankdn);
+        odtot = odepth + odcld;
+        tblind = odtot / (L.bpade + odtot);
+        int ittot = (int)(1.e4 * tblind + 0.5);
+        double tfactot = L.tfn_tbl[ittot];
+        r->bbdtot = plfrac * (blay + tfactot * dplankdn);
+        r->bbd = plfrac * (blay + tfacgas * dplankdn);
+        r->atot = 1.0 - L.exp_tbl[ittot];
+        r->bbugas = plfrac * (blay + tfacgas * dplankup);
+        r->bbutot = plfrac * (blay + tfactot * dplankup);
+    }
+}
+
+/* rtrn (icld = 1, random overlap; rrtmg_lw_rtrn.f90:279-559) and rtrnmr
+ * (icld = 2, 3, maximum-random overlap; rrtmg_lw_rtrnmr.f90:283-777) with
+ * iout = 0, idrv = 0; taut = taug + taua(ngb) (iaer = 10,
+ * rrtmg_lw_rad.nomcica.f90:512-518).  Total-sky and clear-sky fluxes and
+ * heating rates. */
+static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, double (*taucloud)[17],
+                        double (*taua)[17], double *totuflux, double *totdflux, double *htr, double *totuclfl,
+                        double *totdclfl, double *htrc)
+{
+    const int nl = c->nlayers;
+    const double wtdiff = 0.5, rec_6 = 0.166667;
+    const double pi = 2.0 * asin(1.0);
+    const double fluxfac = pi * 2.e4;
+    double secdiff[17];
+    static __thread double odcld[MAXLAY + 2][17], efclfrac[MAXLAY + 2][17];
+    double urad[MAXLAY + 2], drad[MAXLAY + 2], clrurad[MAXLAY + 2], clrdrad[MAXLAY + 2];
+    double atrans[MAXLAY + 2], atot[MAXLAY + 2], bbugas[MAXLAY + 2], bbutot[MAXLAY + 2];
+    double fnet[MAXLAY + 2], fnetc[MAXLAY + 2], cf[MAXLAY + 2];
+    int icldlyr[MAXLAY + 2];
+    static __thread overlap_t ov;
+    for (int ibnd = 1; ibnd <= 16; ++ibnd) {
+        if (ibnd == 1 || ibnd == 4 || ibnd >= 10) {
+            secdiff[ibnd] = 1.66;
+        } else {
+            secdiff[ibnd] = L.a0[ibnd - 1] + L.a1[ibnd - 1] * exp(L.a2[ibnd - 1] * c->pwvcm);
+            if (secdiff[ibnd] > 1.80) secdiff[ibnd] = 1.80;
+            if (secdiff[ibnd] < 1.50) secdiff[ibnd] = 1.50;
+        }
+    }
+    for (int lev = 0; lev <= nl; ++lev) {
+        urad[lev] = drad[lev] = clrurad[lev] = clrdrad[lev] = 0.0;
+        totuflux[lev] = totdflux[lev] = totuclfl[lev] = totdclfl[lev] = 0.0;
+    }
+    cf[0] = 0.0;
+    cf[nl + 1] = 0.0;
+    for (int lay = 1; lay <= nl; ++lay) {
+        cf[lay] = cldfrac[lay];
+        icldlyr[lay] = cf[lay] >= 1.e-6 ? 1 : 0;
+        for (int ib = 1; ib <= 16; ++ib) {
+            if (icldlyr[lay]) {
+                odcld[lay][ib] = secdiff[ib] * taucloud[lay][ib];
+                double transcld = exp(-odcld[lay][ib]);
+                efclfrac[lay][ib] = (1.0 - transcld) * cf[lay];
+            } else {
+                odcld[lay][ib] = 0.0;
+                efclfrac[lay][ib] = 0.0;
+            }
+        }
+    }
+    if (icld != 1) overlap_mr(nl, cf, icldlyr, &ov);
+    int igc = 1;
+    for (int iband = 1; iband <= 16; ++iband) {
+        for (; igc <= L.ngs[iband - 1]; ++igc) {
+            double radld = 0.0, radclrd = 0.0;
+            double cldradd = 0.0, clrradd = 0.0, rad = 0.0, oldcld, oldclr;
+            int iclddn = 0;
+            for (int lev = nl; lev >= 1; --lev) {
+                double plfrac = c->fracs[lev][igc];
+                double blay = c->planklay[lev][iband];
+                double dplankup = c->planklev[lev][iband] - blay;
+                double dplankdn = c->planklev[lev - 1][iband] - blay;
+                double taut = c->taug[lev][igc] + (taua ? taua[lev][iband] : 0.0);
+                double odepth = secdiff[iband] * taut;
+                if (odepth < 0.0) odepth = 0.0;
+                double bbd;
+                if (icldlyr[lev] == 1) {
+                    iclddn = 1;
+                    cldlay_t r;
+                    cloudy_layer(odepth, odcld[lev][iband], plfrac, blay, dplankup, dplankdn, &r);
+                    atrans[lev] = r.atrans;
+                    atot[lev] = r.atot;
+                    bbugas[lev] = r.bbugas;
+                    bbutot[lev] = r.bbutot;
+                    bbd = r.bbd;
+                    if (icld == 1) {
+                        radld = radld - radld * (r.atrans + efclfrac[lev][iband] * (1.0 - r.atrans)) + r.gassrc +
+                                cf[lev] * (r.bbdtot * r.atot - r.gassrc);
+                    } else {
+                        if (ov.istcldd[lev] == 1) {
+                            cldradd = cf[lev] * radld;
+                            clrradd = radld - cldradd;
+                            rad = 0.0;
+                        }
+                        double ttot = 1.0 - r.atot;
+                        double cldsrc = r.bbdtot * r.atot;
+                        cldradd = cldradd * ttot + cf[lev] * cldsrc;
+                        clrradd = clrradd * (1.0 - r.atrans) + (1.0 - cf[lev]) * r.gassrc;
+                        radld = cldradd + clrradd;
+                        double radmod = rad * (ov.facclr1d[lev - 1] * (1.0 - r.atrans) + ov.faccld1d[lev - 1] * ttot) -
+                                        ov.faccmb1d[lev - 1] * r.gassrc + ov.faccmb2d[lev - 1] * cldsrc;
+                        oldcld = cldradd - radmod;
+                        oldclr = clrradd + radmod;
+                        rad = -radmod + ov.facclr2d[lev - 1] * oldclr - ov.faccld2d[lev - 1] * oldcld;
+                        cldradd = cldradd + rad;
+                        clrradd = clrradd - rad;
+                    }
+                    drad[lev - 1] = drad[lev - 1] + radld;
+                } else {
+                    if (odepth <= 0.06) {
+                        atrans[lev] = odepth - 0.5 * odepth * odepth;
+                        odepth = rec_6 * odepth;
+                        bbd = plfrac * (blay + dplankdn * odepth);
+                        bbugas[lev] = plfrac * (blay + dplankup * odepth);
+                    } else {
+                        double tblind = odepth / (L.bpade + odepth);
+                        int itr = (int)(1.e4 * tblind + 0.5);
+                        double transc = L.exp_tbl[itr];
+                        atrans[lev] = 1.0 - transc;
+                        double tausfac = L.tfn_tbl[itr];
+                        bbd = plfrac * (blay + tausfac * dplankdn);
+                        bbugas[lev] = plfrac * (blay + tausfac * dplankup);
+                    }
+                    radld = radld + (bbd - radld) * atrans[lev];
+                    drad[lev - 1] = drad[lev - 1] + radld;
+                }
+                if (iclddn == 1) {
+                    radclrd = radclrd + (bbd - radclrd) * atrans[lev];
+                    clrdrad[lev - 1] = clrdrad[lev - 1] + radclrd;
+                } else {
+                    radclrd = radld;
+                    clrdrad[lev - 1] = drad[lev - 1];
+                }
+            }
+            double rad0 = c->fracs[1][igc] * c->plankbnd[iband];
+            double reflect = 1.0 - c->semiss[iband];
+            double radlu = rad0 + reflect * radld;
+            double radclru = rad0 + reflect * radclrd;
+            double cldradu = 0.0, clrradu = 0.0;
+            urad[0] = urad[0] + radlu;
+            clrurad[0] = clrurad[0] + radclru;
+            for (int lev = 1; lev <= nl; ++lev) {
+                if (icldlyr[lev] == 1) {
+                    double gassrc = bbugas[lev] * atrans[lev];
+                    if (icld == 1) {
+                        radlu = radlu - radlu * (atrans[lev] + efclfrac[lev][iband] * (1.0 - atrans[lev])) + gassrc +
+                                cf[lev] * (bbutot[lev] * atot[lev] - gassrc);
+                    } else {
+                        if (ov.istcld[lev] == 1) {
+                            cldradu = cf[lev] * radlu;
+                            clrradu = radlu - cldradu;
+                            rad = 0.0;
+                        }
+                        double ttot = 1.0 - atot[lev];
+                        double cldsrc = bbutot[lev] * atot[lev];
+                        cldradu = cldradu * ttot + cf[lev] * cldsrc;
+                        clrradu = clrradu * (1.0 - atrans[lev]) + (1.0 - cf[lev]) * gassrc;
+                        radlu = cldradu + clrradu;
+                        double radmod = rad * (ov.facclr1[lev + 1] * (1.0 - atrans[lev]) + ov.faccld1[lev + 1] * ttot) -
+                                        ov.faccmb1[lev + 1] * gassrc + ov.faccmb2[lev + 1] * cldsrc;
+                        oldcld = cldradu - radmod;
+                        oldclr = clrradu + radmod;
+                        rad = -radmod + ov.facclr2[lev + 1] * oldclr - ov.faccld2[lev + 1] * oldcld;
+                        cldradu = cldradu + rad;
+                        clrradu = clrradu - rad;
+                    }
+                } else {
+                    radlu = radlu + (bbugas[lev] - radlu) * atrans[lev];
+                }
+                urad[lev] = urad[lev] + radlu;
+                if (iclddn == 1) {
+                    radclru = radclru + (bbugas[lev] - radclru) * atrans[lev];
+                    clrurad[lev] = clrurad[lev] + radclru;
+                } else {
+                    radclru = radlu;
+                    clrurad[lev] = urad[lev];
+                }
+            }
+        }
+        for (int lev = nl; lev >= 0; --lev) {
+            double dw = L.delwave[iband - 1];
+            totuflux[lev] = totuflux[lev] + urad[lev] * wtdiff * dw;
+            totdflux[lev] = totdflux[lev] + drad[lev] * wtdiff * dw;
+            totuclfl[lev] = totuclfl[lev] + clrurad[lev] * wtdiff * dw;
+            totdclfl[lev] = totdclfl[lev] + clrdrad[lev] * wtdiff * dw;
+            urad[lev] = drad[lev] = clrurad[lev] = clrdrad[lev] = 0.0;
+        }
+    }
+    for (int lev = 0; lev <= nl; ++lev) {
+        totuflux[lev] = totuflux[lev] * fluxfac;
+        totdflux[lev] = totdflux[lev] * fluxfac;
+        fnet[lev] = totuflux[lev] - totdflux[lev];
+        totuclfl[lev] = totuclfl[lev] * fluxfac;
+        totdclfl[lev] = totdclfl[lev] * fluxfac;
+        fnetc[lev] = totuclfl[lev] - totdclfl[lev];
+    }
+    for (int lev = 1; lev <= nl; ++lev) {
+        int l = lev - 1;
+        htr[l] = L.heatfac * (fnet[l] - fnet[lev]) / (c->pz[l] - c->pz[lev]);
+        htrc[l] = L.heatfac * (fnetc[l] - fnetc[lev]) / (c->pz[l] - c->pz[lev]);
+    }
+    htr[nl] = 0.0;
+    htrc[nl] = 0.0;
+}
+
+/* rrtmg_lw with clouds (rrtmg_lw_rad.nomcica.f90:80-569): icld 0..3 (out
+ * of range -> 2, :437; 0 = clear sky, cloud arrays unread), inflglw = 0,
+ * idrv = 0.  cldfr, cicewp, cliqwp
+ * (ncol, nlay); taucld (16, ncol, nlay), band fastest; tauaer (ncol, nlay,
+ * 16) or NULL; cicewp/cliqwp NULL = 0. */
+int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                          const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                          const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                          const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                          const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
+                          const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
+                          double *uflxc, double *dflxc, double *hrc)
+{
+    if (!L.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    if (icld < 0 || icld > 3) icld = 2;
+    /* icld = 0: inatm leaves cldfrac = 0 (:736, :868), every layer clear */
+    const int clouds = icld != 0;
+    int err = 0;
+#pragma omp parallel
+    {
+        lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+        double (*tauc)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
+        double (*taucloud)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
+        double (*taua)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
+        double cf[MAXLAY + 2], cwp[MAXLAY + 2];
+        double tu[MAXLAY + 2], td[MAXLAY + 2], tuc[MAXLAY + 2], tdc[MAXLAY + 2], htr[MAXLAY + 2],
+            htrc[MAXLAY + 2];
+        if (!c || !tauc || !taucloud || !taua) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (err) continue;
+            inatm(c, iplon, ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                  o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis);
+            for (int l = 1; l <= nlay; ++l) {
+                long ix = iplon + (long)ncol * (l - 1);
+                cf[l] = clouds ? cldfr[ix] : 0.0;
+                cwp[l] = clouds ? (cicewp ? cicewp[ix] : 0.0) + (cliqwp ? cliqwp[ix] : 0.0) : 0.0;
+                for (int ib = 1; ib <= 16; ++ib) {
+                    tauc[l][ib] = clouds ? taucld[(ib - 1) + 16 * ix] : 0.0;
+                    taua[l][ib] = tauaer ? tauaer[iplon + (long)ncol * ((l - 1) + (long)nlay * (ib - 1))] : 0.0;
+                }
+            }
+            cldprop0(nlay, cf, tauc, cwp, taucloud);
+            setcoef(c);
+            taumol(c);
+            rtrn_cloudy(c, clouds ? icld : 2, cf, taucloud, tauaer ? taua : NULL, tu, td, htr, tuc, tdc, htrc);
+            for (int k = 0; k <= nlay; ++k) {
+                uflx[iplon + (long)ncol * k] = tu[k];
+                dflx[iplon + (long)ncol * k] = td[k];
+                uflxc[iplon + (long)ncol * k] = tuc[k];
+                dflxc[iplon + (long)ncol * k] = tdc[k];
+            }
+            for (int k = 0; k < nlay; ++k) {
+                hr[iplon + (long)ncol * k] = htr[k];
+                hrc[iplon + (long)ncol * k] = htrc[k];
+            }
+        }
+        free(c);
+        free(tauc);
+        free(taucloud);
+        free(taua);
+    }
+    return err;
+}
+
 static void one_lw(lwcol_t *c, int nlay, const double *play, const double *plev, const double *tlay,
                    const double *tlev, double tsfc, const double *h2ovmr, const double *o3vmr,
                    const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,

@@ -82,6 +82,25 @@ def rrtmg_lw(c):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+def rrtmg_lw_cloud(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None):
+    """rrtmg_lw with clouds (icld 0..3, inflglw = 0).  cldfr (nlay, ncol),
+    taucld (nlay, ncol, 16), tauaer (16, nlay, ncol) or None."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr",
+                                "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr",
+                                "ccl4vmr", "emis")]
+    cl = [_f64(cldfr), _f64(taucld)]
+    opt = [None if a is None else _f64(a) for a in (cicewp, cliqwp, tauaer)]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_lw_cloud(ncol, nlay, int(icld), *[_p(a) for a in ins], *[_p(a) for a in cl],
+                                 *[None if a is None else _p(a) for a in opt], *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_lw_cloud returned %d" % rc)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
 def _col(c, i, key):
     v = c[key]
     return np.ascontiguousarray(v[:, i], dtype=np.float64)

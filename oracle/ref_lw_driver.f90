@@ -19,6 +19,8 @@
 ! The reference column code then runs unchanged:
 !   ref_lw_run    -> rrtmg_lw (rrtmg_lw_rad.nomcica.f90:80), clear sky, with
 !                    the cloud arguments MiMA passes (RR/rrtm_radiation.f90:730-749)
+!   ref_lw_run_cloud -> rrtmg_lw with icld = 1..3 and inflglw = 0 (cldprop,
+!                    then rtrn or rtrnmr with the caller's cloud profiles)
 !   ref_lw_column -> inatm + setcoef + taumol for one column
 module ref_lw_driver
   use iso_c_binding
@@ -143,6 +145,36 @@ contains
                   0, 0, 0, zeros, taucld, zeros, zeros, tencl, tencl, tauaer, &
                   uflx, dflx, hr, uflxc, dflxc, hrc)
   end subroutine ref_lw_run
+
+  ! rrtmg_lw with clouds: icld (1 random overlap -> rtrn, 2/3 maximum-random
+  ! -> rtrnmr), inflglw = 0 (cldprop takes the in-cloud optical depths
+  ! taucld(nbndlw, ncol, nlay) as given), the caller's cloud fraction and
+  ! water paths; effective radii 10 um as MiMA passes them
+  subroutine ref_lw_run_cloud(ncol, nlay, icld_in, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, &
+                              o2, cfc11, cfc12, cfc22, ccl4, emis, cldfr, taucld, cicewp, cliqwp, tauaer, &
+                              uflx, dflx, hr, uflxc, dflxc, hrc) bind(C, name="ref_lw_run_cloud")
+    integer(c_int), value :: ncol, nlay, icld_in
+    real(c_double), intent(in) :: play(ncol, nlay), plev(ncol, nlay + 1), tlay(ncol, nlay)
+    real(c_double), intent(in) :: tlev(ncol, nlay + 1), tsfc(ncol)
+    real(c_double), intent(in) :: h2o(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay)
+    real(c_double), intent(in) :: ch4(ncol, nlay), n2o(ncol, nlay), o2(ncol, nlay)
+    real(c_double), intent(in) :: cfc11(ncol, nlay), cfc12(ncol, nlay), cfc22(ncol, nlay)
+    real(c_double), intent(in) :: ccl4(ncol, nlay), emis(ncol, nbndlw), tauaer(ncol, nlay, nbndlw)
+    real(c_double), intent(in) :: cldfr(ncol, nlay), taucld(nbndlw, ncol, nlay)
+    real(c_double), intent(in) :: cicewp(ncol, nlay), cliqwp(ncol, nlay)
+    real(c_double), intent(out) :: uflx(ncol, nlay + 1), dflx(ncol, nlay + 1), hr(ncol, nlay)
+    real(c_double), intent(out) :: uflxc(ncol, nlay + 1), dflxc(ncol, nlay + 1), hrc(ncol, nlay)
+    integer(kind=im) :: icld, idrv
+    real(kind=rb), allocatable :: tencl(:, :)
+    allocate(tencl(ncol, nlay))
+    tencl = 10.0_rb
+    icld = icld_in
+    idrv = 0
+    call rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
+                  h2o, o3, co2, ch4, n2o, o2, cfc11, cfc12, cfc22, ccl4, emis, &
+                  0, 0, 0, cldfr, taucld, cicewp, cliqwp, tencl, tencl, tauaer, &
+                  uflx, dflx, hr, uflxc, dflxc, hrc)
+  end subroutine ref_lw_run_cloud
 
   subroutine ref_lw_column(nlay, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, &
                            cfc11, cfc12, cfc22, ccl4, &

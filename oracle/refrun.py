@@ -101,3 +101,21 @@ def run_lw(lib, c, tauaer=None):
             np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
     lib.ref_lw_run(n, L, *[_p(a) for a in ins], _p(tau), *[_p(a) for a in outs])
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
+def run_lw_cloud(lib, c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None):
+    """rrtmg_lw of the reference with clouds (icld 1..3, inflglw = 0).
+
+    cldfr (nlay, ncol) and taucld (nlay, ncol, 16) in this package's C order,
+    i.e. the reference's cldfr(ncol, nlay) and taucld(nbndlw, ncol, nlay)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in LW_IN]
+    z = np.zeros((L, n))
+    tau = np.zeros((16, L, n)) if tauaer is None else np.ascontiguousarray(tauaer, dtype=np.float64)
+    cl = [np.ascontiguousarray(a, dtype=np.float64) for a in
+          (cldfr, taucld, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp)]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_lw_run_cloud(n, L, int(icld), *[_p(a) for a in ins], *[_p(a) for a in cl], _p(tau),
+                         *[_p(a) for a in outs])
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
