@@ -20,9 +20,11 @@
  *     without synchronising.
  *   - Return 0 on success, a negative RRTMG_HIP_E* code on failure; outputs
  *     are then left untouched and rrtmg_hip_last_error() describes the cause.
- *     Clear-sky contract: icld != 0, SW iaer != 0 or LW idrv != 0 return
- *     RRTMG_HIP_ENOTSUP (MiMA passes icld = idrv = iaer = 0,
- *     RR/rrtm_radiation.f90:106-108).
+ *     Contract: rrtmg_hip_sw and rrtmg_hip_lw are clear sky (MiMA passes
+ *     icld = idrv = iaer = 0, RR/rrtm_radiation.f90:106-108); icld != 0,
+ *     SW iaer != 0 or LW idrv != 0 return RRTMG_HIP_ENOTSUP there.  LW
+ *     clouds with in-cloud optical depths (inflglw = 0) take
+ *     rrtmg_hip_lw_cloud.
  */
 #ifndef RRTMG_HIP_H
 #define RRTMG_HIP_H
@@ -100,6 +102,34 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
                  double *uflx, double *dflx, double *hr,
                  double *uflxc, double *dflxc, double *hrc,
                  int ptr_kind, void *stream);
+
+/* Replaces rrtmg_lw with clouds (rrtmg_lw/gcm_model/src/rrtmg_lw_rad.nomcica.f90:80-89):
+ * the reference's whole argument list in its order.  icld = 1: random
+ * overlap (rrtmg_lw_rtrn.f90); 2 or 3: maximum-random overlap
+ * (rrtmg_lw_rtrnmr.f90; the nomcica driver runs 3 as 2, :527-540); icld
+ * outside 0..3 is taken as 2 (:437); icld = 0 is rrtmg_hip_lw (cloud arrays
+ * unread).  inflglw = 0 only (cldprop, rrtmg_lw_cldprop.f90:154-176):
+ * taucld (16, ncol, nlay), band fastest, holds the in-cloud optical depths;
+ * a layer's are used where cldfr (ncol, nlay) >= 1e-20 and cicewp + cliqwp
+ * (ncol, nlay; NULL = 0) or the layer's summed depth >= 1e-20, and the layer
+ * is cloudy for the transfer where cldfr >= 1e-6.  iceflglw, liqflglw,
+ * reice and reliq are not read (inflglw = 0 ignores them).  Outputs:
+ * total-sky uflx, dflx, hr and clear-sky uflxc, dflxc, hrc.  inflglw != 0,
+ * idrv != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc,
+                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                       const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *cfc11vmr, const double *cfc12vmr,
+                       const double *cfc22vmr, const double *ccl4vmr,
+                       const double *emis, int inflglw, int iceflglw, int liqflglw,
+                       const double *cldfr, const double *taucld,
+                       const double *cicewp, const double *cliqwp,
+                       const double *reice, const double *reliq, const double *tauaer,
+                       double *uflx, double *dflx, double *hr,
+                       double *uflxc, double *dflxc, double *hrc,
+                       int ptr_kind, void *stream);
 
 /* Backward (vector-Jacobian product) of rrtmg_lw.  No reference counterpart:
  * the reference's only derivative is idrv = 1, dF_up/dTs of the surface

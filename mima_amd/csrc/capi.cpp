@@ -650,18 +650,25 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
                       swdflxc, swhrc, ptr_kind, stream, nullptr);
 }
 
-int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,
-                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
-                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
-                 const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
-                 const double *emis, const double *tauaer, double *uflx, double *dflx, double *hr, double *uflxc,
-                 double *dflxc, double *hrc, int ptr_kind, void *stream)
+// rrtmg_hip_lw and rrtmg_hip_lw_cloud: cl NULL or cl->icld == 0 runs the
+// clear-sky kernels, otherwise the cloud kernels (the cloud arrays already
+// checked by the caller)
+static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev,
+                      const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                      const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                      const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                      const double *cfc22vmr, const double *ccl4vmr, const double *emis, const double *tauaer,
+                      const LwClouds *cl, double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc,
+                      double *hrc, int ptr_kind, void *stream)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
     std::lock_guard<std::mutex> lk(s->mu);
     if (!s->lwd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_lw called before rrtmg_hip_lw_init");
-    if (icld != 0) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: only the clear-sky path is implemented");
+    const bool cloudy = cl && cl->icld != 0;
+    if (icld != 0 && !cloudy) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: rrtmg_hip_lw is clear sky; clouds take rrtmg_hip_lw_cloud");
+    if (cloudy && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
+        return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
     if (idrv != 0) return fail(RRTMG_HIP_ENOTSUP, "idrv != 0: dF/dTs output is not implemented");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
     if (rc) return rc;
@@ -694,6 +701,12 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
         a.push_back({"tsfc", tsfc, (size_t)ncol});
         a.push_back({"emis", emis, (size_t)ncol * 16});
         if (tauaer) a.push_back({"tauaer", tauaer, n2 * 16});
+        if (cloudy) {
+            a.push_back({"cldfr", cl->cldfr, n2});
+            a.push_back({"taucld", cl->taucld, n2 * 16});
+            if (cl->cicewp) a.push_back({"cicewp", cl->cicewp, n2});
+            if (cl->cliqwp) a.push_back({"cliqwp", cl->cliqwp, n2});
+        }
         if ((rc = device_checks(W, st, a, plev, ncol, nlay))) return rc;
     }
     const int mixed = g_prec == RRTMG_HIP_PREC_MIXED;
@@ -715,6 +728,12 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
                                 : (size_t)chunk * lw_fwd_state_bytes(nlay, rs);
     const size_t o_ds = off; off = align_up(off + ds_bytes);
     const size_t o_it = off; off = align_up(off + sizeof(int));
+    // cloud path: flags, overlap records, four-stream band fluxes, scratch rows
+    const int cchunk = vjp_chunk((ncol + TILE_COLS - 1) / TILE_COLS * s->lw.tab.nwaves, lw_cloud_row_bytes(nlay, rs));
+    const size_t o_cfl = off; if (cloudy) off = align_up(off + sizeof(int) * n2);
+    const size_t o_cfa = off; if (cloudy) off = align_up(off + rs * LWC_STRIDE * n2);
+    const size_t o_cbf = off; if (cloudy) off = align_up(off + rs * 16 * 4 * n3);
+    const size_t o_crw = off; if (cloudy) off = align_up(off + (size_t)cchunk * lw_cloud_row_bytes(nlay, rs));
     if ((rc = W.lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)W.lw_work.p;
     LwWork w;
@@ -728,14 +747,25 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     w.dstate = wb + o_ds;
     w.chunk = chunk;
     w.items = (int *)(wb + o_it);
+    LwCloudWork cw;
+    LwClouds cld{};
+    if (cloudy) {
+        cw.flags = (int *)(wb + o_cfl);
+        cw.fac = wb + o_cfa;
+        cw.bflux = wb + o_cbf;
+        cw.rows = wb + o_crw;
+        cw.chunk = cchunk;
+        cld = *cl;
+    }
 
     ColumnInputs in{};
     in.ncol = ncol;
     in.nlay = nlay;
     LwOutputs out{};
     if (ptr_kind == RRTMG_HIP_HOST) {
+        const size_t ncl = cloudy ? 17 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) : 0;
         const size_t need = sizeof(double) * (12 * n2 + 2 * n3 + (size_t)ncol * 17 + (tauaer ? 16 * n2 : 0) +
-                                              4 * n3 + 2 * n2);
+                                              ncl + 4 * n3 + 2 * n2);
         if ((rc = W.lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
         double *p = (double *)W.lw_stage.p;
         const double **dst2[] = {&in.play, &in.tlay, &in.h2o, &in.o3, &in.co2, &in.ch4,
@@ -762,6 +792,17 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
             in.tauaer = p;
             p += n2 * 16;
         }
+        if (cloudy) {
+            const double *src[] = {cl->cldfr, cl->taucld, cl->cicewp, cl->cliqwp};
+            const double **dst[] = {&cld.cldfr, &cld.taucld, &cld.cicewp, &cld.cliqwp};
+            const size_t len[] = {n2, 16 * n2, n2, n2};
+            for (int i = 0; i < 4; ++i) {
+                if (!src[i]) continue;
+                ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
+                *dst[i] = p;
+                p += len[i];
+            }
+        }
         out.uflx = p; p += n3;
         out.dflx = p; p += n3;
         out.uflxc = p; p += n3;
@@ -786,9 +827,10 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
     t.plan = s->lwd.plan;
     t.lds = lds;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "LW: host-to-device copy failed");
-    hipError_t e = launch_lw(in, out, w, t, g_prec, st, s->events(1));
+    hipError_t e = cloudy ? launch_lw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(1))
+                          : launch_lw(in, out, w, t, g_prec, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
-    s->lw_w = w;
+    if (!cloudy) s->lw_w = w;
     s->lw_last_ncol = ncol;
     s->lw_last_nlay = nlay;
     if (ptr_kind == RRTMG_HIP_HOST) {
@@ -802,6 +844,59 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, con
         if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,
+                 const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                 const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                 const double *cfc11vmr, const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                 const double *emis, const double *tauaer, double *uflx, double *dflx, double *hr, double *uflxc,
+                 double *dflxc, double *hrc, int ptr_kind, void *stream)
+{
+    return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, nullptr, uflx, dflx, hr, uflxc,
+                      dflxc, hrc, ptr_kind, stream);
+}
+
+int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev,
+                       const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                       const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                       const double *cfc22vmr, const double *ccl4vmr, const double *emis, int inflglw,
+                       int iceflglw, int liqflglw, const double *cldfr, const double *taucld,
+                       const double *cicewp, const double *cliqwp, const double *reice, const double *reliq,
+                       const double *tauaer, double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc,
+                       double *hrc, int ptr_kind, void *stream)
+{
+    (void)iceflglw;
+    (void)liqflglw;
+    (void)reice;
+    (void)reliq;
+    // rrtmg_lw_rad.nomcica.f90:437: icld outside 0..3 selects maximum-random overlap
+    if (icld < 0 || icld > 3) icld = 2;
+    if (icld == 0)
+        return lw_forward(ncol, nlay, 0, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                          n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, nullptr, uflx, dflx,
+                          hr, uflxc, dflxc, hrc, ptr_kind, stream);
+    if (inflglw != 0)
+        return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0: cloud optics from water paths and effective radii are not "
+                                       "implemented; pass in-cloud optical depths with inflglw = 0");
+    if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
+        const size_t n2 = (size_t)ncol * nlay;
+        int rc;
+        if (ptr_kind == RRTMG_HIP_HOST) {
+            if ((rc = check_host_array(cldfr, n2, "cldfr")) || (rc = check_host_array(taucld, n2 * 16, "taucld")))
+                return rc;
+            if (cicewp && (rc = check_host_array(cicewp, n2, "cicewp"))) return rc;
+            if (cliqwp && (rc = check_host_array(cliqwp, n2, "cliqwp"))) return rc;
+        } else if (!cldfr || !taucld) {
+            return fail(RRTMG_HIP_EINVAL, "null cloud array");
+        }
+    }
+    LwClouds cl{icld == 1 ? 1 : 2, cldfr, taucld, cicewp, cliqwp};
+    return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &cl, uflx, dflx, hr, uflxc, dflxc,
+                      hrc, ptr_kind, stream);
 }
 
 // backward of rrtmg_sw; with swuflx/swdflx/swhr given, also the forward's

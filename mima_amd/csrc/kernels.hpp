@@ -102,6 +102,44 @@ struct LwDevTables {
 };
 
 // ---------------------------------------------------------------------------
+// LW with clouds (rrtmg_lw, icld = 1..3, inflglw = 0), device FP64 inputs
+struct LwClouds {
+    int icld;                         // 1 random overlap (rtrn), 2 and 3 maximum-random (rtrnmr)
+    const double *cldfr;              // [nlay][ncol]
+    const double *taucld;             // [nlay][ncol][16]: the reference's taucld(nbndlw, ncol, nlay)
+    const double *cicewp, *cliqwp;    // [nlay][ncol] or NULL (zero)
+};
+// Per-(layer, column) record of the cloud column kernel, in the band
+// kernel's precision: the layer's fraction, then rtrnmr's six upward
+// overlap factors of index lay+1 and its six downward ones of index lay-1
+// (1-based lay; rrtmg_lw_rtrnmr.f90:325-476).
+enum LwCloudField {
+    LWC_CF,
+    LWC_CLD1, LWC_CLD2, LWC_CLR1, LWC_CLR2, LWC_CMB1, LWC_CMB2,
+    LWC_CLD1D, LWC_CLD2D, LWC_CLR1D, LWC_CLR2D, LWC_CMB1D, LWC_CMB2D,
+    LWC_COUNT
+};
+constexpr int LWC_STRIDE = 16;
+// flag bits per (layer, column): cloudy for the transfer (fraction >= 1e-6),
+// first cloudy layer of a block going up (istcld) and going down (istcldd),
+// and cldprop's cloud test (the layer's taucld is used)
+constexpr int LWC_CLOUDY = 1, LWC_TOP_UP = 2, LWC_TOP_DN = 4, LWC_TAU = 8;
+struct LwCloudWork {
+    int *flags = nullptr;     // [nlay][ncol]
+    void *fac = nullptr;      // [nlay][ncol][LWC_STRIDE] of Real
+    void *bflux = nullptr;    // [16][4][nlay+1][ncol] of Real: total up, total down, clear up, clear down
+    void *rows = nullptr;     // [chunk][4][nlay][64] of Real: per-layer transmittances and sources of a wave
+    int chunk = 0;            // wave units per band-kernel launch (rows)
+};
+inline size_t lw_cloud_row_bytes(int nlay, size_t real_size) { return real_size * 4 * (size_t)nlay * 64; }
+// mode 0 (FP64) or 1 (FP32 with FP64 islands); w: the forward's prep
+// workspace (coef, tlev, ipk, ilev, laytrop, colscal)
+hipError_t launch_lw_cloud(const ColumnInputs &in, const LwClouds &cl, const LwOutputs &out, const LwWork &w,
+                           const LwCloudWork &cw, const LwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
+// the clear-sky forward's prep and column kernels alone
+hipError_t launch_lw_prep(const ColumnInputs &in, const LwWork &w, const LwDevTables &t, int mode, hipStream_t s);
+
+// ---------------------------------------------------------------------------
 // backward (vector-Jacobian product) of one call
 struct VjpCot {
     const double *uflx, *dflx, *hr;  // cotangents of the outputs (NULL = zero)

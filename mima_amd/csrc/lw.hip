@@ -475,6 +475,18 @@ static hipError_t launch_lw_t(const ColumnInputs &in, const LwOutputs &out, cons
     return hipGetLastError();
 }
 
+hipError_t launch_lw_prep(const ColumnInputs &in, const LwWork &w, const LwDevTables &t, int mode, hipStream_t s)
+{
+    const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
+    if (mode == 1)
+        hipLaunchKernelGGL(lw_prep_kernel<float>, dim3(nbl), dim3(256), 0, s, in, w, t.cst);
+    else
+        hipLaunchKernelGGL(lw_prep_kernel<double>, dim3(nbl), dim3(256), 0, s, in, w, t.cst);
+    hipLaunchKernelGGL(lw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
+    return hipGetLastError();
+}
+
 hipError_t launch_lw_finish(const ColumnInputs &in, const LwOutputs &out, void *bflux, const LwDevTables &t,
                             int mode, hipStream_t s)
 {

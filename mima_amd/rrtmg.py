@@ -162,6 +162,30 @@ def rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr
     return tuple(outs)
 
 
+def rrtmg_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                   n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, inflglw, iceflglw, liqflglw,
+                   cldfr, taucld, cicewp=None, cliqwp=None, reice=None, reliq=None, tauaer=None, stream=None):
+    """rrtmg_lw with clouds (rrtmg_lw_rad.nomcica.f90:80-89): icld 1 random,
+    2 or 3 maximum-random overlap; inflglw = 0 (in-cloud optical depths).
+
+    ``cldfr``, ``cicewp``, ``cliqwp`` (nlay, ncol); ``taucld`` (nlay, ncol,
+    16), i.e. the reference's taucld(nbndlw, ncol, nlay).  Returns (uflx,
+    dflx, hr, uflxc, dflxc, hrc): total-sky, then clear-sky."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_lw_cloud(
+        int(ncol), int(nlay), int(icld), int(idrv),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis)],
+        int(inflglw), int(iceflglw), int(liqflglw),
+        *[a.inp(x) for x in (cldfr, taucld, cicewp, cliqwp, reice, reliq, tauaer)], *optr,
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
 def _grad_outputs(dev, like, shapes):
     if dev:
         import torch
@@ -250,6 +274,13 @@ def sw_columns(c, stream=None):
 
 def lw_columns(c, tauaer=None, stream=None):
     out = rrtmg_lw(c["ncol"], c["nlay"], 0, 0, *[c[k] for k in LW_KEYS], tauaer=tauaer, stream=stream)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
+
+
+def lw_cloud_columns(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None, stream=None):
+    """rrtmg_lw_cloud (inflglw = 0) on a mima_amd.columns dict."""
+    out = rrtmg_lw_cloud(c["ncol"], c["nlay"], icld, 0, *[c[k] for k in LW_KEYS], 0, 0, 0, cldfr, taucld,
+                         cicewp=cicewp, cliqwp=cliqwp, tauaer=tauaer, stream=stream)
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
 
 
