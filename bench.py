@@ -199,6 +199,64 @@ def adjoint_timing(torch, rrtmg, config_columns, args):
             "backward_vs_forward_cost": res["backward"] / res["forward"]}
 
 
+def cloudy_timing(torch, rrtmg, config_columns, args):
+    """LW with clouds (rrtmg_hip_lw_cloud, maximum-random overlap, in-cloud
+    optical depths) on the benchmark grid (256 x 128 = 32768 columns x 40
+    levels) with seeded cloud blocks: columns/s in FP32-mixed and FP64, the
+    per-kernel times of the mixed call, the mixed path's largest difference
+    from FP64 and the cost relative to the clear-sky LW call."""
+    from mima_amd.columns import make_clouds
+    import ctypes
+    cols = config_columns("c1_mixed_vs_fp64", seed=0)
+    ncol, nlay = cols["ncol"], cols["nlay"]
+    cf, tau = make_clouds(cols, seed=0, edge_cases=False)
+    d = rrtmg.to_device(cols)
+    dcf = torch.as_tensor(cf, device="cuda")
+    dtau = torch.as_tensor(tau, device="cuda")
+
+    def timed(fn):
+        for _ in range(max(args.warmup, 3)):
+            fn()
+        torch.cuda.synchronize()
+        n = max(3, args.steps // 2)
+        t0 = time.perf_counter()
+        for _ in range(n):
+            o = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n, o
+
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        rrtmg.set_precision(prec)
+        res[prec], o = timed(lambda: rrtmg.lw_cloud_columns(d, 2, dcf, dtau))
+        outs[prec] = {k: v.cpu().numpy() for k, v in o.items()}
+    clear_ms, _ = timed(lambda: rrtmg.lw_columns(d))
+    rrtmg.lib().rrtmg_hip_set_timing(1)
+    kt = np.zeros(3)
+    for _ in range(5):
+        rrtmg.lw_cloud_columns(d, 2, dcf, dtau)
+        ms = (ctypes.c_float * 3)()
+        if rrtmg.lib().rrtmg_hip_last_kernel_ms(1, ms) == 0:
+            kt += np.array(ms[:])
+    rrtmg.lib().rrtmg_hip_set_timing(0)
+    kt /= 5
+    dfl = max(float(np.max(np.abs(outs["mixed"][k] - outs["fp64"][k]))) for k in ("uflx", "dflx"))
+    dhr = float(np.max(np.abs(outs["mixed"]["hr"] - outs["fp64"]["hr"])))
+    print("[bench] cloudy LW %d x %d: mixed %.3f ms, fp64 %.3f ms, clear-sky LW %.3f ms" %
+          (ncol, nlay, res["mixed"] * 1e3, res["fp64"] * 1e3, clear_ms * 1e3), file=sys.stderr, flush=True)
+    bcol = 8 * (29 * nlay + 10)   # the clear-sky LW bytes plus cldfr and taucld (17 doubles per layer)
+    return {"workload": "LW with clouds (icld = 2 maximum-random overlap, inflglw = 0), 256x128 grid = %d columns "
+                        "x %d levels, 0-3 seeded cloud blocks per column (%.0f%% of layers cloudy)"
+                        % (ncol, nlay, 100.0 * float(np.mean(cf > 0))),
+            "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
+            "fp64_ms_per_step": res["fp64"] * 1e3, "clear_sky_lw_ms": clear_ms * 1e3,
+            "cloudy_vs_clear_cost": res["mixed"] / clear_ms,
+            "kernel_ms": {"prep_and_overlap": float(kt[0]), "band": float(kt[1]), "finish": float(kt[2])},
+            "band_kernel_GBps": bcol * ncol / (kt[1] * 1e-3) / 1e9 if kt[1] > 0 else None,
+            "bytes_per_column": bcol,
+            "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
+
+
 def surrogate_timing(torch, rrtmg, config_columns, args):
     """MLP surrogate: trained on the GPU kernels on 262144 columns (512 x 512,
     lapse rate U(4, 9) K/km, RH U(0.2, 1)) labelled by the exact FP32-mixed
@@ -382,8 +440,8 @@ def main():
     ap.add_argument("--config", default="c1_mixed_vs_fp64")
     ap.add_argument("--surrogate-epochs", type=int, default=10,
                     help="training epochs of the MLP surrogate (0 skips the surrogate leg)")
-    ap.add_argument("--legs", default="forward,adjoint,surrogate,large",
-                    help="comma-separated legs to run (forward, adjoint, surrogate, large); the JSON line is the "
+    ap.add_argument("--legs", default="forward,adjoint,surrogate,large,cloudy",
+                    help="comma-separated legs to run (forward, adjoint, surrogate, large, cloudy); the JSON line is the "
                          "benchmark line only when forward is among them (the others are for profiling one leg)")
     ap.add_argument("--large-steps", type=int, default=5,
                     help="timed steps of the 131072 x 60 forward+backward leg (0 skips it)")
@@ -413,6 +471,8 @@ def main():
             out["surrogate"] = surrogate_timing(torch, rrtmg, config_columns, args)
         if "large" in legs and args.large_steps > 0:
             out["large_batch"] = large_batch_timing(torch, rrtmg, config_columns, args)
+        if "cloudy" in legs:
+            out["cloudy_lw"] = cloudy_timing(torch, rrtmg, config_columns, args)
         print(json.dumps(out))
         return
     cols = config_columns(args.config, seed=0)
@@ -473,6 +533,7 @@ def main():
     adj = adjoint_timing(torch, rrtmg, config_columns, args) if "adjoint" in legs else None
     sur = surrogate_timing(torch, rrtmg, config_columns, args) if "surrogate" in legs and args.surrogate_epochs > 0 else None
     large = large_batch_timing(torch, rrtmg, config_columns, args) if "large" in legs and args.large_steps > 0 else None
+    cloudy = cloudy_timing(torch, rrtmg, config_columns, args) if "cloudy" in legs else None
 
     cpu = None
     if not args.no_cpu:
@@ -528,6 +589,7 @@ def main():
         "adjoint": adj,
         "surrogate": sur,
         "large_batch": large,
+        "cloudy_lw": cloudy,
     }
     print(json.dumps(out))
 

@@ -733,7 +733,8 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     const size_t o_cfl = off; if (cloudy) off = align_up(off + sizeof(int) * n2);
     const size_t o_cfa = off; if (cloudy) off = align_up(off + rs * LWC_STRIDE * n2);
     const size_t o_cbf = off; if (cloudy) off = align_up(off + rs * 16 * 4 * n3);
-    const size_t o_crw = off; if (cloudy) off = align_up(off + (size_t)cchunk * lw_cloud_row_bytes(nlay, rs));
+    const size_t crows = lds ? (size_t)lw_lds_grid(s->lwd.plan, ncol, s->lw.tab.nwaves) * LW_WGW : (size_t)cchunk;
+    const size_t o_crw = off; if (cloudy) off = align_up(off + crows * lw_cloud_row_bytes(nlay, rs));
     if ((rc = W.lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)W.lw_work.p;
     LwWork w;

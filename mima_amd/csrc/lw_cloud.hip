@@ -9,12 +9,13 @@
 // coefficients.  lw_cloud_column_kernel (one thread per column, FP64) runs
 // cldprop's cloud test, rtrnmr's cloudy-layer test and its two overlap-factor
 // sweeps, which carry state from layer to layer, and stores one record per
-// (layer, column).  lw_cloud_band_kernel follows the clear-sky band kernel's
-// decomposition (one wave per work unit, lane = (column, g)) with the cloudy
-// branches of the transfer; a cloudy layer keeps four per-layer terms for
-// the upward sweep instead of two, in a global scratch row per wave, and
-// each level sums four radiances over g-points (total and clear, up and
-// down).  The cloud path is separate from the clear-sky kernels, whose code
+// (layer, column).  The band kernels follow the clear-sky band kernels'
+// decomposition (one wave per work unit, lane = (column, g); FP32 with the
+// band's k-tables in LDS in persistent workgroups, FP64 one wave per
+// workgroup) with the cloudy branches of the transfer; a cloudy layer keeps
+// four per-layer terms for the upward sweep instead of two, in a global
+// scratch row per wave, and each level sums four radiances over g-points
+// (total and clear, up and down).  The cloud path is separate from the clear-sky kernels, whose code
 // it leaves unchanged.
 #include <hip/hip_runtime.h>
 
@@ -39,7 +40,7 @@ namespace rrtmg {
 // cldfrac(nlay+1) are taken as 0, the latter as the driver sets it
 // (rrtmg_lw_rad.nomcica.f90:736).
 template <typename Real>
-__global__ __launch_bounds__(256) void lw_cloud_column_kernel(int ncol, int nlay, LwClouds cl, LwCloudWork cw)
+__global__ __launch_bounds__(64) void lw_cloud_column_kernel(int ncol, int nlay, LwClouds cl, LwCloudWork cw)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= ncol) return;
@@ -274,13 +275,16 @@ struct Overlap {
 };
 
 // One band of one work unit with clouds.  ICLD 1: random overlap (rtrn);
-// 2: maximum-random (rtrnmr).  row: this wave's scratch, [4][nlay][64]
+// 2: maximum-random (rtrnmr).  ktab: the k-tables (B's offsets index it: the
+// packed global buffer, or the band's copy in LDS with rebased offsets);
+// tab: the packed global buffer (transmittance and Planck tables).
+// row: this wave's scratch, [4][nlay][64]
 // (gas transmittance, gas upward source, gas-and-cloud absorptance and
 // upward source); s_red: 64 LDS entries for the FP64 g-sum.
 template <int BAND, typename Real, int ICLD>
 __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const LwWork &w, const LwClouds &cl,
-                                                   const LwCloudWork &cw, const LwBand &B, const Real *__restrict__ tab,
-                                                   int exp_off, int tfn_off, int plk_off,
+                                                   const LwCloudWork &cw, const LwBand &B, const Real *__restrict__ ktab,
+                                                   const Real *__restrict__ tab, int exp_off, int tfn_off, int plk_off,
                                                    const LwConst *__restrict__ cst, WaveUnit u, int tile, int lane,
                                                    Real *__restrict__ row, Real *s_red)
 {
@@ -297,7 +301,7 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
     const Real *tfn_tbl = tab + tfn_off;
     const Real *plk = tab + plk_off + (BAND - 1) * 181;
     const Real *fac = (const Real *)cw.fac;
-    LTab<Real> T{tab, NG, g};
+    LTab<Real> T{ktab, NG, g};
     const size_t bs = (size_t)(nlay + 1) * ncol;
     Real *bf = (Real *)cw.bflux + (size_t)(BAND - 1) * 4 * bs;  // [4][nlay+1][ncol]
     Real *s_atr = row, *s_bbu = row + (size_t)nlay * 64, *s_atot = row + 2 * (size_t)nlay * 64,
@@ -392,10 +396,15 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
         const unsigned ix = (unsigned)((lev - 1) * 64 + lane);
         at32(s_atr, ix) = atrans;
         at32(s_bbu, ix) = bbugas;
-        at32(s_atot, ix) = atot;
-        at32(s_bbt, ix) = bbutot;
-        store(1, lev - 1, gsum.sum(active ? radld : (Real)0, s_red));
-        store(3, lev - 1, gsum.sum(active ? radclrd : (Real)0, s_red));
+        if (fl & LWC_CLOUDY) {
+            at32(s_atot, ix) = atot;
+            at32(s_bbt, ix) = bbutot;
+        }
+        const Real sd_tot = gsum.sum(active ? radld : (Real)0, s_red);
+        store(1, lev - 1, sd_tot);
+        // while no column of the wave has met a cloud both streams are equal
+        const bool met = __builtin_amdgcn_ballot_w64(iclddn) != 0;
+        store(3, lev - 1, met ? gsum.sum(active ? radclrd : (Real)0, s_red) : sd_tot);
         plev_up = plev_dn;
         if (lev == 1) fracs1 = plfrac;
     }
@@ -403,8 +412,11 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
     const Real rad0 = fracs1 * plankbnd, reflect = one - semiss;
     Real radlu = rad0 + reflect * radld;
     Real radclru = rad0 + reflect * radclrd;
-    store(0, 0, gsum.sum(active ? radlu : (Real)0, s_red));
-    store(2, 0, gsum.sum(active ? radclru : (Real)0, s_red));
+    // no column of the wave cloudy: the clear stream is the total one throughout
+    const bool any_cloud = __builtin_amdgcn_ballot_w64(iclddn) != 0;
+    const Real su0 = gsum.sum(active ? radlu : (Real)0, s_red);
+    store(0, 0, su0);
+    store(2, 0, any_cloud ? gsum.sum(active ? radclru : (Real)0, s_red) : su0);
     ov = Overlap<Real>{(Real)0, (Real)0, (Real)0};
     for (int lev = 1; lev <= nlay; ++lev) {
         const unsigned ix = (unsigned)((lev - 1) * 64 + lane);
@@ -425,8 +437,9 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
             radlu = radlu + (bbugas - radlu) * atrans;
         }
         radclru = iclddn ? radclru + (bbugas - radclru) * atrans : radlu;
-        store(0, lev, gsum.sum(active ? radlu : (Real)0, s_red));
-        store(2, lev, gsum.sum(active ? radclru : (Real)0, s_red));
+        const Real su = gsum.sum(active ? radlu : (Real)0, s_red);
+        store(0, lev, su);
+        store(2, lev, any_cloud ? gsum.sum(active ? radclru : (Real)0, s_red) : su);
     }
 }
 
@@ -447,7 +460,7 @@ __global__ __launch_bounds__(64) void lw_cloud_band_kernel(ColumnInputs in, LwWo
     Real *row = (Real *)cw.rows + (size_t)blk * 4 * in.nlay * 64;
 #define BODY(b)                                                                                                 \
     case b - 1:                                                                                                 \
-        lw_cloud_band_body<b, Real, ICLD>(in, w, cl, cw, B, tab, exp_off, tfn_off, plk_off, cst, u, tile,        \
+        lw_cloud_band_body<b, Real, ICLD>(in, w, cl, cw, B, tab, tab, exp_off, tfn_off, plk_off, cst, u, tile,   \
                                           (int)threadIdx.x, row, s_red);                                        \
         break;
     switch (u.band) {
@@ -455,6 +468,59 @@ __global__ __launch_bounds__(64) void lw_cloud_band_kernel(ColumnInputs in, LwWo
         BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
     }
 #undef BODY
+}
+
+// FP32 with the band's k-tables in LDS: the schedule of the clear-sky
+// lw_band_lds_kernel (persistent workgroups of LW_WGW waves claiming work
+// items from a global counter; item = (group of LW_WGW tiles, unit r of a
+// tile's unit list); a band's tables copied into LDS before its first item)
+// with the cloudy band body.  Each wave of the persistent grid has one
+// scratch row of 4 x nlay x 64 floats.
+template <int ICLD>
+__global__ __launch_bounds__(64 * LW_WGW) void lw_cloud_band_lds_kernel(ColumnInputs in, LwWork w, LwClouds cl,
+                                                                        LwCloudWork cw, const LwBand *__restrict__ bands,
+                                                                        LwLdsPlan plan, int nwaves, int nitems,
+                                                                        int exp_off, int tfn_off, int plk_off,
+                                                                        const float *__restrict__ tab,
+                                                                        const LwConst *__restrict__ cst,
+                                                                        const WaveUnit *__restrict__ units)
+{
+    extern __shared__ unsigned char smem_raw[];
+    float *ktab = (float *)smem_raw;
+    const int nlay = in.nlay;
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    float *row = (float *)cw.rows + ((size_t)blockIdx.x * LW_WGW + wave) * 4 * nlay * 64;
+    __shared__ int s_item;
+    int loaded = -1;
+    for (;;) {
+        const int item = next_item(w.items, &s_item);
+        if (item >= nitems) break;
+        const int group = item / nwaves, r = item - group * nwaves;
+        const int b = units[r].band;
+        if (b != loaded) {
+            const int t0 = plan.tab0[b], n = plan.tab0[b + 1] - t0;
+#pragma unroll 8
+            for (int i = threadIdx.x; i < n; i += 64 * LW_WGW) ktab[i] = tab[t0 + i];
+            __syncthreads();
+            loaded = b;
+        }
+        const int k = (r - plan.ubeg[b]) * LW_WGW + wave;
+        const int tile = group * LW_WGW + k / plan.upt[b];
+        if (tile >= ntiles) continue;
+        const WaveUnit u = units[plan.ubeg[b] + k % plan.upt[b]];
+        const LwBand B = bands[b];
+#define BODY(bb)                                                                                              \
+    case bb - 1:                                                                                              \
+        lw_cloud_band_body<bb, float, ICLD>(in, w, cl, cw, B, ktab, tab, exp_off, tfn_off, plk_off, cst, u,   \
+                                            tile, lane, row, nullptr);                                        \
+        break;
+        switch (b) {
+            BODY(1) BODY(2) BODY(3) BODY(4) BODY(5) BODY(6) BODY(7) BODY(8)
+            BODY(9) BODY(10) BODY(11) BODY(12) BODY(13) BODY(14) BODY(15) BODY(16)
+        }
+#undef BODY
+    }
 }
 
 // finish: band sums, fluxfac, net fluxes and heating of both streams
@@ -501,15 +567,37 @@ static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, 
                                     const LwWork &w, const LwCloudWork &cw, const LwDevTables &t, int mode,
                                     hipStream_t s, hipEvent_t *ev)
 {
-    const int nb = (in.ncol + 255) / 256;
     const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     if (ev) (void)hipEventRecord(ev[0], s);
     hipError_t e = launch_lw_prep(in, w, t, mode, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(lw_cloud_column_kernel<Real>, dim3(nb), dim3(256), 0, s, in.ncol, in.nlay, cl, cw);
+    // one thread per column, sequential over layers: 64-thread blocks spread
+    // the columns over all compute units
+    hipLaunchKernelGGL(lw_cloud_column_kernel<Real>, dim3((in.ncol + 63) / 64), dim3(64), 0, s, in.ncol, in.nlay, cl,
+                       cw);
     if (ev) (void)hipEventRecord(ev[1], s);
     const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
     const int nblocks = ntiles * t.desc.nwaves;
+    if constexpr (sizeof(Real) == 4) {
+        if (t.lds) {
+            const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * t.desc.nwaves;
+            const dim3 grid(lw_lds_grid(t.plan, in.ncol, t.desc.nwaves)), block(64 * LW_WGW);
+            hipLaunchKernelGGL(reset_items_kernel, dim3(1), dim3(1), 0, s, w.items);
+            if (cl.icld == 1)
+                hipLaunchKernelGGL(lw_cloud_band_lds_kernel<1>, grid, block, (size_t)t.plan.lds_bytes, s, in, w, cl,
+                                   cw, t.bands_lds, t.plan, t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl,
+                                   t.desc.totplnk, (const float *)t.tab, t.cst, t.units);
+            else
+                hipLaunchKernelGGL(lw_cloud_band_lds_kernel<2>, grid, block, (size_t)t.plan.lds_bytes, s, in, w, cl,
+                                   cw, t.bands_lds, t.plan, t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl,
+                                   t.desc.totplnk, (const float *)t.tab, t.cst, t.units);
+            if (ev) (void)hipEventRecord(ev[2], s);
+            hipLaunchKernelGGL(lw_cloud_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cw, out, t.bands,
+                               t.heatfac);
+            if (ev) (void)hipEventRecord(ev[3], s);
+            return hipGetLastError();
+        }
+    }
     for (int b0 = 0; b0 < nblocks; b0 += cw.chunk) {
         const int nb_c = std::min(cw.chunk, nblocks - b0);
         if (cl.icld == 1)

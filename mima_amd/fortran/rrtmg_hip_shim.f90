@@ -88,6 +88,22 @@ module rrtmg_hip_capi
       real(c_double), intent(out) :: uflx(*), dflx(*), hr(*), uflxc(*), dflxc(*), hrc(*)
       type(c_ptr), value :: stream
     end function
+    integer(c_int) function c_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
+                                       h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                                       cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, &
+                                       inflglw, iceflglw, liqflglw, cldfr, taucld, cicewp, cliqwp, &
+                                       reice, reliq, tauaer, &
+                                       uflx, dflx, hr, uflxc, dflxc, hrc, ptr_kind, stream) &
+                                       bind(C, name="rrtmg_hip_lw_cloud")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: ncol, nlay, icld, idrv, inflglw, iceflglw, liqflglw, ptr_kind
+      real(c_double), intent(in) :: play(*), plev(*), tlay(*), tlev(*), tsfc(*)
+      real(c_double), intent(in) :: h2ovmr(*), o3vmr(*), co2vmr(*), ch4vmr(*), n2ovmr(*), o2vmr(*)
+      real(c_double), intent(in) :: cfc11vmr(*), cfc12vmr(*), cfc22vmr(*), ccl4vmr(*), emis(*)
+      real(c_double), intent(in) :: cldfr(*), taucld(*), cicewp(*), cliqwp(*), reice(*), reliq(*), tauaer(*)
+      real(c_double), intent(out) :: uflx(*), dflx(*), hr(*), uflxc(*), dflxc(*), hrc(*)
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function c_lw_vjp(ncol, nlay, play, plev, tlay, tlev, tsfc, &
                                      h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
                                      cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &
@@ -192,12 +208,22 @@ contains
     real(kind=rb), intent(in) :: tauaer(:,:,:)
     real(kind=rb), intent(out) :: uflx(:,:), dflx(:,:), hr(:,:), uflxc(:,:), dflxc(:,:), hrc(:,:)
     real(kind=rb), intent(out), optional :: duflx_dt(:,:), duflxc_dt(:,:)
-    ! clear sky (rrtmg_lw_rad.nomcica.f90:433); out-of-range icld becomes 2
+    ! out-of-range icld becomes 2 (rrtmg_lw_rad.nomcica.f90:437); icld = 0 is
+    ! clear sky, icld = 1..3 with clouds (inflglw = 0: in-cloud optical depths)
     if (icld < 0 .or. icld > 3) icld = 2
-    call check(c_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
-                    h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
-                    cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &
-                    uflx, dflx, hr, uflxc, dflxc, hrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_lw')
+    if (icld == 0) then
+      call check(c_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
+                      h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                      cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &
+                      uflx, dflx, hr, uflxc, dflxc, hrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_lw')
+    else
+      call check(c_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
+                            h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                            cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, &
+                            inflglw, iceflglw, liqflglw, cldfr, taucld, cicewp, cliqwp, &
+                            reice, reliq, tauaer, &
+                            uflx, dflx, hr, uflxc, dflxc, hrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_lw')
+    end if
     if (present(duflx_dt)) duflx_dt = 0.0_rb
     if (present(duflxc_dt)) duflxc_dt = 0.0_rb
   end subroutine rrtmg_lw

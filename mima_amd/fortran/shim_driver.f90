@@ -7,12 +7,15 @@
 ! usage: shim_driver <input.bin> <output.bin>
 !   input  (stream): int32 ncol, nlay; then float64 arrays, top-first
 !          (ncol, nlay) or (ncol, nlay+1): play, plev, tlay, tlev; h2o, o3, co2;
-!          (ncol): tsfc, albedo, coszen; scalars solrad(adjes), solr_cnst(scon)
+!          (ncol): tsfc, albedo, coszen; scalars solrad(adjes), solr_cnst(scon);
+!          then the cloud fields of a cloudy LW call, top-first: cldfr
+!          (ncol, nlay), taucld (nbndlw, ncol, nlay)
 !   output (stream): swuflx, swdflx, swhr, uflx, dflx, hr, surface-first; then
 !          the adjoint entry points' gradients for cotangents hr_bar = swhr_bar =
 !          1 and uflx_bar = dflx_bar = swuflx_bar = swdflx_bar = 0:
 !          LW tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar; SW tlay_bar,
-!          h2ovmr_bar, o3vmr_bar
+!          h2ovmr_bar, o3vmr_bar; then the cloudy LW call's (icld = 2,
+!          inflglw = 0) uflx, dflx, hr, uflxc, dflxc, hrc
 program shim_driver
   use parkind, only : im => kind_im, rb => kind_rb
   use parrrtm, only : nbndlw
@@ -33,6 +36,7 @@ program shim_driver
   real(kind=rb), allocatable :: uflx(:,:), dflx(:,:), hr(:,:), uflxc(:,:), dflxc(:,:), hrc(:,:)
   real(kind=rb), allocatable :: zl(:,:), gtl(:,:), gtv(:,:), gq(:,:), go(:,:), gts(:), sgt(:,:), sgq(:,:), sgo(:,:)
   real(kind=rb), allocatable :: sf(:,:), sh(:,:), sq(:,:), so(:,:), tcopy(:)
+  real(kind=rb), allocatable :: cfr(:,:), tcl(:,:,:)
   real(kind=rb) :: solrad, solr_cnst
   real(kind=rb), parameter :: cp_air = 287.04_rb / (2._rb / 7._rb)
 
@@ -44,6 +48,8 @@ program shim_driver
   allocate(pf(ncol, nlay), ph(ncol, nlay + 1), tf(ncol, nlay), th(ncol, nlay + 1))
   allocate(q(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay), tsrf(ncol), albedo(ncol), cosz(ncol))
   read(u) pf, ph, tf, th, q, o3, co2, tsrf, albedo, cosz, solrad, solr_cnst
+  allocate(cfr(ncol, nlay), tcl(nbndlw, ncol, nlay))
+  read(u) cfr, tcl
   close(u)
 
   allocate(zeros(ncol, nlay), ones(ncol, nlay), emis(ncol, nbndlw))
@@ -109,5 +115,18 @@ program shim_driver
   open(newunit=u, file=trim(fout), access='stream', form='unformatted', status='replace')
   write(u) swuflx, swdflx, swhr, uflx, dflx, hr
   write(u) gtl, gtv, gq, go, gts, sgt, sgq, sgo
+
+  ! LW with clouds: maximum-random overlap, in-cloud optical depths
+  icld = 2
+  call rrtmg_lw(ncol, nlay, icld, idrv, &
+                pf(:, sk:1:-1), ph(:, sk+1:1:-1), tf(:, sk:1:-1), th(:, sk+1:1:-1), tsrf, &
+                q(:, sk:1:-1), o3(:, sk:1:-1), co2(:, sk:1:-1), &
+                zeros, zeros, zeros, &
+                zeros, zeros, zeros, zeros, &
+                emis, 0, 0, 0, &
+                cfr(:, sk:1:-1), tcl(:, :, sk:1:-1), zeros, zeros, 10*ones, 10*ones, &
+                tauaer, &
+                uflx, dflx, hr, uflxc, dflxc, hrc)
+  write(u) uflx, dflx, hr, uflxc, dflxc, hrc
   close(u)
 end program shim_driver

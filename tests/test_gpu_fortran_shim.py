@@ -1,13 +1,14 @@
 """The Fortran drop-in (mima_amd/fortran/rrtmg_hip_shim.f90), called the way
 MiMA's run_rrtmg calls RRTMG, gives bit-identical results to the direct C ABI
-path; so do its adjoint entry points (rrtmg_lw_adj, rrtmg_sw_adj)."""
+path; so do its adjoint entry points (rrtmg_lw_adj, rrtmg_sw_adj) and rrtmg_lw
+with clouds (icld = 2, inflglw = 0)."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from mima_amd.columns import make_columns
+from mima_amd.columns import make_clouds, make_columns
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -22,6 +23,7 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     rrtmg.set_precision("fp64")
     c = make_columns(32, 8, 40, seed=21)
     ncol, nlay = c["ncol"], c["nlay"]
+    cf, tau = make_clouds(c, seed=22)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     with open(fin, "wb") as f:
         np.array([ncol, nlay], dtype=np.int32).tofile(f)
@@ -31,11 +33,14 @@ def test_fortran_shim_matches_c_abi(tmp_path):
         for k in ("tsfc", "aldir", "coszen"):
             c[k].astype(np.float64).tofile(f)
         np.array([c["adjes"], c["scon"]], dtype=np.float64).tofile(f)
+        # Fortran cldfr(ncol, nlay) and taucld(nbndlw, ncol, nlay), top-first
+        np.ascontiguousarray(cf[::-1]).tofile(f)
+        np.ascontiguousarray(tau[::-1]).tofile(f)
     r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, dtype=np.float64)
     n2, n3 = nlay * ncol, (nlay + 1) * ncol
-    sizes = [n3, n3, n2] * 2 + [n2, n3, n2, n2, ncol, n2, n2, n2]
+    sizes = [n3, n3, n2] * 2 + [n2, n3, n2, n2, ncol, n2, n2, n2] + [n3, n3, n2] * 2
     parts, o = [], 0
     for s in sizes:
         parts.append(raw[o:o + s])
@@ -54,3 +59,7 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     for got, key, ref in zip(parts[6:], names, refs):
         np.testing.assert_array_equal(got.reshape(ref.shape), ref, err_msg=key)
     assert np.any(refs[0] != 0) and np.any(refs[5] != 0)
+    lc = rrtmg.lw_cloud_columns(c, 2, cf, tau)
+    for got, key in zip(parts[14:], ("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc")):
+        np.testing.assert_array_equal(got.reshape(lc[key].shape), lc[key], err_msg="cloudy " + key)
+    assert np.max(np.abs(lc["dflx"] - lc["dflxc"])) > 1.0

@@ -202,3 +202,35 @@ def test_error_contract(rr):
     b = rr.rrtmg_lw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 0, 0, 0, cf, tau)
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+def test_lds_kernel_matches_one_wave_kernel(rr, tmp_path):
+    """FP32 runs lw_cloud_band_lds_kernel (k-tables in LDS, persistent
+    workgroups); RRTMG_HIP_LW_LDS=0 at init selects lw_cloud_band_kernel (one
+    wave per unit).  Same body, same order: bitwise equal.  The other kernel
+    runs in a child process (the choice is made at init)."""
+    import subprocess
+    import sys
+    c, cf, tau = _sample(600, 40, 21)
+    np.savez(tmp_path / "cols.npz", cf=cf, tau=tau, **{k: np.asarray(v) for k, v in c.items()})
+    here = {icld: _cloud(rr, c, "mixed", icld, cf, tau) for icld in (1, 2)}
+    child = (
+        "import sys, numpy as np\n"
+        "sys.path.insert(0, %r)\n"
+        "from mima_amd import rrtmg\n"
+        "z = np.load(%r)\n"
+        "c = {k: (z[k] if z[k].ndim else z[k].item()) for k in z.files if k not in ('cf', 'tau')}\n"
+        "rrtmg.rrtmg_lw_ini(); rrtmg.set_precision('mixed')\n"
+        "out = {}\n"
+        "for icld in (1, 2):\n"
+        "    r = rrtmg.lw_cloud_columns(c, icld, z['cf'], z['tau'])\n"
+        "    out.update({'%%d_%%s' %% (icld, k): v for k, v in r.items()})\n"
+        "np.savez(%r, **out)\n"
+    ) % (ROOT, str(tmp_path / "cols.npz"), str(tmp_path / "out.npz"))
+    env = dict(os.environ, RRTMG_HIP_LW_LDS="0")
+    r = subprocess.run([sys.executable, "-c", child], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    other = np.load(tmp_path / "out.npz")
+    for icld in (1, 2):
+        for k in OUT:
+            assert np.array_equal(here[icld][k], other["%d_%s" % (icld, k)]), (icld, k)
