@@ -239,3 +239,53 @@ def make_clouds(cols, seed=0, blocks=(0, 3), edge_cases=True):
             cf[:, i] = 0.0
             tau[:, i] = 0.0
     return cf, tau
+
+
+def make_sw_clouds(cols, seed=0, blocks=(0, 3), edge_cases=True):
+    """Seeded cloud profiles for the SW cloudy path (``icld >= 1``,
+    ``inflgsw = 0``), which without McICA takes overcast or clear layers only.
+
+    Returns ``(cldfr, taucld, ssacld, asmcld, fsfcld)``: fractions ``(nlay,
+    ncol)`` of 0 or 1 and band optics ``(nlay, ncol, 14)`` (the reference's
+    ``(nbndsw, ncol, nlay)`` in memory order).  Each column gets ``U{blocks}``
+    blocks of 1-6 overcast layers; a layer's optical depth is a log-uniform
+    base in [1e-2, 50] times a per-column, per-band factor in [0.5, 1.5],
+    single-scattering albedo 1 - 10^U(-6.5, -1) (near-conservative water
+    clouds to absorbing ones), asymmetry in [0.7, 0.9] and forward fraction
+    asymmetry^2 (delta-M).  With ``edge_cases``, column ``i`` gets, by
+    ``i mod 6``: 1 a fraction of 1e-7 (at most 1e-6: allowed, blended); 2 a
+    fraction of 1 - 1e-7; 3 conservative scattering (albedo 1); 4 a cloud in
+    the top layer; 5 a cloud of optical depth 1e-3 in the surface layer.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, L = cols["ncol"], cols["nlay"]
+    cf = np.zeros((L, n))
+    tau = np.zeros((L, n, 14))
+    ssa = np.ones((L, n, 14))
+    asm = np.zeros((L, n, 14))
+    for i in range(n):
+        spec = rng.uniform(0.5, 1.5, 14)
+        nb = int(rng.integers(blocks[0], blocks[1] + 1))
+        for _ in range(nb):
+            th = int(rng.integers(1, 7))
+            k0 = int(rng.integers(0, max(1, L - th)))
+            for k in range(k0, min(k0 + th, L)):
+                cf[k, i] = 1.0
+                tau[k, i] = np.exp(rng.uniform(np.log(1e-2), np.log(50.0))) * spec
+                ssa[k, i] = 1.0 - 10.0 ** rng.uniform(-6.5, -1.0, 14)
+                asm[k, i] = rng.uniform(0.7, 0.9, 14)
+        if not edge_cases:
+            continue
+        case = i % 6
+        k = int(rng.integers(0, L))
+        if case == 1:
+            cf[k, i], tau[k, i], asm[k, i] = 1e-7, 5.0 * spec, 0.85
+        elif case == 2:
+            cf[k, i], tau[k, i], asm[k, i] = 1.0 - 1e-7, 5.0 * spec, 0.85
+        elif case == 3:
+            cf[k, i], tau[k, i], ssa[k, i], asm[k, i] = 1.0, 8.0 * spec, 1.0, 0.8
+        elif case == 4:
+            cf[L - 1, i], tau[L - 1, i], asm[L - 1, i] = 1.0, 0.5 * spec, 0.8
+        elif case == 5:
+            cf[0, i], tau[0, i], asm[0, i] = 1.0, 1e-3 * spec, 0.8
+    return cf, tau, ssa, asm, asm * asm

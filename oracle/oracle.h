@@ -99,6 +99,21 @@ int oracle_lw_inatm(int nlay, const double *play, const double *plev,
 void oracle_sw_reftra(int nlay, const double *gg, double prmuz, const double *tau,
                       const double *w, double *pref, double *prefd,
                       double *ptra, double *ptrad);
+/* rrtmg_sw with clouds (rrtmg_sw_rad.nomcica.f90:78-731): cldprop_sw with
+ * inflgsw = 0 (delta-M scaled input optics), spcvrt_sw's clear and total
+ * streams; iaer = 0.  cldfr, cicewp, cliqwp (ncol, nlay) (NULL water paths =
+ * 0); taucld, ssacld, asmcld, fsfcld (14, ncol, nlay), band fastest.  0 on
+ * success, -5 on partial cloud in a day column (the reference stops). */
+int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *asdir, const double *asdif,
+                          const double *aldir, const double *aldif, const double *coszen, double adjes,
+                          int dyofyr, double scon, const double *cldfr, const double *taucld,
+                          const double *ssacld, const double *asmcld, const double *fsfcld,
+                          const double *cicewp, const double *cliqwp, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc);
+
 /* rrtmg_lw with clouds (rrtmg_lw_rad.nomcica.f90:80-569): cldprop with
  * inflglw = 0, then rtrn (icld = 1) or rtrnmr (icld = 2, 3); idrv = 0.
  * cldfr, cicewp, cliqwp (ncol, nlay) (NULL water paths = 0); taucld

@@ -1049,6 +1049,140 @@ static void spcvrt_sw(swcol_t *c, double prmu0, const double *palbd, const doubl
     }
 }
 
+/* cldprop_sw with inflag = 0 (rrtmg_sw_cldprop.f90:118-157): a layer that
+ * passes the cloud test (fraction >= 1e-20 and water path or summed depth
+ * >= 1e-20) gets delta-M scaled optics from the input optical depth, single
+ * scattering albedo, asymmetry and forward-scattering fraction; every other
+ * layer has depth 0, albedo 1, asymmetry 0.  Arrays [lay][ib], ib = 1..14. */
+static void cldprop_sw0(int nl, const double *cldfrac, double (*tauc)[15], double (*ssac)[15],
+                        double (*asmc)[15], double (*fsfc)[15], const double *cwp, double (*taucloud)[15],
+                        double (*ssacloud)[15], double (*asmcloud)[15])
+{
+    const double cldmin = 1.e-20;
+    for (int lay = 1; lay <= nl; ++lay) {
+        double tauctot = 0.0;
+        for (int ib = 1; ib <= 14; ++ib) {
+            taucloud[lay][ib] = 0.0;
+            ssacloud[lay][ib] = 1.0;
+            asmcloud[lay][ib] = 0.0;
+            tauctot = tauctot + tauc[lay][ib];
+        }
+        if (cldfrac[lay] >= cldmin && (cwp[lay] >= cldmin || tauctot >= cldmin)) {
+            for (int ib = 1; ib <= 14; ++ib) {
+                double ffp = fsfc[lay][ib];
+                double ffp1 = 1.0 - ffp;
+                double ffpssa = 1.0 - ffp * ssac[lay][ib];
+                ssacloud[lay][ib] = ffp1 * ssac[lay][ib] / ffpssa;
+                taucloud[lay][ib] = ffpssa * tauc[lay][ib];
+                asmcloud[lay][ib] = (asmc[lay][ib] - ffp) / ffp1;
+            }
+        }
+    }
+}
+
+/* spcvrt_sw with clouds (rrtmg_sw_spcvrt.f90:302-600; icpr = 1, idelm = 0,
+ * no aerosol): the clear optics of each layer as in the clear-sky path, the
+ * cloudy optics (clear plus delta-scaled cloud, :459-466) through reftra
+ * where the fraction exceeds 1e-12, the total-sky layer values blended by
+ * the fraction (:502-549), and vrtqdr for the clear and the total stream. */
+static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const double *palbp,
+                            const double *adjflux, const double *pclfr, double (*ptauc)[15],
+                            double (*pasyc)[15], double (*pomgc)[15], double *pbbfd, double *pbbfu,
+                            double *pbbcd, double *pbbcu)
+{
+    const int klev = c->nlayers;
+    const double od_lo = 0.06, repclc = 1.e-12;
+    static __thread double zrefc[MAXLAY + 3], zrefdc[MAXLAY + 3], ztrac[MAXLAY + 3], ztradc[MAXLAY + 3];
+    static __thread double zrefo[MAXLAY + 3], zrefdo[MAXLAY + 3], ztrao[MAXLAY + 3], ztrado[MAXLAY + 3];
+    static __thread double zref[MAXLAY + 3], zrefd[MAXLAY + 3], ztra[MAXLAY + 3], ztrad[MAXLAY + 3];
+    static __thread double zdbtc[MAXLAY + 3], ztdbtc[MAXLAY + 3], zdbt[MAXLAY + 3], ztdbt[MAXLAY + 3];
+    static __thread double zrdndc[MAXLAY + 3], zrupc[MAXLAY + 3], zrupdc[MAXLAY + 3];
+    static __thread double zrdnd[MAXLAY + 3], zrup[MAXLAY + 3], zrupd[MAXLAY + 3];
+    static __thread double ztauc[MAXLAY + 3], zomcc[MAXLAY + 3], zgcc[MAXLAY + 3];
+    static __thread double ztauo[MAXLAY + 3], zomco[MAXLAY + 3], zgco[MAXLAY + 3];
+    static __thread double zcd[MAXLAY + 3], zcu[MAXLAY + 3], zfd[MAXLAY + 3], zfu[MAXLAY + 3];
+    int lrtchkclr[MAXLAY + 3], lrtchkcld[MAXLAY + 3];
+    for (int jk = 1; jk <= klev + 1; ++jk) pbbcd[jk] = pbbcu[jk] = pbbfd[jk] = pbbfu[jk] = 0.0;
+    taumol_sw(c);
+    int iw = 0;
+    for (int jb = 16; jb <= 29; ++jb) {
+        int ibm = jb - 15;
+        int igt = S.ngc[ibm - 1];
+        for (int jg = 1; jg <= igt; ++jg) {
+            iw = iw + 1;
+            double zincflx = adjflux[jb] * c->sfluxzen[iw] * prmu0;
+            ztdbtc[1] = 1.0;
+            zdbtc[klev + 1] = 0.0;
+            ztrac[klev + 1] = 0.0;
+            ztradc[klev + 1] = 0.0;
+            zrefc[klev + 1] = palbp[ibm];
+            zrefdc[klev + 1] = palbd[ibm];
+            zrupc[klev + 1] = palbp[ibm];
+            zrupdc[klev + 1] = palbd[ibm];
+            ztrao[klev + 1] = 0.0;
+            ztrado[klev + 1] = 0.0;
+            zrefo[klev + 1] = palbp[ibm];
+            zrefdo[klev + 1] = palbd[ibm];
+            ztdbt[1] = 1.0;
+            zdbt[klev + 1] = 0.0;
+            ztra[klev + 1] = 0.0;
+            ztrad[klev + 1] = 0.0;
+            zref[klev + 1] = palbp[ibm];
+            zrefd[klev + 1] = palbd[ibm];
+            zrup[klev + 1] = palbp[ibm];
+            zrupd[klev + 1] = palbd[ibm];
+            for (int jk = 1; jk <= klev; ++jk) {
+                int ikl = klev + 1 - jk;
+                lrtchkclr[jk] = 1;
+                lrtchkcld[jk] = pclfr[ikl] > repclc;
+                /* ptaua = 0, pomga = 1, pasya = 0 (iaer = 0) */
+                ztauc[jk] = c->taur[ikl][iw] + c->taug[ikl][iw] + 0.0;
+                zomcc[jk] = c->taur[ikl][iw] * 1.0 + 0.0 * 1.0;
+                zgcc[jk] = 0.0 * 1.0 * 0.0 / zomcc[jk];
+                zomcc[jk] = zomcc[jk] / ztauc[jk];
+                double zf = zgcc[jk] * zgcc[jk];
+                double zwf = zomcc[jk] * zf;
+                ztauc[jk] = (1.0 - zwf) * ztauc[jk];
+                zomcc[jk] = (zomcc[jk] - zwf) / (1.0 - zwf);
+                zgcc[jk] = (zgcc[jk] - zf) / (1.0 - zf);
+                ztauo[jk] = ztauc[jk] + ptauc[ikl][ibm];
+                zomco[jk] = ztauc[jk] * zomcc[jk] + ptauc[ikl][ibm] * pomgc[ikl][ibm];
+                zgco[jk] = (ptauc[ikl][ibm] * pomgc[ikl][ibm] * pasyc[ikl][ibm] + ztauc[jk] * zomcc[jk] * zgcc[jk]) /
+                           zomco[jk];
+                zomco[jk] = zomco[jk] / ztauo[jk];
+            }
+            reftra_sw(klev, lrtchkclr, zgcc, prmu0, ztauc, zomcc, zrefc, zrefdc, ztrac, ztradc);
+            reftra_sw(klev, lrtchkcld, zgco, prmu0, ztauo, zomco, zrefo, zrefdo, ztrao, ztrado);
+            for (int jk = 1; jk <= klev; ++jk) {
+                int ikl = klev + 1 - jk;
+                double zclear = 1.0 - pclfr[ikl];
+                double zcloud = pclfr[ikl];
+                zref[jk] = zclear * zrefc[jk] + zcloud * zrefo[jk];
+                zrefd[jk] = zclear * zrefdc[jk] + zcloud * zrefdo[jk];
+                ztra[jk] = zclear * ztrac[jk] + zcloud * ztrao[jk];
+                ztrad[jk] = zclear * ztradc[jk] + zcloud * ztrado[jk];
+                double ze1 = ztauc[jk] / prmu0;
+                double zdbtmc = ze1 <= od_lo ? 1.0 - ze1 + 0.5 * ze1 * ze1 : tbl_exp(ze1);
+                zdbtc[jk] = zdbtmc;
+                ztdbtc[jk + 1] = zdbtc[jk] * ztdbtc[jk];
+                ze1 = ztauo[jk] / prmu0;
+                double zdbtmo = ze1 <= od_lo ? 1.0 - ze1 + 0.5 * ze1 * ze1 : tbl_exp(ze1);
+                zdbt[jk] = zclear * zdbtmc + zcloud * zdbtmo;
+                ztdbt[jk + 1] = zdbt[jk] * ztdbt[jk];
+            }
+            vrtqdr_sw(klev, zrefc, zrefdc, ztrac, ztradc, zdbtc, zrdndc, zrupc, zrupdc, ztdbtc, zcd, zcu);
+            vrtqdr_sw(klev, zref, zrefd, ztra, ztrad, zdbt, zrdnd, zrup, zrupd, ztdbt, zfd, zfu);
+            for (int jk = 1; jk <= klev + 1; ++jk) {
+                int ikl = klev + 2 - jk;
+                pbbfu[ikl] = pbbfu[ikl] + zincflx * zfu[jk];
+                pbbfd[ikl] = pbbfd[ikl] + zincflx * zfd[jk];
+                pbbcu[ikl] = pbbcu[ikl] + zincflx * zcu[jk];
+                pbbcd[ikl] = pbbcd[ikl] + zincflx * zcd[jk];
+            }
+        }
+    }
+}
+
 static void sw_column(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
                       const double *plev, const double *tlay, const double *tlev,
                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
@@ -1142,6 +1276,112 @@ int oracle_rrtmg_sw(int ncol, int nlay, const double *play, const double *plev, 
 #undef O2
         }
         free(c);
+    }
+    return err;
+}
+
+/* rrtmg_sw with clouds (rrtmg_sw_rad.nomcica.f90:78-731): icld 1..3 (out of
+ * range -> 2; the nomcica path ignores the overlap choice), inflgsw = 0,
+ * iaer = 0.  cldfr, cicewp, cliqwp (ncol, nlay) (NULL water paths = 0);
+ * taucld, ssacld, asmcld, fsfcld (14, ncol, nlay), band fastest.  Returns -5
+ * when a day column has a fraction in (1e-6, 1 - 1e-6) (the reference
+ * stops: partial cloud is not allowed, :535-539). */
+int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *asdir, const double *asdif,
+                          const double *aldir, const double *aldif, const double *coszen, double adjes,
+                          int dyofyr, double scon, const double *cldfr, const double *taucld,
+                          const double *ssacld, const double *asmcld, const double *fsfcld,
+                          const double *cicewp, const double *cliqwp, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
+{
+    if (!S.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    if (icld < 0 || icld > 3) icld = 2;
+    const int clouds = icld != 0;
+    const double zepzen = 1.e-10, zepsec = 1.e-6;
+    double adjflx = adjes;
+    if (dyofyr > 0) adjflx = earth_sun(dyofyr);
+    double adjflux[30];
+    for (int ib = 16; ib <= 29; ++ib) adjflux[ib] = adjflx * (scon / 1.36822e+03);
+    int err = 0;
+#pragma omp parallel
+    {
+        swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
+        double (*a)[MAXLAY + 2][15] = malloc(sizeof(double[7][MAXLAY + 2][15]));
+        double cf[MAXLAY + 3], cwp[MAXLAY + 3];
+        double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], zbbfu[MAXLAY + 3], zbbfd[MAXLAY + 3];
+        double swnflxc[MAXLAY + 3], swnflx[MAXLAY + 3];
+        if (!c || !a) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (err == -3) continue;
+#define O2(x, i, k) x[(i) + (long)ncol * ((k)-1)]
+            if (coszen[iplon] < zepzen) {
+                for (int k = 1; k <= nlay + 1; ++k)
+                    O2(swuflx, iplon, k) = O2(swdflx, iplon, k) = O2(swuflxc, iplon, k) = O2(swdflxc, iplon, k) = 0.0;
+                for (int k = 1; k <= nlay; ++k) O2(swhr, iplon, k) = O2(swhrc, iplon, k) = 0.0;
+                continue;
+            }
+            double (*tauc)[15] = a[0], (*ssac)[15] = a[1], (*asmc)[15] = a[2], (*fsfc)[15] = a[3];
+            double (*ptauc)[15] = a[4], (*pomgc)[15] = a[5], (*pasyc)[15] = a[6];
+            int partial = 0;
+            for (int l = 1; l <= nlay; ++l) {
+                long ix = iplon + (long)ncol * (l - 1);
+                cf[l] = clouds ? cldfr[ix] : 0.0;
+                cwp[l] = clouds ? (cicewp ? cicewp[ix] : 0.0) + (cliqwp ? cliqwp[ix] : 0.0) : 0.0;
+                for (int ib = 1; ib <= 14; ++ib) {
+                    long jx = (ib - 1) + 14 * ix;
+                    tauc[l][ib] = clouds ? taucld[jx] : 0.0;
+                    ssac[l][ib] = clouds ? ssacld[jx] : 0.0;
+                    asmc[l][ib] = clouds ? asmcld[jx] : 0.0;
+                    fsfc[l][ib] = clouds ? fsfcld[jx] : 0.0;
+                }
+                if (cf[l] > zepsec && cf[l] < 1.0 - zepsec) partial = 1;
+            }
+            if (partial) {
+#pragma omp atomic write
+                err = -5;
+                continue;
+            }
+            cldprop_sw0(nlay, cf, tauc, ssac, asmc, fsfc, cwp, ptauc, pomgc, pasyc);
+            sw_column(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
+            double cossza = coszen[iplon];
+            double albdir[15], albdif[15];
+            for (int ib = 1; ib <= 9; ++ib) {
+                albdir[ib] = aldir[iplon];
+                albdif[ib] = aldif[iplon];
+            }
+            albdir[14] = aldir[iplon];
+            albdif[14] = aldif[iplon];
+            for (int ib = 10; ib <= 13; ++ib) {
+                albdir[ib] = asdir[iplon];
+                albdif[ib] = asdif[iplon];
+            }
+            spcvrt_sw_cloud(c, cossza, albdif, albdir, adjflux, cf, ptauc, pasyc, pomgc, zbbfd, zbbfu, zbbcd, zbbcu);
+            for (int i = 1; i <= nlay + 1; ++i) {
+                O2(swuflxc, iplon, i) = zbbcu[i];
+                O2(swdflxc, iplon, i) = zbbcd[i];
+                O2(swuflx, iplon, i) = zbbfu[i];
+                O2(swdflx, iplon, i) = zbbfd[i];
+                swnflxc[i] = zbbcd[i] - zbbcu[i];
+                swnflx[i] = zbbfd[i] - zbbfu[i];
+            }
+            for (int i = 1; i <= nlay; ++i) {
+                double zdpgcp = S.heatfac / c->pdp[i];
+                O2(swhrc, iplon, i) = (swnflxc[i + 1] - swnflxc[i]) * zdpgcp;
+                O2(swhr, iplon, i) = (swnflx[i + 1] - swnflx[i]) * zdpgcp;
+            }
+            O2(swhrc, iplon, nlay) = 0.0;
+            O2(swhr, iplon, nlay) = 0.0;
+#undef O2
+        }
+        free(c);
+        free(a);
     }
     return err;
 }

@@ -82,6 +82,26 @@ def rrtmg_lw(c):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+def rrtmg_sw_cloud(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None):
+    """rrtmg_sw with clouds (icld 0..3, inflgsw = 0).  cldfr (nlay, ncol),
+    cloud optics (nlay, ncol, 14).  Raises on partial cloud in a day column
+    (the reference stops there)."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr",
+                                "n2ovmr", "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")]
+    cl = [_f64(a) for a in (cldfr, taucld, ssacld, asmcld, fsfcld)]
+    opt = [None if a is None else _f64(a) for a in (cicewp, cliqwp)]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_sw_cloud(ncol, nlay, int(icld), *[_p(a) for a in ins], ctypes.c_double(c["adjes"]),
+                                 int(c["dyofyr"]), ctypes.c_double(c["scon"]), *[_p(a) for a in cl],
+                                 *[None if a is None else _p(a) for a in opt], *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_sw_cloud returned %d" % rc)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
 def rrtmg_lw_cloud(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None):
     """rrtmg_lw with clouds (icld 0..3, inflglw = 0).  cldfr (nlay, ncol),
     taucld (nlay, ncol, 16), tauaer (16, nlay, ncol) or None."""

@@ -9,6 +9,7 @@
 !                                     (rrtmg_sw_rad.nomcica.f90:78), with the
 !                                     cloud/aerosol arguments MiMA passes
 !                                     (RR/rrtm_radiation.f90:701-713)
+!   ref_sw_run_cloud(...)          -> rrtmg_sw with clouds (icld >= 1, inflgsw = 0)
 !   ref_sw_column(...)             -> inatm_sw + setcoef_sw + taumol_sw for one
 !                                     column: jp/jt/jt1, laytrop, taug, taur,
 !                                     sfluxzen
@@ -61,6 +62,44 @@ contains
                   aerb, aerb, aerb, ecaer, &
                   swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
   end subroutine ref_sw_run
+
+  ! rrtmg_sw with clouds: icld (the nomcica path takes any icld >= 1 the
+  ! same way), inflgsw = 0 (cldprop_sw delta-scales the input optics), the
+  ! caller's fractions (overcast or clear only: the reference stops on
+  ! partial cloud) and cloud optics taucld, ssacld, asmcld, fsfcld
+  ! (nbndsw, ncol, nlay); effective radii 10 um as MiMA passes them
+  subroutine ref_sw_run_cloud(ncol, nlay, icld_in, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, &
+                              asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                              cldfr, taucld, ssacld, asmcld, fsfcld, cicewp, cliqwp, &
+                              swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc) bind(C, name="ref_sw_run_cloud")
+    integer(c_int), value :: ncol, nlay, dyofyr, icld_in
+    real(c_double), value :: adjes, scon
+    real(c_double), intent(in) :: play(ncol, nlay), plev(ncol, nlay + 1), tlay(ncol, nlay)
+    real(c_double), intent(in) :: tlev(ncol, nlay + 1), tsfc(ncol)
+    real(c_double), intent(in) :: h2o(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay)
+    real(c_double), intent(in) :: ch4(ncol, nlay), n2o(ncol, nlay), o2(ncol, nlay)
+    real(c_double), intent(in) :: asdir(ncol), asdif(ncol), aldir(ncol), aldif(ncol), coszen(ncol)
+    real(c_double), intent(in) :: cldfr(ncol, nlay), cicewp(ncol, nlay), cliqwp(ncol, nlay)
+    real(c_double), intent(in) :: taucld(nbndsw, ncol, nlay), ssacld(nbndsw, ncol, nlay)
+    real(c_double), intent(in) :: asmcld(nbndsw, ncol, nlay), fsfcld(nbndsw, ncol, nlay)
+    real(c_double), intent(out) :: swuflx(ncol, nlay + 1), swdflx(ncol, nlay + 1), swhr(ncol, nlay)
+    real(c_double), intent(out) :: swuflxc(ncol, nlay + 1), swdflxc(ncol, nlay + 1), swhrc(ncol, nlay)
+    integer(kind=im) :: icld, iaer, dy
+    real(kind=rb), allocatable :: tencl(:, :), aerb(:, :, :), ecaer(:, :, :)
+    allocate(tencl(ncol, nlay), aerb(ncol, nlay, nbndsw), ecaer(ncol, nlay, naerec))
+    tencl = 10.0_rb
+    aerb = 0.0_rb
+    ecaer = 0.0_rb
+    icld = icld_in
+    iaer = 0
+    dy = dyofyr
+    call rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                  h2o, o3, co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, &
+                  coszen, adjes, dy, scon, 0, 0, 0, cldfr, &
+                  taucld, ssacld, asmcld, fsfcld, cicewp, cliqwp, tencl, tencl, &
+                  aerb, aerb, aerb, ecaer, &
+                  swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  end subroutine ref_sw_run_cloud
 
   subroutine ref_sw_column(nlay, play, plev, tlay, tlev, h2o, o3, co2, ch4, n2o, o2, &
                            laytrop, jp, jt, jt1, fac00, fac01, fac10, fac11, colmol, &

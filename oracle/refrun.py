@@ -119,3 +119,22 @@ def run_lw_cloud(lib, c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=N
     lib.ref_lw_run_cloud(n, L, int(icld), *[_p(a) for a in ins], *[_p(a) for a in cl], _p(tau),
                          *[_p(a) for a in outs])
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
+def run_sw_cloud(lib, c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None):
+    """rrtmg_sw of the reference with clouds (icld >= 1, inflgsw = 0).
+
+    cldfr (nlay, ncol) and the cloud optics (nlay, ncol, 14) in this
+    package's C order, i.e. the reference's (ncol, nlay) and (nbndsw, ncol,
+    nlay).  The fractions must be 0 or 1 to 1e-6 in day columns (the
+    reference stops otherwise)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
+    z = np.zeros((L, n))
+    cl = [np.ascontiguousarray(a, dtype=np.float64) for a in
+          (cldfr, taucld, ssacld, asmcld, fsfcld, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp)]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_sw_run_cloud(n, L, int(icld), *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
+                         ctypes.c_double(c["scon"]), *[_p(a) for a in cl], *[_p(a) for a in outs])
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
