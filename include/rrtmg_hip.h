@@ -22,9 +22,9 @@
  *     are then left untouched and rrtmg_hip_last_error() describes the cause.
  *     Contract: rrtmg_hip_sw and rrtmg_hip_lw are clear sky (MiMA passes
  *     icld = idrv = iaer = 0, RR/rrtm_radiation.f90:106-108); icld != 0,
- *     SW iaer != 0 or LW idrv != 0 return RRTMG_HIP_ENOTSUP there.  LW
- *     clouds with in-cloud optical depths (inflglw = 0) take
- *     rrtmg_hip_lw_cloud.
+ *     SW iaer != 0 or LW idrv != 0 return RRTMG_HIP_ENOTSUP there.  Clouds
+ *     given by their optical properties (inflglw = inflgsw = 0) take
+ *     rrtmg_hip_lw_cloud and rrtmg_hip_sw_cloud.
  */
 #ifndef RRTMG_HIP_H
 #define RRTMG_HIP_H
@@ -102,6 +102,43 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
                  double *uflx, double *dflx, double *hr,
                  double *uflxc, double *dflxc, double *hrc,
                  int ptr_kind, void *stream);
+
+/* Replaces rrtmg_sw with clouds (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.nomcica.f90:78-88):
+ * the reference's whole argument list in its order.  icld >= 1 (outside
+ * 0..3 taken as 2, :471; the path without McICA treats every icld >= 1
+ * alike); icld = 0 is rrtmg_hip_sw (cloud arrays unread).  inflgsw = 0
+ * only (cldprop_sw, rrtmg_sw_cldprop.f90:118-157): taucld, ssacld, asmcld,
+ * fsfcld (14, ncol, nlay), band fastest, are the cloud optical depth,
+ * single-scattering albedo, asymmetry and forward-scattering fraction,
+ * delta-M scaled by the library; a layer's are used where cldfr
+ * (ncol, nlay) >= 1e-20 and cicewp + cliqwp (NULL = 0) or the summed depth
+ * >= 1e-20.  Day columns (coszen >= 1e-10) take overcast or clear layers
+ * only, as the reference (it stops on a fraction in (1e-6, 1 - 1e-6),
+ * :535-539): EINVAL, checked for host pointers always and for device
+ * pointers when rrtmg_hip_set_checks is on (otherwise the layer's clear and
+ * cloudy values are blended by the fraction, the reference's formula).
+ * iceflgsw, liqflgsw, reice, reliq and the aerosol arrays are not read.
+ * Outputs: total-sky swuflx, swdflx, swhr and clear-sky swuflxc, swdflxc,
+ * swhrc.  inflgsw != 0, iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
+ * RRTMG_HIP_ENOTSUP. */
+int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc,
+                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                       const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *asdir, const double *asdif, const double *aldir,
+                       const double *aldif, const double *coszen,
+                       double adjes, int dyofyr, double scon,
+                       int inflgsw, int iceflgsw, int liqflgsw,
+                       const double *cldfr, const double *taucld, const double *ssacld,
+                       const double *asmcld, const double *fsfcld,
+                       const double *cicewp, const double *cliqwp,
+                       const double *reice, const double *reliq,
+                       const double *tauaer, const double *ssaaer, const double *asmaer,
+                       const double *ecaer,
+                       double *swuflx, double *swdflx, double *swhr,
+                       double *swuflxc, double *swdflxc, double *swhrc,
+                       int ptr_kind, void *stream);
 
 /* Replaces rrtmg_lw with clouds (rrtmg_lw/gcm_model/src/rrtmg_lw_rad.nomcica.f90:80-89):
  * the reference's whole argument list in its order.  icld = 1: random

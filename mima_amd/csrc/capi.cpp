@@ -470,7 +470,7 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
                const double *asdir, const double *asdif, const double *aldir, const double *aldif,
                const double *coszen, double adjes, int dyofyr, double scon, double *swuflx, double *swdflx,
                double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind, void *stream,
-               const FixedWorkspace *fx)
+               const FixedWorkspace *fx, const SwClouds *cl)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
@@ -480,8 +480,11 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         return fail(RRTMG_HIP_EDEVICE, "the no-allocation scratch array has no device address on this device");
     if (fx && ptr_kind != RRTMG_HIP_DEVICE)
         return fail(RRTMG_HIP_EINVAL, "the no-allocation entry point takes device pointers only");
-    if (icld != 0) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: only the clear-sky path is implemented");
+    if (icld != 0 && !cl) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: rrtmg_hip_sw is clear sky; clouds take rrtmg_hip_sw_cloud");
     if (iaer != 0) return fail(RRTMG_HIP_ENOTSUP, "iaer != 0: SW aerosol optics are not implemented");
+    if (cl && fx) return fail(RRTMG_HIP_EINVAL, "the no-allocation entry point is clear sky");
+    if (cl && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
+        return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
     if (rc) return rc;
     const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
@@ -522,7 +525,7 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     const size_t o_bf = off; off = align_up(off + rs * 14 * 2 * n3);
     const size_t o_ds = off;
     const int units = (ncol + TILE_COLS - 1) / TILE_COLS * s->sw.tab.nwaves;
-    const size_t unit_bytes = sw_fwd_state_bytes(nlay, rs);
+    const size_t unit_bytes = cl ? sw_cloud_row_bytes(nlay, rs) : sw_fwd_state_bytes(nlay, rs);
     int chunk;
     char *wb;
     int *flag = nullptr;
@@ -548,11 +551,23 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         a.push_back({"tlev", tlev, n3});
         const char *nm1[] = {"tsfc", "asdir", "asdif", "aldir", "aldif", "coszen"};
         for (int i = 0; i < 6; ++i) a.push_back({nm1[i], ins1[i], (size_t)ncol});
+        if (cl) {
+            a.push_back({"cldfr", cl->cldfr, n2});
+            const double *opt[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
+            const char *nmo[] = {"taucld", "ssacld", "asmcld", "fsfcld"};
+            for (int i = 0; i < 4; ++i) a.push_back({nmo[i], opt[i], 14 * n2});
+            if (cl->cicewp) a.push_back({"cicewp", cl->cicewp, n2});
+            if (cl->cliqwp) a.push_back({"cliqwp", cl->cliqwp, n2});
+        }
         if ((rc = flag ? device_checks(flag, st, a, plev, ncol, nlay) : device_checks(*W, st, a, plev, ncol, nlay)))
             return rc;
     }
     int ck = 0;
     off = align_up(o_ds + (size_t)chunk * unit_bytes);
+    // cloud path: flags, four-stream band fluxes, partial-cloud count
+    const size_t o_cfl = off; if (cl) off = align_up(off + sizeof(int) * n2);
+    const size_t o_cbf = off; if (cl) off = align_up(off + rs * 14 * 4 * n3);
+    const size_t o_cpc = off; if (cl) off = align_up(off + sizeof(int));
     if (W) {
         if ((rc = W->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
         wb = (char *)W->sw_work.p;
@@ -565,6 +580,16 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     w.bflux = wb + o_bf;
     w.dstate = wb + o_ds;
     w.chunk = chunk;
+    SwCloudWork cw;
+    SwClouds cld{};
+    if (cl) {
+        cw.flags = (int *)(wb + o_cfl);
+        cw.bflux = wb + o_cbf;
+        cw.rows = wb + o_ds;
+        cw.chunk = chunk;
+        cw.partial = (int *)(wb + o_cpc);
+        cld = *cl;
+    }
 
     ColumnInputs in{};
     in.ncol = ncol;
@@ -573,7 +598,8 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     SwOutputs out{};
     double *stage = nullptr;
     if (ptr_kind == RRTMG_HIP_HOST) {
-        const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + 4 * n3 + 2 * n2);
+        const size_t ncl = cl ? 57 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) : 0;
+        const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + ncl + 4 * n3 + 2 * n2);
         if ((rc = W->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
         stage = (double *)W->sw_stage.p;
         double *p = stage;
@@ -594,6 +620,18 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
             ck |= hipMemcpyAsync(p, ins1[i], ncol * sizeof(double), hipMemcpyHostToDevice, st);
             *dst1[i] = p;
             p += ncol;
+        }
+        if (cl) {
+            const double *src[] = {cl->cldfr, cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld, cl->cicewp, cl->cliqwp};
+            const double **dst[] = {&cld.cldfr, &cld.taucld, &cld.ssacld, &cld.asmcld, &cld.fsfcld, &cld.cicewp,
+                                    &cld.cliqwp};
+            const size_t len[] = {n2, 14 * n2, 14 * n2, 14 * n2, 14 * n2, n2, n2};
+            for (int i = 0; i < 7; ++i) {
+                if (!src[i]) continue;
+                ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
+                *dst[i] = p;
+                p += len[i];
+            }
         }
         out.uflx = p; p += n3;
         out.dflx = p; p += n3;
@@ -616,9 +654,26 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
-    hipError_t e = launch_sw(in, out, w, t, g_prec, st, s->events(0));
+    hipError_t e;
+    if (cl) {
+        e = launch_sw_cloud_flags(in, cld, cw, st);
+        if (e == hipSuccess && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
+            // the nomcica driver stops on partial cloud in a day column
+            // (rrtmg_sw_rad.nomcica.f90:535-539); the host-pointer path
+            // checked before staging
+            int np = 0;
+            e = hipMemcpyAsync(&np, cw.partial, sizeof(int), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e == hipSuccess && np > 0)
+                return fail(RRTMG_HIP_EINVAL, "partial cloud (fraction in (1e-6, 1 - 1e-6)) in a day column: the "
+                                              "SW path without McICA takes overcast or clear layers only");
+        }
+        if (e == hipSuccess) e = launch_sw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(0));
+    } else {
+        e = launch_sw(in, out, w, t, g_prec, st, s->events(0));
+    }
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("SW kernel launch: ") + hipGetErrorString(e));
-    s->sw_w = w;
+    if (!cl) s->sw_w = w;
     s->sw_last_ncol = ncol;
     s->sw_last_nlay = nlay;
     if (ptr_kind == RRTMG_HIP_HOST) {
@@ -845,6 +900,65 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("LW: ") + hipGetErrorString(e));
     }
     return 0;
+}
+
+int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev,
+                       const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                       const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr, const double *asdir, const double *asdif, const double *aldir,
+                       const double *aldif, const double *coszen, double adjes, int dyofyr, double scon,
+                       int inflgsw, int iceflgsw, int liqflgsw, const double *cldfr, const double *taucld,
+                       const double *ssacld, const double *asmcld, const double *fsfcld, const double *cicewp,
+                       const double *cliqwp, const double *reice, const double *reliq, const double *tauaer,
+                       const double *ssaaer, const double *asmaer, const double *ecaer, double *swuflx,
+                       double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind,
+                       void *stream)
+{
+    (void)iceflgsw;
+    (void)liqflgsw;
+    (void)reice;
+    (void)reliq;
+    (void)tauaer;
+    (void)ssaaer;
+    (void)asmaer;
+    (void)ecaer;
+    // rrtmg_sw_rad.nomcica.f90:471: icld outside 0..3 is taken as 2
+    if (icld < 0 || icld > 3) icld = 2;
+    if (icld == 0)
+        return sw_forward(ncol, nlay, 0, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                          o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr,
+                          swuflxc, swdflxc, swhrc, ptr_kind, stream, nullptr);
+    if (inflgsw != 0)
+        return fail(RRTMG_HIP_ENOTSUP, "inflgsw != 0: cloud optics from water paths and effective radii are not "
+                                       "implemented; pass the cloud optics with inflgsw = 0");
+    if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
+        const size_t n2 = (size_t)ncol * nlay;
+        int rc;
+        if (ptr_kind == RRTMG_HIP_HOST) {
+            const double *a[] = {taucld, ssacld, asmcld, fsfcld};
+            const char *nm[] = {"taucld", "ssacld", "asmcld", "fsfcld"};
+            if ((rc = check_host_array(cldfr, n2, "cldfr"))) return rc;
+            for (int i = 0; i < 4; ++i)
+                if ((rc = check_host_array(a[i], 14 * n2, nm[i]))) return rc;
+            if (cicewp && (rc = check_host_array(cicewp, n2, "cicewp"))) return rc;
+            if (cliqwp && (rc = check_host_array(cliqwp, n2, "cliqwp"))) return rc;
+            if ((rc = check_host_array(coszen, ncol, "coszen"))) return rc;
+            // rrtmg_sw_rad.nomcica.f90:501, 535-539: day columns take
+            // overcast or clear layers only
+            for (size_t i = 0; i < n2; ++i) {
+                const double f = cldfr[i];
+                if (coszen[i % ncol] >= 1.e-10 && f > 1.e-6 && f < 1.0 - 1.e-6)
+                    return fail(RRTMG_HIP_EINVAL, "partial cloud (fraction in (1e-6, 1 - 1e-6)) in a day column: the "
+                                                  "SW path without McICA takes overcast or clear layers only");
+            }
+        } else if (!cldfr || !taucld || !ssacld || !asmcld || !fsfcld) {
+            return fail(RRTMG_HIP_EINVAL, "null cloud array");
+        }
+    }
+    SwClouds cl{cldfr, taucld, ssacld, asmcld, fsfcld, cicewp, cliqwp};
+    return sw_forward(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr, swuflxc,
+                      swdflxc, swhrc, ptr_kind, stream, nullptr, &cl);
 }
 
 int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,

@@ -139,6 +139,35 @@ hipError_t launch_lw_cloud(const ColumnInputs &in, const LwClouds &cl, const LwO
 // the clear-sky forward's prep and column kernels alone
 hipError_t launch_lw_prep(const ColumnInputs &in, const LwWork &w, const LwDevTables &t, int mode, hipStream_t s);
 
+// SW with clouds (rrtmg_sw, icld >= 1, inflgsw = 0, iaer = 0), device FP64
+// inputs
+struct SwClouds {
+    const double *cldfr;                              // [nlay][ncol]
+    const double *taucld, *ssacld, *asmcld, *fsfcld;  // [nlay][ncol][14]: the reference's (nbndsw, ncol, nlay)
+    const double *cicewp, *cliqwp;                    // [nlay][ncol] or NULL (zero)
+};
+// flag bits per (layer, column): a fraction above 0 (the total stream
+// blends the layer's cloudy optics), above 1e-12 (reftra of the cloudy
+// optics, rrtmg_sw_spcvrt.f90:388), cldprop_sw's cloud test (the layer's
+// optics are the input's, delta-M scaled; otherwise depth 0, albedo 1)
+constexpr int SWC_BLEND = 1, SWC_REFTRA = 2, SWC_OPTICS = 4;
+struct SwCloudWork {
+    int *flags = nullptr;     // [nlay][ncol]
+    void *bflux = nullptr;    // [14][4][nlay+1][ncol] of Real: total up, total down, clear up, clear down
+    void *rows = nullptr;     // [chunk][4][nlay+1][64] of Real: prup, prupd of the clear and the total stream
+    int chunk = 0;
+    int *partial = nullptr;   // device int: layers of day columns with a fraction in (1e-6, 1 - 1e-6)
+};
+inline size_t sw_cloud_row_bytes(int nlay, size_t real_size) { return real_size * 4 * (size_t)(nlay + 1) * 64; }
+// flags (and the partial-cloud count, zeroed first) of a cloudy SW call
+hipError_t launch_sw_cloud_flags(const ColumnInputs &in, const SwClouds &cl, const SwCloudWork &cw, hipStream_t s);
+// the rest of the call (mode 0 FP64, 1 FP32 with FP64 islands): the
+// clear-sky prep and column kernels, the cloudy band kernel, the finish
+hipError_t launch_sw_cloud(const ColumnInputs &in, const SwClouds &cl, const SwOutputs &out, const SwWork &w,
+                           const SwCloudWork &cw, const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev);
+// the clear-sky forward's prep and column kernels alone
+hipError_t launch_sw_prep(const ColumnInputs &in, const SwWork &w, const SwDevTables &t, int mode, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // backward (vector-Jacobian product) of one call
 struct VjpCot {

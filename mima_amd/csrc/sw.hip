@@ -329,6 +329,18 @@ static hipError_t launch_sw_t(const ColumnInputs &in, const SwOutputs &out, cons
     return hipGetLastError();
 }
 
+hipError_t launch_sw_prep(const ColumnInputs &in, const SwWork &w, const SwDevTables &t, int mode, hipStream_t s)
+{
+    const int nb = (in.ncol + 255) / 256;
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
+    if (mode == 1)
+        hipLaunchKernelGGL(sw_prep_kernel<float>, dim3(nbl), dim3(256), 0, s, in, w, t.preflog, t.tref);
+    else
+        hipLaunchKernelGGL(sw_prep_kernel<double>, dim3(nbl), dim3(256), 0, s, in, w, t.preflog, t.tref);
+    hipLaunchKernelGGL(sw_column_kernel, dim3(nb), dim3(256), 0, s, in, w);
+    return hipGetLastError();
+}
+
 hipError_t launch_sw_finish(const ColumnInputs &in, const SwOutputs &out, void *bflux, const SwDevTables &t,
                             int mode, hipStream_t s)
 {

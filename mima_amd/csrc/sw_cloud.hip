@@ -1,0 +1,395 @@
+// Shortwave RRTMG with clouds on gfx950: rrtmg_sw with icld >= 1 and
+// inflgsw = 0 (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.nomcica.f90:498-729):
+// cldprop_sw's delta-M scaled cloud optics (rrtmg_sw_cldprop.f90:118-157)
+// and spcvrt_sw's two streams (rrtmg_sw_spcvrt.f90:302-600): the clear one,
+// and the total one whose layer reflectances, transmittances and direct-beam
+// transmittance blend the clear and the cloudy layer values by the cloud
+// fraction.  Without McICA the reference takes overcast or clear layers
+// only (it stops on partial cloud in a day column, :535-539); the blend is
+// its formula either way.
+//
+// The clear-sky forward's prep and column kernels supply the gas optics.
+// sw_cloud_flags_kernel classifies each (layer, column) once.  The band
+// kernel follows the clear-sky one (one wave per work unit, lane = (column,
+// g); both sweeps evaluate the layer optics) and adds, for layers with
+// cloud, the cloudy layer optics: clear plus delta-scaled cloud optical
+// depth, albedo and asymmetry (:459-466) through a general-asymmetry reftra
+// in FP64 in both precisions (cloud albedos near 1 are where a two-stream
+// eigenvalue in FP32 loses its digits).  Each sweep carries the recurrences
+// of both streams (the scratch row holds prup/prupd of each), and each level
+// sums four fluxes over g.  The clear-sky kernels are left unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.hpp"
+#include "rrtmg_device.hpp"
+#include "sw_optics.hpp"
+
+namespace rrtmg {
+
+// flags of each (layer, column) (rrtmg_sw_cldprop.f90:128-138,
+// rrtmg_sw_spcvrt.f90:388) and the count of layers of day columns with a
+// fraction the nomcica driver refuses (rrtmg_sw_rad.nomcica.f90:501, 535-539)
+__global__ __launch_bounds__(256) void sw_cloud_flags_kernel(ColumnInputs in, SwClouds cl, SwCloudWork cw)
+{
+    const int ncol = in.ncol, nlay = in.nlay;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int col = (int)(i % ncol);
+    const double cf = cl.cldfr[i];
+    double tauctot = 0.0;
+    for (int b = 0; b < 14; ++b) tauctot = tauctot + cl.taucld[i * 14 + b];
+    const double cwp = (cl.cicewp ? cl.cicewp[i] : 0.0) + (cl.cliqwp ? cl.cliqwp[i] : 0.0);
+    int fl = 0;
+    if (cf > 0.0) fl |= SWC_BLEND;
+    if (cf > 1.e-12) fl |= SWC_REFTRA;
+    if (cf >= 1.e-20 && (cwp >= 1.e-20 || tauctot >= 1.e-20)) fl |= SWC_OPTICS;
+    cw.flags[i] = fl;
+    if (in.coszen[col] >= 1.e-10 && cf > 1.e-6 && cf < 1.0 - 1.e-6) atomicAdd(cw.partial, 1);
+}
+
+__global__ void sw_cloud_zero_kernel(int *p) { *p = 0; }
+
+// PIFM two-stream reflectance and transmittance with asymmetry g, kmodts = 2
+// (rrtmg_sw_reftra.f90:138-303), FP64; the exp table of the mode (values
+// converted to FP64).
+template <typename Real>
+__device__ void reftra_g(double zto1, double zw, double zg, double prmuz, const Real *__restrict__ exp_tbl,
+                         double &pref, double &prefd, double &ptra, double &ptrad)
+{
+    const double eps = 1.e-08, od_lo = 0.06, zwcrit = 0.9999995;
+    auto ex = [&](double x) { return x <= od_lo ? 1.0 - x + 0.5 * x * x : (double)at32(exp_tbl, (unsigned)tbl_index(x)); };
+    const double zg3 = 3.0 * zg;
+    const double zgamma1 = (8.0 - zw * (5.0 + zg3)) * 0.25;
+    const double zgamma2 = 3.0 * (zw * (1.0 - zg)) * 0.25;
+    const double zgamma3 = (2.0 - zg3 * prmuz) * 0.25;
+    const double zgamma4 = 1.0 - zgamma3;
+    const double q = zg / (1.0 - zg);
+    const double zwo = zw / (1.0 - (1.0 - zw) * (q * q));
+    if (zwo >= zwcrit) {
+        const double za1 = zgamma1 * prmuz - zgamma3;
+        const double zgt = zgamma1 * zto1;
+        const double ze2 = ex(fmin(zto1 / prmuz, 500.0));
+        if (ze2 == 1.0) {
+            pref = 0.0;
+            ptra = 1.0;
+            prefd = 0.0;
+            ptrad = 1.0;
+        } else {
+            pref = (zgt - za1 * (1.0 - ze2)) / (1.0 + zgt);
+            ptra = 1.0 - pref;
+            prefd = zgt / (1.0 + zgt);
+            ptrad = 1.0 - prefd;
+        }
+        return;
+    }
+    const double za1 = zgamma1 * zgamma4 + zgamma2 * zgamma3;
+    const double za2 = zgamma1 * zgamma3 + zgamma2 * zgamma4;
+    const double zrk = sqrt(zgamma1 * zgamma1 - zgamma2 * zgamma2);
+    const double zrp = zrk * prmuz;
+    const double zrp1 = 1.0 + zrp, zrm1 = 1.0 - zrp;
+    const double zrk2 = 2.0 * zrk;
+    const double zrpp = 1.0 - zrp * zrp;
+    const double zrkg = zrk + zgamma1;
+    const double zr1 = zrm1 * (za2 + zrk * zgamma3);
+    const double zr2 = zrp1 * (za2 - zrk * zgamma3);
+    const double zr3 = zrk2 * (zgamma3 - za2 * prmuz);
+    const double zr4 = zrpp * zrkg;
+    const double zr5 = zrpp * (zrk - zgamma1);
+    const double zt1 = zrp1 * (za1 + zrk * zgamma4);
+    const double zt2 = zrm1 * (za1 - zrk * zgamma4);
+    const double zt3 = zrk2 * (zgamma4 + za1 * prmuz);
+    const double zbeta = (zgamma1 - zrk) / zrkg;
+    const double zem1 = ex(fmin(zrk * zto1, 500.0));
+    const double zem2 = ex(fmin(zto1 / prmuz, 500.0));
+    const double zep1 = 1.0 / zem1, zep2 = 1.0 / zem2;
+    const double zdenr = zr4 * zep1 + zr5 * zem1;
+    const double zdent = zdenr;  // zt4 = zr4, zt5 = zr5
+    if (zdenr >= -eps && zdenr <= eps) {
+        pref = eps;
+        ptra = zem2;
+    } else {
+        pref = zw * (zr1 * zep1 - zr2 * zem1 - zr3 * zem2) / zdenr;
+        ptra = zem2 - zem2 * zw * (zt1 * zep1 - zt2 * zem1 - zt3 * zep2) / zdent;
+    }
+    const double zemm = zem1 * zem1;
+    const double zdend = 1.0 / ((1.0 - zbeta * zemm) * zrkg);
+    prefd = zgamma2 * (1.0 - zemm) * zdend;
+    ptrad = zrk2 * zem1 * zdend;
+}
+
+// layer values of one stream
+template <typename Real>
+struct SwLayerRT {
+    Real r, rd, t, td, dbt;
+};
+
+// The clear and total layer values of one (layer, g): the clear optics as
+// the clear-sky kernel forms them; for a layer with cloud the cloudy optics
+// (rrtmg_sw_spcvrt.f90:459-466, no aerosol: the clear asymmetry is 0),
+// their reftra where the fraction exceeds 1e-12 (else the reference's
+// no-reftra values 0, 0, 1, 1) and the direct beam, blended with the clear
+// values by the fraction in FP64 (:502-549).
+template <int JB, typename Real>
+__device__ __forceinline__ void sw_cloud_layer(Real taug, Real taur, Real prmu0, const Real *__restrict__ exp_tbl,
+                                               int fl, double cf, const SwClouds &cl, size_t ix, SwLayerRT<Real> &c,
+                                               SwLayerRT<Real> &t)
+{
+    sw_layer_optics(taug, taur, prmu0, exp_tbl, c.r, c.rd, c.t, c.td, c.dbt);
+    t = c;
+    if (!(fl & SWC_BLEND)) return;
+    double pt = 0.0, pw = 1.0, pg = 0.0;
+    if (fl & SWC_OPTICS) {
+        const size_t j = ix * 14 + (JB - 16);
+        const double tauc = cl.taucld[j], ssac = cl.ssacld[j], asmc = cl.asmcld[j], ffp = cl.fsfcld[j];
+        const double ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssac;
+        pw = ffp1 * ssac / ffpssa;
+        pt = ffpssa * tauc;
+        pg = (asmc - ffp) / ffp1;
+    }
+    const double ztauc = (double)taur + (double)taug;
+    const double zomcc = (double)taur / ztauc;
+    const double ztauo = ztauc + pt;
+    double zomco = ztauc * zomcc + pt * pw;
+    const double zgco = (pt * pw * pg) / zomco;
+    zomco = zomco / ztauo;
+    const double mu0 = (double)prmu0;
+    double ro = 0.0, rdo = 0.0, to = 1.0, tdo = 1.0;
+    if (fl & SWC_REFTRA) reftra_g(ztauo, zomco, zgco, mu0, exp_tbl, ro, rdo, to, tdo);
+    const double ze1 = ztauo / mu0;
+    const double dbto = ze1 <= 0.06 ? 1.0 - ze1 + 0.5 * ze1 * ze1 : (double)at32(exp_tbl, (unsigned)tbl_index(ze1));
+    const double zclear = 1.0 - cf;
+    t.r = (Real)(zclear * (double)c.r + cf * ro);
+    t.rd = (Real)(zclear * (double)c.rd + cf * rdo);
+    t.t = (Real)(zclear * (double)c.t + cf * to);
+    t.td = (Real)(zclear * (double)c.td + cf * tdo);
+    t.dbt = (Real)(zclear * (double)c.dbt + cf * dbto);
+}
+
+// adding-method state of one stream (rrtmg_sw_vrtqdr.f90:103-150)
+template <typename Real>
+struct SwStream {
+    Real prup, prupd;        // bottom-up: reflectances of the atmosphere below the current level
+    Real ztdn, prdnd, ptdbt; // top-down
+    __device__ __forceinline__ void up(const SwLayerRT<Real> &l)
+    {
+        const Real zreflect = fdiv((Real)1, (Real)1 - prupd * l.rd);
+        const Real pu = l.r + (l.td * ((l.t - l.dbt) * prupd + l.dbt * prup)) * zreflect;
+        const Real pud = l.rd + l.td * l.td * prupd * zreflect;
+        prup = pu;
+        prupd = pud;
+    }
+    __device__ __forceinline__ void down(const SwLayerRT<Real> &l)
+    {
+        const Real zreflect = fdiv((Real)1, (Real)1 - l.rd * prdnd);
+        const Real tdn = ptdbt * l.t + (l.td * ((ztdn - ptdbt) + ptdbt * l.r * prdnd)) * zreflect;
+        const Real dnd = l.rd + l.td * l.td * prdnd * zreflect;
+        ptdbt = l.dbt * ptdbt;
+        ztdn = tdn;
+        prdnd = dnd;
+    }
+    // level fluxes for the reflectances (pu, pud) of the atmosphere below
+    __device__ __forceinline__ void flux(Real pu, Real pud, Real &fu, Real &fd) const
+    {
+        const Real zreflect = fdiv((Real)1, (Real)1 - prdnd * pud);
+        fu = (ptdbt * pu + (ztdn - ptdbt) * pud) * zreflect;
+        fd = ptdbt + (ztdn - ptdbt + ptdbt * pu * prdnd) * zreflect;
+    }
+};
+
+// One band JB of one work unit with clouds.  row: this wave's scratch,
+// [4][nlay+1][64] (prup, prupd of the clear stream, then of the total one);
+// s_red: 64 LDS entries for the FP64 g-sum.
+template <int JB, typename Real>
+__device__ __forceinline__ void sw_cloud_band_body(const ColumnInputs &in, const SwWork &w, const SwClouds &cl,
+                                                   const SwCloudWork &cw, const SwBand &B, const Real *__restrict__ ktab,
+                                                   const Real *__restrict__ exp_tbl, WaveUnit u, int tile, int lane,
+                                                   Real *__restrict__ row, Real *s_red)
+{
+    const int ncol = in.ncol, nlay = in.nlay;
+    constexpr int ng = sw_ng(JB);
+    SwBand Bc = B;
+    Bc.nspa = SW_NSPA[JB - 16];
+    Bc.nspb = SW_NSPB[JB - 16];
+    const int cl_ = lane / ng, g = lane - cl_ * ng;
+    const GSum<Real, ng> gsum(lane, g);
+    const int col = tile * TILE_COLS + u.first_col + cl_;
+    const bool active = cl_ < u.ncols && col < ncol && in.coszen[min(col, ncol - 1)] >= 1.e-10;
+    if (!__any(active)) return;  // night: the finish kernel writes zeros
+    const Real *coef = (const Real *)w.coef;
+    Tab<Real> T{ktab, ng, g};
+    const size_t L1 = (size_t)(nlay + 1) * 64;
+    Real *s_cu = row, *s_cud = row + L1, *s_tu = row + 2 * L1, *s_tud = row + 3 * L1;
+
+    Real prmu0 = (Real)1, zincflx = (Real)0, albp = (Real)0, albd = (Real)0;
+    int laytrop = 0;
+    SwLayer<Real> c;
+    if (active) {
+        prmu0 = (Real)in.coszen[col];
+        laytrop = w.laytrop[col];
+        const bool nir = (JB <= 24) || JB == 29;  // rrtmg_sw_rad.nomcica.f90:567-577
+        albp = (Real)(nir ? in.aldir[col] : in.asdir[col]);
+        albd = (Real)(nir ? in.aldif[col] : in.asdif[col]);
+        int lsf = w.laysolfr[(size_t)(JB - 16) * ncol + col];
+        sw_load_layer(c, coef, w.ipk, lsf - 1, col, ncol, nlay);
+        double sfz = sw_sfluxzen(JB, Bc, T, c, lsf <= laytrop);
+        zincflx = (Real)(in.adjflux * sfz * (double)prmu0);
+    }
+    // both streams' layer values of layer `lay` (1-based, bottom-up)
+    auto layer = [&](int lay, SwLayerRT<Real> &lc, SwLayerRT<Real> &lt) {
+        lc = SwLayerRT<Real>{(Real)0, (Real)0, (Real)1, (Real)1, (Real)1};
+        lt = lc;
+        if (!active) return;
+        sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
+        Real taug, taur;
+        sw_gas(JB, Bc, T, c, lay <= laytrop, taug, taur);
+        const size_t ix = (size_t)(lay - 1) * ncol + col;
+        const int fl = cw.flags[ix];
+        sw_cloud_layer<JB, Real>(taug, taur, prmu0, exp_tbl, fl, (fl & SWC_BLEND) ? cl.cldfr[ix] : 0.0, cl, ix, lc,
+                                 lt);
+    };
+
+    // bottom-up sweep
+    SwStream<Real> sc{albp, albd, (Real)1, (Real)0, (Real)1}, st = sc;
+    at32(s_cu, (unsigned)(nlay * 64 + lane)) = albp;
+    at32(s_cud, (unsigned)(nlay * 64 + lane)) = albd;
+    at32(s_tu, (unsigned)(nlay * 64 + lane)) = albp;
+    at32(s_tud, (unsigned)(nlay * 64 + lane)) = albd;
+    for (int lay = 1; lay <= nlay; ++lay) {
+        const int jk = nlay + 1 - lay;
+        SwLayerRT<Real> lc, lt;
+        layer(lay, lc, lt);
+        sc.up(lc);
+        st.up(lt);
+        const unsigned ix = (unsigned)((jk - 1) * 64 + lane);
+        at32(s_cu, ix) = sc.prup;
+        at32(s_cud, ix) = sc.prupd;
+        at32(s_tu, ix) = st.prup;
+        at32(s_tud, ix) = st.prupd;
+    }
+
+    // top-down sweep with level fluxes
+    const size_t bs = (size_t)(nlay + 1) * ncol;
+    Real *bf = (Real *)cw.bflux + (size_t)(JB - 16) * 4 * bs;  // [4][nlay+1][ncol]
+    auto store = [&](int k, int lev, Real v) {
+        if (active && g == 0) bf[(size_t)k * bs + (size_t)lev * ncol + col] = v;
+    };
+    for (int jk = 1; jk <= nlay + 1; ++jk) {
+        if (jk > 1) {
+            SwLayerRT<Real> lc, lt;
+            layer(nlay + 2 - jk, lc, lt);
+            sc.down(lc);
+            st.down(lt);
+        }
+        const unsigned ix = (unsigned)((jk - 1) * 64 + lane);
+        Real cu, cd, tu, td;
+        sc.flux(at32(s_cu, ix), at32(s_cud, ix), cu, cd);
+        st.flux(at32(s_tu, ix), at32(s_tud, ix), tu, td);
+        const int lev = nlay + 1 - jk;
+        store(0, lev, gsum.sum(active ? zincflx * tu : (Real)0, s_red));
+        store(1, lev, gsum.sum(active ? zincflx * td : (Real)0, s_red));
+        store(2, lev, gsum.sum(active ? zincflx * cu : (Real)0, s_red));
+        store(3, lev, gsum.sum(active ? zincflx * cd : (Real)0, s_red));
+    }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(64) void sw_cloud_band_kernel(ColumnInputs in, SwWork w, SwClouds cl, SwCloudWork cw,
+                                                           const SwBand *__restrict__ bands, int nwaves, int blk0,
+                                                           int nblocks, int exp_off, const Real *__restrict__ tab,
+                                                           const WaveUnit *__restrict__ units)
+{
+    __shared__ Real s_red[64];
+    const int blk = xcd_block();
+    if (blk >= nblocks) return;
+    const int tile = (blk0 + blk) / nwaves;
+    const WaveUnit u = units[(blk0 + blk) % nwaves];
+    const SwBand B = bands[u.band];
+    Real *row = (Real *)cw.rows + (size_t)blk * 4 * (in.nlay + 1) * 64;
+    const Real *exp_tbl = tab + exp_off;
+#define BODY(jb)                                                                                                \
+    case jb - 16:                                                                                               \
+        sw_cloud_band_body<jb, Real>(in, w, cl, cw, B, tab, exp_tbl, u, tile, (int)threadIdx.x, row, s_red);    \
+        break;
+    switch (u.band) { SW_BANDS(BODY) }
+#undef BODY
+}
+
+// finish: band sums, net fluxes and heating of both streams
+// (rrtmg_sw_rad.nomcica.f90:700-726), one thread per (layer, column); night
+// columns zero
+template <typename Real>
+__global__ __launch_bounds__(256) void sw_cloud_finish_kernel(ColumnInputs in, SwCloudWork cw, SwOutputs out,
+                                                              double heatfac)
+{
+    const int ncol = in.ncol, nlay = in.nlay;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    const int lay = (int)(i / ncol), col = (int)(i - (size_t)lay * ncol);
+    const Real *bflux = (const Real *)cw.bflux;
+    const size_t bs = (size_t)(nlay + 1) * ncol;
+    const size_t il = (size_t)lay * ncol + col;
+    const bool night = in.coszen[col] < 1.e-10;
+    double net[2][2];
+    for (int j = 0; j < 2; ++j) {
+        const int k = lay + j;
+        double f[4] = {0.0, 0.0, 0.0, 0.0};
+        if (!night)
+            for (int b = 0; b < 14; ++b)
+                for (int s = 0; s < 4; ++s) f[s] += (double)bflux[((size_t)b * 4 + s) * bs + (size_t)k * ncol + col];
+        if (j == 0 || lay == nlay - 1) {
+            const size_t ix = (size_t)k * ncol + col;
+            out.uflx[ix] = f[0];
+            out.dflx[ix] = f[1];
+            out.uflxc[ix] = f[2];
+            out.dflxc[ix] = f[3];
+        }
+        net[0][j] = f[1] - f[0];
+        net[1][j] = f[3] - f[2];
+    }
+    const double zdpgcp = heatfac / (in.plev[il] - in.plev[il + ncol]);
+    const bool top = lay == nlay - 1;
+    out.hr[il] = (night || top) ? 0.0 : (net[0][1] - net[0][0]) * zdpgcp;
+    out.hrc[il] = (night || top) ? 0.0 : (net[1][1] - net[1][0]) * zdpgcp;
+}
+
+hipError_t launch_sw_cloud_flags(const ColumnInputs &in, const SwClouds &cl, const SwCloudWork &cw, hipStream_t s)
+{
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
+    hipLaunchKernelGGL(sw_cloud_zero_kernel, dim3(1), dim3(1), 0, s, cw.partial);
+    hipLaunchKernelGGL(sw_cloud_flags_kernel, dim3(nbl), dim3(256), 0, s, in, cl, cw);
+    return hipGetLastError();
+}
+
+template <typename Real>
+static hipError_t launch_sw_cloud_t(const ColumnInputs &in, const SwClouds &cl, const SwOutputs &out,
+                                    const SwWork &w, const SwCloudWork &cw, const SwDevTables &t, int mode,
+                                    hipStream_t s, hipEvent_t *ev)
+{
+    const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
+    if (ev) (void)hipEventRecord(ev[0], s);
+    hipError_t e = launch_sw_prep(in, w, t, mode, s);
+    if (e != hipSuccess) return e;
+    if (ev) (void)hipEventRecord(ev[1], s);
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    const int nblocks = ntiles * t.desc.nwaves;
+    for (int b0 = 0; b0 < nblocks; b0 += cw.chunk) {
+        const int nb_c = std::min(cw.chunk, nblocks - b0);
+        hipLaunchKernelGGL(sw_cloud_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw, t.bands,
+                           t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+    }
+    if (ev) (void)hipEventRecord(ev[2], s);
+    hipLaunchKernelGGL(sw_cloud_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cw, out, t.heatfac);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_cloud(const ColumnInputs &in, const SwClouds &cl, const SwOutputs &out, const SwWork &w,
+                           const SwCloudWork &cw, const SwDevTables &t, int mode, hipStream_t s, hipEvent_t *ev)
+{
+    return mode == 1 ? launch_sw_cloud_t<float>(in, cl, out, w, cw, t, mode, s, ev)
+                     : launch_sw_cloud_t<double>(in, cl, out, w, cw, t, mode, s, ev);
+}
+
+}  // namespace rrtmg

@@ -162,6 +162,34 @@ def rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr
     return tuple(outs)
 
 
+def rrtmg_sw_cloud(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                   n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, inflgsw, iceflgsw,
+                   liqflgsw, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, reice=None,
+                   reliq=None, stream=None):
+    """rrtmg_sw with clouds (rrtmg_sw_rad.nomcica.f90:78-88): icld >= 1,
+    inflgsw = 0 (cloud optical depth, albedo, asymmetry and forward
+    fraction), overcast or clear layers in day columns.
+
+    ``cldfr``, ``cicewp``, ``cliqwp`` (nlay, ncol); the cloud optics
+    (nlay, ncol, 14), i.e. the reference's (nbndsw, ncol, nlay).  Returns
+    (swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc): total-sky, then
+    clear-sky."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    optr = [ctypes.cast(ctypes.c_void_p(o.data_ptr()), _D) if dev else o.ctypes.data_as(_D) for o in outs]
+    rc = lib().rrtmg_hip_sw_cloud(
+        int(ncol), int(nlay), int(icld), int(iaer),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             asdir, asdif, aldir, aldif, coszen)],
+        ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon), int(inflgsw), int(iceflgsw), int(liqflgsw),
+        *[a.inp(x) for x in (cldfr, taucld, ssacld, asmcld, fsfcld, cicewp, cliqwp, reice, reliq)],
+        None, None, None, None, *optr,
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
 def rrtmg_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
                    n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, inflglw, iceflglw, liqflglw,
                    cldfr, taucld, cicewp=None, cliqwp=None, reice=None, reliq=None, tauaer=None, stream=None):
@@ -275,6 +303,14 @@ def sw_columns(c, stream=None):
 def lw_columns(c, tauaer=None, stream=None):
     out = rrtmg_lw(c["ncol"], c["nlay"], 0, 0, *[c[k] for k in LW_KEYS], tauaer=tauaer, stream=stream)
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
+
+
+def sw_cloud_columns(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, stream=None):
+    """rrtmg_sw_cloud (inflgsw = 0) on a mima_amd.columns dict."""
+    out = rrtmg_sw_cloud(c["ncol"], c["nlay"], icld, 0, *[c[k] for k in SW_KEYS], c["adjes"], c["dyofyr"],
+                         c["scon"], 0, 0, 0, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=cicewp, cliqwp=cliqwp,
+                         stream=stream)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), out))
 
 
 def lw_cloud_columns(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None, stream=None):

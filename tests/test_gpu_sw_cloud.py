@@ -1,0 +1,194 @@
+"""GPU parity of the SW cloudy path (rrtmg_hip_sw_cloud: icld >= 1,
+inflgsw = 0) through the C ABI, against the CPU oracle (pinned to the
+reference Fortran's rrtmg_sw with clouds, tests/test_oracle_sw_cloud.py) and
+against the reference's golden vectors.
+
+Tolerances as for the clear sky: FP64 mode agrees with the oracle to 1e-8
+W/m2 and 1e-8 K/day; FP32-mixed stays within 0.05 W/m2 and 0.05 K/day.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mima_amd.columns import config_columns, make_columns, make_sw_clouds, subset
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = dict(np.load(os.path.join(ROOT, "tests", "golden", "sw_ref_cloud.npz"), allow_pickle=False))
+KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+        "asdir", "asdif", "aldir", "aldif", "coszen")
+CL = ("cldfr", "taucld", "ssacld", "asmcld", "fsfcld")
+OUT = ("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc")
+FP64_ATOL = 1e-8
+MIXED_ATOL = 0.05
+
+
+@pytest.fixture(scope="module")
+def rr():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from mima_amd import rrtmg
+    rrtmg.rrtmg_sw_ini()
+    yield rrtmg
+    rrtmg.set_precision("fp64")
+    rrtmg.set_checks(False)
+
+
+def _np(out):
+    return {k: (v.cpu().numpy() if hasattr(v, "cpu") else v) for k, v in out.items()}
+
+
+def _cloud(rr, c, prec, icld, cl, device=True, **kw):
+    import torch
+    rr.set_precision(prec)
+    if device:
+        t = lambda a: None if a is None else torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+        return _np(rr.sw_cloud_columns(rr.to_device(c), icld, *[t(a) for a in cl],
+                                       **{k: t(v) for k, v in kw.items()}))
+    return _np(rr.sw_cloud_columns(c, icld, *cl, **kw))
+
+
+def _gold(pre):
+    c = {k: GOLD[pre + "_" + k] for k in KEYS}
+    c.update(ncol=int(GOLD[pre + "_ncol"]), nlay=int(GOLD[pre + "_nlay"]), adjes=float(GOLD[pre + "_adjes"]),
+             dyofyr=int(GOLD[pre + "_dyofyr"]), scon=float(GOLD[pre + "_scon"]))
+    kw = dict(cicewp=GOLD["a_cicewp"], cliqwp=GOLD["a_cliqwp"]) if pre == "a" else {}
+    return c, [GOLD[pre + "_" + k] for k in CL], kw
+
+
+def _sample(n=512, nlay=40, seed=0):
+    full = make_columns(64, 32, nlay, seed=seed)
+    c = subset(full, np.sort(np.random.default_rng(seed + 7).choice(full["ncol"], n, replace=False)))
+    return c, make_sw_clouds(c, seed=seed + 9)
+
+
+def _close(a, b, tol, what):
+    for k in OUT:
+        d = float(np.max(np.abs(a[k] - b[k])))
+        assert d < tol, (what, k, d)
+
+
+@pytest.mark.parametrize("pre,icld", [("a", 2), ("b", 1)])
+def test_fp64_matches_reference_golden(rr, pre, icld):
+    c, cl, kw = _gold(pre)
+    g = _cloud(rr, c, "fp64", icld, cl, **kw)
+    for k in OUT:
+        assert float(np.max(np.abs(g[k] - GOLD["%s_out_%s" % (pre, k)]))) < FP64_ATOL, k
+
+
+@pytest.mark.parametrize("nlay,seed", [(40, 0), (60, 1), (20, 2)])
+def test_fp64_matches_oracle(rr, nlay, seed):
+    c, cl = _sample(300, nlay, seed)
+    _close(_cloud(rr, c, "fp64", 2, cl), po.rrtmg_sw_cloud(c, 2, *cl), FP64_ATOL, "fp64")
+
+
+def test_mixed_within_tolerance(rr):
+    c, cl = _sample(1024, 40, 3)
+    _close(_cloud(rr, c, "mixed", 2, cl), po.rrtmg_sw_cloud(c, 2, *cl), MIXED_ATOL, "mixed")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_clear_sky_outputs_and_icld0_are_the_clear_path(rr, prec):
+    """The cloud kernel's clear stream gives rrtmg_hip_sw's results (same gas
+    optics and adding method in another kernel: equal up to the compiler's
+    contraction of a few products); icld = 0 is rrtmg_hip_sw itself."""
+    c, cl = _sample(256, 40, 4)
+    rr.set_precision(prec)
+    clr = _np(rr.sw_columns(rr.to_device(c)))
+    g = _cloud(rr, c, prec, 2, cl)
+    tol = 1e-10 if prec == "fp64" else 1e-2
+    for k in ("swuflx", "swdflx", "swhr"):
+        assert float(np.max(np.abs(g[k + "c"] - clr[k]))) < tol, k
+    g0 = _cloud(rr, c, prec, 0, cl)
+    for k in OUT:
+        np.testing.assert_array_equal(g0[k], clr[k.rstrip("c") if k.endswith("c") else k])
+    assert float(np.max(np.abs(g["swdflx"] - clr["swdflx"]))) > 100.0
+
+
+def test_host_and_device_paths_identical(rr):
+    c, cl = _sample(200, 40, 5)
+    for prec in ("fp64", "mixed"):
+        a = _cloud(rr, c, prec, 2, cl, device=False)
+        b = _cloud(rr, c, prec, 2, cl, device=True)
+        for k in OUT:
+            assert np.array_equal(a[k], b[k]), (prec, k)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_deterministic_and_batch_split_invariant(rr, prec):
+    c, cl = _sample(333, 40, 6)
+    a = _cloud(rr, c, prec, 2, cl)
+    b = _cloud(rr, c, prec, 2, cl)
+    for k in OUT:
+        assert np.array_equal(a[k], b[k]), k
+    parts = [_cloud(rr, subset(c, i), prec, 2, [np.ascontiguousarray(x[:, i]) for x in cl])
+             for i in (np.arange(0, 150), np.arange(150, 333))]
+    for k in OUT:
+        assert np.array_equal(np.concatenate([parts[0][k], parts[1][k]], axis=1), a[k]), k
+
+
+def test_full_size_zero_depth_cloud_and_mixed_tolerance(rr):
+    """256 x 128 x 40 (32768 columns): an overcast cloud of depth 0, albedo 1
+    and asymmetry 0 in every layer leaves the total stream equal to the clear
+    one (rrtmg_sw_spcvrt.f90:459-466); seeded overcast blocks in FP32-mixed
+    within 0.05 of FP64 at this size."""
+    c = config_columns("c1_mixed_vs_fp64", seed=0)
+    L, n = c["nlay"], c["ncol"]
+    z = np.zeros((L, n, 14))
+    zero = [np.ones((L, n)), z, np.ones_like(z), z, z]
+    cl = make_sw_clouds(c, seed=2, edge_cases=False)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        g = _cloud(rr, c, prec, 2, zero)
+        tol = 1e-9 if prec == "fp64" else 1e-2
+        for k in ("swuflx", "swdflx", "swhr"):
+            assert float(np.max(np.abs(g[k] - g[k + "c"]))) < tol, (prec, k)
+        res[prec] = _cloud(rr, c, prec, 2, cl)
+        assert np.all(np.isfinite(res[prec]["swhr"]))
+    _close(res["mixed"], res["fp64"], MIXED_ATOL, "full size")
+
+
+def test_error_contract(rr):
+    from mima_amd.rrtmg import RRTMGError, SW_KEYS
+    c, cl = _sample(32, 40, 11)
+    args = [c[k] for k in SW_KEYS]
+    rr.set_precision("fp64")
+    day = int(np.argmax(c["coszen"]))
+    with pytest.raises(RRTMGError) as e:   # cloud optics from water paths: not implemented
+        rr.rrtmg_sw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 1.0, 0, 1370.0, 2, 1, 1, *cl)
+    assert e.value.code == -95
+    with pytest.raises(RRTMGError) as e:   # aerosols
+        rr.rrtmg_sw_cloud(c["ncol"], c["nlay"], 2, 10, *args, 1.0, 0, 1370.0, 0, 0, 0, *cl)
+    assert e.value.code == -95
+    with pytest.raises(RRTMGError) as e:   # the clear-sky entry refuses clouds
+        rr.rrtmg_sw(c["ncol"], c["nlay"], 2, 0, *args, 1.0, 0, 1370.0)
+    assert e.value.code == -95
+    bad = [x.copy() for x in cl]
+    bad[0][4, day] = 0.5   # partial cloud in a day column (rrtmg_sw_rad.nomcica.f90:535-539)
+    with pytest.raises(RRTMGError) as e:
+        rr.sw_cloud_columns(c, 2, *bad)
+    assert e.value.code == -22
+    rr.set_checks(True)   # device pointers: refused when the input checks are on
+    try:
+        with pytest.raises(RRTMGError) as e:
+            _cloud(rr, c, "fp64", 2, bad)
+        assert e.value.code == -22
+    finally:
+        rr.set_checks(False)
+    bad = [x.copy() for x in cl]
+    bad[1][3, 2, 5] = np.nan
+    with pytest.raises(RRTMGError) as e:
+        rr.sw_cloud_columns(c, 2, *bad)
+    assert e.value.code == -22
+    rr.set_precision("smooth")
+    with pytest.raises(RRTMGError) as e:
+        rr.sw_cloud_columns(c, 2, *cl)
+    assert e.value.code == -95
+    rr.set_precision("fp64")
+    a = rr.sw_cloud_columns(c, 9, *cl)   # out-of-range icld: 2 (rrtmg_sw_rad.nomcica.f90:471)
+    b = rr.sw_cloud_columns(c, 1, *cl)
+    for k in OUT:
+        assert np.array_equal(a[k], b[k])
