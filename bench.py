@@ -200,19 +200,22 @@ def adjoint_timing(torch, rrtmg, config_columns, args):
 
 
 def cloudy_timing(torch, rrtmg, config_columns, args):
-    """LW with clouds (rrtmg_hip_lw_cloud, maximum-random overlap, in-cloud
-    optical depths) on the benchmark grid (256 x 128 = 32768 columns x 40
-    levels) with seeded cloud blocks: columns/s in FP32-mixed and FP64, the
-    per-kernel times of the mixed call, the mixed path's largest difference
-    from FP64 and the cost relative to the clear-sky LW call."""
-    from mima_amd.columns import make_clouds
+    """LW and SW with clouds (rrtmg_hip_lw_cloud with maximum-random overlap
+    and in-cloud optical depths; rrtmg_hip_sw_cloud with overcast layers and
+    their optics) on the benchmark grid (256 x 128 = 32768 columns x 40
+    levels) with seeded cloud blocks: per side, columns/s in FP32-mixed and
+    FP64, the per-kernel times of the mixed call, the mixed path's largest
+    difference from FP64 and the cost relative to the clear-sky call."""
+    from mima_amd.columns import make_clouds, make_sw_clouds
     import ctypes
     cols = config_columns("c1_mixed_vs_fp64", seed=0)
     ncol, nlay = cols["ncol"], cols["nlay"]
-    cf, tau = make_clouds(cols, seed=0, edge_cases=False)
     d = rrtmg.to_device(cols)
-    dcf = torch.as_tensor(cf, device="cuda")
-    dtau = torch.as_tensor(tau, device="cuda")
+    dev = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    lcf, ltau = make_clouds(cols, seed=0, edge_cases=False)
+    scl = make_sw_clouds(cols, seed=0, edge_cases=False)
+    lw_args = (dev(lcf), dev(ltau))
+    sw_args = tuple(dev(a) for a in scl)
 
     def timed(fn):
         for _ in range(max(args.warmup, 3)):
@@ -225,36 +228,40 @@ def cloudy_timing(torch, rrtmg, config_columns, args):
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) / n, o
 
-    res, outs = {}, {}
-    for prec in ("fp64", "mixed"):
-        rrtmg.set_precision(prec)
-        res[prec], o = timed(lambda: rrtmg.lw_cloud_columns(d, 2, dcf, dtau))
-        outs[prec] = {k: v.cpu().numpy() for k, v in o.items()}
-    clear_ms, _ = timed(lambda: rrtmg.lw_columns(d))
-    rrtmg.lib().rrtmg_hip_set_timing(1)
-    kt = np.zeros(3)
-    for _ in range(5):
-        rrtmg.lw_cloud_columns(d, 2, dcf, dtau)
-        ms = (ctypes.c_float * 3)()
-        if rrtmg.lib().rrtmg_hip_last_kernel_ms(1, ms) == 0:
-            kt += np.array(ms[:])
-    rrtmg.lib().rrtmg_hip_set_timing(0)
-    kt /= 5
-    dfl = max(float(np.max(np.abs(outs["mixed"][k] - outs["fp64"][k]))) for k in ("uflx", "dflx"))
-    dhr = float(np.max(np.abs(outs["mixed"]["hr"] - outs["fp64"]["hr"])))
-    print("[bench] cloudy LW %d x %d: mixed %.3f ms, fp64 %.3f ms, clear-sky LW %.3f ms" %
-          (ncol, nlay, res["mixed"] * 1e3, res["fp64"] * 1e3, clear_ms * 1e3), file=sys.stderr, flush=True)
-    bcol = 8 * (29 * nlay + 10)   # the clear-sky LW bytes plus cldfr and taucld (17 doubles per layer)
-    return {"workload": "LW with clouds (icld = 2 maximum-random overlap, inflglw = 0), 256x128 grid = %d columns "
-                        "x %d levels, 0-3 seeded cloud blocks per column (%.0f%% of layers cloudy)"
-                        % (ncol, nlay, 100.0 * float(np.mean(cf > 0))),
-            "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
-            "fp64_ms_per_step": res["fp64"] * 1e3, "clear_sky_lw_ms": clear_ms * 1e3,
-            "cloudy_vs_clear_cost": res["mixed"] / clear_ms,
-            "kernel_ms": {"prep_and_overlap": float(kt[0]), "band": float(kt[1]), "finish": float(kt[2])},
-            "band_kernel_GBps": bcol * ncol / (kt[1] * 1e-3) / 1e9 if kt[1] > 0 else None,
-            "bytes_per_column": bcol,
-            "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
+    out = {}
+    sides = (("lw", 1, lambda: rrtmg.lw_cloud_columns(d, 2, *lw_args), lambda: rrtmg.lw_columns(d),
+              ("uflx", "dflx"), "hr", float(np.mean(lcf > 0)), "maximum-random overlap, in-cloud optical depths"),
+             ("sw", 0, lambda: rrtmg.sw_cloud_columns(d, 2, *sw_args), lambda: rrtmg.sw_columns(d),
+              ("swuflx", "swdflx"), "swhr", float(np.mean(scl[0] > 0)), "overcast layers, delta-M cloud optics"))
+    for side, lwflag, cloudy, clear, fk, hk, frac, what in sides:
+        res, outs = {}, {}
+        for prec in ("fp64", "mixed"):
+            rrtmg.set_precision(prec)
+            res[prec], o = timed(cloudy)
+            outs[prec] = {k: v.cpu().numpy() for k, v in o.items()}
+        clear_ms, _ = timed(clear)
+        rrtmg.lib().rrtmg_hip_set_timing(1)
+        kt = np.zeros(3)
+        for _ in range(5):
+            cloudy()
+            ms = (ctypes.c_float * 3)()
+            if rrtmg.lib().rrtmg_hip_last_kernel_ms(lwflag, ms) == 0:
+                kt += np.array(ms[:])
+        rrtmg.lib().rrtmg_hip_set_timing(0)
+        kt /= 5
+        dfl = max(float(np.max(np.abs(outs["mixed"][k] - outs["fp64"][k]))) for k in fk)
+        dhr = float(np.max(np.abs(outs["mixed"][hk] - outs["fp64"][hk])))
+        print("[bench] cloudy %s %d x %d: mixed %.3f ms, fp64 %.3f ms, clear-sky %.3f ms" %
+              (side.upper(), ncol, nlay, res["mixed"] * 1e3, res["fp64"] * 1e3, clear_ms * 1e3), file=sys.stderr,
+              flush=True)
+        out[side] = {"workload": "%s with clouds (%s), 256x128 grid = %d columns x %d levels, seeded cloud blocks "
+                                 "(%.0f%% of layers cloudy)" % (side.upper(), what, ncol, nlay, 100.0 * frac),
+                     "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
+                     "fp64_ms_per_step": res["fp64"] * 1e3, "clear_sky_ms": clear_ms * 1e3,
+                     "cloudy_vs_clear_cost": res["mixed"] / clear_ms,
+                     "kernel_ms": {"prep": float(kt[0]), "band": float(kt[1]), "finish": float(kt[2])},
+                     "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
+    return out
 
 
 def surrogate_timing(torch, rrtmg, config_columns, args):
@@ -472,7 +479,7 @@ def main():
         if "large" in legs and args.large_steps > 0:
             out["large_batch"] = large_batch_timing(torch, rrtmg, config_columns, args)
         if "cloudy" in legs:
-            out["cloudy_lw"] = cloudy_timing(torch, rrtmg, config_columns, args)
+            out["cloudy"] = cloudy_timing(torch, rrtmg, config_columns, args)
         print(json.dumps(out))
         return
     cols = config_columns(args.config, seed=0)
@@ -589,7 +596,7 @@ def main():
         "adjoint": adj,
         "surrogate": sur,
         "large_batch": large,
-        "cloudy_lw": cloudy,
+        "cloudy": cloudy,
     }
     print(json.dumps(out))
 

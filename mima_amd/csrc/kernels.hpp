@@ -154,11 +154,16 @@ constexpr int SWC_BLEND = 1, SWC_REFTRA = 2, SWC_OPTICS = 4;
 struct SwCloudWork {
     int *flags = nullptr;     // [nlay][ncol]
     void *bflux = nullptr;    // [14][4][nlay+1][ncol] of Real: total up, total down, clear up, clear down
-    void *rows = nullptr;     // [chunk][4][nlay+1][64] of Real: prup, prupd of the clear and the total stream
+    void *rows = nullptr;     // chunk rows of sw_cloud_row_bytes: prup, prupd of both streams (FP32: + kept values)
     int chunk = 0;
     int *partial = nullptr;   // device int: layers of day columns with a fraction in (1e-6, 1 - 1e-6)
 };
-inline size_t sw_cloud_row_bytes(int nlay, size_t real_size) { return real_size * 4 * (size_t)(nlay + 1) * 64; }
+// scratch row of one wave unit of the cloudy SW band kernel: prup, prupd of
+// both streams, and in FP32 the five kept values of each cloudy layer
+__host__ __device__ inline size_t sw_cloud_row_bytes(int nlay, size_t real_size)
+{
+    return real_size * (4 * (size_t)(nlay + 1) + (real_size == 4 ? 5 * (size_t)nlay : 0)) * 64;
+}
 // flags (and the partial-cloud count, zeroed first) of a cloudy SW call
 hipError_t launch_sw_cloud_flags(const ColumnInputs &in, const SwClouds &cl, const SwCloudWork &cw, hipStream_t s);
 // the rest of the call (mode 0 FP64, 1 FP32 with FP64 islands): the

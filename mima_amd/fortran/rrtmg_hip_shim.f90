@@ -88,6 +88,25 @@ module rrtmg_hip_capi
       real(c_double), intent(out) :: uflx(*), dflx(*), hr(*), uflxc(*), dflxc(*), hrc(*)
       type(c_ptr), value :: stream
     end function
+    integer(c_int) function c_sw_cloud(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                                       h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                                       asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                                       inflgsw, iceflgsw, liqflgsw, cldfr, taucld, ssacld, asmcld, fsfcld, &
+                                       cicewp, cliqwp, reice, reliq, tauaer, ssaaer, asmaer, ecaer, &
+                                       swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, ptr_kind, stream) &
+                                       bind(C, name="rrtmg_hip_sw_cloud")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: ncol, nlay, icld, iaer, dyofyr, inflgsw, iceflgsw, liqflgsw, ptr_kind
+      real(c_double), intent(in) :: play(*), plev(*), tlay(*), tlev(*), tsfc(*)
+      real(c_double), intent(in) :: h2ovmr(*), o3vmr(*), co2vmr(*), ch4vmr(*), n2ovmr(*), o2vmr(*)
+      real(c_double), intent(in) :: asdir(*), asdif(*), aldir(*), aldif(*), coszen(*)
+      real(c_double), value :: adjes, scon
+      real(c_double), intent(in) :: cldfr(*), taucld(*), ssacld(*), asmcld(*), fsfcld(*)
+      real(c_double), intent(in) :: cicewp(*), cliqwp(*), reice(*), reliq(*)
+      real(c_double), intent(in) :: tauaer(*), ssaaer(*), asmaer(*), ecaer(*)
+      real(c_double), intent(out) :: swuflx(*), swdflx(*), swhr(*), swuflxc(*), swdflxc(*), swhrc(*)
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function c_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
                                        h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
                                        cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, &
@@ -258,13 +277,24 @@ contains
     real(kind=rb), intent(in) :: taucld(:,:,:), ssacld(:,:,:), asmcld(:,:,:), fsfcld(:,:,:)
     real(kind=rb), intent(in) :: tauaer(:,:,:), ssaaer(:,:,:), asmaer(:,:,:), ecaer(:,:,:)
     real(kind=rb), intent(out) :: swuflx(:,:), swdflx(:,:), swhr(:,:), swuflxc(:,:), swdflxc(:,:), swhrc(:,:)
-    ! flag normalisation as in rrtmg_sw_rad.nomcica.f90:468,477
+    ! flag normalisation as in rrtmg_sw_rad.nomcica.f90:471,479; icld = 0 is
+    ! clear sky, icld >= 1 with clouds (inflgsw = 0: the clouds' optics)
     if (icld < 0 .or. icld > 3) icld = 2
     if (iaer /= 0 .and. iaer /= 6 .and. iaer /= 10) iaer = 0
-    call check(c_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
-                    h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
-                    asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
-                    swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_sw')
+    if (icld == 0) then
+      call check(c_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                      h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                      asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                      swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, RRTMG_HIP_HOST, c_null_ptr), 'rrtmg_sw')
+    else
+      call check(c_sw_cloud(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                            h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, &
+                            asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                            inflgsw, iceflgsw, liqflgsw, cldfr, taucld, ssacld, asmcld, fsfcld, &
+                            cicewp, cliqwp, reice, reliq, tauaer, ssaaer, asmaer, ecaer, &
+                            swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc, RRTMG_HIP_HOST, c_null_ptr), &
+                 'rrtmg_sw')
+    end if
   end subroutine rrtmg_sw
 end module rrtmg_sw_rad
 

@@ -9,13 +9,17 @@
 !          (ncol, nlay) or (ncol, nlay+1): play, plev, tlay, tlev; h2o, o3, co2;
 !          (ncol): tsfc, albedo, coszen; scalars solrad(adjes), solr_cnst(scon);
 !          then the cloud fields of a cloudy LW call, top-first: cldfr
-!          (ncol, nlay), taucld (nbndlw, ncol, nlay)
+!          (ncol, nlay), taucld (nbndlw, ncol, nlay); then those of a cloudy
+!          SW call: cldfr (ncol, nlay), taucld, ssacld, asmcld, fsfcld
+!          (nbndsw, ncol, nlay)
 !   output (stream): swuflx, swdflx, swhr, uflx, dflx, hr, surface-first; then
 !          the adjoint entry points' gradients for cotangents hr_bar = swhr_bar =
 !          1 and uflx_bar = dflx_bar = swuflx_bar = swdflx_bar = 0:
 !          LW tlay_bar, tlev_bar, h2ovmr_bar, o3vmr_bar, tsfc_bar; SW tlay_bar,
 !          h2ovmr_bar, o3vmr_bar; then the cloudy LW call's (icld = 2,
-!          inflglw = 0) uflx, dflx, hr, uflxc, dflxc, hrc
+!          inflglw = 0) uflx, dflx, hr, uflxc, dflxc, hrc; then the cloudy
+!          SW call's (icld = 2, inflgsw = 0) swuflx, swdflx, swhr, swuflxc,
+!          swdflxc, swhrc
 program shim_driver
   use parkind, only : im => kind_im, rb => kind_rb
   use parrrtm, only : nbndlw
@@ -37,6 +41,7 @@ program shim_driver
   real(kind=rb), allocatable :: zl(:,:), gtl(:,:), gtv(:,:), gq(:,:), go(:,:), gts(:), sgt(:,:), sgq(:,:), sgo(:,:)
   real(kind=rb), allocatable :: sf(:,:), sh(:,:), sq(:,:), so(:,:), tcopy(:)
   real(kind=rb), allocatable :: cfr(:,:), tcl(:,:,:)
+  real(kind=rb), allocatable :: scf(:,:), stau(:,:,:), sssa(:,:,:), sasm(:,:,:), sfsf(:,:,:)
   real(kind=rb) :: solrad, solr_cnst
   real(kind=rb), parameter :: cp_air = 287.04_rb / (2._rb / 7._rb)
 
@@ -50,6 +55,9 @@ program shim_driver
   read(u) pf, ph, tf, th, q, o3, co2, tsrf, albedo, cosz, solrad, solr_cnst
   allocate(cfr(ncol, nlay), tcl(nbndlw, ncol, nlay))
   read(u) cfr, tcl
+  allocate(scf(ncol, nlay), stau(nbndsw, ncol, nlay), sssa(nbndsw, ncol, nlay))
+  allocate(sasm(nbndsw, ncol, nlay), sfsf(nbndsw, ncol, nlay))
+  read(u) scf, stau, sssa, sasm, sfsf
   close(u)
 
   allocate(zeros(ncol, nlay), ones(ncol, nlay), emis(ncol, nbndlw))
@@ -128,5 +136,19 @@ program shim_driver
                 tauaer, &
                 uflx, dflx, hr, uflxc, dflxc, hrc)
   write(u) uflx, dflx, hr, uflxc, dflxc, hrc
+
+  ! SW with clouds: overcast layers, the clouds' optics (inflgsw = 0)
+  icld = 2
+  call rrtmg_sw(ncol, nlay, icld, iaer, &
+                pf(:, sk:1:-1), ph(:, sk+1:1:-1), tf(:, sk:1:-1), th(:, sk+1:1:-1), tsrf, &
+                q(:, sk:1:-1), o3(:, sk:1:-1), co2(:, sk:1:-1), zeros, zeros, zeros, &
+                albedo, albedo, albedo, albedo, &
+                cosz, solrad, dyofyr, solr_cnst, &
+                0, 0, 0, &
+                scf(:, sk:1:-1), stau(:, :, sk:1:-1), sssa(:, :, sk:1:-1), sasm(:, :, sk:1:-1), sfsf(:, :, sk:1:-1), &
+                zeros, zeros, 10*ones, 10*ones, &
+                zro_sw, zro_sw, zro_sw, zro_sw, &
+                swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  write(u) swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc
   close(u)
 end program shim_driver

@@ -1,14 +1,15 @@
 """The Fortran drop-in (mima_amd/fortran/rrtmg_hip_shim.f90), called the way
 MiMA's run_rrtmg calls RRTMG, gives bit-identical results to the direct C ABI
-path; so do its adjoint entry points (rrtmg_lw_adj, rrtmg_sw_adj) and rrtmg_lw
-with clouds (icld = 2, inflglw = 0)."""
+path; so do its adjoint entry points (rrtmg_lw_adj, rrtmg_sw_adj), rrtmg_lw
+with clouds (icld = 2, inflglw = 0) and rrtmg_sw with clouds (icld = 2,
+inflgsw = 0)."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from mima_amd.columns import make_clouds, make_columns
+from mima_amd.columns import make_clouds, make_columns, make_sw_clouds
 
 pytestmark = pytest.mark.gpu
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -24,6 +25,7 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     c = make_columns(32, 8, 40, seed=21)
     ncol, nlay = c["ncol"], c["nlay"]
     cf, tau = make_clouds(c, seed=22)
+    scl = make_sw_clouds(c, seed=23)
     fin, fout = str(tmp_path / "in.bin"), str(tmp_path / "out.bin")
     with open(fin, "wb") as f:
         np.array([ncol, nlay], dtype=np.int32).tofile(f)
@@ -36,11 +38,14 @@ def test_fortran_shim_matches_c_abi(tmp_path):
         # Fortran cldfr(ncol, nlay) and taucld(nbndlw, ncol, nlay), top-first
         np.ascontiguousarray(cf[::-1]).tofile(f)
         np.ascontiguousarray(tau[::-1]).tofile(f)
+        # Fortran cldfr(ncol, nlay) and taucld, ssacld, asmcld, fsfcld(nbndsw, ncol, nlay), top-first
+        for a in scl:
+            np.ascontiguousarray(a[::-1]).tofile(f)
     r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     raw = np.fromfile(fout, dtype=np.float64)
     n2, n3 = nlay * ncol, (nlay + 1) * ncol
-    sizes = [n3, n3, n2] * 2 + [n2, n3, n2, n2, ncol, n2, n2, n2] + [n3, n3, n2] * 2
+    sizes = [n3, n3, n2] * 2 + [n2, n3, n2, n2, ncol, n2, n2, n2] + [n3, n3, n2] * 2 + [n3, n3, n2] * 2
     parts, o = [], 0
     for s in sizes:
         parts.append(raw[o:o + s])
@@ -63,3 +68,7 @@ def test_fortran_shim_matches_c_abi(tmp_path):
     for got, key in zip(parts[14:], ("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc")):
         np.testing.assert_array_equal(got.reshape(lc[key].shape), lc[key], err_msg="cloudy " + key)
     assert np.max(np.abs(lc["dflx"] - lc["dflxc"])) > 1.0
+    sc = rrtmg.sw_cloud_columns(c, 2, *scl)
+    for got, key in zip(parts[20:], ("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc")):
+        np.testing.assert_array_equal(got.reshape(sc[key].shape), sc[key], err_msg="cloudy " + key)
+    assert np.max(np.abs(sc["swdflx"] - sc["swdflxc"])) > 10.0
