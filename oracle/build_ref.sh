@@ -70,3 +70,19 @@ $FC -fdefault-real-8 -fPIC -shared -Wl,--no-undefined -o "$OUT/librrtmg_lw_ref.s
     "$OUT/obj_lw/ref_lw_driver.o" -L"$(dirname "$(command -v $FC)")/../lib" -lflang_rt.runtime 2>/dev/null || \
   $FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_lw_ref.so" $LOBJS "$OUT/obj_lw/ref_lw_driver.o"
 echo "built $OUT/librrtmg_lw_ref.so"
+
+# McICA: its driver rrtmg_lw_rad.f90 defines the module rrtmg_lw_rad as the
+# nomcica driver does, so it gets a module directory and library of its own
+# (the LW modules, setcoef and taumol objects are shared)
+mkdir -p "$OUT/mod_lwm" "$OUT/obj_lwm"
+MFLAGS="-fdefault-real-8 -fdefault-double-8 -fPIC -O2 -module-dir $OUT/mod_lwm -I$OUT/mod_lwm -I$OUT/mod_lw -I$HERE"
+MOBJS=""
+for m in $LMODS; do MOBJS="$MOBJS $OUT/obj_lw/$m.o"; done
+MOBJS="$MOBJS $OUT/obj_lw/rrtmg_lw_setcoef.o $OUT/obj_lw/rrtmg_lw_taumol.o"
+for s in mcica_random_numbers mcica_subcol_gen_lw rrtmg_lw_cldprmc rrtmg_lw_rtrnmc rrtmg_lw_rad; do
+  $FC $MFLAGS -c "$LW/src/$s.f90" -o "$OUT/obj_lwm/$s.o"
+  MOBJS="$MOBJS $OUT/obj_lwm/$s.o"
+done
+$FC $MFLAGS -c "$HERE/ref_lw_mcica_driver.f90" -o "$OUT/obj_lwm/ref_lw_mcica_driver.o"
+$FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_lw_mcica_ref.so" $MOBJS "$OUT/obj_lwm/ref_lw_mcica_driver.o"
+echo "built $OUT/librrtmg_lw_mcica_ref.so"

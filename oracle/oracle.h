@@ -128,6 +128,22 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
                           const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
                           double *uflxc, double *dflxc, double *hrc);
 
+/* McICA LW: the stochastic subcolumns of mcica_subcol_lw with the KISS
+ * generator (irng = 0; mcica_subcol_gen_lw.f90:48-520) and rrtmg_lw of the
+ * McICA driver on them (rrtmg_lw_rad.f90:80-572: cldprmc with inflglw = 0,
+ * rtrnmc).  Cloud inputs (ncol, nlay), tauc (16, ncol, nlay); subcolumn
+ * arrays (140, ncol, nlay).  0 on success. */
+int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                           const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                           double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl);
+int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                          const double *cfc22vmr, const double *ccl4vmr, const double *emis,
+                          const double *cldfmcl, const double *taucmcl, const double *tauaer, double *uflx,
+                          double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc);
+
 /* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...): copies up to
  * n values (Fortran order) into out and returns the element count, or -1. */
 long oracle_reduced(const char *name, double *out, long n);

@@ -14,6 +14,7 @@
 #include "oracle_internal.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -200,12 +201,33 @@ typedef struct {
 } lwcol_t;
 
 /* inatm (rrtmg_lw_rad.nomcica.f90:729-852), clear sky, iaer handled by caller */
+static void inatm_u(lwcol_t *c, int iplon, int ncol, int nlay, const double *play, const double *plev,
+                    const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                    const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                    const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                    const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                    const double *emis, int vmr);
+/* MiMA's nomcica inatm: specific humidity and O3 mass mixing ratio in */
 static void inatm(lwcol_t *c, int iplon, int ncol, int nlay, const double *play, const double *plev,
                   const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
                   const double *o3vmr, const double *co2vmr, const double *ch4vmr,
                   const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
                   const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
                   const double *emis)
+{
+    inatm_u(c, iplon, ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+            cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, 0);
+}
+
+/* vmr = 0: MiMA's units, converted as its nomcica inatm does
+ * (rrtmg_lw_rad.nomcica.f90:775-778); vmr = 1: volume mixing ratios as the
+ * McICA driver's inatm takes them (rrtmg_lw_rad.f90:778, 784) */
+static void inatm_u(lwcol_t *c, int iplon, int ncol, int nlay, const double *play, const double *plev,
+                    const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                    const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                    const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                    const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                    const double *emis, int vmr)
 {
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
 #define IN(a, i, k) a[(i) + (long)ncol * ((k)-1)]
@@ -223,9 +245,9 @@ static void inatm(lwcol_t *c, int iplon, int ncol, int nlay, const double *play,
         c->tavel[l] = IN(tlay, iplon, l);
         c->pz[l] = IN(plev, iplon, l + 1);
         c->tz[l] = IN(tlev, iplon, l + 1);
-        c->wkl[1][l] = (IN(h2ovmr, iplon, l) / (1.0 - IN(h2ovmr, iplon, l))) * amdw;
+        c->wkl[1][l] = vmr ? IN(h2ovmr, iplon, l) : (IN(h2ovmr, iplon, l) / (1.0 - IN(h2ovmr, iplon, l))) * amdw;
         c->wkl[2][l] = IN(co2vmr, iplon, l);
-        c->wkl[3][l] = IN(o3vmr, iplon, l) * amdo;
+        c->wkl[3][l] = vmr ? IN(o3vmr, iplon, l) : IN(o3vmr, iplon, l) * amdo;
         c->wkl[4][l] = IN(n2ovmr, iplon, l);
         c->wkl[6][l] = IN(ch4vmr, iplon, l);
         c->wkl[7][l] = IN(o2vmr, iplon, l);
@@ -1609,6 +1631,304 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
         free(c);
         free(tauc);
         free(taucloud);
+        free(taua);
+    }
+    return err;
+}
+
+/* ---- McICA (rrtmg_lw_rad.f90 with mcica_subcol_gen_lw.f90) ----------------- */
+
+/* One step of the KISS generator (mcica_subcol_gen_lw.f90:530-562): a
+ * congruential, a 3-shift register and two multiply-with-carry generators
+ * in 32-bit wrap-around arithmetic (the reference's default integers),
+ * their sum read as a signed integer and mapped to (0, 1). */
+static double kiss_step(uint32_t s[4])
+{
+    s[0] = 69069u * s[0] + 1327217885u;
+    uint32_t k = s[1];
+    k ^= k << 13;
+    k ^= k >> 17;
+    k ^= k << 5;
+    s[1] = k;
+    s[2] = 18000u * (s[2] & 65535u) + (s[2] >> 16);
+    s[3] = 30903u * (s[3] & 65535u) + (s[3] >> 16);
+    uint32_t sum = s[0] + s[1] + (s[2] << 16) + s[3];
+    return (double)(int32_t)sum * 2.328306e-10 + 0.5;
+}
+
+/* mcica_subcol_lw with the KISS generator (irng = 0;
+ * mcica_subcol_gen_lw.f90:48-152, generate_stochastic_clouds :158-520): per
+ * column, seeds from the fractional Pa of the four lowest layers, advanced
+ * permuteseed times; one random number per (subcolumn, layer) for random
+ * (icld = 1) and maximum-random (2) overlap, the latter then carried up
+ * through cloudy layers; one per subcolumn for maximum overlap (3).  The
+ * subcolumn of g-point ig is cloudy where its number >= 1 - cldfrac, and
+ * then takes the band's in-cloud depth and the layer's water paths.
+ * Arrays: cldfrac, ciwp, clwp (ncol, nlay); tauc (16, ncol, nlay);
+ * outputs (140, ncol, nlay).  -6 for a column whose layer pressures do not
+ * fall from the surface (the reference stops), -7 for nlay < 4. */
+int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                           const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                           double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl)
+{
+    if (!L.ready) return -1;
+    if (nlay < 4 || nlay > MAXLAY) return -7;
+    if (icld < 1 || icld > 3) return -2;
+    for (int i = 0; i < ncol; ++i)
+        if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
+    int band_of[NGLW];
+    for (int b = 0, ig = 0; b < 16; ++b)
+        for (; ig < L.ngs[b]; ++ig) band_of[ig] = b;
+    static __thread double cdf[NGLW][MAXLAY + 1];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ncol; ++i) {
+        uint32_t s[4];
+        for (int k = 0; k < 4; ++k) {
+            double p = play[i + (long)ncol * k] * 1.e2;
+            s[k] = (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
+        }
+        for (int n = 0; n < permuteseed; ++n) (void)kiss_step(s);
+        double cldf[MAXLAY + 1];
+        for (int l = 0; l < nlay; ++l) {
+            double f = cldfrac[i + (long)ncol * l];
+            cldf[l] = f < 1.e-20 ? 0.0 : f;
+        }
+        for (int ig = 0; ig < NGLW; ++ig) {
+            if (icld == 3) {
+                double r = kiss_step(s);
+                for (int l = 0; l < nlay; ++l) cdf[ig][l] = r;
+            } else {
+                for (int l = 0; l < nlay; ++l) cdf[ig][l] = kiss_step(s);
+            }
+        }
+        if (icld == 2)
+            for (int l = 1; l < nlay; ++l)
+                for (int ig = 0; ig < NGLW; ++ig) {
+                    if (cdf[ig][l - 1] > 1.0 - cldf[l - 1])
+                        cdf[ig][l] = cdf[ig][l - 1];
+                    else
+                        cdf[ig][l] = cdf[ig][l] * (1.0 - cldf[l - 1]);
+                }
+        for (int l = 0; l < nlay; ++l) {
+            long ix = i + (long)ncol * l;
+            for (int ig = 0; ig < NGLW; ++ig) {
+                long o = ig + (long)NGLW * ix;
+                int cloudy = cdf[ig][l] >= 1.0 - cldf[l];
+                cldfmcl[o] = cloudy ? 1.0 : 0.0;
+                clwpmcl[o] = cloudy ? clwp[ix] : 0.0;
+                ciwpmcl[o] = cloudy ? ciwp[ix] : 0.0;
+                taucmcl[o] = cloudy ? tauc[band_of[ig] + 16 * ix] : 0.0;
+            }
+        }
+    }
+    return 0;
+}
+
+/* rtrnmc (rrtmg_lw_rtrnmc.f90:225-579; iout = 0, idrv = 0): rtrn's random
+ * overlap with a cloud fraction of 0 or 1 per (g-point, layer); a layer with
+ * any cloudy g-point takes the cloudy branch for all of them. */
+static void rtrnmc(const lwcol_t *c, double (*cldfmc)[NGLW + 1], double (*taucmc)[NGLW + 1],
+                   double (*taua)[17], double *totuflux, double *totdflux, double *htr, double *totuclfl,
+                   double *totdclfl, double *htrc)
+{
+    const int nl = c->nlayers;
+    const double wtdiff = 0.5, rec_6 = 0.166667;
+    const double pi = 2.0 * asin(1.0);
+    const double fluxfac = pi * 2.e4;
+    double secdiff[17];
+    static __thread double efclfrac[MAXLAY + 2][NGLW + 1], odcld[MAXLAY + 2][NGLW + 1];
+    double urad[MAXLAY + 2], drad[MAXLAY + 2], clrurad[MAXLAY + 2], clrdrad[MAXLAY + 2];
+    double atrans[MAXLAY + 2], atot[MAXLAY + 2], bbugas[MAXLAY + 2], bbutot[MAXLAY + 2];
+    double fnet[MAXLAY + 2], fnetc[MAXLAY + 2];
+    int icldlyr[MAXLAY + 2], ngb[NGLW + 1];
+    for (int b = 1, ig = 1; b <= 16; ++b)
+        for (; ig <= L.ngs[b - 1]; ++ig) ngb[ig] = b;
+    for (int ibnd = 1; ibnd <= 16; ++ibnd) {
+        if (ibnd == 1 || ibnd == 4 || ibnd >= 10) {
+            secdiff[ibnd] = 1.66;
+        } else {
+            secdiff[ibnd] = L.a0[ibnd - 1] + L.a1[ibnd - 1] * exp(L.a2[ibnd - 1] * c->pwvcm);
+            if (secdiff[ibnd] > 1.80) secdiff[ibnd] = 1.80;
+            if (secdiff[ibnd] < 1.50) secdiff[ibnd] = 1.50;
+        }
+    }
+    for (int lev = 0; lev <= nl; ++lev) {
+        urad[lev] = drad[lev] = clrurad[lev] = clrdrad[lev] = 0.0;
+        totuflux[lev] = totdflux[lev] = totuclfl[lev] = totdclfl[lev] = 0.0;
+    }
+    for (int lay = 1; lay <= nl; ++lay) {
+        icldlyr[lay] = 0;
+        for (int ig = 1; ig <= NGLW; ++ig) {
+            if (cldfmc[lay][ig] == 1.0) {
+                odcld[lay][ig] = secdiff[ngb[ig]] * taucmc[lay][ig];
+                double transcld = exp(-odcld[lay][ig]);
+                efclfrac[lay][ig] = (1.0 - transcld) * cldfmc[lay][ig];
+                icldlyr[lay] = 1;
+            } else {
+                odcld[lay][ig] = 0.0;
+                efclfrac[lay][ig] = 0.0;
+            }
+        }
+    }
+    int igc = 1;
+    for (int iband = 1; iband <= 16; ++iband) {
+        for (; igc <= L.ngs[iband - 1]; ++igc) {
+            double radld = 0.0, radclrd = 0.0;
+            int iclddn = 0;
+            for (int lev = nl; lev >= 1; --lev) {
+                double plfrac = c->fracs[lev][igc];
+                double blay = c->planklay[lev][iband];
+                double dplankup = c->planklev[lev][iband] - blay;
+                double dplankdn = c->planklev[lev - 1][iband] - blay;
+                double taut = c->taug[lev][igc] + (taua ? taua[lev][iband] : 0.0);
+                double odepth = secdiff[iband] * taut;
+                if (odepth < 0.0) odepth = 0.0;
+                double bbd;
+                if (icldlyr[lev] == 1) {
+                    iclddn = 1;
+                    cldlay_t r;
+                    cloudy_layer(odepth, odcld[lev][igc], plfrac, blay, dplankup, dplankdn, &r);
+                    atrans[lev] = r.atrans;
+                    atot[lev] = r.atot;
+                    bbugas[lev] = r.bbugas;
+                    bbutot[lev] = r.bbutot;
+                    bbd = r.bbd;
+                    radld = radld - radld * (r.atrans + efclfrac[lev][igc] * (1.0 - r.atrans)) + r.gassrc +
+                            cldfmc[lev][igc] * (r.bbdtot * r.atot - r.gassrc);
+                } else {
+                    if (odepth <= 0.06) {
+                        atrans[lev] = odepth - 0.5 * odepth * odepth;
+                        odepth = rec_6 * odepth;
+                        bbd = plfrac * (blay + dplankdn * odepth);
+                        bbugas[lev] = plfrac * (blay + dplankup * odepth);
+                    } else {
+                        double tblind = odepth / (L.bpade + odepth);
+                        int itr = (int)(1.e4 * tblind + 0.5);
+                        atrans[lev] = 1.0 - L.exp_tbl[itr];
+                        double tausfac = L.tfn_tbl[itr];
+                        bbd = plfrac * (blay + tausfac * dplankdn);
+                        bbugas[lev] = plfrac * (blay + tausfac * dplankup);
+                    }
+                    radld = radld + (bbd - radld) * atrans[lev];
+                }
+                drad[lev - 1] = drad[lev - 1] + radld;
+                if (iclddn == 1) {
+                    radclrd = radclrd + (bbd - radclrd) * atrans[lev];
+                    clrdrad[lev - 1] = clrdrad[lev - 1] + radclrd;
+                } else {
+                    radclrd = radld;
+                    clrdrad[lev - 1] = drad[lev - 1];
+                }
+            }
+            double rad0 = c->fracs[1][igc] * c->plankbnd[iband];
+            double reflect = 1.0 - c->semiss[iband];
+            double radlu = rad0 + reflect * radld;
+            double radclru = rad0 + reflect * radclrd;
+            urad[0] = urad[0] + radlu;
+            clrurad[0] = clrurad[0] + radclru;
+            for (int lev = 1; lev <= nl; ++lev) {
+                if (icldlyr[lev] == 1) {
+                    double gassrc = bbugas[lev] * atrans[lev];
+                    radlu = radlu - radlu * (atrans[lev] + efclfrac[lev][igc] * (1.0 - atrans[lev])) + gassrc +
+                            cldfmc[lev][igc] * (bbutot[lev] * atot[lev] - gassrc);
+                } else {
+                    radlu = radlu + (bbugas[lev] - radlu) * atrans[lev];
+                }
+                urad[lev] = urad[lev] + radlu;
+                if (iclddn == 1) {
+                    radclru = radclru + (bbugas[lev] - radclru) * atrans[lev];
+                    clrurad[lev] = clrurad[lev] + radclru;
+                } else {
+                    radclru = radlu;
+                    clrurad[lev] = urad[lev];
+                }
+            }
+        }
+        for (int lev = nl; lev >= 0; --lev) {
+            double dw = L.delwave[iband - 1];
+            totuflux[lev] = totuflux[lev] + urad[lev] * wtdiff * dw;
+            totdflux[lev] = totdflux[lev] + drad[lev] * wtdiff * dw;
+            totuclfl[lev] = totuclfl[lev] + clrurad[lev] * wtdiff * dw;
+            totdclfl[lev] = totdclfl[lev] + clrdrad[lev] * wtdiff * dw;
+            urad[lev] = drad[lev] = clrurad[lev] = clrdrad[lev] = 0.0;
+        }
+    }
+    for (int lev = 0; lev <= nl; ++lev) {
+        totuflux[lev] = totuflux[lev] * fluxfac;
+        totdflux[lev] = totdflux[lev] * fluxfac;
+        fnet[lev] = totuflux[lev] - totdflux[lev];
+        totuclfl[lev] = totuclfl[lev] * fluxfac;
+        totdclfl[lev] = totdclfl[lev] * fluxfac;
+        fnetc[lev] = totuclfl[lev] - totdclfl[lev];
+    }
+    for (int lev = 1; lev <= nl; ++lev) {
+        int l = lev - 1;
+        htr[l] = L.heatfac * (fnet[l] - fnet[lev]) / (c->pz[l] - c->pz[lev]);
+        htrc[l] = L.heatfac * (fnetc[l] - fnetc[lev]) / (c->pz[l] - c->pz[lev]);
+    }
+    htr[nl] = 0.0;
+    htrc[nl] = 0.0;
+}
+
+/* rrtmg_lw of the McICA driver (rrtmg_lw_rad.f90:80-572) on given
+ * subcolumns, icld 1..3, inflglw = 0 (cldprmc leaves taucmc as given),
+ * idrv = 0; h2ovmr and o3vmr are volume mixing ratios, as that driver's
+ * inatm takes them (it lacks MiMA's conversions).  cldfmcl, taucmcl (140, ncol, nlay); tauaer (ncol, nlay, 16) or
+ * NULL. */
+int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                          const double *cfc22vmr, const double *ccl4vmr, const double *emis,
+                          const double *cldfmcl, const double *taucmcl, const double *tauaer, double *uflx,
+                          double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc)
+{
+    if (!L.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    int err = 0;
+#pragma omp parallel
+    {
+        lwcol_t *c = (lwcol_t *)malloc(sizeof(lwcol_t));
+        double (*cf)[NGLW + 1] = malloc(sizeof(double[MAXLAY + 2][NGLW + 1]));
+        double (*tc)[NGLW + 1] = malloc(sizeof(double[MAXLAY + 2][NGLW + 1]));
+        double (*taua)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
+        double tu[MAXLAY + 2], td[MAXLAY + 2], tuc[MAXLAY + 2], tdc[MAXLAY + 2], htr[MAXLAY + 2],
+            htrc[MAXLAY + 2];
+        if (!c || !cf || !tc || !taua) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (err) continue;
+            inatm_u(c, iplon, ncol, nlay, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                    o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, 1);
+            for (int l = 1; l <= nlay; ++l) {
+                long ix = iplon + (long)ncol * (l - 1);
+                for (int ig = 1; ig <= NGLW; ++ig) {
+                    cf[l][ig] = cldfmcl[(ig - 1) + (long)NGLW * ix];
+                    tc[l][ig] = taucmcl[(ig - 1) + (long)NGLW * ix];
+                }
+                for (int ib = 1; ib <= 16; ++ib)
+                    taua[l][ib] = tauaer ? tauaer[iplon + (long)ncol * ((l - 1) + (long)nlay * (ib - 1))] : 0.0;
+            }
+            setcoef(c);
+            taumol(c);
+            rtrnmc(c, cf, tc, tauaer ? taua : NULL, tu, td, htr, tuc, tdc, htrc);
+            for (int k = 0; k <= nlay; ++k) {
+                uflx[iplon + (long)ncol * k] = tu[k];
+                dflx[iplon + (long)ncol * k] = td[k];
+                uflxc[iplon + (long)ncol * k] = tuc[k];
+                dflxc[iplon + (long)ncol * k] = tdc[k];
+            }
+            for (int k = 0; k < nlay; ++k) {
+                hr[iplon + (long)ncol * k] = htr[k];
+                hrc[iplon + (long)ncol * k] = htrc[k];
+            }
+        }
+        free(c);
+        free(cf);
+        free(tc);
         free(taua);
     }
     return err;

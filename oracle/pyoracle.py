@@ -121,6 +121,45 @@ def rrtmg_lw_cloud(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+def lib_ngs():
+    """Cumulative LW g-points per band (the reduced table's ngs)."""
+    from mima_amd.tables.extract_tables import read_table
+    tabs, _, _ = read_table(LW_TABLE)
+    return np.cumsum(np.asarray(tabs["lw/ngc"], dtype=np.int64))
+
+
+def lw_mcica_subcol(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permuteseed=0):
+    """mcica_subcol_lw with the KISS generator: cldfmcl, ciwpmcl, clwpmcl,
+    taucmcl as (nlay, ncol, 140)."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    z = np.zeros((nlay, ncol))
+    ins = [_f64(a) for a in (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp,
+                             taucld)]
+    outs = [np.zeros((nlay, ncol, 140)) for _ in range(4)]
+    rc = L.oracle_lw_mcica_subcol(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins],
+                                  *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_lw_mcica_subcol returned %d" % rc)
+    return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl"), outs))
+
+
+def rrtmg_lw_mcica(c, cldfmcl, taucmcl, tauaer=None):
+    """rrtmg_lw of the McICA driver on subcolumns (nlay, ncol, 140)."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr",
+                                "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr",
+                                "ccl4vmr", "emis")]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_lw_mcica(ncol, nlay, *[_p(a) for a in ins], _p(_f64(cldfmcl)), _p(_f64(taucmcl)),
+                                 None if tauaer is None else _p(_f64(tauaer)), *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_lw_mcica returned %d" % rc)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
 def _col(c, i, key):
     v = c[key]
     return np.ascontiguousarray(v[:, i], dtype=np.float64)

@@ -33,92 +33,7 @@ module ref_lw_driver
   implicit none
 contains
 
-  integer function put(dst, m, src, n)
-    integer, intent(in) :: m, n
-    real(kind=rb), intent(inout) :: dst(*)
-    real(c_double), intent(in) :: src(n)
-    if (m /= n) then
-      put = -2
-      return
-    end if
-    dst(1:n) = src(1:n)
-    put = 0
-  end function put
-
-  ! name: NUL-terminated reduced-array name as declared in rrlw_kgNN (e.g. "ka")
-  integer(c_int) function ref_lw_set(band, name, src, n) bind(C, name="ref_lw_set")
-    integer(c_int), value :: band, n
-    character(kind=c_char), intent(in) :: name(*)
-    real(c_double), intent(in) :: src(n)
-    character(len=16) :: nm
-    integer :: i
-    nm = ' '
-    do i = 1, 16
-      if (name(i) == c_null_char) exit
-      nm(i:i) = name(i)
-    end do
-    ref_lw_set = -1
-    select case (band)
-    include 'ref_lw_cases.inc'
-    end select
-  end function ref_lw_set
-
-  subroutine ref_lw_set_bands(ngc_in, nspa_in, nspb_in, delwave_in) bind(C, name="ref_lw_set_bands")
-    use rrlw_wvn, only : ng, nspa, nspb, delwave, ngc, ngs, ngb, ixindx, nxmol
-    integer(c_int), intent(in) :: ngc_in(nbndlw), nspa_in(nbndlw), nspb_in(nbndlw)
-    real(c_double), intent(in) :: delwave_in(nbndlw)
-    integer :: ib, ig, k
-    ng(:) = 16
-    nspa(:) = nspa_in
-    nspb(:) = nspb_in
-    delwave(:) = delwave_in
-    ngc(:) = ngc_in
-    k = 0
-    do ib = 1, nbndlw
-      do ig = 1, ngc(ib)
-        k = k + 1
-        ngb(k) = ib
-      end do
-      ngs(ib) = k
-    end do
-    nxmol = 4
-    ixindx(:) = 0
-    ixindx(1:4) = (/ 1, 2, 3, 4 /)
-  end subroutine ref_lw_set_bands
-
-  subroutine ref_lw_init(cpdair) bind(C, name="ref_lw_init")
-    use rrlw_con, only : heatfac, grav, avogad, secdy
-    use rrlw_tbl, only : ntbl, pade, bpade, tau_tbl, exp_tbl, tfn_tbl
-    real(c_double), value :: cpdair
-    real(kind=rb), parameter :: expeps = 1.e-20_rb
-    real(kind=rb) :: tfn
-    integer :: itr
-    grav = 9.8066_rb
-    avogad = 6.02214199e+23_rb
-    secdy = 8.6400e4_rb
-    heatfac = grav * secdy / (cpdair * 1.e2_rb)
-    call lwatmref
-    call lwavplank
-    call lwavplankderiv
-    tau_tbl(0) = 0.0_rb
-    tau_tbl(ntbl) = 1.e10_rb
-    exp_tbl(0) = 1.0_rb
-    exp_tbl(ntbl) = expeps
-    tfn_tbl(0) = 0.0_rb
-    tfn_tbl(ntbl) = 1.0_rb
-    bpade = 1.0_rb / pade
-    do itr = 1, ntbl - 1
-      tfn = real(itr, rb) / real(ntbl, rb)
-      tau_tbl(itr) = bpade * tfn / (1._rb - tfn)
-      exp_tbl(itr) = exp(-tau_tbl(itr))
-      if (exp_tbl(itr) .le. expeps) exp_tbl(itr) = expeps
-      if (tau_tbl(itr) .lt. 0.06_rb) then
-        tfn_tbl(itr) = tau_tbl(itr) / 6._rb
-      else
-        tfn_tbl(itr) = 1._rb - 2._rb * ((1._rb / tau_tbl(itr)) - (exp_tbl(itr) / (1._rb - exp_tbl(itr))))
-      end if
-    end do
-  end subroutine ref_lw_init
+  include 'ref_lw_common.inc'
 
   subroutine ref_lw_run(ncol, nlay, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, &
                         cfc11, cfc12, cfc22, ccl4, emis, tauaer, &

@@ -25,6 +25,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SW_LIB = os.path.join(HERE, "_ref", "librrtmg_sw_ref.so")
 LW_LIB = os.path.join(HERE, "_ref", "librrtmg_lw_ref.so")
+LW_MCICA_LIB = os.path.join(HERE, "_ref", "librrtmg_lw_mcica_ref.so")
 CPDAIR = 287.04 / (2.0 / 7.0)   # MiMA's cp_air (constants.F90: rdgas / kappa)
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
@@ -63,14 +64,15 @@ def run_sw(lib, c):
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
 
 
-def load_lw():
-    """Load the reference LW column code and fill its module state; returns
+def load_lw(path=LW_LIB):
+    """Load the reference LW column code (the nomcica build, or with
+    path=LW_MCICA_LIB the McICA one) and fill its module state; returns
     (lib, number of reduced tables set)."""
     sys.path.insert(0, HERE)
     sys.path.insert(0, ROOT)
     import pyoracle as po
     from mima_amd.tables.extract_tables import read_table
-    lib = ctypes.CDLL(LW_LIB)
+    lib = ctypes.CDLL(path)
     lib.ref_lw_set.restype = ctypes.c_int
     po.lib()  # oracle init: reduces the seeded tables
     tabs, _, _ = read_table(po.LW_TABLE)
@@ -138,3 +140,35 @@ def run_sw_cloud(lib, c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=Non
     lib.ref_sw_run_cloud(n, L, int(icld), *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
                          ctypes.c_double(c["scon"]), *[_p(a) for a in cl], *[_p(a) for a in outs])
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
+def run_lw_mcica_subcol(lib, c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permuteseed=0, irng=0):
+    """mcica_subcol_lw of the reference (McICA library): the stochastic
+    subcolumns of cldfr (nlay, ncol) and taucld (nlay, ncol, 16); returns
+    cldfmcl and taucmcl as (nlay, ncol, 140) (the reference's (ngptlw, ncol,
+    nlay)), and the water paths likewise."""
+    n, L = c["ncol"], c["nlay"]
+    z = np.zeros((L, n))
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in
+           (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp, 10.0 + z, 10.0 + z,
+            taucld)]
+    outs = [np.zeros((L, n, 140)), np.zeros((L, n, 140)), np.zeros((L, n, 140)), np.zeros((L, n)),
+            np.zeros((L, n)), np.zeros((L, n, 140))]
+    lib.ref_lw_mcica_subcol(n, L, int(icld), int(permuteseed), int(irng), *[_p(a) for a in ins],
+                            *[_p(a) for a in outs])
+    return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "reicmcl", "relqmcl", "taucmcl"), outs))
+
+
+def run_lw_mcica(lib, c, icld, sub, tauaer=None):
+    """rrtmg_lw of the reference's McICA driver on subcolumns `sub` (as
+    run_lw_mcica_subcol returns them)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in LW_IN]
+    tau = np.zeros((16, L, n)) if tauaer is None else np.ascontiguousarray(tauaer, dtype=np.float64)
+    sb = [np.ascontiguousarray(sub[k], dtype=np.float64) for k in
+          ("cldfmcl", "taucmcl", "ciwpmcl", "clwpmcl", "reicmcl", "relqmcl")]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_lw_mcica_run(n, L, int(icld), *[_p(a) for a in ins], *[_p(a) for a in sb], _p(tau),
+                         *[_p(a) for a in outs])
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
