@@ -24,7 +24,8 @@
  *     icld = idrv = iaer = 0, RR/rrtm_radiation.f90:106-108); icld != 0,
  *     SW iaer != 0 or LW idrv != 0 return RRTMG_HIP_ENOTSUP there.  Clouds
  *     given by their optical properties (inflglw = inflgsw = 0) take
- *     rrtmg_hip_lw_cloud and rrtmg_hip_sw_cloud.
+ *     rrtmg_hip_lw_cloud and rrtmg_hip_sw_cloud, and McICA subcolumns
+ *     rrtmg_hip_mcica_subcol_lw and rrtmg_hip_lw_mcica.
  */
 #ifndef RRTMG_HIP_H
 #define RRTMG_HIP_H
@@ -164,6 +165,59 @@ int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv,
                        const double *cldfr, const double *taucld,
                        const double *cicewp, const double *cliqwp,
                        const double *reice, const double *reliq, const double *tauaer,
+                       double *uflx, double *dflx, double *hr,
+                       double *uflxc, double *dflxc, double *hrc,
+                       int ptr_kind, void *stream);
+
+/* Replaces mcica_subcol_lw (rrtmg_lw/gcm_model/src/mcica_subcol_gen_lw.f90:48-154):
+ * the McICA stochastic cloud subcolumns, one per LW g-point (140), with the
+ * reference's whole argument list in its order.  irng = 0 (KISS generator,
+ * :323-358, 530-562) only; irng != 0 (Mersenne Twister) returns
+ * RRTMG_HIP_ENOTSUP.  Seeds come from the fractional Pa of the four lowest
+ * layers of play (ncol, nlay), advanced permuteseed times.  icld = 1 random,
+ * 2 maximum-random, 3 maximum overlap (:364-445); icld = 0 returns at once
+ * with the outputs untouched; icld outside 0..3 and nlay < 4 return
+ * RRTMG_HIP_EINVAL, and so does a column whose second layer has the higher
+ * pressure (the reference stops, :327-329; checked on the host for host
+ * pointers, on the device for device pointers with rrtmg_hip_set_checks on).
+ * cldfrac, ciwp, clwp, rei, rel (ncol, nlay); tauc (16, ncol, nlay), band
+ * fastest.  Outputs cldfmcl, ciwpmcl, clwpmcl, taucmcl (140, ncol, nlay),
+ * g-point fastest, and reicmcl, relqmcl (ncol, nlay) copies of rei, rel.
+ * ciwp, clwp, rei and rel may be NULL (read as zero) and the outputs other
+ * than cldfmcl and taucmcl NULL (not written).  iplon is not read. */
+int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
+                              const double *play, const double *cldfrac,
+                              const double *ciwp, const double *clwp,
+                              const double *rei, const double *rel, const double *tauc,
+                              double *cldfmcl, double *ciwpmcl, double *clwpmcl,
+                              double *reicmcl, double *relqmcl, double *taucmcl,
+                              int ptr_kind, void *stream);
+
+/* Replaces rrtmg_lw of the McICA driver (rrtmg_lw/gcm_model/src/rrtmg_lw_rad.f90:80-89):
+ * the reference's whole argument list in its order, on the subcolumns of
+ * rrtmg_hip_mcica_subcol_lw.  The transfer is rtrnmc (rrtmg_lw_rtrnmc.f90):
+ * random overlap per g-point with the subcolumn's cloud fraction, a layer
+ * taking the cloudy branch for every g-point where any g-point's cldfmcl is
+ * 1.  inflglw = 0 only (cldprmc, rrtmg_lw_cldprmc.f90:163-168): taucmcl
+ * holds the subcolumns' in-cloud optical depths.  cldfmcl, taucmcl (140,
+ * ncol, nlay), g-point fastest; ciwpmcl, clwpmcl, reicmcl, relqmcl,
+ * iceflglw and liqflglw are not read.  Unlike the other LW entry points,
+ * h2ovmr and o3vmr are volume mixing ratios: this driver's inatm takes them
+ * as given (rrtmg_lw_rad.f90:778, 784).  icld outside 0..3 is taken as 2
+ * (:451); icld = 0 leaves the subcolumns at zero (:743-744, 874) and runs
+ * the clear-sky kernels.  inflglw != 0, idrv != 0 and
+ * RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc,
+                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                       const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *cfc11vmr, const double *cfc12vmr,
+                       const double *cfc22vmr, const double *ccl4vmr,
+                       const double *emis, int inflglw, int iceflglw, int liqflglw,
+                       const double *cldfmcl, const double *taucmcl,
+                       const double *ciwpmcl, const double *clwpmcl,
+                       const double *reicmcl, const double *relqmcl, const double *tauaer,
                        double *uflx, double *dflx, double *hr,
                        double *uflxc, double *dflxc, double *hrc,
                        int ptr_kind, void *stream);

@@ -705,16 +705,18 @@ int rrtmg_hip_sw(int ncol, int nlay, int icld, int iaer, const double *play, con
                       swdflxc, swhrc, ptr_kind, stream, nullptr);
 }
 
-// rrtmg_hip_lw and rrtmg_hip_lw_cloud: cl NULL or cl->icld == 0 runs the
-// clear-sky kernels, otherwise the cloud kernels (the cloud arrays already
-// checked by the caller)
+// rrtmg_hip_lw, rrtmg_hip_lw_cloud and rrtmg_hip_lw_mcica: cl NULL or
+// cl->icld == 0 runs the clear-sky kernels, otherwise the cloud kernels (the
+// cloud arrays already checked by the caller).  vmr: water vapour and ozone
+// arrive as volume mixing ratios (the McICA driver's inatm) rather than as
+// MiMA's mass mixing ratios.
 static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev,
                       const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
                       const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
                       const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
                       const double *cfc22vmr, const double *ccl4vmr, const double *emis, const double *tauaer,
                       const LwClouds *cl, double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc,
-                      double *hrc, int ptr_kind, void *stream)
+                      double *hrc, int ptr_kind, void *stream, bool vmr = false)
 {
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
@@ -756,7 +758,10 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         a.push_back({"tsfc", tsfc, (size_t)ncol});
         a.push_back({"emis", emis, (size_t)ncol * 16});
         if (tauaer) a.push_back({"tauaer", tauaer, n2 * 16});
-        if (cloudy) {
+        if (cloudy && cl->icld == LW_MCICA) {
+            a.push_back({"cldfmcl", cl->cldfmc, n2 * 140});
+            a.push_back({"taucmcl", cl->taucmc, n2 * 140});
+        } else if (cloudy) {
             a.push_back({"cldfr", cl->cldfr, n2});
             a.push_back({"taucld", cl->taucld, n2 * 16});
             if (cl->cicewp) a.push_back({"cicewp", cl->cicewp, n2});
@@ -817,9 +822,12 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     ColumnInputs in{};
     in.ncol = ncol;
     in.nlay = nlay;
+    in.vmr = vmr ? 1 : 0;
     LwOutputs out{};
     if (ptr_kind == RRTMG_HIP_HOST) {
-        const size_t ncl = cloudy ? 17 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) : 0;
+        const size_t ncl = !cloudy ? 0
+                           : cl->icld == LW_MCICA ? 280 * n2
+                                                  : 17 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0);
         const size_t need = sizeof(double) * (12 * n2 + 2 * n3 + (size_t)ncol * 17 + (tauaer ? 16 * n2 : 0) +
                                               ncl + 4 * n3 + 2 * n2);
         if ((rc = W.lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
@@ -849,9 +857,12 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
             p += n2 * 16;
         }
         if (cloudy) {
-            const double *src[] = {cl->cldfr, cl->taucld, cl->cicewp, cl->cliqwp};
-            const double **dst[] = {&cld.cldfr, &cld.taucld, &cld.cicewp, &cld.cliqwp};
-            const size_t len[] = {n2, 16 * n2, n2, n2};
+            const bool mc = cl->icld == LW_MCICA;
+            const double *src[] = {mc ? cl->cldfmc : cl->cldfr, mc ? cl->taucmc : cl->taucld,
+                                   mc ? nullptr : cl->cicewp, mc ? nullptr : cl->cliqwp};
+            const double **dst[] = {mc ? &cld.cldfmc : &cld.cldfr, mc ? &cld.taucmc : &cld.taucld, &cld.cicewp,
+                                    &cld.cliqwp};
+            const size_t len[] = {mc ? 140 * n2 : n2, mc ? 140 * n2 : 16 * n2, n2, n2};
             for (int i = 0; i < 4; ++i) {
                 if (!src[i]) continue;
                 ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
@@ -1012,6 +1023,148 @@ int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv, const double *pla
     return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                       o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &cl, uflx, dflx, hr, uflxc, dflxc,
                       hrc, ptr_kind, stream);
+}
+
+int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev,
+                       const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                       const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                       const double *cfc22vmr, const double *ccl4vmr, const double *emis, int inflglw,
+                       int iceflglw, int liqflglw, const double *cldfmcl, const double *taucmcl,
+                       const double *ciwpmcl, const double *clwpmcl, const double *reicmcl,
+                       const double *relqmcl, const double *tauaer, double *uflx, double *dflx, double *hr,
+                       double *uflxc, double *dflxc, double *hrc, int ptr_kind, void *stream)
+{
+    (void)iceflglw;
+    (void)liqflglw;
+    (void)ciwpmcl;
+    (void)clwpmcl;
+    (void)reicmcl;
+    (void)relqmcl;
+    // rrtmg_lw_rad.f90:451: icld outside 0..3 is taken as 2
+    if (icld < 0 || icld > 3) icld = 2;
+    // icld = 0: inatm leaves the subcolumns at zero (rrtmg_lw_rad.f90:743-744, 874)
+    if (icld == 0)
+        return lw_forward(ncol, nlay, 0, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                          n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, nullptr, uflx, dflx,
+                          hr, uflxc, dflxc, hrc, ptr_kind, stream, true);
+    if (inflglw != 0)
+        return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0: cloud optics from water paths and effective radii are not "
+                                       "implemented; pass the subcolumns' in-cloud optical depths with inflglw = 0");
+    if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
+        const size_t n = (size_t)ncol * nlay * 140;
+        int rc;
+        if (ptr_kind == RRTMG_HIP_HOST) {
+            if ((rc = check_host_array(cldfmcl, n, "cldfmcl")) || (rc = check_host_array(taucmcl, n, "taucmcl")))
+                return rc;
+        } else if (!cldfmcl || !taucmcl) {
+            return fail(RRTMG_HIP_EINVAL, "null cloud array");
+        }
+    }
+    LwClouds cl{};
+    cl.icld = LW_MCICA;
+    cl.cldfmc = cldfmcl;
+    cl.taucmc = taucmcl;
+    return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &cl, uflx, dflx, hr, uflxc, dflxc,
+                      hrc, ptr_kind, stream, true);
+}
+
+int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
+                              const double *play, const double *cldfrac, const double *ciwp, const double *clwp,
+                              const double *rei, const double *rel, const double *tauc, double *cldfmcl,
+                              double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
+                              double *taucmcl, int ptr_kind, void *stream)
+{
+    (void)iplon;
+    // mcica_subcol_gen_lw.f90:119-123: clear sky returns at once, icld
+    // outside 0..3 stops
+    if (icld == 0) return 0;
+    if (icld < 0 || icld > 3) return fail(RRTMG_HIP_EINVAL, "invalid icld (0..3)");
+    if (irng != 0)
+        return fail(RRTMG_HIP_ENOTSUP, "irng != 0: the Mersenne Twister generator is not implemented; use irng = 0 "
+                                       "(KISS)");
+    if (ncol <= 0) return fail(RRTMG_HIP_EINVAL, "ncol must be positive");
+    // the KISS seeds read the four lowest layers (mcica_subcol_gen_lw.f90:330-333)
+    if (nlay < 4 || nlay > RRTMG_HIP_MAX_NLAY) return fail(RRTMG_HIP_EINVAL, "nlay out of range [4, 200]");
+    if (ptr_kind != RRTMG_HIP_HOST && ptr_kind != RRTMG_HIP_DEVICE) return fail(RRTMG_HIP_EINVAL, "bad ptr_kind");
+    if (!play || !cldfrac || !tauc || !cldfmcl || !taucmcl) return fail(RRTMG_HIP_EINVAL, "null array argument");
+    DeviceState *s = state();
+    if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!s->lwd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_mcica_subcol_lw called before rrtmg_hip_lw_init");
+    const size_t n2 = (size_t)ncol * nlay, ns = n2 * 140;
+    const double *ins[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc};
+    const char *nm[] = {"play", "cldfrac", "ciwp", "clwp", "rei", "rel", "tauc"};
+    const size_t len[] = {n2, n2, n2, n2, n2, n2, 16 * n2};
+    double *outs[] = {cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl};
+    const size_t olen[] = {ns, ns, ns, n2, n2, ns};
+    int rc;
+    hipStream_t st = (hipStream_t)stream;
+    StreamWork &W = s->work(st);
+    if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
+    McicaSubcolArgs a{};
+    a.ncol = ncol;
+    a.nlay = nlay;
+    a.icld = icld;
+    a.permuteseed = permuteseed;
+    int ck = 0;
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        for (int i = 0; i < 7; ++i)
+            if (ins[i] && (rc = check_host_array(ins[i], len[i], nm[i]))) return rc;
+        for (int i = 0; i < ncol; ++i)
+            if (play[i] * 1.e2 < play[(size_t)ncol + i] * 1.e2)
+                return fail(RRTMG_HIP_EINVAL, "play must not rise from the first layer to the second (the KISS "
+                                              "seeds read the four lowest layers)");
+        size_t need = 0;
+        for (int i = 0; i < 7; ++i) need += ins[i] ? len[i] : 0;
+        for (int i = 0; i < 6; ++i) need += outs[i] ? olen[i] : 0;
+        if ((rc = W.lw_stage.ensure(need * sizeof(double)))) return fail(rc, "device staging allocation failed");
+        double *p = (double *)W.lw_stage.p;
+        const double **din[] = {&a.play, &a.cldfrac, &a.ciwp, &a.clwp, &a.rei, &a.rel, &a.tauc};
+        for (int i = 0; i < 7; ++i) {
+            if (!ins[i]) continue;
+            ck |= hipMemcpyAsync(p, ins[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
+            *din[i] = p;
+            p += len[i];
+        }
+        double **dout[] = {&a.cldfmcl, &a.ciwpmcl, &a.clwpmcl, &a.reicmcl, &a.relqmcl, &a.taucmcl};
+        for (int i = 0; i < 6; ++i) {
+            if (!outs[i]) continue;
+            *dout[i] = p;
+            p += olen[i];
+        }
+    } else {
+        if (g_checks) {
+            std::vector<NamedArray> v;
+            for (int i = 0; i < 7; ++i)
+                if (ins[i]) v.push_back({nm[i], ins[i], len[i]});
+            if ((rc = device_checks(W, st, v, nullptr, ncol, nlay))) return rc;
+            a.bad = (int *)W.flag.p + 16;
+        }
+        a.play = play; a.cldfrac = cldfrac; a.ciwp = ciwp; a.clwp = clwp; a.rei = rei; a.rel = rel; a.tauc = tauc;
+        a.cldfmcl = cldfmcl; a.ciwpmcl = ciwpmcl; a.clwpmcl = clwpmcl; a.reicmcl = reicmcl; a.relqmcl = relqmcl;
+        a.taucmcl = taucmcl;
+    }
+    if (ck) return fail(RRTMG_HIP_EDEVICE, "McICA subcolumns: host-to-device copy failed");
+    hipError_t e = launch_lw_mcica_subcol(a, (const LwBand *)s->lwd.bands, st);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumn launch: ") + hipGetErrorString(e));
+    if (ptr_kind == RRTMG_HIP_HOST) {
+        const double *src[] = {a.cldfmcl, a.ciwpmcl, a.clwpmcl, a.reicmcl, a.relqmcl, a.taucmcl};
+        for (int i = 0; i < 6; ++i)
+            if (outs[i]) ck |= hipMemcpyAsync(outs[i], src[i], olen[i] * sizeof(double), hipMemcpyDeviceToHost, st);
+        e = hipStreamSynchronize(st);
+        if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumns: ") + hipGetErrorString(e));
+    } else if (a.bad) {
+        int nbad = 0;
+        e = hipMemcpyAsync(&nbad, a.bad, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumns: ") + hipGetErrorString(e));
+        if (nbad)
+            return fail(RRTMG_HIP_EINVAL, "play must not rise from the first layer to the second (the KISS seeds "
+                                          "read the four lowest layers); those columns are left unwritten");
+    }
+    return 0;
 }
 
 // backward of rrtmg_sw; with swuflx/swdflx/swhr given, also the forward's

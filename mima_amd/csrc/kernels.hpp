@@ -17,6 +17,9 @@ struct ColumnInputs {
     double adjflux;  // adjes (or earth_sun) * scon / 1368.22
     // LW only
     const double *cfc11, *cfc12, *cfc22, *ccl4, *emis, *tauaer;
+    // 1: h2o and o3 are volume mixing ratios (the McICA LW driver's inatm);
+    // 0: specific humidity and O3 mass mixing ratio (MiMA's nomcica drivers)
+    int vmr;
 };
 
 struct SwOutputs {
@@ -104,11 +107,22 @@ struct LwDevTables {
 // ---------------------------------------------------------------------------
 // LW with clouds (rrtmg_lw, icld = 1..3, inflglw = 0), device FP64 inputs
 struct LwClouds {
-    int icld;                         // 1 random overlap (rtrn), 2 and 3 maximum-random (rtrnmr)
+    int icld;                         // 1 random overlap (rtrn), 2 and 3 maximum-random (rtrnmr), 4 McICA (rtrnmc)
     const double *cldfr;              // [nlay][ncol]
     const double *taucld;             // [nlay][ncol][16]: the reference's taucld(nbndlw, ncol, nlay)
     const double *cicewp, *cliqwp;    // [nlay][ncol] or NULL (zero)
+    const double *cldfmc, *taucmc;    // McICA subcolumns [nlay][ncol][140]: the reference's (ngptlw, ncol, nlay)
 };
+constexpr int LW_MCICA = 4;  // LwClouds::icld of the McICA driver's transfer
+// McICA subcolumns of mcica_subcol_lw with the KISS generator: inputs
+// [nlay][ncol] (tauc [nlay][ncol][16]), subcolumn outputs [nlay][ncol][140]
+struct McicaSubcolArgs {
+    int ncol, nlay, icld, permuteseed;
+    const double *play, *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc;  // ciwp, clwp, rei, rel may be NULL (zero)
+    double *cldfmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl, *taucmcl;  // only cldfmcl and taucmcl required
+    int *bad;  // columns whose pressure rises from the first layer to the second; NULL: not counted
+};
+hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, hipStream_t s);
 // Per-(layer, column) record of the cloud column kernel, in the band
 // kernel's precision: the layer's fraction, then rtrnmr's six upward
 // overlap factors of index lay+1 and its six downward ones of index lay-1

@@ -58,8 +58,12 @@ __global__ __launch_bounds__(256) void lw_prep_kernel(ColumnInputs in, LwWork w,
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
         double h2o = in.h2o[ix];
-        double wkl1 = (h2o / (1.0 - h2o)) * amdw;
-        double wkl2 = in.co2[ix], wkl3 = in.o3[ix] * amdo, wkl4 = in.n2o[ix], wkl6 = in.ch4[ix], wkl7 = in.o2[ix];
+        // MiMA's driver passes specific humidity and an O3 mass mixing
+        // ratio (rrtmg_lw_rad.nomcica.f90:775-778); the McICA driver's inatm
+        // takes volume mixing ratios (rrtmg_lw_rad.f90:778, 784)
+        double wkl1 = in.vmr ? h2o : (h2o / (1.0 - h2o)) * amdw;
+        double wkl2 = in.co2[ix], wkl3 = in.vmr ? in.o3[ix] : in.o3[ix] * amdo, wkl4 = in.n2o[ix], wkl6 = in.ch4[ix],
+               wkl7 = in.o2[ix];
         double amm = (1.0 - wkl1) * amd + wkl1 * amw;
         double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1.0 + wkl1));
         double summol = 0.0;
@@ -174,7 +178,7 @@ __global__ __launch_bounds__(256) void lw_column_kernel(ColumnInputs in, LwWork 
         const size_t ix = (size_t)lay * ncol + col;
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
         double h2o = in.h2o[ix];
-        double wkl1 = (h2o / (1.0 - h2o)) * amdw;
+        double wkl1 = in.vmr ? h2o : (h2o / (1.0 - h2o)) * amdw;
         double amm = (1.0 - wkl1) * amd + wkl1 * amw;
         double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1.0 + wkl1));
         wkl1 = coldry * wkl1;

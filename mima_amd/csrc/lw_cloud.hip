@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 
 #include "kernels.hpp"
 #include "lw_optics.hpp"
@@ -178,6 +179,107 @@ __global__ __launch_bounds__(64) void lw_cloud_column_kernel(int ncol, int nlay,
     }
 }
 
+// ---------------------------------------------------------------------------
+// McICA (the reference's rrtmg_lw_rad.f90 driver with mcica_subcol_gen_lw.f90)
+
+// One step of the KISS generator (mcica_subcol_gen_lw.f90:530-562): a
+// congruential, a 3-shift register and two multiply-with-carry generators
+// in 32-bit wrap-around arithmetic, their sum read as a signed integer and
+// mapped to (0, 1) by a product and a sum rounded separately (no fused
+// multiply-add), as the reference computes it.
+__device__ __forceinline__ double kiss_step(uint32_t &s1, uint32_t &s2, uint32_t &s3, uint32_t &s4)
+{
+#pragma clang fp contract(off)
+    s1 = 69069u * s1 + 1327217885u;
+    uint32_t k = s2;
+    k ^= k << 13;
+    k ^= k >> 17;
+    k ^= k << 5;
+    s2 = k;
+    s3 = 18000u * (s3 & 65535u) + (s3 >> 16);
+    s4 = 30903u * (s4 & 65535u) + (s4 >> 16);
+    const uint32_t sum = s1 + s2 + (s3 << 16) + s4;
+    return (double)(int32_t)sum * 2.328306e-10 + 0.5;
+}
+
+// mcica_subcol_lw with the KISS generator (mcica_subcol_gen_lw.f90:48-152,
+// generate_stochastic_clouds :158-520), one thread per column, subcolumn
+// (g-point) by subcolumn: seeds from the fractional Pa of the four lowest
+// layers advanced permuteseed times; a number per (subcolumn, layer) for
+// random (1) and maximum-random (2) overlap, the latter carried up through
+// cloudy layers; one per subcolumn for maximum overlap (3); cloudy where the
+// number >= 1 - fraction, with the band's depth and the layer's water paths.
+// A column whose second layer has the higher pressure (the reference stops,
+// :327-329) is counted in *bad (when given) and left unwritten.  NULL water
+// paths or radii read as zero; NULL outputs other than cldfmcl and taucmcl
+// are skipped.
+__global__ __launch_bounds__(64) void lw_mcica_subcol_kernel(McicaSubcolArgs a, const LwBand *__restrict__ bands)
+{
+#pragma clang fp contract(off)
+    const int ncol = a.ncol, nlay = a.nlay, icld = a.icld;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    if (a.play[col] * 1.e2 < a.play[(size_t)ncol + col] * 1.e2) {
+        if (a.bad) atomicAdd(a.bad, 1);
+        return;
+    }
+    // seeds from the fractional Pa of the four lowest layers (:330-333),
+    // each operation rounded on its own as the reference's: contraction is
+    // off in this kernel, since a fused play * 100 - int(play * 100) keeps
+    // bits the reference rounds away and changes the seed
+    uint32_t s[4];
+    for (int k = 0; k < 4; ++k) {
+        const double p = a.play[(size_t)k * ncol + col] * 1.e2;
+        s[k] = (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
+    }
+    for (int n = 0; n < a.permuteseed; ++n) (void)kiss_step(s[0], s[1], s[2], s[3]);
+    for (int l = 0; l < nlay; ++l) {
+        const size_t ix = (size_t)l * ncol + col;
+        if (a.reicmcl) a.reicmcl[ix] = a.rei ? a.rei[ix] : 0.0;
+        if (a.relqmcl) a.relqmcl[ix] = a.rel ? a.rel[ix] : 0.0;
+    }
+    int b = 0;
+    for (int ig = 0; ig < 140; ++ig) {
+        while (ig >= bands[b].ngs + bands[b].ng) ++b;
+        const double r = icld == 3 ? kiss_step(s[0], s[1], s[2], s[3]) : 0.0;
+        double prev = 0.0, fprev = 0.0;
+        for (int l = 0; l < nlay; ++l) {
+            double x = icld == 3 ? r : kiss_step(s[0], s[1], s[2], s[3]);
+            const size_t ix = (size_t)l * ncol + col, o = ix * 140 + ig;
+            double f = a.cldfrac[ix];
+            if (f < 1.e-20) f = 0.0;
+            if (icld == 2 && l > 0) x = prev > 1.0 - fprev ? prev : x * (1.0 - fprev);
+            prev = x;
+            fprev = f;
+            const bool cloudy = x >= 1.0 - f;
+            a.cldfmcl[o] = cloudy ? 1.0 : 0.0;
+            a.taucmcl[o] = cloudy ? a.tauc[ix * 16 + b] : 0.0;
+            if (a.ciwpmcl) a.ciwpmcl[o] = cloudy && a.ciwp ? a.ciwp[ix] : 0.0;
+            if (a.clwpmcl) a.clwpmcl[o] = cloudy && a.clwp ? a.clwp[ix] : 0.0;
+        }
+    }
+}
+
+__global__ void lw_mcica_zero_kernel(int *bad) { *bad = 0; }
+
+hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, hipStream_t s)
+{
+    if (a.bad) hipLaunchKernelGGL(lw_mcica_zero_kernel, dim3(1), dim3(1), 0, s, a.bad);
+    hipLaunchKernelGGL(lw_mcica_subcol_kernel, dim3((a.ncol + 63) / 64), dim3(64), 0, s, a, bands);
+    return hipGetLastError();
+}
+
+// the McICA transfer's layer flag (rrtmg_lw_rtrnmc.f90:297-313): cloudy
+// where any g-point's subcolumn is
+__global__ __launch_bounds__(256) void lw_mcica_flags_kernel(int ncol, int nlay, LwClouds cl, LwCloudWork cw)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)nlay * ncol) return;
+    bool any = false;
+    for (int g = 0; g < 140; ++g) any = any || cl.cldfmc[i * 140 + g] == 1.0;
+    cw.flags[i] = any ? LWC_CLOUDY : 0;
+}
+
 // tau_tbl(itr) = bpade t / (1 - t), t = itr / 10000, tau_tbl(10000) = 1e10
 // (rrtmg_lw_init.f90:106-123): evaluated, not stored; FP64 gives the
 // table's value exactly
@@ -331,7 +433,19 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
     // the layer's cloud optical depth on the diffusivity slant path
     // (rrtmg_lw_rtrnmr.f90:314-323: secdiff of the band times cldprop's depth)
     auto odcld_of = [&](int lay, int fl) {
+        if constexpr (ICLD == LW_MCICA) {
+            // the lane's g-point subcolumn (rrtmg_lw_rtrnmc.f90:297-306)
+            const size_t o = ((size_t)lay * ncol + col) * 140 + B.ngs + g;
+            return cl.cldfmc[o] == 1.0 ? secdiff * (Real)cl.taucmc[o] : (Real)0;
+        }
         return (fl & LWC_TAU) ? secdiff * (Real)cl.taucld[((size_t)lay * ncol + col) * 16 + (BAND - 1)] : (Real)0;
+    };
+    // the cloud fraction the transfer weights a cloudy layer with: the
+    // layer's, or in McICA the lane's subcolumn's (0 or 1)
+    auto cf_of = [&](int lay, const Real *rec) {
+        if constexpr (ICLD == LW_MCICA)
+            return (Real)cl.cldfmc[((size_t)lay * ncol + col) * 140 + B.ngs + g];
+        return rec[LWC_CF];
     };
     const double *tauaer = in.tauaer;
     auto store = [&](int k, int lev, Real v) {
@@ -362,7 +476,7 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
         Real atrans, bbd, bbugas, atot = (Real)0, bbutot = (Real)0;
         if (fl & LWC_CLOUDY) {
             iclddn = true;
-            const Real cf = rec[LWC_CF];
+            const Real cf = cf_of(lev - 1, rec);
             const Real odcld = odcld_of(lev - 1, fl);
             const CloudyLayer<Real> r =
                 lw_cloudy_layer(odepth, odcld, plfrac, blay, dplankup, dplankdn, exp_tbl, tfn_tbl);
@@ -371,7 +485,7 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
             bbd = r.bbd;
             bbugas = r.bbugas;
             bbutot = r.bbutot;
-            if constexpr (ICLD == 1) {
+            if constexpr (ICLD == 1 || ICLD == LW_MCICA) {
                 const Real efcl = (one - exp(-odcld)) * cf;
                 radld = radld - radld * (atrans + efcl * (one - atrans)) + r.gassrc + cf * (r.bbdtot * atot - r.gassrc);
             } else {
@@ -424,10 +538,10 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
         const int fl = cw.flags[(size_t)(lev - 1) * ncol + col];
         if (fl & LWC_CLOUDY) {
             const Real *rec = fac + ((size_t)(lev - 1) * ncol + col) * LWC_STRIDE;
-            const Real cf = rec[LWC_CF];
+            const Real cf = cf_of(lev - 1, rec);
             const Real atot = at32(s_atot, ix), bbutot = at32(s_bbt, ix);
             const Real gassrc = bbugas * atrans;
-            if constexpr (ICLD == 1) {
+            if constexpr (ICLD == 1 || ICLD == LW_MCICA) {
                 const Real efcl = (one - exp(-odcld_of(lev - 1, fl))) * cf;
                 radlu = radlu - radlu * (atrans + efcl * (one - atrans)) + gassrc + cf * (bbutot * atot - gassrc);
             } else {
@@ -562,6 +676,32 @@ __global__ __launch_bounds__(256) void lw_cloud_finish_kernel(ColumnInputs in, L
     out.hrc[il] = heatfac * (fnet[1][0] - fnet[1][1]) / dp;
 }
 
+// the band kernels of one transfer (ICLD 1 rtrn, 2 rtrnmr, LW_MCICA rtrnmc)
+template <typename Real, int ICLD>
+static void launch_lw_cloud_bands(const ColumnInputs &in, const LwClouds &cl, const LwWork &w,
+                                  const LwCloudWork &cw, const LwDevTables &t, hipStream_t s)
+{
+    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
+    if constexpr (sizeof(Real) == 4) {
+        if (t.lds) {
+            const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * t.desc.nwaves;
+            const dim3 grid(lw_lds_grid(t.plan, in.ncol, t.desc.nwaves)), block(64 * LW_WGW);
+            hipLaunchKernelGGL(reset_items_kernel, dim3(1), dim3(1), 0, s, w.items);
+            hipLaunchKernelGGL(lw_cloud_band_lds_kernel<ICLD>, grid, block, (size_t)t.plan.lds_bytes, s, in, w, cl, cw,
+                               t.bands_lds, t.plan, t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl,
+                               t.desc.totplnk, (const float *)t.tab, t.cst, t.units);
+            return;
+        }
+    }
+    const int nblocks = ntiles * t.desc.nwaves;
+    for (int b0 = 0; b0 < nblocks; b0 += cw.chunk) {
+        const int nb_c = std::min(cw.chunk, nblocks - b0);
+        hipLaunchKernelGGL((lw_cloud_band_kernel<Real, ICLD>), dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw,
+                           t.bands, t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,
+                           (const Real *)t.tab, t.cst, t.units);
+    }
+}
+
 template <typename Real>
 static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, const LwOutputs &out,
                                     const LwWork &w, const LwCloudWork &cw, const LwDevTables &t, int mode,
@@ -571,44 +711,21 @@ static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, 
     if (ev) (void)hipEventRecord(ev[0], s);
     hipError_t e = launch_lw_prep(in, w, t, mode, s);
     if (e != hipSuccess) return e;
-    // one thread per column, sequential over layers: 64-thread blocks spread
-    // the columns over all compute units
-    hipLaunchKernelGGL(lw_cloud_column_kernel<Real>, dim3((in.ncol + 63) / 64), dim3(64), 0, s, in.ncol, in.nlay, cl,
-                       cw);
+    if (cl.icld == LW_MCICA) {
+        hipLaunchKernelGGL(lw_mcica_flags_kernel, dim3(nbl), dim3(256), 0, s, in.ncol, in.nlay, cl, cw);
+    } else {
+        // one thread per column, sequential over layers: 64-thread blocks
+        // spread the columns over all compute units
+        hipLaunchKernelGGL(lw_cloud_column_kernel<Real>, dim3((in.ncol + 63) / 64), dim3(64), 0, s, in.ncol, in.nlay,
+                           cl, cw);
+    }
     if (ev) (void)hipEventRecord(ev[1], s);
-    const int ntiles = (in.ncol + TILE_COLS - 1) / TILE_COLS;
-    const int nblocks = ntiles * t.desc.nwaves;
-    if constexpr (sizeof(Real) == 4) {
-        if (t.lds) {
-            const int nitems = (ntiles + LW_WGW - 1) / LW_WGW * t.desc.nwaves;
-            const dim3 grid(lw_lds_grid(t.plan, in.ncol, t.desc.nwaves)), block(64 * LW_WGW);
-            hipLaunchKernelGGL(reset_items_kernel, dim3(1), dim3(1), 0, s, w.items);
-            if (cl.icld == 1)
-                hipLaunchKernelGGL(lw_cloud_band_lds_kernel<1>, grid, block, (size_t)t.plan.lds_bytes, s, in, w, cl,
-                                   cw, t.bands_lds, t.plan, t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl,
-                                   t.desc.totplnk, (const float *)t.tab, t.cst, t.units);
-            else
-                hipLaunchKernelGGL(lw_cloud_band_lds_kernel<2>, grid, block, (size_t)t.plan.lds_bytes, s, in, w, cl,
-                                   cw, t.bands_lds, t.plan, t.desc.nwaves, nitems, t.desc.exp_tbl, t.desc.tfn_tbl,
-                                   t.desc.totplnk, (const float *)t.tab, t.cst, t.units);
-            if (ev) (void)hipEventRecord(ev[2], s);
-            hipLaunchKernelGGL(lw_cloud_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cw, out, t.bands,
-                               t.heatfac);
-            if (ev) (void)hipEventRecord(ev[3], s);
-            return hipGetLastError();
-        }
-    }
-    for (int b0 = 0; b0 < nblocks; b0 += cw.chunk) {
-        const int nb_c = std::min(cw.chunk, nblocks - b0);
-        if (cl.icld == 1)
-            hipLaunchKernelGGL((lw_cloud_band_kernel<Real, 1>), dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw,
-                               t.bands, t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,
-                               (const Real *)t.tab, t.cst, t.units);
-        else
-            hipLaunchKernelGGL((lw_cloud_band_kernel<Real, 2>), dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw,
-                               t.bands, t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, t.desc.tfn_tbl, t.desc.totplnk,
-                               (const Real *)t.tab, t.cst, t.units);
-    }
+    if (cl.icld == 1)
+        launch_lw_cloud_bands<Real, 1>(in, cl, w, cw, t, s);
+    else if (cl.icld == LW_MCICA)
+        launch_lw_cloud_bands<Real, LW_MCICA>(in, cl, w, cw, t, s);
+    else
+        launch_lw_cloud_bands<Real, 2>(in, cl, w, cw, t, s);
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(lw_cloud_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cw, out, t.bands, t.heatfac);
     if (ev) (void)hipEventRecord(ev[3], s);

@@ -214,6 +214,55 @@ def rrtmg_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr,
     return tuple(outs)
 
 
+def mcica_subcol_lw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei, rel, tauc, iplon=0,
+                    stream=None):
+    """mcica_subcol_lw (mcica_subcol_gen_lw.f90:48-154): McICA cloud
+    subcolumns with the KISS generator (irng = 0).
+
+    ``play``, ``cldfrac``, ``ciwp``, ``clwp``, ``rei``, ``rel`` (nlay, ncol);
+    ``tauc`` (nlay, ncol, 16).  ciwp, clwp, rei, rel may be None (zero).
+    Returns a dict of cldfmcl, ciwpmcl, clwpmcl, taucmcl (nlay, ncol, 140),
+    i.e. the reference's (ngptlw, ncol, nlay), and reicmcl, relqmcl (nlay,
+    ncol); icld = 0 returns them zero-filled (the reference leaves its
+    outputs untouched)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    shapes = [(nlay, ncol, 140)] * 3 + [(nlay, ncol)] * 2 + [(nlay, ncol, 140)]
+    if dev:
+        import torch
+        outs = [torch.zeros(sh, dtype=torch.float64, device=play.device) for sh in shapes]
+    else:
+        outs = [np.zeros(sh) for sh in shapes]
+    rc = lib().rrtmg_hip_mcica_subcol_lw(
+        int(iplon), int(ncol), int(nlay), int(icld), int(permuteseed), int(irng),
+        *[a.inp(x) for x in (play, cldfrac, ciwp, clwp, rei, rel, tauc)], *_ptrs(dev, outs),
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "reicmcl", "relqmcl", "taucmcl"), outs))
+
+
+def rrtmg_lw_mcica(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                   n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, inflglw, iceflglw, liqflglw,
+                   cldfmcl, taucmcl, ciwpmcl=None, clwpmcl=None, reicmcl=None, relqmcl=None, tauaer=None,
+                   stream=None):
+    """rrtmg_lw of the McICA driver (rrtmg_lw_rad.f90:80-89) on subcolumns
+    ``cldfmcl``, ``taucmcl`` (nlay, ncol, 140); inflglw = 0.  h2ovmr and
+    o3vmr are volume mixing ratios, as this driver's inatm takes them.
+    Returns (uflx, dflx, hr, uflxc, dflxc, hrc)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    rc = lib().rrtmg_hip_lw_mcica(
+        int(ncol), int(nlay), int(icld), int(idrv),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis)],
+        int(inflglw), int(iceflglw), int(liqflglw),
+        *[a.inp(x) for x in (cldfmcl, taucmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, tauaer)], *_ptrs(dev, outs),
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
 def _grad_outputs(dev, like, shapes):
     if dev:
         import torch
@@ -318,6 +367,18 @@ def lw_cloud_columns(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=No
     out = rrtmg_lw_cloud(c["ncol"], c["nlay"], icld, 0, *[c[k] for k in LW_KEYS], 0, 0, 0, cldfr, taucld,
                          cicewp=cicewp, cliqwp=cliqwp, tauaer=tauaer, stream=stream)
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out))
+
+
+def lw_mcica_columns(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permuteseed=0, tauaer=None, stream=None):
+    """McICA LW on a mima_amd.columns dict: the subcolumns of
+    mcica_subcol_lw (KISS) and rrtmg_lw of the McICA driver on them, both on
+    the GPU.  Returns (outputs dict, subcolumns dict)."""
+    ncol, nlay = c["ncol"], c["nlay"]
+    sub = mcica_subcol_lw(ncol, nlay, icld, permuteseed, 0, c["play"], cldfr, cicewp, cliqwp, None, None, taucld,
+                          stream=stream)
+    out = rrtmg_lw_mcica(ncol, nlay, icld, 0, *[c[k] for k in LW_KEYS], 0, 0, 0, sub["cldfmcl"], sub["taucmcl"],
+                         tauaer=tauaer, stream=stream)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out)), sub
 
 
 def lw_vjp_columns(c, uflx_bar=None, dflx_bar=None, hr_bar=None, tauaer=None, stream=None, forward=False):
