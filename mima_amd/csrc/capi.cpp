@@ -68,6 +68,7 @@ struct SideTables {
     LwBand *bands_lds = nullptr;                 // LW: offsets relative to each band's first k-table element
     LwLdsPlan plan{};                            // LW: schedule of the LDS-table kernel
     bool lds = false;                            // LW: that kernel fits this device
+    uint32_t *xs_jump = nullptr;                 // LW: the KISS 3-shift register's squarings (McICA)
     double cpdair = 0.0;
 };
 
@@ -442,6 +443,9 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     LwBand rebased[16];
     if (!rc && !make_lw_plan(s->lw, s->lwd.plan, rebased, err)) return fail(RRTMG_HIP_ENOENT, err);
     if (!rc) rc = upload(&s->lwd.bands_lds, rebased, sizeof rebased);
+    std::vector<uint32_t> xs(KISS_XS_JUMPS * 32);
+    kiss_xorshift_jumps(xs.data());
+    if (!rc) rc = upload(&s->lwd.xs_jump, xs.data(), xs.size() * sizeof(uint32_t));
     if (rc) return fail(rc, "uploading LW tables failed");
     {
         int ncu = 0, lds = 0;
@@ -1103,6 +1107,7 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
     if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
+    const size_t jump_bytes = align_up(140 * sizeof(McicaJump));
     McicaSubcolArgs a{};
     a.ncol = ncol;
     a.nlay = nlay;
@@ -1119,8 +1124,9 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
         size_t need = 0;
         for (int i = 0; i < 7; ++i) need += ins[i] ? len[i] : 0;
         for (int i = 0; i < 6; ++i) need += outs[i] ? olen[i] : 0;
-        if ((rc = W.lw_stage.ensure(need * sizeof(double)))) return fail(rc, "device staging allocation failed");
-        double *p = (double *)W.lw_stage.p;
+        if ((rc = W.lw_stage.ensure(jump_bytes + need * sizeof(double))))
+            return fail(rc, "device staging allocation failed");
+        double *p = (double *)((char *)W.lw_stage.p + jump_bytes);
         const double **din[] = {&a.play, &a.cldfrac, &a.ciwp, &a.clwp, &a.rei, &a.rel, &a.tauc};
         for (int i = 0; i < 7; ++i) {
             if (!ins[i]) continue;
@@ -1135,6 +1141,7 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
             p += olen[i];
         }
     } else {
+        if ((rc = W.lw_stage.ensure(jump_bytes))) return fail(rc, "device scratch allocation failed");
         if (g_checks) {
             std::vector<NamedArray> v;
             for (int i = 0; i < 7; ++i)
@@ -1147,7 +1154,8 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
         a.taucmcl = taucmcl;
     }
     if (ck) return fail(RRTMG_HIP_EDEVICE, "McICA subcolumns: host-to-device copy failed");
-    hipError_t e = launch_lw_mcica_subcol(a, (const LwBand *)s->lwd.bands, st);
+    hipError_t e = launch_lw_mcica_subcol(a, (const LwBand *)s->lwd.bands, s->lwd.xs_jump,
+                                          (McicaJump *)W.lw_stage.p, st);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumn launch: ") + hipGetErrorString(e));
     if (ptr_kind == RRTMG_HIP_HOST) {
         const double *src[] = {a.cldfmcl, a.ciwpmcl, a.clwpmcl, a.reicmcl, a.relqmcl, a.taucmcl};

@@ -122,7 +122,18 @@ struct McicaSubcolArgs {
     double *cldfmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl, *taucmcl;  // only cldfmcl and taucmcl required
     int *bad;  // columns whose pressure rises from the first layer to the second; NULL: not counted
 };
-hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, hipStream_t s);
+// jump constants of one subcolumn's skip over the generator's sequence
+struct McicaJump {
+    uint32_t a1, c1;  // congruential part: s -> a1 s + c1
+    uint32_t p3, p4;  // multiply-with-carry parts: a^skip mod (a 2^16 - 1)
+    int band, pad[3];
+};
+constexpr int KISS_XS_JUMPS = 34;  // squarings M^(2^k) of the 3-shift register's matrix, k < 34
+// the KISS_XS_JUMPS x 32 words of those squarings (column j of M^(2^k) at [k*32 + j]); host code
+void kiss_xorshift_jumps(uint32_t *t);
+// jump: device scratch of 140 McicaJump; xs_jump: the squarings on the device
+hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, const uint32_t *xs_jump,
+                                  McicaJump *jump, hipStream_t s);
 // Per-(layer, column) record of the cloud column kernel, in the band
 // kernel's precision: the layer's fraction, then rtrnmr's six upward
 // overlap factors of index lay+1 and its six downward ones of index lay-1

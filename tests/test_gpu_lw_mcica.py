@@ -105,10 +105,11 @@ def test_subcolumns_match_reference_masks(rr, icld, seed):
 def test_subcolumns_match_oracle(rr, icld):
     """512 columns x 40 layers with the cloud edge cases (fractions at and
     around the 1e-20 floor, overcast, thin and thick depths) and water
-    paths; permuteseed 0 and 7: every subcolumn array bitwise, the radii
-    copied."""
+    paths; permuteseed 0, 1, 7 and 100003 (the kernel jumps over the
+    generator's sequence instead of stepping through it): every subcolumn
+    array bitwise, the radii copied."""
     c, cf, tau, wp = _sample(512, 40, icld)
-    for seed in (0, 7):
+    for seed in (0, 1, 7, 100003):
         g = _subcol(rr, c, icld, cf, tau, wp, seed)
         o = po.lw_mcica_subcol(c, icld, cf, tau, cicewp=wp, cliqwp=wp, permuteseed=seed)
         for k in SUB:
