@@ -261,6 +261,29 @@ def cloudy_timing(torch, rrtmg, config_columns, args):
                      "cloudy_vs_clear_cost": res["mixed"] / clear_ms,
                      "kernel_ms": {"prep": float(kt[0]), "band": float(kt[1]), "finish": float(kt[2])},
                      "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
+    # McICA LW: the subcolumn generator (KISS, maximum-random overlap) and
+    # the McICA driver's transfer on its subcolumns, same clouds
+    sub = rrtmg.mcica_subcol_lw(ncol, nlay, 2, 0, 0, d["play"], lw_args[0], None, None, None, None, lw_args[1])
+    subcol = lambda: rrtmg.mcica_subcol_lw(ncol, nlay, 2, 0, 0, d["play"], lw_args[0], None, None, None,  # noqa: E731
+                                           None, lw_args[1])
+    mcica = lambda: dict(zip(("uflx", "dflx", "hr"), rrtmg.rrtmg_lw_mcica(  # noqa: E731
+        ncol, nlay, 2, 0, *[d[k] for k in rrtmg.LW_KEYS], 0, 0, 0, sub["cldfmcl"], sub["taucmcl"])[:3]))
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        rrtmg.set_precision(prec)
+        res[prec], o = timed(mcica)
+        outs[prec] = {k: v.cpu().numpy() for k, v in o.items()}
+    sub_s, _ = timed(subcol)
+    dfl = max(float(np.max(np.abs(outs["mixed"][k] - outs["fp64"][k]))) for k in ("uflx", "dflx"))
+    dhr = float(np.max(np.abs(outs["mixed"]["hr"] - outs["fp64"]["hr"])))
+    print("[bench] McICA LW %d x %d: subcolumns %.3f ms, mixed %.3f ms, fp64 %.3f ms" %
+          (ncol, nlay, sub_s * 1e3, res["mixed"] * 1e3, res["fp64"] * 1e3), file=sys.stderr, flush=True)
+    out["lw_mcica"] = {"workload": "McICA LW (rrtmg_hip_mcica_subcol_lw with the KISS generator, maximum-random "
+                                   "overlap, 140 subcolumns; rrtmg_hip_lw_mcica on them), same grid and clouds",
+                       "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
+                       "fp64_ms_per_step": res["fp64"] * 1e3, "subcolumns_ms": sub_s * 1e3,
+                       "subcolumn_bytes_written": ncol * nlay * 140 * 8 * 4,
+                       "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
     return out
 
 

@@ -383,14 +383,23 @@ hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands,
 }
 
 // the McICA transfer's layer flag (rrtmg_lw_rtrnmc.f90:297-313): cloudy
-// where any g-point's subcolumn is
+// where any g-point's subcolumn is.  A block reads the 140 fractions of
+// MCICA_FLAG_PAIRS consecutive (layer, column) pairs as one contiguous run
+// (coalesced loads; one thread per pair would stride 1120 bytes across a
+// wave) and marks the pairs in LDS.
+constexpr int MCICA_FLAG_PAIRS = 32;
 __global__ __launch_bounds__(256) void lw_mcica_flags_kernel(int ncol, int nlay, LwClouds cl, LwCloudWork cw)
 {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)nlay * ncol) return;
-    bool any = false;
-    for (int g = 0; g < 140; ++g) any = any || cl.cldfmc[i * 140 + g] == 1.0;
-    cw.flags[i] = any ? LWC_CLOUDY : 0;
+    __shared__ int s_fl[MCICA_FLAG_PAIRS];
+    const size_t n2 = (size_t)nlay * ncol, p0 = (size_t)blockIdx.x * MCICA_FLAG_PAIRS;
+    if (threadIdx.x < MCICA_FLAG_PAIRS) s_fl[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t np = n2 - p0 < (size_t)MCICA_FLAG_PAIRS ? n2 - p0 : (size_t)MCICA_FLAG_PAIRS;
+    const double *f = cl.cldfmc + p0 * 140;
+    for (int i = threadIdx.x; i < (int)(np * 140); i += blockDim.x)
+        if (f[i] == 1.0) s_fl[i / 140] = LWC_CLOUDY;
+    __syncthreads();
+    if (threadIdx.x < np) cw.flags[p0 + threadIdx.x] = s_fl[threadIdx.x];
 }
 
 // tau_tbl(itr) = bpade t / (1 - t), t = itr / 10000, tau_tbl(10000) = 1e10
@@ -825,7 +834,9 @@ static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, 
     hipError_t e = launch_lw_prep(in, w, t, mode, s);
     if (e != hipSuccess) return e;
     if (cl.icld == LW_MCICA) {
-        hipLaunchKernelGGL(lw_mcica_flags_kernel, dim3(nbl), dim3(256), 0, s, in.ncol, in.nlay, cl, cw);
+        const size_t n2 = (size_t)in.nlay * in.ncol;
+        hipLaunchKernelGGL(lw_mcica_flags_kernel, dim3((unsigned)((n2 + MCICA_FLAG_PAIRS - 1) / MCICA_FLAG_PAIRS)),
+                           dim3(256), 0, s, in.ncol, in.nlay, cl, cw);
     } else {
         // one thread per column, sequential over layers: 64-thread blocks
         // spread the columns over all compute units

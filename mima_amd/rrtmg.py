@@ -228,11 +228,13 @@ def mcica_subcol_lw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, cl
     dev = _is_torch(play) and play.is_cuda
     a = _Args(dev)
     shapes = [(nlay, ncol, 140)] * 3 + [(nlay, ncol)] * 2 + [(nlay, ncol, 140)]
+    # the call writes every element unless icld = 0
     if dev:
         import torch
-        outs = [torch.zeros(sh, dtype=torch.float64, device=play.device) for sh in shapes]
+        mk = torch.zeros if icld == 0 else torch.empty
+        outs = [mk(sh, dtype=torch.float64, device=play.device) for sh in shapes]
     else:
-        outs = [np.zeros(sh) for sh in shapes]
+        outs = [(np.zeros if icld == 0 else np.empty)(sh) for sh in shapes]
     rc = lib().rrtmg_hip_mcica_subcol_lw(
         int(iplon), int(ncol), int(nlay), int(icld), int(permuteseed), int(irng),
         *[a.inp(x) for x in (play, cldfrac, ciwp, clwp, rei, rel, tauc)], *_ptrs(dev, outs),
