@@ -2,7 +2,8 @@
 # Build the reference RRTMG Fortran (as MiMA compiles it: default REAL is
 # 8 bytes, -r8 in CMakeLists.txt:90-97) from the sources where they lie in the
 # MiMA reference tree, plus this directory's C-callable drivers, into
-# oracle/_ref/librrtmg_sw_ref.so and oracle/_ref/librrtmg_lw_ref.so.
+# oracle/_ref/librrtmg_sw_ref.so and oracle/_ref/librrtmg_lw_ref.so, and the
+# McICA drivers into librrtmg_lw_mcica_ref.so and librrtmg_sw_mcica_ref.so.
 # Test infrastructure only.
 #
 # RRTMG_SW is built whole, its init included: its k-table data file
@@ -86,3 +87,21 @@ done
 $FC $MFLAGS -c "$HERE/ref_lw_mcica_driver.f90" -o "$OUT/obj_lwm/ref_lw_mcica_driver.o"
 $FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_lw_mcica_ref.so" $MOBJS "$OUT/obj_lwm/ref_lw_mcica_driver.o"
 echo "built $OUT/librrtmg_lw_mcica_ref.so"
+
+# McICA SW: as for LW, its driver rrtmg_sw_rad.f90 redefines the module
+# rrtmg_sw_rad, so it gets its own module directory and library (the SW
+# modules, data, init, setcoef, taumol, reftra and vrtqdr objects are shared)
+mkdir -p "$OUT/mod_swm" "$OUT/obj_swm"
+SMFLAGS="-fdefault-real-8 -fPIC -O2 -module-dir $OUT/mod_swm -I$OUT/mod_swm -I$OUT/mod"
+SMOBJS=""
+for m in $MODS; do SMOBJS="$SMOBJS $OUT/obj/$m.o"; done
+for s in rrtmg_sw_k_g rrtmg_sw_setcoef rrtmg_sw_init rrtmg_sw_taumol rrtmg_sw_reftra rrtmg_sw_vrtqdr; do
+  SMOBJS="$SMOBJS $OUT/obj/$s.o"
+done
+for s in mcica_random_numbers mcica_subcol_gen_sw rrtmg_sw_cldprmc rrtmg_sw_spcvmc rrtmg_sw_rad; do
+  $FC $SMFLAGS -c "$SW/src/$s.f90" -o "$OUT/obj_swm/$s.o"
+  SMOBJS="$SMOBJS $OUT/obj_swm/$s.o"
+done
+$FC $SMFLAGS -c "$HERE/ref_sw_mcica_driver.f90" -o "$OUT/obj_swm/ref_sw_mcica_driver.o"
+$FC -fdefault-real-8 -fPIC -shared -o "$OUT/librrtmg_sw_mcica_ref.so" $SMOBJS "$OUT/obj_swm/ref_sw_mcica_driver.o"
+echo "built $OUT/librrtmg_sw_mcica_ref.so"

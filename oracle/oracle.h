@@ -144,6 +144,30 @@ int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *
                           const double *cldfmcl, const double *taucmcl, const double *tauaer, double *uflx,
                           double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc);
 
+/* The KISS generator's cloudy-subcolumn mask of one column (both sides'
+ * generators, mcica_subcol_gen_lw.f90:158-520 / mcica_subcol_gen_sw.f90:
+ * 163-536, nsub 140 or 112): mask[ig * 256 + l].  -6 on rising pressure. */
+int oracle_mcica_mask(int i, int ncol, int nlay, int icld, int permuteseed, int nsub, const double *play,
+                      const double *cldfrac, unsigned char *mask);
+/* McICA SW: mcica_subcol_sw with the KISS generator and rrtmg_sw of the
+ * McICA driver (rrtmg_sw_rad.f90:80-745: volume mixing ratios, zenith
+ * cosine floored at 1e-10, cldprmc_sw with inflgsw = 0, spcvmc_sw).
+ * Cloud inputs (ncol, nlay), optics (14, ncol, nlay); subcolumns (112,
+ * ncol, nlay).  0 on success. */
+int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                           const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                           const double *ssac, const double *asmc, const double *fsfc, double *cldfmcl,
+                           double *ciwpmcl, double *clwpmcl, double *taucmcl, double *ssacmcl, double *asmcmcl,
+                           double *fsfcmcl);
+int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                          const double *ch4vmr, const double *n2ovmr, const double *o2vmr, const double *asdir,
+                          const double *asdif, const double *aldir, const double *aldif, const double *coszen,
+                          double adjes, int dyofyr, double scon, const double *cldfmcl, const double *taucmcl,
+                          const double *ssacmcl, const double *asmcmcl, const double *fsfcmcl,
+                          const double *ciwpmcl, const double *clwpmcl, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc);
+
 /* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...): copies up to
  * n values (Fortran order) into out and returns the element count, or -1. */
 long oracle_reduced(const char *name, double *out, long n);

@@ -1656,17 +1656,57 @@ static double kiss_step(uint32_t s[4])
     return (double)(int32_t)sum * 2.328306e-10 + 0.5;
 }
 
+/* The KISS generator's cloudy-subcolumn mask of column i (irng = 0;
+ * generate_stochastic_clouds, mcica_subcol_gen_lw.f90:158-520, and its SW
+ * twin generate_stochastic_clouds_sw, mcica_subcol_gen_sw.f90:163-536, which
+ * differ only in nsub): seeds from the fractional Pa of the four lowest
+ * layers, advanced permuteseed times; one random number per (subcolumn,
+ * layer) for random (icld = 1) and maximum-random (2) overlap, the latter
+ * then carried up through cloudy layers; one per subcolumn for maximum
+ * overlap (3).  Subcolumn ig of layer l is cloudy where its number >= 1 -
+ * cldfrac: mask[ig * MAXLAY + l] (0-based).  -6 when the second layer's
+ * pressure is the higher (the reference stops). */
+int oracle_mcica_mask(int i, int ncol, int nlay, int icld, int permuteseed, int nsub, const double *play,
+                      const double *cldfrac, unsigned char *mask)
+{
+    if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
+    uint32_t s[4];
+    for (int k = 0; k < 4; ++k) {
+        double p = play[i + (long)ncol * k] * 1.e2;
+        s[k] = (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
+    }
+    for (int n = 0; n < permuteseed; ++n) (void)kiss_step(s);
+    double cldf[MAXLAY + 1];
+    for (int l = 0; l < nlay; ++l) {
+        double f = cldfrac[i + (long)ncol * l];
+        cldf[l] = f < 1.e-20 ? 0.0 : f;
+    }
+    for (int ig = 0; ig < nsub; ++ig) {
+        double cdf[MAXLAY + 1];
+        if (icld == 3) {
+            double r = kiss_step(s);
+            for (int l = 0; l < nlay; ++l) cdf[l] = r;
+        } else {
+            for (int l = 0; l < nlay; ++l) cdf[l] = kiss_step(s);
+        }
+        if (icld == 2)
+            for (int l = 1; l < nlay; ++l) {
+                if (cdf[l - 1] > 1.0 - cldf[l - 1])
+                    cdf[l] = cdf[l - 1];
+                else
+                    cdf[l] = cdf[l] * (1.0 - cldf[l - 1]);
+            }
+        for (int l = 0; l < nlay; ++l) mask[ig * MAXLAY + l] = cdf[l] >= 1.0 - cldf[l];
+    }
+    return 0;
+}
+
 /* mcica_subcol_lw with the KISS generator (irng = 0;
- * mcica_subcol_gen_lw.f90:48-152, generate_stochastic_clouds :158-520): per
- * column, seeds from the fractional Pa of the four lowest layers, advanced
- * permuteseed times; one random number per (subcolumn, layer) for random
- * (icld = 1) and maximum-random (2) overlap, the latter then carried up
- * through cloudy layers; one per subcolumn for maximum overlap (3).  The
- * subcolumn of g-point ig is cloudy where its number >= 1 - cldfrac, and
- * then takes the band's in-cloud depth and the layer's water paths.
- * Arrays: cldfrac, ciwp, clwp (ncol, nlay); tauc (16, ncol, nlay);
- * outputs (140, ncol, nlay).  -6 for a column whose layer pressures do not
- * fall from the surface (the reference stops), -7 for nlay < 4. */
+ * mcica_subcol_gen_lw.f90:48-152): the mask of oracle_mcica_mask; a cloudy
+ * subcolumn takes the band's in-cloud depth and the layer's water paths.
+ * Arrays: cldfrac, ciwp, clwp (ncol, nlay); tauc (16, ncol, nlay); outputs
+ * (140, ncol, nlay).  -6 for a column whose layer pressures do not fall from
+ * the surface (the reference stops), -7 for nlay < 4. */
 int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
                            const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
                            double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl)
@@ -1679,41 +1719,15 @@ int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
     int band_of[NGLW];
     for (int b = 0, ig = 0; b < 16; ++b)
         for (; ig < L.ngs[b]; ++ig) band_of[ig] = b;
-    static __thread double cdf[NGLW][MAXLAY + 1];
+    static __thread unsigned char mask[NGLW * MAXLAY];
 #pragma omp parallel for schedule(dynamic, 16)
     for (int i = 0; i < ncol; ++i) {
-        uint32_t s[4];
-        for (int k = 0; k < 4; ++k) {
-            double p = play[i + (long)ncol * k] * 1.e2;
-            s[k] = (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
-        }
-        for (int n = 0; n < permuteseed; ++n) (void)kiss_step(s);
-        double cldf[MAXLAY + 1];
-        for (int l = 0; l < nlay; ++l) {
-            double f = cldfrac[i + (long)ncol * l];
-            cldf[l] = f < 1.e-20 ? 0.0 : f;
-        }
-        for (int ig = 0; ig < NGLW; ++ig) {
-            if (icld == 3) {
-                double r = kiss_step(s);
-                for (int l = 0; l < nlay; ++l) cdf[ig][l] = r;
-            } else {
-                for (int l = 0; l < nlay; ++l) cdf[ig][l] = kiss_step(s);
-            }
-        }
-        if (icld == 2)
-            for (int l = 1; l < nlay; ++l)
-                for (int ig = 0; ig < NGLW; ++ig) {
-                    if (cdf[ig][l - 1] > 1.0 - cldf[l - 1])
-                        cdf[ig][l] = cdf[ig][l - 1];
-                    else
-                        cdf[ig][l] = cdf[ig][l] * (1.0 - cldf[l - 1]);
-                }
+        oracle_mcica_mask(i, ncol, nlay, icld, permuteseed, NGLW, play, cldfrac, mask);
         for (int l = 0; l < nlay; ++l) {
             long ix = i + (long)ncol * l;
             for (int ig = 0; ig < NGLW; ++ig) {
                 long o = ig + (long)NGLW * ix;
-                int cloudy = cdf[ig][l] >= 1.0 - cldf[l];
+                int cloudy = mask[ig * MAXLAY + l];
                 cldfmcl[o] = cloudy ? 1.0 : 0.0;
                 clwpmcl[o] = cloudy ? clwp[ix] : 0.0;
                 ciwpmcl[o] = cloudy ? ciwp[ix] : 0.0;

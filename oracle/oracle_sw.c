@@ -181,11 +181,14 @@ typedef struct {
     double taug[MAXLAY + 1][NGSW + 1], taur[MAXLAY + 1][NGSW + 1];
 } swcol_t;
 
-/* inatm_sw (rrtmg_sw_rad.nomcica.f90:761-1101), the clear-sky / iaer=0 part */
+/* inatm_sw (rrtmg_sw_rad.nomcica.f90:761-1101), the clear-sky / iaer=0 part.
+ * vmr = 0: h2o as specific humidity and O3 as a mass mixing ratio, MiMA's
+ * usage (:1004, 1007); vmr = 1: both as volume mixing ratios, as the McICA
+ * driver's inatm takes them (rrtmg_sw_rad.f90:1015, 1021) */
 static void inatm_sw(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
                      const double *plev, const double *tlay, const double *tlev,
                      const double *h2ovmr, const double *o3vmr, const double *co2vmr,
-                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr)
+                     const double *ch4vmr, const double *n2ovmr, const double *o2vmr, int vmr)
 {
     const double amd = 28.9660, amw = 18.0160, amdw = 1.607793, amdo = 0.603428;
 #define A2(a, i, k) a[(i) + (long)ncol * ((k)-1)]
@@ -200,9 +203,9 @@ static void inatm_sw(swcol_t *c, int iplon, int ncol, int nlay, const double *pl
         c->pz[l] = A2(plev, iplon, l + 1);
         c->tz[l] = A2(tlev, iplon, l + 1);
         c->pdp[l] = c->pz[l - 1] - c->pz[l];
-        c->wkl[1][l] = (A2(h2ovmr, iplon, l) / (1.0 - A2(h2ovmr, iplon, l))) * amdw;
+        c->wkl[1][l] = vmr ? A2(h2ovmr, iplon, l) : (A2(h2ovmr, iplon, l) / (1.0 - A2(h2ovmr, iplon, l))) * amdw;
         c->wkl[2][l] = A2(co2vmr, iplon, l);
-        c->wkl[3][l] = A2(o3vmr, iplon, l) * amdo;
+        c->wkl[3][l] = vmr ? A2(o3vmr, iplon, l) : A2(o3vmr, iplon, l) * amdo;
         c->wkl[4][l] = A2(n2ovmr, iplon, l);
         c->wkl[6][l] = A2(ch4vmr, iplon, l);
         c->wkl[7][l] = A2(o2vmr, iplon, l);
@@ -1084,10 +1087,15 @@ static void cldprop_sw0(int nl, const double *cldfrac, double (*tauc)[15], doubl
  * no aerosol): the clear optics of each layer as in the clear-sky path, the
  * cloudy optics (clear plus delta-scaled cloud, :459-466) through reftra
  * where the fraction exceeds 1e-12, the total-sky layer values blended by
- * the fraction (:502-549), and vrtqdr for the clear and the total stream. */
+ * the fraction (:502-549), and vrtqdr for the clear and the total stream.
+ * The cloud inputs are per (layer, g-point): spcvmc_sw
+ * (rrtmg_sw_spcvmc.f90:302-600) is spcvrt_sw with pcldfmc(ikl, iw),
+ * ptaucmc(ikl, iw), ... in place of pclfr(ikl), ptauc(ikl, ibm), ..., so
+ * the nomcica driver passes its per-layer fraction and per-band optics
+ * repeated over the g-points and the McICA driver its subcolumns. */
 static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const double *palbp,
-                            const double *adjflux, const double *pclfr, double (*ptauc)[15],
-                            double (*pasyc)[15], double (*pomgc)[15], double *pbbfd, double *pbbfu,
+                            const double *adjflux, double (*pcf)[NGSW + 1], double (*ptauc)[NGSW + 1],
+                            double (*pasyc)[NGSW + 1], double (*pomgc)[NGSW + 1], double *pbbfd, double *pbbfu,
                             double *pbbcd, double *pbbcu)
 {
     const int klev = c->nlayers;
@@ -1134,7 +1142,7 @@ static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const
             for (int jk = 1; jk <= klev; ++jk) {
                 int ikl = klev + 1 - jk;
                 lrtchkclr[jk] = 1;
-                lrtchkcld[jk] = pclfr[ikl] > repclc;
+                lrtchkcld[jk] = pcf[ikl][iw] > repclc;
                 /* ptaua = 0, pomga = 1, pasya = 0 (iaer = 0) */
                 ztauc[jk] = c->taur[ikl][iw] + c->taug[ikl][iw] + 0.0;
                 zomcc[jk] = c->taur[ikl][iw] * 1.0 + 0.0 * 1.0;
@@ -1145,9 +1153,9 @@ static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const
                 ztauc[jk] = (1.0 - zwf) * ztauc[jk];
                 zomcc[jk] = (zomcc[jk] - zwf) / (1.0 - zwf);
                 zgcc[jk] = (zgcc[jk] - zf) / (1.0 - zf);
-                ztauo[jk] = ztauc[jk] + ptauc[ikl][ibm];
-                zomco[jk] = ztauc[jk] * zomcc[jk] + ptauc[ikl][ibm] * pomgc[ikl][ibm];
-                zgco[jk] = (ptauc[ikl][ibm] * pomgc[ikl][ibm] * pasyc[ikl][ibm] + ztauc[jk] * zomcc[jk] * zgcc[jk]) /
+                ztauo[jk] = ztauc[jk] + ptauc[ikl][iw];
+                zomco[jk] = ztauc[jk] * zomcc[jk] + ptauc[ikl][iw] * pomgc[ikl][iw];
+                zgco[jk] = (ptauc[ikl][iw] * pomgc[ikl][iw] * pasyc[ikl][iw] + ztauc[jk] * zomcc[jk] * zgcc[jk]) /
                            zomco[jk];
                 zomco[jk] = zomco[jk] / ztauo[jk];
             }
@@ -1155,8 +1163,8 @@ static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const
             reftra_sw(klev, lrtchkcld, zgco, prmu0, ztauo, zomco, zrefo, zrefdo, ztrao, ztrado);
             for (int jk = 1; jk <= klev; ++jk) {
                 int ikl = klev + 1 - jk;
-                double zclear = 1.0 - pclfr[ikl];
-                double zcloud = pclfr[ikl];
+                double zclear = 1.0 - pcf[ikl][iw];
+                double zcloud = pcf[ikl][iw];
                 zref[jk] = zclear * zrefc[jk] + zcloud * zrefo[jk];
                 zrefd[jk] = zclear * zrefdc[jk] + zcloud * zrefdo[jk];
                 ztra[jk] = zclear * ztrac[jk] + zcloud * ztrao[jk];
@@ -1183,12 +1191,21 @@ static void spcvrt_sw_cloud(swcol_t *c, double prmu0, const double *palbd, const
     }
 }
 
+static void sw_column_u(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
+                        const double *plev, const double *tlay, const double *tlev,
+                        const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                        const double *ch4vmr, const double *n2ovmr, const double *o2vmr, int vmr)
+{
+    inatm_sw(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, vmr);
+    setcoef_sw(c);
+}
+
 static void sw_column(swcol_t *c, int iplon, int ncol, int nlay, const double *play,
                       const double *plev, const double *tlay, const double *tlev,
                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
                       const double *ch4vmr, const double *n2ovmr, const double *o2vmr)
 {
-    inatm_sw(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
+    inatm_sw(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, 0);
     setcoef_sw(c);
 }
 
@@ -1310,10 +1327,11 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
     {
         swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
         double (*a)[MAXLAY + 2][15] = malloc(sizeof(double[7][MAXLAY + 2][15]));
+        double (*gq)[MAXLAY + 2][NGSW + 1] = malloc(sizeof(double[4][MAXLAY + 2][NGSW + 1]));
         double cf[MAXLAY + 3], cwp[MAXLAY + 3];
         double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], zbbfu[MAXLAY + 3], zbbfd[MAXLAY + 3];
         double swnflxc[MAXLAY + 3], swnflx[MAXLAY + 3];
-        if (!c || !a) {
+        if (!c || !a || !gq) {
 #pragma omp atomic write
             err = -3;
         }
@@ -1362,7 +1380,16 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
                 albdir[ib] = asdir[iplon];
                 albdif[ib] = asdif[iplon];
             }
-            spcvrt_sw_cloud(c, cossza, albdif, albdir, adjflux, cf, ptauc, pasyc, pomgc, zbbfd, zbbfu, zbbcd, zbbcu);
+            for (int l = 1; l <= nlay; ++l)
+                for (int ib = 1, iw = 1; ib <= 14; ++ib)
+                    for (int jg = 1; jg <= S.ngc[ib - 1]; ++jg, ++iw) {
+                        gq[0][l][iw] = cf[l];
+                        gq[1][l][iw] = ptauc[l][ib];
+                        gq[2][l][iw] = pasyc[l][ib];
+                        gq[3][l][iw] = pomgc[l][ib];
+                    }
+            spcvrt_sw_cloud(c, cossza, albdif, albdir, adjflux, gq[0], gq[1], gq[2], gq[3], zbbfd, zbbfu, zbbcd,
+                            zbbcu);
             for (int i = 1; i <= nlay + 1; ++i) {
                 O2(swuflxc, iplon, i) = zbbcu[i];
                 O2(swdflxc, iplon, i) = zbbcd[i];
@@ -1382,6 +1409,146 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
         }
         free(c);
         free(a);
+        free(gq);
+    }
+    return err;
+}
+
+/* mcica_subcol_sw with the KISS generator (irng = 0;
+ * mcica_subcol_gen_sw.f90:47-159, generate_stochastic_clouds_sw :163-536):
+ * the mask of oracle_mcica_mask over 112 subcolumns; a cloudy subcolumn
+ * takes its band's in-cloud optics and the layer's water paths, a clear one
+ * depth 0, albedo 1, asymmetry 0 and forward fraction 0 (:477-510).  Inputs
+ * cldfrac, ciwp, clwp (ncol, nlay); tauc, ssac, asmc, fsfc (14, ncol, nlay);
+ * outputs (112, ncol, nlay).  -6, -7 as oracle_lw_mcica_subcol. */
+int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                           const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                           const double *ssac, const double *asmc, const double *fsfc, double *cldfmcl,
+                           double *ciwpmcl, double *clwpmcl, double *taucmcl, double *ssacmcl, double *asmcmcl,
+                           double *fsfcmcl)
+{
+    if (!S.ready) return -1;
+    if (nlay < 4 || nlay > MAXLAY) return -7;
+    if (icld < 1 || icld > 3) return -2;
+    for (int i = 0; i < ncol; ++i)
+        if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
+    int band_of[NGSW];
+    for (int b = 0, ig = 0; b < 14; ++b)
+        for (int k = 0; k < S.ngc[b]; ++k) band_of[ig++] = b;
+    static __thread unsigned char mask[NGSW * MAXLAY];
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ncol; ++i) {
+        oracle_mcica_mask(i, ncol, nlay, icld, permuteseed, NGSW, play, cldfrac, mask);
+        for (int l = 0; l < nlay; ++l) {
+            long ix = i + (long)ncol * l;
+            for (int ig = 0; ig < NGSW; ++ig) {
+                long o = ig + (long)NGSW * ix, j = band_of[ig] + 14 * ix;
+                int cloudy = mask[ig * MAXLAY + l];
+                cldfmcl[o] = cloudy ? 1.0 : 0.0;
+                clwpmcl[o] = cloudy ? clwp[ix] : 0.0;
+                ciwpmcl[o] = cloudy ? ciwp[ix] : 0.0;
+                taucmcl[o] = cloudy ? tauc[j] : 0.0;
+                ssacmcl[o] = cloudy ? ssac[j] : 1.0;
+                asmcmcl[o] = cloudy ? asmc[j] : 0.0;
+                fsfcmcl[o] = cloudy ? fsfc[j] : 0.0;
+            }
+        }
+    }
+    return 0;
+}
+
+/* rrtmg_sw of the McICA driver (rrtmg_sw_rad.f90:80-745; iaer = 0,
+ * inflgsw = 0): inatm with h2o and O3 as volume mixing ratios; no night
+ * early exit, the zenith cosine floored at 1e-10 instead (:571-572);
+ * cldprmc_sw (rrtmg_sw_cldprmc.f90:112-143): a subcolumn whose fraction
+ * and water path or depth reach 1e-20 has its optics delta-M scaled by its
+ * forward fraction, any other keeps its input optics; spcvmc_sw on them.
+ * Subcolumn inputs (112, ncol, nlay) (NULL water paths = 0). */
+int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                          const double *ch4vmr, const double *n2ovmr, const double *o2vmr, const double *asdir,
+                          const double *asdif, const double *aldir, const double *aldif, const double *coszen,
+                          double adjes, int dyofyr, double scon, const double *cldfmcl, const double *taucmcl,
+                          const double *ssacmcl, const double *asmcmcl, const double *fsfcmcl,
+                          const double *ciwpmcl, const double *clwpmcl, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
+{
+    if (!S.ready) return -1;
+    if (nlay < 2 || nlay > MAXLAY) return -2;
+    const double zepzen = 1.e-10, cldmin = 1.e-20;
+    double adjflx = adjes;
+    if (dyofyr > 0) adjflx = earth_sun(dyofyr);
+    double adjflux[30];
+    for (int ib = 16; ib <= 29; ++ib) adjflux[ib] = adjflx * (scon / 1.36822e+03);
+    int err = 0;
+#pragma omp parallel
+    {
+        swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
+        double (*gq)[MAXLAY + 2][NGSW + 1] = malloc(sizeof(double[4][MAXLAY + 2][NGSW + 1]));
+        double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], zbbfu[MAXLAY + 3], zbbfd[MAXLAY + 3];
+        double swnflxc[MAXLAY + 3], swnflx[MAXLAY + 3];
+        if (!c || !gq) {
+#pragma omp atomic write
+            err = -3;
+        }
+#pragma omp for schedule(dynamic, 16)
+        for (int iplon = 0; iplon < ncol; ++iplon) {
+            if (err == -3) continue;
+#define O2(x, i, k) x[(i) + (long)ncol * ((k)-1)]
+            for (int l = 1; l <= nlay; ++l) {
+                long ix = iplon + (long)ncol * (l - 1);
+                for (int iw = 1; iw <= NGSW; ++iw) {
+                    long o = (iw - 1) + (long)NGSW * ix;
+                    double cf = cldfmcl[o], tau = taucmcl[o], ssa = ssacmcl[o], asm_ = asmcmcl[o];
+                    double cwp = (ciwpmcl ? ciwpmcl[o] : 0.0) + (clwpmcl ? clwpmcl[o] : 0.0);
+                    if (cf >= cldmin && (cwp >= cldmin || tau >= cldmin)) {
+                        double ffp = fsfcmcl[o], ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssa;
+                        ssa = ffp1 * ssa / ffpssa;
+                        tau = ffpssa * tau;
+                        asm_ = (asm_ - ffp) / ffp1;
+                    }
+                    gq[0][l][iw] = cf;
+                    gq[1][l][iw] = tau;
+                    gq[2][l][iw] = asm_;
+                    gq[3][l][iw] = ssa;
+                }
+            }
+            sw_column_u(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                        1);
+            double cossza = coszen[iplon];
+            if (cossza < zepzen) cossza = zepzen;
+            double albdir[15], albdif[15];
+            for (int ib = 1; ib <= 9; ++ib) {
+                albdir[ib] = aldir[iplon];
+                albdif[ib] = aldif[iplon];
+            }
+            albdir[14] = aldir[iplon];
+            albdif[14] = aldif[iplon];
+            for (int ib = 10; ib <= 13; ++ib) {
+                albdir[ib] = asdir[iplon];
+                albdif[ib] = asdif[iplon];
+            }
+            spcvrt_sw_cloud(c, cossza, albdif, albdir, adjflux, gq[0], gq[1], gq[2], gq[3], zbbfd, zbbfu, zbbcd,
+                            zbbcu);
+            for (int i = 1; i <= nlay + 1; ++i) {
+                O2(swuflxc, iplon, i) = zbbcu[i];
+                O2(swdflxc, iplon, i) = zbbcd[i];
+                O2(swuflx, iplon, i) = zbbfu[i];
+                O2(swdflx, iplon, i) = zbbfd[i];
+                swnflxc[i] = zbbcd[i] - zbbcu[i];
+                swnflx[i] = zbbfd[i] - zbbfu[i];
+            }
+            for (int i = 1; i <= nlay; ++i) {
+                double zdpgcp = S.heatfac / c->pdp[i];
+                O2(swhrc, iplon, i) = (swnflxc[i + 1] - swnflxc[i]) * zdpgcp;
+                O2(swhr, iplon, i) = (swnflx[i + 1] - swnflx[i]) * zdpgcp;
+            }
+            O2(swhrc, iplon, nlay) = 0.0;
+            O2(swhr, iplon, nlay) = 0.0;
+#undef O2
+        }
+        free(c);
+        free(gq);
     }
     return err;
 }

@@ -160,6 +160,39 @@ def rrtmg_lw_mcica(c, cldfmcl, taucmcl, tauaer=None):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+def sw_mcica_subcol(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, permuteseed=0):
+    """mcica_subcol_sw with the KISS generator: cldfmcl, ciwpmcl, clwpmcl,
+    taucmcl, ssacmcl, asmcmcl, fsfcmcl as (nlay, ncol, 112)."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    z = np.zeros((nlay, ncol))
+    ins = [_f64(a) for a in (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp,
+                             taucld, ssacld, asmcld, fsfcld)]
+    outs = [np.zeros((nlay, ncol, 112)) for _ in range(7)]
+    rc = L.oracle_sw_mcica_subcol(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins],
+                                  *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_sw_mcica_subcol returned %d" % rc)
+    return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl"), outs))
+
+
+def rrtmg_sw_mcica(c, sub):
+    """rrtmg_sw of the McICA driver on subcolumns (nlay, ncol, 112)."""
+    L = lib()
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr",
+                                "n2ovmr", "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")]
+    sb = [None if sub.get(k) is None else _p(_f64(sub[k])) for k in
+          ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl")]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_sw_mcica(ncol, nlay, *[_p(a) for a in ins], ctypes.c_double(c["adjes"]),
+                                 int(c["dyofyr"]), ctypes.c_double(c["scon"]), *sb, *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_sw_mcica returned %d" % rc)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
 def _col(c, i, key):
     v = c[key]
     return np.ascontiguousarray(v[:, i], dtype=np.float64)

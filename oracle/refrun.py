@@ -26,6 +26,7 @@ ROOT = os.path.dirname(HERE)
 SW_LIB = os.path.join(HERE, "_ref", "librrtmg_sw_ref.so")
 LW_LIB = os.path.join(HERE, "_ref", "librrtmg_lw_ref.so")
 LW_MCICA_LIB = os.path.join(HERE, "_ref", "librrtmg_lw_mcica_ref.so")
+SW_MCICA_LIB = os.path.join(HERE, "_ref", "librrtmg_sw_mcica_ref.so")
 CPDAIR = 287.04 / (2.0 / 7.0)   # MiMA's cp_air (constants.F90: rdgas / kappa)
 D = ctypes.POINTER(ctypes.c_double)
 I = ctypes.POINTER(ctypes.c_int)
@@ -50,6 +51,14 @@ def available():
 def load_sw():
     lib = ctypes.CDLL(SW_LIB)
     lib.ref_sw_init(ctypes.c_double(CPDAIR))
+    return lib
+
+
+def load_sw_mcica():
+    """The reference's McICA RRTMG_SW library (its own copy of the SW
+    modules, initialised here)."""
+    lib = ctypes.CDLL(SW_MCICA_LIB)
+    lib.ref_sw_mcica_init(ctypes.c_double(CPDAIR))
     return lib
 
 
@@ -172,3 +181,33 @@ def run_lw_mcica(lib, c, icld, sub, tauaer=None):
     lib.ref_lw_mcica_run(n, L, int(icld), *[_p(a) for a in ins], *[_p(a) for a in sb], _p(tau),
                          *[_p(a) for a in outs])
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
+def run_sw_mcica_subcol(lib, c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None,
+                        permuteseed=0):
+    """mcica_subcol_sw of the reference (KISS): subcolumns of cldfr (nlay,
+    ncol) and the cloud optics (nlay, ncol, 14); returns cldfmcl, ciwpmcl,
+    clwpmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl as (nlay, ncol, 112)."""
+    n, L = c["ncol"], c["nlay"]
+    z = np.zeros((L, n))
+    ins = [np.ascontiguousarray(a, dtype=np.float64) for a in
+           (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp,
+            taucld, ssacld, asmcld, fsfcld)]
+    outs = [np.zeros((L, n, 112)) for _ in range(7)]
+    lib.ref_sw_mcica_subcol(n, L, int(icld), int(permuteseed), *[_p(a) for a in ins], *[_p(a) for a in outs])
+    return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl"), outs))
+
+
+def run_sw_mcica(lib, c, icld, sub):
+    """rrtmg_sw of the reference's McICA driver on subcolumns `sub` (as
+    run_sw_mcica_subcol returns them); h2ovmr, o3vmr read as volume mixing
+    ratios."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
+    sb = [np.ascontiguousarray(sub[k], dtype=np.float64) for k in
+          ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl")]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_sw_mcica_run(n, L, int(icld), *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
+                         ctypes.c_double(c["scon"]), *[_p(a) for a in sb], *[_p(a) for a in outs])
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
