@@ -24,8 +24,9 @@
  *     icld = idrv = iaer = 0, RR/rrtm_radiation.f90:106-108); icld != 0,
  *     SW iaer != 0 or LW idrv != 0 return RRTMG_HIP_ENOTSUP there.  Clouds
  *     given by their optical properties (inflglw = inflgsw = 0) take
- *     rrtmg_hip_lw_cloud and rrtmg_hip_sw_cloud, and McICA subcolumns
- *     rrtmg_hip_mcica_subcol_lw and rrtmg_hip_lw_mcica.
+ *     rrtmg_hip_lw_cloud and rrtmg_hip_sw_cloud; McICA takes
+ *     rrtmg_hip_mcica_subcol_lw / rrtmg_hip_lw_mcica and
+ *     rrtmg_hip_mcica_subcol_sw / rrtmg_hip_sw_mcica.
  */
 #ifndef RRTMG_HIP_H
 #define RRTMG_HIP_H
@@ -220,6 +221,60 @@ int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv,
                        const double *reicmcl, const double *relqmcl, const double *tauaer,
                        double *uflx, double *dflx, double *hr,
                        double *uflxc, double *dflxc, double *hrc,
+                       int ptr_kind, void *stream);
+
+/* Replaces mcica_subcol_sw (rrtmg_sw/gcm_model/src/mcica_subcol_gen_sw.f90:47-159):
+ * the McICA subcolumns of the SW g-points (112), the reference's whole
+ * argument list in its order; the generator, its limits and its error
+ * behaviour as rrtmg_hip_mcica_subcol_lw.  tauc, ssac, asmc, fsfc (14,
+ * ncol, nlay), band fastest; outputs cldfmcl, ciwpmcl, clwpmcl, taucmcl,
+ * ssacmcl, asmcmcl, fsfcmcl (112, ncol, nlay), g-point fastest (a clear
+ * subcolumn: depth 0, albedo 1, asymmetry 0, forward fraction 0,
+ * :477-510), reicmcl, relqmcl (ncol, nlay).  ssac, asmc, fsfc and their
+ * outputs are required. */
+int rrtmg_hip_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
+                              const double *play, const double *cldfrac,
+                              const double *ciwp, const double *clwp,
+                              const double *rei, const double *rel, const double *tauc,
+                              const double *ssac, const double *asmc, const double *fsfc,
+                              double *cldfmcl, double *ciwpmcl, double *clwpmcl,
+                              double *reicmcl, double *relqmcl, double *taucmcl,
+                              double *ssacmcl, double *asmcmcl, double *fsfcmcl,
+                              int ptr_kind, void *stream);
+
+/* Replaces rrtmg_sw of the McICA driver (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.f90:80-90):
+ * the reference's whole argument list in its order, on the subcolumns of
+ * rrtmg_hip_mcica_subcol_sw.  cldprmc_sw with inflgsw = 0
+ * (rrtmg_sw_cldprmc.f90:127-143): a subcolumn whose fraction and water path
+ * or depth reach 1e-20 has its depth, albedo and asymmetry delta-M scaled
+ * by its forward fraction, any other keeps them as given; spcvmc_sw: the
+ * clear stream and the total one blending each g-point's clear and cloudy
+ * layer values by its subcolumn's fraction, which may be partial.  Unlike
+ * rrtmg_hip_sw: h2ovmr and o3vmr are volume mixing ratios (this driver's
+ * inatm, :1015, 1021), and night columns are computed with the zenith
+ * cosine floored at 1e-10 rather than zeroed (:571-572), as the reference
+ * does.  cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl (112, ncol, nlay),
+ * ciwpmcl, clwpmcl likewise or NULL (zero); reicmcl, relqmcl, iceflgsw,
+ * liqflgsw and the aerosol arrays are not read.  icld outside 0..3 is
+ * taken as 2 (:500); icld = 0 uses no subcolumns (:591-597).  inflgsw !=
+ * 0, iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer,
+                       const double *play, const double *plev, const double *tlay,
+                       const double *tlev, const double *tsfc,
+                       const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                       const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
+                       const double *asdir, const double *asdif, const double *aldir,
+                       const double *aldif, const double *coszen,
+                       double adjes, int dyofyr, double scon,
+                       int inflgsw, int iceflgsw, int liqflgsw,
+                       const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+                       const double *asmcmcl, const double *fsfcmcl,
+                       const double *ciwpmcl, const double *clwpmcl,
+                       const double *reicmcl, const double *relqmcl,
+                       const double *tauaer, const double *ssaaer, const double *asmaer,
+                       const double *ecaer,
+                       double *swuflx, double *swdflx, double *swhr,
+                       double *swuflxc, double *swdflxc, double *swhrc,
                        int ptr_kind, void *stream);
 
 /* Backward (vector-Jacobian product) of rrtmg_lw.  No reference counterpart:

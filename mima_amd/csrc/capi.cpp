@@ -68,7 +68,6 @@ struct SideTables {
     LwBand *bands_lds = nullptr;                 // LW: offsets relative to each band's first k-table element
     LwLdsPlan plan{};                            // LW: schedule of the LDS-table kernel
     bool lds = false;                            // LW: that kernel fits this device
-    uint32_t *xs_jump = nullptr;                 // LW: the KISS 3-shift register's squarings (McICA)
     double cpdair = 0.0;
 };
 
@@ -114,6 +113,7 @@ struct DeviceState {
     std::map<hipStream_t, StreamWork> ws;
     StreamWork &work(hipStream_t st) { return ws[st]; }
     int sw_last_ncol = 0, sw_last_nlay = 0, lw_last_ncol = 0, lw_last_nlay = 0;
+    uint32_t *xs_jump = nullptr;  // the KISS 3-shift register's squarings (McICA subcolumns), on first use
     SwWork sw_w;
     LwWork lw_w;
     // timing events per call kind: 0 SW, 1 LW, 2 SW backward, 3 LW backward
@@ -443,9 +443,6 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     LwBand rebased[16];
     if (!rc && !make_lw_plan(s->lw, s->lwd.plan, rebased, err)) return fail(RRTMG_HIP_ENOENT, err);
     if (!rc) rc = upload(&s->lwd.bands_lds, rebased, sizeof rebased);
-    std::vector<uint32_t> xs(KISS_XS_JUMPS * 32);
-    kiss_xorshift_jumps(xs.data());
-    if (!rc) rc = upload(&s->lwd.xs_jump, xs.data(), xs.size() * sizeof(uint32_t));
     if (rc) return fail(rc, "uploading LW tables failed");
     {
         int ncu = 0, lds = 0;
@@ -555,7 +552,12 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         a.push_back({"tlev", tlev, n3});
         const char *nm1[] = {"tsfc", "asdir", "asdif", "aldir", "aldif", "coszen"};
         for (int i = 0; i < 6; ++i) a.push_back({nm1[i], ins1[i], (size_t)ncol});
-        if (cl) {
+        if (cl && cl->mcica) {
+            const double *opt[] = {cl->cldfmc, cl->taucmc, cl->ssacmc, cl->asmcmc, cl->fsfcmc, cl->ciwpmc, cl->clwpmc};
+            const char *nmo[] = {"cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl"};
+            for (int i = 0; i < 7; ++i)
+                if (opt[i]) a.push_back({nmo[i], opt[i], 112 * n2});
+        } else if (cl) {
             a.push_back({"cldfr", cl->cldfr, n2});
             const double *opt[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
             const char *nmo[] = {"taucld", "ssacld", "asmcld", "fsfcld"};
@@ -572,6 +574,8 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     const size_t o_cfl = off; if (cl) off = align_up(off + sizeof(int) * n2);
     const size_t o_cbf = off; if (cl) off = align_up(off + rs * 14 * 4 * n3);
     const size_t o_cpc = off; if (cl) off = align_up(off + sizeof(int));
+    const bool mcica = cl && cl->mcica;
+    const size_t o_csz = off; if (mcica) off = align_up(off + sizeof(double) * ncol);
     if (W) {
         if ((rc = W->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
         wb = (char *)W->sw_work.p;
@@ -602,7 +606,13 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     SwOutputs out{};
     double *stage = nullptr;
     if (ptr_kind == RRTMG_HIP_HOST) {
-        const size_t ncl = cl ? 57 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) : 0;
+        size_t ncl = 0;
+        if (mcica) {
+            const double *sc[] = {cl->cldfmc, cl->taucmc, cl->ssacmc, cl->asmcmc, cl->fsfcmc, cl->ciwpmc, cl->clwpmc};
+            for (const double *q : sc) ncl += q ? 112 * n2 : 0;
+        } else if (cl) {
+            ncl = 57 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0);
+        }
         const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + ncl + 4 * n3 + 2 * n2);
         if ((rc = W->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
         stage = (double *)W->sw_stage.p;
@@ -625,7 +635,17 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
             *dst1[i] = p;
             p += ncol;
         }
-        if (cl) {
+        if (mcica) {
+            const double *src[] = {cl->cldfmc, cl->taucmc, cl->ssacmc, cl->asmcmc, cl->fsfcmc, cl->ciwpmc, cl->clwpmc};
+            const double **dst[] = {&cld.cldfmc, &cld.taucmc, &cld.ssacmc, &cld.asmcmc, &cld.fsfcmc, &cld.ciwpmc,
+                                    &cld.clwpmc};
+            for (int i = 0; i < 7; ++i) {
+                if (!src[i]) continue;
+                ck |= hipMemcpyAsync(p, src[i], 112 * n2 * sizeof(double), hipMemcpyHostToDevice, st);
+                *dst[i] = p;
+                p += 112 * n2;
+            }
+        } else if (cl) {
             const double *src[] = {cl->cldfr, cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld, cl->cicewp, cl->cliqwp};
             const double **dst[] = {&cld.cldfr, &cld.taucld, &cld.ssacld, &cld.asmcld, &cld.fsfcld, &cld.cicewp,
                                     &cld.cliqwp};
@@ -659,7 +679,16 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     t.heatfac = s->sw.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
     hipError_t e;
-    if (cl) {
+    if (mcica) {
+        // the McICA driver's inatm reads h2o and O3 as volume mixing ratios
+        // and floors the zenith cosine at 1e-10 (rrtmg_sw_rad.f90:571-572,
+        // 1015, 1021)
+        in.vmr = 1;
+        e = launch_sw_coszen_floor(ncol, in.coszen, (double *)(wb + o_csz), st);
+        in.coszen = (const double *)(wb + o_csz);
+        if (e == hipSuccess) e = launch_sw_cloud_flags(in, cld, cw, st);
+        if (e == hipSuccess) e = launch_sw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(0));
+    } else if (cl) {
         e = launch_sw_cloud_flags(in, cld, cw, st);
         if (e == hipSuccess && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
             // the nomcica driver stops on partial cloud in a day column
@@ -976,6 +1005,63 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *pla
                       swdflxc, swhrc, ptr_kind, stream, nullptr, &cl);
 }
 
+int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer, const double *play, const double *plev,
+                       const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                       const double *o3vmr, const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                       const double *o2vmr, const double *asdir, const double *asdif, const double *aldir,
+                       const double *aldif, const double *coszen, double adjes, int dyofyr, double scon,
+                       int inflgsw, int iceflgsw, int liqflgsw, const double *cldfmcl, const double *taucmcl,
+                       const double *ssacmcl, const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+                       const double *clwpmcl, const double *reicmcl, const double *relqmcl, const double *tauaer,
+                       const double *ssaaer, const double *asmaer, const double *ecaer, double *swuflx,
+                       double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind,
+                       void *stream)
+{
+    (void)iceflgsw;
+    (void)liqflgsw;
+    (void)reicmcl;
+    (void)relqmcl;
+    (void)tauaer;
+    (void)ssaaer;
+    (void)asmaer;
+    (void)ecaer;
+    // rrtmg_sw_rad.f90:500: icld outside 0..3 is taken as 2
+    if (icld < 0 || icld > 3) icld = 2;
+    SwClouds cl{};
+    cl.mcica = 1;
+    // icld = 0: no subcolumns (rrtmg_sw_rad.f90:591-597, 954)
+    if (icld != 0) {
+        if (inflgsw != 0)
+            return fail(RRTMG_HIP_ENOTSUP, "inflgsw != 0: cloud optics from water paths and effective radii are not "
+                                           "implemented; pass the subcolumns' optics with inflgsw = 0");
+        const double *arr[] = {cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
+        const char *nm[] = {"cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl"};
+        if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
+            const size_t n = (size_t)ncol * nlay * 112;
+            int rc;
+            if (ptr_kind == RRTMG_HIP_HOST) {
+                for (int i = 0; i < 5; ++i)
+                    if ((rc = check_host_array(arr[i], n, nm[i]))) return rc;
+                if (ciwpmcl && (rc = check_host_array(ciwpmcl, n, "ciwpmcl"))) return rc;
+                if (clwpmcl && (rc = check_host_array(clwpmcl, n, "clwpmcl"))) return rc;
+            } else {
+                for (int i = 0; i < 5; ++i)
+                    if (!arr[i]) return fail(RRTMG_HIP_EINVAL, "null cloud array");
+            }
+        }
+        cl.cldfmc = cldfmcl;
+        cl.taucmc = taucmcl;
+        cl.ssacmc = ssacmcl;
+        cl.asmcmc = asmcmcl;
+        cl.fsfcmc = fsfcmcl;
+        cl.ciwpmc = ciwpmcl;
+        cl.clwpmc = clwpmcl;
+    }
+    return sw_forward(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                      o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr, swuflxc,
+                      swdflxc, swhrc, ptr_kind, stream, nullptr, &cl);
+}
+
 int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv, const double *play, const double *plev, const double *tlay,
                  const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
                  const double *co2vmr, const double *ch4vmr, const double *n2ovmr, const double *o2vmr,
@@ -1074,15 +1160,19 @@ int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv, const double *pla
                       hrc, ptr_kind, stream, true);
 }
 
-int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
-                              const double *play, const double *cldfrac, const double *ciwp, const double *clwp,
-                              const double *rei, const double *rel, const double *tauc, double *cldfmcl,
-                              double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
-                              double *taucmcl, int ptr_kind, void *stream)
+// rrtmg_hip_mcica_subcol_lw (nsub 140) and rrtmg_hip_mcica_subcol_sw (nsub
+// 112, with the single-scattering optics)
+static int mcica_subcol(const char *who, int nsub, int ncol, int nlay, int icld, int permuteseed, int irng,
+                        const double *play, const double *cldfrac, const double *ciwp, const double *clwp,
+                        const double *rei, const double *rel, const double *tauc, const double *ssac,
+                        const double *asmc, const double *fsfc, double *cldfmcl, double *ciwpmcl, double *clwpmcl,
+                        double *reicmcl, double *relqmcl, double *taucmcl, double *ssacmcl, double *asmcmcl,
+                        double *fsfcmcl, int ptr_kind, void *stream)
 {
-    (void)iplon;
-    // mcica_subcol_gen_lw.f90:119-123: clear sky returns at once, icld
-    // outside 0..3 stops
+    const bool sw = nsub == 112;
+    const int nband = sw ? 14 : 16;
+    // mcica_subcol_gen_lw.f90:119-123 (SW :115-119): clear sky returns at
+    // once, icld outside 0..3 stops
     if (icld == 0) return 0;
     if (icld < 0 || icld > 3) return fail(RRTMG_HIP_EINVAL, "invalid icld (0..3)");
     if (irng != 0)
@@ -1093,16 +1183,25 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
     if (nlay < 4 || nlay > RRTMG_HIP_MAX_NLAY) return fail(RRTMG_HIP_EINVAL, "nlay out of range [4, 200]");
     if (ptr_kind != RRTMG_HIP_HOST && ptr_kind != RRTMG_HIP_DEVICE) return fail(RRTMG_HIP_EINVAL, "bad ptr_kind");
     if (!play || !cldfrac || !tauc || !cldfmcl || !taucmcl) return fail(RRTMG_HIP_EINVAL, "null array argument");
+    if (sw && (!ssac || !asmc || !fsfc || !ssacmcl || !asmcmcl || !fsfcmcl))
+        return fail(RRTMG_HIP_EINVAL, "null array argument");
     DeviceState *s = state();
     if (!s) return fail(RRTMG_HIP_EDEVICE, "no HIP device");
     std::lock_guard<std::mutex> lk(s->mu);
-    if (!s->lwd.ready) return fail(RRTMG_HIP_ENOINIT, "rrtmg_hip_mcica_subcol_lw called before rrtmg_hip_lw_init");
-    const size_t n2 = (size_t)ncol * nlay, ns = n2 * 140;
-    const double *ins[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc};
-    const char *nm[] = {"play", "cldfrac", "ciwp", "clwp", "rei", "rel", "tauc"};
-    const size_t len[] = {n2, n2, n2, n2, n2, n2, 16 * n2};
-    double *outs[] = {cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl};
-    const size_t olen[] = {ns, ns, ns, n2, n2, ns};
+    if (!s->xs_jump) {
+        std::vector<uint32_t> xs(KISS_XS_JUMPS * 32);
+        kiss_xorshift_jumps(xs.data());
+        if (upload(&s->xs_jump, xs.data(), xs.size() * sizeof(uint32_t)))
+            return fail(RRTMG_HIP_EDEVICE, std::string(who) + ": uploading the generator's jump tables failed");
+    }
+    const size_t n2 = (size_t)ncol * nlay, ns = n2 * nsub, nb = n2 * nband;
+    const double *ins[] = {play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc};
+    const char *nm[] = {"play", "cldfrac", "ciwp", "clwp", "rei", "rel", "tauc", "ssac", "asmc", "fsfc"};
+    const size_t len[] = {n2, n2, n2, n2, n2, n2, nb, nb, nb, nb};
+    const int nin = sw ? 10 : 7;
+    double *outs[] = {cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
+    const size_t olen[] = {ns, ns, ns, n2, n2, ns, ns, ns, ns};
+    const int nout = sw ? 9 : 6;
     int rc;
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
@@ -1113,29 +1212,31 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
     a.nlay = nlay;
     a.icld = icld;
     a.permuteseed = permuteseed;
+    a.nsub = nsub;
     int ck = 0;
+    const double **din[] = {&a.play, &a.cldfrac, &a.ciwp, &a.clwp, &a.rei, &a.rel, &a.tauc, &a.ssac, &a.asmc, &a.fsfc};
+    double **dout[] = {&a.cldfmcl, &a.ciwpmcl, &a.clwpmcl, &a.reicmcl, &a.relqmcl, &a.taucmcl,
+                       &a.ssacmcl, &a.asmcmcl, &a.fsfcmcl};
     if (ptr_kind == RRTMG_HIP_HOST) {
-        for (int i = 0; i < 7; ++i)
+        for (int i = 0; i < nin; ++i)
             if (ins[i] && (rc = check_host_array(ins[i], len[i], nm[i]))) return rc;
         for (int i = 0; i < ncol; ++i)
             if (play[i] * 1.e2 < play[(size_t)ncol + i] * 1.e2)
                 return fail(RRTMG_HIP_EINVAL, "play must not rise from the first layer to the second (the KISS "
                                               "seeds read the four lowest layers)");
         size_t need = 0;
-        for (int i = 0; i < 7; ++i) need += ins[i] ? len[i] : 0;
-        for (int i = 0; i < 6; ++i) need += outs[i] ? olen[i] : 0;
+        for (int i = 0; i < nin; ++i) need += ins[i] ? len[i] : 0;
+        for (int i = 0; i < nout; ++i) need += outs[i] ? olen[i] : 0;
         if ((rc = W.lw_stage.ensure(jump_bytes + need * sizeof(double))))
             return fail(rc, "device staging allocation failed");
         double *p = (double *)((char *)W.lw_stage.p + jump_bytes);
-        const double **din[] = {&a.play, &a.cldfrac, &a.ciwp, &a.clwp, &a.rei, &a.rel, &a.tauc};
-        for (int i = 0; i < 7; ++i) {
+        for (int i = 0; i < nin; ++i) {
             if (!ins[i]) continue;
             ck |= hipMemcpyAsync(p, ins[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
             *din[i] = p;
             p += len[i];
         }
-        double **dout[] = {&a.cldfmcl, &a.ciwpmcl, &a.clwpmcl, &a.reicmcl, &a.relqmcl, &a.taucmcl};
-        for (int i = 0; i < 6; ++i) {
+        for (int i = 0; i < nout; ++i) {
             if (!outs[i]) continue;
             *dout[i] = p;
             p += olen[i];
@@ -1144,35 +1245,57 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
         if ((rc = W.lw_stage.ensure(jump_bytes))) return fail(rc, "device scratch allocation failed");
         if (g_checks) {
             std::vector<NamedArray> v;
-            for (int i = 0; i < 7; ++i)
+            for (int i = 0; i < nin; ++i)
                 if (ins[i]) v.push_back({nm[i], ins[i], len[i]});
             if ((rc = device_checks(W, st, v, nullptr, ncol, nlay))) return rc;
             a.bad = (int *)W.flag.p + 16;
         }
-        a.play = play; a.cldfrac = cldfrac; a.ciwp = ciwp; a.clwp = clwp; a.rei = rei; a.rel = rel; a.tauc = tauc;
-        a.cldfmcl = cldfmcl; a.ciwpmcl = ciwpmcl; a.clwpmcl = clwpmcl; a.reicmcl = reicmcl; a.relqmcl = relqmcl;
-        a.taucmcl = taucmcl;
+        for (int i = 0; i < nin; ++i) *din[i] = ins[i];
+        for (int i = 0; i < nout; ++i) *dout[i] = outs[i];
     }
-    if (ck) return fail(RRTMG_HIP_EDEVICE, "McICA subcolumns: host-to-device copy failed");
-    hipError_t e = launch_lw_mcica_subcol(a, (const LwBand *)s->lwd.bands, s->lwd.xs_jump,
-                                          (McicaJump *)W.lw_stage.p, st);
-    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumn launch: ") + hipGetErrorString(e));
+    if (ck) return fail(RRTMG_HIP_EDEVICE, std::string(who) + ": host-to-device copy failed");
+    hipError_t e = launch_mcica_subcol(a, s->xs_jump, (McicaJump *)W.lw_stage.p, st);
+    if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string(who) + " launch: " + hipGetErrorString(e));
     if (ptr_kind == RRTMG_HIP_HOST) {
-        const double *src[] = {a.cldfmcl, a.ciwpmcl, a.clwpmcl, a.reicmcl, a.relqmcl, a.taucmcl};
-        for (int i = 0; i < 6; ++i)
-            if (outs[i]) ck |= hipMemcpyAsync(outs[i], src[i], olen[i] * sizeof(double), hipMemcpyDeviceToHost, st);
+        for (int i = 0; i < nout; ++i)
+            if (outs[i]) ck |= hipMemcpyAsync(outs[i], *dout[i], olen[i] * sizeof(double), hipMemcpyDeviceToHost, st);
         e = hipStreamSynchronize(st);
-        if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumns: ") + hipGetErrorString(e));
+        if (e != hipSuccess || ck) return fail(RRTMG_HIP_EDEVICE, std::string(who) + ": " + hipGetErrorString(e));
     } else if (a.bad) {
         int nbad = 0;
         e = hipMemcpyAsync(&nbad, a.bad, sizeof(int), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("McICA subcolumns: ") + hipGetErrorString(e));
+        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string(who) + ": " + hipGetErrorString(e));
         if (nbad)
             return fail(RRTMG_HIP_EINVAL, "play must not rise from the first layer to the second (the KISS seeds "
                                           "read the four lowest layers); those columns are left unwritten");
     }
     return 0;
+}
+
+int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
+                              const double *play, const double *cldfrac, const double *ciwp, const double *clwp,
+                              const double *rei, const double *rel, const double *tauc, double *cldfmcl,
+                              double *ciwpmcl, double *clwpmcl, double *reicmcl, double *relqmcl,
+                              double *taucmcl, int ptr_kind, void *stream)
+{
+    (void)iplon;
+    return mcica_subcol("McICA LW subcolumns", 140, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp,
+                        rei, rel, tauc, nullptr, nullptr, nullptr, cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl,
+                        taucmcl, nullptr, nullptr, nullptr, ptr_kind, stream);
+}
+
+int rrtmg_hip_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permuteseed, int irng,
+                              const double *play, const double *cldfrac, const double *ciwp, const double *clwp,
+                              const double *rei, const double *rel, const double *tauc, const double *ssac,
+                              const double *asmc, const double *fsfc, double *cldfmcl, double *ciwpmcl,
+                              double *clwpmcl, double *reicmcl, double *relqmcl, double *taucmcl, double *ssacmcl,
+                              double *asmcmcl, double *fsfcmcl, int ptr_kind, void *stream)
+{
+    (void)iplon;
+    return mcica_subcol("McICA SW subcolumns", 112, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp,
+                        rei, rel, tauc, ssac, asmc, fsfc, cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, taucmcl,
+                        ssacmcl, asmcmcl, fsfcmcl, ptr_kind, stream);
 }
 
 // backward of rrtmg_sw; with swuflx/swdflx/swhr given, also the forward's

@@ -114,12 +114,16 @@ struct LwClouds {
     const double *cldfmc, *taucmc;    // McICA subcolumns [nlay][ncol][140]: the reference's (ngptlw, ncol, nlay)
 };
 constexpr int LW_MCICA = 4;  // LwClouds::icld of the McICA driver's transfer
-// McICA subcolumns of mcica_subcol_lw with the KISS generator: inputs
-// [nlay][ncol] (tauc [nlay][ncol][16]), subcolumn outputs [nlay][ncol][140]
+// McICA subcolumns of mcica_subcol_lw / mcica_subcol_sw with the KISS
+// generator: inputs [nlay][ncol] (tauc, ssac, asmc, fsfc [nlay][ncol][16 or
+// 14]), subcolumn outputs [nlay][ncol][nsub]
 struct McicaSubcolArgs {
     int ncol, nlay, icld, permuteseed;
+    int nsub;  // 140 (LW) or 112 (SW)
     const double *play, *cldfrac, *ciwp, *clwp, *rei, *rel, *tauc;  // ciwp, clwp, rei, rel may be NULL (zero)
+    const double *ssac, *asmc, *fsfc;                               // SW only
     double *cldfmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl, *taucmcl;  // only cldfmcl and taucmcl required
+    double *ssacmcl, *asmcmcl, *fsfcmcl;                                // SW only, required there
     int *bad;  // columns whose pressure rises from the first layer to the second; NULL: not counted
 };
 // jump constants of one subcolumn's skip over the generator's sequence
@@ -131,9 +135,8 @@ struct McicaJump {
 constexpr int KISS_XS_JUMPS = 34;  // squarings M^(2^k) of the 3-shift register's matrix, k < 34
 // the KISS_XS_JUMPS x 32 words of those squarings (column j of M^(2^k) at [k*32 + j]); host code
 void kiss_xorshift_jumps(uint32_t *t);
-// jump: device scratch of 140 McicaJump; xs_jump: the squarings on the device
-hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, const uint32_t *xs_jump,
-                                  McicaJump *jump, hipStream_t s);
+// jump: device scratch of nsub McicaJump; xs_jump: the squarings on the device
+hipError_t launch_mcica_subcol(const McicaSubcolArgs &a, const uint32_t *xs_jump, McicaJump *jump, hipStream_t s);
 // Per-(layer, column) record of the cloud column kernel, in the band
 // kernel's precision: the layer's fraction, then rtrnmr's six upward
 // overlap factors of index lay+1 and its six downward ones of index lay-1
@@ -170,6 +173,10 @@ struct SwClouds {
     const double *cldfr;                              // [nlay][ncol]
     const double *taucld, *ssacld, *asmcld, *fsfcld;  // [nlay][ncol][14]: the reference's (nbndsw, ncol, nlay)
     const double *cicewp, *cliqwp;                    // [nlay][ncol] or NULL (zero)
+    // McICA (spcvmc_sw): per-g-point subcolumns [nlay][ncol][112] in place
+    // of the fields above (water paths NULL for zero)
+    int mcica;
+    const double *cldfmc, *taucmc, *ssacmc, *asmcmc, *fsfcmc, *ciwpmc, *clwpmc;
 };
 // flag bits per (layer, column): a fraction above 0 (the total stream
 // blends the layer's cloudy optics), above 1e-12 (reftra of the cloudy
@@ -190,6 +197,8 @@ __host__ __device__ inline size_t sw_cloud_row_bytes(int nlay, size_t real_size)
     return real_size * (4 * (size_t)(nlay + 1) + (real_size == 4 ? 5 * (size_t)nlay : 0)) * 64;
 }
 // flags (and the partial-cloud count, zeroed first) of a cloudy SW call
+// McICA: o[i] = max(c[i], 1e-10) (the driver's zenith-cosine floor)
+hipError_t launch_sw_coszen_floor(int ncol, const double *c, double *o, hipStream_t s);
 hipError_t launch_sw_cloud_flags(const ColumnInputs &in, const SwClouds &cl, const SwCloudWork &cw, hipStream_t s);
 // the rest of the call (mode 0 FP64, 1 FP32 with FP64 islands): the
 // clear-sky prep and column kernels, the cloudy band kernel, the finish

@@ -180,207 +180,7 @@ __global__ __launch_bounds__(64) void lw_cloud_column_kernel(int ncol, int nlay,
 }
 
 // ---------------------------------------------------------------------------
-// McICA (the reference's rrtmg_lw_rad.f90 driver with mcica_subcol_gen_lw.f90)
-
-// One step of the KISS generator (mcica_subcol_gen_lw.f90:530-562): a
-// congruential, a 3-shift register and two multiply-with-carry generators
-// in 32-bit wrap-around arithmetic, their sum read as a signed integer and
-// mapped to (0, 1) by a product and a sum rounded separately (no fused
-// multiply-add), as the reference computes it.
-__device__ __forceinline__ double kiss_step(uint32_t &s1, uint32_t &s2, uint32_t &s3, uint32_t &s4)
-{
-#pragma clang fp contract(off)
-    s1 = 69069u * s1 + 1327217885u;
-    uint32_t k = s2;
-    k ^= k << 13;
-    k ^= k >> 17;
-    k ^= k << 5;
-    s2 = k;
-    s3 = 18000u * (s3 & 65535u) + (s3 >> 16);
-    s4 = 30903u * (s4 & 65535u) + (s4 >> 16);
-    const uint32_t sum = s1 + s2 + (s3 << 16) + s4;
-    return (double)(int32_t)sum * 2.328306e-10 + 0.5;
-}
-
-// Jump-ahead of the KISS generator: each of its four parts is linear, so
-// the state d steps on follows from the seed without the d steps between.
-//   congruential s1 -> 69069 s1 + 1327217885 (mod 2^32): affine, its d-th
-//     power A s1 + C by repeated squaring;
-//   3-shift register: linear over GF(2), the d-th power of its 32 x 32 bit
-//     matrix as a product of the squarings M^(2^k) (KISS_XS_JUMPS of them);
-//   multiply-with-carry s -> a (s mod 2^16) + s / 2^16: with m = a 2^16 - 1,
-//     2^16 s' = s (mod m), so s' = a s (mod m) and s_d = a^d s (mod m); from
-//     the second step on the state lies in [0, m), except the fixed point m
-//     reached from a nonzero multiple of m.
-constexpr uint64_t MWC3_A = 18000u, MWC4_A = 30903u;
-constexpr uint64_t MWC3_M = MWC3_A * 65536u - 1u, MWC4_M = MWC4_A * 65536u - 1u;
-
-void kiss_xorshift_jumps(uint32_t *t)
-{
-    auto xs = [](uint32_t k) {
-        k ^= k << 13;
-        k ^= k >> 17;
-        k ^= k << 5;
-        return k;
-    };
-    auto apply = [](const uint32_t *m, uint32_t v) {
-        uint32_t r = 0;
-        for (int j = 0; j < 32; ++j)
-            if ((v >> j) & 1u) r ^= m[j];
-        return r;
-    };
-    for (int j = 0; j < 32; ++j) t[j] = xs(1u << j);
-    for (int k = 1; k < KISS_XS_JUMPS; ++k)
-        for (int j = 0; j < 32; ++j) t[k * 32 + j] = apply(t + (k - 1) * 32, t[(k - 1) * 32 + j]);
-}
-
-__device__ __forceinline__ uint64_t mulmod(uint64_t x, uint64_t y, uint64_t m) { return x * y % m; }
-
-__device__ uint64_t powmod(uint64_t a, uint64_t e, uint64_t m)
-{
-    uint64_t r = 1 % m;
-    a %= m;
-    for (; e; e >>= 1) {
-        if (e & 1u) r = mulmod(r, a, m);
-        a = mulmod(a, a, m);
-    }
-    return r;
-}
-
-// the generator steps before subcolumn ig's first number: permuteseed, then
-// nlay numbers per earlier subcolumn (one for maximum overlap)
-__device__ __forceinline__ uint64_t mcica_skip(int permuteseed, int icld, int nlay, int ig)
-{
-    return (uint64_t)(permuteseed > 0 ? permuteseed : 0) + (uint64_t)ig * (uint64_t)(icld == 3 ? 1 : nlay);
-}
-
-// per subcolumn (one thread each): the jump constants of its skip and its band
-__global__ __launch_bounds__(192) void lw_mcica_jump_kernel(int nlay, int icld, int permuteseed,
-                                                            const LwBand *__restrict__ bands, McicaJump *jump)
-{
-    const int ig = threadIdx.x;
-    if (ig >= 140) return;
-    const uint64_t d = mcica_skip(permuteseed, icld, nlay, ig);
-    uint32_t A = 1u, C = 0u, a = 69069u, c = 1327217885u;
-    for (uint64_t e = d; e; e >>= 1) {
-        if (e & 1u) {
-            A = a * A;
-            C = a * C + c;
-        }
-        c = a * c + c;
-        a = a * a;
-    }
-    int b = 0;
-    while (ig >= bands[b].ngs + bands[b].ng) ++b;
-    McicaJump J;
-    J.a1 = A;
-    J.c1 = C;
-    J.p3 = (uint32_t)powmod(MWC3_A, d, MWC3_M);
-    J.p4 = (uint32_t)powmod(MWC4_A, d, MWC4_M);
-    J.band = b;
-    jump[ig] = J;
-}
-
-// s_d of a multiply-with-carry part from s_0 (p = a^d mod m)
-__device__ __forceinline__ uint32_t mwc_jump(uint32_t s, uint64_t d, uint64_t p, uint64_t a, uint64_t m)
-{
-    if (d < 2) {
-        for (uint64_t n = 0; n < d; ++n) s = (uint32_t)a * (s & 65535u) + (s >> 16);
-        return s;
-    }
-    const uint32_t r = (uint32_t)mulmod(p, s % m, m);
-    return r == 0 && s != 0 ? (uint32_t)m : r;
-}
-
-// mcica_subcol_lw with the KISS generator (mcica_subcol_gen_lw.f90:48-152,
-// generate_stochastic_clouds :158-520).  One thread per (column, subcolumn)
-// in g-point-fastest order, so a wave's stores of a layer are contiguous:
-// the thread seeds the generator from the fractional Pa of its column's four
-// lowest layers, jumps over the numbers that precede its subcolumn
-// (permuteseed, then nlay per earlier subcolumn, or one for maximum
-// overlap), and draws its subcolumn's numbers in layer order: one per layer
-// for random (1) and maximum-random (2) overlap, the latter carried up
-// through cloudy layers; one for all layers for maximum overlap (3).  Cloudy
-// where the number >= 1 - fraction, with the band's depth and the layer's
-// water paths.  A column whose second layer has the higher pressure (the
-// reference stops, :327-329) is counted in *bad (when given) and left
-// unwritten.  NULL water paths or radii read as zero; NULL outputs other
-// than cldfmcl and taucmcl are skipped.  Contraction is off: the reference
-// rounds every product and sum on its own, and a fused play * 100 -
-// int(play * 100) would keep bits it rounds away and change the seed.
-__global__ __launch_bounds__(256) void lw_mcica_subcol_kernel(McicaSubcolArgs a, const McicaJump *__restrict__ jump,
-                                                              const uint32_t *__restrict__ xs_jump)
-{
-#pragma clang fp contract(off)
-    __shared__ uint32_t s_xs[KISS_XS_JUMPS * 32];
-    for (int i = threadIdx.x; i < KISS_XS_JUMPS * 32; i += blockDim.x) s_xs[i] = xs_jump[i];
-    __syncthreads();
-    const int ncol = a.ncol, nlay = a.nlay, icld = a.icld;
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)ncol * 140) return;
-    const int col = (int)(t / 140), ig = (int)(t - (size_t)col * 140);
-    if (a.play[col] * 1.e2 < a.play[(size_t)ncol + col] * 1.e2) {
-        if (a.bad && ig == 0) atomicAdd(a.bad, 1);
-        return;
-    }
-    // seeds (:330-333)
-    uint32_t s[4];
-    for (int k = 0; k < 4; ++k) {
-        const double p = a.play[(size_t)k * ncol + col] * 1.e2;
-        s[k] = (uint32_t)(int32_t)((p - (double)(int32_t)p) * 1000000000.0);
-    }
-    const McicaJump J = jump[ig];
-    const uint64_t d = mcica_skip(a.permuteseed, icld, nlay, ig);
-    s[0] = J.a1 * s[0] + J.c1;
-    uint32_t v = s[1];
-    for (int k = 0; k < KISS_XS_JUMPS && (d >> k); ++k) {
-        if (!((d >> k) & 1u)) continue;
-        uint32_t r = 0;
-        for (int j = 0; j < 32; ++j)
-            if ((v >> j) & 1u) r ^= s_xs[k * 32 + j];
-        v = r;
-    }
-    s[1] = v;
-    s[2] = mwc_jump(s[2], d, J.p3, MWC3_A, MWC3_M);
-    s[3] = mwc_jump(s[3], d, J.p4, MWC4_A, MWC4_M);
-    if (ig == 0) {
-        for (int l = 0; l < nlay; ++l) {
-            const size_t ix = (size_t)l * ncol + col;
-            if (a.reicmcl) a.reicmcl[ix] = a.rei ? a.rei[ix] : 0.0;
-            if (a.relqmcl) a.relqmcl[ix] = a.rel ? a.rel[ix] : 0.0;
-        }
-    }
-    const int b = J.band;
-    const double r3 = icld == 3 ? kiss_step(s[0], s[1], s[2], s[3]) : 0.0;
-    double prev = 0.0, fprev = 0.0;
-    for (int l = 0; l < nlay; ++l) {
-        double x = icld == 3 ? r3 : kiss_step(s[0], s[1], s[2], s[3]);
-        const size_t ix = (size_t)l * ncol + col, o = ix * 140 + ig;
-        double f = a.cldfrac[ix];
-        if (f < 1.e-20) f = 0.0;
-        if (icld == 2 && l > 0) x = prev > 1.0 - fprev ? prev : x * (1.0 - fprev);
-        prev = x;
-        fprev = f;
-        const bool cloudy = x >= 1.0 - f;
-        a.cldfmcl[o] = cloudy ? 1.0 : 0.0;
-        a.taucmcl[o] = cloudy ? a.tauc[ix * 16 + b] : 0.0;
-        if (a.ciwpmcl) a.ciwpmcl[o] = cloudy && a.ciwp ? a.ciwp[ix] : 0.0;
-        if (a.clwpmcl) a.clwpmcl[o] = cloudy && a.clwp ? a.clwp[ix] : 0.0;
-    }
-}
-
-__global__ void lw_mcica_zero_kernel(int *bad) { *bad = 0; }
-
-hipError_t launch_lw_mcica_subcol(const McicaSubcolArgs &a, const LwBand *bands, const uint32_t *xs_jump,
-                                  McicaJump *jump, hipStream_t s)
-{
-    if (a.bad) hipLaunchKernelGGL(lw_mcica_zero_kernel, dim3(1), dim3(1), 0, s, a.bad);
-    hipLaunchKernelGGL(lw_mcica_jump_kernel, dim3(1), dim3(192), 0, s, a.nlay, a.icld, a.permuteseed, bands, jump);
-    const size_t n = (size_t)a.ncol * 140;
-    hipLaunchKernelGGL(lw_mcica_subcol_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a,
-                       (const McicaJump *)jump, xs_jump);
-    return hipGetLastError();
-}
+// McICA transfer (rtrnmc); the subcolumn generator is in mcica.hip
 
 // the McICA transfer's layer flag (rrtmg_lw_rtrnmc.f90:297-313): cloudy
 // where any g-point's subcolumn is.  A block reads the 140 fractions of

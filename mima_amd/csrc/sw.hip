@@ -45,12 +45,15 @@ __global__ __launch_bounds__(256) void sw_prep_kernel(ColumnInputs in, SwWork w,
         double pavel = in.play[ix], tavel = in.tlay[ix];
         double pz0 = in.plev[ix], pz1 = in.plev[ix + ncol];
         double h2o = in.h2o[ix];
-        double wkl1 = (h2o / (1.0 - h2o)) * amdw;
+        // in.vmr: h2o and O3 as volume mixing ratios (the McICA driver's
+        // inatm, rrtmg_sw_rad.f90:1015, 1021), else MiMA's specific
+        // humidity and O3 mass mixing ratio (rrtmg_sw_rad.nomcica.f90:1004, 1007)
+        double wkl1 = in.vmr ? h2o : (h2o / (1.0 - h2o)) * amdw;
         double amm = (1.0 - wkl1) * amd + wkl1 * amw;
         double coldry = (pz0 - pz1) * 1.e3 * avogad / (1.e2 * grav * amm * (1.0 + wkl1));
         double colh2o = 1.e-20 * (coldry * wkl1);
         double colco2 = 1.e-20 * (coldry * in.co2[ix]);
-        double colo3 = 1.e-20 * (coldry * (in.o3[ix] * amdo));
+        double colo3 = 1.e-20 * (coldry * (in.vmr ? in.o3[ix] : in.o3[ix] * amdo));
         double colch4 = 1.e-20 * (coldry * in.ch4[ix]);
         double colo2 = 1.e-20 * (coldry * in.o2[ix]);
         double colmol = 1.e-20 * coldry + colh2o;

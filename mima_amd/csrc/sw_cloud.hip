@@ -18,6 +18,13 @@
 // eigenvalue in FP32 loses its digits).  Each sweep carries the recurrences
 // of both streams (the scratch row holds prup/prupd of each), and each level
 // sums four fluxes over g.  The clear-sky kernels are left unchanged.
+//
+// McICA (rrtmg_sw_rad.f90 with cldprmc_sw and spcvmc_sw): spcvmc_sw is
+// spcvrt_sw with the fraction and cloud optics of each g-point's subcolumn
+// in place of the layer's and the band's (rrtmg_sw_spcvmc.f90:302-600), so
+// the same band kernel runs it with each lane reading its own g-point's
+// subcolumn (MC = true); the layer flag then only says that some subcolumn
+// is cloudy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +57,44 @@ __global__ __launch_bounds__(256) void sw_cloud_flags_kernel(ColumnInputs in, Sw
 }
 
 __global__ void sw_cloud_zero_kernel(int *p) { *p = 0; }
+
+// McICA: SWC_BLEND where some g-point's subcolumn has cloud.  A block reads
+// the 112 fractions of SWMC_FLAG_PAIRS consecutive (layer, column) pairs as
+// one contiguous run and marks the pairs in LDS.
+constexpr int SWMC_FLAG_PAIRS = 32;
+__global__ __launch_bounds__(256) void sw_mcica_flags_kernel(int ncol, int nlay, SwClouds cl, SwCloudWork cw)
+{
+    __shared__ int s_fl[SWMC_FLAG_PAIRS];
+    const size_t n2 = (size_t)nlay * ncol, p0 = (size_t)blockIdx.x * SWMC_FLAG_PAIRS;
+    if (threadIdx.x < SWMC_FLAG_PAIRS) s_fl[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t np = n2 - p0 < (size_t)SWMC_FLAG_PAIRS ? n2 - p0 : (size_t)SWMC_FLAG_PAIRS;
+    const double *f = cl.cldfmc + p0 * 112;
+    for (int i = threadIdx.x; i < (int)(np * 112); i += blockDim.x)
+        if (f[i] > 0.0) s_fl[i / 112] = SWC_BLEND;
+    __syncthreads();
+    if (threadIdx.x < np) cw.flags[p0 + threadIdx.x] = s_fl[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void sw_flags_clear_kernel(size_t n, int *flags)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[i] = 0;
+}
+
+// the McICA driver floors the zenith cosine at 1e-10 and computes night
+// columns there instead of zeroing them (rrtmg_sw_rad.f90:571-572)
+__global__ __launch_bounds__(256) void sw_coszen_floor_kernel(int ncol, const double *__restrict__ c, double *o)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncol) o[i] = c[i] < 1.e-10 ? 1.e-10 : c[i];
+}
+
+hipError_t launch_sw_coszen_floor(int ncol, const double *c, double *o, hipStream_t s)
+{
+    hipLaunchKernelGGL(sw_coszen_floor_kernel, dim3((ncol + 255) / 256), dim3(256), 0, s, ncol, c, o);
+    return hipGetLastError();
+}
 
 // PIFM two-stream reflectance and transmittance with asymmetry g, kmodts = 2
 // (rrtmg_sw_reftra.f90:138-303), FP64; the exp table of the mode (values
@@ -158,16 +203,31 @@ __device__ __forceinline__ SwLayerRT<Real> sw_cloudy_blend(double ztauc, double 
 // the clear-sky kernel forms them and, for a layer with cloud, their blend
 // with the cloudy ones (cldprop_sw's delta-M scaling of the input optics,
 // rrtmg_sw_cldprop.f90:143-156, where the layer passes its cloud test).
-template <int JB, typename Real>
+// McICA (MC): the lane's subcolumn o = (layer, column, g-point) instead of
+// the layer and band; cldprmc_sw scales the optics of a subcolumn that
+// passes its cloud test and leaves any other's as input
+// (rrtmg_sw_cldprmc.f90:127-143), where cldprop_sw sets depth 0, albedo 1.
+template <int JB, bool MC, typename Real>
 __device__ __forceinline__ void sw_cloud_layer(Real taug, Real taur, Real prmu0, const Real *__restrict__ exp_tbl,
-                                               int fl, double cf, const SwClouds &cl, size_t ix, SwLayerRT<Real> &c,
-                                               SwLayerRT<Real> &t)
+                                               int fl, double cf, const SwClouds &cl, size_t ix, size_t o,
+                                               SwLayerRT<Real> &c, SwLayerRT<Real> &t)
 {
     sw_layer_optics(taug, taur, prmu0, exp_tbl, c.r, c.rd, c.t, c.td, c.dbt);
     t = c;
     if (!(fl & SWC_BLEND)) return;
     double pt = 0.0, pw = 1.0, pg = 0.0;
-    if (fl & SWC_OPTICS) {
+    if constexpr (MC) {
+        pt = cl.taucmc[o];
+        pw = cl.ssacmc[o];
+        pg = cl.asmcmc[o];
+        if (fl & SWC_OPTICS) {
+            const double ffp = cl.fsfcmc[o];
+            const double ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * pw;
+            pw = ffp1 * pw / ffpssa;
+            pt = ffpssa * pt;
+            pg = (pg - ffp) / ffp1;
+        }
+    } else if (fl & SWC_OPTICS) {
         const size_t j = ix * 14 + (JB - 16);
         const double tauc = cl.taucld[j], ssac = cl.ssacld[j], asmc = cl.asmcld[j], ffp = cl.fsfcld[j];
         const double ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssac;
@@ -214,7 +274,7 @@ struct SwStream {
 // [4][nlay+1][64] (prup, prupd of the clear stream, then of the total one;
 // FP32: then the kept cloudy-layer values);
 // s_red: 64 LDS entries for the FP64 g-sum.
-template <int JB, typename Real>
+template <int JB, bool MC, typename Real>
 __device__ __forceinline__ void sw_cloud_band_body(const ColumnInputs &in, const SwWork &w, const SwClouds &cl,
                                                    const SwCloudWork &cw, const SwBand &B, const Real *__restrict__ ktab,
                                                    const Real *__restrict__ exp_tbl, WaveUnit u, int tile, int lane,
@@ -264,8 +324,21 @@ __device__ __forceinline__ void sw_cloud_band_body(const ColumnInputs &in, const
         sw_load_layer(c, coef, w.ipk, lay - 1, col, ncol, nlay);
         Real taug, taur;
         sw_gas(JB, Bc, T, c, lay <= laytrop, taug, taur);
-        const size_t ix = (size_t)(lay - 1) * ncol + col;
-        const int fl = cw.flags[ix];
+        const size_t ix = (size_t)(lay - 1) * ncol + col, o = ix * 112 + Bc.ngs + g;
+        int fl = cw.flags[ix];
+        double cf = 0.0;
+        if constexpr (MC) {
+            // the lane's subcolumn: the tests of spcvmc_sw (:388, 502-549) and
+            // cldprmc_sw (rrtmg_sw_cldprmc.f90:127-129) on its own fraction
+            if (fl & SWC_BLEND) {
+                cf = cl.cldfmc[o];
+                const double cwp = (cl.ciwpmc ? cl.ciwpmc[o] : 0.0) + (cl.clwpmc ? cl.clwpmc[o] : 0.0);
+                fl = (cf > 0.0 ? SWC_BLEND : 0) | (cf > 1.e-12 ? SWC_REFTRA : 0) |
+                     (cf >= 1.e-20 && (cwp >= 1.e-20 || cl.taucmc[o] >= 1.e-20) ? SWC_OPTICS : 0);
+            }
+        } else {
+            cf = (fl & SWC_BLEND) ? cl.cldfr[ix] : 0.0;
+        }
         const unsigned k = (unsigned)((lay - 1) * 64 + lane);
         if (keep && !first && (fl & SWC_BLEND)) {
             sw_layer_optics(taug, taur, prmu0, exp_tbl, lc.r, lc.rd, lc.t, lc.td, lc.dbt);
@@ -276,8 +349,7 @@ __device__ __forceinline__ void sw_cloud_band_body(const ColumnInputs &in, const
             lt.dbt = at32(s_keep + 4 * LK, k);
             return;
         }
-        sw_cloud_layer<JB, Real>(taug, taur, prmu0, exp_tbl, fl, (fl & SWC_BLEND) ? cl.cldfr[ix] : 0.0, cl, ix, lc,
-                                 lt);
+        sw_cloud_layer<JB, MC, Real>(taug, taur, prmu0, exp_tbl, fl, cf, cl, ix, o, lc, lt);
         if (keep && (fl & SWC_BLEND)) {
             at32(s_keep, k) = lt.r;
             at32(s_keep + LK, k) = lt.rd;
@@ -331,7 +403,7 @@ __device__ __forceinline__ void sw_cloud_band_body(const ColumnInputs &in, const
     }
 }
 
-template <typename Real>
+template <typename Real, bool MC>
 __global__ __launch_bounds__(64) void sw_cloud_band_kernel(ColumnInputs in, SwWork w, SwClouds cl, SwCloudWork cw,
                                                            const SwBand *__restrict__ bands, int nwaves, int blk0,
                                                            int nblocks, int exp_off, const Real *__restrict__ tab,
@@ -347,7 +419,7 @@ __global__ __launch_bounds__(64) void sw_cloud_band_kernel(ColumnInputs in, SwWo
     const Real *exp_tbl = tab + exp_off;
 #define BODY(jb)                                                                                                \
     case jb - 16:                                                                                               \
-        sw_cloud_band_body<jb, Real>(in, w, cl, cw, B, tab, exp_tbl, u, tile, (int)threadIdx.x, row, s_red);    \
+        sw_cloud_band_body<jb, MC, Real>(in, w, cl, cw, B, tab, exp_tbl, u, tile, (int)threadIdx.x, row, s_red);\
         break;
     switch (u.band) { SW_BANDS(BODY) }
 #undef BODY
@@ -395,7 +467,16 @@ hipError_t launch_sw_cloud_flags(const ColumnInputs &in, const SwClouds &cl, con
 {
     const int nbl = (int)(((size_t)in.nlay * in.ncol + 255) / 256);
     hipLaunchKernelGGL(sw_cloud_zero_kernel, dim3(1), dim3(1), 0, s, cw.partial);
-    hipLaunchKernelGGL(sw_cloud_flags_kernel, dim3(nbl), dim3(256), 0, s, in, cl, cw);
+    if (cl.mcica && !cl.cldfmc) {
+        // McICA driver with icld = 0: no subcolumns, every layer clear
+        hipLaunchKernelGGL(sw_flags_clear_kernel, dim3(nbl), dim3(256), 0, s, (size_t)in.nlay * in.ncol, cw.flags);
+    } else if (cl.mcica) {
+        const size_t n2 = (size_t)in.nlay * in.ncol;
+        hipLaunchKernelGGL(sw_mcica_flags_kernel, dim3((unsigned)((n2 + SWMC_FLAG_PAIRS - 1) / SWMC_FLAG_PAIRS)),
+                           dim3(256), 0, s, in.ncol, in.nlay, cl, cw);
+    } else {
+        hipLaunchKernelGGL(sw_cloud_flags_kernel, dim3(nbl), dim3(256), 0, s, in, cl, cw);
+    }
     return hipGetLastError();
 }
 
@@ -413,8 +494,12 @@ static hipError_t launch_sw_cloud_t(const ColumnInputs &in, const SwClouds &cl, 
     const int nblocks = ntiles * t.desc.nwaves;
     for (int b0 = 0; b0 < nblocks; b0 += cw.chunk) {
         const int nb_c = std::min(cw.chunk, nblocks - b0);
-        hipLaunchKernelGGL(sw_cloud_band_kernel<Real>, dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw, t.bands,
-                           t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+        if (cl.mcica)
+            hipLaunchKernelGGL((sw_cloud_band_kernel<Real, true>), dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl, cw,
+                               t.bands, t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, (const Real *)t.tab, t.units);
+        else
+            hipLaunchKernelGGL((sw_cloud_band_kernel<Real, false>), dim3(xcd_grid(nb_c)), dim3(64), 0, s, in, w, cl,
+                               cw, t.bands, t.desc.nwaves, b0, nb_c, t.desc.exp_tbl, (const Real *)t.tab, t.units);
     }
     if (ev) (void)hipEventRecord(ev[2], s);
     hipLaunchKernelGGL(sw_cloud_finish_kernel<Real>, dim3(nbl), dim3(256), 0, s, in, cw, out, t.heatfac);

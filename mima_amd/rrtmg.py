@@ -265,6 +265,55 @@ def rrtmg_lw_mcica(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr,
     return tuple(outs)
 
 
+def mcica_subcol_sw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc,
+                    fsfc, iplon=0, stream=None):
+    """mcica_subcol_sw (mcica_subcol_gen_sw.f90:47-159): McICA cloud
+    subcolumns of the SW g-points with the KISS generator (irng = 0).
+
+    ``play``, ``cldfrac``, ``ciwp``, ``clwp``, ``rei``, ``rel`` (nlay, ncol);
+    ``tauc``, ``ssac``, ``asmc``, ``fsfc`` (nlay, ncol, 14).  Returns a dict
+    of cldfmcl, ciwpmcl, clwpmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl (nlay,
+    ncol, 112) and reicmcl, relqmcl (nlay, ncol)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    names = ("cldfmcl", "ciwpmcl", "clwpmcl", "reicmcl", "relqmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl")
+    shapes = [(nlay, ncol, 112)] * 3 + [(nlay, ncol)] * 2 + [(nlay, ncol, 112)] * 4
+    if dev:
+        import torch
+        mk = torch.zeros if icld == 0 else torch.empty
+        outs = [mk(sh, dtype=torch.float64, device=play.device) for sh in shapes]
+    else:
+        outs = [(np.zeros if icld == 0 else np.empty)(sh) for sh in shapes]
+    rc = lib().rrtmg_hip_mcica_subcol_sw(
+        int(iplon), int(ncol), int(nlay), int(icld), int(permuteseed), int(irng),
+        *[a.inp(x) for x in (play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc, fsfc)], *_ptrs(dev, outs),
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return dict(zip(names, outs))
+
+
+def rrtmg_sw_mcica(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                   o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, inflgsw, iceflgsw, liqflgsw,
+                   cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl=None, clwpmcl=None, stream=None):
+    """rrtmg_sw of the McICA driver (rrtmg_sw_rad.f90:80-90) on subcolumns
+    (nlay, ncol, 112); inflgsw = 0.  h2ovmr and o3vmr are volume mixing
+    ratios; night columns are computed at a zenith cosine of 1e-10.
+    Returns (swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)."""
+    dev = _is_torch(play) and play.is_cuda
+    a = _Args(dev)
+    outs = _outputs(play if dev else None, nlay, ncol)
+    rc = lib().rrtmg_hip_sw_mcica(
+        int(ncol), int(nlay), int(icld), int(iaer),
+        *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
+                             asdir, asdif, aldir, aldif, coszen)],
+        ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon), int(inflgsw), int(iceflgsw), int(liqflgsw),
+        *[a.inp(x) for x in (cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl)],
+        None, None, None, None, None, None, *_ptrs(dev, outs),
+        DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
+    _check(rc)
+    return tuple(outs)
+
+
 def _grad_outputs(dev, like, shapes):
     if dev:
         import torch
@@ -381,6 +430,20 @@ def lw_mcica_columns(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permutese
     out = rrtmg_lw_mcica(ncol, nlay, icld, 0, *[c[k] for k in LW_KEYS], 0, 0, 0, sub["cldfmcl"], sub["taucmcl"],
                          tauaer=tauaer, stream=stream)
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), out)), sub
+
+
+def sw_mcica_columns(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, permuteseed=0,
+                     stream=None):
+    """McICA SW on a mima_amd.columns dict: the subcolumns of
+    mcica_subcol_sw (KISS) and rrtmg_sw of the McICA driver on them, both on
+    the GPU.  Returns (outputs dict, subcolumns dict)."""
+    ncol, nlay = c["ncol"], c["nlay"]
+    sub = mcica_subcol_sw(ncol, nlay, icld, permuteseed, 0, c["play"], cldfr, cicewp, cliqwp, None, None, taucld,
+                          ssacld, asmcld, fsfcld, stream=stream)
+    out = rrtmg_sw_mcica(ncol, nlay, icld, 0, *[c[k] for k in SW_KEYS], c["adjes"], c["dyofyr"], c["scon"], 0, 0,
+                         0, *[sub[k] for k in ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl",
+                                               "clwpmcl")], stream=stream)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), out)), sub
 
 
 def lw_vjp_columns(c, uflx_bar=None, dflx_bar=None, hr_bar=None, tauaer=None, stream=None, forward=False):
