@@ -205,7 +205,9 @@ def cloudy_timing(torch, rrtmg, config_columns, args):
     their optics) on the benchmark grid (256 x 128 = 32768 columns x 40
     levels) with seeded cloud blocks: per side, columns/s in FP32-mixed and
     FP64, the per-kernel times of the mixed call, the mixed path's largest
-    difference from FP64 and the cost relative to the clear-sky call."""
+    difference from FP64 and the cost relative to the clear-sky call; then
+    McICA on the same clouds (the SW side with partial fractions): the
+    subcolumn generator and the McICA drivers' transfer, per side."""
     from mima_amd.columns import make_clouds, make_sw_clouds
     import ctypes
     cols = config_columns("c1_mixed_vs_fp64", seed=0)
@@ -283,6 +285,34 @@ def cloudy_timing(torch, rrtmg, config_columns, args):
                        "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
                        "fp64_ms_per_step": res["fp64"] * 1e3, "subcolumns_ms": sub_s * 1e3,
                        "subcolumn_bytes_written": ncol * nlay * 140 * 8 * 4,
+                       "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
+    # McICA SW: the SW cloud blocks with partial fractions U(0.05, 1) (McICA
+    # allows them), maximum-random subcolumns, the McICA driver on them
+    rng = np.random.default_rng(1)
+    pcf = np.where(scl[0] > 0, rng.uniform(0.05, 1.0, scl[0].shape), 0.0)
+    sw_opt = sw_args[1:]
+    ssub = rrtmg.mcica_subcol_sw(ncol, nlay, 2, 0, 0, d["play"], dev(pcf), None, None, None, None, *sw_opt)
+    ssubcol = lambda: rrtmg.mcica_subcol_sw(ncol, nlay, 2, 0, 0, d["play"], dev(pcf), None, None, None,  # noqa: E731
+                                            None, *sw_opt)
+    smcica = lambda: dict(zip(("swuflx", "swdflx", "swhr"), rrtmg.rrtmg_sw_mcica(  # noqa: E731
+        ncol, nlay, 2, 0, *[d[k] for k in rrtmg.SW_KEYS], cols["adjes"], cols["dyofyr"], cols["scon"], 0, 0, 0,
+        *[ssub[k] for k in ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl")])[:3]))
+    res, outs = {}, {}
+    for prec in ("fp64", "mixed"):
+        rrtmg.set_precision(prec)
+        res[prec], o = timed(smcica)
+        outs[prec] = {k: v.cpu().numpy() for k, v in o.items()}
+    sub_s, _ = timed(ssubcol)
+    dfl = max(float(np.max(np.abs(outs["mixed"][k] - outs["fp64"][k]))) for k in ("swuflx", "swdflx"))
+    dhr = float(np.max(np.abs(outs["mixed"]["swhr"] - outs["fp64"]["swhr"])))
+    print("[bench] McICA SW %d x %d: subcolumns %.3f ms, mixed %.3f ms, fp64 %.3f ms" %
+          (ncol, nlay, sub_s * 1e3, res["mixed"] * 1e3, res["fp64"] * 1e3), file=sys.stderr, flush=True)
+    out["sw_mcica"] = {"workload": "McICA SW (rrtmg_hip_mcica_subcol_sw with the KISS generator, maximum-random "
+                                   "overlap, 112 subcolumns; rrtmg_hip_sw_mcica on them), same grid and cloud "
+                                   "blocks with fractions U(0.05, 1)",
+                       "value": ncol / res["mixed"], "unit": "columns/s", "ms_per_step": res["mixed"] * 1e3,
+                       "fp64_ms_per_step": res["fp64"] * 1e3, "subcolumns_ms": sub_s * 1e3,
+                       "subcolumn_bytes_written": ncol * nlay * 112 * 8 * 7,
                        "max_abs_dflux_wm2_mixed_vs_fp64": dfl, "max_abs_dhr_kday_mixed_vs_fp64": dhr}
     return out
 
