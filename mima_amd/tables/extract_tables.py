@@ -13,6 +13,8 @@ compiler would parse each ``_rb`` literal, i.e. correctly rounded doubles):
   bands 16-29 (``rrtmg_sw_k_g.f90``), the band/g-point bookkeeping of
   ``swdatinit``/``swcmbdat`` (``rrtmg_sw_init.f90:157-367``) and the MLS
   reference atmosphere of ``swatmref`` (``rrtmg_sw_setcoef.f90:289-343``).
+* ``rrtmg_lw_cld.tbl`` -- the LW cloud absorption coefficients of ``lwcldpr``
+  (``rrtmg_lw_init.f90``), for cldprop's water-path cloud optics.
 * ``rrtmg_lw.tbl`` -- RRTMG_LW bookkeeping (``rrtmg_lw_init.f90:178-363``),
   reference atmosphere and ``chi_mls`` (``rrtmg_lw_setcoef.f90:418-578``),
   the Planck table ``totplnk`` (``rrtmg_lw_setcoef.f90:581-1284``), the
@@ -328,6 +330,27 @@ def extract_lw_data(ref):
     return out, provenance
 
 
+def extract_lw_cloud(ref):
+    """The LW cloud absorption coefficients of lwcldpr (rrtmg_lw_init.f90,
+    module rrlw_cld): abscld1, absice0(2), absice1(2,5), absice2(43,16),
+    absice3(46,16), absliq0, absliq1(58,16), for cldprop's inflglw = 1, 2."""
+    init = _read(ref, LW + "/src/rrtmg_lw_init.f90")
+    prov = {LW + "/src/rrtmg_lw_init.f90": _sha(ref, LW + "/src/rrtmg_lw_init.f90")}
+    fa = FortranArrays()
+    shapes = {"abscld1": [1], "absice0": [2], "absice1": [2, 5], "absice2": [43, 16], "absice3": [46, 16],
+              "absliq0": [1], "absliq1": [58, 16]}
+    for n, sh in shapes.items():
+        fa.declare(n, sh)
+    fa.scan(_subroutine_body(init, "lwcldpr"), set(shapes))
+    out = {}
+    for n in shapes:
+        a = fa.arrays[n][0]
+        if np.isnan(a).any():
+            raise ValueError("lwcld %s has unset entries" % n)
+        out["lwcld/" + n] = a
+    return out, prov
+
+
 def lw_shapes(ref):
     """Declared shapes of the original-resolution (16 g) LW arrays per band."""
     shapes = {}
@@ -487,15 +510,32 @@ def main():
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--out", default=HERE)
     ap.add_argument("--lw-seed", type=int, default=20261202)
+    ap.add_argument("--lw-cloud-only", action="store_true",
+                    help="only write rrtmg_lw_cld.tbl (lwcldpr's cloud absorption coefficients)")
     ap.add_argument("--shapes-header", default=None,
                     help="only write the band-array shape header (C++) of the committed tables to this path")
     args = ap.parse_args()
     if args.shapes_header:
         write_shapes_header(args.shapes_header)
         return 0
+    if args.lw_cloud_only:
+        cld, prov = extract_lw_cloud(args.ref)
+        path = os.path.join(args.out, "rrtmg_lw_cld.tbl")
+        write_table(path, cld, 0)
+        mpath = os.path.join(args.out, "manifest.json")
+        manifest = json.load(open(mpath)) if os.path.exists(mpath) else {}
+        manifest["rrtmg_lw_cld.tbl"] = {"sha256": hashlib.sha256(open(path, "rb").read()).hexdigest(),
+                                        "sources": prov}
+        with open(mpath, "w") as f:
+            json.dump(manifest, f, indent=1, sort_keys=True)
+        print("wrote", len(cld), "LW cloud entries")
+        return 0
 
     sw, prov_sw = extract_sw(args.ref)
     write_table(os.path.join(args.out, "rrtmg_sw.tbl"), sw, 0)
+
+    cld, prov_cld = extract_lw_cloud(args.ref)
+    write_table(os.path.join(args.out, "rrtmg_lw_cld.tbl"), cld, 0)
 
     lw, prov_lw = extract_lw_data(args.ref)
     shapes, prov_sh = lw_shapes(args.ref)
@@ -513,6 +553,8 @@ def main():
                           "synthetic_kg": True, "synthetic_seed": args.lw_seed,
                           "synthetic_note": "AER rrtmg_lw_k_g.f90 is absent; LW k-tables are seeded "
                                             "shape-exact synthetic values, not physics."},
+        "rrtmg_lw_cld.tbl": {"sha256": hashlib.sha256(open(os.path.join(args.out, "rrtmg_lw_cld.tbl"),
+                                                           "rb").read()).hexdigest(), "sources": prov_cld},
     }
     with open(os.path.join(args.out, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

@@ -128,6 +128,23 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
                           const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
                           double *uflxc, double *dflxc, double *hrc);
 
+/* rrtmg_lw with clouds given by water paths (inflglw = 1: abscld1 times
+ * the path; 2: ice and liquid absorption coefficients from lwcldpr's
+ * tables by iceflglw 0-3 and liqflglw 0-1, rrtmg_lw_cldprop.f90:118-291),
+ * the tables loaded by oracle_lw_cld_init from rrtmg_lw_cld.tbl; reice,
+ * reliq (ncol, nlay) effective radii / sizes in microns.  taucld (16, ncol,
+ * nlay) enters only the cloud test.  -8 for a radius the reference stops
+ * on. */
+int oracle_lw_cld_init(const char *path);
+int oracle_rrtmg_lw_cloud_wp(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play,
+                             const double *plev, const double *tlay, const double *tlev, const double *tsfc,
+                             const double *h2ovmr, const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                             const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                             const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
+                             const double *cliqwp, const double *reice, const double *reliq, const double *tauaer,
+                             double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc);
+
 /* McICA LW: the stochastic subcolumns of mcica_subcol_lw with the KISS
  * generator (irng = 0; mcica_subcol_gen_lw.f90:48-520) and rrtmg_lw of the
  * McICA driver on them (rrtmg_lw_rad.f90:80-572: cldprmc with inflglw = 0,

@@ -1143,6 +1143,122 @@ int oracle_rrtmg_lw(int ncol, int nlay, const double *play, const double *plev, 
  * for cldprop when its fraction and its water path or summed optical depth
  * reach cldmin = 1e-20; its in-cloud optical depths are then the input's,
  * otherwise zero.  tauc[lay][ib], 1-based. */
+/* lwcldpr's cloud absorption coefficients (rrtmg_lw_init.f90, module
+ * rrlw_cld), from rrtmg_lw_cld.tbl; Fortran (k, ib) arrays as [ib][k] */
+static struct {
+    int ready;
+    double abscld1, absice0[2], absice1[5][2], absice2[16][43], absice3[16][46], absliq0, absliq1[16][58];
+} LC;
+
+int oracle_lw_cld_init(const char *path)
+{
+    otbl_t t;
+    if (otbl_load(&t, path) != 0) return -1;
+    const char *nm[] = {"lwcld/abscld1", "lwcld/absice0", "lwcld/absice1", "lwcld/absice2", "lwcld/absice3",
+                        "lwcld/absliq0", "lwcld/absliq1"};
+    double *dst[] = {&LC.abscld1, LC.absice0, &LC.absice1[0][0], &LC.absice2[0][0], &LC.absice3[0][0],
+                     &LC.absliq0, &LC.absliq1[0][0]};
+    const long len[] = {1, 2, 10, 688, 736, 1, 928};
+    for (int i = 0; i < 7; ++i) {
+        long n = 0;
+        const double *v = otbl_d(&t, nm[i], &n);
+        if (!v || n != len[i]) return -2;
+        memcpy(dst[i], v, sizeof(double) * len[i]);
+    }
+    LC.ready = 1;
+    return 0;
+}
+
+/* cldprop (rrtmg_lw_cldprop.f90:118-291) for inflag 1 (abscld1 times the
+ * water path) and 2 (ice: iceflag 0 absice0, 1 absice1 (5 bands), 2 absice2,
+ * 3 absice3 interpolated in the radius; liquid: liqflag 0 absliq0, 1
+ * absliq1 interpolated).  As in the reference, ncbands carries from layer to
+ * layer (it starts at 1, and iceflag 0 with liqflag 0 or no liquid leaves
+ * it), so only the first ncbands bands of a layer get a depth, and a layer
+ * with 5-band ice and no 16-band liquid reads its 5 ice coefficients by
+ * band (iceind 1 -> 2).  The final ncbands goes to the transfer, which
+ * maps each spectral band to a cloud band by it.  -8 for a radius the
+ * reference stops on. */
+static int cldprop_wp(int nl, int inflag, int iceflag, int liqflag, const double *cldfrac, double (*tauc)[17],
+                      const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                      double (*taucloud)[17], int *ncbands_out)
+{
+    static const int icb[3][16] = {{1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+                                   {1, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 5, 5, 5, 5},
+                                   {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16}};
+    const double cldmin = 1.e-20;
+    int ncbands = 1;
+    double abscoice[17], abscoliq[17];
+    for (int lay = 1; lay <= nl; ++lay) {
+        double tauctot = 0.0;
+        for (int ib = 1; ib <= 16; ++ib) {
+            taucloud[lay][ib] = 0.0;
+            tauctot = tauctot + tauc[lay][ib];
+        }
+        double cwp = ciwp[lay] + clwp[lay];
+        if (!(cldfrac[lay] >= cldmin && (cwp >= cldmin || tauctot >= cldmin))) continue;
+        if (inflag == 1) {
+            ncbands = 16;
+            for (int ib = 1; ib <= ncbands; ++ib) taucloud[lay][ib] = LC.abscld1 * cwp;
+            continue;
+        }
+        double radice = rei[lay];
+        int iceind = 0, liqind = 0;
+        if (ciwp[lay] == 0.0) {
+            abscoice[1] = 0.0;
+            iceind = 0;
+        } else if (iceflag == 0) {
+            if (radice < 10.0) return -8;
+            abscoice[1] = LC.absice0[0] + LC.absice0[1] / radice;
+            iceind = 0;
+        } else if (iceflag == 1) {
+            if (radice < 13.0 || radice > 130.0) return -8;
+            ncbands = 5;
+            for (int ib = 1; ib <= ncbands; ++ib) abscoice[ib] = LC.absice1[ib - 1][0] + LC.absice1[ib - 1][1] / radice;
+            iceind = 1;
+        } else if (iceflag == 2 || iceflag == 3) {
+            const int nr = iceflag == 2 ? 43 : 46;
+            if (radice < 5.0 || radice > (iceflag == 2 ? 131.0 : 140.0)) return -8;
+            ncbands = 16;
+            double factor = (radice - 2.0) / 3.0;
+            int index = (int)factor;
+            if (index == nr) index = nr - 1;
+            double fint = factor - (double)index;
+            for (int ib = 1; ib <= ncbands; ++ib) {
+                const double *a = iceflag == 2 ? LC.absice2[ib - 1] : LC.absice3[ib - 1];
+                abscoice[ib] = a[index - 1] + fint * (a[index] - (a[index - 1]));
+            }
+            iceind = 2;
+        }
+        if (clwp[lay] == 0.0) {
+            abscoliq[1] = 0.0;
+            liqind = 0;
+            if (iceind == 1) iceind = 2;
+        } else if (liqflag == 0) {
+            abscoliq[1] = LC.absliq0;
+            liqind = 0;
+            if (iceind == 1) iceind = 2;
+        } else if (liqflag == 1) {
+            double radliq = rel[lay];
+            if (radliq < 2.5 || radliq > 60.0) return -8;
+            int index = (int)(radliq - 1.5);
+            if (index == 0) index = 1;
+            if (index == 58) index = 57;
+            double fint = radliq - 1.5 - (double)index;
+            ncbands = 16;
+            for (int ib = 1; ib <= ncbands; ++ib) {
+                const double *a = LC.absliq1[ib - 1];
+                abscoliq[ib] = a[index - 1] + fint * (a[index] - (a[index - 1]));
+            }
+            liqind = 2;
+        }
+        for (int ib = 1; ib <= ncbands; ++ib)
+            taucloud[lay][ib] = ciwp[lay] * abscoice[icb[iceind][ib - 1]] + clwp[lay] * abscoliq[icb[liqind][ib - 1]];
+    }
+    *ncbands_out = ncbands;
+    return 0;
+}
+
 static void cldprop0(int nl, const double *cldfrac, double (*tauc)[17], const double *cwp, double (*taucloud)[17])
 {
     const double cldmin = 1.e-20;
@@ -1377,7 +1493,7 @@ static void cloudy_layer(double odepth, double odcld, double plfrac, double blay
  * iout = 0, idrv = 0; taut = taug + taua(ngb) (iaer = 10,
  * rrtmg_lw_rad.nomcica.f90:512-518).  Total-sky and clear-sky fluxes and
  * heating rates. */
-static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, double (*taucloud)[17],
+static void rtrn_cloudy(const lwcol_t *c, int icld, int ncbands, const double *cldfrac, double (*taucloud)[17],
                         double (*taua)[17], double *totuflux, double *totdflux, double *htr, double *totuclfl,
                         double *totdclfl, double *htrc)
 {
@@ -1392,6 +1508,13 @@ static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, doubl
     double fnet[MAXLAY + 2], fnetc[MAXLAY + 2], cf[MAXLAY + 2];
     int icldlyr[MAXLAY + 2];
     static __thread overlap_t ov;
+    /* the cloud band of each spectral band for cldprop's ncbands
+     * (rrtmg_lw_rtrnmr.f90:487-493, ipat of :246-248);
+     * the cloud depth takes the cloud band's diffusivity (:315-317) */
+    static const int ipat[3][17] = {{0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+                                    {0, 1, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 5, 5, 5, 5},
+                                    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16}};
+    const int *cb = ipat[ncbands == 1 ? 0 : ncbands == 5 ? 1 : 2];
     for (int ibnd = 1; ibnd <= 16; ++ibnd) {
         if (ibnd == 1 || ibnd == 4 || ibnd >= 10) {
             secdiff[ibnd] = 1.66;
@@ -1440,14 +1563,14 @@ static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, doubl
                 if (icldlyr[lev] == 1) {
                     iclddn = 1;
                     cldlay_t r;
-                    cloudy_layer(odepth, odcld[lev][iband], plfrac, blay, dplankup, dplankdn, &r);
+                    cloudy_layer(odepth, odcld[lev][cb[iband]], plfrac, blay, dplankup, dplankdn, &r);
                     atrans[lev] = r.atrans;
                     atot[lev] = r.atot;
                     bbugas[lev] = r.bbugas;
                     bbutot[lev] = r.bbutot;
                     bbd = r.bbd;
                     if (icld == 1) {
-                        radld = radld - radld * (r.atrans + efclfrac[lev][iband] * (1.0 - r.atrans)) + r.gassrc +
+                        radld = radld - radld * (r.atrans + efclfrac[lev][cb[iband]] * (1.0 - r.atrans)) + r.gassrc +
                                 cf[lev] * (r.bbdtot * r.atot - r.gassrc);
                     } else {
                         if (ov.istcldd[lev] == 1) {
@@ -1506,7 +1629,7 @@ static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, doubl
                 if (icldlyr[lev] == 1) {
                     double gassrc = bbugas[lev] * atrans[lev];
                     if (icld == 1) {
-                        radlu = radlu - radlu * (atrans[lev] + efclfrac[lev][iband] * (1.0 - atrans[lev])) + gassrc +
+                        radlu = radlu - radlu * (atrans[lev] + efclfrac[lev][cb[iband]] * (1.0 - atrans[lev])) + gassrc +
                                 cf[lev] * (bbutot[lev] * atot[lev] - gassrc);
                     } else {
                         if (ov.istcld[lev] == 1) {
@@ -1571,13 +1694,13 @@ static void rtrn_cloudy(const lwcol_t *c, int icld, const double *cldfrac, doubl
  * idrv = 0.  cldfr, cicewp, cliqwp
  * (ncol, nlay); taucld (16, ncol, nlay), band fastest; tauaer (ncol, nlay,
  * 16) or NULL; cicewp/cliqwp NULL = 0. */
-int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+static int lw_cloud_driver(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play, const double *plev,
                           const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
                           const double *o3vmr, const double *co2vmr, const double *ch4vmr,
                           const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
                           const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
                           const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
-                          const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
+                          const double *cliqwp, const double *reice, const double *reliq, const double *tauaer, double *uflx, double *dflx, double *hr,
                           double *uflxc, double *dflxc, double *hrc)
 {
     if (!L.ready) return -1;
@@ -1592,7 +1715,7 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
         double (*tauc)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
         double (*taucloud)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
         double (*taua)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
-        double cf[MAXLAY + 2], cwp[MAXLAY + 2];
+        double cf[MAXLAY + 2], cwp[MAXLAY + 2], ciw[MAXLAY + 2], clw[MAXLAY + 2], rei[MAXLAY + 2], rel[MAXLAY + 2];
         double tu[MAXLAY + 2], td[MAXLAY + 2], tuc[MAXLAY + 2], tdc[MAXLAY + 2], htr[MAXLAY + 2],
             htrc[MAXLAY + 2];
         if (!c || !tauc || !taucloud || !taua) {
@@ -1607,16 +1730,28 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
             for (int l = 1; l <= nlay; ++l) {
                 long ix = iplon + (long)ncol * (l - 1);
                 cf[l] = clouds ? cldfr[ix] : 0.0;
+                ciw[l] = clouds && cicewp ? cicewp[ix] : 0.0;
+                clw[l] = clouds && cliqwp ? cliqwp[ix] : 0.0;
+                rei[l] = clouds && reice ? reice[ix] : 0.0;
+                rel[l] = clouds && reliq ? reliq[ix] : 0.0;
                 cwp[l] = clouds ? (cicewp ? cicewp[ix] : 0.0) + (cliqwp ? cliqwp[ix] : 0.0) : 0.0;
                 for (int ib = 1; ib <= 16; ++ib) {
-                    tauc[l][ib] = clouds ? taucld[(ib - 1) + 16 * ix] : 0.0;
+                    tauc[l][ib] = clouds && taucld ? taucld[(ib - 1) + 16 * ix] : 0.0;
                     taua[l][ib] = tauaer ? tauaer[iplon + (long)ncol * ((l - 1) + (long)nlay * (ib - 1))] : 0.0;
                 }
             }
-            cldprop0(nlay, cf, tauc, cwp, taucloud);
+            int ncbands = 16;
+            if (inflag == 0 || !clouds) {
+                cldprop0(nlay, cf, tauc, cwp, taucloud);
+            } else if (cldprop_wp(nlay, inflag, iceflag, liqflag, cf, tauc, ciw, clw, rei, rel, taucloud,
+                                  &ncbands) != 0) {
+#pragma omp atomic write
+                err = -8;
+                continue;
+            }
             setcoef(c);
             taumol(c);
-            rtrn_cloudy(c, clouds ? icld : 2, cf, taucloud, tauaer ? taua : NULL, tu, td, htr, tuc, tdc, htrc);
+            rtrn_cloudy(c, clouds ? icld : 2, ncbands, cf, taucloud, tauaer ? taua : NULL, tu, td, htr, tuc, tdc, htrc);
             for (int k = 0; k <= nlay; ++k) {
                 uflx[iplon + (long)ncol * k] = tu[k];
                 dflx[iplon + (long)ncol * k] = td[k];
@@ -1634,6 +1769,35 @@ int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, cons
         free(taua);
     }
     return err;
+}
+
+int oracle_rrtmg_lw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                          const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                          const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                          const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                          const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
+                          const double *cliqwp, const double *tauaer, double *uflx, double *dflx, double *hr,
+                          double *uflxc, double *dflxc, double *hrc)
+{
+    return lw_cloud_driver(ncol, nlay, icld, 0, 0, 0, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                           n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, cldfr, taucld, cicewp, cliqwp,
+                           NULL, NULL, tauaer, uflx, dflx, hr, uflxc, dflxc, hrc);
+}
+
+int oracle_rrtmg_lw_cloud_wp(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play,
+                             const double *plev, const double *tlay, const double *tlev, const double *tsfc,
+                             const double *h2ovmr, const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                             const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                             const double *emis, const double *cldfr, const double *taucld, const double *cicewp,
+                             const double *cliqwp, const double *reice, const double *reliq, const double *tauaer,
+                             double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc)
+{
+    if (inflag != 0 && !LC.ready) return -1;
+    return lw_cloud_driver(ncol, nlay, icld, inflag, iceflag, liqflag, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, cldfr, taucld,
+                           cicewp, cliqwp, reice, reliq, tauaer, uflx, dflx, hr, uflxc, dflxc, hrc);
 }
 
 /* ---- McICA (rrtmg_lw_rad.f90 with mcica_subcol_gen_lw.f90) ----------------- */

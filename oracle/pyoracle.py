@@ -121,6 +121,38 @@ def rrtmg_lw_cloud(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, tauaer=None
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+LW_CLD_TABLE = os.path.join(os.path.dirname(LW_TABLE), "rrtmg_lw_cld.tbl")
+_cld_ready = False
+
+
+def rrtmg_lw_cloud_wp(c, icld, inflag, iceflag, liqflag, cldfr, cicewp, cliqwp, reice, reliq, taucld=None,
+                      tauaer=None):
+    """rrtmg_lw with clouds given by water paths (inflglw 1 or 2, cldprop's
+    optics on lwcldpr's coefficients): cldfr, cicewp, cliqwp, reice, reliq
+    (nlay, ncol); taucld (nlay, ncol, 16) only for the cloud test (zero if
+    None)."""
+    global _cld_ready
+    L = lib()
+    if not _cld_ready:
+        if L.oracle_lw_cld_init(LW_CLD_TABLE.encode()) != 0:
+            raise RuntimeError("oracle_lw_cld_init failed")
+        _cld_ready = True
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr",
+                                "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr",
+                                "ccl4vmr", "emis")]
+    tc = np.zeros((nlay, ncol, 16)) if taucld is None else taucld
+    cl = [_f64(a) for a in (cldfr, tc, cicewp, cliqwp, reice, reliq)]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_lw_cloud_wp(ncol, nlay, int(icld), int(inflag), int(iceflag), int(liqflag),
+                                    *[_p(a) for a in ins], *[_p(a) for a in cl],
+                                    None if tauaer is None else _p(_f64(tauaer)), *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_lw_cloud_wp returned %d" % rc)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
 def lib_ngs():
     """Cumulative LW g-points per band (the reduced table's ngs)."""
     from mima_amd.tables.extract_tables import read_table

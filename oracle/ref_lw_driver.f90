@@ -21,6 +21,9 @@
 !                    the cloud arguments MiMA passes (RR/rrtm_radiation.f90:730-749)
 !   ref_lw_run_cloud -> rrtmg_lw with icld = 1..3 and inflglw = 0 (cldprop,
 !                    then rtrn or rtrnmr with the caller's cloud profiles)
+!   ref_lw_set_cld + ref_lw_run_cloud_wp -> the same with inflglw = 1, 2
+!                    (cldprop's water-path optics on lwcldpr's coefficients,
+!                    which the caller passes as extracted from rrtmg_lw_init.f90)
 !   ref_lw_column -> inatm + setcoef + taumol for one column
 module ref_lw_driver
   use iso_c_binding
@@ -29,6 +32,7 @@ module ref_lw_driver
   use rrtmg_lw_rad, only : rrtmg_lw, inatm
   use rrtmg_lw_setcoef, only : setcoef, lwatmref, lwavplank, lwavplankderiv
   use rrtmg_lw_taumol, only : taumol
+  use rrlw_cld, only : abscld1, absice0, absice1, absice2, absice3, absliq0, absliq1
   include 'ref_lw_setters.inc'
   implicit none
 contains
@@ -90,6 +94,50 @@ contains
                   0, 0, 0, cldfr, taucld, cicewp, cliqwp, tencl, tencl, tauaer, &
                   uflx, dflx, hr, uflxc, dflxc, hrc)
   end subroutine ref_lw_run_cloud
+
+  ! lwcldpr's cloud absorption coefficients (rrtmg_lw_init.f90, module
+  ! rrlw_cld), which this build does not run: the caller passes the values
+  ! extracted from that file (rrtmg_lw_cld.tbl)
+  subroutine ref_lw_set_cld(c1, i0, i1, i2, i3, l0, l1) bind(C, name="ref_lw_set_cld")
+    real(c_double), value :: c1, l0
+    real(c_double), intent(in) :: i0(2), i1(2, 5), i2(43, 16), i3(46, 16), l1(58, 16)
+    abscld1 = c1
+    absice0 = i0
+    absice1 = i1
+    absice2 = i2
+    absice3 = i3
+    absliq0 = l0
+    absliq1 = l1
+  end subroutine ref_lw_set_cld
+
+  ! rrtmg_lw with clouds given by water paths: inflglw, iceflglw, liqflglw
+  ! and the effective radii / sizes reice, reliq (ncol, nlay)
+  subroutine ref_lw_run_cloud_wp(ncol, nlay, icld_in, inflag_in, iceflag_in, liqflag_in, play, plev, tlay, &
+                                 tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, cfc11, cfc12, cfc22, ccl4, emis, &
+                                 cldfr, taucld, cicewp, cliqwp, reice, reliq, tauaer, &
+                                 uflx, dflx, hr, uflxc, dflxc, hrc) bind(C, name="ref_lw_run_cloud_wp")
+    integer(c_int), value :: ncol, nlay, icld_in, inflag_in, iceflag_in, liqflag_in
+    real(c_double), intent(in) :: play(ncol, nlay), plev(ncol, nlay + 1), tlay(ncol, nlay)
+    real(c_double), intent(in) :: tlev(ncol, nlay + 1), tsfc(ncol)
+    real(c_double), intent(in) :: h2o(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay)
+    real(c_double), intent(in) :: ch4(ncol, nlay), n2o(ncol, nlay), o2(ncol, nlay)
+    real(c_double), intent(in) :: cfc11(ncol, nlay), cfc12(ncol, nlay), cfc22(ncol, nlay)
+    real(c_double), intent(in) :: ccl4(ncol, nlay), emis(ncol, nbndlw), tauaer(ncol, nlay, nbndlw)
+    real(c_double), intent(in) :: cldfr(ncol, nlay), taucld(nbndlw, ncol, nlay)
+    real(c_double), intent(in) :: cicewp(ncol, nlay), cliqwp(ncol, nlay), reice(ncol, nlay), reliq(ncol, nlay)
+    real(c_double), intent(out) :: uflx(ncol, nlay + 1), dflx(ncol, nlay + 1), hr(ncol, nlay)
+    real(c_double), intent(out) :: uflxc(ncol, nlay + 1), dflxc(ncol, nlay + 1), hrc(ncol, nlay)
+    integer(kind=im) :: icld, idrv, inflag, iceflag, liqflag
+    icld = icld_in
+    idrv = 0
+    inflag = inflag_in
+    iceflag = iceflag_in
+    liqflag = liqflag_in
+    call rrtmg_lw(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, &
+                  h2o, o3, co2, ch4, n2o, o2, cfc11, cfc12, cfc22, ccl4, emis, &
+                  inflag, iceflag, liqflag, cldfr, taucld, cicewp, cliqwp, reice, reliq, tauaer, &
+                  uflx, dflx, hr, uflxc, dflxc, hrc)
+  end subroutine ref_lw_run_cloud_wp
 
   subroutine ref_lw_column(nlay, play, plev, tlay, tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, &
                            cfc11, cfc12, cfc22, ccl4, &
