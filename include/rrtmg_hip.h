@@ -147,14 +147,23 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer,
  * overlap (rrtmg_lw_rtrn.f90); 2 or 3: maximum-random overlap
  * (rrtmg_lw_rtrnmr.f90; the nomcica driver runs 3 as 2, :527-540); icld
  * outside 0..3 is taken as 2 (:437); icld = 0 is rrtmg_hip_lw (cloud arrays
- * unread).  inflglw = 0 only (cldprop, rrtmg_lw_cldprop.f90:154-176):
- * taucld (16, ncol, nlay), band fastest, holds the in-cloud optical depths;
- * a layer's are used where cldfr (ncol, nlay) >= 1e-20 and cicewp + cliqwp
- * (ncol, nlay; NULL = 0) or the layer's summed depth >= 1e-20, and the layer
- * is cloudy for the transfer where cldfr >= 1e-6.  iceflglw, liqflglw,
- * reice and reliq are not read (inflglw = 0 ignores them).  Outputs:
- * total-sky uflx, dflx, hr and clear-sky uflxc, dflxc, hrc.  inflglw != 0,
- * idrv != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+ * unread).  cldprop (rrtmg_lw_cldprop.f90:118-291): a layer has cloud
+ * optics where cldfr (ncol, nlay) >= 1e-20 and cicewp + cliqwp (ncol, nlay;
+ * NULL = 0) or its summed taucld >= 1e-20, and is cloudy for the transfer
+ * where cldfr >= 1e-6.  inflglw = 0: taucld (16, ncol, nlay), band fastest,
+ * holds the in-cloud optical depths; iceflglw, liqflglw, reice, reliq are
+ * not read.  inflglw = 1: depths abscld1 times the water path.  inflglw =
+ * 2: ice absorption by iceflglw 0-3 and liquid by liqflglw 0-1 from
+ * lwcldpr's coefficients (rrtmg_lw_init.f90; the library reads them from
+ * rrtmg_lw_cld.tbl beside rrtmg_lw.tbl) with the effective ice radius /
+ * size reice and liquid radius reliq (ncol, nlay, microns); taucld may then
+ * be NULL (it only enters the cloud test).  A radius outside its
+ * parameterisation's range in a layer with cloud optics returns
+ * RRTMG_HIP_EINVAL (the reference stops; device pointers: with
+ * rrtmg_hip_set_checks on).  Outputs: total-sky uflx, dflx, hr and
+ * clear-sky uflxc, dflxc, hrc.  inflglw outside 0..2 returns
+ * RRTMG_HIP_EINVAL; idrv != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
+ * RRTMG_HIP_ENOTSUP. */
 int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv,
                        const double *play, const double *plev, const double *tlay,
                        const double *tlev, const double *tsfc,

@@ -68,6 +68,7 @@ struct SideTables {
     LwBand *bands_lds = nullptr;                 // LW: offsets relative to each band's first k-table element
     LwLdsPlan plan{};                            // LW: schedule of the LDS-table kernel
     bool lds = false;                            // LW: that kernel fits this device
+    double *cld = nullptr;                       // LW: lwcldpr's coefficients (LW_CLD_TAB), when the file is there
     double cpdair = 0.0;
 };
 
@@ -443,6 +444,24 @@ int rrtmg_hip_lw_init(double cpdair, const char *table_dir)
     LwBand rebased[16];
     if (!rc && !make_lw_plan(s->lw, s->lwd.plan, rebased, err)) return fail(RRTMG_HIP_ENOENT, err);
     if (!rc) rc = upload(&s->lwd.bands_lds, rebased, sizeof rebased);
+    {
+        // optional: the water-path cloud optics' coefficients (inflglw 1, 2)
+        TableFile ct;
+        std::string cerr;
+        if (!rc && ct.load(dir + "/rrtmg_lw_cld.tbl", cerr)) {
+            std::vector<double> pk;
+            const char *nm[] = {"lwcld/abscld1", "lwcld/absice0", "lwcld/absice1", "lwcld/absice2",
+                                "lwcld/absice3", "lwcld/absliq0", "lwcld/absliq1"};
+            const size_t len[] = {1, 2, 10, 688, 736, 1, 928};
+            bool ok = true;
+            for (int i = 0; i < 7; ++i) {
+                std::vector<double> v = ct.find(nm[i]) ? ct.doubles(nm[i]) : std::vector<double>();
+                ok = ok && v.size() == len[i];
+                pk.insert(pk.end(), v.begin(), v.end());
+            }
+            if (ok && pk.size() == (size_t)LW_CLD_TAB) rc = upload(&s->lwd.cld, pk.data(), pk.size() * sizeof(double));
+        }
+    }
     if (rc) return fail(rc, "uploading LW tables failed");
     {
         int ncu = 0, lds = 0;
@@ -759,6 +778,10 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     if (icld != 0 && !cloudy) return fail(RRTMG_HIP_ENOTSUP, "icld != 0: rrtmg_hip_lw is clear sky; clouds take rrtmg_hip_lw_cloud");
     if (cloudy && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
         return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
+    const bool wp = cloudy && cl->icld != LW_MCICA && cl->inflag != 0;
+    if (wp && !s->lwd.cld)
+        return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0 needs the cloud optics table rrtmg_lw_cld.tbl beside "
+                                       "rrtmg_lw.tbl at rrtmg_hip_lw_init");
     if (idrv != 0) return fail(RRTMG_HIP_ENOTSUP, "idrv != 0: dF/dTs output is not implemented");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
     if (rc) return rc;
@@ -796,7 +819,9 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
             a.push_back({"taucmcl", cl->taucmc, n2 * 140});
         } else if (cloudy) {
             a.push_back({"cldfr", cl->cldfr, n2});
-            a.push_back({"taucld", cl->taucld, n2 * 16});
+            if (cl->taucld) a.push_back({"taucld", cl->taucld, n2 * 16});
+            if (wp && cl->reice) a.push_back({"reice", cl->reice, n2});
+            if (wp && cl->reliq) a.push_back({"reliq", cl->reliq, n2});
             if (cl->cicewp) a.push_back({"cicewp", cl->cicewp, n2});
             if (cl->cliqwp) a.push_back({"cliqwp", cl->cliqwp, n2});
         }
@@ -828,6 +853,8 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     const size_t o_cbf = off; if (cloudy) off = align_up(off + rs * 16 * 4 * n3);
     const size_t crows = lds ? (size_t)lw_lds_grid(s->lwd.plan, ncol, s->lw.tab.nwaves) * LW_WGW : (size_t)cchunk;
     const size_t o_crw = off; if (cloudy) off = align_up(off + crows * lw_cloud_row_bytes(nlay, rs));
+    const size_t o_cod = off; if (wp) off = align_up(off + sizeof(double) * 16 * n2);
+    const size_t o_cbd = off; if (wp) off = align_up(off + sizeof(int));
     if ((rc = W.lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)W.lw_work.p;
     LwWork w;
@@ -850,6 +877,11 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         cw.rows = wb + o_crw;
         cw.chunk = cchunk;
         cld = *cl;
+        if (wp) {
+            cw.od = (double *)(wb + o_cod);
+            cw.bad = (int *)(wb + o_cbd);
+            cw.cldtab = s->lwd.cld;
+        }
     }
 
     ColumnInputs in{};
@@ -860,7 +892,9 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     if (ptr_kind == RRTMG_HIP_HOST) {
         const size_t ncl = !cloudy ? 0
                            : cl->icld == LW_MCICA ? 280 * n2
-                                                  : 17 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0);
+                                                  : n2 + (cl->taucld ? 16 * n2 : 0) + (cl->cicewp ? n2 : 0) +
+                                                        (cl->cliqwp ? n2 : 0) + (wp && cl->reice ? n2 : 0) +
+                                                        (wp && cl->reliq ? n2 : 0);
         const size_t need = sizeof(double) * (12 * n2 + 2 * n3 + (size_t)ncol * 17 + (tauaer ? 16 * n2 : 0) +
                                               ncl + 4 * n3 + 2 * n2);
         if ((rc = W.lw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
@@ -892,11 +926,12 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         if (cloudy) {
             const bool mc = cl->icld == LW_MCICA;
             const double *src[] = {mc ? cl->cldfmc : cl->cldfr, mc ? cl->taucmc : cl->taucld,
-                                   mc ? nullptr : cl->cicewp, mc ? nullptr : cl->cliqwp};
+                                   mc ? nullptr : cl->cicewp, mc ? nullptr : cl->cliqwp,
+                                   wp ? cl->reice : nullptr, wp ? cl->reliq : nullptr};
             const double **dst[] = {mc ? &cld.cldfmc : &cld.cldfr, mc ? &cld.taucmc : &cld.taucld, &cld.cicewp,
-                                    &cld.cliqwp};
-            const size_t len[] = {mc ? 140 * n2 : n2, mc ? 140 * n2 : 16 * n2, n2, n2};
-            for (int i = 0; i < 4; ++i) {
+                                    &cld.cliqwp, &cld.reice, &cld.reliq};
+            const size_t len[] = {mc ? 140 * n2 : n2, mc ? 140 * n2 : 16 * n2, n2, n2, n2, n2};
+            for (int i = 0; i < 6; ++i) {
                 if (!src[i]) continue;
                 ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
                 *dst[i] = p;
@@ -930,6 +965,18 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     hipError_t e = cloudy ? launch_lw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(1))
                           : launch_lw(in, out, w, t, g_prec, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
+    if (wp && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
+        // the radii cldprop stops on (rrtmg_lw_cldprop.f90:184-259); the
+        // host-pointer path checked before staging
+        int nb = 0;
+        e = hipMemcpyAsync(&nb, cw.bad, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW: ") + hipGetErrorString(e));
+        if (nb)
+            return fail(RRTMG_HIP_EINVAL, "an ice or liquid effective radius outside the range of its "
+                                          "parameterisation in a cloudy layer (the reference stops); outputs "
+                                          "written from clamped radii");
+    }
     if (!cloudy) s->lw_w = w;
     s->lw_last_ncol = ncol;
     s->lw_last_nlay = nlay;
@@ -1084,32 +1131,54 @@ int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv, const double *pla
                        const double *tauaer, double *uflx, double *dflx, double *hr, double *uflxc, double *dflxc,
                        double *hrc, int ptr_kind, void *stream)
 {
-    (void)iceflglw;
-    (void)liqflglw;
-    (void)reice;
-    (void)reliq;
     // rrtmg_lw_rad.nomcica.f90:437: icld outside 0..3 selects maximum-random overlap
     if (icld < 0 || icld > 3) icld = 2;
     if (icld == 0)
         return lw_forward(ncol, nlay, 0, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
                           n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, nullptr, uflx, dflx,
                           hr, uflxc, dflxc, hrc, ptr_kind, stream);
-    if (inflglw != 0)
-        return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0: cloud optics from water paths and effective radii are not "
-                                       "implemented; pass in-cloud optical depths with inflglw = 0");
+    if (inflglw < 0 || inflglw > 2) return fail(RRTMG_HIP_EINVAL, "inflglw must be 0, 1 or 2");
+    if (inflglw == 2 && (iceflglw < 0 || iceflglw > 3 || liqflglw < 0 || liqflglw > 1))
+        return fail(RRTMG_HIP_EINVAL, "iceflglw must be 0..3 and liqflglw 0..1 with inflglw = 2");
     if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
         const size_t n2 = (size_t)ncol * nlay;
         int rc;
         if (ptr_kind == RRTMG_HIP_HOST) {
-            if ((rc = check_host_array(cldfr, n2, "cldfr")) || (rc = check_host_array(taucld, n2 * 16, "taucld")))
-                return rc;
+            if ((rc = check_host_array(cldfr, n2, "cldfr"))) return rc;
+            if ((inflglw == 0 || taucld) && (rc = check_host_array(taucld, n2 * 16, "taucld"))) return rc;
             if (cicewp && (rc = check_host_array(cicewp, n2, "cicewp"))) return rc;
             if (cliqwp && (rc = check_host_array(cliqwp, n2, "cliqwp"))) return rc;
-        } else if (!cldfr || !taucld) {
+            if (inflglw == 2 && reice && (rc = check_host_array(reice, n2, "reice"))) return rc;
+            if (inflglw == 2 && reliq && (rc = check_host_array(reliq, n2, "reliq"))) return rc;
+            if (inflglw == 2) {
+                // the radii cldprop stops on (rrtmg_lw_cldprop.f90:184-259),
+                // in the layers that pass its cloud test
+                for (size_t i = 0; i < n2; ++i) {
+                    const double ci = cicewp ? cicewp[i] : 0.0, cw = cliqwp ? cliqwp[i] : 0.0;
+                    double tt = 0.0;
+                    for (int b = 0; b < 16 && taucld; ++b) tt = tt + taucld[i * 16 + b];
+                    if (!(cldfr[i] >= 1.e-20 && (ci + cw >= 1.e-20 || tt >= 1.e-20))) continue;
+                    const double ri = reice ? reice[i] : 0.0, rl = reliq ? reliq[i] : 0.0;
+                    const bool bi = ci != 0.0 && (iceflglw == 0   ? ri < 10.0
+                                                  : iceflglw == 1 ? ri < 13.0 || ri > 130.0
+                                                  : iceflglw == 2 ? ri < 5.0 || ri > 131.0
+                                                                  : ri < 5.0 || ri > 140.0);
+                    const bool bl = cw != 0.0 && liqflglw == 1 && (rl < 2.5 || rl > 60.0);
+                    if (bi || bl)
+                        return fail(RRTMG_HIP_EINVAL, "an ice or liquid effective radius outside the range of its "
+                                                      "parameterisation in a cloudy layer (the reference stops)");
+                }
+            }
+        } else if (!cldfr || (inflglw == 0 && !taucld)) {
             return fail(RRTMG_HIP_EINVAL, "null cloud array");
         }
     }
     LwClouds cl{icld == 1 ? 1 : 2, cldfr, taucld, cicewp, cliqwp};
+    cl.inflag = inflglw;
+    cl.iceflag = iceflglw;
+    cl.liqflag = liqflglw;
+    cl.reice = reice;
+    cl.reliq = reliq;
     return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                       o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &cl, uflx, dflx, hr, uflxc, dflxc,
                       hrc, ptr_kind, stream);

@@ -112,7 +112,30 @@ struct LwClouds {
     const double *taucld;             // [nlay][ncol][16]: the reference's taucld(nbndlw, ncol, nlay)
     const double *cicewp, *cliqwp;    // [nlay][ncol] or NULL (zero)
     const double *cldfmc, *taucmc;    // McICA subcolumns [nlay][ncol][140]: the reference's (ngptlw, ncol, nlay)
+    // slant != 0: taucld holds the cloud's depth on each band's diffusivity
+    // path, odcld of the transfer (zero where cldprop gave none), as
+    // lw_cloud_wp_kernel forms it from water paths
+    int slant;
+    // cldprop's input mode (inflglw, iceflglw, liqflglw) and radii [nlay][ncol]
+    int inflag, iceflag, liqflag;
+    const double *reice, *reliq;
 };
+// Water-path cloud optics (inflglw 1, 2): cldprop's inputs and lwcldpr's
+// coefficients, packed as abscld1, absice0[2], absice1[5][2],
+// absice2[16][43], absice3[16][46], absliq0, absliq1[16][58] (LW_CLD_TAB
+// doubles)
+constexpr int LW_CLD_TAB = 1 + 2 + 10 + 688 + 736 + 1 + 928;
+struct LwCloudWp {
+    int inflag, iceflag, liqflag;
+    const double *cldfr, *taucld, *cicewp, *cliqwp, *reice, *reliq;  // taucld, paths, radii may be NULL (zero)
+    const double *tab;                                                // LW_CLD_TAB doubles
+    double *od;                                                       // out: [nlay][ncol][16] slant depths
+    int *bad;                                                         // out: layers with a radius the reference stops on
+};
+// after launch_lw_prep (the columns' precipitable water sets the
+// diffusivities); a.bad is cleared first
+hipError_t launch_lw_cloud_wp(const ColumnInputs &in, const LwWork &w, const LwCloudWp &a, const LwBand *bands,
+                              hipStream_t s);
 constexpr int LW_MCICA = 4;  // LwClouds::icld of the McICA driver's transfer
 // McICA subcolumns of mcica_subcol_lw / mcica_subcol_sw with the KISS
 // generator: inputs [nlay][ncol] (tauc, ssac, asmc, fsfc [nlay][ncol][16 or
@@ -158,6 +181,9 @@ struct LwCloudWork {
     void *bflux = nullptr;    // [16][4][nlay+1][ncol] of Real: total up, total down, clear up, clear down
     void *rows = nullptr;     // [chunk][4][nlay][64] of Real: per-layer transmittances and sources of a wave
     int chunk = 0;            // wave units per band-kernel launch (rows)
+    double *od = nullptr;     // water-path clouds: [nlay][ncol][16] slant depths
+    int *bad = nullptr;       // water-path clouds: radius count, or NULL
+    const double *cldtab = nullptr;  // water-path clouds: lwcldpr's coefficients on the device
 };
 inline size_t lw_cloud_row_bytes(int nlay, size_t real_size) { return real_size * 4 * (size_t)nlay * 64; }
 // mode 0 (FP64) or 1 (FP32 with FP64 islands); w: the forward's prep

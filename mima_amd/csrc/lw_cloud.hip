@@ -59,7 +59,7 @@ __global__ __launch_bounds__(64) void lw_cloud_column_kernel(int ncol, int nlay,
         if (cloudy(lay)) fl |= LWC_CLOUDY;
         if (!cloudy(lay - 1)) fl |= LWC_TOP_UP;
         if (!cloudy(lay + 1)) fl |= LWC_TOP_DN;
-        if (f >= 1.e-20 && (cwp >= 1.e-20 || tauctot >= 1.e-20)) fl |= LWC_TAU;
+        if (cl.slant || (f >= 1.e-20 && (cwp >= 1.e-20 || tauctot >= 1.e-20))) fl |= LWC_TAU;
         cw.flags[ix] = fl;
         put(lay, LWC_CF, f);
         for (int k = LWC_CLD1; k < LWC_COUNT; ++k) put(lay, k, 0.0);
@@ -177,6 +177,143 @@ __global__ __launch_bounds__(64) void lw_cloud_column_kernel(int ncol, int nlay,
         c2 = cld2;
         l2 = clr2;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Water-path cloud optics (inflglw = 1, 2)
+
+// cldprop with inflag 1 or 2 (rrtmg_lw_cldprop.f90:118-291) and the
+// transfer's use of its result (rrtmg_lw_rtrnmr.f90:315-317, 487-493), one
+// thread per column in FP64: per layer, the cloud test on the fraction, the
+// water path and the input depths; inflag 1: abscld1 times the path in
+// every band; inflag 2: ice absorption by iceflag 0 (absice0, one
+// coefficient), 1 (absice1, five bands), 2 or 3 (absice2 / absice3
+// interpolated in the radius) and liquid by liqflag 0 (absliq0) or 1
+// (absliq1 interpolated), combined over ncbands bands.  As in the
+// reference, ncbands starts at 1 and carries from layer to layer (iceflag 0
+// with no 16-band liquid leaves it), five-band ice without 16-band liquid
+// reads its coefficients by band (iceind 1 -> 2), and after the column the
+// final ncbands maps each spectral band to a cloud band (all to 1, the
+// five ice bands, or one to one), whose diffusivity multiplies that cloud
+// band's depth.  Out: the slant depths odcld per (layer, spectral band).
+// A radius the reference stops on is counted in *bad and clamped into the
+// table (the call then fails).  Contraction is off, as the reference
+// rounds each product and sum.
+__global__ __launch_bounds__(64) void lw_cloud_wp_kernel(ColumnInputs in, LwWork w, LwCloudWp a,
+                                                         const LwBand *__restrict__ bands)
+{
+#pragma clang fp contract(off)
+    const int ncol = in.ncol, nlay = in.nlay;
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= ncol) return;
+    const double *T = a.tab;
+    const double abscld1 = T[0], *absice0 = T + 1, *absice1 = T + 3, *absice2 = T + 13, *absice3 = T + 701;
+    const double absliq0 = T[1437], *absliq1 = T + 1438;
+    const unsigned char icb1[16] = {1, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 5, 5, 5, 5};
+    auto icb = [&](int ib, int ind) { return ind == 0 ? 1 : ind == 1 ? (int)icb1[ib - 1] : ib; };
+    int ncbands = 1, nbad = 0;
+    double abscoice[17], abscoliq[17];
+    for (int lay = 0; lay < nlay; ++lay) {
+        const size_t ix = (size_t)lay * ncol + col;
+        double *tc = a.od + ix * 16;
+        double tauctot = 0.0;
+        for (int b = 0; b < 16; ++b) {
+            tc[b] = 0.0;
+            tauctot = tauctot + (a.taucld ? a.taucld[ix * 16 + b] : 0.0);
+        }
+        const double ciwp = a.cicewp ? a.cicewp[ix] : 0.0, clwp = a.cliqwp ? a.cliqwp[ix] : 0.0;
+        const double cwp = ciwp + clwp;
+        if (!(a.cldfr[ix] >= 1.e-20 && (cwp >= 1.e-20 || tauctot >= 1.e-20))) continue;
+        if (a.inflag == 1) {
+            ncbands = 16;
+            for (int ib = 0; ib < 16; ++ib) tc[ib] = abscld1 * cwp;
+            continue;
+        }
+        const double radice = a.reice ? a.reice[ix] : 0.0;
+        int iceind = 0, liqind = 0;
+        if (ciwp == 0.0) {
+            abscoice[1] = 0.0;
+        } else if (a.iceflag == 0) {
+            nbad += radice < 10.0;
+            abscoice[1] = absice0[0] + absice0[1] / radice;
+        } else if (a.iceflag == 1) {
+            nbad += radice < 13.0 || radice > 130.0;
+            ncbands = 5;
+            for (int ib = 1; ib <= 5; ++ib) abscoice[ib] = absice1[(ib - 1) * 2] + absice1[(ib - 1) * 2 + 1] / radice;
+            iceind = 1;
+        } else {
+            const int nr = a.iceflag == 2 ? 43 : 46;
+            nbad += radice < 5.0 || radice > (a.iceflag == 2 ? 131.0 : 140.0);
+            ncbands = 16;
+            const double factor = (radice - 2.0) / 3.0;
+            int index = (int)factor;
+            if (index == nr) index = nr - 1;
+            index = index < 1 ? 1 : (index > nr - 1 ? nr - 1 : index);  // in-table only for valid radii
+            const double fint = factor - (double)index;
+            const double *tb = a.iceflag == 2 ? absice2 : absice3;
+            for (int ib = 1; ib <= 16; ++ib) {
+                const double *r = tb + (ib - 1) * nr;
+                abscoice[ib] = r[index - 1] + fint * (r[index] - (r[index - 1]));
+            }
+            iceind = 2;
+        }
+        if (clwp == 0.0) {
+            abscoliq[1] = 0.0;
+            if (iceind == 1) iceind = 2;
+        } else if (a.liqflag == 0) {
+            abscoliq[1] = absliq0;
+            if (iceind == 1) iceind = 2;
+        } else {
+            const double radliq = a.reliq ? a.reliq[ix] : 0.0;
+            nbad += radliq < 2.5 || radliq > 60.0;
+            int index = (int)(radliq - 1.5);
+            if (index == 0) index = 1;
+            if (index == 58) index = 57;
+            index = index < 1 ? 1 : (index > 57 ? 57 : index);
+            const double fint = radliq - 1.5 - (double)index;
+            ncbands = 16;
+            for (int ib = 1; ib <= 16; ++ib) {
+                const double *r = absliq1 + (ib - 1) * 58;
+                abscoliq[ib] = r[index - 1] + fint * (r[index] - (r[index - 1]));
+            }
+            liqind = 2;
+        }
+        for (int ib = 1; ib <= ncbands; ++ib)
+            tc[ib - 1] = ciwp * abscoice[icb(ib, iceind)] + clwp * abscoliq[icb(ib, liqind)];
+    }
+    if (nbad && a.bad) atomicAdd(a.bad, nbad);
+    // the transfer: spectral band iband takes cloud band ib = ipat(iband)
+    // with ib's diffusivity (rrtmg_lw_rtrnmr.f90:246-262, 315-317)
+    const double pwvcm = w.colscal[col];
+    double sd[16];
+    for (int b = 0; b < 16; ++b) {
+        if (b == 0 || b == 3 || b >= 9) {
+            sd[b] = 1.66;
+        } else {
+            const LwBand &B = bands[b];
+            double v = B.a0 + B.a1 * exp(B.a2 * pwvcm);
+            sd[b] = v > 1.80 ? 1.80 : (v < 1.50 ? 1.50 : v);
+        }
+    }
+    for (int lay = 0; lay < nlay; ++lay) {
+        double *tc = a.od + ((size_t)lay * ncol + col) * 16;
+        double t[16];
+        for (int b = 0; b < 16; ++b) t[b] = tc[b];
+        for (int b = 0; b < 16; ++b) {
+            const int ib = ncbands == 1 ? 0 : ncbands == 5 ? (int)icb1[b] - 1 : b;
+            tc[b] = sd[ib] * t[ib];
+        }
+    }
+}
+
+__global__ void lw_cloud_wp_zero_kernel(int *bad) { *bad = 0; }
+
+hipError_t launch_lw_cloud_wp(const ColumnInputs &in, const LwWork &w, const LwCloudWp &a, const LwBand *bands,
+                              hipStream_t s)
+{
+    if (a.bad) hipLaunchKernelGGL(lw_cloud_wp_zero_kernel, dim3(1), dim3(1), 0, s, a.bad);
+    hipLaunchKernelGGL(lw_cloud_wp_kernel, dim3((in.ncol + 63) / 64), dim3(64), 0, s, in, w, a, bands);
+    return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------
@@ -360,6 +497,7 @@ __device__ __forceinline__ void lw_cloud_band_body(const ColumnInputs &in, const
             const size_t o = ((size_t)lay * ncol + col) * 140 + B.ngs + g;
             return cl.cldfmc[o] == 1.0 ? secdiff * (Real)cl.taucmc[o] : (Real)0;
         }
+        if (cl.slant) return (fl & LWC_TAU) ? (Real)cl.taucld[((size_t)lay * ncol + col) * 16 + (BAND - 1)] : (Real)0;
         return (fl & LWC_TAU) ? secdiff * (Real)cl.taucld[((size_t)lay * ncol + col) * 16 + (BAND - 1)] : (Real)0;
     };
     // the cloud fraction the transfer weights a cloudy layer with: the
@@ -625,7 +763,7 @@ static void launch_lw_cloud_bands(const ColumnInputs &in, const LwClouds &cl, co
 }
 
 template <typename Real>
-static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, const LwOutputs &out,
+static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl_in, const LwOutputs &out,
                                     const LwWork &w, const LwCloudWork &cw, const LwDevTables &t, int mode,
                                     hipStream_t s, hipEvent_t *ev)
 {
@@ -633,6 +771,16 @@ static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl, 
     if (ev) (void)hipEventRecord(ev[0], s);
     hipError_t e = launch_lw_prep(in, w, t, mode, s);
     if (e != hipSuccess) return e;
+    LwClouds cl = cl_in;
+    if (cl.inflag != 0 && cl.icld != LW_MCICA) {
+        // water paths: cldprop's optics, then the transfer on its slant depths
+        LwCloudWp a{cl.inflag, cl.iceflag, cl.liqflag, cl.cldfr, cl.taucld, cl.cicewp, cl.cliqwp,
+                    cl.reice, cl.reliq, cw.cldtab, cw.od, cw.bad};
+        e = launch_lw_cloud_wp(in, w, a, t.bands, s);
+        if (e != hipSuccess) return e;
+        cl.taucld = cw.od;
+        cl.slant = 1;
+    }
     if (cl.icld == LW_MCICA) {
         const size_t n2 = (size_t)in.nlay * in.ncol;
         hipLaunchKernelGGL(lw_mcica_flags_kernel, dim3((unsigned)((n2 + MCICA_FLAG_PAIRS - 1) / MCICA_FLAG_PAIRS)),

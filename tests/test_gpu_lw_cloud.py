@@ -178,9 +178,9 @@ def test_error_contract(rr):
     c, cf, tau = _sample(16, 40, 11)
     args = [c[k] for k in KEYS]
     rr.set_precision("fp64")
-    with pytest.raises(RRTMGError) as e:   # cloud optics from water paths: not implemented
-        rr.rrtmg_lw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 1, 0, 0, cf, tau)
-    assert e.value.code == -95
+    with pytest.raises(RRTMGError) as e:   # no such cloud-input mode
+        rr.rrtmg_lw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 3, 0, 0, cf, tau)
+    assert e.value.code == -22
     with pytest.raises(RRTMGError) as e:   # idrv = 1
         rr.rrtmg_lw_cloud(c["ncol"], c["nlay"], 2, 1, *args, 0, 0, 0, cf, tau)
     assert e.value.code == -95
