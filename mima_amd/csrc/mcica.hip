@@ -202,14 +202,16 @@ __global__ __launch_bounds__(256) void mcica_subcol_kernel(McicaSubcolArgs a, co
         prev = x;
         fprev = f;
         const bool cloudy = x >= 1.0 - f;
-        a.cldfmcl[o] = cloudy ? 1.0 : 0.0;
-        a.taucmcl[o] = cloudy ? a.tauc[j] : 0.0;
-        if (a.ciwpmcl) a.ciwpmcl[o] = cloudy && a.ciwp ? a.ciwp[ix] : 0.0;
-        if (a.clwpmcl) a.clwpmcl[o] = cloudy && a.clwp ? a.clwp[ix] : 0.0;
+        // streaming stores: the subcolumns are written once and read by a
+        // later call
+        __builtin_nontemporal_store(cloudy ? 1.0 : 0.0, a.cldfmcl + o);
+        __builtin_nontemporal_store(cloudy ? a.tauc[j] : 0.0, a.taucmcl + o);
+        if (a.ciwpmcl) __builtin_nontemporal_store(cloudy && a.ciwp ? a.ciwp[ix] : 0.0, a.ciwpmcl + o);
+        if (a.clwpmcl) __builtin_nontemporal_store(cloudy && a.clwp ? a.clwp[ix] : 0.0, a.clwpmcl + o);
         if (sw) {
-            a.ssacmcl[o] = cloudy ? a.ssac[j] : 1.0;
-            a.asmcmcl[o] = cloudy ? a.asmc[j] : 0.0;
-            a.fsfcmcl[o] = cloudy ? a.fsfc[j] : 0.0;
+            __builtin_nontemporal_store(cloudy ? a.ssac[j] : 1.0, a.ssacmcl + o);
+            __builtin_nontemporal_store(cloudy ? a.asmc[j] : 0.0, a.asmcmcl + o);
+            __builtin_nontemporal_store(cloudy ? a.fsfc[j] : 0.0, a.fsfcmcl + o);
         }
     }
 }
