@@ -208,15 +208,21 @@ int rrtmg_hip_mcica_subcol_lw(int iplon, int ncol, int nlay, int icld, int permu
  * rrtmg_hip_mcica_subcol_lw.  The transfer is rtrnmc (rrtmg_lw_rtrnmc.f90):
  * random overlap per g-point with the subcolumn's cloud fraction, a layer
  * taking the cloudy branch for every g-point where any g-point's cldfmcl is
- * 1.  inflglw = 0 only (cldprmc, rrtmg_lw_cldprmc.f90:163-168): taucmcl
- * holds the subcolumns' in-cloud optical depths.  cldfmcl, taucmcl (140,
- * ncol, nlay), g-point fastest; ciwpmcl, clwpmcl, reicmcl, relqmcl,
- * iceflglw and liqflglw are not read.  Unlike the other LW entry points,
+ * 1.  cldprmc (rrtmg_lw_cldprmc.f90:150-230): inflglw = 0, taucmcl holds
+ * the subcolumns' in-cloud optical depths (ciwpmcl, clwpmcl, reicmcl,
+ * relqmcl, iceflglw, liqflglw not read); inflglw = 2, a subcolumn passing
+ * the cloud test takes its depth from its water paths ciwpmcl, clwpmcl
+ * (140, ncol, nlay; NULL = 0) and the layer's radii reicmcl, relqmcl
+ * (ncol, nlay) with lwcldpr's coefficients (rrtmg_lw_cld.tbl), by
+ * iceflglw 0-3 and liqflglw 0-1, a radius outside its range returning
+ * RRTMG_HIP_EINVAL as for rrtmg_hip_lw_cloud; inflglw = 1 returns
+ * RRTMG_HIP_EINVAL (the reference stops).  cldfmcl, taucmcl (140, ncol,
+ * nlay), g-point fastest.  Unlike the other LW entry points,
  * h2ovmr and o3vmr are volume mixing ratios: this driver's inatm takes them
  * as given (rrtmg_lw_rad.f90:778, 784).  icld outside 0..3 is taken as 2
  * (:451); icld = 0 leaves the subcolumns at zero (:743-744, 874) and runs
- * the clear-sky kernels.  inflglw != 0, idrv != 0 and
- * RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+ * the clear-sky kernels.  idrv != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
+ * RRTMG_HIP_ENOTSUP. */
 int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv,
                        const double *play, const double *plev, const double *tlay,
                        const double *tlev, const double *tsfc,

@@ -779,7 +779,8 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     if (cloudy && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
         return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
     const bool wp = cloudy && cl->icld != LW_MCICA && cl->inflag != 0;
-    if (wp && !s->lwd.cld)
+    const bool mwp = cloudy && cl->icld == LW_MCICA && cl->inflag == 2;
+    if ((wp || mwp) && !s->lwd.cld)
         return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0 needs the cloud optics table rrtmg_lw_cld.tbl beside "
                                        "rrtmg_lw.tbl at rrtmg_hip_lw_init");
     if (idrv != 0) return fail(RRTMG_HIP_ENOTSUP, "idrv != 0: dF/dTs output is not implemented");
@@ -817,6 +818,10 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         if (cloudy && cl->icld == LW_MCICA) {
             a.push_back({"cldfmcl", cl->cldfmc, n2 * 140});
             a.push_back({"taucmcl", cl->taucmc, n2 * 140});
+            if (mwp && cl->ciwpmc) a.push_back({"ciwpmcl", cl->ciwpmc, n2 * 140});
+            if (mwp && cl->clwpmc) a.push_back({"clwpmcl", cl->clwpmc, n2 * 140});
+            if (mwp && cl->reice) a.push_back({"reicmcl", cl->reice, n2});
+            if (mwp && cl->reliq) a.push_back({"relqmcl", cl->reliq, n2});
         } else if (cloudy) {
             a.push_back({"cldfr", cl->cldfr, n2});
             if (cl->taucld) a.push_back({"taucld", cl->taucld, n2 * 16});
@@ -853,8 +858,8 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     const size_t o_cbf = off; if (cloudy) off = align_up(off + rs * 16 * 4 * n3);
     const size_t crows = lds ? (size_t)lw_lds_grid(s->lwd.plan, ncol, s->lw.tab.nwaves) * LW_WGW : (size_t)cchunk;
     const size_t o_crw = off; if (cloudy) off = align_up(off + crows * lw_cloud_row_bytes(nlay, rs));
-    const size_t o_cod = off; if (wp) off = align_up(off + sizeof(double) * 16 * n2);
-    const size_t o_cbd = off; if (wp) off = align_up(off + sizeof(int));
+    const size_t o_cod = off; if (wp || mwp) off = align_up(off + sizeof(double) * (mwp ? 140 : 16) * n2);
+    const size_t o_cbd = off; if (wp || mwp) off = align_up(off + sizeof(int));
     if ((rc = W.lw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
     char *wb = (char *)W.lw_work.p;
     LwWork w;
@@ -877,7 +882,7 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
         cw.rows = wb + o_crw;
         cw.chunk = cchunk;
         cld = *cl;
-        if (wp) {
+        if (wp || mwp) {
             cw.od = (double *)(wb + o_cod);
             cw.bad = (int *)(wb + o_cbd);
             cw.cldtab = s->lwd.cld;
@@ -891,7 +896,9 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     LwOutputs out{};
     if (ptr_kind == RRTMG_HIP_HOST) {
         const size_t ncl = !cloudy ? 0
-                           : cl->icld == LW_MCICA ? 280 * n2
+                           : cl->icld == LW_MCICA ? 280 * n2 + (mwp && cl->ciwpmc ? 140 * n2 : 0) +
+                                                        (mwp && cl->clwpmc ? 140 * n2 : 0) +
+                                                        (mwp && cl->reice ? n2 : 0) + (mwp && cl->reliq ? n2 : 0)
                                                   : n2 + (cl->taucld ? 16 * n2 : 0) + (cl->cicewp ? n2 : 0) +
                                                         (cl->cliqwp ? n2 : 0) + (wp && cl->reice ? n2 : 0) +
                                                         (wp && cl->reliq ? n2 : 0);
@@ -927,11 +934,12 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
             const bool mc = cl->icld == LW_MCICA;
             const double *src[] = {mc ? cl->cldfmc : cl->cldfr, mc ? cl->taucmc : cl->taucld,
                                    mc ? nullptr : cl->cicewp, mc ? nullptr : cl->cliqwp,
-                                   wp ? cl->reice : nullptr, wp ? cl->reliq : nullptr};
+                                   wp || mwp ? cl->reice : nullptr, wp || mwp ? cl->reliq : nullptr,
+                                   mwp ? cl->ciwpmc : nullptr, mwp ? cl->clwpmc : nullptr};
             const double **dst[] = {mc ? &cld.cldfmc : &cld.cldfr, mc ? &cld.taucmc : &cld.taucld, &cld.cicewp,
-                                    &cld.cliqwp, &cld.reice, &cld.reliq};
-            const size_t len[] = {mc ? 140 * n2 : n2, mc ? 140 * n2 : 16 * n2, n2, n2, n2, n2};
-            for (int i = 0; i < 6; ++i) {
+                                    &cld.cliqwp, &cld.reice, &cld.reliq, &cld.ciwpmc, &cld.clwpmc};
+            const size_t len[] = {mc ? 140 * n2 : n2, mc ? 140 * n2 : 16 * n2, n2, n2, n2, n2, 140 * n2, 140 * n2};
+            for (int i = 0; i < 8; ++i) {
                 if (!src[i]) continue;
                 ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
                 *dst[i] = p;
@@ -965,7 +973,7 @@ static int lw_forward(int ncol, int nlay, int icld, int idrv, const double *play
     hipError_t e = cloudy ? launch_lw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(1))
                           : launch_lw(in, out, w, t, g_prec, st, s->events(1));
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string("LW kernel launch: ") + hipGetErrorString(e));
-    if (wp && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
+    if ((wp || mwp) && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
         // the radii cldprop stops on (rrtmg_lw_cldprop.f90:184-259); the
         // host-pointer path checked before staging
         int nb = 0;
@@ -1194,12 +1202,6 @@ int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv, const double *pla
                        const double *relqmcl, const double *tauaer, double *uflx, double *dflx, double *hr,
                        double *uflxc, double *dflxc, double *hrc, int ptr_kind, void *stream)
 {
-    (void)iceflglw;
-    (void)liqflglw;
-    (void)ciwpmcl;
-    (void)clwpmcl;
-    (void)reicmcl;
-    (void)relqmcl;
     // rrtmg_lw_rad.f90:451: icld outside 0..3 is taken as 2
     if (icld < 0 || icld > 3) icld = 2;
     // icld = 0: inatm leaves the subcolumns at zero (rrtmg_lw_rad.f90:743-744, 874)
@@ -1207,15 +1209,40 @@ int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv, const double *pla
         return lw_forward(ncol, nlay, 0, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr,
                           n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, nullptr, uflx, dflx,
                           hr, uflxc, dflxc, hrc, ptr_kind, stream, true);
-    if (inflglw != 0)
-        return fail(RRTMG_HIP_ENOTSUP, "inflglw != 0: cloud optics from water paths and effective radii are not "
-                                       "implemented; pass the subcolumns' in-cloud optical depths with inflglw = 0");
+    // rrtmg_lw_cldprmc.f90:156-158: inflag 1 stops with McICA
+    if (inflglw != 0 && inflglw != 2)
+        return fail(RRTMG_HIP_EINVAL, "inflglw must be 0 or 2 with McICA (the reference stops on 1)");
+    if (inflglw == 2 && (iceflglw < 0 || iceflglw > 3 || liqflglw < 0 || liqflglw > 1))
+        return fail(RRTMG_HIP_EINVAL, "iceflglw must be 0..3 and liqflglw 0..1 with inflglw = 2");
     if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
-        const size_t n = (size_t)ncol * nlay * 140;
+        const size_t n2 = (size_t)ncol * nlay, n = n2 * 140;
         int rc;
         if (ptr_kind == RRTMG_HIP_HOST) {
             if ((rc = check_host_array(cldfmcl, n, "cldfmcl")) || (rc = check_host_array(taucmcl, n, "taucmcl")))
                 return rc;
+            if (inflglw == 2) {
+                if ((ciwpmcl && (rc = check_host_array(ciwpmcl, n, "ciwpmcl"))) ||
+                    (clwpmcl && (rc = check_host_array(clwpmcl, n, "clwpmcl"))) ||
+                    (reicmcl && (rc = check_host_array(reicmcl, n2, "reicmcl"))) ||
+                    (relqmcl && (rc = check_host_array(relqmcl, n2, "relqmcl"))))
+                    return rc;
+                // the radii cldprmc stops on (rrtmg_lw_cldprmc.f90:163-218),
+                // in the subcolumns that pass its cloud test
+                for (size_t t = 0; t < n; ++t) {
+                    const double ci = ciwpmcl ? ciwpmcl[t] : 0.0, cw = clwpmcl ? clwpmcl[t] : 0.0;
+                    if (!(cldfmcl[t] >= 1.e-20 && (ci + cw >= 1.e-20 || taucmcl[t] >= 1.e-20))) continue;
+                    const size_t ix = t / 140;
+                    const double ri = reicmcl ? reicmcl[ix] : 0.0, rl = relqmcl ? relqmcl[ix] : 0.0;
+                    const bool bi = ci != 0.0 && (iceflglw == 0   ? ri < 10.0
+                                                  : iceflglw == 1 ? ri < 13.0 || ri > 130.0
+                                                  : iceflglw == 2 ? ri < 5.0 || ri > 131.0
+                                                                  : ri < 5.0 || ri > 140.0);
+                    const bool bl = cw != 0.0 && liqflglw == 1 && (rl < 2.5 || rl > 60.0);
+                    if (bi || bl)
+                        return fail(RRTMG_HIP_EINVAL, "an ice or liquid effective radius outside the range of its "
+                                                      "parameterisation in a cloudy subcolumn (the reference stops)");
+                }
+            }
         } else if (!cldfmcl || !taucmcl) {
             return fail(RRTMG_HIP_EINVAL, "null cloud array");
         }
@@ -1224,6 +1251,15 @@ int rrtmg_hip_lw_mcica(int ncol, int nlay, int icld, int idrv, const double *pla
     cl.icld = LW_MCICA;
     cl.cldfmc = cldfmcl;
     cl.taucmc = taucmcl;
+    if (inflglw == 2) {
+        cl.inflag = 2;
+        cl.iceflag = iceflglw;
+        cl.liqflag = liqflglw;
+        cl.ciwpmc = ciwpmcl;
+        cl.clwpmc = clwpmcl;
+        cl.reice = reicmcl;
+        cl.reliq = relqmcl;
+    }
     return lw_forward(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                       o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, tauaer, &cl, uflx, dflx, hr, uflxc, dflxc,
                       hrc, ptr_kind, stream, true);

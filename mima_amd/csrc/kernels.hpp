@@ -119,6 +119,7 @@ struct LwClouds {
     // cldprop's input mode (inflglw, iceflglw, liqflglw) and radii [nlay][ncol]
     int inflag, iceflag, liqflag;
     const double *reice, *reliq;
+    const double *ciwpmc, *clwpmc;  // McICA with inflag 2: subcolumn water paths [nlay][ncol][140]
 };
 // Water-path cloud optics (inflglw 1, 2): cldprop's inputs and lwcldpr's
 // coefficients, packed as abscld1, absice0[2], absice1[5][2],
@@ -181,7 +182,7 @@ struct LwCloudWork {
     void *bflux = nullptr;    // [16][4][nlay+1][ncol] of Real: total up, total down, clear up, clear down
     void *rows = nullptr;     // [chunk][4][nlay][64] of Real: per-layer transmittances and sources of a wave
     int chunk = 0;            // wave units per band-kernel launch (rows)
-    double *od = nullptr;     // water-path clouds: [nlay][ncol][16] slant depths
+    double *od = nullptr;     // water-path clouds: [nlay][ncol][16] slant depths (McICA: [nlay][ncol][140] depths)
     int *bad = nullptr;       // water-path clouds: radius count, or NULL
     const double *cldtab = nullptr;  // water-path clouds: lwcldpr's coefficients on the device
 };

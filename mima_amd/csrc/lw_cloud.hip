@@ -306,6 +306,73 @@ __global__ __launch_bounds__(64) void lw_cloud_wp_kernel(ColumnInputs in, LwWork
     }
 }
 
+// cldprmc with inflag 2 (rrtmg_lw_cldprmc.f90:150-230), one thread per
+// (layer, column, g-point): where the subcolumn passes the cloud test, its
+// in-cloud depth from its water paths and the layer's radii with the
+// coefficients of the g-point's band (five-band ice by icb of the band);
+// elsewhere the input depth.  Contraction off, as above.
+__global__ __launch_bounds__(256) void lw_mcica_wp_kernel(int ncol, int nlay, LwClouds cl, const double *tab,
+                                                          double *out, int *bad)
+{
+#pragma clang fp contract(off)
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)nlay * ncol * 140) return;
+    const size_t ix = t / 140;
+    const int ig = (int)(t - ix * 140);
+    const double *T = tab;
+    const double *absice0 = T + 1, *absice1 = T + 3, *absice2 = T + 13, *absice3 = T + 701;
+    const double absliq0 = T[1437], *absliq1 = T + 1438;
+    const double cf = cl.cldfmc[t], tin = cl.taucmc[t];
+    const double ci = cl.ciwpmc ? cl.ciwpmc[t] : 0.0, cw = cl.clwpmc ? cl.clwpmc[t] : 0.0;
+    double tau = tin;
+    int nbad = 0;
+    if (cf >= 1.e-20 && (ci + cw >= 1.e-20 || tin >= 1.e-20)) {
+        int b = 0, end = lw_ng(1);
+        while (ig >= end) end += lw_ng(++b + 1);
+        const int ib = b + 1;
+        const double radice = cl.reice ? cl.reice[ix] : 0.0;
+        double aice = 0.0, aliq = 0.0;
+        if (ci == 0.0) {
+            aice = 0.0;
+        } else if (cl.iceflag == 0) {
+            nbad += radice < 10.0;
+            aice = absice0[0] + absice0[1] / radice;
+        } else if (cl.iceflag == 1) {
+            nbad += radice < 13.0 || radice > 130.0;
+            const int b5 = ib <= 1 ? 1 : ib == 2 ? 2 : ib <= 5 ? 3 : ib <= 8 ? 4 : 5;
+            aice = absice1[(b5 - 1) * 2] + absice1[(b5 - 1) * 2 + 1] / radice;
+        } else {
+            const int nr = cl.iceflag == 2 ? 43 : 46;
+            nbad += radice < 5.0 || radice > (cl.iceflag == 2 ? 131.0 : 140.0);
+            const double factor = (radice - 2.0) / 3.0;
+            int index = (int)factor;
+            if (index == nr) index = nr - 1;
+            index = index < 1 ? 1 : (index > nr - 1 ? nr - 1 : index);
+            const double fint = factor - (double)index;
+            const double *r = (cl.iceflag == 2 ? absice2 : absice3) + (ib - 1) * nr;
+            aice = r[index - 1] + fint * (r[index] - (r[index - 1]));
+        }
+        if (cw == 0.0) {
+            aliq = 0.0;
+        } else if (cl.liqflag == 0) {
+            aliq = absliq0;
+        } else {
+            const double radliq = cl.reliq ? cl.reliq[ix] : 0.0;
+            nbad += radliq < 2.5 || radliq > 60.0;
+            int index = (int)(radliq - 1.5);
+            if (index == 0) index = 1;
+            if (index == 58) index = 57;
+            index = index < 1 ? 1 : (index > 57 ? 57 : index);
+            const double fint = radliq - 1.5 - (double)index;
+            const double *r = absliq1 + (ib - 1) * 58;
+            aliq = r[index - 1] + fint * (r[index] - (r[index - 1]));
+        }
+        tau = ci * aice + cw * aliq;
+    }
+    out[t] = tau;
+    if (nbad && bad) atomicAdd(bad, nbad);
+}
+
 __global__ void lw_cloud_wp_zero_kernel(int *bad) { *bad = 0; }
 
 hipError_t launch_lw_cloud_wp(const ColumnInputs &in, const LwWork &w, const LwCloudWp &a, const LwBand *bands,
@@ -772,7 +839,14 @@ static hipError_t launch_lw_cloud_t(const ColumnInputs &in, const LwClouds &cl_i
     hipError_t e = launch_lw_prep(in, w, t, mode, s);
     if (e != hipSuccess) return e;
     LwClouds cl = cl_in;
-    if (cl.inflag != 0 && cl.icld != LW_MCICA) {
+    if (cl.inflag == 2 && cl.icld == LW_MCICA) {
+        // McICA with water paths: cldprmc's depths per subcolumn
+        const size_t n = (size_t)in.nlay * in.ncol * 140;
+        if (cw.bad) hipLaunchKernelGGL(lw_cloud_wp_zero_kernel, dim3(1), dim3(1), 0, s, cw.bad);
+        hipLaunchKernelGGL(lw_mcica_wp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in.ncol, in.nlay,
+                           cl, cw.cldtab, cw.od, cw.bad);
+        cl.taucmc = cw.od;
+    } else if (cl.inflag != 0 && cl.icld != LW_MCICA) {
         // water paths: cldprop's optics, then the transfer on its slant depths
         LwCloudWp a{cl.inflag, cl.iceflag, cl.liqflag, cl.cldfr, cl.taucld, cl.cicewp, cl.cliqwp,
                     cl.reice, cl.reliq, cw.cldtab, cw.od, cw.bad};

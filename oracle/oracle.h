@@ -185,6 +185,21 @@ int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *
                           const double *ciwpmcl, const double *clwpmcl, double *swuflx, double *swdflx,
                           double *swhr, double *swuflxc, double *swdflxc, double *swhrc);
 
+/* oracle_rrtmg_lw_mcica with cldprmc's water-path optics (inflglw = 2,
+ * rrtmg_lw_cldprmc.f90:150-230): per subcolumn that passes the cloud test,
+ * its depth from ciwpmcl, clwpmcl (140, ncol, nlay) and the layer's radii
+ * reicmcl, relqmcl (ncol, nlay) with its g-point's band's coefficients
+ * (after oracle_lw_cld_init).  -8 on a radius the reference stops on. */
+int oracle_rrtmg_lw_mcica_wp(int ncol, int nlay, int iceflag, int liqflag, const double *play, const double *plev,
+                             const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                             const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                             const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                             const double *emis, const double *cldfmcl, const double *taucmcl,
+                             const double *ciwpmcl, const double *clwpmcl, const double *reicmcl,
+                             const double *relqmcl, const double *tauaer, double *uflx, double *dflx, double *hr,
+                             double *uflxc, double *dflxc, double *hrc);
+
 /* A reduced table by name ("sw/b16/ka", "lw/b03/fracrefa", ...): copies up to
  * n values (Fortran order) into out and returns the element count, or -1. */
 long oracle_reduced(const char *name, double *out, long n);

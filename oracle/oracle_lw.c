@@ -2053,7 +2053,8 @@ static void rtrnmc(const lwcol_t *c, double (*cldfmc)[NGLW + 1], double (*taucmc
  * idrv = 0; h2ovmr and o3vmr are volume mixing ratios, as that driver's
  * inatm takes them (it lacks MiMA's conversions).  cldfmcl, taucmcl (140, ncol, nlay); tauaer (ncol, nlay, 16) or
  * NULL. */
-int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+static int lw_mcica_driver(int ncol, int nlay, int iceflag, int liqflag, const double *ciwpmcl,
+                           const double *clwpmcl, const double *reicmcl, const double *relqmcl, const double *play, const double *plev, const double *tlay,
                           const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
                           const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
                           const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
@@ -2072,6 +2073,7 @@ int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *
         double (*taua)[17] = malloc(sizeof(double[MAXLAY + 2][17]));
         double tu[MAXLAY + 2], td[MAXLAY + 2], tuc[MAXLAY + 2], tdc[MAXLAY + 2], htr[MAXLAY + 2],
             htrc[MAXLAY + 2];
+        int bad = 0;
         if (!c || !cf || !tc || !taua) {
 #pragma omp atomic write
             err = -3;
@@ -2087,8 +2089,62 @@ int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *
                     cf[l][ig] = cldfmcl[(ig - 1) + (long)NGLW * ix];
                     tc[l][ig] = taucmcl[(ig - 1) + (long)NGLW * ix];
                 }
+                /* cldprmc with inflag 2 (rrtmg_lw_cldprmc.f90:150-230): per
+                 * g-point, where the subcolumn passes the cloud test, its
+                 * depth from its water paths and the layer's radii with the
+                 * coefficients of the g-point's band */
+                if (iceflag >= 0) {
+                    static const int icb[16] = {1, 2, 3, 3, 3, 4, 4, 4, 5, 5, 5, 5, 5, 5, 5, 5};
+                    for (int ig = 1; ig <= NGLW; ++ig) {
+                        long o = (ig - 1) + (long)NGLW * ix;
+                        double ci = ciwpmcl ? ciwpmcl[o] : 0.0, cw = clwpmcl ? clwpmcl[o] : 0.0;
+                        if (!(cf[l][ig] >= 1.e-20 && (ci + cw >= 1.e-20 || tc[l][ig] >= 1.e-20))) continue;
+                        int ib = 1;
+                        while (ig > L.ngs[ib - 1]) ++ib;
+                        double radice = reicmcl ? reicmcl[ix] : 0.0, aice = 0.0, aliq = 0.0;
+                        if (ci == 0.0) {
+                            aice = 0.0;
+                        } else if (iceflag == 0) {
+                            if (radice < 10.0) bad = 1;
+                            aice = LC.absice0[0] + LC.absice0[1] / radice;
+                        } else if (iceflag == 1) {
+                            if (radice < 13.0 || radice > 130.0) bad = 1;
+                            int b5 = icb[ib - 1];
+                            aice = LC.absice1[b5 - 1][0] + LC.absice1[b5 - 1][1] / radice;
+                        } else {
+                            const int nr = iceflag == 2 ? 43 : 46;
+                            if (radice < 5.0 || radice > (iceflag == 2 ? 131.0 : 140.0)) bad = 1;
+                            double factor = (radice - 2.0) / 3.0;
+                            int index = (int)factor;
+                            if (index == nr) index = nr - 1;
+                            double fint = factor - (double)index;
+                            const double *a = iceflag == 2 ? LC.absice2[ib - 1] : LC.absice3[ib - 1];
+                            aice = a[index - 1] + fint * (a[index] - (a[index - 1]));
+                        }
+                        if (cw == 0.0) {
+                            aliq = 0.0;
+                        } else if (liqflag == 0) {
+                            aliq = LC.absliq0;
+                        } else {
+                            double radliq = relqmcl ? relqmcl[ix] : 0.0;
+                            if (radliq < 2.5 || radliq > 60.0) bad = 1;
+                            int index = (int)(radliq - 1.5);
+                            if (index == 0) index = 1;
+                            if (index == 58) index = 57;
+                            double fint = radliq - 1.5 - (double)index;
+                            const double *a = LC.absliq1[ib - 1];
+                            aliq = a[index - 1] + fint * (a[index] - (a[index - 1]));
+                        }
+                        tc[l][ig] = ci * aice + cw * aliq;
+                    }
+                }
                 for (int ib = 1; ib <= 16; ++ib)
                     taua[l][ib] = tauaer ? tauaer[iplon + (long)ncol * ((l - 1) + (long)nlay * (ib - 1))] : 0.0;
+            }
+            if (bad) {
+#pragma omp atomic write
+                err = -8;
+                continue;
             }
             setcoef(c);
             taumol(c);
@@ -2110,6 +2166,35 @@ int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *
         free(taua);
     }
     return err;
+}
+
+int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *tsfc, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *cfc11vmr, const double *cfc12vmr,
+                          const double *cfc22vmr, const double *ccl4vmr, const double *emis,
+                          const double *cldfmcl, const double *taucmcl, const double *tauaer, double *uflx,
+                          double *dflx, double *hr, double *uflxc, double *dflxc, double *hrc)
+{
+    return lw_mcica_driver(ncol, nlay, -1, 0, NULL, NULL, NULL, NULL, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr, ccl4vmr, emis, cldfmcl,
+                           taucmcl, tauaer, uflx, dflx, hr, uflxc, dflxc, hrc);
+}
+
+int oracle_rrtmg_lw_mcica_wp(int ncol, int nlay, int iceflag, int liqflag, const double *play, const double *plev,
+                             const double *tlay, const double *tlev, const double *tsfc, const double *h2ovmr,
+                             const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *cfc11vmr,
+                             const double *cfc12vmr, const double *cfc22vmr, const double *ccl4vmr,
+                             const double *emis, const double *cldfmcl, const double *taucmcl,
+                             const double *ciwpmcl, const double *clwpmcl, const double *reicmcl,
+                             const double *relqmcl, const double *tauaer, double *uflx, double *dflx, double *hr,
+                             double *uflxc, double *dflxc, double *hrc)
+{
+    if (!LC.ready || iceflag < 0 || iceflag > 3) return -1;
+    return lw_mcica_driver(ncol, nlay, iceflag, liqflag, ciwpmcl, clwpmcl, reicmcl, relqmcl, play, plev, tlay,
+                           tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr, cfc11vmr, cfc12vmr, cfc22vmr,
+                           ccl4vmr, emis, cldfmcl, taucmcl, tauaer, uflx, dflx, hr, uflxc, dflxc, hrc);
 }
 
 static void one_lw(lwcol_t *c, int nlay, const double *play, const double *plev, const double *tlay,

@@ -153,6 +153,31 @@ def rrtmg_lw_cloud_wp(c, icld, inflag, iceflag, liqflag, cldfr, cicewp, cliqwp, 
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+def rrtmg_lw_mcica_wp(c, iceflag, liqflag, sub, reicmcl, relqmcl, tauaer=None):
+    """rrtmg_lw of the McICA driver with cldprmc's water-path optics
+    (inflglw = 2) on subcolumns (nlay, ncol, 140) incl. ciwpmcl, clwpmcl;
+    radii (nlay, ncol)."""
+    global _cld_ready
+    L = lib()
+    if not _cld_ready:
+        if L.oracle_lw_cld_init(LW_CLD_TABLE.encode()) != 0:
+            raise RuntimeError("oracle_lw_cld_init failed")
+        _cld_ready = True
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr",
+                                "ch4vmr", "n2ovmr", "o2vmr", "cfc11vmr", "cfc12vmr", "cfc22vmr",
+                                "ccl4vmr", "emis")]
+    sb = [_f64(a) for a in (sub["cldfmcl"], sub["taucmcl"], sub["ciwpmcl"], sub["clwpmcl"], reicmcl, relqmcl)]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_lw_mcica_wp(ncol, nlay, int(iceflag), int(liqflag), *[_p(a) for a in ins],
+                                    *[_p(a) for a in sb], None if tauaer is None else _p(_f64(tauaer)),
+                                    *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_lw_mcica_wp returned %d" % rc)
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
 def lib_ngs():
     """Cumulative LW g-points per band (the reduced table's ngs)."""
     from mima_amd.tables.extract_tables import read_table

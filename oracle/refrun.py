@@ -213,15 +213,16 @@ def run_sw_mcica(lib, c, icld, sub):
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
 
 
-def set_lw_cld(lib):
+def set_lw_cld(lib, setter="ref_lw_set_cld"):
     """Fill the LW reference library's rrlw_cld module with lwcldpr's
-    coefficients as extracted into rrtmg_lw_cld.tbl."""
+    coefficients as extracted into rrtmg_lw_cld.tbl (setter
+    ref_lw_mcica_set_cld for the McICA library)."""
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "mima_amd", "tables"))
     from extract_tables import read_table
     t, _, _ = read_table(os.path.join(os.path.dirname(HERE), "mima_amd", "tables", "rrtmg_lw_cld.tbl"))
     f = lambda k: np.asfortranarray(t["lwcld/" + k], dtype=np.float64)  # noqa: E731
     arrs = [f(k) for k in ("absice0", "absice1", "absice2", "absice3", "absliq1")]
-    lib.ref_lw_set_cld(ctypes.c_double(float(t["lwcld/abscld1"][0])), *[a.ctypes.data_as(ctypes.c_void_p) for a in
+    getattr(lib, setter)(ctypes.c_double(float(t["lwcld/abscld1"][0])), *[a.ctypes.data_as(ctypes.c_void_p) for a in
                        arrs[:4]], ctypes.c_double(float(t["lwcld/absliq0"][0])),
                        arrs[4].ctypes.data_as(ctypes.c_void_p))
     return arrs
@@ -240,4 +241,20 @@ def run_lw_cloud_wp(lib, c, icld, inflag, iceflag, liqflag, cldfr, cicewp, cliqw
             np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
     lib.ref_lw_run_cloud_wp(n, L, int(icld), int(inflag), int(iceflag), int(liqflag), *[_p(a) for a in ins],
                             *[_p(a) for a in cl], _p(tau), *[_p(a) for a in outs])
+    return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
+def run_lw_mcica_wp(lib, c, icld, iceflag, liqflag, sub, reicmcl, relqmcl, tauaer=None):
+    """rrtmg_lw of the reference's McICA driver with cldprmc's water-path
+    optics (inflglw = 2) on subcolumns `sub` (after set_lw_cld(lib,
+    "ref_lw_mcica_set_cld")); radii (nlay, ncol)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in LW_IN]
+    tau = np.zeros((16, L, n)) if tauaer is None else np.ascontiguousarray(tauaer, dtype=np.float64)
+    sb = [np.ascontiguousarray(a, dtype=np.float64) for a in
+          (sub["cldfmcl"], sub["taucmcl"], sub["ciwpmcl"], sub["clwpmcl"], reicmcl, relqmcl)]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_lw_mcica_run_wp(n, L, int(icld), 2, int(iceflag), int(liqflag), *[_p(a) for a in ins],
+                            *[_p(a) for a in sb], _p(tau), *[_p(a) for a in outs])
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))

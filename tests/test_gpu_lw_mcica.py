@@ -249,7 +249,8 @@ def test_errors(rr):
     """Refusals: irng = 1 (Mersenne Twister not built), icld outside 0..3
     and nlay < 4 for the generator (the reference stops), a column whose
     pressure rises from the first layer (host check; device check with
-    rrtmg_hip_set_checks on), inflglw != 0, the smooth mode."""
+    rrtmg_hip_set_checks on), inflglw = 1 (the reference stops), the smooth
+    mode."""
     c, cf, tau, _ = _sample(64, 40, 70)
     ncol, nlay = c["ncol"], c["nlay"]
     with pytest.raises(rr.RRTMGError, match="irng"):
@@ -269,11 +270,56 @@ def test_errors(rr):
     finally:
         rr.set_checks(False)
     sub = _subcol(rr, c, 2, cf, tau)
-    with pytest.raises(rr.RRTMGError, match="inflglw"):
-        rr.rrtmg_lw_mcica(ncol, nlay, 2, 0, *[c[k] for k in KEYS], 2, 0, 0, sub["cldfmcl"], sub["taucmcl"])
+    with pytest.raises(rr.RRTMGError, match="inflglw"):   # inflag 1 stops with McICA
+        rr.rrtmg_lw_mcica(ncol, nlay, 2, 0, *[c[k] for k in KEYS], 1, 0, 0, sub["cldfmcl"], sub["taucmcl"])
     rr.set_precision("smooth")
     try:
         with pytest.raises(rr.RRTMGError, match="smooth"):
             rr.rrtmg_lw_mcica(ncol, nlay, 2, 0, *[c[k] for k in KEYS], 0, 0, 0, sub["cldfmcl"], sub["taucmcl"])
     finally:
         rr.set_precision("fp64")
+
+
+WP = dict(np.load(os.path.join(ROOT, "tests", "golden", "lw_ref_mcica_wp.npz"), allow_pickle=False))
+
+
+@pytest.mark.parametrize("k", range(7))
+def test_water_path_subcolumns_match_reference(rr, k):
+    """inflglw = 2 (cldprmc, rrtmg_lw_cldprmc.f90:150-230): each subcolumn's
+    depth from its water paths and the layer's radii; the reference McICA
+    driver's fluxes (tests/golden/lw_ref_mcica_wp.npz) for every ice and
+    liquid parameterisation, FP64 ≤1e-8, mixed ≤0.05 of FP64."""
+    c = {kk: WP[kk] for kk in KEYS}
+    c["ncol"], c["nlay"] = int(WP["ncol"]), int(WP["nlay"])
+    iceflag, liqflag = (int(x) for x in WP["cases"][k])
+    d = rr.to_device(c)
+    sub = [_t(WP["sub_" + kk]) for kk in ("cldfmcl", "taucmcl", "ciwpmcl", "clwpmcl")]
+    out = {}
+    for prec in ("fp64", "mixed"):
+        rr.set_precision(prec)
+        r = rr.rrtmg_lw_mcica(c["ncol"], c["nlay"], 2, 0, *[d[kk] for kk in KEYS], 2, iceflag, liqflag, sub[0],
+                              sub[1], ciwpmcl=sub[2], clwpmcl=sub[3], reicmcl=_t(WP["reice"]),
+                              relqmcl=_t(WP["reliq"]))
+        out[prec] = dict(zip(OUT, [x.cpu().numpy() for x in r]))
+    for kk in OUT:
+        np.testing.assert_allclose(out["fp64"][kk], WP["c%d_%s" % (k, kk)], rtol=0, atol=FP64_ATOL, err_msg=kk)
+        np.testing.assert_allclose(out["mixed"][kk], out["fp64"][kk], rtol=0, atol=MIXED_ATOL, err_msg=kk)
+
+
+def test_water_path_host_path_and_radius_check(rr):
+    c = {kk: WP[kk] for kk in KEYS}
+    c["ncol"], c["nlay"] = int(WP["ncol"]), int(WP["nlay"])
+    sub = {kk: WP["sub_" + kk] for kk in ("cldfmcl", "taucmcl", "ciwpmcl", "clwpmcl")}
+    rr.set_precision("fp64")
+    r = rr.rrtmg_lw_mcica(c["ncol"], c["nlay"], 2, 0, *[c[kk] for kk in KEYS], 2, 2, 1, sub["cldfmcl"],
+                          sub["taucmcl"], ciwpmcl=sub["ciwpmcl"], clwpmcl=sub["clwpmcl"], reicmcl=WP["reice"],
+                          relqmcl=WP["reliq"])
+    o = po.rrtmg_lw_mcica_wp(c, 2, 1, sub, WP["reice"], WP["reliq"])
+    for kk, v in zip(OUT, r):
+        np.testing.assert_allclose(v, o[kk], rtol=0, atol=FP64_ATOL, err_msg=kk)
+    bad = WP["reliq"].copy()
+    bad[:] = 70.0
+    with pytest.raises(rr.RRTMGError, match="radius"):
+        rr.rrtmg_lw_mcica(c["ncol"], c["nlay"], 2, 0, *[c[kk] for kk in KEYS], 2, 2, 1, sub["cldfmcl"],
+                          sub["taucmcl"], ciwpmcl=sub["ciwpmcl"], clwpmcl=sub["clwpmcl"], reicmcl=WP["reice"],
+                          relqmcl=bad)

@@ -73,3 +73,19 @@ def test_zero_paths_are_clear():
     o = po.rrtmg_lw_cloud_wp(c, 2, 2, 3, 1, GOLD["cldfr"], z, z, GOLD["reice"], GOLD["reliq"])
     for a, b in (("uflx", "uflxc"), ("dflx", "dflxc"), ("hr", "hrc")):
         np.testing.assert_allclose(o[a], o[b], rtol=0, atol=1e-9)
+
+
+MC = dict(np.load(os.path.join(ROOT, "tests", "golden", "lw_ref_mcica_wp.npz"), allow_pickle=False))
+
+
+@pytest.mark.parametrize("k", range(7))
+def test_mcica_water_path_optics_match_reference(k):
+    """cldprmc with inflglw = 2 (rrtmg_lw_cldprmc.f90:150-230) under the
+    McICA driver: the reference's fluxes on its own subcolumns."""
+    c = {kk: MC[kk] for kk in KEYS}
+    c["ncol"], c["nlay"] = int(MC["ncol"]), int(MC["nlay"])
+    sub = {kk: MC["sub_" + kk] for kk in ("cldfmcl", "taucmcl", "ciwpmcl", "clwpmcl")}
+    iceflag, liqflag = (int(x) for x in MC["cases"][k])
+    o = po.rrtmg_lw_mcica_wp(c, iceflag, liqflag, sub, MC["reice"], MC["reliq"])
+    for kk in OUT:
+        np.testing.assert_allclose(o[kk], MC["c%d_%s" % (k, kk)], rtol=1e-12, atol=1e-11, err_msg=kk)
