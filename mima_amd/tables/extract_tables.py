@@ -13,8 +13,9 @@ compiler would parse each ``_rb`` literal, i.e. correctly rounded doubles):
   bands 16-29 (``rrtmg_sw_k_g.f90``), the band/g-point bookkeeping of
   ``swdatinit``/``swcmbdat`` (``rrtmg_sw_init.f90:157-367``) and the MLS
   reference atmosphere of ``swatmref`` (``rrtmg_sw_setcoef.f90:289-343``).
-* ``rrtmg_lw_cld.tbl`` -- the LW cloud absorption coefficients of ``lwcldpr``
-  (``rrtmg_lw_init.f90``), for cldprop's water-path cloud optics.
+* ``rrtmg_lw_cld.tbl`` / ``rrtmg_sw_cld.tbl`` -- the cloud optics
+  coefficients of ``lwcldpr`` (``rrtmg_lw_init.f90``) and ``swcldpr``
+  (``rrtmg_sw_init.f90``), for cldprop's water-path cloud optics.
 * ``rrtmg_lw.tbl`` -- RRTMG_LW bookkeeping (``rrtmg_lw_init.f90:178-363``),
   reference atmosphere and ``chi_mls`` (``rrtmg_lw_setcoef.f90:418-578``),
   the Planck table ``totplnk`` (``rrtmg_lw_setcoef.f90:581-1284``), the
@@ -351,6 +352,41 @@ def extract_lw_cloud(ref):
     return out, prov
 
 
+def extract_sw_cloud(ref):
+    """The SW cloud optics coefficients of swcldpr (rrtmg_sw_init.f90,
+    module rrsw_cld) for cldprop_sw's inflgsw = 2, bands 16-29 as columns
+    1-14, and the bands' upper wavenumbers (swdatinit) that pick the
+    iceflgsw = 1 coefficients."""
+    init = _read(ref, SW + "/src/rrtmg_sw_init.f90")
+    prov = {SW + "/src/rrtmg_sw_init.f90": _sha(ref, SW + "/src/rrtmg_sw_init.f90")}
+    fa = FortranArrays()
+    shapes = {}
+    for n in ("extliq1", "ssaliq1", "asyliq1"):
+        shapes[n] = [58, (16, 29)]
+    for n in ("extice2", "ssaice2", "asyice2"):
+        shapes[n] = [43, (16, 29)]
+    for n in ("extice3", "ssaice3", "asyice3", "fdlice3"):
+        shapes[n] = [46, (16, 29)]
+    for n in ("abari", "bbari", "cbari", "dbari", "ebari", "fbari"):
+        shapes[n] = [5]
+    for n, sh in shapes.items():
+        fa.declare(n, sh)
+    fa.scan(_subroutine_body(init, "swcldpr"), set(shapes))
+    fw = FortranArrays()
+    fw.declare("wavenum2", [(16, 29)])
+    fw.scan(_subroutine_body(init, "swdatinit"), {"wavenum2"})
+    out = {}
+    for n in shapes:
+        a = fa.arrays[n][0]
+        if np.isnan(a).any():
+            raise ValueError("swcld %s has unset entries" % n)
+        out["swcld/" + n] = a
+    if np.isnan(fw.arrays["wavenum2"][0]).any():
+        raise ValueError("sw wavenum2 has unset entries")
+    out["swcld/wavenum2"] = fw.arrays["wavenum2"][0]
+    return out, prov
+
+
 def lw_shapes(ref):
     """Declared shapes of the original-resolution (16 g) LW arrays per band."""
     shapes = {}
@@ -511,7 +547,8 @@ def main():
     ap.add_argument("--out", default=HERE)
     ap.add_argument("--lw-seed", type=int, default=20261202)
     ap.add_argument("--lw-cloud-only", action="store_true",
-                    help="only write rrtmg_lw_cld.tbl (lwcldpr's cloud absorption coefficients)")
+                    help="only write rrtmg_lw_cld.tbl and rrtmg_sw_cld.tbl (lwcldpr's and swcldpr's cloud "
+                         "optics coefficients)")
     ap.add_argument("--shapes-header", default=None,
                     help="only write the band-array shape header (C++) of the committed tables to this path")
     args = ap.parse_args()
@@ -519,16 +556,16 @@ def main():
         write_shapes_header(args.shapes_header)
         return 0
     if args.lw_cloud_only:
-        cld, prov = extract_lw_cloud(args.ref)
-        path = os.path.join(args.out, "rrtmg_lw_cld.tbl")
-        write_table(path, cld, 0)
         mpath = os.path.join(args.out, "manifest.json")
         manifest = json.load(open(mpath)) if os.path.exists(mpath) else {}
-        manifest["rrtmg_lw_cld.tbl"] = {"sha256": hashlib.sha256(open(path, "rb").read()).hexdigest(),
-                                        "sources": prov}
+        for name, fn in (("rrtmg_lw_cld.tbl", extract_lw_cloud), ("rrtmg_sw_cld.tbl", extract_sw_cloud)):
+            cld, prov = fn(args.ref)
+            path = os.path.join(args.out, name)
+            write_table(path, cld, 0)
+            manifest[name] = {"sha256": hashlib.sha256(open(path, "rb").read()).hexdigest(), "sources": prov}
+            print("wrote", len(cld), "entries to", name)
         with open(mpath, "w") as f:
             json.dump(manifest, f, indent=1, sort_keys=True)
-        print("wrote", len(cld), "LW cloud entries")
         return 0
 
     sw, prov_sw = extract_sw(args.ref)
@@ -536,6 +573,8 @@ def main():
 
     cld, prov_cld = extract_lw_cloud(args.ref)
     write_table(os.path.join(args.out, "rrtmg_lw_cld.tbl"), cld, 0)
+    scld, prov_scld = extract_sw_cloud(args.ref)
+    write_table(os.path.join(args.out, "rrtmg_sw_cld.tbl"), scld, 0)
 
     lw, prov_lw = extract_lw_data(args.ref)
     shapes, prov_sh = lw_shapes(args.ref)
@@ -555,6 +594,8 @@ def main():
                                             "shape-exact synthetic values, not physics."},
         "rrtmg_lw_cld.tbl": {"sha256": hashlib.sha256(open(os.path.join(args.out, "rrtmg_lw_cld.tbl"),
                                                            "rb").read()).hexdigest(), "sources": prov_cld},
+        "rrtmg_sw_cld.tbl": {"sha256": hashlib.sha256(open(os.path.join(args.out, "rrtmg_sw_cld.tbl"),
+                                                           "rb").read()).hexdigest(), "sources": prov_scld},
     }
     with open(os.path.join(args.out, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)

@@ -185,6 +185,23 @@ int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *
                           const double *ciwpmcl, const double *clwpmcl, double *swuflx, double *swdflx,
                           double *swhr, double *swuflxc, double *swdflxc, double *swhrc);
 
+/* rrtmg_sw with clouds given by water paths (inflgsw = 2, cldprop_sw's
+ * ice parameterisations 1-3 and liquid 1 on swcldpr's coefficients,
+ * rrtmg_sw_cldprop.f90:143-338, loaded by oracle_sw_cld_init from
+ * rrtmg_sw_cld.tbl; inflgsw = 1 gives the clouds no optics, as the
+ * reference).  reice, reliq (ncol, nlay); taucld (14, ncol, nlay) only in
+ * the cloud test (NULL = 0).  -8 where the reference stops. */
+int oracle_sw_cld_init(const char *path);
+int oracle_rrtmg_sw_cloud_wp(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play,
+                             const double *plev, const double *tlay, const double *tlev, const double *h2ovmr,
+                             const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *asdir, const double *asdif,
+                             const double *aldir, const double *aldif, const double *coszen, double adjes,
+                             int dyofyr, double scon, const double *cldfr, const double *taucld,
+                             const double *cicewp, const double *cliqwp, const double *reice, const double *reliq,
+                             double *swuflx, double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                             double *swhrc);
+
 /* oracle_rrtmg_lw_mcica with cldprmc's water-path optics (inflglw = 2,
  * rrtmg_lw_cldprmc.f90:150-230): per subcolumn that passes the cloud test,
  * its depth from ciwpmcl, clwpmcl (140, ncol, nlay) and the layer's radii

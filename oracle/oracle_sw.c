@@ -1057,6 +1057,158 @@ static void spcvrt_sw(swcol_t *c, double prmu0, const double *palbd, const doubl
  * >= 1e-20) gets delta-M scaled optics from the input optical depth, single
  * scattering albedo, asymmetry and forward-scattering fraction; every other
  * layer has depth 0, albedo 1, asymmetry 0.  Arrays [lay][ib], ib = 1..14. */
+/* swcldpr's cloud optics coefficients (rrtmg_sw_init.f90, module rrsw_cld)
+ * from rrtmg_sw_cld.tbl; Fortran (k, ib) arrays, bands 16-29, as [ib][k] */
+static struct {
+    int ready;
+    double extliq1[14][58], ssaliq1[14][58], asyliq1[14][58];
+    double extice2[14][43], ssaice2[14][43], asyice2[14][43];
+    double extice3[14][46], ssaice3[14][46], asyice3[14][46], fdlice3[14][46];
+    double abari[5], bbari[5], cbari[5], dbari[5], ebari[5], fbari[5], wavenum2[14];
+} SC;
+
+int oracle_sw_cld_init(const char *path)
+{
+    otbl_t t;
+    if (otbl_load(&t, path) != 0) return -1;
+    const char *nm[] = {"swcld/extliq1", "swcld/ssaliq1", "swcld/asyliq1", "swcld/extice2", "swcld/ssaice2",
+                        "swcld/asyice2", "swcld/extice3", "swcld/ssaice3", "swcld/asyice3", "swcld/fdlice3",
+                        "swcld/abari",   "swcld/bbari",   "swcld/cbari",   "swcld/dbari",   "swcld/ebari",
+                        "swcld/fbari",   "swcld/wavenum2"};
+    double *dst[] = {&SC.extliq1[0][0], &SC.ssaliq1[0][0], &SC.asyliq1[0][0], &SC.extice2[0][0],
+                     &SC.ssaice2[0][0], &SC.asyice2[0][0], &SC.extice3[0][0], &SC.ssaice3[0][0],
+                     &SC.asyice3[0][0], &SC.fdlice3[0][0], SC.abari, SC.bbari, SC.cbari, SC.dbari, SC.ebari,
+                     SC.fbari, SC.wavenum2};
+    const long len[] = {812, 812, 812, 602, 602, 602, 644, 644, 644, 644, 5, 5, 5, 5, 5, 5, 14};
+    for (int i = 0; i < 17; ++i) {
+        long n = 0;
+        const double *v = otbl_d(&t, nm[i], &n);
+        if (!v || n != len[i]) return -2;
+        memcpy(dst[i], v, sizeof(double) * len[i]);
+    }
+    SC.ready = 1;
+    return 0;
+}
+
+/* cldprop_sw with inflag 2 (rrtmg_sw_cldprop.f90:143-338): per layer that
+ * passes the cloud test, ice extinction, albedo and asymmetry by iceflag 1
+ * (the abari..fbari fits in the radius, band group by the band's upper
+ * wavenumber), 2 or 3 (tables interpolated in the radius; 3 with the delta
+ * fraction fdlice3), liquid by liqflag 1 (tables in the radius); each
+ * phase delta-scaled by its forward fraction, then combined, with the
+ * reference's floors (cldmin for a zero total depth or ice scattering).
+ * taucloud, ssacloud, asmcloud as cldprop_sw0's (0, 1, 0 where the test
+ * fails).  -8 where the reference stops (radius out of range, optics out of
+ * [0, 1], a negative extinction); iceflag 1-3 and liqflag 1 only. */
+static int cldprop_sw2(int nl, int iceflag, int liqflag, const double *cldfrac, double (*tauc)[15],
+                       const double *ciwp, const double *clwp, const double *rei, const double *rel,
+                       double (*taucloud)[15], double (*ssacloud)[15], double (*asmcloud)[15])
+{
+    const double eps = 1.e-06, cldmin = 1.e-20;
+    double extcoice[15], ssacoice[15], gice[15], forwice[15], extcoliq[15], ssacoliq[15], gliq[15], forwliq[15];
+    for (int lay = 1; lay <= nl; ++lay) {
+        double tauctot = 0.0;
+        for (int ib = 1; ib <= 14; ++ib) {
+            taucloud[lay][ib] = 0.0;
+            ssacloud[lay][ib] = 1.0;
+            asmcloud[lay][ib] = 0.0;
+            tauctot = tauctot + tauc[lay][ib];
+        }
+        double cwp = ciwp[lay] + clwp[lay];
+        if (!(cldfrac[lay] >= cldmin && (cwp >= cldmin || tauctot >= cldmin))) continue;
+        double radice = rei[lay];
+        if (ciwp[lay] == 0.0) {
+            for (int ib = 1; ib <= 14; ++ib) extcoice[ib] = ssacoice[ib] = gice[ib] = forwice[ib] = 0.0;
+        } else if (iceflag == 1) {
+            if (radice < 13.0 || radice > 130.0) return -8;
+            for (int ib = 1; ib <= 14; ++ib) {
+                double w2 = SC.wavenum2[ib - 1];
+                int icx = w2 > 1.43e04 ? 1 : w2 > 7.7e03 ? 2 : w2 > 5.3e03 ? 3 : w2 > 4.0e03 ? 4 : 5;
+                extcoice[ib] = SC.abari[icx - 1] + SC.bbari[icx - 1] / radice;
+                ssacoice[ib] = 1.0 - SC.cbari[icx - 1] - SC.dbari[icx - 1] * radice;
+                gice[ib] = SC.ebari[icx - 1] + SC.fbari[icx - 1] * radice;
+                if (gice[ib] >= 1.0) gice[ib] = 1.0 - eps;
+                forwice[ib] = gice[ib] * gice[ib];
+                if (extcoice[ib] < 0.0 || ssacoice[ib] > 1.0 || ssacoice[ib] < 0.0 || gice[ib] > 1.0 || gice[ib] < 0.0)
+                    return -8;
+            }
+        } else if (iceflag == 2 || iceflag == 3) {
+            const int nr = iceflag == 2 ? 43 : 46;
+            if (radice < 5.0 || radice > (iceflag == 2 ? 131.0 : 140.0)) return -8;
+            double factor = (radice - 2.0) / 3.0;
+            int index = (int)factor;
+            if (index == nr) index = nr - 1;
+            double fint = factor - (double)index;
+            for (int ib = 1; ib <= 14; ++ib) {
+                const double *ex = iceflag == 2 ? SC.extice2[ib - 1] : SC.extice3[ib - 1];
+                const double *ss = iceflag == 2 ? SC.ssaice2[ib - 1] : SC.ssaice3[ib - 1];
+                const double *as = iceflag == 2 ? SC.asyice2[ib - 1] : SC.asyice3[ib - 1];
+                extcoice[ib] = ex[index - 1] + fint * (ex[index] - ex[index - 1]);
+                ssacoice[ib] = ss[index - 1] + fint * (ss[index] - ss[index - 1]);
+                gice[ib] = as[index - 1] + fint * (as[index] - as[index - 1]);
+                if (iceflag == 2) {
+                    forwice[ib] = gice[ib] * gice[ib];
+                } else {
+                    const double *fd = SC.fdlice3[ib - 1];
+                    double fdelta = fd[index - 1] + fint * (fd[index] - fd[index - 1]);
+                    if (fdelta < 0.0 || fdelta > 1.0) return -8;
+                    forwice[ib] = fdelta + 0.5 / ssacoice[ib];
+                    if (forwice[ib] > gice[ib]) forwice[ib] = gice[ib];
+                }
+                if (extcoice[ib] < 0.0 || ssacoice[ib] > 1.0 || ssacoice[ib] < 0.0 || gice[ib] > 1.0 || gice[ib] < 0.0)
+                    return -8;
+            }
+        } else {
+            return -8;
+        }
+        if (clwp[lay] == 0.0) {
+            for (int ib = 1; ib <= 14; ++ib) extcoliq[ib] = ssacoliq[ib] = gliq[ib] = forwliq[ib] = 0.0;
+        } else if (liqflag == 1) {
+            double radliq = rel[lay];
+            if (radliq < 2.5 || radliq > 60.0) return -8;
+            int index = (int)(radliq - 1.5);
+            if (index == 0) index = 1;
+            if (index == 58) index = 57;
+            double fint = radliq - 1.5 - (double)index;
+            for (int ib = 1; ib <= 14; ++ib) {
+                const double *ex = SC.extliq1[ib - 1], *ss = SC.ssaliq1[ib - 1], *as = SC.asyliq1[ib - 1];
+                extcoliq[ib] = ex[index - 1] + fint * (ex[index] - ex[index - 1]);
+                ssacoliq[ib] = ss[index - 1] + fint * (ss[index] - ss[index - 1]);
+                if (fint < 0.0 && ssacoliq[ib] > 1.0) ssacoliq[ib] = ss[index - 1];
+                gliq[ib] = as[index - 1] + fint * (as[index] - as[index - 1]);
+                forwliq[ib] = gliq[ib] * gliq[ib];
+                if (extcoliq[ib] < 0.0 || ssacoliq[ib] > 1.0 || ssacoliq[ib] < 0.0 || gliq[ib] > 1.0 || gliq[ib] < 0.0)
+                    return -8;
+            }
+        } else {
+            return -8;
+        }
+        for (int ib = 1; ib <= 14; ++ib) {
+            double tauliqorig = clwp[lay] * extcoliq[ib];
+            double tauiceorig = ciwp[lay] * extcoice[ib];
+            double ssaliq = ssacoliq[ib] * (1.0 - forwliq[ib]) / (1.0 - forwliq[ib] * ssacoliq[ib]);
+            double tauliq = (1.0 - forwliq[ib] * ssacoliq[ib]) * tauliqorig;
+            double ssaice = ssacoice[ib] * (1.0 - forwice[ib]) / (1.0 - forwice[ib] * ssacoice[ib]);
+            double tauice = (1.0 - forwice[ib] * ssacoice[ib]) * tauiceorig;
+            double scatliq = ssaliq * tauliq;
+            double scatice = ssaice * tauice;
+            taucloud[lay][ib] = tauliq + tauice;
+            if (taucloud[lay][ib] == 0.0) taucloud[lay][ib] = cldmin;
+            if (scatice == 0.0) scatice = cldmin;
+            ssacloud[lay][ib] = (scatliq + scatice) / taucloud[lay][ib];
+            if (iceflag == 3)
+                asmcloud[lay][ib] = (1.0 / (scatliq + scatice)) *
+                                    (scatliq * (gliq[ib] - forwliq[ib]) / (1.0 - forwliq[ib]) +
+                                     scatice * ((gice[ib] - forwice[ib]) / (1.0 - forwice[ib])));
+            else
+                asmcloud[lay][ib] = (scatliq * (gliq[ib] - forwliq[ib]) / (1.0 - forwliq[ib]) +
+                                     scatice * (gice[ib] - forwice[ib]) / (1.0 - forwice[ib])) /
+                                    (scatliq + scatice);
+        }
+    }
+    return 0;
+}
+
 static void cldprop_sw0(int nl, const double *cldfrac, double (*tauc)[15], double (*ssac)[15],
                         double (*asmc)[15], double (*fsfc)[15], const double *cwp, double (*taucloud)[15],
                         double (*ssacloud)[15], double (*asmcloud)[15])
@@ -1303,14 +1455,14 @@ int oracle_rrtmg_sw(int ncol, int nlay, const double *play, const double *plev, 
  * taucld, ssacld, asmcld, fsfcld (14, ncol, nlay), band fastest.  Returns -5
  * when a day column has a fraction in (1e-6, 1 - 1e-6) (the reference
  * stops: partial cloud is not allowed, :535-539). */
-int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+static int sw_cloud_driver(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play, const double *plev,
                           const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
                           const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
                           const double *o2vmr, const double *asdir, const double *asdif,
                           const double *aldir, const double *aldif, const double *coszen, double adjes,
                           int dyofyr, double scon, const double *cldfr, const double *taucld,
                           const double *ssacld, const double *asmcld, const double *fsfcld,
-                          const double *cicewp, const double *cliqwp, double *swuflx, double *swdflx,
+                          const double *cicewp, const double *cliqwp, const double *reice, const double *reliq, double *swuflx, double *swdflx,
                           double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
 {
     if (!S.ready) return -1;
@@ -1328,7 +1480,7 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
         swcol_t *c = (swcol_t *)malloc(sizeof(swcol_t));
         double (*a)[MAXLAY + 2][15] = malloc(sizeof(double[7][MAXLAY + 2][15]));
         double (*gq)[MAXLAY + 2][NGSW + 1] = malloc(sizeof(double[4][MAXLAY + 2][NGSW + 1]));
-        double cf[MAXLAY + 3], cwp[MAXLAY + 3];
+        double cf[MAXLAY + 3], cwp[MAXLAY + 3], ciw[MAXLAY + 3], clw[MAXLAY + 3], rei[MAXLAY + 3], rel[MAXLAY + 3];
         double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], zbbfu[MAXLAY + 3], zbbfd[MAXLAY + 3];
         double swnflxc[MAXLAY + 3], swnflx[MAXLAY + 3];
         if (!c || !a || !gq) {
@@ -1352,12 +1504,16 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
                 long ix = iplon + (long)ncol * (l - 1);
                 cf[l] = clouds ? cldfr[ix] : 0.0;
                 cwp[l] = clouds ? (cicewp ? cicewp[ix] : 0.0) + (cliqwp ? cliqwp[ix] : 0.0) : 0.0;
+                ciw[l] = clouds && cicewp ? cicewp[ix] : 0.0;
+                clw[l] = clouds && cliqwp ? cliqwp[ix] : 0.0;
+                rei[l] = clouds && reice ? reice[ix] : 0.0;
+                rel[l] = clouds && reliq ? reliq[ix] : 0.0;
                 for (int ib = 1; ib <= 14; ++ib) {
                     long jx = (ib - 1) + 14 * ix;
-                    tauc[l][ib] = clouds ? taucld[jx] : 0.0;
-                    ssac[l][ib] = clouds ? ssacld[jx] : 0.0;
-                    asmc[l][ib] = clouds ? asmcld[jx] : 0.0;
-                    fsfc[l][ib] = clouds ? fsfcld[jx] : 0.0;
+                    tauc[l][ib] = clouds && taucld ? taucld[jx] : 0.0;
+                    ssac[l][ib] = clouds && ssacld ? ssacld[jx] : 0.0;
+                    asmc[l][ib] = clouds && asmcld ? asmcld[jx] : 0.0;
+                    fsfc[l][ib] = clouds && fsfcld ? fsfcld[jx] : 0.0;
                 }
                 if (cf[l] > zepsec && cf[l] < 1.0 - zepsec) partial = 1;
             }
@@ -1366,7 +1522,20 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
                 err = -5;
                 continue;
             }
-            cldprop_sw0(nlay, cf, tauc, ssac, asmc, fsfc, cwp, ptauc, pomgc, pasyc);
+            if (inflag == 2 && clouds) {
+                if (cldprop_sw2(nlay, iceflag, liqflag, cf, tauc, ciw, clw, rei, rel, ptauc, pomgc, pasyc) != 0) {
+#pragma omp atomic write
+                    err = -8;
+                    continue;
+                }
+            } else if (inflag == 1 && clouds) {
+                /* cldprop_sw has no inflag 1 branch: every layer keeps depth 0,
+                 * albedo 1, asymmetry 0 (rrtmg_sw_cldprop.f90:126-133) */
+                for (int l = 1; l <= nlay; ++l)
+                    for (int ib = 1; ib <= 14; ++ib) ptauc[l][ib] = 0.0, pomgc[l][ib] = 1.0, pasyc[l][ib] = 0.0;
+            } else {
+                cldprop_sw0(nlay, cf, tauc, ssac, asmc, fsfc, cwp, ptauc, pomgc, pasyc);
+            }
             sw_column(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr);
             double cossza = coszen[iplon];
             double albdir[15], albdif[15];
@@ -1412,6 +1581,38 @@ int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, cons
         free(gq);
     }
     return err;
+}
+
+int oracle_rrtmg_sw_cloud(int ncol, int nlay, int icld, const double *play, const double *plev,
+                          const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
+                          const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                          const double *o2vmr, const double *asdir, const double *asdif,
+                          const double *aldir, const double *aldif, const double *coszen, double adjes,
+                          int dyofyr, double scon, const double *cldfr, const double *taucld,
+                          const double *ssacld, const double *asmcld, const double *fsfcld,
+                          const double *cicewp, const double *cliqwp, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
+{
+    return sw_cloud_driver(ncol, nlay, icld, 0, 0, 0, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
+                           o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, cldfr, taucld, ssacld,
+                           asmcld, fsfcld, cicewp, cliqwp, NULL, NULL, swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc);
+}
+
+int oracle_rrtmg_sw_cloud_wp(int ncol, int nlay, int icld, int inflag, int iceflag, int liqflag, const double *play,
+                             const double *plev, const double *tlay, const double *tlev, const double *h2ovmr,
+                             const double *o3vmr, const double *co2vmr, const double *ch4vmr,
+                             const double *n2ovmr, const double *o2vmr, const double *asdir, const double *asdif,
+                             const double *aldir, const double *aldif, const double *coszen, double adjes,
+                             int dyofyr, double scon, const double *cldfr, const double *taucld,
+                             const double *cicewp, const double *cliqwp, const double *reice, const double *reliq,
+                             double *swuflx, double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
+                             double *swhrc)
+{
+    if (inflag == 2 && !SC.ready) return -1;
+    return sw_cloud_driver(ncol, nlay, icld, inflag, iceflag, liqflag, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr,
+                           ch4vmr, n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, cldfr,
+                           taucld, NULL, NULL, NULL, cicewp, cliqwp, reice, reliq, swuflx, swdflx, swhr, swuflxc,
+                           swdflxc, swhrc);
 }
 
 /* mcica_subcol_sw with the KISS generator (irng = 0;

@@ -178,6 +178,35 @@ def rrtmg_lw_mcica_wp(c, iceflag, liqflag, sub, reicmcl, relqmcl, tauaer=None):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
+SW_CLD_TABLE = os.path.join(os.path.dirname(LW_TABLE), "rrtmg_sw_cld.tbl")
+_swcld_ready = False
+
+
+def rrtmg_sw_cloud_wp(c, icld, inflag, iceflag, liqflag, cldfr, cicewp, cliqwp, reice, reliq, taucld=None):
+    """rrtmg_sw with clouds given by water paths (inflgsw 1 or 2): cldfr,
+    cicewp, cliqwp, reice, reliq (nlay, ncol); taucld (nlay, ncol, 14) only
+    for the cloud test (zero if None)."""
+    global _swcld_ready
+    L = lib()
+    if not _swcld_ready:
+        if L.oracle_sw_cld_init(SW_CLD_TABLE.encode()) != 0:
+            raise RuntimeError("oracle_sw_cld_init failed")
+        _swcld_ready = True
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr",
+                                "n2ovmr", "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")]
+    tc = np.zeros((nlay, ncol, 14)) if taucld is None else taucld
+    cl = [_f64(a) for a in (cldfr, tc, cicewp, cliqwp, reice, reliq)]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_sw_cloud_wp(ncol, nlay, int(icld), int(inflag), int(iceflag), int(liqflag),
+                                    *[_p(a) for a in ins], ctypes.c_double(c["adjes"]), int(c["dyofyr"]),
+                                    ctypes.c_double(c["scon"]), *[_p(a) for a in cl], *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_sw_cloud_wp returned %d" % rc)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
 def lib_ngs():
     """Cumulative LW g-points per band (the reduced table's ngs)."""
     from mima_amd.tables.extract_tables import read_table

@@ -10,6 +10,8 @@
 !                                     cloud/aerosol arguments MiMA passes
 !                                     (RR/rrtm_radiation.f90:701-713)
 !   ref_sw_run_cloud(...)          -> rrtmg_sw with clouds (icld >= 1, inflgsw = 0)
+!   ref_sw_run_cloud_wp(...)       -> the same with water-path cloud optics
+!                                     (inflgsw = 2, swcldpr's coefficients)
 !   ref_sw_column(...)             -> inatm_sw + setcoef_sw + taumol_sw for one
 !                                     column: jp/jt/jt1, laytrop, taug, taur,
 !                                     sfluxzen
@@ -100,6 +102,44 @@ contains
                   aerb, aerb, aerb, ecaer, &
                   swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
   end subroutine ref_sw_run_cloud
+
+  ! rrtmg_sw with clouds given by water paths (inflgsw, iceflgsw, liqflgsw;
+  ! swcldpr's coefficients as rrtmg_sw_ini set them) and effective radii
+  ! reice, reliq (ncol, nlay); taucld only enters the cloud test
+  subroutine ref_sw_run_cloud_wp(ncol, nlay, icld_in, inflag_in, iceflag_in, liqflag_in, play, plev, tlay, &
+                                 tlev, tsfc, h2o, o3, co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, coszen, &
+                                 adjes, dyofyr, scon, cldfr, taucld, cicewp, cliqwp, reice, reliq, &
+                                 swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc) bind(C, name="ref_sw_run_cloud_wp")
+    integer(c_int), value :: ncol, nlay, dyofyr, icld_in, inflag_in, iceflag_in, liqflag_in
+    real(c_double), value :: adjes, scon
+    real(c_double), intent(in) :: play(ncol, nlay), plev(ncol, nlay + 1), tlay(ncol, nlay)
+    real(c_double), intent(in) :: tlev(ncol, nlay + 1), tsfc(ncol)
+    real(c_double), intent(in) :: h2o(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay)
+    real(c_double), intent(in) :: ch4(ncol, nlay), n2o(ncol, nlay), o2(ncol, nlay)
+    real(c_double), intent(in) :: asdir(ncol), asdif(ncol), aldir(ncol), aldif(ncol), coszen(ncol)
+    real(c_double), intent(in) :: cldfr(ncol, nlay), cicewp(ncol, nlay), cliqwp(ncol, nlay)
+    real(c_double), intent(in) :: reice(ncol, nlay), reliq(ncol, nlay), taucld(nbndsw, ncol, nlay)
+    real(c_double), intent(out) :: swuflx(ncol, nlay + 1), swdflx(ncol, nlay + 1), swhr(ncol, nlay)
+    real(c_double), intent(out) :: swuflxc(ncol, nlay + 1), swdflxc(ncol, nlay + 1), swhrc(ncol, nlay)
+    integer(kind=im) :: icld, iaer, dy, inflag, iceflag, liqflag
+    real(kind=rb), allocatable :: cldb(:, :, :), aerb(:, :, :), ecaer(:, :, :)
+    allocate(cldb(nbndsw, ncol, nlay), aerb(ncol, nlay, nbndsw), ecaer(ncol, nlay, naerec))
+    cldb = 0.0_rb
+    aerb = 0.0_rb
+    ecaer = 0.0_rb
+    icld = icld_in
+    iaer = 0
+    dy = dyofyr
+    inflag = inflag_in
+    iceflag = iceflag_in
+    liqflag = liqflag_in
+    call rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                  h2o, o3, co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, &
+                  coszen, adjes, dy, scon, inflag, iceflag, liqflag, cldfr, &
+                  taucld, cldb, cldb, cldb, cicewp, cliqwp, reice, reliq, &
+                  aerb, aerb, aerb, ecaer, &
+                  swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  end subroutine ref_sw_run_cloud_wp
 
   subroutine ref_sw_column(nlay, play, plev, tlay, tlev, h2o, o3, co2, ch4, n2o, o2, &
                            laytrop, jp, jt, jt1, fac00, fac01, fac10, fac11, colmol, &

@@ -258,3 +258,18 @@ def run_lw_mcica_wp(lib, c, icld, iceflag, liqflag, sub, reicmcl, relqmcl, tauae
     lib.ref_lw_mcica_run_wp(n, L, int(icld), 2, int(iceflag), int(liqflag), *[_p(a) for a in ins],
                             *[_p(a) for a in sb], _p(tau), *[_p(a) for a in outs])
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
+
+
+def run_sw_cloud_wp(lib, c, icld, inflag, iceflag, liqflag, cldfr, cicewp, cliqwp, reice, reliq, taucld=None):
+    """rrtmg_sw of the reference with water-path cloud optics (inflgsw 1, 2);
+    radii (nlay, ncol); taucld (nlay, ncol, 14) only for the cloud test."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
+    tc = np.zeros((L, n, 14)) if taucld is None else taucld
+    cl = [np.ascontiguousarray(a, dtype=np.float64) for a in (cldfr, tc, cicewp, cliqwp, reice, reliq)]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_sw_run_cloud_wp(n, L, int(icld), int(inflag), int(iceflag), int(liqflag), *[_p(a) for a in ins],
+                            ctypes.c_double(c["adjes"]), int(c["dyofyr"]), ctypes.c_double(c["scon"]),
+                            *[_p(a) for a in cl], *[_p(a) for a in outs])
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))

@@ -89,3 +89,28 @@ def test_mcica_water_path_optics_match_reference(k):
     o = po.rrtmg_lw_mcica_wp(c, iceflag, liqflag, sub, MC["reice"], MC["reliq"])
     for kk in OUT:
         np.testing.assert_allclose(o[kk], MC["c%d_%s" % (k, kk)], rtol=1e-12, atol=1e-11, err_msg=kk)
+
+
+SWG = dict(np.load(os.path.join(ROOT, "tests", "golden", "sw_ref_cloud_wp.npz"), allow_pickle=False))
+SW_KEYS = ("play", "plev", "tlay", "tlev", "tsfc", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr", "n2ovmr", "o2vmr",
+           "asdir", "asdif", "aldir", "aldif", "coszen")
+SW_OUT = ("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc")
+
+
+@pytest.mark.parametrize("k", range(5))
+def test_sw_water_path_optics_match_reference(k):
+    """cldprop_sw with inflgsw = 2 (rrtmg_sw_cldprop.f90:143-338: ice
+    parameterisations 1-3, liquid 1, each phase delta-scaled, the
+    reference's floors) and inflgsw = 1 (no cloud optics): the reference's
+    rrtmg_sw with swcldpr's own coefficients (the SW init is built whole),
+    day and night columns, icld 1-3."""
+    c = {kk: SWG[kk] for kk in SW_KEYS}
+    c["ncol"], c["nlay"] = int(SWG["ncol"]), int(SWG["nlay"])
+    c["adjes"], c["dyofyr"], c["scon"] = float(SWG["adjes"]), int(SWG["dyofyr"]), float(SWG["scon"])
+    inflag, iceflag, liqflag, icld = (int(x) for x in SWG["cases"][k])
+    o = po.rrtmg_sw_cloud_wp(c, icld, inflag, iceflag, liqflag, SWG["cldfr"], SWG["cicewp"], SWG["cliqwp"],
+                             SWG["reice"], SWG["reliq"])
+    for kk in SW_OUT:
+        np.testing.assert_allclose(o[kk], SWG["c%d_%s" % (k, kk)], rtol=1e-12, atol=1e-11, err_msg=kk)
+    if inflag == 2:
+        assert np.max(np.abs(o["swdflx"] - o["swdflxc"])) > 100.0
