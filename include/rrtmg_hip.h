@@ -108,20 +108,27 @@ int rrtmg_hip_lw(int ncol, int nlay, int icld, int idrv,
 /* Replaces rrtmg_sw with clouds (rrtmg_sw/gcm_model/src/rrtmg_sw_rad.nomcica.f90:78-88):
  * the reference's whole argument list in its order.  icld >= 1 (outside
  * 0..3 taken as 2, :471; the path without McICA treats every icld >= 1
- * alike); icld = 0 is rrtmg_hip_sw (cloud arrays unread).  inflgsw = 0
- * only (cldprop_sw, rrtmg_sw_cldprop.f90:118-157): taucld, ssacld, asmcld,
- * fsfcld (14, ncol, nlay), band fastest, are the cloud optical depth,
- * single-scattering albedo, asymmetry and forward-scattering fraction,
- * delta-M scaled by the library; a layer's are used where cldfr
- * (ncol, nlay) >= 1e-20 and cicewp + cliqwp (NULL = 0) or the summed depth
- * >= 1e-20.  Day columns (coszen >= 1e-10) take overcast or clear layers
+ * alike); icld = 0 is rrtmg_hip_sw (cloud arrays unread).  cldprop_sw
+ * (rrtmg_sw_cldprop.f90:118-338): a layer has cloud optics where cldfr
+ * (ncol, nlay) >= 1e-20 and cicewp + cliqwp (NULL = 0) or its summed taucld
+ * >= 1e-20.  inflgsw = 0: taucld, ssacld, asmcld, fsfcld (14, ncol, nlay),
+ * band fastest, are the cloud optical depth, single-scattering albedo,
+ * asymmetry and forward-scattering fraction, delta-M scaled by the library.
+ * inflgsw = 2: ice optics by iceflgsw 1-3 and liquid by liqflgsw 1 from
+ * swcldpr's coefficients (read from rrtmg_sw_cld.tbl beside rrtmg_sw.tbl)
+ * with the radii reice, reliq (ncol, nlay, microns); ssacld, asmcld, fsfcld
+ * are not read and taucld may be NULL; optics the reference stops on
+ * (a radius out of range, optics out of [0, 1]) return RRTMG_HIP_EINVAL
+ * (host pointers always, device pointers with rrtmg_hip_set_checks on).
+ * inflgsw = 1: clouds without optics, as the reference.  Day columns (coszen >= 1e-10) take overcast or clear layers
  * only, as the reference (it stops on a fraction in (1e-6, 1 - 1e-6),
  * :535-539): EINVAL, checked for host pointers always and for device
  * pointers when rrtmg_hip_set_checks is on (otherwise the layer's clear and
  * cloudy values are blended by the fraction, the reference's formula).
- * iceflgsw, liqflgsw, reice, reliq and the aerosol arrays are not read.
- * Outputs: total-sky swuflx, swdflx, swhr and clear-sky swuflxc, swdflxc,
- * swhrc.  inflgsw != 0, iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
+ * The aerosol arrays are not read.  Outputs: total-sky swuflx, swdflx,
+ * swhr and clear-sky swuflxc, swdflxc, swhrc.  inflgsw outside 0..2, or
+ * iceflgsw / liqflgsw outside 1..3 / 1 with inflgsw = 2, return
+ * RRTMG_HIP_EINVAL; iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
  * RRTMG_HIP_ENOTSUP. */
 int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer,
                        const double *play, const double *plev, const double *tlay,

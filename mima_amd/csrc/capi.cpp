@@ -68,7 +68,7 @@ struct SideTables {
     LwBand *bands_lds = nullptr;                 // LW: offsets relative to each band's first k-table element
     LwLdsPlan plan{};                            // LW: schedule of the LDS-table kernel
     bool lds = false;                            // LW: that kernel fits this device
-    double *cld = nullptr;                       // LW: lwcldpr's coefficients (LW_CLD_TAB), when the file is there
+    double *cld = nullptr;                       // lwcldpr's / swcldpr's coefficients (LW_CLD_TAB / SW_CLD_TAB), when the file is there
     double cpdair = 0.0;
 };
 
@@ -354,6 +354,26 @@ int rrtmg_hip_sw_init(double cpdair, const char *table_dir)
     if (!rc) rc = upload(&s->swd.units, s->sw.units.data(), s->sw.units.size() * sizeof(WaveUnit));
     if (!rc) rc = upload(&s->swd.preflog, s->sw.preflog.data(), 59 * sizeof(double));
     if (!rc) rc = upload(&s->swd.tref, s->sw.tref.data(), 59 * sizeof(double));
+    {
+        // optional: the water-path cloud optics' coefficients (inflgsw 2)
+        TableFile ct;
+        std::string cerr;
+        if (!rc && ct.load(dir + "/rrtmg_sw_cld.tbl", cerr)) {
+            const char *nm[] = {"swcld/extliq1", "swcld/ssaliq1", "swcld/asyliq1", "swcld/extice2", "swcld/ssaice2",
+                                "swcld/asyice2", "swcld/extice3", "swcld/ssaice3", "swcld/asyice3", "swcld/fdlice3",
+                                "swcld/abari",   "swcld/bbari",   "swcld/cbari",   "swcld/dbari",   "swcld/ebari",
+                                "swcld/fbari",   "swcld/wavenum2"};
+            const size_t len[] = {812, 812, 812, 602, 602, 602, 644, 644, 644, 644, 5, 5, 5, 5, 5, 5, 14};
+            std::vector<double> pk;
+            bool ok = true;
+            for (int i = 0; i < 17; ++i) {
+                std::vector<double> v = ct.find(nm[i]) ? ct.doubles(nm[i]) : std::vector<double>();
+                ok = ok && v.size() == len[i];
+                pk.insert(pk.end(), v.begin(), v.end());
+            }
+            if (ok && pk.size() == (size_t)SW_CLD_TAB) rc = upload(&s->swd.cld, pk.data(), pk.size() * sizeof(double));
+        }
+    }
     if (rc) return fail(rc, "uploading SW tables failed");
     s->swd.cpdair = cpdair;
     s->swd.ready = true;
@@ -505,6 +525,9 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     if (cl && fx) return fail(RRTMG_HIP_EINVAL, "the no-allocation entry point is clear sky");
     if (cl && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
         return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
+    if (cl && !cl->mcica && cl->inflag == 2 && !s->swd.cld)
+        return fail(RRTMG_HIP_ENOTSUP, "inflgsw = 2 needs the cloud optics table rrtmg_sw_cld.tbl beside "
+                                       "rrtmg_sw.tbl at rrtmg_hip_sw_init");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
     if (rc) return rc;
     const size_t n2 = (size_t)ncol * nlay, n3 = (size_t)ncol * (nlay + 1);
@@ -580,7 +603,10 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
             a.push_back({"cldfr", cl->cldfr, n2});
             const double *opt[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
             const char *nmo[] = {"taucld", "ssacld", "asmcld", "fsfcld"};
-            for (int i = 0; i < 4; ++i) a.push_back({nmo[i], opt[i], 14 * n2});
+            for (int i = 0; i < 4; ++i)
+                if (opt[i]) a.push_back({nmo[i], opt[i], 14 * n2});
+            if (cl->inflag == 2 && cl->reice) a.push_back({"reice", cl->reice, n2});
+            if (cl->inflag == 2 && cl->reliq) a.push_back({"reliq", cl->reliq, n2});
             if (cl->cicewp) a.push_back({"cicewp", cl->cicewp, n2});
             if (cl->cliqwp) a.push_back({"cliqwp", cl->cliqwp, n2});
         }
@@ -595,6 +621,9 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     const size_t o_cpc = off; if (cl) off = align_up(off + sizeof(int));
     const bool mcica = cl && cl->mcica;
     const size_t o_csz = off; if (mcica) off = align_up(off + sizeof(double) * ncol);
+    const bool swp = cl && !cl->mcica && cl->inflag != 0;
+    const size_t o_wpo = off; if (swp) off = align_up(off + sizeof(double) * 3 * 14 * n2);
+    const size_t o_wpb = off; if (swp) off = align_up(off + sizeof(int));
     if (W) {
         if ((rc = W->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
         wb = (char *)W->sw_work.p;
@@ -630,7 +659,10 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
             const double *sc[] = {cl->cldfmc, cl->taucmc, cl->ssacmc, cl->asmcmc, cl->fsfcmc, cl->ciwpmc, cl->clwpmc};
             for (const double *q : sc) ncl += q ? 112 * n2 : 0;
         } else if (cl) {
-            ncl = 57 * n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0);
+            const double *sc[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
+            ncl = n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) + (swp && cl->reice ? n2 : 0) +
+                  (swp && cl->reliq ? n2 : 0);
+            for (const double *q : sc) ncl += q ? 14 * n2 : 0;
         }
         const size_t need = sizeof(double) * (8 * n2 + 2 * n3 + 6 * (size_t)ncol + ncl + 4 * n3 + 2 * n2);
         if ((rc = W->sw_stage.ensure(need))) return fail(rc, "device staging allocation failed");
@@ -665,11 +697,12 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
                 p += 112 * n2;
             }
         } else if (cl) {
-            const double *src[] = {cl->cldfr, cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld, cl->cicewp, cl->cliqwp};
+            const double *src[] = {cl->cldfr, cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld, cl->cicewp, cl->cliqwp,
+                                   swp ? cl->reice : nullptr, swp ? cl->reliq : nullptr};
             const double **dst[] = {&cld.cldfr, &cld.taucld, &cld.ssacld, &cld.asmcld, &cld.fsfcld, &cld.cicewp,
-                                    &cld.cliqwp};
-            const size_t len[] = {n2, 14 * n2, 14 * n2, 14 * n2, 14 * n2, n2, n2};
-            for (int i = 0; i < 7; ++i) {
+                                    &cld.cliqwp, &cld.reice, &cld.reliq};
+            const size_t len[] = {n2, 14 * n2, 14 * n2, 14 * n2, 14 * n2, n2, n2, n2, n2};
+            for (int i = 0; i < 9; ++i) {
                 if (!src[i]) continue;
                 ck |= hipMemcpyAsync(p, src[i], len[i] * sizeof(double), hipMemcpyHostToDevice, st);
                 *dst[i] = p;
@@ -697,7 +730,7 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     t.units = s->swd.units;
     t.heatfac = s->sw.heatfac;
     if (ck) return fail(RRTMG_HIP_EDEVICE, "SW: host-to-device copy failed");
-    hipError_t e;
+    hipError_t e = hipSuccess;
     if (mcica) {
         // the McICA driver's inatm reads h2o and O3 as volume mixing ratios
         // and floors the zenith cosine at 1e-10 (rrtmg_sw_rad.f90:571-572,
@@ -708,7 +741,28 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         if (e == hipSuccess) e = launch_sw_cloud_flags(in, cld, cw, st);
         if (e == hipSuccess) e = launch_sw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(0));
     } else if (cl) {
-        e = launch_sw_cloud_flags(in, cld, cw, st);
+        int *wpbad = nullptr;
+        if (swp) {
+            // water paths: cldprop_sw's final optics, then the cloudy path on them
+            double *o3b = (double *)(wb + o_wpo);
+            wpbad = (int *)(wb + o_wpb);
+            e = launch_sw_cloud_wp(in, cld, s->swd.cld, o3b, o3b + 14 * n2, o3b + 28 * n2, wpbad, st);
+            cld.taucld = o3b;
+            cld.ssacld = o3b + 14 * n2;
+            cld.asmcld = o3b + 28 * n2;
+            cld.fsfcld = nullptr;
+            cld.scaled = 1;
+            if (e == hipSuccess && (ptr_kind == RRTMG_HIP_HOST || g_checks)) {
+                int nb = 0;
+                e = hipMemcpyAsync(&nb, wpbad, sizeof(int), hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e == hipSuccess && nb > 0)
+                    return fail(RRTMG_HIP_EINVAL, "cloud water-path optics outside their parameterisation's range "
+                                                  "(an effective radius, or optics out of [0, 1]) in a cloudy layer "
+                                                  "(the reference stops)");
+            }
+        }
+        if (e == hipSuccess) e = launch_sw_cloud_flags(in, cld, cw, st);
         if (e == hipSuccess && ptr_kind == RRTMG_HIP_DEVICE && g_checks) {
             // the nomcica driver stops on partial cloud in a day column
             // (rrtmg_sw_rad.nomcica.f90:535-539); the host-pointer path
@@ -1013,10 +1067,6 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *pla
                        double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind,
                        void *stream)
 {
-    (void)iceflgsw;
-    (void)liqflgsw;
-    (void)reice;
-    (void)reliq;
     (void)tauaer;
     (void)ssaaer;
     (void)asmaer;
@@ -1027,9 +1077,11 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *pla
         return sw_forward(ncol, nlay, 0, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                           o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr,
                           swuflxc, swdflxc, swhrc, ptr_kind, stream, nullptr);
-    if (inflgsw != 0)
-        return fail(RRTMG_HIP_ENOTSUP, "inflgsw != 0: cloud optics from water paths and effective radii are not "
-                                       "implemented; pass the cloud optics with inflgsw = 0");
+    if (inflgsw < 0 || inflgsw > 2) return fail(RRTMG_HIP_EINVAL, "inflgsw must be 0, 1 or 2");
+    // cldprop_sw reads ice optics for iceflgsw 1-3 and liquid for liqflgsw
+    // 1 only (rrtmg_sw_cldprop.f90:169-287)
+    if (inflgsw == 2 && (iceflgsw < 1 || iceflgsw > 3 || liqflgsw != 1))
+        return fail(RRTMG_HIP_EINVAL, "iceflgsw must be 1..3 and liqflgsw 1 with inflgsw = 2");
     if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
         const size_t n2 = (size_t)ncol * nlay;
         int rc;
@@ -1038,7 +1090,9 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *pla
             const char *nm[] = {"taucld", "ssacld", "asmcld", "fsfcld"};
             if ((rc = check_host_array(cldfr, n2, "cldfr"))) return rc;
             for (int i = 0; i < 4; ++i)
-                if ((rc = check_host_array(a[i], 14 * n2, nm[i]))) return rc;
+                if ((inflgsw == 0 || (i == 0 && a[i])) && (rc = check_host_array(a[i], 14 * n2, nm[i]))) return rc;
+            if (inflgsw == 2 && reice && (rc = check_host_array(reice, n2, "reice"))) return rc;
+            if (inflgsw == 2 && reliq && (rc = check_host_array(reliq, n2, "reliq"))) return rc;
             if (cicewp && (rc = check_host_array(cicewp, n2, "cicewp"))) return rc;
             if (cliqwp && (rc = check_host_array(cliqwp, n2, "cliqwp"))) return rc;
             if ((rc = check_host_array(coszen, ncol, "coszen"))) return rc;
@@ -1050,11 +1104,17 @@ int rrtmg_hip_sw_cloud(int ncol, int nlay, int icld, int iaer, const double *pla
                     return fail(RRTMG_HIP_EINVAL, "partial cloud (fraction in (1e-6, 1 - 1e-6)) in a day column: the "
                                                   "SW path without McICA takes overcast or clear layers only");
             }
-        } else if (!cldfr || !taucld || !ssacld || !asmcld || !fsfcld) {
+        } else if (!cldfr || (inflgsw == 0 && (!taucld || !ssacld || !asmcld || !fsfcld))) {
             return fail(RRTMG_HIP_EINVAL, "null cloud array");
         }
     }
-    SwClouds cl{cldfr, taucld, ssacld, asmcld, fsfcld, cicewp, cliqwp};
+    SwClouds cl{cldfr, taucld, inflgsw == 0 ? ssacld : nullptr,
+                inflgsw == 0 ? asmcld : nullptr, inflgsw == 0 ? fsfcld : nullptr, cicewp, cliqwp};
+    cl.inflag = inflgsw;
+    cl.iceflag = iceflgsw;
+    cl.liqflag = liqflgsw;
+    cl.reice = reice;
+    cl.reliq = reliq;
     return sw_forward(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                       o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr, swuflxc,
                       swdflxc, swhrc, ptr_kind, stream, nullptr, &cl);

@@ -204,7 +204,20 @@ struct SwClouds {
     // of the fields above (water paths NULL for zero)
     int mcica;
     const double *cldfmc, *taucmc, *ssacmc, *asmcmc, *fsfcmc, *ciwpmc, *clwpmc;
+    // scaled != 0: taucld, ssacld, asmcld hold cldprop's final (delta-scaled)
+    // cloud optics, as sw_cloud_wp_kernel forms them from water paths
+    int scaled;
+    int inflag, iceflag, liqflag;
+    const double *reice, *reliq;  // [nlay][ncol]
 };
+// swcldpr's coefficients packed as extliq1, ssaliq1, asyliq1 [14][58],
+// extice2, ssaice2, asyice2 [14][43], extice3, ssaice3, asyice3, fdlice3
+// [14][46], abari..fbari [6][5], wavenum2 [14]
+constexpr int SW_CLD_TAB = 3 * 812 + 3 * 602 + 4 * 644 + 30 + 14;
+// water-path cloud optics (inflgsw 1, 2) into tau, ssa, asm [nlay][ncol][14];
+// bad counts the layers where the reference stops (cleared first)
+hipError_t launch_sw_cloud_wp(const ColumnInputs &in, const SwClouds &cl, const double *tab, double *tau,
+                              double *ssa, double *asm_, int *bad, hipStream_t s);
 // flag bits per (layer, column): a fraction above 0 (the total stream
 // blends the layer's cloudy optics), above 1e-12 (reftra of the cloudy
 // optics, rrtmg_sw_spcvrt.f90:388), cldprop_sw's cloud test (the layer's

@@ -157,9 +157,9 @@ def test_error_contract(rr):
     args = [c[k] for k in SW_KEYS]
     rr.set_precision("fp64")
     day = int(np.argmax(c["coszen"]))
-    with pytest.raises(RRTMGError) as e:   # cloud optics from water paths: not implemented
-        rr.rrtmg_sw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 1.0, 0, 1370.0, 2, 1, 1, *cl)
-    assert e.value.code == -95
+    with pytest.raises(RRTMGError) as e:   # no such cloud-input mode
+        rr.rrtmg_sw_cloud(c["ncol"], c["nlay"], 2, 0, *args, 1.0, 0, 1370.0, 3, 1, 1, *cl)
+    assert e.value.code == -22
     with pytest.raises(RRTMGError) as e:   # aerosols
         rr.rrtmg_sw_cloud(c["ncol"], c["nlay"], 2, 10, *args, 1.0, 0, 1370.0, 0, 0, 0, *cl)
     assert e.value.code == -95
