@@ -276,10 +276,19 @@ int rrtmg_hip_mcica_subcol_sw(int iplon, int ncol, int nlay, int icld, int permu
  * inatm, :1015, 1021), and night columns are computed with the zenith
  * cosine floored at 1e-10 rather than zeroed (:571-572), as the reference
  * does.  cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl (112, ncol, nlay),
- * ciwpmcl, clwpmcl likewise or NULL (zero); reicmcl, relqmcl, iceflgsw,
- * liqflgsw and the aerosol arrays are not read.  icld outside 0..3 is
- * taken as 2 (:500); icld = 0 uses no subcolumns (:591-597).  inflgsw !=
- * 0, iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return RRTMG_HIP_ENOTSUP. */
+ * ciwpmcl, clwpmcl likewise or NULL (zero).  inflgsw = 2
+ * (rrtmg_sw_cldprmc.f90:144-330): a subcolumn passing that test takes
+ * cldprop_sw's final optics of its band ngb(g) from its own paths and the
+ * layer radii reicmcl, relqmcl (ncol, nlay; iceflgsw 1-3, liqflgsw 1; the
+ * table rrtmg_sw_cld.tbl beside rrtmg_sw.tbl at init), any other keeps its
+ * depth, albedo and asymmetry as given; fsfcmcl is not read.  A radius or
+ * optics the reference stops on in a cloudy subcolumn returns
+ * RRTMG_HIP_EINVAL (host pointers; device pointers under
+ * rrtmg_hip_set_checks).  inflgsw 1 (the reference stops) or outside 0..2
+ * returns RRTMG_HIP_EINVAL.  The aerosol arrays are not read.  icld
+ * outside 0..3 is taken as 2 (:500); icld = 0 uses no subcolumns
+ * (:591-597).  iaer != 0 and RRTMG_HIP_PREC_FP64_SMOOTH return
+ * RRTMG_HIP_ENOTSUP. */
 int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer,
                        const double *play, const double *plev, const double *tlay,
                        const double *tlev, const double *tsfc,

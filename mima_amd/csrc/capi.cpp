@@ -525,7 +525,7 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     if (cl && fx) return fail(RRTMG_HIP_EINVAL, "the no-allocation entry point is clear sky");
     if (cl && g_prec == RRTMG_HIP_PREC_FP64_SMOOTH)
         return fail(RRTMG_HIP_ENOTSUP, "the cloudy path has no smooth (FP64_SMOOTH) mode");
-    if (cl && !cl->mcica && cl->inflag == 2 && !s->swd.cld)
+    if (cl && cl->inflag == 2 && !s->swd.cld)
         return fail(RRTMG_HIP_ENOTSUP, "inflgsw = 2 needs the cloud optics table rrtmg_sw_cld.tbl beside "
                                        "rrtmg_sw.tbl at rrtmg_hip_sw_init");
     int rc = check_common(ncol, nlay, plev, ptr_kind);
@@ -599,6 +599,8 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
             const char *nmo[] = {"cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl"};
             for (int i = 0; i < 7; ++i)
                 if (opt[i]) a.push_back({nmo[i], opt[i], 112 * n2});
+            if (cl->inflag == 2 && cl->reice) a.push_back({"reicmcl", cl->reice, n2});
+            if (cl->inflag == 2 && cl->reliq) a.push_back({"relqmcl", cl->reliq, n2});
         } else if (cl) {
             a.push_back({"cldfr", cl->cldfr, n2});
             const double *opt[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
@@ -624,6 +626,9 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
     const bool swp = cl && !cl->mcica && cl->inflag != 0;
     const size_t o_wpo = off; if (swp) off = align_up(off + sizeof(double) * 3 * 14 * n2);
     const size_t o_wpb = off; if (swp) off = align_up(off + sizeof(int));
+    const bool mwp = mcica && cl->inflag == 2;
+    const size_t o_mwo = off; if (mwp) off = align_up(off + sizeof(double) * 3 * 112 * n2);
+    const size_t o_mwb = off; if (mwp) off = align_up(off + sizeof(int));
     if (W) {
         if ((rc = W->sw_work.ensure(off))) return fail(rc, "device workspace allocation failed");
         wb = (char *)W->sw_work.p;
@@ -658,6 +663,7 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         if (mcica) {
             const double *sc[] = {cl->cldfmc, cl->taucmc, cl->ssacmc, cl->asmcmc, cl->fsfcmc, cl->ciwpmc, cl->clwpmc};
             for (const double *q : sc) ncl += q ? 112 * n2 : 0;
+            ncl += (mwp && cl->reice ? n2 : 0) + (mwp && cl->reliq ? n2 : 0);
         } else if (cl) {
             const double *sc[] = {cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld};
             ncl = n2 + (cl->cicewp ? n2 : 0) + (cl->cliqwp ? n2 : 0) + (swp && cl->reice ? n2 : 0) +
@@ -695,6 +701,14 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
                 ck |= hipMemcpyAsync(p, src[i], 112 * n2 * sizeof(double), hipMemcpyHostToDevice, st);
                 *dst[i] = p;
                 p += 112 * n2;
+            }
+            const double *rsrc[] = {mwp ? cl->reice : nullptr, mwp ? cl->reliq : nullptr};
+            const double **rdst[] = {&cld.reice, &cld.reliq};
+            for (int i = 0; i < 2; ++i) {
+                if (!rsrc[i]) continue;
+                ck |= hipMemcpyAsync(p, rsrc[i], n2 * sizeof(double), hipMemcpyHostToDevice, st);
+                *rdst[i] = p;
+                p += n2;
             }
         } else if (cl) {
             const double *src[] = {cl->cldfr, cl->taucld, cl->ssacld, cl->asmcld, cl->fsfcld, cl->cicewp, cl->cliqwp,
@@ -738,6 +752,27 @@ int sw_forward(int ncol, int nlay, int icld, int iaer, const double *play, const
         in.vmr = 1;
         e = launch_sw_coszen_floor(ncol, in.coszen, (double *)(wb + o_csz), st);
         in.coszen = (const double *)(wb + o_csz);
+        if (e == hipSuccess && mwp) {
+            // water paths: cldprmc_sw's final optics per subcolumn, then the
+            // McICA transfer on them
+            double *o3b = (double *)(wb + o_mwo);
+            int *mbad = (int *)(wb + o_mwb);
+            e = launch_sw_mcica_wp(in, cld, s->swd.cld, o3b, o3b + 112 * n2, o3b + 224 * n2, mbad, st);
+            cld.taucmc = o3b;
+            cld.ssacmc = o3b + 112 * n2;
+            cld.asmcmc = o3b + 224 * n2;
+            cld.fsfcmc = nullptr;
+            cld.scaled = 1;
+            if (e == hipSuccess && (ptr_kind == RRTMG_HIP_HOST || g_checks)) {
+                int nb = 0;
+                e = hipMemcpyAsync(&nb, mbad, sizeof(int), hipMemcpyDeviceToHost, st);
+                if (e == hipSuccess) e = hipStreamSynchronize(st);
+                if (e == hipSuccess && nb > 0)
+                    return fail(RRTMG_HIP_EINVAL, "cloud water-path optics outside their parameterisation's range "
+                                                  "(an effective radius, or optics out of [0, 1]) in a cloudy "
+                                                  "subcolumn (the reference stops)");
+            }
+        }
         if (e == hipSuccess) e = launch_sw_cloud_flags(in, cld, cw, st);
         if (e == hipSuccess) e = launch_sw_cloud(in, cld, out, w, cw, t, g_prec, st, s->events(0));
     } else if (cl) {
@@ -1132,10 +1167,6 @@ int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer, const double *pla
                        double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc, int ptr_kind,
                        void *stream)
 {
-    (void)iceflgsw;
-    (void)liqflgsw;
-    (void)reicmcl;
-    (void)relqmcl;
     (void)tauaer;
     (void)ssaaer;
     (void)asmaer;
@@ -1146,10 +1177,15 @@ int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer, const double *pla
     cl.mcica = 1;
     // icld = 0: no subcolumns (rrtmg_sw_rad.f90:591-597, 954)
     if (icld != 0) {
-        if (inflgsw != 0)
-            return fail(RRTMG_HIP_ENOTSUP, "inflgsw != 0: cloud optics from water paths and effective radii are not "
-                                           "implemented; pass the subcolumns' optics with inflgsw = 0");
-        const double *arr[] = {cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl};
+        // rrtmg_sw_cldprmc.f90:146-147: inflag 1 stops under McICA
+        if (inflgsw != 0 && inflgsw != 2)
+            return fail(RRTMG_HIP_EINVAL, "inflgsw must be 0 or 2 with McICA (the reference stops on 1)");
+        if (inflgsw == 2 && (iceflgsw < 1 || iceflgsw > 3))
+            return fail(RRTMG_HIP_EINVAL, "iceflgsw must be 1, 2 or 3 with inflgsw = 2");
+        if (inflgsw == 2 && liqflgsw != 1)
+            return fail(RRTMG_HIP_EINVAL, "liqflgsw must be 1 with inflgsw = 2");
+        // inflgsw 2 reads fsfcmcl nowhere (cldprmc_sw replaces the optics it scales)
+        const double *arr[] = {cldfmcl, taucmcl, ssacmcl, asmcmcl, inflgsw == 2 ? taucmcl : fsfcmcl};
         const char *nm[] = {"cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl"};
         if (ncol > 0 && nlay >= 2 && nlay <= RRTMG_HIP_MAX_NLAY) {
             const size_t n = (size_t)ncol * nlay * 112;
@@ -1159,6 +1195,8 @@ int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer, const double *pla
                     if ((rc = check_host_array(arr[i], n, nm[i]))) return rc;
                 if (ciwpmcl && (rc = check_host_array(ciwpmcl, n, "ciwpmcl"))) return rc;
                 if (clwpmcl && (rc = check_host_array(clwpmcl, n, "clwpmcl"))) return rc;
+                if (inflgsw == 2 && reicmcl && (rc = check_host_array(reicmcl, n / 112, "reicmcl"))) return rc;
+                if (inflgsw == 2 && relqmcl && (rc = check_host_array(relqmcl, n / 112, "relqmcl"))) return rc;
             } else {
                 for (int i = 0; i < 5; ++i)
                     if (!arr[i]) return fail(RRTMG_HIP_EINVAL, "null cloud array");
@@ -1171,6 +1209,13 @@ int rrtmg_hip_sw_mcica(int ncol, int nlay, int icld, int iaer, const double *pla
         cl.fsfcmc = fsfcmcl;
         cl.ciwpmc = ciwpmcl;
         cl.clwpmc = clwpmcl;
+        cl.inflag = inflgsw;
+        cl.iceflag = iceflgsw;
+        cl.liqflag = liqflgsw;
+        if (inflgsw == 2) {
+            cl.reice = reicmcl;
+            cl.reliq = relqmcl;
+        }
     }
     return sw_forward(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                       o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, swuflx, swdflx, swhr, swuflxc,

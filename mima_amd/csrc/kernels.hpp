@@ -218,6 +218,10 @@ constexpr int SW_CLD_TAB = 3 * 812 + 3 * 602 + 4 * 644 + 30 + 14;
 // bad counts the layers where the reference stops (cleared first)
 hipError_t launch_sw_cloud_wp(const ColumnInputs &in, const SwClouds &cl, const double *tab, double *tau,
                               double *ssa, double *asm_, int *bad, hipStream_t s);
+// cldprmc_sw's water-path optics per subcolumn (tau, ssa, asm_: nlay x ncol x
+// 112, g fastest), *bad counting where the reference stops
+hipError_t launch_sw_mcica_wp(const ColumnInputs &in, const SwClouds &cl, const double *tab, double *tau,
+                              double *ssa, double *asm_, int *bad, hipStream_t s);
 // flag bits per (layer, column): a fraction above 0 (the total stream
 // blends the layer's cloudy optics), above 1e-12 (reftra of the cloudy
 // optics, rrtmg_sw_spcvrt.f90:388), cldprop_sw's cloud test (the layer's

@@ -82,46 +82,24 @@ __global__ __launch_bounds__(256) void sw_flags_clear_kernel(size_t n, int *flag
     if (i < n) flags[i] = 0;
 }
 
-// cldprop_sw with inflag 2 (rrtmg_sw_cldprop.f90:143-338), one thread per
-// (layer, column), FP64 with contraction off: for a layer that passes the
-// cloud test, ice optics by iceflag 1 (the abari..fbari fits in the radius,
-// the band group by the band's upper wavenumber), 2 or 3 (tables
-// interpolated in the radius, 3 with the delta fraction), liquid by liqflag
-// 1; each phase delta-scaled by its forward fraction and combined, with the
-// reference's floors.  inflag 1: no optics (cldprop_sw has no such branch).
-// Where the reference stops, *bad counts the layer (indices stay in the
-// tables).
-__global__ __launch_bounds__(256) void sw_cloud_wp_kernel(int ncol, int nlay, SwClouds cl, const double *tab,
-                                                          double *otau, double *ossa, double *oasm, int *bad)
+// cldprop_sw's table positions for the radii (rrtmg_sw_cldprop.f90:204-208,
+// 236-240, 279-283): 1-based lower index and fraction for ice (iceflag 2 and
+// 3) and liquid; *nbad counts a radius the reference stops on (the indices
+// stay in the tables).
+__device__ __forceinline__ void sw_wp_index(int iceflag, double ciwp, double clwp, double radice, double radliq,
+                                            int &nbad, int &iindex, double &ifint, int &lindex, double &lfint)
 {
 #pragma clang fp contract(off)
-    const size_t ix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ix >= (size_t)nlay * ncol) return;
-    const double eps = 1.e-06, cldmin = 1.e-20;
-    const double *extliq1 = tab, *ssaliq1 = tab + 812, *asyliq1 = tab + 1624;
-    const double *extice2 = tab + 2436, *ssaice2 = tab + 3038, *asyice2 = tab + 3640;
-    const double *extice3 = tab + 4242, *ssaice3 = tab + 4886, *asyice3 = tab + 5530, *fdlice3 = tab + 6174;
-    const double *bar = tab + 6818, *wavenum2 = tab + 6848;
-    double *tau = otau + ix * 14, *ssa = ossa + ix * 14, *asy = oasm + ix * 14;
-    double tauctot = 0.0;
-    for (int b = 0; b < 14; ++b) {
-        tau[b] = 0.0;
-        ssa[b] = 1.0;
-        asy[b] = 0.0;
-        tauctot = tauctot + (cl.taucld ? cl.taucld[ix * 14 + b] : 0.0);
-    }
-    const double ciwp = cl.cicewp ? cl.cicewp[ix] : 0.0, clwp = cl.cliqwp ? cl.cliqwp[ix] : 0.0;
-    if (cl.inflag != 2 || !(cl.cldfr[ix] >= cldmin && (ciwp + clwp >= cldmin || tauctot >= cldmin))) return;
-    const double radice = cl.reice ? cl.reice[ix] : 0.0, radliq = cl.reliq ? cl.reliq[ix] : 0.0;
-    int nbad = 0;
-    int iindex = 1, lindex = 1;
-    double ifint = 0.0, lfint = 0.0;
+    iindex = 1;
+    lindex = 1;
+    ifint = 0.0;
+    lfint = 0.0;
     if (ciwp != 0.0) {
-        if (cl.iceflag == 1) {
+        if (iceflag == 1) {
             nbad += radice < 13.0 || radice > 130.0;
         } else {
-            const int nr = cl.iceflag == 2 ? 43 : 46;
-            nbad += radice < 5.0 || radice > (cl.iceflag == 2 ? 131.0 : 140.0);
+            const int nr = iceflag == 2 ? 43 : 46;
+            nbad += radice < 5.0 || radice > (iceflag == 2 ? 131.0 : 140.0);
             const double factor = (radice - 2.0) / 3.0;
             iindex = (int)factor;
             if (iindex == nr) iindex = nr - 1;
@@ -137,68 +115,149 @@ __global__ __launch_bounds__(256) void sw_cloud_wp_kernel(int ncol, int nlay, Sw
         lindex = lindex < 1 ? 1 : (lindex > 57 ? 57 : lindex);
         lfint = radliq - 1.5 - (double)lindex;
     }
-    for (int b = 0; b < 14; ++b) {
-        double extcoice = 0.0, ssacoice = 0.0, gice = 0.0, forwice = 0.0;
-        if (ciwp != 0.0) {
-            if (cl.iceflag == 1) {
-                const double w2 = wavenum2[b];
-                const int icx = w2 > 1.43e04 ? 0 : w2 > 7.7e03 ? 1 : w2 > 5.3e03 ? 2 : w2 > 4.0e03 ? 3 : 4;
-                extcoice = bar[icx] + bar[5 + icx] / radice;
-                ssacoice = 1.0 - bar[10 + icx] - bar[15 + icx] * radice;
-                gice = bar[20 + icx] + bar[25 + icx] * radice;
-                if (gice >= 1.0) gice = 1.0 - eps;
+}
+
+// cldprop_sw's final optics of band b (0-based) from the paths
+// (rrtmg_sw_cldprop.f90:169-337): ice by iceflag 1 (the abari..fbari fits in
+// the radius, the band group by the band's upper wavenumber), 2 or 3 (tables
+// interpolated in the radius, 3 with the delta fraction), liquid by liqflag
+// 1; each phase delta-scaled by its forward fraction and combined, with the
+// reference's floors.  FP64 with contraction off.
+__device__ __forceinline__ void sw_wp_band(const double *tab, int iceflag, int b, double ciwp, double clwp,
+                                           double radice, int iindex, double ifint, int lindex, double lfint,
+                                           int &nbad, double &otau, double &ossa, double &oasm)
+{
+#pragma clang fp contract(off)
+    const double eps = 1.e-06, cldmin = 1.e-20;
+    const double *extliq1 = tab, *ssaliq1 = tab + 812, *asyliq1 = tab + 1624;
+    const double *extice2 = tab + 2436, *ssaice2 = tab + 3038, *asyice2 = tab + 3640;
+    const double *extice3 = tab + 4242, *ssaice3 = tab + 4886, *asyice3 = tab + 5530, *fdlice3 = tab + 6174;
+    const double *bar = tab + 6818, *wavenum2 = tab + 6848;
+    double extcoice = 0.0, ssacoice = 0.0, gice = 0.0, forwice = 0.0;
+    if (ciwp != 0.0) {
+        if (iceflag == 1) {
+            const double w2 = wavenum2[b];
+            const int icx = w2 > 1.43e04 ? 0 : w2 > 7.7e03 ? 1 : w2 > 5.3e03 ? 2 : w2 > 4.0e03 ? 3 : 4;
+            extcoice = bar[icx] + bar[5 + icx] / radice;
+            ssacoice = 1.0 - bar[10 + icx] - bar[15 + icx] * radice;
+            gice = bar[20 + icx] + bar[25 + icx] * radice;
+            if (gice >= 1.0) gice = 1.0 - eps;
+            forwice = gice * gice;
+        } else {
+            const int nr = iceflag == 2 ? 43 : 46;
+            const double *ex = (iceflag == 2 ? extice2 : extice3) + b * nr;
+            const double *ss = (iceflag == 2 ? ssaice2 : ssaice3) + b * nr;
+            const double *as = (iceflag == 2 ? asyice2 : asyice3) + b * nr;
+            const int i = iindex;
+            extcoice = ex[i - 1] + ifint * (ex[i] - ex[i - 1]);
+            ssacoice = ss[i - 1] + ifint * (ss[i] - ss[i - 1]);
+            gice = as[i - 1] + ifint * (as[i] - as[i - 1]);
+            if (iceflag == 2) {
                 forwice = gice * gice;
             } else {
-                const int nr = cl.iceflag == 2 ? 43 : 46;
-                const double *ex = (cl.iceflag == 2 ? extice2 : extice3) + b * nr;
-                const double *ss = (cl.iceflag == 2 ? ssaice2 : ssaice3) + b * nr;
-                const double *as = (cl.iceflag == 2 ? asyice2 : asyice3) + b * nr;
-                const int i = iindex;
-                extcoice = ex[i - 1] + ifint * (ex[i] - ex[i - 1]);
-                ssacoice = ss[i - 1] + ifint * (ss[i] - ss[i - 1]);
-                gice = as[i - 1] + ifint * (as[i] - as[i - 1]);
-                if (cl.iceflag == 2) {
-                    forwice = gice * gice;
-                } else {
-                    const double *fd = fdlice3 + b * nr;
-                    const double fdelta = fd[i - 1] + ifint * (fd[i] - fd[i - 1]);
-                    nbad += fdelta < 0.0 || fdelta > 1.0;
-                    forwice = fdelta + 0.5 / ssacoice;
-                    if (forwice > gice) forwice = gice;
-                }
+                const double *fd = fdlice3 + b * nr;
+                const double fdelta = fd[i - 1] + ifint * (fd[i] - fd[i - 1]);
+                nbad += fdelta < 0.0 || fdelta > 1.0;
+                forwice = fdelta + 0.5 / ssacoice;
+                if (forwice > gice) forwice = gice;
             }
-            nbad += extcoice < 0.0 || ssacoice > 1.0 || ssacoice < 0.0 || gice > 1.0 || gice < 0.0;
         }
-        double extcoliq = 0.0, ssacoliq = 0.0, gliq = 0.0, forwliq = 0.0;
-        if (clwp != 0.0) {
-            const int i = lindex;
-            const double *ex = extliq1 + b * 58, *ss = ssaliq1 + b * 58, *as = asyliq1 + b * 58;
-            extcoliq = ex[i - 1] + lfint * (ex[i] - ex[i - 1]);
-            ssacoliq = ss[i - 1] + lfint * (ss[i] - ss[i - 1]);
-            if (lfint < 0.0 && ssacoliq > 1.0) ssacoliq = ss[i - 1];
-            gliq = as[i - 1] + lfint * (as[i] - as[i - 1]);
-            forwliq = gliq * gliq;
-            nbad += extcoliq < 0.0 || ssacoliq > 1.0 || ssacoliq < 0.0 || gliq > 1.0 || gliq < 0.0;
-        }
-        const double tauliqorig = clwp * extcoliq, tauiceorig = ciwp * extcoice;
-        const double ssaliq = ssacoliq * (1.0 - forwliq) / (1.0 - forwliq * ssacoliq);
-        const double tauliq = (1.0 - forwliq * ssacoliq) * tauliqorig;
-        const double ssaice = ssacoice * (1.0 - forwice) / (1.0 - forwice * ssacoice);
-        const double tauice = (1.0 - forwice * ssacoice) * tauiceorig;
-        const double scatliq = ssaliq * tauliq;
-        double scatice = ssaice * tauice;
-        double t = tauliq + tauice;
-        if (t == 0.0) t = cldmin;
-        if (scatice == 0.0) scatice = cldmin;
-        tau[b] = t;
-        ssa[b] = (scatliq + scatice) / t;
-        if (cl.iceflag == 3)
-            asy[b] = (1.0 / (scatliq + scatice)) * (scatliq * (gliq - forwliq) / (1.0 - forwliq) +
-                                                    scatice * ((gice - forwice) / (1.0 - forwice)));
-        else
-            asy[b] = (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * (gice - forwice) / (1.0 - forwice)) /
-                     (scatliq + scatice);
+        nbad += extcoice < 0.0 || ssacoice > 1.0 || ssacoice < 0.0 || gice > 1.0 || gice < 0.0;
     }
+    double extcoliq = 0.0, ssacoliq = 0.0, gliq = 0.0, forwliq = 0.0;
+    if (clwp != 0.0) {
+        const int i = lindex;
+        const double *ex = extliq1 + b * 58, *ss = ssaliq1 + b * 58, *as = asyliq1 + b * 58;
+        extcoliq = ex[i - 1] + lfint * (ex[i] - ex[i - 1]);
+        ssacoliq = ss[i - 1] + lfint * (ss[i] - ss[i - 1]);
+        if (lfint < 0.0 && ssacoliq > 1.0) ssacoliq = ss[i - 1];
+        gliq = as[i - 1] + lfint * (as[i] - as[i - 1]);
+        forwliq = gliq * gliq;
+        nbad += extcoliq < 0.0 || ssacoliq > 1.0 || ssacoliq < 0.0 || gliq > 1.0 || gliq < 0.0;
+    }
+    const double tauliqorig = clwp * extcoliq, tauiceorig = ciwp * extcoice;
+    const double ssaliq = ssacoliq * (1.0 - forwliq) / (1.0 - forwliq * ssacoliq);
+    const double tauliq = (1.0 - forwliq * ssacoliq) * tauliqorig;
+    const double ssaice = ssacoice * (1.0 - forwice) / (1.0 - forwice * ssacoice);
+    const double tauice = (1.0 - forwice * ssacoice) * tauiceorig;
+    const double scatliq = ssaliq * tauliq;
+    double scatice = ssaice * tauice;
+    double t = tauliq + tauice;
+    if (t == 0.0) t = cldmin;
+    if (scatice == 0.0) scatice = cldmin;
+    otau = t;
+    ossa = (scatliq + scatice) / t;
+    if (iceflag == 3)
+        oasm = (1.0 / (scatliq + scatice)) *
+               (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * ((gice - forwice) / (1.0 - forwice)));
+    else
+        oasm = (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * (gice - forwice) / (1.0 - forwice)) /
+               (scatliq + scatice);
+}
+
+// cldprop_sw with inflag 2 (rrtmg_sw_cldprop.f90:143-338), one thread per
+// (layer, column): a layer that passes the cloud test gets each band's
+// final optics from its paths; any other depth 0, albedo 1, asymmetry 0.
+// inflag 1: no optics (cldprop_sw has no such branch).  Where the
+// reference stops, *bad counts the layer.
+__global__ __launch_bounds__(256) void sw_cloud_wp_kernel(int ncol, int nlay, SwClouds cl, const double *tab,
+                                                          double *otau, double *ossa, double *oasm, int *bad)
+{
+#pragma clang fp contract(off)
+    const size_t ix = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ix >= (size_t)nlay * ncol) return;
+    const double cldmin = 1.e-20;
+    double *tau = otau + ix * 14, *ssa = ossa + ix * 14, *asy = oasm + ix * 14;
+    double tauctot = 0.0;
+    for (int b = 0; b < 14; ++b) {
+        tau[b] = 0.0;
+        ssa[b] = 1.0;
+        asy[b] = 0.0;
+        tauctot = tauctot + (cl.taucld ? cl.taucld[ix * 14 + b] : 0.0);
+    }
+    const double ciwp = cl.cicewp ? cl.cicewp[ix] : 0.0, clwp = cl.cliqwp ? cl.cliqwp[ix] : 0.0;
+    if (cl.inflag != 2 || !(cl.cldfr[ix] >= cldmin && (ciwp + clwp >= cldmin || tauctot >= cldmin))) return;
+    const double radice = cl.reice ? cl.reice[ix] : 0.0, radliq = cl.reliq ? cl.reliq[ix] : 0.0;
+    int nbad = 0, iindex, lindex;
+    double ifint, lfint;
+    sw_wp_index(cl.iceflag, ciwp, clwp, radice, radliq, nbad, iindex, ifint, lindex, lfint);
+    for (int b = 0; b < 14; ++b)
+        sw_wp_band(tab, cl.iceflag, b, ciwp, clwp, radice, iindex, ifint, lindex, lfint, nbad, tau[b], ssa[b], asy[b]);
+    if (nbad && bad) atomicAdd(bad, nbad);
+}
+
+// cldprmc_sw with inflag 2 (rrtmg_sw_cldprmc.f90:124-330), one thread per
+// subcolumn o = (layer, column, g-point), g fastest: a subcolumn that passes
+// its cloud test gets cldprop_sw's optics of its band ngb(g) from its own
+// paths and the layer's radii (reicmc, relqmc); any other keeps its input
+// depth, albedo and asymmetry.
+__global__ __launch_bounds__(256) void sw_mcica_wp_kernel(int ncol, int nlay, SwClouds cl, const double *tab,
+                                                          double *otau, double *ossa, double *oasm, int *bad)
+{
+#pragma clang fp contract(off)
+    const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= (size_t)nlay * ncol * 112) return;
+    const size_t ix = o / 112;
+    const int g = (int)(o - ix * 112);
+    const double cldmin = 1.e-20;
+    const double cf = cl.cldfmc[o], tin = cl.taucmc[o];
+    const double ciwp = cl.ciwpmc ? cl.ciwpmc[o] : 0.0, clwp = cl.clwpmc ? cl.clwpmc[o] : 0.0;
+    double t = tin, w = cl.ssacmc[o], a = cl.asmcmc[o];
+    int nbad = 0;
+    if (cf >= cldmin && (ciwp + clwp >= cldmin || tin >= cldmin)) {
+        const int ngc[14] = {6, 12, 8, 8, 10, 10, 2, 10, 8, 6, 6, 8, 6, 12};
+        int b = 0;
+        for (int acc = ngc[0]; g >= acc && b < 13; acc += ngc[++b]) {
+        }
+        const double radice = cl.reice ? cl.reice[ix] : 0.0, radliq = cl.reliq ? cl.reliq[ix] : 0.0;
+        int iindex, lindex;
+        double ifint, lfint;
+        sw_wp_index(cl.iceflag, ciwp, clwp, radice, radliq, nbad, iindex, ifint, lindex, lfint);
+        sw_wp_band(tab, cl.iceflag, b, ciwp, clwp, radice, iindex, ifint, lindex, lfint, nbad, t, w, a);
+    }
+    otau[o] = t;
+    ossa[o] = w;
+    oasm[o] = a;
     if (nbad && bad) atomicAdd(bad, nbad);
 }
 
@@ -208,6 +267,16 @@ hipError_t launch_sw_cloud_wp(const ColumnInputs &in, const SwClouds &cl, const 
     const size_t n2 = (size_t)in.nlay * in.ncol;
     if (bad) hipLaunchKernelGGL(sw_cloud_zero_kernel, dim3(1), dim3(1), 0, s, bad);
     hipLaunchKernelGGL(sw_cloud_wp_kernel, dim3((unsigned)((n2 + 255) / 256)), dim3(256), 0, s, in.ncol, in.nlay, cl,
+                       tab, tau, ssa, asm_, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_mcica_wp(const ColumnInputs &in, const SwClouds &cl, const double *tab, double *tau,
+                              double *ssa, double *asm_, int *bad, hipStream_t s)
+{
+    const size_t n = (size_t)in.nlay * in.ncol * 112;
+    if (bad) hipLaunchKernelGGL(sw_cloud_zero_kernel, dim3(1), dim3(1), 0, s, bad);
+    hipLaunchKernelGGL(sw_mcica_wp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, in.ncol, in.nlay, cl,
                        tab, tau, ssa, asm_, bad);
     return hipGetLastError();
 }
@@ -336,7 +405,8 @@ __device__ __forceinline__ SwLayerRT<Real> sw_cloudy_blend(double ztauc, double 
 // McICA (MC): the lane's subcolumn o = (layer, column, g-point) instead of
 // the layer and band; cldprmc_sw scales the optics of a subcolumn that
 // passes its cloud test and leaves any other's as input
-// (rrtmg_sw_cldprmc.f90:127-143), where cldprop_sw sets depth 0, albedo 1.
+// (rrtmg_sw_cldprmc.f90:127-143), where cldprop_sw sets depth 0, albedo 1;
+// with water paths (scaled) the subcolumn's optics are cldprmc_sw's final ones.
 template <int JB, bool MC, typename Real>
 __device__ __forceinline__ void sw_cloud_layer(Real taug, Real taur, Real prmu0, const Real *__restrict__ exp_tbl,
                                                int fl, double cf, const SwClouds &cl, size_t ix, size_t o,
@@ -356,7 +426,7 @@ __device__ __forceinline__ void sw_cloud_layer(Real taug, Real taur, Real prmu0,
         pt = cl.taucmc[o];
         pw = cl.ssacmc[o];
         pg = cl.asmcmc[o];
-        if (fl & SWC_OPTICS) {
+        if ((fl & SWC_OPTICS) && !cl.scaled) {
             const double ffp = cl.fsfcmc[o];
             const double ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * pw;
             pw = ffp1 * pw / ffpssa;

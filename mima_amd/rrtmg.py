@@ -294,11 +294,14 @@ def mcica_subcol_sw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, cl
 
 def rrtmg_sw_mcica(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr,
                    o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, inflgsw, iceflgsw, liqflgsw,
-                   cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl=None, clwpmcl=None, stream=None):
+                   cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl=None, clwpmcl=None, reicmcl=None,
+                   relqmcl=None, stream=None):
     """rrtmg_sw of the McICA driver (rrtmg_sw_rad.f90:80-90) on subcolumns
-    (nlay, ncol, 112); inflgsw = 0.  h2ovmr and o3vmr are volume mixing
-    ratios; night columns are computed at a zenith cosine of 1e-10.
-    Returns (swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)."""
+    (nlay, ncol, 112).  inflgsw 0: the subcolumns' optics as given; 2:
+    cldprmc_sw's optics from ciwpmcl, clwpmcl and the layer radii reicmcl,
+    relqmcl ((nlay, ncol); iceflgsw 1-3, liqflgsw 1).  h2ovmr and o3vmr are
+    volume mixing ratios; night columns are computed at a zenith cosine of
+    1e-10.  Returns (swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)."""
     dev = _is_torch(play) and play.is_cuda
     a = _Args(dev)
     outs = _outputs(play if dev else None, nlay, ncol)
@@ -307,8 +310,8 @@ def rrtmg_sw_mcica(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, h2ovmr,
         *[a.inp(x) for x in (play, plev, tlay, tlev, tsfc, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
                              asdir, asdif, aldir, aldif, coszen)],
         ctypes.c_double(adjes), int(dyofyr), ctypes.c_double(scon), int(inflgsw), int(iceflgsw), int(liqflgsw),
-        *[a.inp(x) for x in (cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl)],
-        None, None, None, None, None, None, *_ptrs(dev, outs),
+        *[a.inp(x) for x in (cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl)],
+        None, None, None, None, *_ptrs(dev, outs),
         DEVICE if dev else HOST, _stream_ptr(stream) if dev else ctypes.c_void_p(0))
     _check(rc)
     return tuple(outs)

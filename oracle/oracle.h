@@ -202,6 +202,21 @@ int oracle_rrtmg_sw_cloud_wp(int ncol, int nlay, int icld, int inflag, int icefl
                              double *swuflx, double *swdflx, double *swhr, double *swuflxc, double *swdflxc,
                              double *swhrc);
 
+/* oracle_rrtmg_sw_mcica with cldprmc_sw's water-path optics (inflgsw = 2,
+ * rrtmg_sw_cldprmc.f90:144-330: cldprop_sw's per-band optics for each
+ * subcolumn passing the cloud test, from its paths and the layer's radii
+ * reicmcl, relqmcl (ncol, nlay); any other keeps its input optics), after
+ * oracle_sw_cld_init.  -8 where the reference stops. */
+int oracle_rrtmg_sw_mcica_wp(int ncol, int nlay, int iceflag, int liqflag, const double *play, const double *plev,
+                             const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
+                             const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                             const double *o2vmr, const double *asdir, const double *asdif, const double *aldir,
+                             const double *aldif, const double *coszen, double adjes, int dyofyr, double scon,
+                             const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+                             const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+                             const double *clwpmcl, const double *reicmcl, const double *relqmcl, double *swuflx,
+                             double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc);
+
 /* oracle_rrtmg_lw_mcica with cldprmc's water-path optics (inflglw = 2,
  * rrtmg_lw_cldprmc.f90:150-230): per subcolumn that passes the cloud test,
  * its depth from ciwpmcl, clwpmcl (140, ncol, nlay) and the layer's radii

@@ -1090,6 +1090,86 @@ int oracle_sw_cld_init(const char *path)
     return 0;
 }
 
+/* cldprop_sw's water-path optics of one band (0-based ib) for the given
+ * paths and radii (rrtmg_sw_cldprop.f90:169-337, per band): r = final
+ * depth, albedo, asymmetry.  -8 where the reference stops. */
+static int sw_wp_optics(int ib, int iceflag, int liqflag, double ciwp, double clwp, double radice, double radliq,
+                        double r[3])
+{
+    const double eps = 1.e-06, cldmin = 1.e-20;
+    double extcoice = 0.0, ssacoice = 0.0, gice = 0.0, forwice = 0.0;
+    double extcoliq = 0.0, ssacoliq = 0.0, gliq = 0.0, forwliq = 0.0;
+    if (ciwp != 0.0) {
+        if (iceflag == 1) {
+            if (radice < 13.0 || radice > 130.0) return -8;
+            double w2 = SC.wavenum2[ib];
+            int icx = w2 > 1.43e04 ? 1 : w2 > 7.7e03 ? 2 : w2 > 5.3e03 ? 3 : w2 > 4.0e03 ? 4 : 5;
+            extcoice = (SC.abari[icx - 1] + SC.bbari[icx - 1] / radice);
+            ssacoice = 1.0 - SC.cbari[icx - 1] - SC.dbari[icx - 1] * radice;
+            gice = SC.ebari[icx - 1] + SC.fbari[icx - 1] * radice;
+            if (gice >= 1.0) gice = 1.0 - eps;
+            forwice = gice * gice;
+        } else if (iceflag == 2 || iceflag == 3) {
+            const int nr = iceflag == 2 ? 43 : 46;
+            if (radice < 5.0 || radice > (iceflag == 2 ? 131.0 : 140.0)) return -8;
+            double factor = (radice - 2.0) / 3.0;
+            int index = (int)factor;
+            if (index == nr) index = nr - 1;
+            double fint = factor - (double)index;
+            const double *ex = iceflag == 2 ? SC.extice2[ib] : SC.extice3[ib];
+            const double *ss = iceflag == 2 ? SC.ssaice2[ib] : SC.ssaice3[ib];
+            const double *as = iceflag == 2 ? SC.asyice2[ib] : SC.asyice3[ib];
+            extcoice = ex[index - 1] + fint * (ex[index] - ex[index - 1]);
+            ssacoice = ss[index - 1] + fint * (ss[index] - ss[index - 1]);
+            gice = as[index - 1] + fint * (as[index] - as[index - 1]);
+            if (iceflag == 2) {
+                forwice = gice * gice;
+            } else {
+                const double *fd = SC.fdlice3[ib];
+                double fdelta = fd[index - 1] + fint * (fd[index] - fd[index - 1]);
+                if (fdelta < 0.0 || fdelta > 1.0) return -8;
+                forwice = fdelta + 0.5 / ssacoice;
+                if (forwice > gice) forwice = gice;
+            }
+        } else {
+            return -8;
+        }
+        if (extcoice < 0.0 || ssacoice > 1.0 || ssacoice < 0.0 || gice > 1.0 || gice < 0.0) return -8;
+    }
+    if (clwp != 0.0) {
+        if (liqflag != 1 || radliq < 2.5 || radliq > 60.0) return -8;
+        int index = (int)(radliq - 1.5);
+        if (index == 0) index = 1;
+        if (index == 58) index = 57;
+        double fint = radliq - 1.5 - (double)index;
+        const double *ex = SC.extliq1[ib], *ss = SC.ssaliq1[ib], *as = SC.asyliq1[ib];
+        extcoliq = ex[index - 1] + fint * (ex[index] - ex[index - 1]);
+        ssacoliq = ss[index - 1] + fint * (ss[index] - ss[index - 1]);
+        if (fint < 0.0 && ssacoliq > 1.0) ssacoliq = ss[index - 1];
+        gliq = as[index - 1] + fint * (as[index] - as[index - 1]);
+        forwliq = gliq * gliq;
+        if (extcoliq < 0.0 || ssacoliq > 1.0 || ssacoliq < 0.0 || gliq > 1.0 || gliq < 0.0) return -8;
+    }
+    double tauliqorig = clwp * extcoliq, tauiceorig = ciwp * extcoice;
+    double ssaliq = ssacoliq * (1.0 - forwliq) / (1.0 - forwliq * ssacoliq);
+    double tauliq = (1.0 - forwliq * ssacoliq) * tauliqorig;
+    double ssaice = ssacoice * (1.0 - forwice) / (1.0 - forwice * ssacoice);
+    double tauice = (1.0 - forwice * ssacoice) * tauiceorig;
+    double scatliq = ssaliq * tauliq, scatice = ssaice * tauice;
+    double t = tauliq + tauice;
+    if (t == 0.0) t = cldmin;
+    if (scatice == 0.0) scatice = cldmin;
+    r[0] = t;
+    r[1] = (scatliq + scatice) / t;
+    if (iceflag == 3)
+        r[2] = (1.0 / (scatliq + scatice)) *
+               (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * ((gice - forwice) / (1.0 - forwice)));
+    else
+        r[2] = (scatliq * (gliq - forwliq) / (1.0 - forwliq) + scatice * (gice - forwice) / (1.0 - forwice)) /
+               (scatliq + scatice);
+    return 0;
+}
+
 /* cldprop_sw with inflag 2 (rrtmg_sw_cldprop.f90:143-338): per layer that
  * passes the cloud test, ice extinction, albedo and asymmetry by iceflag 1
  * (the abari..fbari fits in the radius, band group by the band's upper
@@ -1665,7 +1745,8 @@ int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
  * and water path or depth reach 1e-20 has its optics delta-M scaled by its
  * forward fraction, any other keeps its input optics; spcvmc_sw on them.
  * Subcolumn inputs (112, ncol, nlay) (NULL water paths = 0). */
-int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+static int sw_mcica_driver(int ncol, int nlay, int iceflag, int liqflag, const double *reicmcl,
+                           const double *relqmcl, const double *play, const double *plev, const double *tlay,
                           const double *tlev, const double *h2ovmr, const double *o3vmr, const double *co2vmr,
                           const double *ch4vmr, const double *n2ovmr, const double *o2vmr, const double *asdir,
                           const double *asdif, const double *aldir, const double *aldif, const double *coszen,
@@ -1688,6 +1769,7 @@ int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *
         double (*gq)[MAXLAY + 2][NGSW + 1] = malloc(sizeof(double[4][MAXLAY + 2][NGSW + 1]));
         double zbbcu[MAXLAY + 3], zbbcd[MAXLAY + 3], zbbfu[MAXLAY + 3], zbbfd[MAXLAY + 3];
         double swnflxc[MAXLAY + 3], swnflx[MAXLAY + 3];
+        int bad = 0;
         if (!c || !gq) {
 #pragma omp atomic write
             err = -3;
@@ -1703,16 +1785,37 @@ int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *
                     double cf = cldfmcl[o], tau = taucmcl[o], ssa = ssacmcl[o], asm_ = asmcmcl[o];
                     double cwp = (ciwpmcl ? ciwpmcl[o] : 0.0) + (clwpmcl ? clwpmcl[o] : 0.0);
                     if (cf >= cldmin && (cwp >= cldmin || tau >= cldmin)) {
-                        double ffp = fsfcmcl[o], ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssa;
-                        ssa = ffp1 * ssa / ffpssa;
-                        tau = ffpssa * tau;
-                        asm_ = (asm_ - ffp) / ffp1;
+                        if (iceflag > 0) {
+                            /* cldprmc_sw with inflag 2 (rrtmg_sw_cldprmc.f90:144-330):
+                             * cldprop_sw's optics per subcolumn, band ngb(ig) */
+                            int ib = 0;
+                            for (int acc = S.ngc[0]; iw > acc; acc += S.ngc[++ib]) {
+                            }
+                            double ci = ciwpmcl ? ciwpmcl[o] : 0.0, cw = clwpmcl ? clwpmcl[o] : 0.0;
+                            double r[3];
+                            if (sw_wp_optics(ib, iceflag, liqflag, ci, cw, reicmcl ? reicmcl[ix] : 0.0,
+                                             relqmcl ? relqmcl[ix] : 0.0, r) != 0)
+                                bad = 1;
+                            tau = r[0];
+                            ssa = r[1];
+                            asm_ = r[2];
+                        } else {
+                            double ffp = fsfcmcl[o], ffp1 = 1.0 - ffp, ffpssa = 1.0 - ffp * ssa;
+                            ssa = ffp1 * ssa / ffpssa;
+                            tau = ffpssa * tau;
+                            asm_ = (asm_ - ffp) / ffp1;
+                        }
                     }
                     gq[0][l][iw] = cf;
                     gq[1][l][iw] = tau;
                     gq[2][l][iw] = asm_;
                     gq[3][l][iw] = ssa;
                 }
+            }
+            if (bad) {
+#pragma omp atomic write
+                err = -8;
+                continue;
             }
             sw_column_u(c, iplon, ncol, nlay, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr, n2ovmr, o2vmr,
                         1);
@@ -1752,6 +1855,38 @@ int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *
         free(gq);
     }
     return err;
+}
+
+int oracle_rrtmg_sw_mcica(int ncol, int nlay, const double *play, const double *plev, const double *tlay,
+                          const double *tlev, const double *h2ovmr, const double *o3vmr, const double *co2vmr,
+                          const double *ch4vmr, const double *n2ovmr, const double *o2vmr, const double *asdir,
+                          const double *asdif, const double *aldir, const double *aldif, const double *coszen,
+                          double adjes, int dyofyr, double scon, const double *cldfmcl, const double *taucmcl,
+                          const double *ssacmcl, const double *asmcmcl, const double *fsfcmcl,
+                          const double *ciwpmcl, const double *clwpmcl, double *swuflx, double *swdflx,
+                          double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
+{
+    return sw_mcica_driver(ncol, nlay, 0, 0, NULL, NULL, play, plev, tlay, tlev, h2ovmr, o3vmr, co2vmr, ch4vmr,
+                           n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, cldfmcl, taucmcl,
+                           ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl, swuflx, swdflx, swhr, swuflxc, swdflxc,
+                           swhrc);
+}
+
+int oracle_rrtmg_sw_mcica_wp(int ncol, int nlay, int iceflag, int liqflag, const double *play, const double *plev,
+                             const double *tlay, const double *tlev, const double *h2ovmr, const double *o3vmr,
+                             const double *co2vmr, const double *ch4vmr, const double *n2ovmr,
+                             const double *o2vmr, const double *asdir, const double *asdif, const double *aldir,
+                             const double *aldif, const double *coszen, double adjes, int dyofyr, double scon,
+                             const double *cldfmcl, const double *taucmcl, const double *ssacmcl,
+                             const double *asmcmcl, const double *fsfcmcl, const double *ciwpmcl,
+                             const double *clwpmcl, const double *reicmcl, const double *relqmcl, double *swuflx,
+                             double *swdflx, double *swhr, double *swuflxc, double *swdflxc, double *swhrc)
+{
+    if (!SC.ready || iceflag < 1 || iceflag > 3) return -1;
+    return sw_mcica_driver(ncol, nlay, iceflag, liqflag, reicmcl, relqmcl, play, plev, tlay, tlev, h2ovmr, o3vmr,
+                           co2vmr, ch4vmr, n2ovmr, o2vmr, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon,
+                           cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl, swuflx, swdflx, swhr,
+                           swuflxc, swdflxc, swhrc);
 }
 
 /* ---- single-column intermediates ---- */

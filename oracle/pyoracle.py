@@ -279,6 +279,31 @@ def rrtmg_sw_mcica(c, sub):
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
 
 
+def rrtmg_sw_mcica_wp(c, iceflag, liqflag, sub, reicmcl, relqmcl):
+    """rrtmg_sw of the McICA driver with water-path optics (inflgsw = 2) on
+    subcolumns (nlay, ncol, 112) and layer radii (nlay, ncol)."""
+    global _swcld_ready
+    L = lib()
+    if not _swcld_ready:
+        if L.oracle_sw_cld_init(SW_CLD_TABLE.encode()) != 0:
+            raise RuntimeError("oracle_sw_cld_init failed")
+        _swcld_ready = True
+    ncol, nlay = c["ncol"], c["nlay"]
+    ins = [_f64(c[k]) for k in ("play", "plev", "tlay", "tlev", "h2ovmr", "o3vmr", "co2vmr", "ch4vmr",
+                                "n2ovmr", "o2vmr", "asdir", "asdif", "aldir", "aldif", "coszen")]
+    sb = [None if sub.get(k) is None else _p(_f64(sub[k])) for k in
+          ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl")]
+    sb += [_p(_f64(reicmcl)), _p(_f64(relqmcl))]
+    outs = [np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol)),
+            np.zeros((nlay + 1, ncol)), np.zeros((nlay + 1, ncol)), np.zeros((nlay, ncol))]
+    rc = L.oracle_rrtmg_sw_mcica_wp(ncol, nlay, int(iceflag), int(liqflag), *[_p(a) for a in ins],
+                                    ctypes.c_double(c["adjes"]), int(c["dyofyr"]), ctypes.c_double(c["scon"]), *sb,
+                                    *[_p(a) for a in outs])
+    if rc != 0:
+        raise RuntimeError("oracle_rrtmg_sw_mcica_wp returned %d" % rc)
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
 def _col(c, i, key):
     v = c[key]
     return np.ascontiguousarray(v[:, i], dtype=np.float64)

@@ -10,6 +10,8 @@
 !   ref_sw_mcica_subcol(...)    -> mcica_subcol_sw (mcica_subcol_gen_sw.f90:47)
 !   ref_sw_mcica_run(...)       -> rrtmg_sw of the McICA driver
 !                                  (rrtmg_sw_rad.f90:80), iaer = 0, inflgsw = 0
+!   ref_sw_mcica_run_wp(...)    -> the same with inflgsw = 2 (cldprmc_sw's
+!                                  water-path optics) and given flags, radii
 module ref_sw_mcica_driver
   use iso_c_binding
   use parkind, only : im => kind_im, rb => kind_rb
@@ -87,4 +89,42 @@ contains
                   swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
   end subroutine ref_sw_mcica_run
 
+  ! rrtmg_sw of the McICA driver with water-path optics (inflgsw = 2):
+  ! cldprmc_sw forms each cloudy subcolumn's optics from ciwpmcl, clwpmcl
+  ! and the layer radii reicmcl, relqmcl (ncol, nlay)
+  subroutine ref_sw_mcica_run_wp(ncol, nlay, icld_in, iceflag, liqflag, play, plev, tlay, tlev, tsfc, h2o, o3, &
+                                 co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, coszen, adjes, dyofyr, scon, &
+                                 cldfmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, &
+                                 swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc) bind(C, name="ref_sw_mcica_run_wp")
+    integer(c_int), value :: ncol, nlay, dyofyr, icld_in, iceflag, liqflag
+    real(c_double), value :: adjes, scon
+    real(c_double), intent(in) :: play(ncol, nlay), plev(ncol, nlay + 1), tlay(ncol, nlay)
+    real(c_double), intent(in) :: tlev(ncol, nlay + 1), tsfc(ncol)
+    real(c_double), intent(in) :: h2o(ncol, nlay), o3(ncol, nlay), co2(ncol, nlay)
+    real(c_double), intent(in) :: ch4(ncol, nlay), n2o(ncol, nlay), o2(ncol, nlay)
+    real(c_double), intent(in) :: asdir(ncol), asdif(ncol), aldir(ncol), aldif(ncol), coszen(ncol)
+    real(c_double), intent(in) :: cldfmcl(ngptsw, ncol, nlay), taucmcl(ngptsw, ncol, nlay)
+    real(c_double), intent(in) :: ssacmcl(ngptsw, ncol, nlay), asmcmcl(ngptsw, ncol, nlay)
+    real(c_double), intent(in) :: fsfcmcl(ngptsw, ncol, nlay), ciwpmcl(ngptsw, ncol, nlay)
+    real(c_double), intent(in) :: clwpmcl(ngptsw, ncol, nlay), reicmcl(ncol, nlay), relqmcl(ncol, nlay)
+    real(c_double), intent(out) :: swuflx(ncol, nlay + 1), swdflx(ncol, nlay + 1), swhr(ncol, nlay)
+    real(c_double), intent(out) :: swuflxc(ncol, nlay + 1), swdflxc(ncol, nlay + 1), swhrc(ncol, nlay)
+    integer(kind=im) :: icld, iaer, dy, inflg, iceflg, liqflg
+    real(kind=rb), allocatable :: aerb(:, :, :), ecaer(:, :, :)
+    allocate(aerb(ncol, nlay, nbndsw), ecaer(ncol, nlay, naerec))
+    aerb = 0.0_rb
+    ecaer = 0.0_rb
+    icld = icld_in
+    iaer = 0
+    dy = dyofyr
+    inflg = 2
+    iceflg = iceflag
+    liqflg = liqflag
+    call rrtmg_sw(ncol, nlay, icld, iaer, play, plev, tlay, tlev, tsfc, &
+                  h2o, o3, co2, ch4, n2o, o2, asdir, asdif, aldir, aldif, &
+                  coszen, adjes, dy, scon, inflg, iceflg, liqflg, cldfmcl, &
+                  taucmcl, ssacmcl, asmcmcl, fsfcmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, &
+                  aerb, aerb, aerb, ecaer, &
+                  swuflx, swdflx, swhr, swuflxc, swdflxc, swhrc)
+  end subroutine ref_sw_mcica_run_wp
 end module ref_sw_mcica_driver

@@ -213,6 +213,22 @@ def run_sw_mcica(lib, c, icld, sub):
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
 
 
+def run_sw_mcica_wp(lib, c, icld, iceflag, liqflag, sub, reicmcl, relqmcl):
+    """rrtmg_sw of the reference's McICA driver with inflgsw = 2 on
+    subcolumns `sub`; radii (nlay, ncol)."""
+    n, L = c["ncol"], c["nlay"]
+    ins = [np.ascontiguousarray(c[k], dtype=np.float64) for k in SW_IN]
+    sb = [np.ascontiguousarray(sub[k], dtype=np.float64) for k in
+          ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl", "ciwpmcl", "clwpmcl")]
+    sb += [np.ascontiguousarray(a, dtype=np.float64) for a in (reicmcl, relqmcl)]
+    outs = [np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n)),
+            np.zeros((L + 1, n)), np.zeros((L + 1, n)), np.zeros((L, n))]
+    lib.ref_sw_mcica_run_wp(n, L, int(icld), int(iceflag), int(liqflag), *[_p(a) for a in ins],
+                            ctypes.c_double(c["adjes"]), int(c["dyofyr"]), ctypes.c_double(c["scon"]),
+                            *[_p(a) for a in sb], *[_p(a) for a in outs])
+    return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
 def set_lw_cld(lib, setter="ref_lw_set_cld"):
     """Fill the LW reference library's rrlw_cld module with lwcldpr's
     coefficients as extracted into rrtmg_lw_cld.tbl (setter

@@ -228,7 +228,7 @@ def test_binary_clouds_equal_nomcica(rr):
 
 def test_errors(rr):
     """Refusals: irng = 1, icld outside 0..3 and nlay < 4 for the
-    generator, inflgsw != 0, iaer != 0, the smooth mode."""
+    generator, inflgsw 1 (the reference stops), iaer != 0, the smooth mode."""
     c, cf, opt, wp = _sample(64, 40, 70)
     ncol, nlay = c["ncol"], c["nlay"]
     with pytest.raises(rr.RRTMGError, match="irng"):
@@ -241,7 +241,7 @@ def test_errors(rr):
     arrs = [sub[k] for k in ("cldfmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl")]
     sc = (c["adjes"], c["dyofyr"], c["scon"])
     with pytest.raises(rr.RRTMGError, match="inflgsw"):
-        rr.rrtmg_sw_mcica(ncol, nlay, 2, 0, *[c[k] for k in KEYS], *sc, 2, 0, 0, *arrs)
+        rr.rrtmg_sw_mcica(ncol, nlay, 2, 0, *[c[k] for k in KEYS], *sc, 1, 0, 0, *arrs)
     with pytest.raises(rr.RRTMGError, match="iaer"):
         rr.rrtmg_sw_mcica(ncol, nlay, 2, 10, *[c[k] for k in KEYS], *sc, 0, 0, 0, *arrs)
     rr.set_precision("smooth")

@@ -118,3 +118,64 @@ def test_subcolumn_cover_follows_fraction():
         sigma = np.sqrt(cf * (1.0 - cf) / 112.0)
         assert np.all(np.abs(share - cf) <= 4.5 * sigma + 1e-12)
         assert np.all(share[cf == 0] == 0.0)
+
+
+WGOLD = dict(np.load(os.path.join(ROOT, "tests", "golden", "sw_ref_mcica_wp.npz"), allow_pickle=False))
+
+
+@pytest.mark.parametrize("k", range(4))
+def test_mcica_water_paths_match_reference(k):
+    """cldprmc_sw with inflgsw = 2 (rrtmg_sw_cldprmc.f90:144-330): every ice
+    parameterisation with liquid 1, icld 1-3, radii at the ends of the
+    tables, subcolumns that pass the cloud test on depth alone and some that
+    fail it; the reference's fluxes."""
+    c = {kk: WGOLD[kk] for kk in KEYS}
+    c["ncol"], c["nlay"] = int(WGOLD["ncol"]), int(WGOLD["nlay"])
+    c["adjes"], c["dyofyr"], c["scon"] = float(WGOLD["adjes"]), int(WGOLD["dyofyr"]), float(WGOLD["scon"])
+    iceflag, liqflag, icld = (int(x) for x in WGOLD["cases"][k])
+    sub = po.sw_mcica_subcol(c, icld, WGOLD["cldfr"], *[WGOLD[kk] for kk in OPT], cicewp=WGOLD["cicewp"],
+                             cliqwp=WGOLD["cliqwp"])
+    np.testing.assert_array_equal(sub["cldfmcl"].astype(np.uint8), WGOLD["m%d" % k])
+    o = po.rrtmg_sw_mcica_wp(c, iceflag, liqflag, sub, WGOLD["reice"], WGOLD["reliq"])
+    for kk in OUT:
+        np.testing.assert_allclose(o[kk], WGOLD["c%d_%s" % (k, kk)], rtol=1e-12, atol=1e-11, err_msg=kk)
+    assert np.max(np.abs(o["swdflx"] - o["swdflxc"])) > 50.0
+
+
+def test_binary_water_path_clouds_are_the_nomcica_path():
+    """Fractions 0 or 1: cldprmc_sw's optics are cldprop_sw's per g-point,
+    so the McICA driver's day columns equal the nomcica driver's with the
+    same water paths (1e-10 W/m2)."""
+    c = _cols()
+    cf = make_sw_clouds(c, seed=5, edge_cases=False)[0]
+    rng = np.random.default_rng(15)
+    ice = np.where((cf > 0) & (rng.random(cf.shape) < 0.7), rng.uniform(0.5, 60.0, cf.shape), 0.0)
+    liq = np.where((cf > 0) & (rng.random(cf.shape) < 0.7), rng.uniform(0.5, 80.0, cf.shape), 0.0)
+    rei, rel = rng.uniform(13.0, 130.0, cf.shape), rng.uniform(2.5, 60.0, cf.shape)
+    v = dict(c)
+    v["h2ovmr"] = (c["h2ovmr"] / (1.0 - c["h2ovmr"])) * 1.607793
+    v["o3vmr"] = c["o3vmr"] * 0.603428
+    day = c["coszen"] >= 1e-10
+    z = np.zeros(cf.shape + (14,))
+    for iceflag in (1, 2, 3):
+        ref = po.rrtmg_sw_cloud_wp(c, 2, 2, iceflag, 1, cf, ice, liq, rei, rel)
+        sub = po.sw_mcica_subcol(v, 2, cf, z, z + 1.0, z, z, cicewp=ice, cliqwp=liq)
+        o = po.rrtmg_sw_mcica_wp(v, iceflag, 1, sub, rei, rel)
+        for k in OUT:
+            np.testing.assert_allclose(o[k][:, day], ref[k][:, day], rtol=0, atol=1e-10, err_msg="%d %s" % (iceflag, k))
+        assert np.max(np.abs(o["swdflx"] - o["swdflxc"])) > 50.0
+
+
+def test_mcica_water_path_refusals():
+    # a radius the reference stops on in a cloudy subcolumn: the oracle reports -8
+    c = _cols(32, 4)
+    cf = make_sw_clouds(c, seed=5, edge_cases=False)[0]
+    z = np.zeros(cf.shape + (14,))
+    w = np.where(cf > 0, 10.0, 0.0)
+    sub = po.sw_mcica_subcol(c, 2, cf, z, z + 1.0, z, z, cicewp=w, cliqwp=w)
+    rei, rel = np.full(cf.shape, 40.0), np.full(cf.shape, 10.0)
+    po.rrtmg_sw_mcica_wp(c, 3, 1, sub, rei, rel)
+    bad = rei.copy()
+    bad[np.argwhere(cf > 0)[0][0], np.argwhere(cf > 0)[0][1]] = 150.0
+    with pytest.raises(RuntimeError, match="-8"):
+        po.rrtmg_sw_mcica_wp(c, 3, 1, sub, bad, rel)
