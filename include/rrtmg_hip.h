@@ -188,15 +188,19 @@ int rrtmg_hip_lw_cloud(int ncol, int nlay, int icld, int idrv,
 
 /* Replaces mcica_subcol_lw (rrtmg_lw/gcm_model/src/mcica_subcol_gen_lw.f90:48-154):
  * the McICA stochastic cloud subcolumns, one per LW g-point (140), with the
- * reference's whole argument list in its order.  irng = 0 (KISS generator,
- * :323-358, 530-562) only; irng != 0 (Mersenne Twister) returns
- * RRTMG_HIP_ENOTSUP.  Seeds come from the fractional Pa of the four lowest
- * layers of play (ncol, nlay), advanced permuteseed times.  icld = 1 random,
- * 2 maximum-random, 3 maximum overlap (:364-445); icld = 0 returns at once
- * with the outputs untouched; icld outside 0..3 and nlay < 4 return
- * RRTMG_HIP_EINVAL, and so does a column whose second layer has the higher
- * pressure (the reference stops, :327-329; checked on the host for host
- * pointers, on the device for device pointers with rrtmg_hip_set_checks on).
+ * reference's whole argument list in its order.  irng = 0: the KISS
+ * generator (:323-358, 530-562), seeds from the fractional Pa of the four
+ * lowest layers of play (ncol, nlay), advanced permuteseed times; nlay < 4
+ * and a column whose second layer has the higher pressure return
+ * RRTMG_HIP_EINVAL (the reference stops, :327-329; checked on the host for
+ * host pointers, on the device for device pointers with
+ * rrtmg_hip_set_checks on).  irng != 0: the Mersenne Twister
+ * (mcica_random_numbers.f90's MT19937, :305, 338-339), one stream per call
+ * seeded with permuteseed and drawn in the order subcolumn, column, layer,
+ * so a column's subcolumns depend on ncol; the stream is serial (one
+ * workgroup), 4 bytes of scratch per draw.  icld = 1 random, 2
+ * maximum-random, 3 maximum overlap (:364-445); icld = 0 returns at once
+ * with the outputs untouched; icld outside 0..3 returns RRTMG_HIP_EINVAL.
  * cldfrac, ciwp, clwp, rei, rel (ncol, nlay); tauc (16, ncol, nlay), band
  * fastest.  Outputs cldfmcl, ciwpmcl, clwpmcl, taucmcl (140, ncol, nlay),
  * g-point fastest, and reicmcl, relqmcl (ncol, nlay) copies of rei, rel.

@@ -1385,12 +1385,12 @@ static int mcica_subcol(const char *who, int nsub, int ncol, int nlay, int icld,
     // once, icld outside 0..3 stops
     if (icld == 0) return 0;
     if (icld < 0 || icld > 3) return fail(RRTMG_HIP_EINVAL, "invalid icld (0..3)");
-    if (irng != 0)
-        return fail(RRTMG_HIP_ENOTSUP, "irng != 0: the Mersenne Twister generator is not implemented; use irng = 0 "
-                                       "(KISS)");
+    // mcica_subcol_gen_lw.f90:305: any irng != 0 selects the Mersenne Twister
+    const bool mt = irng != 0;
     if (ncol <= 0) return fail(RRTMG_HIP_EINVAL, "ncol must be positive");
     // the KISS seeds read the four lowest layers (mcica_subcol_gen_lw.f90:330-333)
-    if (nlay < 4 || nlay > RRTMG_HIP_MAX_NLAY) return fail(RRTMG_HIP_EINVAL, "nlay out of range [4, 200]");
+    if (nlay < (mt ? 2 : 4) || nlay > RRTMG_HIP_MAX_NLAY)
+        return fail(RRTMG_HIP_EINVAL, mt ? "nlay out of range [2, 200]" : "nlay out of range [4, 200] with KISS");
     if (ptr_kind != RRTMG_HIP_HOST && ptr_kind != RRTMG_HIP_DEVICE) return fail(RRTMG_HIP_EINVAL, "bad ptr_kind");
     if (!play || !cldfrac || !tauc || !cldfmcl || !taucmcl) return fail(RRTMG_HIP_EINVAL, "null array argument");
     if (sw && (!ssac || !asmc || !fsfc || !ssacmcl || !asmcmcl || !fsfcmcl))
@@ -1416,7 +1416,9 @@ static int mcica_subcol(const char *who, int nsub, int ncol, int nlay, int icld,
     hipStream_t st = (hipStream_t)stream;
     StreamWork &W = s->work(st);
     if ((rc = W.flag.ensure(256))) return fail(rc, "device allocation for the input checks failed");
-    const size_t jump_bytes = align_up(140 * sizeof(McicaJump));
+    // the jump constants, then (Mersenne Twister) the call's stream
+    const size_t jump_bytes = align_up(140 * sizeof(McicaJump)) +
+                              (mt ? align_up(sizeof(uint32_t) * mcica_mt_count(ncol, nlay, icld, nsub)) : 0);
     McicaSubcolArgs a{};
     a.ncol = ncol;
     a.nlay = nlay;
@@ -1430,7 +1432,7 @@ static int mcica_subcol(const char *who, int nsub, int ncol, int nlay, int icld,
     if (ptr_kind == RRTMG_HIP_HOST) {
         for (int i = 0; i < nin; ++i)
             if (ins[i] && (rc = check_host_array(ins[i], len[i], nm[i]))) return rc;
-        for (int i = 0; i < ncol; ++i)
+        for (int i = 0; i < ncol && !mt; ++i)
             if (play[i] * 1.e2 < play[(size_t)ncol + i] * 1.e2)
                 return fail(RRTMG_HIP_EINVAL, "play must not rise from the first layer to the second (the KISS "
                                               "seeds read the four lowest layers)");
@@ -1464,6 +1466,7 @@ static int mcica_subcol(const char *who, int nsub, int ncol, int nlay, int icld,
         for (int i = 0; i < nout; ++i) *dout[i] = outs[i];
     }
     if (ck) return fail(RRTMG_HIP_EDEVICE, std::string(who) + ": host-to-device copy failed");
+    if (mt) a.mt = (uint32_t *)((char *)W.lw_stage.p + align_up(140 * sizeof(McicaJump)));
     hipError_t e = launch_mcica_subcol(a, s->xs_jump, (McicaJump *)W.lw_stage.p, st);
     if (e != hipSuccess) return fail(RRTMG_HIP_EDEVICE, std::string(who) + " launch: " + hipGetErrorString(e));
     if (ptr_kind == RRTMG_HIP_HOST) {

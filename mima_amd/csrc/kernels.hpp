@@ -149,7 +149,14 @@ struct McicaSubcolArgs {
     double *cldfmcl, *ciwpmcl, *clwpmcl, *reicmcl, *relqmcl, *taucmcl;  // only cldfmcl and taucmcl required
     double *ssacmcl, *asmcmcl, *fsfcmcl;                                // SW only, required there
     int *bad;  // columns whose pressure rises from the first layer to the second; NULL: not counted
+    uint32_t *mt;  // irng = 1: the Mersenne Twister stream (mcica_mt_count values); NULL: KISS
 };
+// length of the Mersenne Twister stream one generator call draws: one value
+// per (subcolumn, column, layer), one per (subcolumn, column) for icld 3
+inline size_t mcica_mt_count(int ncol, int nlay, int icld, int nsub)
+{
+    return (size_t)nsub * ncol * (icld == 3 ? 1 : nlay);
+}
 // jump constants of one subcolumn's skip over the generator's sequence
 struct McicaJump {
     uint32_t a1, c1;  // congruential part: s -> a1 s + c1

@@ -146,6 +146,102 @@ __device__ __forceinline__ uint32_t mwc_jump(uint32_t s, uint64_t d, uint64_t p,
 // skipped.  Contraction is off: the reference rounds every product and sum
 // on its own, and a fused play * 100 - int(play * 100) would keep bits it
 // rounds away and change the seed.
+// The Mersenne Twister of mcica_random_numbers.f90 (MT19937, Matsumoto and
+// Nishimura 1998): one stream per generator call, seeded with permuteseed
+// (mcica_subcol_gen_lw.f90:338-339; the scalar initialisation, :169-185)
+// and drawn in the order subcolumn, column, layer (:360-367, :385-392,
+// :418-425).  The stream is serial, so one workgroup produces it: per block
+// of 624 words, the twist (:134-152) in four dependent phases over the
+// state in LDS, then the tempered words (:154-165) are stored.
+__device__ __forceinline__ uint32_t mt_twist(uint32_t u, uint32_t v)
+{
+    const uint32_t m = (u & 0x80000000u) | (v & 0x7fffffffu);
+    return (m >> 1) ^ ((v & 1u) ? 0x9908b0dfu : 0u);
+}
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y)
+{
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    return y ^ (y >> 18);
+}
+
+__global__ __launch_bounds__(640) void mt_stream_kernel(uint32_t seed, size_t count, uint32_t *__restrict__ out)
+{
+    __shared__ uint32_t st[624];
+    const int k = (int)threadIdx.x;
+    if (k == 0) {
+        uint32_t x = seed;
+        st[0] = x;
+        for (int i = 1; i < 624; ++i) {
+            x = 1812433253u * (x ^ (x >> 30)) + (uint32_t)i;
+            st[i] = x;
+        }
+    }
+    __syncthreads();
+    for (size_t base = 0; base < count; base += 624) {
+        // the first draw twists first (the state starts with its element
+        // counter at the block size)
+        uint32_t t = 0, far = 0, last = 0;
+        if (k < 623) t = mt_twist(st[k], st[k + 1]);
+        if (k < 227) far = st[k + 397];
+        if (k == 623) last = st[623];
+        __syncthreads();
+        if (k < 227) st[k] = far ^ t;
+        __syncthreads();
+        if (k >= 227 && k < 454) st[k] = st[k - 227] ^ t;
+        __syncthreads();
+        if (k >= 454 && k < 623) st[k] = st[k - 227] ^ t;
+        __syncthreads();
+        if (k == 623) st[623] = st[396] ^ mt_twist(last, st[0]);
+        __syncthreads();
+        if (k < 624 && base + k < count) out[base + k] = mt_temper(st[k]);
+    }
+}
+
+// one (column, subcolumn) of the generator with the Mersenne Twister's
+// numbers: getRandomReal (genrand_real1, mcica_random_numbers.f90:276-295)
+// of the stream word, then the overlap as with KISS (no seeds, so no
+// pressure-order condition)
+__device__ __forceinline__ void mcica_subcol_mt(const McicaSubcolArgs &a, int col, int ig)
+{
+#pragma clang fp contract(off)
+    const int ncol = a.ncol, nlay = a.nlay, icld = a.icld, nsub = a.nsub;
+    const int nband = nsub == 140 ? 16 : 14;
+    const bool sw = nsub == 112;
+    int b = 0, end = mcica_band_ng(nsub, 0);
+    while (ig >= end) end += mcica_band_ng(nsub, ++b);
+    if (ig == 0) {
+        for (int l = 0; l < nlay; ++l) {
+            const size_t ix = (size_t)l * ncol + col;
+            if (a.reicmcl) a.reicmcl[ix] = a.rei ? a.rei[ix] : 0.0;
+            if (a.relqmcl) a.relqmcl[ix] = a.rel ? a.rel[ix] : 0.0;
+        }
+    }
+    const size_t k0 = ((size_t)ig * ncol + col) * (icld == 3 ? 1 : nlay);
+    double prev = 0.0, fprev = 0.0;
+    for (int l = 0; l < nlay; ++l) {
+        double x = (double)a.mt[k0 + (icld == 3 ? 0 : l)] / 4294967295.0;
+        const size_t ix = (size_t)l * ncol + col, o = ix * nsub + ig, j = ix * nband + b;
+        double f = a.cldfrac[ix];
+        if (f < 1.e-20) f = 0.0;
+        if (icld == 2 && l > 0) x = prev > 1.0 - fprev ? prev : x * (1.0 - fprev);
+        prev = x;
+        fprev = f;
+        const bool cloudy = x >= 1.0 - f;
+        __builtin_nontemporal_store(cloudy ? 1.0 : 0.0, a.cldfmcl + o);
+        __builtin_nontemporal_store(cloudy ? a.tauc[j] : 0.0, a.taucmcl + o);
+        if (a.ciwpmcl) __builtin_nontemporal_store(cloudy && a.ciwp ? a.ciwp[ix] : 0.0, a.ciwpmcl + o);
+        if (a.clwpmcl) __builtin_nontemporal_store(cloudy && a.clwp ? a.clwp[ix] : 0.0, a.clwpmcl + o);
+        if (sw) {
+            __builtin_nontemporal_store(cloudy ? a.ssac[j] : 1.0, a.ssacmcl + o);
+            __builtin_nontemporal_store(cloudy ? a.asmc[j] : 0.0, a.asmcmcl + o);
+            __builtin_nontemporal_store(cloudy ? a.fsfc[j] : 0.0, a.fsfcmcl + o);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void mcica_subcol_kernel(McicaSubcolArgs a, const McicaJump *__restrict__ jump,
                                                            const uint32_t *__restrict__ xs_jump)
 {
@@ -158,6 +254,10 @@ __global__ __launch_bounds__(256) void mcica_subcol_kernel(McicaSubcolArgs a, co
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (size_t)ncol * nsub) return;
     const int col = (int)(t / nsub), ig = (int)(t - (size_t)col * nsub);
+    if (a.mt) {
+        mcica_subcol_mt(a, col, ig);
+        return;
+    }
     if (a.play[col] * 1.e2 < a.play[(size_t)ncol + col] * 1.e2) {
         if (a.bad && ig == 0) atomicAdd(a.bad, 1);
         return;
@@ -221,7 +321,12 @@ __global__ void mcica_zero_kernel(int *bad) { *bad = 0; }
 hipError_t launch_mcica_subcol(const McicaSubcolArgs &a, const uint32_t *xs_jump, McicaJump *jump, hipStream_t s)
 {
     if (a.bad) hipLaunchKernelGGL(mcica_zero_kernel, dim3(1), dim3(1), 0, s, a.bad);
-    hipLaunchKernelGGL(mcica_jump_kernel, dim3(1), dim3(192), 0, s, a.nlay, a.icld, a.permuteseed, a.nsub, jump);
+    if (a.mt)
+        hipLaunchKernelGGL(mt_stream_kernel, dim3(1), dim3(640), 0, s, (uint32_t)a.permuteseed,
+                           mcica_mt_count(a.ncol, a.nlay, a.icld, a.nsub), a.mt);
+    else
+        hipLaunchKernelGGL(mcica_jump_kernel, dim3(1), dim3(192), 0, s, a.nlay, a.icld, a.permuteseed, a.nsub,
+                           jump);
     const size_t n = (size_t)a.ncol * a.nsub;
     hipLaunchKernelGGL(mcica_subcol_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a,
                        (const McicaJump *)jump, xs_jump);

@@ -217,7 +217,8 @@ def rrtmg_lw_cloud(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr,
 def mcica_subcol_lw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei, rel, tauc, iplon=0,
                     stream=None):
     """mcica_subcol_lw (mcica_subcol_gen_lw.f90:48-154): McICA cloud
-    subcolumns with the KISS generator (irng = 0).
+    subcolumns with the KISS generator (irng = 0) or the Mersenne Twister
+    (irng != 0, seeded with permuteseed).
 
     ``play``, ``cldfrac``, ``ciwp``, ``clwp``, ``rei``, ``rel`` (nlay, ncol);
     ``tauc`` (nlay, ncol, 16).  ciwp, clwp, rei, rel may be None (zero).
@@ -268,7 +269,8 @@ def rrtmg_lw_mcica(ncol, nlay, icld, idrv, play, plev, tlay, tlev, tsfc, h2ovmr,
 def mcica_subcol_sw(ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei, rel, tauc, ssac, asmc,
                     fsfc, iplon=0, stream=None):
     """mcica_subcol_sw (mcica_subcol_gen_sw.f90:47-159): McICA cloud
-    subcolumns of the SW g-points with the KISS generator (irng = 0).
+    subcolumns of the SW g-points with the KISS generator (irng = 0) or
+    the Mersenne Twister (irng != 0, seeded with permuteseed).
 
     ``play``, ``cldfrac``, ``ciwp``, ``clwp``, ``rei``, ``rel`` (nlay, ncol);
     ``tauc``, ``ssac``, ``asmc``, ``fsfc`` (nlay, ncol, 14).  Returns a dict

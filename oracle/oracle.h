@@ -18,6 +18,8 @@
 #ifndef RRTMG_ORACLE_H
 #define RRTMG_ORACLE_H
 
+#include <stdint.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -166,6 +168,22 @@ int oracle_rrtmg_lw_mcica(int ncol, int nlay, const double *play, const double *
  * 163-536, nsub 140 or 112): mask[ig * 256 + l].  -6 on rising pressure. */
 int oracle_mcica_mask(int i, int ncol, int nlay, int icld, int permuteseed, int nsub, const double *play,
                       const double *cldfrac, unsigned char *mask);
+/* MT19937's first `count` tempered words for init_genrand(seed)
+ * (mcica_random_numbers.f90:134-185, 243-258). */
+void oracle_mt_stream(uint32_t seed, long count, uint32_t *out);
+/* the Mersenne Twister stream oracle_mcica_mask draws from (NULL: KISS) */
+void oracle_mcica_set_stream(const uint32_t *mt);
+/* the subcolumn generators with the Mersenne Twister (irng = 1), seeded
+ * with permuteseed; arguments as oracle_lw_mcica_subcol /
+ * oracle_sw_mcica_subcol.  Not thread-safe (one stream per call). */
+int oracle_lw_mcica_subcol_mt(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                              const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                              double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl);
+int oracle_sw_mcica_subcol_mt(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                              const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                              const double *ssac, const double *asmc, const double *fsfc, double *cldfmcl,
+                              double *ciwpmcl, double *clwpmcl, double *taucmcl, double *ssacmcl, double *asmcmcl,
+                              double *fsfcmcl);
 /* McICA SW: mcica_subcol_sw with the KISS generator and rrtmg_sw of the
  * McICA driver (rrtmg_sw_rad.f90:80-745: volume mixing ratios, zenith
  * cosine floored at 1e-10, cldprmc_sw with inflgsw = 0, spcvmc_sw).

@@ -1830,9 +1830,69 @@ static double kiss_step(uint32_t s[4])
  * overlap (3).  Subcolumn ig of layer l is cloudy where its number >= 1 -
  * cldfrac: mask[ig * MAXLAY + l] (0-based).  -6 when the second layer's
  * pressure is the higher (the reference stops). */
+/* MT19937 (Matsumoto and Nishimura, ACM TOMACS 8, 1998), as
+ * mcica_random_numbers.f90 states it: init_genrand's scalar seeding
+ * (:169-185), the twist of all 624 words once they are used up (:134-152),
+ * the tempering (:154-165).  out receives the first `count` tempered words
+ * (genrand_int32). */
+void oracle_mt_stream(uint32_t seed, long count, uint32_t *out)
+{
+    uint32_t mt[624];
+    mt[0] = seed;
+    for (int i = 1; i < 624; ++i) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+    int idx = 624;
+    for (long n = 0; n < count; ++n) {
+        if (idx >= 624) {
+            for (int k = 0; k < 624; ++k) {
+                uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+                mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((mt[(k + 1) % 624] & 1u) ? 0x9908b0dfu : 0u);
+            }
+            idx = 0;
+        }
+        uint32_t y = mt[idx++];
+        y ^= y >> 11;
+        y ^= (y << 7) & 0x9d2c5680u;
+        y ^= (y << 15) & 0xefc60000u;
+        y ^= y >> 18;
+        out[n] = y;
+    }
+}
+
+/* the Mersenne Twister stream of the current generator call (irng = 1), or
+ * NULL for KISS; set around the column loop by the subcolumn drivers */
+static const uint32_t *mcica_mt;
+
+void oracle_mcica_set_stream(const uint32_t *mt) { mcica_mt = mt; }
+
 int oracle_mcica_mask(int i, int ncol, int nlay, int icld, int permuteseed, int nsub, const double *play,
                       const double *cldfrac, unsigned char *mask)
 {
+    if (mcica_mt) {
+        /* irng = 1 (mcica_subcol_gen_lw.f90:360-367, 385-392, 418-425): the
+         * stream in the order subcolumn, column, layer (one per subcolumn and
+         * column for icld 3), each word w as getRandomReal's w / (2^32 - 1) */
+        double cldf[MAXLAY + 1];
+        for (int l = 0; l < nlay; ++l) {
+            double f = cldfrac[i + (long)ncol * l];
+            cldf[l] = f < 1.e-20 ? 0.0 : f;
+        }
+        for (int ig = 0; ig < nsub; ++ig) {
+            double cdf[MAXLAY + 1];
+            for (int l = 0; l < nlay; ++l) {
+                long k = icld == 3 ? (long)ig * ncol + i : ((long)ig * ncol + i) * nlay + l;
+                cdf[l] = (double)mcica_mt[k] / 4294967295.0;
+            }
+            if (icld == 2)
+                for (int l = 1; l < nlay; ++l) {
+                    if (cdf[l - 1] > 1.0 - cldf[l - 1])
+                        cdf[l] = cdf[l - 1];
+                    else
+                        cdf[l] = cdf[l] * (1.0 - cldf[l - 1]);
+                }
+            for (int l = 0; l < nlay; ++l) mask[ig * MAXLAY + l] = cdf[l] >= 1.0 - cldf[l];
+        }
+        return 0;
+    }
     if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
     uint32_t s[4];
     for (int k = 0; k < 4; ++k) {
@@ -1876,9 +1936,9 @@ int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
                            double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl)
 {
     if (!L.ready) return -1;
-    if (nlay < 4 || nlay > MAXLAY) return -7;
+    if (nlay < (mcica_mt ? 2 : 4) || nlay > MAXLAY) return -7;
     if (icld < 1 || icld > 3) return -2;
-    for (int i = 0; i < ncol; ++i)
+    for (int i = 0; i < ncol && !mcica_mt; ++i)
         if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
     int band_of[NGLW];
     for (int b = 0, ig = 0; b < 16; ++b)
@@ -1900,6 +1960,26 @@ int oracle_lw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
         }
     }
     return 0;
+}
+
+/* oracle_lw_mcica_subcol with the Mersenne Twister (irng = 1), seeded with
+ * permuteseed (mcica_subcol_gen_lw.f90:338-339); no pressure-order or
+ * four-layer condition (those belong to the KISS seeds). */
+int oracle_lw_mcica_subcol_mt(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                              const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                              double *cldfmcl, double *ciwpmcl, double *clwpmcl, double *taucmcl)
+{
+    if (icld < 1 || icld > 3) return -2;
+    long n = (long)NGLW * ncol * (icld == 3 ? 1 : nlay);
+    uint32_t *mt = malloc(sizeof(uint32_t) * (size_t)n);
+    if (!mt) return -3;
+    oracle_mt_stream((uint32_t)permuteseed, n, mt);
+    oracle_mcica_set_stream(mt);
+    int rc = oracle_lw_mcica_subcol(ncol, nlay, icld, permuteseed, play, cldfrac, ciwp, clwp, tauc, cldfmcl,
+                                    ciwpmcl, clwpmcl, taucmcl);
+    oracle_mcica_set_stream(NULL);
+    free(mt);
+    return rc;
 }
 
 /* rtrnmc (rrtmg_lw_rtrnmc.f90:225-579; iout = 0, idrv = 0): rtrn's random

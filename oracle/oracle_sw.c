@@ -1702,6 +1702,10 @@ int oracle_rrtmg_sw_cloud_wp(int ncol, int nlay, int icld, int inflag, int icefl
  * depth 0, albedo 1, asymmetry 0 and forward fraction 0 (:477-510).  Inputs
  * cldfrac, ciwp, clwp (ncol, nlay); tauc, ssac, asmc, fsfc (14, ncol, nlay);
  * outputs (112, ncol, nlay).  -6, -7 as oracle_lw_mcica_subcol. */
+/* set while oracle_sw_mcica_subcol_mt runs the generator on a Mersenne
+ * Twister stream */
+static int sw_mt;
+
 int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const double *play,
                            const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
                            const double *ssac, const double *asmc, const double *fsfc, double *cldfmcl,
@@ -1709,9 +1713,9 @@ int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
                            double *fsfcmcl)
 {
     if (!S.ready) return -1;
-    if (nlay < 4 || nlay > MAXLAY) return -7;
+    if (nlay < (sw_mt ? 2 : 4) || nlay > MAXLAY) return -7;
     if (icld < 1 || icld > 3) return -2;
-    for (int i = 0; i < ncol; ++i)
+    for (int i = 0; i < ncol && !sw_mt; ++i)
         if (play[i] * 1.e2 < play[i + ncol] * 1.e2) return -6;
     int band_of[NGSW];
     for (int b = 0, ig = 0; b < 14; ++b)
@@ -1736,6 +1740,29 @@ int oracle_sw_mcica_subcol(int ncol, int nlay, int icld, int permuteseed, const 
         }
     }
     return 0;
+}
+
+/* oracle_sw_mcica_subcol with the Mersenne Twister (irng = 1, seeded with
+ * permuteseed; mcica_subcol_gen_sw.f90 as the LW generator). */
+int oracle_sw_mcica_subcol_mt(int ncol, int nlay, int icld, int permuteseed, const double *play,
+                              const double *cldfrac, const double *ciwp, const double *clwp, const double *tauc,
+                              const double *ssac, const double *asmc, const double *fsfc, double *cldfmcl,
+                              double *ciwpmcl, double *clwpmcl, double *taucmcl, double *ssacmcl, double *asmcmcl,
+                              double *fsfcmcl)
+{
+    if (icld < 1 || icld > 3) return -2;
+    long n = (long)NGSW * ncol * (icld == 3 ? 1 : nlay);
+    uint32_t *mt = malloc(sizeof(uint32_t) * (size_t)n);
+    if (!mt) return -3;
+    oracle_mt_stream((uint32_t)permuteseed, n, mt);
+    oracle_mcica_set_stream(mt);
+    sw_mt = 1;
+    int rc = oracle_sw_mcica_subcol(ncol, nlay, icld, permuteseed, play, cldfrac, ciwp, clwp, tauc, ssac, asmc,
+                                    fsfc, cldfmcl, ciwpmcl, clwpmcl, taucmcl, ssacmcl, asmcmcl, fsfcmcl);
+    sw_mt = 0;
+    oracle_mcica_set_stream(NULL);
+    free(mt);
+    return rc;
 }
 
 /* rrtmg_sw of the McICA driver (rrtmg_sw_rad.f90:80-745; iaer = 0,
@@ -1792,7 +1819,7 @@ static int sw_mcica_driver(int ncol, int nlay, int iceflag, int liqflag, const d
                             for (int acc = S.ngc[0]; iw > acc; acc += S.ngc[++ib]) {
                             }
                             double ci = ciwpmcl ? ciwpmcl[o] : 0.0, cw = clwpmcl ? clwpmcl[o] : 0.0;
-                            double r[3];
+                            double r[3] = {0.0, 1.0, 0.0};
                             if (sw_wp_optics(ib, iceflag, liqflag, ci, cw, reicmcl ? reicmcl[ix] : 0.0,
                                              relqmcl ? relqmcl[ix] : 0.0, r) != 0)
                                 bad = 1;

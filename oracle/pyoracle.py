@@ -214,17 +214,18 @@ def lib_ngs():
     return np.cumsum(np.asarray(tabs["lw/ngc"], dtype=np.int64))
 
 
-def lw_mcica_subcol(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permuteseed=0):
-    """mcica_subcol_lw with the KISS generator: cldfmcl, ciwpmcl, clwpmcl,
-    taucmcl as (nlay, ncol, 140)."""
+def lw_mcica_subcol(c, icld, cldfr, taucld, cicewp=None, cliqwp=None, permuteseed=0, irng=0):
+    """mcica_subcol_lw with the KISS generator (irng 0) or the Mersenne
+    Twister (irng 1): cldfmcl, ciwpmcl, clwpmcl, taucmcl as (nlay, ncol,
+    140)."""
     L = lib()
     ncol, nlay = c["ncol"], c["nlay"]
     z = np.zeros((nlay, ncol))
     ins = [_f64(a) for a in (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp,
                              taucld)]
     outs = [np.zeros((nlay, ncol, 140)) for _ in range(4)]
-    rc = L.oracle_lw_mcica_subcol(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins],
-                                  *[_p(a) for a in outs])
+    f = L.oracle_lw_mcica_subcol_mt if irng else L.oracle_lw_mcica_subcol
+    rc = f(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins], *[_p(a) for a in outs])
     if rc != 0:
         raise RuntimeError("oracle_lw_mcica_subcol returned %d" % rc)
     return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl"), outs))
@@ -246,17 +247,19 @@ def rrtmg_lw_mcica(c, cldfmcl, taucmcl, tauaer=None):
     return dict(zip(("uflx", "dflx", "hr", "uflxc", "dflxc", "hrc"), outs))
 
 
-def sw_mcica_subcol(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, permuteseed=0):
-    """mcica_subcol_sw with the KISS generator: cldfmcl, ciwpmcl, clwpmcl,
-    taucmcl, ssacmcl, asmcmcl, fsfcmcl as (nlay, ncol, 112)."""
+def sw_mcica_subcol(c, icld, cldfr, taucld, ssacld, asmcld, fsfcld, cicewp=None, cliqwp=None, permuteseed=0,
+                    irng=0):
+    """mcica_subcol_sw with the KISS generator (irng 0) or the Mersenne
+    Twister (irng 1): cldfmcl, ciwpmcl, clwpmcl, taucmcl, ssacmcl, asmcmcl,
+    fsfcmcl as (nlay, ncol, 112)."""
     L = lib()
     ncol, nlay = c["ncol"], c["nlay"]
     z = np.zeros((nlay, ncol))
     ins = [_f64(a) for a in (c["play"], cldfr, z if cicewp is None else cicewp, z if cliqwp is None else cliqwp,
                              taucld, ssacld, asmcld, fsfcld)]
     outs = [np.zeros((nlay, ncol, 112)) for _ in range(7)]
-    rc = L.oracle_sw_mcica_subcol(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins],
-                                  *[_p(a) for a in outs])
+    f = L.oracle_sw_mcica_subcol_mt if irng else L.oracle_sw_mcica_subcol
+    rc = f(ncol, nlay, int(icld), int(permuteseed), *[_p(a) for a in ins], *[_p(a) for a in outs])
     if rc != 0:
         raise RuntimeError("oracle_sw_mcica_subcol returned %d" % rc)
     return dict(zip(("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl", "ssacmcl", "asmcmcl", "fsfcmcl"), outs))
@@ -302,6 +305,14 @@ def rrtmg_sw_mcica_wp(c, iceflag, liqflag, sub, reicmcl, relqmcl):
     if rc != 0:
         raise RuntimeError("oracle_rrtmg_sw_mcica_wp returned %d" % rc)
     return dict(zip(("swuflx", "swdflx", "swhr", "swuflxc", "swdflxc", "swhrc"), outs))
+
+
+def mt_stream(seed, count):
+    """The first `count` tempered MT19937 words for init_genrand(seed)."""
+    out = np.zeros(int(count), np.uint32)
+    lib().oracle_mt_stream(ctypes.c_uint32(seed & 0xffffffff), ctypes.c_long(int(count)),
+                           out.ctypes.data_as(ctypes.c_void_p))
+    return out
 
 
 def _col(c, i, key):

@@ -20,14 +20,21 @@ module ref_sw_mcica_driver
   use rrtmg_sw_rad, only : rrtmg_sw
   use mcica_subcol_gen_sw, only : mcica_subcol_sw
   implicit none
+  ! the generator ref_sw_mcica_subcol runs: 0 KISS, 1 Mersenne Twister
+  integer(kind=im), save :: g_irng = 0
 contains
+  subroutine ref_sw_mcica_set_irng(irng) bind(C, name="ref_sw_mcica_set_irng")
+    integer(c_int), value :: irng
+    g_irng = irng
+  end subroutine ref_sw_mcica_set_irng
 
   subroutine ref_sw_mcica_init(cpdair) bind(C, name="ref_sw_mcica_init")
     real(c_double), value :: cpdair
     call rrtmg_sw_ini(cpdair)
   end subroutine ref_sw_mcica_init
 
-  ! the subcolumns for icld, permuteseed, irng = 0 (KISS); effective radii
+  ! the subcolumns for icld, permuteseed and the generator of
+  ! ref_sw_mcica_set_irng (KISS unless set); effective radii
   ! zero (copied through, unread with inflgsw = 0)
   subroutine ref_sw_mcica_subcol(ncol, nlay, icld_in, permuteseed_in, play, cldfrac, ciwp, clwp, &
                                  tauc, ssac, asmc, fsfc, cldfmcl, ciwpmcl, clwpmcl, taucmcl, &
@@ -47,7 +54,7 @@ contains
     rel = 0.0_rb
     icld = icld_in
     permuteseed = permuteseed_in
-    irng = 0
+    irng = g_irng
     call mcica_subcol_sw(1, ncol, nlay, icld, permuteseed, irng, play, cldfrac, ciwp, clwp, rei, rel, &
                          tauc, ssac, asmc, fsfc, cldfmcl, ciwpmcl, clwpmcl, reicmcl, relqmcl, &
                          taucmcl, ssacmcl, asmcmcl, fsfcmcl)

@@ -246,15 +246,13 @@ def test_lds_kernel_matches_one_wave_kernel(rr, tmp_path):
 
 
 def test_errors(rr):
-    """Refusals: irng = 1 (Mersenne Twister not built), icld outside 0..3
+    """Refusals: icld outside 0..3
     and nlay < 4 for the generator (the reference stops), a column whose
     pressure rises from the first layer (host check; device check with
     rrtmg_hip_set_checks on), inflglw = 1 (the reference stops), the smooth
     mode."""
     c, cf, tau, _ = _sample(64, 40, 70)
     ncol, nlay = c["ncol"], c["nlay"]
-    with pytest.raises(rr.RRTMGError, match="irng"):
-        rr.mcica_subcol_lw(ncol, nlay, 2, 0, 1, c["play"], cf, None, None, None, None, tau)
     with pytest.raises(rr.RRTMGError, match="icld"):
         rr.mcica_subcol_lw(ncol, nlay, 4, 0, 0, c["play"], cf, None, None, None, None, tau)
     with pytest.raises(rr.RRTMGError, match="nlay"):
@@ -323,3 +321,43 @@ def test_water_path_host_path_and_radius_check(rr):
         rr.rrtmg_lw_mcica(c["ncol"], c["nlay"], 2, 0, *[c[kk] for kk in KEYS], 2, 2, 1, sub["cldfmcl"],
                           sub["taucmcl"], ciwpmcl=sub["ciwpmcl"], clwpmcl=sub["clwpmcl"], reicmcl=WP["reice"],
                           relqmcl=bad)
+
+
+MTGOLD = dict(np.load(os.path.join(ROOT, "tests", "golden", "mcica_ref_mt.npz"), allow_pickle=False))
+
+
+@pytest.mark.parametrize("icld", [1, 2, 3])
+@pytest.mark.parametrize("seed", [0, 123457])
+def test_mersenne_twister_masks_match_reference(rr, icld, seed):
+    """irng = 1 (mcica_random_numbers.f90's MT19937): the reference's LW and
+    SW masks bit for bit, device pointers."""
+    import torch
+    n, L = int(MTGOLD["ncol"]), int(MTGOLD["nlay"])
+    t = lambda a: torch.as_tensor(np.ascontiguousarray(a), device="cuda")  # noqa: E731
+    cf, tau = MTGOLD["cldfr"], MTGOLD["taucld"]
+    lw = rr.mcica_subcol_lw(n, L, icld, seed, 1, t(MTGOLD["play"]), t(cf), None, None, None, None,
+                            t(np.repeat(tau[..., :1], 16, axis=2)))
+    np.testing.assert_array_equal(lw["cldfmcl"].cpu().numpy().astype(np.uint8), MTGOLD["lw%d_%d" % (icld, seed)])
+    rr.rrtmg_sw_ini()
+    sw = rr.mcica_subcol_sw(n, L, icld, seed, 1, t(MTGOLD["play"]), t(cf), None, None, None, None, t(tau),
+                            t(MTGOLD["ssacld"]), t(MTGOLD["asmcld"]), t(MTGOLD["fsfcld"]))
+    np.testing.assert_array_equal(sw["cldfmcl"].cpu().numpy().astype(np.uint8), MTGOLD["sw%d_%d" % (icld, seed)])
+
+
+@pytest.mark.parametrize("icld", [1, 2, 3])
+def test_mersenne_twister_subcolumns_match_oracle(rr, icld):
+    """A stream of several thousand twist blocks (300 columns x 40 layers x
+    140 subcolumns): every LW subcolumn output equals the oracle's, host
+    pointers; play rising from the first layer is allowed (no KISS seeds)."""
+    full = make_columns(64, 32, 40, seed=21)
+    c = subset(full, np.arange(300))
+    cf, tau = make_clouds(c, seed=22, edge_cases=False)
+    cf = np.where(cf > 0, np.random.default_rng(23).uniform(0.05, 1.0, cf.shape), 0.0)
+    wp = np.where(cf > 0, 7.0, 0.0)
+    c["play"] = c["play"].copy()
+    c["play"][0, 3] = c["play"][1, 3] * 0.99
+    o = po.lw_mcica_subcol(c, icld, cf, tau, cicewp=wp, cliqwp=wp, permuteseed=99, irng=1)
+    g = rr.mcica_subcol_lw(c["ncol"], c["nlay"], icld, 99, 1, c["play"], cf, wp, wp, None, None, tau)
+    for k in ("cldfmcl", "ciwpmcl", "clwpmcl", "taucmcl"):
+        np.testing.assert_array_equal(g[k], o[k], err_msg=k)
+    assert 0.0 < g["cldfmcl"].mean() < 1.0

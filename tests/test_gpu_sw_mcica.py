@@ -227,12 +227,10 @@ def test_binary_clouds_equal_nomcica(rr):
 
 
 def test_errors(rr):
-    """Refusals: irng = 1, icld outside 0..3 and nlay < 4 for the
+    """Refusals: icld outside 0..3 and nlay < 4 for the
     generator, inflgsw 1 (the reference stops), iaer != 0, the smooth mode."""
     c, cf, opt, wp = _sample(64, 40, 70)
     ncol, nlay = c["ncol"], c["nlay"]
-    with pytest.raises(rr.RRTMGError, match="irng"):
-        rr.mcica_subcol_sw(ncol, nlay, 2, 0, 1, c["play"], cf, None, None, None, None, *opt)
     with pytest.raises(rr.RRTMGError, match="icld"):
         rr.mcica_subcol_sw(ncol, nlay, 4, 0, 0, c["play"], cf, None, None, None, None, *opt)
     with pytest.raises(rr.RRTMGError, match="nlay"):
